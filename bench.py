@@ -1,0 +1,238 @@
+#!/usr/bin/env python3
+"""Nonbonded-step benchmark on one MI355X.
+
+Workload (default): the 288,000-atom water box - 96,000 rigid 3-site waters
+on a seeded jittered lattice in a 14.2 nm cubic box, rc = rlist = 1.0 nm,
+potential-shifted LJ on oxygen, Ewald real-space Coulomb with
+ewald_rtol = 1e-5 (tabulated correction, as the reference's plain-C kernel).
+A step is one pass of the nonbonded path over resident device buffers:
+coordinate conversion into the cluster layout, the cluster-pair kernel with
+forces, energies and shift forces, the force reduction into atom order and
+the FP64 virial (gmxcalculator.cpp:70-83 plus the energy/virial outputs).
+The grid and pair search run once before timing and are reported apart.
+
+Throughput is useful pair interactions per second: the exact number of
+non-excluded atom pairs within rc (the quantity the reference benchmark
+estimates as N/2 (rho 4/3 pi rc^3 + 1), bench_setup.cpp:273-276) divided by
+the step time.  The kernel's roofline uses the reference's stated flop count
+per pair (107 for Ewald + LJ with energies, gmxlib/nrnb.cpp:106) times the
+pairs the kernel computes (16 per cluster pair in its list), divided by the
+kernel's duration measured with HIP events on its stream.
+
+One GPU only: --gpus above 1 exits with a message.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "NB pair-interactions/s & ms/step, 288k-atom water; batched and fwd+bwd vs plain"
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+FLOPS_PER_PAIR_VF = 107  # Ewald + LJ, forces and energies (nrnb.cpp:106)
+FLOPS_PER_PAIR_F = 66  # forces only (nrnb.cpp:105)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def excluded_pairs_within(system, rc):
+    import numpy as np
+
+    n = 0
+    for a in range(system.natoms):
+        for b in system.excl_idx[system.excl_off[a]:system.excl_off[a + 1]]:
+            if b > a:
+                d = system.x[a] - system.x[b]
+                d -= system.box * np.round(d / system.box)
+                if float(d @ d) < rc * rc:
+                    n += 1
+    return n
+
+
+def excluded_pairs_water(system, rc):
+    # all three intramolecular pairs of a rigid water are within rc
+    return 3 * (system.natoms // 3)
+
+
+def cpu_baseline(system, budget_s=30.0):
+    """The oracle (plain-C restatement of the reference kernel, float) timed
+    on one host core over the same workload: one full step."""
+    from oracle import oracle as O
+
+    t0 = time.perf_counter()
+    o = O.Oracle("f32", "ewald", rlist=system.rlist, rcoulomb=system.rcoulomb)
+    o.setup(system.box, system.x, system.types, system.charges, O.nblib_nbfp(system.c6, system.c12, "f32"),
+            system.ntypes, system.excl_off, system.excl_idx)
+    t_setup = time.perf_counter() - t0
+    times = []
+    while sum(times) < budget_s / 3 and len(times) < 10:
+        t0 = time.perf_counter()
+        o.compute(system.x, energy=True)
+        times.append(time.perf_counter() - t0)
+    return sorted(times)[len(times) // 2], t_setup, len(times)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="water288k", choices=["water288k", "water9k", "chains150k"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the batched ensemble and "
+              "spatial decomposition would shard replicas or bricks across GPUs; see DESIGN.md). "
+              "Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from gmxapi_amd import nbx, systems
+
+    if not torch.cuda.is_available():
+        log("no GPU visible")
+        sys.exit(1)
+    index = {"water9k": 0, "water288k": 1, "chains150k": 2}[args.workload]
+    t0 = time.perf_counter()
+    s = systems.config(index)
+    log(f"[bench] workload {args.workload}: {s.natoms} atoms, box {s.box[0]} nm, rc {s.rcoulomb}, "
+        f"rlist {s.rlist} (generated in {time.perf_counter() - t0:.1f} s)")
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    ctx = nbx.Context(rlist=s.rlist, rcoulomb=s.rcoulomb, device=0)
+    ctx.set_stream(stream.cuda_stream)
+    m6, m12 = nbx.nblib_c6c12_matrices(s.c6, s.c12)
+    ctx.set_nonbonded_parameters(m6, m12)
+    ctx.set_topology(s.types, s.charges, s.excl_off, s.excl_idx)
+    x_d = torch.from_numpy(s.x.astype(np.float32).reshape(-1)).to(dev)
+    f_d = torch.empty_like(x_d)
+    torch.cuda.synchronize()
+
+    # grid + search (the reference's nbnxn_put_on_grid + constructPairlist)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    ctx.put_on_grid(s.box, x_device_ptr=x_d.data_ptr())
+    ctx.construct_pairlist()
+    torch.cuda.synchronize()
+    ev[0].record(stream)
+    ctx.put_on_grid(s.box, x_device_ptr=x_d.data_ptr())
+    ev[1].record(stream)
+    ctx.construct_pairlist()
+    ev[2].record(stream)
+    torch.cuda.synchronize()
+    grid_ms = ev[0].elapsed_time(ev[1])
+    search_ms = ev[1].elapsed_time(ev[2])
+    info = ctx.info()
+    computed_pairs = 16 * info["num_j_entries"]
+    within_rc = ctx.count_pairs(s.rcoulomb)
+    excl = excluded_pairs_water(s, s.rcoulomb) if args.workload.startswith("water") else \
+        excluded_pairs_within(s, s.rcoulomb)
+    useful = within_rc - excl
+    log(f"[bench] grid {grid_ms:.3f} ms, search {search_ms:.3f} ms; clusters {info['num_clusters']}, "
+        f"i-entries {info['num_i_entries']}, cluster pairs {info['num_j_entries']}; "
+        f"useful pairs {useful} (estimate {s.useful_pairs_estimate():.0f}), computed {computed_pairs}")
+
+    flags = nbx.COMPUTE_FORCES | nbx.COMPUTE_ENERGY | nbx.COMPUTE_SHIFT_FORCES | nbx.COMPUTE_VIRIAL
+
+    def step():
+        ctx.compute_device(x_d.data_ptr(), f_d.data_ptr(), flags)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(args.steps):
+        step()
+    stop.record(stream)
+    torch.cuda.synchronize()
+    ms_per_step = start.elapsed_time(stop) / args.steps
+
+    # kernel alone (dispatch = clear nbat f + cluster-pair kernel + FP64 reduction of partials)
+    kstart, kstop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    kstart.record(stream)
+    for _ in range(args.steps):
+        ctx.dispatch_kernel(nbx.COMPUTE_FORCES | nbx.COMPUTE_ENERGY | nbx.COMPUTE_SHIFT_FORCES)
+    kstop.record(stream)
+    torch.cuda.synchronize()
+    kernel_ms = kstart.elapsed_time(kstop) / args.steps
+    fstart, fstop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    fstart.record(stream)
+    for _ in range(args.steps):
+        ctx.dispatch_kernel(nbx.COMPUTE_FORCES)
+    fstop.record(stream)
+    torch.cuda.synchronize()
+    kernel_f_ms = fstart.elapsed_time(fstop) / args.steps
+    step()
+    torch.cuda.synchronize()
+    vlj, vc = ctx.energies()
+
+    achieved = FLOPS_PER_PAIR_VF * computed_pairs / (kernel_ms * 1e-3) / 1e12
+    out = {
+        "metric": METRIC,
+        "value": useful / (ms_per_step * 1e-3),
+        "unit": "pair-interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": {"water288k": "water-288k: 96,000 rigid 3-site waters, 14.2 nm cubic box",
+                         "water9k": "water-9k: 3,000 rigid 3-site waters, 4.48 nm cubic box",
+                         "chains150k": "chains-150k: 15,000 10-atom chains, 20 LJ types, 11.45 nm box"}[
+                args.workload] + f", rc {s.rcoulomb} nm, rlist {s.rlist} nm, Ewald rtol 1e-5 (table), "
+                "LJ potential shift; step = x convert + F/E/shift-force kernel + f reduce + FP64 virial",
+            "natoms": s.natoms,
+            "useful_pairs": useful,
+            "computed_pairs": computed_pairs,
+            "list_efficiency": useful / computed_pairs,
+            "kernel_ms_fe": kernel_ms,
+            "kernel_ms_f": kernel_f_ms,
+            "grid_ms": grid_ms,
+            "search_ms": search_ms,
+            "energies_kj_mol": {"lj": vlj, "coulomb_real_space": vc},
+        },
+        "roofline": {
+            "bound": "mfma",
+            "unit": "TFLOP/s",
+            "achieved": achieved,
+            "peak": PEAK_FP32_TFLOPS,
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": None,
+            "kernel": "nonbondedKernel<EwaldTable, energy, shift>",
+            "compute_unit": "FP32 vector ALU (pair math is not a dense contraction; peak is MI355X FP32 157.3 TFLOP/s)",
+            "flops_per_pair": FLOPS_PER_PAIR_VF,
+            "pairs_per_launch": computed_pairs,
+        },
+    }
+    if not args.no_cpu_baseline:
+        t_step, t_setup, nrep = cpu_baseline(s)
+        out["cpu_baseline"] = {
+            "value": useful / t_step,
+            "unit": "pair-interactions/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": f"one full F+E step of the same {s.natoms}-atom workload with the oracle (plain-C "
+                      f"restatement of the reference kernel, float, 1 thread), median of {nrep}: "
+                      f"{t_step:.2f} s per step (grid + search {t_setup:.2f} s not included)",
+        }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

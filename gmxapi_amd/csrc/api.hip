@@ -1,0 +1,1100 @@
+// C ABI of libnbx: context management and the host-side sequencing of the
+// grid, search and kernel launches.  See include/nbx.h for the contract.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/nbx.h"
+#include "nbx_internal.h"
+
+using namespace nbx;
+
+namespace
+{
+
+thread_local std::string g_lastError;
+
+struct InputError : std::runtime_error
+{
+    using std::runtime_error::runtime_error;
+};
+
+struct StateError : std::runtime_error
+{
+    using std::runtime_error::runtime_error;
+};
+
+struct DeviceError : std::runtime_error
+{
+    using std::runtime_error::runtime_error;
+};
+
+void check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess)
+    {
+        throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+
+template<typename T>
+struct DevBuf
+{
+    T*     p   = nullptr;
+    size_t cap = 0;
+    void   ensure(size_t n)
+    {
+        if (n <= cap && p)
+        {
+            return;
+        }
+        if (p)
+        {
+            check(hipFree(p), "hipFree");
+            p = nullptr;
+        }
+        size_t want = std::max<size_t>(n, 1);
+        check(hipMalloc(reinterpret_cast<void**>(&p), want * sizeof(T)), "hipMalloc");
+        cap = want;
+    }
+    void release()
+    {
+        if (p)
+        {
+            (void)hipFree(p);
+        }
+        p   = nullptr;
+        cap = 0;
+    }
+    ~DevBuf() { release(); }
+};
+
+// ---------------------------------------------------------------------------
+// Interaction constants, restated from the reference in single precision
+// ---------------------------------------------------------------------------
+
+// calc_ewaldcoeff_q (ewald/ewald_utils.cpp:46-74) in float
+float ewaldCoeff(float rc, float rtol)
+{
+    float beta = 5, low, high;
+    int   n, i = 0;
+    do
+    {
+        i++;
+        beta *= 2;
+    } while (std::erfc(beta * rc) > rtol);
+    n    = i + 60;
+    low  = 0;
+    high = beta;
+    for (i = 0; i < n; i++)
+    {
+        beta = (low + high) / 2;
+        if (std::erfc(beta * rc) > rtol)
+        {
+            low = beta;
+        }
+        else
+        {
+            high = beta;
+        }
+    }
+    return beta;
+}
+
+// ewald_spline3_table_scale, Coulomb part (tables/forcetable.cpp:304-358)
+float ewaldTableScale(float beta, float rc)
+{
+    const double erf_x_d3  = 1.0522;
+    const double etol      = 0.1 * std::erfc(beta * rc);
+    const double deriv_tol = FLT_EPSILON;
+    const double x_scale   = beta;
+    const double sc_deriv  = std::sqrt(erf_x_d3 / (6 * 4 * deriv_tol * x_scale)) * x_scale;
+    const double func_tol  = std::max(etol, static_cast<double>(FLT_EPSILON));
+    const double sc_func   = std::cbrt(erf_x_d3 / (6 * 12 * std::sqrt(3.0) * func_tol)) * x_scale;
+    const float  sc_q      = static_cast<float>(std::max(sc_deriv, sc_func));
+    return std::max(0.0f, sc_q);
+}
+
+double vEwaldLr(double beta, double r)
+{
+    return r == 0 ? beta * 2 / std::sqrt(M_PI) : std::erf(beta * r) / r;
+}
+
+// generateEwaldCorrectionTables (tables/forcetable.cpp:144-298): quadratic
+// spline of erf(beta r)/r; returns FDV0 quadruplets {F_i, F_i+1 - F_i, V_i, 0}.
+std::vector<float> ewaldTableFDV0(int numPoints, double tableScaling, float beta)
+{
+    std::vector<float> tf(numPoints, 0.0f), tv(numPoints, 0.0f);
+    const double dx       = 1 / tableScaling;
+    const float  tabMax   = static_cast<float>(FLT_MAX * 0.0001);
+    bool         outRange = false;
+    int          iIn      = numPoints;
+    double       vIn = 0, dc = 0;
+    for (int i = numPoints - 1; i >= 0; i--)
+    {
+        const double xr0 = i * dx;
+        const double vr0 = vEwaldLr(beta, xr0);
+        double       vi;
+        if (!outRange)
+        {
+            iIn = i;
+            vIn = vr0;
+            vi  = vr0;
+        }
+        else
+        {
+            vi = vIn - dc * (i - iIn) * dx;
+        }
+        tv[i] = static_cast<float>(vi);
+        if (i == 0)
+        {
+            continue;
+        }
+        const double vr1 = vEwaldLr(beta, (i - 1) * dx);
+        if (vr1 != vr1 || vr1 < -tabMax || vr1 > tabMax)
+        {
+            outRange = true;
+        }
+        if (!outRange)
+        {
+            const double a2dx = (vr0 + vr1 - 2 * vEwaldLr(beta, xr0 - 0.5 * dx)) / (0.25 * dx);
+            dc                = (vr0 - vr1) / dx + 0.5 * a2dx;
+        }
+        if (i == numPoints - 1)
+        {
+            tf[i] = static_cast<float>(-dc);
+        }
+        else
+        {
+            tf[i] += -0.5 * dc;
+        }
+        if (!outRange)
+        {
+            const double a1    = dc;
+            const double a2dx  = (a1 * dx + vr1 - vr0) * 2 / dx;
+            const double dcNew = a1 - a2dx;
+            if (dcNew != dcNew || dcNew < -tabMax || dcNew > tabMax)
+            {
+                outRange = true;
+            }
+            else
+            {
+                dc = dcNew;
+            }
+        }
+        tf[i - 1] = static_cast<float>(-0.5 * dc);
+    }
+    tf[0] *= 2;
+    std::vector<float> fdv0(4 * numPoints);
+    for (int i = 0; i < numPoints - 1; i++)
+    {
+        fdv0[4 * i]     = tf[i];
+        fdv0[4 * i + 1] = tf[i + 1] - tf[i];
+        fdv0[4 * i + 2] = tv[i];
+        fdv0[4 * i + 3] = 0;
+    }
+    const int last       = numPoints - 1;
+    fdv0[4 * last]     = tf[last];
+    fdv0[4 * last + 1] = -tf[last];
+    fdv0[4 * last + 2] = tv[last];
+    fdv0[4 * last + 3] = 0;
+    return fdv0;
+}
+
+double one4PiEps0()
+{
+    const double eCharge  = 1.602176634e-19;
+    const double avogadro = 6.02214076e23;
+    const double eps0SI   = 8.8541878128e-12;
+    const double eps0     = (eps0SI * 1e-9 * 1e3) / (eCharge * eCharge * avogadro);
+    return 1.0 / (4.0 * M_PI * eps0);
+}
+
+} // namespace
+
+struct nbx_context
+{
+    nbx_options      opt{};
+    int              device     = 0;
+    hipStream_t      ownStream  = nullptr;
+    hipStream_t      stream     = nullptr;
+    int              numCU      = 256;
+    int              elec       = kElecEwaldTable;
+    InteractionConst ic{};
+    float            rlist = 1.0f, rcoulomb = 1.0f;
+
+    // parameters
+    int            ntypeUser = -1;
+    int            ntype     = 0; // incl. padding
+    DevBuf<float2> nbfp;
+    DevBuf<float4> tabFDV0;
+    DevBuf<float4> shiftVec;
+
+    // topology
+    int         natoms = -1;
+    DevBuf<int> type;
+    DevBuf<float> charge;
+    DevBuf<int> exclOff, exclIdx;
+
+    // grid
+    GridDims    g{};
+    bool        gridValid = false;
+    int         ncluster  = 0;
+    DevBuf<float> x;          // device copy of coordinates in atom order
+    DevBuf<int>   cxy_na, cxy_ind, atomCol, atomRank, bucket, atomIndices, cells, nbtype, counter;
+    DevBuf<float4> xq, bb, fnb;
+
+    // list
+    bool           listValid = false;
+    int            nci = 0, ncj = 0;
+    DevBuf<int>    countJ, countI, offJ, offI, totals;
+    DevBuf<int4>   ientry;
+    DevBuf<uint2>  jentry;
+
+    // outputs
+    int             numBlocks = 0;
+    DevBuf<double>  energyPartial, energy, fshift, virialPartial, virial;
+    DevBuf<float>   fshiftPartial, fOut;
+    bool            haveEnergy = false, haveShift = false, haveVirial = false;
+};
+
+namespace
+{
+
+int fail(int code, const std::string& msg)
+{
+    g_lastError = msg;
+    return code;
+}
+
+template<typename F>
+int guarded(F&& f)
+{
+    try
+    {
+        f();
+        g_lastError.clear();
+        return NBX_SUCCESS;
+    }
+    catch (const InputError& e)
+    {
+        return fail(NBX_ERROR_INPUT, e.what());
+    }
+    catch (const StateError& e)
+    {
+        return fail(NBX_ERROR_STATE, e.what());
+    }
+    catch (const DeviceError& e)
+    {
+        return fail(NBX_ERROR_DEVICE, e.what());
+    }
+    catch (const std::exception& e)
+    {
+        return fail(NBX_ERROR_DEVICE, e.what());
+    }
+}
+
+void setDevice(nbx_context* c)
+{
+    check(hipSetDevice(c->device), "hipSetDevice");
+}
+
+void requireGrid(const nbx_context* c)
+{
+    if (!c->gridValid)
+    {
+        throw StateError("no grid: call nbx_put_on_grid first");
+    }
+}
+
+void requireList(const nbx_context* c)
+{
+    requireGrid(c);
+    if (!c->listValid)
+    {
+        throw StateError("no pair list: call nbx_construct_pairlist after nbx_put_on_grid");
+    }
+}
+
+const float* deviceCoordinates(nbx_context* c, const float* x, int onDevice)
+{
+    if (x == nullptr)
+    {
+        throw InputError("coordinates must not be NULL");
+    }
+    if (onDevice)
+    {
+        return x;
+    }
+    c->x.ensure(3 * static_cast<size_t>(c->natoms));
+    check(hipMemcpyAsync(c->x.p, x, sizeof(float) * 3 * c->natoms, hipMemcpyHostToDevice, c->stream), "upload x");
+    return c->x.p;
+}
+
+} // namespace
+
+extern "C" {
+
+const char* nbx_last_error(void)
+{
+    return g_lastError.c_str();
+}
+
+void nbx_default_options(nbx_options* o)
+{
+    if (!o)
+    {
+        return;
+    }
+    o->coulomb_type          = NBX_COULOMB_EWALD;
+    o->ewald_correction      = NBX_EWALD_TABLE;
+    o->pairlist_cutoff       = 1.0;
+    o->coulomb_cutoff        = 0.0;
+    o->ewald_rtol            = 1e-5;
+    o->epsilon_r             = 1.0;
+    o->epsilon_rf            = 1.0;
+    o->ewald_potential_shift = 0;
+    o->device                = 0;
+}
+
+int nbx_create(const nbx_options* options, nbx_context** context)
+{
+    if (!options || !context)
+    {
+        return fail(NBX_ERROR_INPUT, "nbx_create: NULL argument");
+    }
+    *context = nullptr;
+    auto* c  = new nbx_context();
+    int   rc = guarded([&] {
+        c->opt = *options;
+        const nbx_options& o = c->opt;
+        if (!(o.pairlist_cutoff > 0) || !std::isfinite(o.pairlist_cutoff))
+        {
+            throw InputError("pairlist cut-off must be positive");
+        }
+        const double rcd = o.coulomb_cutoff > 0 ? o.coulomb_cutoff : o.pairlist_cutoff;
+        if (rcd > o.pairlist_cutoff)
+        {
+            throw InputError("the interaction cut-off cannot exceed the pair-list cut-off");
+        }
+        if (o.coulomb_type < 0 || o.coulomb_type > 2)
+        {
+            throw InputError("Unsupported electrostatic interaction");
+        }
+        if (o.ewald_correction != NBX_EWALD_TABLE && o.ewald_correction != NBX_EWALD_ANALYTICAL)
+        {
+            throw InputError("unknown Ewald correction type");
+        }
+        if (o.ewald_correction == NBX_EWALD_ANALYTICAL && o.coulomb_type == NBX_COULOMB_EWALD)
+        {
+            throw InputError("the analytical Ewald correction is not available in this build; use NBX_EWALD_TABLE");
+        }
+        int ndev = 0;
+        check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
+        if (o.device < 0 || o.device >= ndev)
+        {
+            throw InputError("no such HIP device");
+        }
+        c->device = o.device;
+        setDevice(c);
+        hipDeviceProp_t prop;
+        check(hipGetDeviceProperties(&prop, c->device), "hipGetDeviceProperties");
+        c->numCU = prop.multiProcessorCount;
+        check(hipStreamCreateWithFlags(&c->ownStream, hipStreamNonBlocking), "hipStreamCreate");
+        c->stream = c->ownStream;
+
+        // setupInteractionConst (gmxsetup.cpp:226-284), float arithmetic
+        c->rlist    = static_cast<float>(o.pairlist_cutoff);
+        c->rcoulomb = static_cast<float>(rcd);
+        const float rvdw = c->rcoulomb;
+        const float p3   = rvdw * (rvdw * rvdw);
+        const float p6   = p3 * p3;
+        const float p12  = p6 * p6;
+        InteractionConst& ic = c->ic;
+        ic.rlist2    = c->rlist * c->rlist;
+        ic.rcoulomb2 = c->rcoulomb * c->rcoulomb;
+        ic.cpot_disp = static_cast<float>(-1.0 / p6);
+        ic.cpot_rep  = static_cast<float>(-1.0 / p12);
+        const float epsr = static_cast<float>(o.epsilon_r);
+        ic.epsfac        = epsr != 0 ? static_cast<float>(one4PiEps0() / epsr) : 0.0f;
+        if (o.coulomb_type == NBX_COULOMB_REACTION_FIELD)
+        {
+            // calc_rffac (mdlib/rf_util.cpp:54-66)
+            const float epsrf = static_cast<float>(o.epsilon_rf), Rc = c->rcoulomb;
+            ic.k_rf = (epsrf == 0) ? 1 / (2 * Rc * Rc * Rc) : (epsrf - epsr) / (2 * epsrf + epsr) / (Rc * Rc * Rc);
+            ic.c_rf = 1 / Rc + ic.k_rf * Rc * Rc;
+        }
+        else if (o.coulomb_type == NBX_COULOMB_CUTOFF)
+        {
+            // plain cut-off with the potential shift (forcerec.cpp:902-915)
+            ic.k_rf = 0;
+            ic.c_rf = 1 / c->rcoulomb;
+        }
+        if (o.coulomb_type == NBX_COULOMB_EWALD)
+        {
+            c->elec      = kElecEwaldTable;
+            ic.beta      = ewaldCoeff(c->rcoulomb, static_cast<float>(o.ewald_rtol));
+            if (!(ic.beta > 0))
+            {
+                throw InputError("Ewald coefficient should be > 0");
+            }
+            ic.sh_ewald  = o.ewald_potential_shift ? std::erfc(ic.beta * c->rcoulomb) / c->rcoulomb : 0.0f;
+            ic.tab_scale = ewaldTableScale(ic.beta, c->rcoulomb);
+            ic.tab_n     = static_cast<int>(c->rcoulomb * ic.tab_scale) + 2;
+            std::vector<float> fdv0 = ewaldTableFDV0(ic.tab_n, ic.tab_scale, ic.beta);
+            ic.self_v0   = fdv0[2];
+            c->tabFDV0.ensure(ic.tab_n);
+            check(hipMemcpy(c->tabFDV0.p, fdv0.data(), sizeof(float) * fdv0.size(), hipMemcpyHostToDevice), "upload table");
+        }
+        else
+        {
+            c->elec      = kElecRF;
+            ic.tab_scale = 1.0f;
+            ic.tab_n     = 0;
+            ic.self_v0   = ic.c_rf;
+        }
+        c->shiftVec.ensure(kNumShifts);
+    });
+    if (rc != NBX_SUCCESS)
+    {
+        nbx_destroy(c);
+        return rc;
+    }
+    *context = c;
+    return NBX_SUCCESS;
+}
+
+int nbx_destroy(nbx_context* c)
+{
+    if (!c)
+    {
+        return NBX_SUCCESS;
+    }
+    (void)hipSetDevice(c->device);
+    if (c->ownStream)
+    {
+        (void)hipStreamSynchronize(c->ownStream);
+        (void)hipStreamDestroy(c->ownStream);
+    }
+    delete c;
+    return NBX_SUCCESS;
+}
+
+int nbx_set_stream(nbx_context* c, void* stream)
+{
+    if (!c)
+    {
+        return fail(NBX_ERROR_INPUT, "NULL context");
+    }
+    c->stream = stream ? static_cast<hipStream_t>(stream) : c->ownStream;
+    return NBX_SUCCESS;
+}
+
+int nbx_set_nonbonded_parameters(nbx_context* c, int numTypes, const double* c6, const double* c12)
+{
+    if (!c)
+    {
+        return fail(NBX_ERROR_INPUT, "NULL context");
+    }
+    return guarded([&] {
+        if (numTypes <= 0 || !c6 || !c12)
+        {
+            throw InputError("need at least one particle type and C6/C12 tables");
+        }
+        if (numTypes > kMaxTypes)
+        {
+            throw InputError("at most " + std::to_string(kMaxTypes) + " particle types are supported");
+        }
+        setDevice(c);
+        const int t = numTypes + 1;
+        std::vector<float2> nb(static_cast<size_t>(t) * t, make_float2(0.0f, 0.0f));
+        for (int i = 0; i < numTypes; i++)
+        {
+            for (int j = 0; j < numTypes; j++)
+            {
+                const double a = c6[i * numTypes + j], b = c12[i * numTypes + j];
+                if (!std::isfinite(a) || !std::isfinite(b))
+                {
+                    throw InputError("non-finite LJ parameter");
+                }
+                if (a != c6[j * numTypes + i] || b != c12[j * numTypes + i])
+                {
+                    throw InputError("the C6/C12 tables must be symmetric");
+                }
+                // nbfp stores 6*C6 and 12*C12 in real (gmxsetup.cpp:153-166)
+                nb[i * t + j] = make_float2(static_cast<float>(a) * 6.0f, static_cast<float>(b) * 12.0f);
+            }
+        }
+        c->nbfp.ensure(nb.size());
+        check(hipMemcpy(c->nbfp.p, nb.data(), sizeof(float2) * nb.size(), hipMemcpyHostToDevice), "upload nbfp");
+        c->ntypeUser = numTypes;
+        c->ntype     = t;
+        c->gridValid = false;
+        c->listValid = false;
+    });
+}
+
+int nbx_set_topology(nbx_context* c, int numAtoms, const int* types, const float* charges, const int* exclOff,
+                     const int* exclIdx)
+{
+    if (!c)
+    {
+        return fail(NBX_ERROR_INPUT, "NULL context");
+    }
+    return guarded([&] {
+        if (c->ntypeUser < 0)
+        {
+            throw StateError("set the non-bonded parameters before the topology");
+        }
+        if (numAtoms < 0 || (numAtoms > 0 && (!types || !charges || !exclOff)))
+        {
+            throw InputError("invalid topology arrays");
+        }
+        for (int a = 0; a < numAtoms; a++)
+        {
+            if (types[a] < 0 || types[a] >= c->ntypeUser)
+            {
+                throw InputError("Missing nonbonded interaction parameters for an atom type");
+            }
+            if (!std::isfinite(charges[a]))
+            {
+                throw InputError("non-finite charge");
+            }
+        }
+        if (exclOff[0] != 0)
+        {
+            throw InputError("exclusion offsets must start at 0");
+        }
+        for (int a = 0; a < numAtoms; a++)
+        {
+            if (exclOff[a + 1] < exclOff[a])
+            {
+                throw InputError("exclusion offsets must be non-decreasing");
+            }
+        }
+        const int nnz = numAtoms > 0 ? exclOff[numAtoms] : 0;
+        for (int k = 0; k < nnz; k++)
+        {
+            if (exclIdx[k] < 0 || exclIdx[k] >= numAtoms)
+            {
+                throw InputError("exclusion index out of range");
+            }
+        }
+        setDevice(c);
+        c->natoms = numAtoms;
+        c->type.ensure(numAtoms);
+        c->charge.ensure(numAtoms);
+        c->exclOff.ensure(numAtoms + 1);
+        c->exclIdx.ensure(std::max(nnz, 1));
+        if (numAtoms > 0)
+        {
+            check(hipMemcpy(c->type.p, types, sizeof(int) * numAtoms, hipMemcpyHostToDevice), "upload types");
+            check(hipMemcpy(c->charge.p, charges, sizeof(float) * numAtoms, hipMemcpyHostToDevice), "upload q");
+        }
+        check(hipMemcpy(c->exclOff.p, exclOff, sizeof(int) * (numAtoms + 1), hipMemcpyHostToDevice), "upload excl");
+        if (nnz > 0)
+        {
+            check(hipMemcpy(c->exclIdx.p, exclIdx, sizeof(int) * nnz, hipMemcpyHostToDevice), "upload excl");
+        }
+        c->gridValid = false;
+        c->listValid = false;
+    });
+}
+
+int nbx_put_on_grid(nbx_context* c, const float* box, const float* x, int xOnDevice)
+{
+    if (!c)
+    {
+        return fail(NBX_ERROR_INPUT, "NULL context");
+    }
+    return guarded([&] {
+        if (c->natoms < 0)
+        {
+            throw StateError("set the topology before gridding");
+        }
+        if (!box)
+        {
+            throw InputError("box must not be NULL");
+        }
+        for (int d = 0; d < 3; d++)
+        {
+            if (!std::isfinite(box[d]) || !(box[d] > 0))
+            {
+                throw InputError("box edges must be positive and finite");
+            }
+            if (c->rlist > 0.5f * box[d])
+            {
+                throw InputError("the pair-list cut-off must be at most half the box edge");
+            }
+        }
+        setDevice(c);
+        GridDims& g = c->g;
+        const int n = c->natoms;
+        g.natoms    = n;
+        for (int d = 0; d < 3; d++)
+        {
+            g.box[d] = box[d];
+        }
+        // Grid::setDimensions for the simple 4x4 geometry (grid.cpp:103-262)
+        // with the density nblib passes (gmxcalculator.cpp:97): N / det(box)
+        const float det     = box[0] * (box[1] * box[2]);
+        const float density = static_cast<float>(n) / det;
+        if (n > kClusterSize)
+        {
+            const float tlen = std::cbrt(static_cast<float>(kClusterSize) / density);
+            g.ncx            = std::max(1, static_cast<int>(box[0] / tlen));
+            g.ncy            = std::max(1, static_cast<int>(box[1] / tlen));
+        }
+        else
+        {
+            g.ncx = 1;
+            g.ncy = 1;
+        }
+        g.cellSize[0]    = box[0] / g.ncx;
+        g.cellSize[1]    = box[1] / g.ncy;
+        g.invCellSize[0] = 1 / g.cellSize[0];
+        g.invCellSize[1] = 1 / g.cellSize[1];
+        g.invBoxZ        = static_cast<float>(1.0 / static_cast<double>(box[2]));
+        g.ncol           = g.ncx * g.ncy;
+        const size_t maxCluster = static_cast<size_t>(n) / kClusterSize + g.ncol + 1;
+        const size_t maxSlot    = maxCluster * kClusterSize;
+
+        // shift vectors (calc_shifts, pbc.cpp:1187-1202)
+        std::vector<float4> sv(kNumShifts);
+        for (int n_ = 0, m = -1; m <= 1; m++)
+        {
+            for (int l = -1; l <= 1; l++)
+            {
+                for (int k = -2; k <= 2; k++, n_++)
+                {
+                    sv[n_] = make_float4(k * box[0], l * box[1], m * box[2], 0.0f);
+                }
+            }
+        }
+        check(hipMemcpyAsync(c->shiftVec.p, sv.data(), sizeof(float4) * kNumShifts, hipMemcpyHostToDevice, c->stream),
+              "upload shifts");
+
+        const float* xd = deviceCoordinates(c, x, xOnDevice);
+        if (xd != c->x.p)
+        {
+            c->x.ensure(3 * static_cast<size_t>(n));
+            check(hipMemcpyAsync(c->x.p, xd, sizeof(float) * 3 * n, hipMemcpyDeviceToDevice, c->stream), "copy x");
+        }
+        c->cxy_na.ensure(g.ncol + 1);
+        c->cxy_ind.ensure(g.ncol + 1);
+        c->atomCol.ensure(n);
+        c->atomRank.ensure(n);
+        c->cells.ensure(n);
+        c->bucket.ensure(maxSlot);
+        c->atomIndices.ensure(maxSlot);
+        c->counter.ensure(4);
+        check(hipMemsetAsync(c->cxy_na.p, 0, sizeof(int) * (g.ncol + 1), c->stream), "clear counts");
+        launchGridColumns(c->stream, g, c->x.p, c->atomCol.p, c->atomRank.p, c->cxy_na.p);
+        launchGridScan(c->stream, g.ncol, c->cxy_na.p, c->cxy_ind.p, c->counter.p);
+        int hostCounts[3] = { 0, 0, 0 };
+        check(hipMemcpyAsync(&hostCounts[0], c->counter.p, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream),
+              "read count");
+        check(hipMemcpyAsync(&hostCounts[2], c->cxy_na.p + g.ncol, sizeof(int), hipMemcpyDeviceToHost, c->stream),
+              "read flag");
+        check(hipStreamSynchronize(c->stream), "grid sync");
+        if (hostCounts[2] != 0)
+        {
+            c->gridValid = false;
+            throw InputError("atoms must lie inside the box [0, L) for gridding");
+        }
+        if (hostCounts[1] > kMaxColumnAtoms)
+        {
+            c->gridValid = false;
+            throw InputError("more than " + std::to_string(kMaxColumnAtoms)
+                             + " atoms fall in one grid column: the system is far from uniform");
+        }
+        c->ncluster = hostCounts[0];
+        const int nslot = c->ncluster * kClusterSize;
+        launchGridSortColumns(c->stream, g, c->x.p, c->atomCol.p, c->atomRank.p, c->cxy_na.p, c->cxy_ind.p, c->bucket.p,
+                              c->atomIndices.p, c->cells.p);
+        c->xq.ensure(nslot);
+        c->nbtype.ensure(nslot);
+        c->bb.ensure(2 * static_cast<size_t>(c->ncluster));
+        c->fnb.ensure(nslot);
+        launchGridFill(c->stream, c->ncluster, c->x.p, c->atomIndices.p, c->type.p, c->charge.p, c->ntype - 1, c->xq.p,
+                       c->nbtype.p, c->bb.p);
+        check(hipGetLastError(), "grid kernels");
+        c->gridValid = true;
+        c->listValid = false;
+    });
+}
+
+int nbx_construct_pairlist(nbx_context* c)
+{
+    if (!c)
+    {
+        return fail(NBX_ERROR_INPUT, "NULL context");
+    }
+    return guarded([&] {
+        requireGrid(c);
+        setDevice(c);
+        const int nc = c->ncluster;
+        c->countJ.ensure(nc + 1);
+        c->countI.ensure(nc + 1);
+        c->offJ.ensure(nc + 1);
+        c->offI.ensure(nc + 1);
+        c->totals.ensure(2);
+        SearchArgs a{};
+        a.g           = c->g;
+        a.rlist       = c->rlist;
+        a.rlist2      = c->rlist * c->rlist;
+        a.ncluster    = nc;
+        a.cxy_ind     = c->cxy_ind.p;
+        a.bb          = c->bb.p;
+        a.xq          = c->xq.p;
+        a.atomIndices = c->atomIndices.p;
+        a.cells       = c->cells.p;
+        a.exclOff     = c->exclOff.p;
+        a.exclIdx     = c->exclIdx.p;
+        a.countJ      = c->countJ.p;
+        a.countI      = c->countI.p;
+        launchSearch(c->stream, a, false);
+        launchExclusiveScan(c->stream, nc, c->countJ.p, c->offJ.p, c->totals.p);
+        launchExclusiveScan(c->stream, nc, c->countI.p, c->offI.p, c->totals.p + 1);
+        int tot[2] = { 0, 0 };
+        check(hipMemcpyAsync(tot, c->totals.p, sizeof(int) * 2, hipMemcpyDeviceToHost, c->stream), "read totals");
+        check(hipStreamSynchronize(c->stream), "search sync");
+        c->ncj = tot[0];
+        c->nci = tot[1];
+        c->ientry.ensure(c->nci);
+        c->jentry.ensure(c->ncj);
+        a.offJ   = c->offJ.p;
+        a.offI   = c->offI.p;
+        a.ientry = c->ientry.p;
+        a.jentry = c->jentry.p;
+        launchSearch(c->stream, a, true);
+        check(hipGetLastError(), "search kernels");
+        // kernel grid: enough work-groups to fill every CU several times over
+        c->numBlocks = std::max(1, std::min((c->nci + 3) / 4, 8 * c->numCU));
+        c->energyPartial.ensure(2 * static_cast<size_t>(c->numBlocks));
+        c->fshiftPartial.ensure(static_cast<size_t>(c->numBlocks) * kNumShifts * 3);
+        c->energy.ensure(2);
+        c->fshift.ensure(kNumShifts * 3);
+        c->virialPartial.ensure(9 * 1024);
+        c->virial.ensure(9);
+        c->listValid  = true;
+        c->haveEnergy = c->haveShift = c->haveVirial = false;
+    });
+}
+
+int nbx_convert_coordinates(nbx_context* c, const float* x, int xOnDevice)
+{
+    if (!c)
+    {
+        return fail(NBX_ERROR_INPUT, "NULL context");
+    }
+    return guarded([&] {
+        requireGrid(c);
+        setDevice(c);
+        const float* xd = deviceCoordinates(c, x, xOnDevice);
+        launchConvertX(c->stream, c->ncluster * kClusterSize, c->atomIndices.p, xd, c->xq.p);
+        check(hipGetLastError(), "convert coordinates");
+    });
+}
+
+int nbx_dispatch_kernel(nbx_context* c, unsigned flags)
+{
+    if (!c)
+    {
+        return fail(NBX_ERROR_INPUT, "NULL context");
+    }
+    return guarded([&] {
+        requireList(c);
+        setDevice(c);
+        const bool energy = (flags & NBX_COMPUTE_ENERGY) != 0;
+        const bool shift  = (flags & (NBX_COMPUTE_SHIFT_FORCES | NBX_COMPUTE_VIRIAL)) != 0;
+        launchClearF4(c->stream, c->fnb.p, c->ncluster * kClusterSize);
+        KernelArgs a{};
+        a.ientry        = c->ientry.p;
+        a.nci           = c->nci;
+        a.jentry        = c->jentry.p;
+        a.xq            = c->xq.p;
+        a.type          = c->nbtype.p;
+        a.nbfp          = c->nbfp.p;
+        a.ntype         = c->ntype;
+        a.tabFDV0       = c->tabFDV0.p;
+        a.shiftVec      = c->shiftVec.p;
+        a.ic            = c->ic;
+        a.fnb           = c->fnb.p;
+        a.energyPartial = c->energyPartial.p;
+        a.fshiftPartial = c->fshiftPartial.p;
+        launchNonbonded(c->stream, a, c->elec, energy, shift, c->numBlocks);
+        if (energy || shift)
+        {
+            launchFinalize(c->stream, c->numBlocks, c->energyPartial.p, c->fshiftPartial.p, shift, c->energy.p,
+                           c->fshift.p);
+        }
+        check(hipGetLastError(), "nonbonded kernel");
+        c->haveEnergy = energy;
+        c->haveShift  = shift;
+        c->haveVirial = false;
+    });
+}
+
+int nbx_add_nbat_f_to_f(nbx_context* c, float* f, int fOnDevice)
+{
+    if (!c)
+    {
+        return fail(NBX_ERROR_INPUT, "NULL context");
+    }
+    return guarded([&] {
+        requireList(c);
+        if (!f)
+        {
+            throw InputError("force output must not be NULL");
+        }
+        setDevice(c);
+        float* fd = f;
+        if (!fOnDevice)
+        {
+            c->fOut.ensure(3 * static_cast<size_t>(c->natoms));
+            fd = c->fOut.p;
+        }
+        launchReduceF(c->stream, c->natoms, c->cells.p, c->fnb.p, fd);
+        if (!fOnDevice)
+        {
+            check(hipMemcpyAsync(f, fd, sizeof(float) * 3 * c->natoms, hipMemcpyDeviceToHost, c->stream), "download f");
+            check(hipStreamSynchronize(c->stream), "sync");
+        }
+    });
+}
+
+int nbx_compute(nbx_context* c, const float* x, float* f, unsigned flags, int onDevice)
+{
+    if (!c)
+    {
+        return fail(NBX_ERROR_INPUT, "NULL context");
+    }
+    return guarded([&] {
+        requireList(c);
+        if (!f)
+        {
+            throw InputError("force output must not be NULL");
+        }
+        setDevice(c);
+        const float* xd = deviceCoordinates(c, x, onDevice);
+        launchConvertX(c->stream, c->ncluster * kClusterSize, c->atomIndices.p, xd, c->xq.p);
+        if (nbx_dispatch_kernel(c, flags) != NBX_SUCCESS)
+        {
+            throw DeviceError(g_lastError);
+        }
+        float* fd = f;
+        if (!onDevice)
+        {
+            c->fOut.ensure(3 * static_cast<size_t>(c->natoms));
+            fd = c->fOut.p;
+        }
+        launchReduceF(c->stream, c->natoms, c->cells.p, c->fnb.p, fd);
+        if (flags & NBX_COMPUTE_VIRIAL)
+        {
+            const int vb = std::max(1, std::min(1024, (c->natoms + 255) / 256));
+            launchVirial(c->stream, c->natoms, xd, fd, c->fshift.p, c->shiftVec.p, c->virialPartial.p, vb, c->virial.p);
+            c->haveVirial = true;
+        }
+        check(hipGetLastError(), "compute");
+        if (!onDevice)
+        {
+            check(hipMemcpyAsync(f, fd, sizeof(float) * 3 * c->natoms, hipMemcpyDeviceToHost, c->stream), "download f");
+            check(hipStreamSynchronize(c->stream), "sync");
+        }
+    });
+}
+
+int nbx_get_energies(nbx_context* c, double* vlj, double* vc)
+{
+    if (!c)
+    {
+        return fail(NBX_ERROR_INPUT, "NULL context");
+    }
+    return guarded([&] {
+        if (!c->haveEnergy)
+        {
+            throw StateError("no energies: compute with NBX_COMPUTE_ENERGY first");
+        }
+        setDevice(c);
+        double e[2];
+        check(hipMemcpyAsync(e, c->energy.p, sizeof(e), hipMemcpyDeviceToHost, c->stream), "read energies");
+        check(hipStreamSynchronize(c->stream), "sync");
+        if (vlj)
+        {
+            *vlj = e[0];
+        }
+        if (vc)
+        {
+            *vc = e[1];
+        }
+    });
+}
+
+int nbx_get_shift_forces(nbx_context* c, double* fshift)
+{
+    if (!c)
+    {
+        return fail(NBX_ERROR_INPUT, "NULL context");
+    }
+    return guarded([&] {
+        if (!c->haveShift)
+        {
+            throw StateError("no shift forces: compute with NBX_COMPUTE_SHIFT_FORCES first");
+        }
+        setDevice(c);
+        check(hipMemcpyAsync(fshift, c->fshift.p, sizeof(double) * kNumShifts * 3, hipMemcpyDeviceToHost, c->stream),
+              "read fshift");
+        check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+int nbx_get_virial(nbx_context* c, double* virial)
+{
+    if (!c)
+    {
+        return fail(NBX_ERROR_INPUT, "NULL context");
+    }
+    return guarded([&] {
+        if (!c->haveVirial)
+        {
+            throw StateError("no virial: compute with NBX_COMPUTE_VIRIAL first");
+        }
+        setDevice(c);
+        check(hipMemcpyAsync(virial, c->virial.p, sizeof(double) * 9, hipMemcpyDeviceToHost, c->stream), "read virial");
+        check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+int nbx_synchronize(nbx_context* c)
+{
+    if (!c)
+    {
+        return fail(NBX_ERROR_INPUT, "NULL context");
+    }
+    return guarded([&] {
+        setDevice(c);
+        check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+int nbx_get_pairlist_info(nbx_context* c, nbx_pairlist_info* info)
+{
+    if (!c || !info)
+    {
+        return fail(NBX_ERROR_INPUT, "NULL argument");
+    }
+    return guarded([&] {
+        info->num_atoms     = std::max(c->natoms, 0);
+        info->num_clusters  = c->gridValid ? c->ncluster : 0;
+        info->num_columns_x = c->gridValid ? c->g.ncx : 0;
+        info->num_columns_y = c->gridValid ? c->g.ncy : 0;
+        info->num_i_entries = c->listValid ? c->nci : 0;
+        info->num_j_entries = c->listValid ? c->ncj : 0;
+    });
+}
+
+int nbx_get_grid(nbx_context* c, int* atomIndices, int* cells)
+{
+    if (!c)
+    {
+        return fail(NBX_ERROR_INPUT, "NULL context");
+    }
+    return guarded([&] {
+        requireGrid(c);
+        setDevice(c);
+        if (atomIndices)
+        {
+            check(hipMemcpyAsync(atomIndices, c->atomIndices.p, sizeof(int) * c->ncluster * kClusterSize,
+                                 hipMemcpyDeviceToHost, c->stream),
+                  "read grid");
+        }
+        if (cells)
+        {
+            check(hipMemcpyAsync(cells, c->cells.p, sizeof(int) * c->natoms, hipMemcpyDeviceToHost, c->stream),
+                  "read cells");
+        }
+        check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+int nbx_get_pairlist(nbx_context* c, int* iEntries, uint32_t* jEntries)
+{
+    if (!c)
+    {
+        return fail(NBX_ERROR_INPUT, "NULL context");
+    }
+    return guarded([&] {
+        requireList(c);
+        setDevice(c);
+        if (iEntries)
+        {
+            check(hipMemcpyAsync(iEntries, c->ientry.p, sizeof(int4) * c->nci, hipMemcpyDeviceToHost, c->stream),
+                  "read ientries");
+        }
+        if (jEntries)
+        {
+            check(hipMemcpyAsync(jEntries, c->jentry.p, sizeof(uint2) * c->ncj, hipMemcpyDeviceToHost, c->stream),
+                  "read jentries");
+        }
+        check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+int64_t nbx_list_pairs(nbx_context* c, double r, int64_t* out, int64_t cap)
+{
+    if (!c)
+    {
+        return -fail(NBX_ERROR_INPUT, "NULL context");
+    }
+    int64_t result = 0;
+    int     rc     = guarded([&] {
+        requireList(c);
+        setDevice(c);
+        DevBuf<unsigned long long> counter;
+        counter.ensure(1);
+        DevBuf<long long> buf;
+        buf.ensure(static_cast<size_t>(std::max<int64_t>(cap, 1)));
+        check(hipMemsetAsync(counter.p, 0, sizeof(unsigned long long), c->stream), "clear counter");
+        const float rf = static_cast<float>(r);
+        launchExpandPairs(c->stream, c->ientry.p, c->nci, c->jentry.p, c->xq.p, c->atomIndices.p, c->shiftVec.p,
+                          rf * rf, counter.p, buf.p, cap);
+        check(hipGetLastError(), "expand pairs");
+        unsigned long long n = 0;
+        check(hipMemcpyAsync(&n, counter.p, sizeof(n), hipMemcpyDeviceToHost, c->stream), "read count");
+        check(hipStreamSynchronize(c->stream), "sync");
+        result = static_cast<int64_t>(n);
+        if (out && cap > 0)
+        {
+            const int64_t m = std::min<int64_t>(result, cap);
+            check(hipMemcpy(out, buf.p, sizeof(long long) * m, hipMemcpyDeviceToHost), "read pairs");
+        }
+    });
+    return rc == NBX_SUCCESS ? result : -rc;
+}
+
+int nbx_get_interaction_constants(nbx_context* c, double* out)
+{
+    if (!c || !out)
+    {
+        return fail(NBX_ERROR_INPUT, "NULL argument");
+    }
+    const InteractionConst& ic = c->ic;
+    out[0] = ic.beta;
+    out[1] = ic.tab_scale;
+    out[2] = ic.tab_n;
+    out[3] = ic.epsfac;
+    out[4] = ic.k_rf;
+    out[5] = ic.c_rf;
+    out[6] = ic.sh_ewald;
+    out[7] = ic.cpot_disp;
+    out[8] = ic.cpot_rep;
+    return NBX_SUCCESS;
+}
+
+} // extern "C"

@@ -1,0 +1,289 @@
+// Coordinate and force buffer operations and output reductions.
+//
+//   convertX    nbnxn_atomdata_copy_x_to_nbat_x (atomdata.cpp:1018-1073):
+//               gather x[atomIndices[slot]] into the float4 {x,y,z,q} slots,
+//               fillers untouched
+//   reduceF     atomdata_add_nbat_f_to_f after nblib's zero fill
+//               (gmxcalculator.cpp:76-82, atomdata.cpp:1181-1228):
+//               f[a] = fnb[cells[a]]
+//   finalize    energy partials (reduce_energies_over_lists,
+//               kernel_common.cpp:109-133) and shift-force partials
+//               (nbnxn_atomdata_add_nbat_fshift_to_fshift,
+//               atomdata.cpp:1471-1487) summed in double in a fixed order
+//   virial      -1/2 sum x (x) f - 1/2 sum shift (x) fshift in double
+//               (calcvir.cpp:103-148, sim_util.cpp:157-177)
+// All are HBM-bandwidth bound: one coalesced pass over the arrays.
+#include "nbx_internal.h"
+
+namespace nbx
+{
+
+namespace
+{
+
+__global__ void clearF4Kernel(float4* __restrict__ p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+    {
+        p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+__global__ void convertXKernel(int nslot, const int* __restrict__ atomIndices, const float* __restrict__ x,
+                               float4* __restrict__ xq)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslot)
+    {
+        return;
+    }
+    const int a = atomIndices[s];
+    if (a >= 0)
+    {
+        float4 v = xq[s];
+        v.x      = x[3 * a + 0];
+        v.y      = x[3 * a + 1];
+        v.z      = x[3 * a + 2];
+        xq[s]    = v;
+    }
+}
+
+__global__ void reduceFKernel(int natoms, const int* __restrict__ cells, const float4* __restrict__ fnb,
+                              float* __restrict__ f)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= natoms)
+    {
+        return;
+    }
+    const float4 v = fnb[cells[a]];
+    f[3 * a + 0]   = v.x;
+    f[3 * a + 1]   = v.y;
+    f[3 * a + 2]   = v.z;
+}
+
+constexpr int kFinalizeThreads = 256;
+
+__global__ __launch_bounds__(kFinalizeThreads) void finalizeKernel(int numPartials, const double* __restrict__ ePart,
+                                                                   const float* __restrict__ fshPart, int doShift,
+                                                                   double* __restrict__ eOut,
+                                                                   double* __restrict__ fshOut)
+{
+    // block 0: energies, blocks 1..: one shift-force component each
+    __shared__ double s[kFinalizeThreads];
+    const int comp = blockIdx.x;
+    double    acc  = 0.0;
+    if (comp < 2)
+    {
+        for (int p = threadIdx.x; p < numPartials; p += kFinalizeThreads)
+        {
+            acc += ePart[2 * p + comp];
+        }
+    }
+    else
+    {
+        const int k = comp - 2;
+        for (int p = threadIdx.x; p < numPartials; p += kFinalizeThreads)
+        {
+            acc += static_cast<double>(fshPart[p * kNumShifts * 3 + k]);
+        }
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = kFinalizeThreads / 2; off > 0; off >>= 1)
+    {
+        if (threadIdx.x < off)
+        {
+            s[threadIdx.x] += s[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        if (comp < 2)
+        {
+            eOut[comp] = s[0];
+        }
+        else
+        {
+            fshOut[comp - 2] = s[0];
+        }
+    }
+    (void)doShift;
+}
+
+constexpr int kVirThreads = 256;
+
+__global__ __launch_bounds__(kVirThreads) void virialPartialKernel(int natoms, const float* __restrict__ x,
+                                                                   const float* __restrict__ f,
+                                                                   double* __restrict__ part)
+{
+    __shared__ double s[9][kVirThreads / 64];
+    double v[9] = { 0, 0, 0, 0, 0, 0, 0, 0, 0 };
+    for (int a = blockIdx.x * kVirThreads + threadIdx.x; a < natoms; a += gridDim.x * kVirThreads)
+    {
+        const double xa[3] = { x[3 * a], x[3 * a + 1], x[3 * a + 2] };
+        const double fa[3] = { f[3 * a], f[3 * a + 1], f[3 * a + 2] };
+#pragma unroll
+        for (int d1 = 0; d1 < 3; d1++)
+        {
+#pragma unroll
+            for (int d2 = 0; d2 < 3; d2++)
+            {
+                v[d1 * 3 + d2] -= 0.5 * xa[d1] * fa[d2];
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 9; k++)
+    {
+        for (int m = 1; m < 64; m *= 2)
+        {
+            v[k] += __shfl_xor(v[k], m, 64);
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x / 64;
+    if (lane == 0)
+    {
+        for (int k = 0; k < 9; k++)
+        {
+            s[k][wave] = v[k];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 9)
+    {
+        double t = 0;
+        for (int w = 0; w < kVirThreads / 64; w++)
+        {
+            t += s[threadIdx.x][w];
+        }
+        part[blockIdx.x * 9 + threadIdx.x] = t;
+    }
+}
+
+__global__ void virialFinalKernel(int numPartials, const double* __restrict__ part, const double* __restrict__ fshift,
+                                  const float4* __restrict__ shiftVec, double* __restrict__ out)
+{
+    const int k = threadIdx.x;
+    if (k >= 9)
+    {
+        return;
+    }
+    const int d1 = k / 3, d2 = k % 3;
+    double    t  = 0.0;
+    for (int p = 0; p < numPartials; p++)
+    {
+        t += part[p * 9 + k];
+    }
+    for (int sh = 0; sh < kNumShifts; sh++)
+    {
+        const float4 sv  = shiftVec[sh];
+        const double svd = (d1 == 0) ? sv.x : ((d1 == 1) ? sv.y : sv.z);
+        t -= 0.5 * svd * fshift[sh * 3 + d2];
+    }
+    out[k] = t;
+}
+
+// One thread per j-entry: the 16 atom pairs with canonical d^2 < r2.
+__global__ void expandPairsKernel(const int4* __restrict__ ientry, int nci, const uint2* __restrict__ jentry,
+                                  const float4* __restrict__ xq, const int* __restrict__ atomIndices,
+                                  const float4* __restrict__ shiftVec, float r2, unsigned long long* counter,
+                                  long long* __restrict__ out, long long cap)
+{
+    const int e = blockIdx.x;
+    if (e >= nci)
+    {
+        return;
+    }
+    const int4   ie = ientry[e];
+    const float4 sv = shiftVec[ie.y];
+    for (int k = ie.z + threadIdx.x; k < ie.w; k += blockDim.x)
+    {
+        const int cj = static_cast<int>(jentry[k].x);
+        for (int i = 0; i < kClusterSize; i++)
+        {
+            const int ai = atomIndices[ie.x * kClusterSize + i];
+            if (ai < 0)
+            {
+                continue;
+            }
+            const float4 xi = xq[ie.x * kClusterSize + i];
+            const float  xs = __fadd_rn(xi.x, sv.x), ys = __fadd_rn(xi.y, sv.y), zs = __fadd_rn(xi.z, sv.z);
+            for (int j = 0; j < kClusterSize; j++)
+            {
+                const int aj = atomIndices[cj * kClusterSize + j];
+                if (aj < 0 || (ie.y == kCentral && cj == ie.x && j <= i))
+                {
+                    continue;
+                }
+                const float4 xj = xq[cj * kClusterSize + j];
+                const float  dx = __fsub_rn(xs, xj.x), dy = __fsub_rn(ys, xj.y), dz = __fsub_rn(zs, xj.z);
+                const float  d2 = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+                if (d2 < r2)
+                {
+                    const unsigned long long n = atomicAdd(counter, 1ull);
+                    if (static_cast<long long>(n) < cap)
+                    {
+                        const long long lo = ai < aj ? ai : aj, hi = ai < aj ? aj : ai;
+                        out[n]             = (lo << 32) | hi;
+                    }
+                }
+            }
+        }
+    }
+}
+
+} // namespace
+
+void launchClearF4(hipStream_t s, float4* p, int n)
+{
+    if (n > 0)
+    {
+        clearF4Kernel<<<(n + 255) / 256, 256, 0, s>>>(p, n);
+    }
+}
+
+void launchConvertX(hipStream_t s, int nslot, const int* atomIndices, const float* x, float4* xq)
+{
+    if (nslot > 0)
+    {
+        convertXKernel<<<(nslot + 255) / 256, 256, 0, s>>>(nslot, atomIndices, x, xq);
+    }
+}
+
+void launchReduceF(hipStream_t s, int natoms, const int* cells, const float4* fnb, float* f)
+{
+    if (natoms > 0)
+    {
+        reduceFKernel<<<(natoms + 255) / 256, 256, 0, s>>>(natoms, cells, fnb, f);
+    }
+}
+
+void launchFinalize(hipStream_t s, int numPartials, const double* energyPartial, const float* fshiftPartial,
+                    bool shiftForces, double* energyOut, double* fshiftOut)
+{
+    const int blocks = shiftForces ? 2 + kNumShifts * 3 : 2;
+    finalizeKernel<<<blocks, kFinalizeThreads, 0, s>>>(numPartials, energyPartial, fshiftPartial, shiftForces ? 1 : 0,
+                                                       energyOut, fshiftOut);
+}
+
+void launchVirial(hipStream_t s, int natoms, const float* x, const float* f, const double* fshift,
+                  const float4* shiftVec, double* virialPartial, int numBlocks, double* virialOut)
+{
+    virialPartialKernel<<<numBlocks, kVirThreads, 0, s>>>(natoms, x, f, virialPartial);
+    virialFinalKernel<<<1, 64, 0, s>>>(numBlocks, virialPartial, fshift, shiftVec, virialOut);
+}
+
+void launchExpandPairs(hipStream_t s, const int4* ientry, int nci, const uint2* jentry, const float4* xq,
+                       const int* atomIndices, const float4* shiftVec, float r2, unsigned long long* counter,
+                       long long* out, long long cap)
+{
+    if (nci > 0)
+    {
+        expandPairsKernel<<<nci, 64, 0, s>>>(ientry, nci, jentry, xq, atomIndices, shiftVec, r2, counter, out, cap);
+    }
+}
+
+} // namespace nbx

@@ -1,0 +1,326 @@
+// Cluster-pair LJ + Coulomb (Ewald real space or reaction-field) kernel for
+// CDNA4 (gfx950, wave64).
+//
+// Physics and masking follow the reference's plain-C kernel
+// (src/gromacs/nbnxm/kernels_reference/kernel_ref_outer.h:85-409 and
+// kernel_ref_inner.h:44-375, the "LJ + Coulomb with exclusion check" loop):
+//   skipmask = 0 beyond rcoulomb and for j <= i on the central diagonal;
+//   rsq is floored at 3.82e-7 before 1/sqrt; LJ uses nbfp = {6*C6, 12*C12}
+//   with potential-shift energies; excluded pairs inside the cut-off keep the
+//   Ewald (or RF) correction; the self term -1/2 q_i^2 epsfac V(0) is added
+//   once per central i-entry; shift forces are the sum of i-forces per shift.
+//
+// Mapping to the hardware: a wavefront owns one i-entry (4 i-atoms, one
+// shift) and walks its j-list four j-clusters at a time, so one instruction
+// covers 4 i x 4 j x 4 j-clusters = 64 pairs; lane = i + 4*j + 16*k.
+//   * j coordinates+charge arrive as one float4 load per lane (16 atoms x 16 B
+//     per instruction, contiguous per cluster), types as one int load;
+//   * the i-force stays in registers for the whole entry and is reduced over
+//     the 16 lanes that share i with cross-lane shuffles once per entry;
+//   * the j-force of each tile is summed over the 4 i-lanes of a quad with two
+//     DPP adds (quad_perm) and scattered with one float atomic per component,
+//     lane i = 0,1,2 carrying x,y,z;
+//   * LJ parameters and the Ewald F/D/V table sit in LDS;
+//   * energies accumulate per lane in float over one entry, then in double;
+//     per-workgroup double partials are summed by a separate kernel, which
+//     keeps the energies deterministic.
+// Pair arithmetic is FP32 on the vector ALUs; no matrix instructions are
+// used (the pair interaction is not a dense contraction).
+#include "nbx_internal.h"
+
+namespace nbx
+{
+
+namespace
+{
+
+constexpr int kWave       = 64;
+constexpr int kWavesBlock = 4;
+constexpr int kThreads    = kWave * kWavesBlock;
+
+__device__ __forceinline__ float dppQuadXor1(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ float dppQuadXor2(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ float quadSum(float v)
+{
+    v = v + dppQuadXor1(v);
+    return v + dppQuadXor2(v);
+}
+
+template<int kElec, bool kEnergy, bool kShift>
+__global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
+{
+    extern __shared__ float4 smem4[];
+    const int ntype  = a.ntype;
+    const int ntype2 = ntype * ntype;
+    float2*   sNbfp  = reinterpret_cast<float2*>(smem4);
+    float4*   sTab   = smem4 + ((ntype2 + 1) / 2);
+    const int tabN   = (kElec == kElecEwaldTable) ? a.ic.tab_n : 0;
+    float*    sFsh   = reinterpret_cast<float*>(sTab + tabN);
+    double*   sEner  = reinterpret_cast<double*>(sFsh + (kShift ? kNumShifts * 3 + 1 : 0));
+
+    for (int t = threadIdx.x; t < ntype2; t += kThreads)
+    {
+        sNbfp[t] = a.nbfp[t];
+    }
+    if (kElec == kElecEwaldTable)
+    {
+        for (int t = threadIdx.x; t < tabN; t += kThreads)
+        {
+            sTab[t] = a.tabFDV0[t];
+        }
+    }
+    if (kShift)
+    {
+        for (int t = threadIdx.x; t < kNumShifts * 3; t += kThreads)
+        {
+            sFsh[t] = 0.0f;
+        }
+    }
+    __syncthreads();
+
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int ii   = lane & 3;
+    const int jj   = (lane >> 2) & 3;
+    const int kk   = lane >> 4;
+
+    const InteractionConst ic = a.ic;
+    const float rc2       = ic.rcoulomb2;
+    const float k_rf2     = 2.0f * ic.k_rf;
+    const float halfsp    = 0.5f / ic.tab_scale;
+    const float selfFac   = 0.5f * ic.self_v0;
+    const float sixth     = 1.0f / 6.0f;
+    const float twelfth   = 1.0f / 12.0f;
+
+    double eVdw = 0.0, eCoul = 0.0;
+
+    for (int e = blockIdx.x * kWavesBlock + wave; e < a.nci; e += gridDim.x * kWavesBlock)
+    {
+        const int4 ie      = a.ientry[__builtin_amdgcn_readfirstlane(e)];
+        const int  ci      = __builtin_amdgcn_readfirstlane(ie.x);
+        const int  shift   = __builtin_amdgcn_readfirstlane(ie.y);
+        const int  js      = __builtin_amdgcn_readfirstlane(ie.z);
+        const int  je      = __builtin_amdgcn_readfirstlane(ie.w);
+        const bool central = (shift == kCentral);
+
+        const float4 sv  = a.shiftVec[shift];
+        const float4 xqi = a.xq[ci * kClusterSize + ii];
+        const float  xi  = xqi.x + sv.x;
+        const float  yi  = xqi.y + sv.y;
+        const float  zi  = xqi.z + sv.z;
+        const float  qi  = ic.epsfac * xqi.w;
+        const int    tiOff = a.type[ci * kClusterSize + ii] * ntype;
+
+        float fix = 0.0f, fiy = 0.0f, fiz = 0.0f;
+        float vlj = 0.0f, vc = 0.0f;
+        if (kEnergy && central && jj == 0 && kk == 0)
+        {
+            // kernel_ref_outer.h:286-326, Vc -= qi * q_i * V(0)/2
+            vc -= qi * xqi.w * selfFac;
+        }
+
+        for (int jb = js; jb < je; jb += 4)
+        {
+            const int   jidx  = jb + kk;
+            const bool  valid = jidx < je;
+            const uint2 jent  = a.jentry[valid ? jidx : js];
+            const int   cj    = static_cast<int>(jent.x);
+            const int   sj    = cj * kClusterSize + jj;
+            const float4 xqj  = a.xq[sj];
+            const int   tj    = a.type[sj];
+
+            const float interact = static_cast<float>((jent.y >> (ii * 4 + jj)) & 1u);
+            float skipmask = (!valid || (central && cj == ci && jj <= ii)) ? 0.0f : 1.0f;
+
+            const float dx  = xi - xqj.x;
+            const float dy  = yi - xqj.y;
+            const float dz  = zi - xqj.z;
+            float       rsq = dx * dx + dy * dy + dz * dz;
+            skipmask        = (rsq >= rc2) ? 0.0f : skipmask;
+            rsq             = fmaxf(rsq, kMinRsq);
+            const float rinv   = __builtin_amdgcn_rsqf(rsq) * skipmask;
+            const float rinvsq = rinv * rinv;
+
+            const float2 c       = sNbfp[tiOff + tj];
+            const float  rinvsix = interact * rinvsq * rinvsq * rinvsq;
+            const float  FrLJ6   = c.x * rinvsix;
+            const float  FrLJ12  = c.y * rinvsix * rinvsix;
+            const float  frLJ    = FrLJ12 - FrLJ6;
+            if (kEnergy)
+            {
+                const float VLJ = ((FrLJ12 + c.y * ic.cpot_rep) * twelfth - (FrLJ6 + c.x * ic.cpot_disp) * sixth)
+                                  * interact * skipmask;
+                vlj += VLJ;
+            }
+
+            const float qq = skipmask * qi * xqj.w;
+            float       fcoul;
+            if (kElec == kElecRF)
+            {
+                fcoul = qq * (interact * rinv * rinvsq - k_rf2);
+                if (kEnergy)
+                {
+                    vc += qq * (interact * rinv + ic.k_rf * rsq - ic.c_rf);
+                }
+            }
+            else
+            {
+                const float  rs    = rsq * rinv * ic.tab_scale;
+                // clamped so that no input (NaN, Inf) can index past the table
+                const int    ri    = min(static_cast<int>(rs), tabN - 2);
+                const float  frac  = rs - static_cast<float>(ri);
+                const float4 tab   = sTab[ri];
+                const float  fexcl = tab.x + frac * tab.y;
+                fcoul              = interact * rinvsq - fexcl;
+                if (kEnergy)
+                {
+                    vc += qq * (interact * (rinv - ic.sh_ewald) - (tab.z - halfsp * frac * (tab.x + fexcl)));
+                }
+                fcoul *= qq * rinv;
+            }
+
+            const float fscal = frLJ * rinvsq + fcoul;
+            const float fx    = fscal * dx;
+            const float fy    = fscal * dy;
+            const float fz    = fscal * dz;
+            fix += fx;
+            fiy += fy;
+            fiz += fz;
+            // j-force: sum over the four i-lanes of the quad, lane i = 0,1,2
+            // scatters component x,y,z
+            const float gx   = quadSum(fx);
+            const float gy   = quadSum(fy);
+            const float gz   = quadSum(fz);
+            const float comp = (ii == 0) ? gx : ((ii == 1) ? gy : gz);
+            if (valid && ii < 3)
+            {
+                unsafeAtomicAdd(reinterpret_cast<float*>(a.fnb + sj) + ii, -comp);
+            }
+        }
+
+        // i-force: reduce over the 16 lanes sharing ii (lane bits 2..5)
+#pragma unroll
+        for (int m = 4; m < kWave; m *= 2)
+        {
+            fix += __shfl_xor(fix, m, kWave);
+            fiy += __shfl_xor(fiy, m, kWave);
+            fiz += __shfl_xor(fiz, m, kWave);
+        }
+        if (kk == 0 && jj < 3)
+        {
+            const float comp = (jj == 0) ? fix : ((jj == 1) ? fiy : fiz);
+            unsafeAtomicAdd(reinterpret_cast<float*>(a.fnb + ci * kClusterSize + ii) + jj, comp);
+        }
+        if (kShift)
+        {
+            float sx = quadSum(fix);
+            float sy = quadSum(fiy);
+            float sz = quadSum(fiz);
+            if (lane < 3)
+            {
+                const float comp = (lane == 0) ? sx : ((lane == 1) ? sy : sz);
+                atomicAdd(sFsh + shift * 3 + lane, comp);
+            }
+        }
+        if (kEnergy)
+        {
+            eVdw += static_cast<double>(vlj);
+            eCoul += static_cast<double>(vc);
+        }
+    }
+
+    if (kEnergy)
+    {
+#pragma unroll
+        for (int m = 1; m < kWave; m *= 2)
+        {
+            eVdw += __shfl_xor(eVdw, m, kWave);
+            eCoul += __shfl_xor(eCoul, m, kWave);
+        }
+        if (lane == 0)
+        {
+            sEner[2 * wave]     = eVdw;
+            sEner[2 * wave + 1] = eCoul;
+        }
+    }
+    __syncthreads();
+    if (kEnergy && threadIdx.x < 2)
+    {
+        double s = 0.0;
+        for (int w = 0; w < kWavesBlock; w++)
+        {
+            s += sEner[2 * w + threadIdx.x];
+        }
+        a.energyPartial[2 * blockIdx.x + threadIdx.x] = s;
+    }
+    if (kShift)
+    {
+        for (int t = threadIdx.x; t < kNumShifts * 3; t += kThreads)
+        {
+            a.fshiftPartial[blockIdx.x * kNumShifts * 3 + t] = sFsh[t];
+        }
+    }
+}
+
+template<int kElec>
+void launchElec(hipStream_t s, const KernelArgs& a, bool energy, bool shiftForces, int numBlocks, int lds)
+{
+    if (energy && shiftForces)
+    {
+        nonbondedKernel<kElec, true, true><<<numBlocks, kThreads, lds, s>>>(a);
+    }
+    else if (energy)
+    {
+        nonbondedKernel<kElec, true, false><<<numBlocks, kThreads, lds, s>>>(a);
+    }
+    else if (shiftForces)
+    {
+        nonbondedKernel<kElec, false, true><<<numBlocks, kThreads, lds, s>>>(a);
+    }
+    else
+    {
+        nonbondedKernel<kElec, false, false><<<numBlocks, kThreads, lds, s>>>(a);
+    }
+}
+
+} // namespace
+
+int nonbondedLdsBytes(int ntype, int tabN, int elec, bool /*energy*/)
+{
+    int bytes = ((ntype * ntype + 1) / 2) * 16;
+    if (elec == kElecEwaldTable)
+    {
+        bytes += tabN * 16;
+    }
+    bytes += (kNumShifts * 3 + 1) * 4;
+    bytes = (bytes + 15) & ~15;
+    bytes += kWavesBlock * 2 * 8;
+    return bytes;
+}
+
+void launchNonbonded(hipStream_t s, const KernelArgs& a, int elec, bool energy, bool shiftForces, int numBlocks)
+{
+    if (a.nci == 0 || numBlocks == 0)
+    {
+        return;
+    }
+    const int lds = nonbondedLdsBytes(a.ntype, a.ic.tab_n, elec, energy);
+    if (elec == kElecRF)
+    {
+        launchElec<kElecRF>(s, a, energy, shiftForces, numBlocks, lds);
+    }
+    else
+    {
+        launchElec<kElecEwaldTable>(s, a, energy, shiftForces, numBlocks, lds);
+    }
+}
+
+} // namespace nbx

@@ -1,0 +1,136 @@
+// Internal declarations shared by the HIP translation units of libnbx.
+//
+// Data layout in HBM (one context = one system on one device):
+//   grid      cxy_na[ncol], cxy_ind[ncol+1] (first cluster of each column),
+//             atomIndices[nslot] (-1 = filler), cells[natoms] (atom -> slot),
+//             bb[2*ncluster] (float4 lower, float4 upper, w unused)
+//   atoms     xq[nslot] float4 {x, y, z, q}, type[nslot] int
+//   list      ientry[nci] int4 {ci, shift, cj_start, cj_end},
+//             jentry[ncj] uint2 {cj, exclusion mask (bit 4*i + j)}
+//   forces    fnb[nslot] float4 accumulation buffer, f[natoms*3] output
+//   params    nbfp[(T+1)^2] float2 {6*C6, 12*C12}, Ewald FDV0 table float4,
+//             shift vectors float4[45]
+// A slot is one atom position of a 4-atom cluster; nslot = 4 * ncluster.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+namespace nbx
+{
+
+constexpr int kClusterSize = 4;
+constexpr int kNumShifts   = 45;
+constexpr int kCentral     = 22;
+constexpr unsigned kMaskAll  = 0xffffffffu;
+constexpr unsigned kMaskDiag = 0x08ceu;
+constexpr float kFarAway     = -1000000.0f;
+// LJ types held in LDS by the kernel: (63 + 1)^2 float2 = 32 KiB
+constexpr int kMaxTypes = 63;
+// the per-column rank sort is quadratic in the column population
+constexpr int kMaxColumnAtoms = 65536;
+// rsq floor for float (pairlist.h:146): r > 6.2e-4 nm
+constexpr float kMinRsq = 3.82e-07f;
+
+// Coulomb flavours of the kernel
+enum Elec : int
+{
+    kElecRF         = 0, // reaction-field, also plain cut-off with eps_rf = 1
+    kElecEwaldTable = 1, // quadratic-spline correction table (plain-C reference)
+    kElecEwaldAna   = 2, // analytical correction
+};
+
+struct InteractionConst
+{
+    float rlist2;
+    float rcoulomb2;
+    float epsfac;
+    float k_rf;
+    float c_rf;
+    float beta;
+    float sh_ewald;
+    float cpot_disp;
+    float cpot_rep;
+    float tab_scale;
+    int   tab_n;
+    float self_v0; // V(0) of the Ewald correction (or c_rf for RF)
+};
+
+struct GridDims
+{
+    int   natoms;
+    int   ncx, ncy, ncol;
+    float box[3];
+    float cellSize[2];
+    float invCellSize[2];
+    float invBoxZ;
+};
+
+// -- grid.hip
+void launchGridColumns(hipStream_t s, const GridDims& g, const float* x, int* atomCol, int* atomRank, int* cxy_na);
+void launchGridScan(hipStream_t s, int ncol, const int* cxy_na, int* cxy_ind, int* ncluster_out);
+void launchGridSortColumns(hipStream_t s, const GridDims& g, const float* x, const int* atomCol, const int* atomRank,
+                           const int* cxy_na, const int* cxy_ind, int* bucket, int* atomIndices, int* cells);
+void launchGridFill(hipStream_t s, int ncluster, const float* x, const int* atomIndices, const int* type, const float* q,
+                    int paddingType, float4* xq, int* nbtype, float4* bb);
+
+// -- search.hip
+struct SearchArgs
+{
+    GridDims      g;
+    float         rlist;
+    float         rlist2;
+    int           ncluster;
+    const int*    cxy_ind;
+    const float4* bb;
+    const float4* xq;
+    const int*    atomIndices;
+    const int*    cells;
+    const int*    exclOff;
+    const int*    exclIdx;
+    // outputs (count pass writes counts; fill pass writes the lists)
+    int*   countJ;   // per ci: number of j entries
+    int*   countI;   // per ci: number of i entries
+    const int* offJ; // per ci: first j entry (exclusive scan of countJ)
+    const int* offI; // per ci: first i entry
+    int4*  ientry;
+    uint2* jentry;
+};
+void launchSearch(hipStream_t s, const SearchArgs& a, bool fill);
+void launchExclusiveScan(hipStream_t s, int n, const int* in, int* out, int* total);
+
+// -- kernel.hip
+struct KernelArgs
+{
+    const int4*   ientry;
+    int           nci;
+    const uint2*  jentry;
+    const float4* xq;
+    const int*    type;
+    const float2* nbfp;
+    int           ntype; // including the padding type
+    const float4* tabFDV0;
+    const float4* shiftVec;
+    InteractionConst ic;
+    float4*  fnb;
+    double*  energyPartial; // [gridDim.x][2]
+    float*   fshiftPartial; // [gridDim.x][45*3]
+};
+void launchNonbonded(hipStream_t s, const KernelArgs& a, int elec, bool energy, bool shiftForces, int numBlocks);
+int  nonbondedLdsBytes(int ntype, int tabN, int elec, bool energy);
+
+// -- buffers.hip
+void launchClearF4(hipStream_t s, float4* p, int n);
+void launchConvertX(hipStream_t s, int nslot, const int* atomIndices, const float* x, float4* xq);
+void launchReduceF(hipStream_t s, int natoms, const int* cells, const float4* fnb, float* f);
+void launchFinalize(hipStream_t s, int numPartials, const double* energyPartial, const float* fshiftPartial,
+                    bool shiftForces, double* energyOut, double* fshiftOut);
+void launchVirial(hipStream_t s, int natoms, const float* x, const float* f, const double* fshift, const float4* shiftVec,
+                  double* virialPartial, int numBlocks, double* virialOut);
+void launchExpandPairs(hipStream_t s, const int4* ientry, int nci, const uint2* jentry, const float4* xq,
+                       const int* atomIndices, const float4* shiftVec, float r2, unsigned long long* counter,
+                       long long* out, long long cap);
+
+} // namespace nbx

@@ -1,0 +1,380 @@
+// Cluster-pair search on the device.
+//
+// Same list contract as the reference's simple search
+// (src/gromacs/nbnxm/pairlist.cpp:3076-3582, nbnxn_make_pairlist_part):
+//   * i-entries are (i-cluster, shift) with shift index <= CENTRAL (22), so a
+//     pair across a periodic boundary is listed once, from the atom whose
+//     shift is "backward" (c_pbcShiftBackward, pairlist.cpp:94,3339-3342);
+//   * the central shift keeps only cj >= ci and the diagonal entry carries
+//     the mask 0x08ce (j > i only, get_imask :907-910);
+//   * exclusion bits 4*i + j are cleared for topology exclusions found in the
+//     entry (setExclusionsForIEntry :1367-1435), skipping j <= i centrally.
+// The candidate clusters are enumerated per column and z range like the
+// reference (:3398-3481), but every candidate is then tested individually:
+// accept when the float bounding-box distance is below rlist^2 and at least
+// one atom pair has canonical squared distance ((dx*dx + dy*dy) + dz*dz,
+// round-to-nearest, no fused multiply-add) below rlist^2.  The bounding-box
+// test is a lower bound of every pair distance under monotone rounding, so
+// the set of atom pairs within rlist equals the reference's exactly; the
+// list itself is shorter (the reference keeps interior clusters unchecked).
+// The diagonal entry is always kept, as in the reference, where it is the
+// first j entry of every central i-entry.
+//
+// Work decomposition: one wavefront per i-cluster; lanes take one candidate
+// cluster each, hits are compacted in candidate order with a ballot, so the
+// output is deterministic.  The same kernel runs twice: a count pass (per
+// i-cluster j and i counts, scanned on the device) and a fill pass.
+#include "nbx_internal.h"
+
+namespace nbx
+{
+
+namespace
+{
+
+constexpr int kWave          = 64;
+constexpr int kSearchWaves   = 4;
+
+__device__ __forceinline__ int xyz2is(int x, int y, int z)
+{
+    return 5 * (3 * (z + 1) + y + 1) + x + 2;
+}
+
+// Canonical squared distance: fl(fl(fl(dx*dx) + fl(dy*dy)) + fl(dz*dz)).
+__device__ __forceinline__ float canonD2(float dx, float dy, float dz)
+{
+    return __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+}
+
+// clusterBoundingBoxDistance2 (pairlist.cpp:351-372) with the same rounding order.
+__device__ __forceinline__ float bbDist2(float4 ilo, float4 ihi, float4 jlo, float4 jhi)
+{
+    float d[3];
+    const float il[3] = { ilo.x, ilo.y, ilo.z }, ih[3] = { ihi.x, ihi.y, ihi.z };
+    const float jl[3] = { jlo.x, jlo.y, jlo.z }, jh[3] = { jhi.x, jhi.y, jhi.z };
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+    {
+        const float dl = __fsub_rn(il[k], jh[k]);
+        const float dh = __fsub_rn(jl[k], ih[k]);
+        d[k]           = fmaxf(fmaxf(dl, dh), 0.0f);
+    }
+    return canonD2(d[0], d[1], d[2]);
+}
+
+__device__ __forceinline__ int wavePrefixExclusive(int v, int lane, int* total)
+{
+    int x = v;
+#pragma unroll
+    for (int off = 1; off < kWave; off *= 2)
+    {
+        const int t = __shfl_up(x, off, kWave);
+        if (lane >= off)
+        {
+            x += t;
+        }
+    }
+    *total = __shfl(x, kWave - 1, kWave);
+    return x - v;
+}
+
+template<bool kFill>
+__global__ __launch_bounds__(kWave* kSearchWaves) void searchKernel(SearchArgs a)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int ci   = blockIdx.x * kSearchWaves + wave;
+    if (ci >= a.ncluster)
+    {
+        return;
+    }
+    const GridDims& g = a.g;
+    const float4 bbLo = a.bb[2 * ci];
+    const float4 bbHi = a.bb[2 * ci + 1];
+    float4 xi[kClusterSize];
+    int    iIndexReal[kClusterSize];
+#pragma unroll
+    for (int i = 0; i < kClusterSize; i++)
+    {
+        xi[i]         = a.xq[ci * kClusterSize + i];
+        iIndexReal[i] = a.atomIndices[ci * kClusterSize + i];
+    }
+    const float rl     = a.rlist;
+    const float rlist2 = a.rlist2;
+
+    int jpos = 0, ipos = 0;
+    if (kFill)
+    {
+        jpos = a.offJ[ci];
+        ipos = a.offI[ci];
+    }
+    int countJ = 0, countI = 0;
+
+    for (int tz = -1; tz <= 1; tz++)
+    {
+        for (int ty = -1; ty <= 1; ty++)
+        {
+            for (int tx = -1; tx <= 1; tx++)
+            {
+                const int shift = xyz2is(tx, ty, tz);
+                if (shift > kCentral)
+                {
+                    continue;
+                }
+                const bool central = (shift == kCentral);
+                const float shx = static_cast<float>(tx) * g.box[0];
+                const float shy = static_cast<float>(ty) * g.box[1];
+                const float shz = static_cast<float>(tz) * g.box[2];
+                // set_icell_bb_simple: shifted i bounding box
+                const float4 lo4 = make_float4(__fadd_rn(bbLo.x, shx), __fadd_rn(bbLo.y, shy), __fadd_rn(bbLo.z, shz), 0);
+                const float4 hi4 = make_float4(__fadd_rn(bbHi.x, shx), __fadd_rn(bbHi.y, shy), __fadd_rn(bbHi.z, shz), 0);
+                // conservative column and z ranges (margin 1% of a cell)
+                if (lo4.x - rl > g.box[0] || hi4.x + rl < 0 || lo4.y - rl > g.box[1] || hi4.y + rl < 0
+                    || lo4.z - rl > g.box[2] || hi4.z + rl < 0)
+                {
+                    continue;
+                }
+                const int cxf = max(0, static_cast<int>(floorf((lo4.x - rl) * g.invCellSize[0] - 0.01f)));
+                const int cxl = min(g.ncx - 1, static_cast<int>(floorf((hi4.x + rl) * g.invCellSize[0] + 0.01f)));
+                const int cyf = max(0, static_cast<int>(floorf((lo4.y - rl) * g.invCellSize[1] - 0.01f)));
+                const int cyl = min(g.ncy - 1, static_cast<int>(floorf((hi4.y + rl) * g.invCellSize[1] + 0.01f)));
+                if (cxf > cxl || cyf > cyl)
+                {
+                    continue;
+                }
+                const float zlo = lo4.z - rl - 1e-3f;
+                const float zhi = hi4.z + rl + 1e-3f;
+                float xs[kClusterSize], ys[kClusterSize], zs[kClusterSize];
+#pragma unroll
+                for (int i = 0; i < kClusterSize; i++)
+                {
+                    xs[i] = __fadd_rn(xi[i].x, shx);
+                    ys[i] = __fadd_rn(xi[i].y, shy);
+                    zs[i] = __fadd_rn(xi[i].z, shz);
+                }
+                const int nyc      = cyl - cyf + 1;
+                const int numCols  = (cxl - cxf + 1) * nyc;
+                int       entryJ   = 0; // j entries of this (ci, shift)
+                const int entryPos = jpos + countJ;
+                for (int c0 = 0; c0 < numCols; c0 += kWave)
+                {
+                    // per-lane column: z range of candidate clusters
+                    const int cc = c0 + lane;
+                    int       k0 = 0, n = 0;
+                    if (cc < numCols)
+                    {
+                        const int cx  = cxf + cc / nyc;
+                        const int cy  = cyf + cc % nyc;
+                        const int col = cx * g.ncy + cy;
+                        int       s   = a.cxy_ind[col];
+                        int       e   = a.cxy_ind[col + 1];
+                        // first cluster with upper z >= zlo
+                        int l = s, r = e;
+                        while (l < r)
+                        {
+                            const int m = (l + r) >> 1;
+                            if (a.bb[2 * m + 1].z < zlo)
+                            {
+                                l = m + 1;
+                            }
+                            else
+                            {
+                                r = m;
+                            }
+                        }
+                        const int first = l;
+                        // first cluster with lower z > zhi
+                        l = first;
+                        r = e;
+                        while (l < r)
+                        {
+                            const int m = (l + r) >> 1;
+                            if (a.bb[2 * m].z <= zhi)
+                            {
+                                l = m + 1;
+                            }
+                            else
+                            {
+                                r = m;
+                            }
+                        }
+                        int last = l; // exclusive
+                        k0       = first;
+                        if (central)
+                        {
+                            k0 = max(k0, ci);
+                        }
+                        n = max(0, last - k0);
+                    }
+                    int total;
+                    const int pref     = wavePrefixExclusive(n, lane, &total);
+                    const int ncolHere = min(kWave, numCols - c0);
+                    for (int t0 = 0; t0 < total; t0 += kWave)
+                    {
+                        const int t = t0 + lane;
+                        // column slot holding candidate t: the last slot whose
+                        // prefix is <= t (uniform trip count, all lanes shuffle)
+                        int lo = 0, hiIdx = ncolHere - 1;
+#pragma unroll
+                        for (int it = 0; it < 7; it++)
+                        {
+                            const int m  = (lo + hiIdx + 1) >> 1;
+                            const int pm = __shfl(pref, m, kWave);
+                            if (lo < hiIdx)
+                            {
+                                if (pm <= t)
+                                {
+                                    lo = m;
+                                }
+                                else
+                                {
+                                    hiIdx = m - 1;
+                                }
+                            }
+                        }
+                        const int colStart = __shfl(k0, lo, kWave);
+                        const int colPref  = __shfl(pref, lo, kWave);
+                        bool      hit      = false;
+                        int       cj       = -1;
+                        if (t < total)
+                        {
+                            cj = colStart + (t - colPref);
+                            if (central && cj == ci)
+                            {
+                                hit = true;
+                            }
+                            else
+                            {
+                                const float4 jlo = a.bb[2 * cj];
+                                const float4 jhi = a.bb[2 * cj + 1];
+                                if (bbDist2(lo4, hi4, jlo, jhi) < rlist2)
+                                {
+#pragma unroll
+                                    for (int j = 0; j < kClusterSize; j++)
+                                    {
+                                        const float4 xj = a.xq[cj * kClusterSize + j];
+#pragma unroll
+                                        for (int i = 0; i < kClusterSize; i++)
+                                        {
+                                            const float d2 = canonD2(__fsub_rn(xs[i], xj.x), __fsub_rn(ys[i], xj.y),
+                                                                     __fsub_rn(zs[i], xj.z));
+                                            hit = hit || (iIndexReal[i] >= 0 && d2 < rlist2);
+                                        }
+                                    }
+                                }
+                            }
+                        }
+                        const unsigned long long ballot = __ballot(hit);
+                        const int before = __popcll(ballot & ((1ull << lane) - 1ull));
+                        if (kFill && hit)
+                        {
+                            unsigned mask = (central && cj == ci) ? kMaskDiag : kMaskAll;
+                            for (int i = 0; i < kClusterSize; i++)
+                            {
+                                const int iAtom = iIndexReal[i];
+                                if (iAtom < 0)
+                                {
+                                    continue;
+                                }
+                                const int iIndex = ci * kClusterSize + i;
+                                for (int k = a.exclOff[iAtom]; k < a.exclOff[iAtom + 1]; k++)
+                                {
+                                    const int jAtom = a.exclIdx[k];
+                                    if (jAtom == iAtom)
+                                    {
+                                        continue;
+                                    }
+                                    const int jIndex = a.cells[jAtom];
+                                    if (central && jIndex <= iIndex)
+                                    {
+                                        continue;
+                                    }
+                                    if ((jIndex >> 2) == cj)
+                                    {
+                                        mask &= ~(1u << (i * kClusterSize + (jIndex & 3)));
+                                    }
+                                }
+                            }
+                            a.jentry[entryPos + entryJ + before] = make_uint2(static_cast<unsigned>(cj), mask);
+                        }
+                        entryJ += __popcll(ballot);
+                    }
+                }
+                if (entryJ > 0)
+                {
+                    if (kFill && lane == 0)
+                    {
+                        a.ientry[ipos + countI] = make_int4(ci, shift, entryPos, entryPos + entryJ);
+                    }
+                    countJ += entryJ;
+                    countI += 1;
+                }
+            }
+        }
+    }
+    if (!kFill && lane == 0)
+    {
+        a.countJ[ci] = countJ;
+        a.countI[ci] = countI;
+    }
+}
+
+// Exclusive scan over n ints in one workgroup; writes the total.
+__global__ __launch_bounds__(1024) void exclusiveScanKernel(int n, const int* __restrict__ in, int* __restrict__ out,
+                                                            int* __restrict__ total)
+{
+    __shared__ int s[1024];
+    int carry = 0;
+    for (int base = 0; base < n; base += 1024)
+    {
+        const int i = base + threadIdx.x;
+        const int v = (i < n) ? in[i] : 0;
+        s[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off *= 2)
+        {
+            const int t = (threadIdx.x >= off) ? s[threadIdx.x - off] : 0;
+            __syncthreads();
+            s[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < n)
+        {
+            out[i] = carry + s[threadIdx.x] - v;
+        }
+        const int blockTotal = s[1023];
+        __syncthreads();
+        carry += blockTotal;
+    }
+    if (threadIdx.x == 0)
+    {
+        out[n]  = carry;
+        *total  = carry;
+    }
+}
+
+} // namespace
+
+void launchSearch(hipStream_t s, const SearchArgs& a, bool fill)
+{
+    if (a.ncluster == 0)
+    {
+        return;
+    }
+    const int blocks = (a.ncluster + kSearchWaves - 1) / kSearchWaves;
+    if (fill)
+    {
+        searchKernel<true><<<blocks, kWave * kSearchWaves, 0, s>>>(a);
+    }
+    else
+    {
+        searchKernel<false><<<blocks, kWave * kSearchWaves, 0, s>>>(a);
+    }
+}
+
+void launchExclusiveScan(hipStream_t s, int n, const int* in, int* out, int* total)
+{
+    exclusiveScanKernel<<<1, 1024, 0, s>>>(n, in, out, total);
+}
+
+} // namespace nbx

@@ -1,0 +1,311 @@
+"""Python binding of the libnbx C ABI (include/nbx.h) through ctypes.
+
+This is the layer the gmxapi-style Python code and the tests use.  It does no
+arithmetic: every call goes to the HIP library.  If the library has not been
+built the import of a Context fails with NbxLibraryMissing, never falling
+back to anything else.
+
+Error behaviour follows nblib: invalid input raises NbxInputError (nblib
+throws nblib::InputException, api/nblib/exception.h), device failures raise
+NbxDeviceError, calls out of order raise NbxStateError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+from . import build as _build
+
+NBX_SUCCESS = 0
+NBX_ERROR_INPUT = 1
+NBX_ERROR_DEVICE = 2
+NBX_ERROR_STATE = 3
+
+COULOMB = {"ewald": 0, "pme": 0, "cutoff": 1, "reaction-field": 2, "rf": 2}
+EWALD_CORRECTION = {"table": 0, "analytical": 1}
+
+COMPUTE_FORCES = 1
+COMPUTE_ENERGY = 2
+COMPUTE_SHIFT_FORCES = 4
+COMPUTE_VIRIAL = 8
+
+
+class NbxError(RuntimeError):
+    pass
+
+
+class NbxInputError(NbxError, ValueError):
+    """Invalid input (nblib::InputException)."""
+
+
+class NbxDeviceError(NbxError):
+    pass
+
+
+class NbxStateError(NbxError):
+    pass
+
+
+class NbxLibraryMissing(NbxError, ImportError):
+    pass
+
+
+class _Options(C.Structure):
+    _fields_ = [("coulomb_type", C.c_int), ("ewald_correction", C.c_int), ("pairlist_cutoff", C.c_double),
+                ("coulomb_cutoff", C.c_double), ("ewald_rtol", C.c_double), ("epsilon_r", C.c_double),
+                ("epsilon_rf", C.c_double), ("ewald_potential_shift", C.c_int), ("device", C.c_int)]
+
+
+class _Info(C.Structure):
+    _fields_ = [("num_atoms", C.c_int64), ("num_clusters", C.c_int64), ("num_columns_x", C.c_int64),
+                ("num_columns_y", C.c_int64), ("num_i_entries", C.c_int64), ("num_j_entries", C.c_int64)]
+
+
+_lib = None
+
+
+def header_symbols() -> list[str]:
+    """Every function include/nbx.h declares."""
+    hdr = os.path.join(os.path.dirname(_build.HERE), "include", "nbx.h")
+    with open(hdr) as fh:
+        text = fh.read()
+    return sorted(set(re.findall(r"\b(nbx_[a-z0-9_]+)\s*\(", text)))
+
+
+def library() -> C.CDLL:
+    """Load libnbx.so (built in-tree by gmxapi_amd.build)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = _build.library_path()
+    if not os.path.exists(path):
+        raise NbxLibraryMissing(f"{path} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    lib = C.CDLL(path)
+    vp, i, u, d = C.c_void_p, C.c_int, C.c_uint, C.c_double
+    lib.nbx_last_error.restype = C.c_char_p
+    lib.nbx_default_options.argtypes = [C.POINTER(_Options)]
+    lib.nbx_default_options.restype = None
+    lib.nbx_create.argtypes = [C.POINTER(_Options), C.POINTER(vp)]
+    lib.nbx_destroy.argtypes = [vp]
+    lib.nbx_set_stream.argtypes = [vp, vp]
+    lib.nbx_set_nonbonded_parameters.argtypes = [vp, i, vp, vp]
+    lib.nbx_set_topology.argtypes = [vp, i, vp, vp, vp, vp]
+    lib.nbx_put_on_grid.argtypes = [vp, vp, vp, i]
+    lib.nbx_construct_pairlist.argtypes = [vp]
+    lib.nbx_convert_coordinates.argtypes = [vp, vp, i]
+    lib.nbx_dispatch_kernel.argtypes = [vp, u]
+    lib.nbx_add_nbat_f_to_f.argtypes = [vp, vp, i]
+    lib.nbx_compute.argtypes = [vp, vp, vp, u, i]
+    lib.nbx_get_energies.argtypes = [vp, vp, vp]
+    lib.nbx_get_shift_forces.argtypes = [vp, vp]
+    lib.nbx_get_virial.argtypes = [vp, vp]
+    lib.nbx_synchronize.argtypes = [vp]
+    lib.nbx_get_pairlist_info.argtypes = [vp, C.POINTER(_Info)]
+    lib.nbx_get_grid.argtypes = [vp, vp, vp]
+    lib.nbx_get_pairlist.argtypes = [vp, vp, vp]
+    lib.nbx_list_pairs.argtypes = [vp, d, vp, C.c_int64]
+    lib.nbx_list_pairs.restype = C.c_int64
+    lib.nbx_get_interaction_constants.argtypes = [vp, vp]
+    _lib = lib
+    return lib
+
+
+def _raise(code: int, what: str):
+    msg = library().nbx_last_error().decode(errors="replace")
+    cls = {NBX_ERROR_INPUT: NbxInputError, NBX_ERROR_STATE: NbxStateError}.get(code, NbxDeviceError)
+    raise cls(f"{what}: {msg}")
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+class Context:
+    """One system on one GPU: the nonbonded_verlet_t replacement.
+
+    Methods mirror the calls nblib makes (gmxsetup.cpp:306-325 and
+    gmxcalculator.cpp:70-103).
+    """
+
+    def __init__(self, coulomb: str = "ewald", ewald_correction: str = "table", rlist: float = 1.0,
+                 rcoulomb: float | None = None, ewald_rtol: float = 1e-5, epsilon_r: float = 1.0,
+                 epsilon_rf: float = 1.0, ewald_potential_shift: bool = False, device: int = 0):
+        lib = library()
+        self._lib = lib
+        o = _Options()
+        lib.nbx_default_options(C.byref(o))
+        o.coulomb_type = COULOMB[coulomb]
+        o.ewald_correction = EWALD_CORRECTION[ewald_correction]
+        o.pairlist_cutoff = rlist
+        o.coulomb_cutoff = 0.0 if rcoulomb is None else rcoulomb
+        o.ewald_rtol = ewald_rtol
+        o.epsilon_r = epsilon_r
+        o.epsilon_rf = epsilon_rf
+        o.ewald_potential_shift = 1 if ewald_potential_shift else 0
+        o.device = device
+        h = C.c_void_p()
+        rc = lib.nbx_create(C.byref(o), C.byref(h))
+        if rc != NBX_SUCCESS:
+            _raise(rc, "nbx_create")
+        self._h = h
+        self.natoms = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.nbx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != NBX_SUCCESS:
+            _raise(rc, what)
+
+    @property
+    def handle(self) -> int:
+        return self._h.value
+
+    def set_stream(self, stream_ptr: int | None):
+        self._check(self._lib.nbx_set_stream(self._h, stream_ptr or None), "nbx_set_stream")
+
+    def set_nonbonded_parameters(self, c6: np.ndarray, c12: np.ndarray):
+        c6 = np.ascontiguousarray(c6, np.float64)
+        c12 = np.ascontiguousarray(c12, np.float64)
+        t = c6.shape[0]
+        self._check(self._lib.nbx_set_nonbonded_parameters(self._h, t, _ptr(c6), _ptr(c12)),
+                    "nbx_set_nonbonded_parameters")
+
+    def set_topology(self, types, charges, excl_off, excl_idx):
+        t = np.ascontiguousarray(types, np.int32)
+        q = np.ascontiguousarray(charges, np.float32)
+        eo = np.ascontiguousarray(excl_off, np.int32)
+        ei = np.ascontiguousarray(excl_idx, np.int32)
+        if len(ei) == 0:
+            ei = np.zeros(1, np.int32)
+        self.natoms = len(t)
+        self._check(self._lib.nbx_set_topology(self._h, len(t), _ptr(t), _ptr(q), _ptr(eo), _ptr(ei)),
+                    "nbx_set_topology")
+
+    def put_on_grid(self, box, x=None, x_device_ptr: int | None = None):
+        b = np.ascontiguousarray(box, np.float32)
+        if x_device_ptr is not None:
+            self._check(self._lib.nbx_put_on_grid(self._h, _ptr(b), x_device_ptr, 1), "nbx_put_on_grid")
+        else:
+            xa = np.ascontiguousarray(x, np.float32).reshape(-1)
+            self._check(self._lib.nbx_put_on_grid(self._h, _ptr(b), _ptr(xa), 0), "nbx_put_on_grid")
+
+    def construct_pairlist(self):
+        self._check(self._lib.nbx_construct_pairlist(self._h), "nbx_construct_pairlist")
+
+    def compute(self, x, energy=False, shift_forces=False, virial=False) -> np.ndarray:
+        """Forces in atom order (host arrays, synchronous)."""
+        xa = np.ascontiguousarray(x, np.float32).reshape(-1)
+        f = np.empty_like(xa)
+        flags = COMPUTE_FORCES | (COMPUTE_ENERGY if energy else 0) | (COMPUTE_SHIFT_FORCES if shift_forces else 0)
+        flags |= COMPUTE_VIRIAL if virial else 0
+        self._check(self._lib.nbx_compute(self._h, _ptr(xa), _ptr(f), flags, 0), "nbx_compute")
+        return f.reshape(-1, 3)
+
+    def compute_device(self, x_ptr: int, f_ptr: int, flags: int = COMPUTE_FORCES):
+        """Asynchronous compute on device buffers of the context's device."""
+        self._check(self._lib.nbx_compute(self._h, x_ptr, f_ptr, flags, 1), "nbx_compute")
+
+    def convert_coordinates_device(self, x_ptr: int):
+        self._check(self._lib.nbx_convert_coordinates(self._h, x_ptr, 1), "nbx_convert_coordinates")
+
+    def dispatch_kernel(self, flags: int = COMPUTE_FORCES):
+        self._check(self._lib.nbx_dispatch_kernel(self._h, flags), "nbx_dispatch_kernel")
+
+    def add_nbat_f_to_f_device(self, f_ptr: int):
+        self._check(self._lib.nbx_add_nbat_f_to_f(self._h, f_ptr, 1), "nbx_add_nbat_f_to_f")
+
+    def synchronize(self):
+        self._check(self._lib.nbx_synchronize(self._h), "nbx_synchronize")
+
+    def energies(self) -> tuple[float, float]:
+        vlj = C.c_double()
+        vc = C.c_double()
+        self._check(self._lib.nbx_get_energies(self._h, C.byref(vlj), C.byref(vc)), "nbx_get_energies")
+        return vlj.value, vc.value
+
+    def shift_forces(self) -> np.ndarray:
+        out = np.zeros(45 * 3)
+        self._check(self._lib.nbx_get_shift_forces(self._h, _ptr(out)), "nbx_get_shift_forces")
+        return out.reshape(45, 3)
+
+    def virial(self) -> np.ndarray:
+        out = np.zeros(9)
+        self._check(self._lib.nbx_get_virial(self._h, _ptr(out)), "nbx_get_virial")
+        return out.reshape(3, 3)
+
+    def info(self) -> dict:
+        inf = _Info()
+        self._check(self._lib.nbx_get_pairlist_info(self._h, C.byref(inf)), "nbx_get_pairlist_info")
+        return {k: int(getattr(inf, k)) for k, _ in _Info._fields_}
+
+    def grid(self) -> tuple[np.ndarray, np.ndarray]:
+        inf = self.info()
+        ai = np.zeros(4 * inf["num_clusters"], np.int32)
+        cells = np.zeros(inf["num_atoms"], np.int32)
+        self._check(self._lib.nbx_get_grid(self._h, _ptr(ai), _ptr(cells)), "nbx_get_grid")
+        return ai, cells
+
+    def pairlist(self) -> tuple[np.ndarray, np.ndarray]:
+        inf = self.info()
+        ie = np.zeros((max(inf["num_i_entries"], 1), 4), np.int32)
+        je = np.zeros((max(inf["num_j_entries"], 1), 2), np.uint32)
+        self._check(self._lib.nbx_get_pairlist(self._h, _ptr(ie), _ptr(je)), "nbx_get_pairlist")
+        return ie[:inf["num_i_entries"]], je[:inf["num_j_entries"]]
+
+    def count_pairs(self, r: float) -> int:
+        """Number of atom pairs the list holds within r (canonical rounding)."""
+        n = self._lib.nbx_list_pairs(self._h, r, None, 0)
+        if n < 0:
+            _raise(int(-n), "nbx_list_pairs")
+        return int(n)
+
+    def list_pairs(self, r: float) -> np.ndarray:
+        n = self._lib.nbx_list_pairs(self._h, r, None, 0)
+        if n < 0:
+            _raise(int(-n), "nbx_list_pairs")
+        out = np.zeros(max(n, 1), np.int64)
+        m = self._lib.nbx_list_pairs(self._h, r, _ptr(out), n)
+        if m < 0:
+            _raise(int(-m), "nbx_list_pairs")
+        return np.sort(out[:n])
+
+    def interaction_constants(self) -> dict:
+        out = np.zeros(9)
+        self._check(self._lib.nbx_get_interaction_constants(self._h, _ptr(out)), "nbx_get_interaction_constants")
+        keys = ["beta", "tab_scale", "tab_n", "epsfac", "k_rf", "c_rf", "sh_ewald", "cpot_disp", "cpot_rep"]
+        return dict(zip(keys, out.tolist()))
+
+
+def nblib_c6c12_matrices(c6_type, c12_type):
+    """Geometric combination as nblib does it in float
+    (api/nblib/interactions.cpp:83-93): C_ij = sqrt(C_i C_j)."""
+    c6 = np.asarray(c6_type, np.float32)
+    c12 = np.asarray(c12_type, np.float32)
+    m6 = np.sqrt((c6[:, None] * c6[None, :]).astype(np.float32)).astype(np.float64)
+    m12 = np.sqrt((c12[:, None] * c12[None, :]).astype(np.float32)).astype(np.float64)
+    return m6, m12
+
+
+def context_for(system, coulomb="ewald", ewald_correction="table", device=0, potential_shift=False) -> Context:
+    """Context set up on a gmxapi_amd.systems.System the way nblib sets up its
+    calculator: parameters, topology, grid, pair list."""
+    ctx = Context(coulomb=coulomb, ewald_correction=ewald_correction, rlist=system.rlist,
+                  rcoulomb=system.rcoulomb, ewald_potential_shift=potential_shift, device=device)
+    m6, m12 = nblib_c6c12_matrices(system.c6, system.c12)
+    ctx.set_nonbonded_parameters(m6, m12)
+    ctx.set_topology(system.types, system.charges, system.excl_off, system.excl_idx)
+    ctx.put_on_grid(system.box, system.x)
+    ctx.construct_pairlist()
+    return ctx

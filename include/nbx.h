@@ -1,0 +1,195 @@
+/*
+ * nbx.h - C ABI of the MI355X-native NBNxM short-range nonbonded engine.
+ *
+ * One nbx_context holds one periodic rectangular system on one HIP device:
+ * the LJ parameters, the topology (types, charges, exclusions), the grid of
+ * 4-atom clusters, the cluster-pair list and the force/energy buffers.  It
+ * replaces the reference's nonbonded_verlet_t behind nblib's
+ * ForceCalculator (api/nblib/gmxcalculator.h:92-111); the entry points map
+ * one to one onto the calls nblib makes (gmxsetup.cpp:306-325 for set-up,
+ * gmxcalculator.cpp:70-83 for a step).
+ *
+ * Plain pointers and sizes only.  Coordinates and forces are float xyz
+ * triplets in the caller's atom order; a pointer is a host pointer unless
+ * the *_on_device argument says it is device memory of the context's device.
+ * Every function returns NBX_SUCCESS or an error code and records a message
+ * for nbx_last_error() (per thread).  Calls on one context are not
+ * re-entrant; different contexts are independent.
+ *
+ * Units: nm, kJ/mol, e.  Precision: FP32 pair arithmetic, FP64 energy,
+ * shift-force and virial accumulation.
+ */
+#ifndef NBX_H
+#define NBX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NBX_SUCCESS 0
+/* invalid user input: nblib throws nblib::InputException for these
+ * (api/nblib/exception.h; e.g. gmxsetup.cpp:80-98, gmxcalculator.cpp:91-94) */
+#define NBX_ERROR_INPUT 1
+/* a HIP runtime error */
+#define NBX_ERROR_DEVICE 2
+/* calls out of order (e.g. compute before a pair list exists) */
+#define NBX_ERROR_STATE 3
+
+/* Coulomb flavour: nblib CoulombType {Pme, Cutoff, ReactionField}
+ * (api/nblib/kerneloptions.h:74-80).  Cutoff runs the reaction-field kernel
+ * with eps_rf = eps_r (forcerec.cpp:902-915, kerneldispatch.cpp:166-169). */
+#define NBX_COULOMB_EWALD 0
+#define NBX_COULOMB_CUTOFF 1
+#define NBX_COULOMB_REACTION_FIELD 2
+
+/* Ewald exclusion correction: the reference's quadratic-spline table
+ * (Nbnxm::EwaldExclusionType::Table, what the plain-C kernel always uses,
+ * gmxsetup.cpp:117-121) or an analytical evaluation. */
+#define NBX_EWALD_TABLE 0
+#define NBX_EWALD_ANALYTICAL 1
+
+/* compute flags (gmx::StepWorkload computeForces / computeEnergy /
+ * computeVirial, gmxsetup.cpp:214-224) */
+#define NBX_COMPUTE_FORCES 1u
+#define NBX_COMPUTE_ENERGY 2u
+#define NBX_COMPUTE_SHIFT_FORCES 4u
+#define NBX_COMPUTE_VIRIAL 8u
+
+typedef struct nbx_context nbx_context;
+
+/* NBKernelOptions subset (api/nblib/kerneloptions.h:86-112) plus the
+ * extensions the benchmark needs: a Coulomb cut-off below the list cut-off,
+ * the Ewald tolerance and mdrun's potential-shift convention. */
+typedef struct nbx_options
+{
+    int    coulomb_type;          /* NBX_COULOMB_*, default EWALD (nblib Pme) */
+    int    ewald_correction;      /* NBX_EWALD_*, default TABLE */
+    double pairlist_cutoff;       /* rlist, nm (nblib pairlistCutoff, 1.0) */
+    double coulomb_cutoff;        /* rcoulomb = rvdw <= rlist; 0 means rlist */
+    double ewald_rtol;            /* erfc(beta rc) = rtol, 1e-5 (gmxsetup.cpp:275) */
+    double epsilon_r;             /* 1 (gmxsetup.cpp:251-252) */
+    double epsilon_rf;            /* 1 */
+    int    ewald_potential_shift; /* 0: sh_ewald = 0 as nblib; 1: mdrun's shift */
+    int    device;                /* HIP device ordinal */
+} nbx_options;
+
+typedef struct nbx_pairlist_info
+{
+    int64_t num_atoms;
+    int64_t num_clusters;     /* 4-atom clusters incl. partly filled ones */
+    int64_t num_columns_x;    /* grid columns (Grid::setDimensions) */
+    int64_t num_columns_y;
+    int64_t num_i_entries;    /* (i-cluster, shift) entries */
+    int64_t num_j_entries;    /* cluster pairs; 16 atom pairs each */
+} nbx_pairlist_info;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* nbx_last_error(void);
+
+/* nblib's defaults (NBKernelOptions, kerneloptions.h:86-112). */
+void nbx_default_options(nbx_options* options);
+
+/* GmxSetupDirector/NbvSetupUtil (gmxsetup.cpp:100-325): validates options,
+ * computes the interaction constants (setupInteractionConst :226-284: Ewald
+ * coefficient calc_ewaldcoeff_q ewald_utils.cpp:46-74, correction tables
+ * init_interaction_const_tables forcerec.cpp:726-778, LJ potential shifts,
+ * reaction-field constants) and binds the device. */
+int nbx_create(const nbx_options* options, nbx_context** context);
+
+/* Releases the context and its device memory. */
+int nbx_destroy(nbx_context* context);
+
+/* Work is queued on this HIP stream (hipStream_t); NULL selects the
+ * context's own stream.  No counterpart in the reference (CPU only there). */
+int nbx_set_stream(nbx_context* context, void* stream);
+
+/* NbvSetupUtil::setNonBondedParameters (gmxsetup.cpp:143-166) and
+ * nbnxn_atomdata_init's parameter table (atomdata.cpp:440-620): c6 and c12
+ * are num_types x num_types matrices (row-major, unscaled C6/C12); the
+ * kernel stores {6*C6, 12*C12} plus a zero padding type for fillers. */
+int nbx_set_nonbonded_parameters(nbx_context* context, int num_types, const double* c6, const double* c12);
+
+/* The Topology inputs nblib feeds to nbnxm (topology.h:72-121): per-atom
+ * type ids and charges, and the exclusions as a list of lists in CSR form
+ * (gmx::ListOfLists<int>, every atom excluding itself, molecules.cpp:86-88).
+ * All arrays are host memory. */
+int nbx_set_topology(nbx_context* context, int num_atoms, const int* types, const float* charges,
+                     const int* exclusion_offsets, const int* exclusion_indices);
+
+/* nbnxn_put_on_grid (nbnxm.h:453-464) as GmxForceCalculator::setParticlesOnGrid
+ * calls it (gmxcalculator.cpp:85-103: lower corner 0, upper corner the box
+ * diagonal, density N/det(box)), followed by setAtomProperties
+ * (nbnxm.cpp:122-127, atomdata.cpp:955-976).  box holds the three edge
+ * lengths of a rectangular box; atoms must lie in [0, L). */
+int nbx_put_on_grid(nbx_context* context, const float* box, const float* x, int x_on_device);
+
+/* nonbonded_verlet_t::constructPairlist (pairlist.cpp:4247-4262) for the
+ * local intra-grid list with full periodicity. */
+int nbx_construct_pairlist(nbx_context* context);
+
+/* nonbonded_verlet_t::convertCoordinates (nbnxm.cpp:129-141). */
+int nbx_convert_coordinates(nbx_context* context, const float* x, int x_on_device);
+
+/* nonbonded_verlet_t::dispatchNonbondedKernel (kerneldispatch.cpp:420-456)
+ * with enbvClearFYes: clears the cluster force buffer, then runs the
+ * cluster-pair kernel; flags select energies and shift forces. */
+int nbx_dispatch_kernel(nbx_context* context, unsigned flags);
+
+/* nonbonded_verlet_t::atomdata_add_nbat_f_to_f (nbnxm.cpp:162-180) after
+ * nblib's zero fill (gmxcalculator.cpp:76-82): f is overwritten with the
+ * nonbonded forces in the caller's atom order. */
+int nbx_add_nbat_f_to_f(nbx_context* context, float* f, int f_on_device);
+
+/* GmxForceCalculator::compute (gmxcalculator.cpp:70-83): convert
+ * coordinates, dispatch the kernel, reduce forces into f (overwritten).
+ * With NBX_COMPUTE_VIRIAL the virial is also formed (calc_vir,
+ * calcvir.cpp:103-148).  Device pointers are used asynchronously on the
+ * context's stream; host pointers make the call synchronous. */
+int nbx_compute(nbx_context* context, const float* x, float* f, unsigned flags, int on_device);
+
+/* Energies of the last compute with NBX_COMPUTE_ENERGY (gmx_enerdata_t
+ * terms reduced by reduce_energies_over_lists, kernel_common.cpp:109-133):
+ * LJ and Coulomb (real-space) in kJ/mol.  Synchronises. */
+int nbx_get_energies(nbx_context* context, double* v_lj, double* v_coulomb);
+
+/* Shift forces of the last compute (nbnxn_atomdata_add_nbat_fshift_to_fshift,
+ * atomdata.cpp:1471-1487): 45 x 3 doubles in XYZ2IS order (ishift.h:40-55). */
+int nbx_get_shift_forces(nbx_context* context, double* fshift);
+
+/* Virial of the last compute with NBX_COMPUTE_VIRIAL: 3 x 3 doubles,
+ * -1/2 sum x_i (x) f_i - 1/2 sum s (x) fshift_s (sim_util.cpp:157-177). */
+int nbx_get_virial(nbx_context* context, double* virial);
+
+/* Waits for the context's queued work. */
+int nbx_synchronize(nbx_context* context);
+
+/* Grid and list sizes (the reference prints these via print_nblist_statistics,
+ * pairlist.cpp:2955-3018). */
+int nbx_get_pairlist_info(nbx_context* context, nbx_pairlist_info* info);
+
+/* The grid's atom order: atom_indices[4 * num_clusters] (-1 = filler) and
+ * cells[num_atoms] (GridSetData atomIndices / cells, gridsetdata.h). */
+int nbx_get_grid(nbx_context* context, int* atom_indices, int* cells);
+
+/* The cluster-pair list: i_entries[4 * num_i_entries] {ci, shift, cj_start,
+ * cj_end} and j_entries[2 * num_j_entries] {cj, exclusion mask}
+ * (NbnxnPairlistCpu ci/cj arrays, pairlist.h:226-259). */
+int nbx_get_pairlist(nbx_context* context, int* i_entries, uint32_t* j_entries);
+
+/* The atom pairs the list holds with squared distance below r^2, computed
+ * on the device with the canonical float rounding, as keys
+ * (min atom << 32 | max atom), unsorted.  Returns the number of pairs (or a
+ * negative error); writes at most cap. */
+int64_t nbx_list_pairs(nbx_context* context, double r, int64_t* out, int64_t cap);
+
+/* beta, table scale, table size, epsfac, k_rf, c_rf, sh_ewald, LJ
+ * dispersion and repulsion potential shifts (interaction_const_t). */
+int nbx_get_interaction_constants(nbx_context* context, double* out9);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NBX_H */

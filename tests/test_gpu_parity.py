@@ -1,0 +1,201 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Bars (the path's contract):
+  * grid: atom order bit-identical to the oracle's restatement of the
+    reference grid;
+  * pair search: the set of atom pairs within rlist bit-exact with the
+    oracle's list and with brute-force enumeration;
+  * forces within 5e-4 of the RMS force, energies within 1e-6 relative of
+    the FP64 oracle (the reference kernel in double precision), and close
+    to the FP32 oracle (the reference plain-C kernel in single precision).
+"""
+import numpy as np
+import pytest
+
+from gmxapi_amd import nbx, systems as S
+from oracle import oracle as O
+from tests.helpers import load_golden, oracle_for, rms
+
+pytestmark = pytest.mark.gpu
+
+FORCE_TOL_RMS = 5e-4
+ENERGY_TOL_REL = 1e-6
+
+
+def _systems():
+    return [
+        ("water216", S.small_water_box(216, seed=11)),
+        ("water500_rc0.9", S.water_box(500, 2.6, seed=9, rc=0.9, rlist=1.0)),
+        ("chains1200", S.hetero_chains(1200, 2.3, ntypes=5, seed=3, rc=0.9, rlist=1.0)),
+    ]
+
+
+@pytest.fixture(scope="module")
+def water3000():
+    return S.config(0)
+
+
+@pytest.mark.parametrize("name,s", _systems())
+def test_grid_is_bit_identical(name, s):
+    print(f"[grid] {name} natoms={s.natoms}", flush=True)
+    ctx = nbx.context_for(s)
+    o = oracle_for(s, "f32")
+    g = o.grid()
+    ai, cells = ctx.grid()
+    info = ctx.info()
+    assert info["num_columns_x"] == g["ncx"] and info["num_columns_y"] == g["ncy"]
+    np.testing.assert_array_equal(ai, g["atomIndices"])
+    np.testing.assert_array_equal(cells, g["cells"])
+
+
+@pytest.mark.parametrize("name,s", _systems())
+def test_pair_set_is_bit_exact(name, s):
+    print(f"[pairs] {name}", flush=True)
+    ctx = nbx.context_for(s)
+    gpu = ctx.list_pairs(s.rlist)
+    assert len(np.unique(gpu)) == len(gpu)
+    np.testing.assert_array_equal(gpu, oracle_for(s, "f32").list_pairs(s.rlist))
+    np.testing.assert_array_equal(gpu, O.brute_pairs(s.x, s.box, s.rlist, "f32"))
+
+
+def test_pair_set_config0(water3000):
+    print("[pairs] config0 9000 atoms", flush=True)
+    s = water3000
+    ctx = nbx.context_for(s)
+    np.testing.assert_array_equal(ctx.list_pairs(s.rlist), oracle_for(s, "f32").list_pairs(s.rlist))
+
+
+@pytest.mark.parametrize("name,s", _systems())
+def test_list_contract(name, s):
+    ctx = nbx.context_for(s)
+    ie, je = ctx.pairlist()
+    ai, cells = ctx.grid()
+    assert np.all(ie[:, 1] <= 22)
+    excl = set()
+    for a in range(s.natoms):
+        for b in s.excl_idx[s.excl_off[a]:s.excl_off[a + 1]]:
+            excl.add((a, int(b)))
+    central = ie[ie[:, 1] == 22]
+    # every non-empty cluster has exactly one central entry holding its diagonal
+    assert len(np.unique(central[:, 0])) == len(central) == info_clusters(ctx)
+    for c, sh, a0, b0 in ie:
+        js = je[a0:b0, 0].astype(np.int64)
+        if sh == 22:
+            assert np.all(js >= c) and c in js
+        for k in range(a0, b0):
+            m = int(je[k, 1])
+            cj = int(je[k, 0])
+            for i in range(4):
+                for j in range(4):
+                    if (m >> (4 * i + j)) & 1:
+                        continue
+                    if sh == 22 and cj == c and j <= i:
+                        continue
+                    assert (int(ai[c * 4 + i]), int(ai[cj * 4 + j])) in excl
+
+
+def info_clusters(ctx):
+    return ctx.info()["num_clusters"]
+
+
+def _compare(s, ctx, energy_tol=ENERGY_TOL_REL, coulomb="ewald"):
+    f = ctx.compute(s.x, energy=True, virial=True)
+    vlj, vc = ctx.energies()
+    r64 = oracle_for(s, "f64", coulomb=coulomb).compute(s.x)
+    r32 = oracle_for(s, "f32", coulomb=coulomb).compute(s.x)
+    ferr64 = rms(f - r64["f"]) / rms(r64["f"])
+    ferr32 = rms(f - r32["f"]) / rms(r64["f"])
+    e64 = r64["Vvdw"] + r64["Vc"]
+    print(f"  force rms err vs f64 {ferr64:.2e} vs f32 {ferr32:.2e}; "
+          f"Vlj {vlj:.6f} ({r64['Vvdw']:.6f}) Vc {vc:.6f} ({r64['Vc']:.6f})", flush=True)
+    assert ferr64 <= FORCE_TOL_RMS
+    assert ferr32 <= FORCE_TOL_RMS
+    assert abs(vlj - r64["Vvdw"]) <= energy_tol * max(abs(r64["Vvdw"]), abs(e64))
+    assert abs(vc - r64["Vc"]) <= energy_tol * max(abs(r64["Vc"]), abs(e64))
+    vir = ctx.virial()
+    np.testing.assert_allclose(vir, r64["virial"], rtol=0, atol=1e-4 * np.abs(r64["virial"]).max())
+    fsh = ctx.shift_forces()
+    np.testing.assert_allclose(fsh, r64["fshift"], rtol=0, atol=1e-4 * max(1.0, np.abs(r64["fshift"]).max()))
+    return f
+
+
+@pytest.mark.parametrize("name,s", _systems())
+def test_forces_energies_virial(name, s):
+    print(f"[forces] {name}", flush=True)
+    ctx = nbx.context_for(s)
+    _compare(s, ctx)
+
+
+def test_forces_config0(water3000):
+    print("[forces] config0 9000 atoms", flush=True)
+    ctx = nbx.context_for(water3000)
+    _compare(water3000, ctx)
+
+
+@pytest.mark.parametrize("coulomb", ["cutoff", "reaction-field"])
+def test_reaction_field_flavours(coulomb):
+    s = S.water_box(300, 2.1, seed=5)
+    ctx = nbx.context_for(s, coulomb=coulomb)
+    _compare(s, ctx, coulomb="rf")
+
+
+@pytest.mark.parametrize("name", ["nblib_argon.json", "nblib_spc_methanol.json"])
+def test_nblib_golden_forces(name):
+    # api/nblib/tests/nbkernelsystem.cpp:69-85,187-203 refdata
+    d = load_golden(name)
+    ctx = nbx.Context(coulomb="cutoff", rlist=d["rc"])
+    m6, m12 = nbx.nblib_c6c12_matrices(d["c6"], d["c12"])
+    ctx.set_nonbonded_parameters(m6, m12)
+    ctx.set_topology(d["types"], d["charges"], d["excl_off"], d["excl_idx"])
+    ctx.put_on_grid([d["box"]] * 3, d["x"])
+    ctx.construct_pairlist()
+    f = ctx.compute(d["x"])
+    ref = np.array(d["forces"])
+    assert np.abs(f - ref).max() <= 5e-5 * np.abs(ref).max()
+
+
+def test_repeated_compute_is_reproducible():
+    # nbkernelsystem.cpp:150-167; float atomics may reorder the j-force sums
+    s = S.small_water_box(216, seed=11)
+    ctx = nbx.context_for(s)
+    a = ctx.compute(s.x, energy=True)
+    ea = ctx.energies()
+    b = ctx.compute(s.x, energy=True)
+    eb = ctx.energies()
+    assert rms(a - b) <= 1e-6 * rms(a)
+    assert ea == eb
+
+
+def test_update_changes_forces():
+    # nbkernelsystem.cpp:169-184: forces change after the coordinates move
+    s = S.small_water_box(216, seed=11)
+    ctx = nbx.context_for(s)
+    a = ctx.compute(s.x)
+    x2 = s.x + 0.01 * np.random.default_rng(0).standard_normal(s.x.shape)
+    x2 = np.mod(x2, s.box)
+    b = ctx.compute(x2)
+    assert np.all(np.abs(a - b).sum(axis=1) > 0)
+
+
+def test_input_errors():
+    s = S.small_water_box(216, seed=11)
+    ctx = nbx.Context(rlist=1.0)
+    with pytest.raises(nbx.NbxStateError):
+        ctx.construct_pairlist()
+    m6, m12 = nbx.nblib_c6c12_matrices(s.c6, s.c12)
+    ctx.set_nonbonded_parameters(m6, m12)
+    bad = s.types.copy()
+    bad[0] = 7
+    with pytest.raises(nbx.NbxInputError):
+        ctx.set_topology(bad, s.charges, s.excl_off, s.excl_idx)
+    ctx.set_topology(s.types, s.charges, s.excl_off, s.excl_idx)
+    x = s.x.copy()
+    x[5, 0] = -1.5
+    with pytest.raises(nbx.NbxInputError):
+        ctx.put_on_grid(s.box, x)
+    with pytest.raises(nbx.NbxInputError):
+        ctx.put_on_grid([1.5, 1.5, 1.5], s.x)
+    with pytest.raises(nbx.NbxStateError):
+        ctx.compute(s.x)
+    with pytest.raises(nbx.NbxInputError):
+        nbx.Context(rlist=1.0, rcoulomb=1.2)
