@@ -109,7 +109,10 @@ def main():
         f"rlist {s.rlist} (generated in {time.perf_counter() - t0:.1f} s)")
 
     dev = torch.device("cuda:0")
-    stream = torch.cuda.current_stream(dev)
+    # a stream of our own (default priority): the library and the events
+    # below run on it, so the events bracket exactly the library's work
+    stream = torch.cuda.Stream(device=dev)
+    torch.cuda.set_stream(stream)
     ctx = nbx.Context(rlist=s.rlist, rcoulomb=s.rcoulomb, device=0)
     ctx.set_stream(stream.cuda_stream)
     m6, m12 = nbx.nblib_c6c12_matrices(s.c6, s.c12)

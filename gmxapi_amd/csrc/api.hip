@@ -420,6 +420,8 @@ int nbx_create(const nbx_options* options, nbx_context** context)
         ic.rcoulomb2 = c->rcoulomb * c->rcoulomb;
         ic.cpot_disp = static_cast<float>(-1.0 / p6);
         ic.cpot_rep  = static_cast<float>(-1.0 / p12);
+        ic.cpot_disp6 = static_cast<float>(static_cast<double>(ic.cpot_disp) / 6.0);
+        ic.cpot_rep12 = static_cast<float>(static_cast<double>(ic.cpot_rep) / 12.0);
         const float epsr = static_cast<float>(o.epsilon_r);
         ic.epsfac        = epsr != 0 ? static_cast<float>(one4PiEps0() / epsr) : 0.0f;
         if (o.coulomb_type == NBX_COULOMB_REACTION_FIELD)
@@ -491,7 +493,7 @@ int nbx_set_stream(nbx_context* c, void* stream)
     {
         return fail(NBX_ERROR_INPUT, "NULL context");
     }
-    c->stream = stream ? static_cast<hipStream_t>(stream) : c->ownStream;
+    c->stream = static_cast<hipStream_t>(stream);
     return NBX_SUCCESS;
 }
 

@@ -38,6 +38,12 @@ constexpr int kWave       = 64;
 constexpr int kWavesBlock = 4;
 constexpr int kThreads    = kWave * kWavesBlock;
 
+// 1/12 and 1/6 split into a float and its float remainder
+constexpr float kTwelfthHi = static_cast<float>(1.0 / 12.0);
+constexpr float kTwelfthLo = static_cast<float>(1.0 / 12.0 - static_cast<double>(static_cast<float>(1.0 / 12.0)));
+constexpr float kSixthHi   = static_cast<float>(1.0 / 6.0);
+constexpr float kSixthLo   = static_cast<float>(1.0 / 6.0 - static_cast<double>(static_cast<float>(1.0 / 6.0)));
+
 __device__ __forceinline__ float dppQuadXor1(float v)
 {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
@@ -97,8 +103,6 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
     const float k_rf2     = 2.0f * ic.k_rf;
     const float halfsp    = 0.5f / ic.tab_scale;
     const float selfFac   = 0.5f * ic.self_v0;
-    const float sixth     = 1.0f / 6.0f;
-    const float twelfth   = 1.0f / 12.0f;
 
     double eVdw = 0.0, eCoul = 0.0;
 
@@ -146,7 +150,15 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
             float       rsq = dx * dx + dy * dy + dz * dz;
             skipmask        = (rsq >= rc2) ? 0.0f : skipmask;
             rsq             = fmaxf(rsq, kMinRsq);
-            const float rinv   = __builtin_amdgcn_rsqf(rsq) * skipmask;
+            float rinv = __builtin_amdgcn_rsqf(rsq);
+            if (kEnergy)
+            {
+                // one Newton-Raphson step: removes the systematic bias of the
+                // hardware estimate, which r^-12 would amplify in the energy sum
+                const float hx = 0.5f * rsq;
+                rinv           = rinv * fmaf(-hx * rinv, rinv, 1.5f);
+            }
+            rinv               = rinv * skipmask;
             const float rinvsq = rinv * rinv;
 
             const float2 c       = sNbfp[tiOff + tj];
@@ -156,8 +168,12 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
             const float  frLJ    = FrLJ12 - FrLJ6;
             if (kEnergy)
             {
-                const float VLJ = ((FrLJ12 + c.y * ic.cpot_rep) * twelfth - (FrLJ6 + c.x * ic.cpot_disp) * sixth)
-                                  * interact * skipmask;
+                // (FrLJ12 + c12 cpot_rep)/12 - (FrLJ6 + c6 cpot_disp)/6 with the
+                // divisions as x*h + x*l (h + l = 1/12 or 1/6 to 2^-48), so the
+                // quotient is rounded once and without bias
+                const float v12 = fmaf(FrLJ12, kTwelfthHi, FrLJ12 * kTwelfthLo) + c.y * ic.cpot_rep12;
+                const float v6  = fmaf(FrLJ6, kSixthHi, FrLJ6 * kSixthLo) + c.x * ic.cpot_disp6;
+                const float VLJ = (v12 - v6) * interact * skipmask;
                 vlj += VLJ;
             }
 

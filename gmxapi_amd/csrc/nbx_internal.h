@@ -53,6 +53,8 @@ struct InteractionConst
     float sh_ewald;
     float cpot_disp;
     float cpot_rep;
+    float cpot_disp6; // cpot_disp / 6, rounded once
+    float cpot_rep12; // cpot_rep / 12, rounded once
     float tab_scale;
     int   tab_n;
     float self_v0; // V(0) of the Ewald correction (or c_rf for RF)

@@ -173,6 +173,7 @@ class Context:
         return self._h.value
 
     def set_stream(self, stream_ptr: int | None):
+        """Queue work on this hipStream_t (an int handle); 0/None is the null stream."""
         self._check(self._lib.nbx_set_stream(self._h, stream_ptr or None), "nbx_set_stream")
 
     def set_nonbonded_parameters(self, c6: np.ndarray, c12: np.ndarray):

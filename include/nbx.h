@@ -101,8 +101,9 @@ int nbx_create(const nbx_options* options, nbx_context** context);
 /* Releases the context and its device memory. */
 int nbx_destroy(nbx_context* context);
 
-/* Work is queued on this HIP stream (hipStream_t); NULL selects the
- * context's own stream.  No counterpart in the reference (CPU only there). */
+/* Work is queued on this HIP stream (hipStream_t) from now on; NULL is
+ * HIP's default (null) stream.  A new context uses a non-blocking stream of
+ * its own.  No counterpart in the reference (CPU only there). */
 int nbx_set_stream(nbx_context* context, void* stream);
 
 /* NbvSetupUtil::setNonBondedParameters (gmxsetup.cpp:143-166) and
