@@ -899,7 +899,7 @@ int nbx_compute(nbx_context* c, const float* x, float* f, unsigned flags, int on
         launchReduceF(c->stream, c->natoms, c->cells.p, c->fnb.p, fd);
         if (flags & NBX_COMPUTE_VIRIAL)
         {
-            const int vb = std::max(1, std::min(1024, (c->natoms + 255) / 256));
+            const int vb = std::max(1, std::min(256, (c->natoms + 1023) / 1024));
             launchVirial(c->stream, c->natoms, xd, fd, c->fshift.p, c->shiftVec.p, c->virialPartial.p, vb, c->virial.p);
             c->haveVirial = true;
         }
@@ -1061,11 +1061,14 @@ int64_t nbx_list_pairs(nbx_context* c, double r, int64_t* out, int64_t cap)
         DevBuf<unsigned long long> counter;
         counter.ensure(1);
         DevBuf<long long> buf;
-        buf.ensure(static_cast<size_t>(std::max<int64_t>(cap, 1)));
+        if (out && cap > 0)
+        {
+            buf.ensure(static_cast<size_t>(cap));
+        }
         check(hipMemsetAsync(counter.p, 0, sizeof(unsigned long long), c->stream), "clear counter");
         const float rf = static_cast<float>(r);
         launchExpandPairs(c->stream, c->ientry.p, c->nci, c->jentry.p, c->xq.p, c->atomIndices.p, c->shiftVec.p,
-                          rf * rf, counter.p, buf.p, cap);
+                          rf * rf, counter.p, (out && cap > 0) ? buf.p : nullptr, cap);
         check(hipGetLastError(), "expand pairs");
         unsigned long long n = 0;
         check(hipMemcpyAsync(&n, counter.p, sizeof(n), hipMemcpyDeviceToHost, c->stream), "read count");

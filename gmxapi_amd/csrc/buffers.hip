@@ -163,27 +163,84 @@ __global__ __launch_bounds__(kVirThreads) void virialPartialKernel(int natoms, c
     }
 }
 
-__global__ void virialFinalKernel(int numPartials, const double* __restrict__ part, const double* __restrict__ fshift,
-                                  const float4* __restrict__ shiftVec, double* __restrict__ out)
+// One wavefront per virial component: strided sum of the partials plus the
+// shift-force term, then a shuffle reduction.
+__global__ __launch_bounds__(64 * 9) void virialFinalKernel(int numPartials, const double* __restrict__ part,
+                                                            const double* __restrict__ fshift,
+                                                            const float4* __restrict__ shiftVec,
+                                                            double* __restrict__ out)
 {
-    const int k = threadIdx.x;
-    if (k >= 9)
-    {
-        return;
-    }
+    const int k    = threadIdx.x / 64;
+    const int lane = threadIdx.x & 63;
     const int d1 = k / 3, d2 = k % 3;
     double    t  = 0.0;
-    for (int p = 0; p < numPartials; p++)
+    for (int p = lane; p < numPartials; p += 64)
     {
         t += part[p * 9 + k];
     }
-    for (int sh = 0; sh < kNumShifts; sh++)
+    if (lane < kNumShifts)
     {
-        const float4 sv  = shiftVec[sh];
+        const float4 sv  = shiftVec[lane];
         const double svd = (d1 == 0) ? sv.x : ((d1 == 1) ? sv.y : sv.z);
-        t -= 0.5 * svd * fshift[sh * 3 + d2];
+        t -= 0.5 * svd * fshift[lane * 3 + d2];
     }
-    out[k] = t;
+    for (int m = 1; m < 64; m *= 2)
+    {
+        t += __shfl_xor(t, m, 64);
+    }
+    if (lane == 0)
+    {
+        out[k] = t;
+    }
+}
+
+// Pair count only: per-thread counts, one atomic per workgroup.
+__global__ __launch_bounds__(64) void countPairsKernel(const int4* __restrict__ ientry, int nci,
+                                                       const uint2* __restrict__ jentry,
+                                                       const float4* __restrict__ xq,
+                                                       const int* __restrict__ atomIndices,
+                                                       const float4* __restrict__ shiftVec, float r2,
+                                                       unsigned long long* counter)
+{
+    const int e = blockIdx.x;
+    unsigned  n = 0;
+    if (e < nci)
+    {
+        const int4   ie = ientry[e];
+        const float4 sv = shiftVec[ie.y];
+        for (int k = ie.z + threadIdx.x; k < ie.w; k += blockDim.x)
+        {
+            const int cj = static_cast<int>(jentry[k].x);
+            for (int i = 0; i < kClusterSize; i++)
+            {
+                if (atomIndices[ie.x * kClusterSize + i] < 0)
+                {
+                    continue;
+                }
+                const float4 xi = xq[ie.x * kClusterSize + i];
+                const float  xs = __fadd_rn(xi.x, sv.x), ys = __fadd_rn(xi.y, sv.y), zs = __fadd_rn(xi.z, sv.z);
+                for (int j = 0; j < kClusterSize; j++)
+                {
+                    if (atomIndices[cj * kClusterSize + j] < 0 || (ie.y == kCentral && cj == ie.x && j <= i))
+                    {
+                        continue;
+                    }
+                    const float4 xj = xq[cj * kClusterSize + j];
+                    const float  dx = __fsub_rn(xs, xj.x), dy = __fsub_rn(ys, xj.y), dz = __fsub_rn(zs, xj.z);
+                    const float  d2 = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+                    n += (d2 < r2) ? 1u : 0u;
+                }
+            }
+        }
+    }
+    for (int m = 1; m < 64; m *= 2)
+    {
+        n += __shfl_xor(n, m, 64);
+    }
+    if (threadIdx.x == 0 && n > 0)
+    {
+        atomicAdd(counter, static_cast<unsigned long long>(n));
+    }
 }
 
 // One thread per j-entry: the 16 atom pairs with canonical d^2 < r2.
@@ -273,14 +330,18 @@ void launchVirial(hipStream_t s, int natoms, const float* x, const float* f, con
                   const float4* shiftVec, double* virialPartial, int numBlocks, double* virialOut)
 {
     virialPartialKernel<<<numBlocks, kVirThreads, 0, s>>>(natoms, x, f, virialPartial);
-    virialFinalKernel<<<1, 64, 0, s>>>(numBlocks, virialPartial, fshift, shiftVec, virialOut);
+    virialFinalKernel<<<1, 64 * 9, 0, s>>>(numBlocks, virialPartial, fshift, shiftVec, virialOut);
 }
 
 void launchExpandPairs(hipStream_t s, const int4* ientry, int nci, const uint2* jentry, const float4* xq,
                        const int* atomIndices, const float4* shiftVec, float r2, unsigned long long* counter,
                        long long* out, long long cap)
 {
-    if (nci > 0)
+    if (nci > 0 && out == nullptr)
+    {
+        countPairsKernel<<<nci, 64, 0, s>>>(ientry, nci, jentry, xq, atomIndices, shiftVec, r2, counter);
+    }
+    else if (nci > 0)
     {
         expandPairsKernel<<<nci, 64, 0, s>>>(ientry, nci, jentry, xq, atomIndices, shiftVec, r2, counter, out, cap);
     }

@@ -46,18 +46,134 @@ constexpr float kSixthLo   = static_cast<float>(1.0 / 6.0 - static_cast<double>(
 
 __device__ __forceinline__ float dppQuadXor1(float v)
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));
 }
 
 __device__ __forceinline__ float dppQuadXor2(float v)
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, true));
 }
 
 __device__ __forceinline__ float quadSum(float v)
 {
     v = v + dppQuadXor1(v);
     return v + dppQuadXor2(v);
+}
+
+// 32-bit byte offset from a wave-uniform base: lets the compiler use the
+// scalar-base + vector-offset form of global loads
+template<typename T>
+__device__ __forceinline__ T ldg(const T* base, unsigned index)
+{
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + index * static_cast<unsigned>(sizeof(T)));
+}
+
+// Per i-entry constants of one lane (i-atom ii of the entry's cluster).
+struct IEntry
+{
+    float xi, yi, zi, qi;
+    int   tiOff;
+    int   ci;
+    bool  central;
+};
+
+// One 4 i x 4 j x 4 j-cluster tile.  kChecks: exclusion bits, the central
+// diagonal (j <= i skipped) and lanes past the end of the j-list; without it
+// the tile is known to be full and free of masks.
+template<int kElec, bool kEnergy, bool kChecks>
+__device__ __forceinline__ void pairTile(const IEntry& I, const uint2 jent, const float4 xqj, const int tj,
+                                         const bool valid, const unsigned sj, const int ii, const int jj,
+                                         const InteractionConst& ic, const float2* __restrict__ sNbfp,
+                                         const float4* __restrict__ sTab, const int tabN, float4* __restrict__ fnb,
+                                         float& fix, float& fiy, float& fiz, float& vlj, float& vc)
+{
+    const float dx  = I.xi - xqj.x;
+    const float dy  = I.yi - xqj.y;
+    const float dz  = I.zi - xqj.z;
+    float       rsq = dx * dx + dy * dy + dz * dz;
+    float interact  = 1.0f;
+    float skipmask;
+    if (kChecks)
+    {
+        interact = static_cast<float>((jent.y >> (ii * 4 + jj)) & 1u);
+        const bool skip = !valid || (I.central && static_cast<int>(jent.x) == I.ci && jj <= ii) || rsq >= ic.rcoulomb2;
+        skipmask        = skip ? 0.0f : 1.0f;
+    }
+    else
+    {
+        skipmask = (rsq >= ic.rcoulomb2) ? 0.0f : 1.0f;
+    }
+    rsq        = fmaxf(rsq, kMinRsq);
+    float rinv = __builtin_amdgcn_rsqf(rsq);
+    if (kEnergy)
+    {
+        // one Newton-Raphson step: removes the systematic bias of the
+        // hardware estimate, which r^-12 would amplify in the energy sum
+        const float hx = 0.5f * rsq;
+        rinv           = rinv * fmaf(-hx * rinv, rinv, 1.5f);
+    }
+    rinv               = rinv * skipmask;
+    const float rinvsq = rinv * rinv;
+
+    const float2 c       = sNbfp[I.tiOff + tj];
+    const float  rinv6   = kChecks ? interact * rinvsq * rinvsq * rinvsq : rinvsq * rinvsq * rinvsq;
+    const float  FrLJ6   = c.x * rinv6;
+    const float  FrLJ12  = c.y * rinv6 * rinv6;
+    const float  frLJ    = FrLJ12 - FrLJ6;
+    if (kEnergy)
+    {
+        // (FrLJ12 + c12 cpot_rep)/12 - (FrLJ6 + c6 cpot_disp)/6 with the
+        // divisions as x*h + x*l (h + l = 1/12 or 1/6 to 2^-48), so the
+        // quotient is rounded once and without bias
+        const float v12 = fmaf(FrLJ12, kTwelfthHi, FrLJ12 * kTwelfthLo) + c.y * ic.cpot_rep12;
+        const float v6  = fmaf(FrLJ6, kSixthHi, FrLJ6 * kSixthLo) + c.x * ic.cpot_disp6;
+        vlj += kChecks ? (v12 - v6) * interact * skipmask : (v12 - v6) * skipmask;
+    }
+
+    const float qq = skipmask * I.qi * xqj.w;
+    float       fcoul;
+    if (kElec == kElecRF)
+    {
+        fcoul = qq * (interact * rinv * rinvsq - 2.0f * ic.k_rf);
+        if (kEnergy)
+        {
+            vc += qq * (interact * rinv + ic.k_rf * rsq - ic.c_rf);
+        }
+    }
+    else
+    {
+        const float  rs    = rsq * rinv * ic.tab_scale;
+        // clamped so that no input (NaN, Inf) can index past the table
+        const int    ri    = min(static_cast<int>(rs), tabN - 2);
+        const float  frac  = rs - static_cast<float>(ri);
+        const float4 tab   = sTab[ri];
+        const float  fexcl = tab.x + frac * tab.y;
+        fcoul              = interact * rinvsq - fexcl;
+        if (kEnergy)
+        {
+            const float halfsp = 0.5f / ic.tab_scale;
+            vc += qq * (interact * (rinv - ic.sh_ewald) - (tab.z - halfsp * frac * (tab.x + fexcl)));
+        }
+        fcoul *= qq * rinv;
+    }
+
+    const float fscal = frLJ * rinvsq + fcoul;
+    const float fx    = fscal * dx;
+    const float fy    = fscal * dy;
+    const float fz    = fscal * dz;
+    fix += fx;
+    fiy += fy;
+    fiz += fz;
+    // j-force: sum over the four i-lanes of the quad; lane i = 0,1,2 scatters
+    // component x,y,z
+    const float gx   = quadSum(fx);
+    const float gy   = quadSum(fy);
+    const float gz   = quadSum(fz);
+    const float comp = (ii == 0) ? gx : ((ii == 1) ? gy : gz);
+    if ((!kChecks || valid) && ii < 3)
+    {
+        unsafeAtomicAdd(reinterpret_cast<float*>(reinterpret_cast<char*>(fnb) + sj * 16u) + ii, -comp);
+    }
 }
 
 template<int kElec, bool kEnergy, bool kShift>
@@ -99,126 +215,71 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
     const int kk   = lane >> 4;
 
     const InteractionConst ic = a.ic;
-    const float rc2       = ic.rcoulomb2;
-    const float k_rf2     = 2.0f * ic.k_rf;
-    const float halfsp    = 0.5f / ic.tab_scale;
-    const float selfFac   = 0.5f * ic.self_v0;
+    const float selfFac       = 0.5f * ic.self_v0;
 
     double eVdw = 0.0, eCoul = 0.0;
 
     for (int e = blockIdx.x * kWavesBlock + wave; e < a.nci; e += gridDim.x * kWavesBlock)
     {
-        const int4 ie      = a.ientry[__builtin_amdgcn_readfirstlane(e)];
-        const int  ci      = __builtin_amdgcn_readfirstlane(ie.x);
-        const int  shift   = __builtin_amdgcn_readfirstlane(ie.y);
-        const int  js      = __builtin_amdgcn_readfirstlane(ie.z);
-        const int  je      = __builtin_amdgcn_readfirstlane(ie.w);
-        const bool central = (shift == kCentral);
+        const int4 ie = a.ientry[__builtin_amdgcn_readfirstlane(e)];
+        const int  ci = __builtin_amdgcn_readfirstlane(ie.x);
+        const int  sh = __builtin_amdgcn_readfirstlane(ie.y);
+        const int  js = __builtin_amdgcn_readfirstlane(ie.z);
+        const int  je = __builtin_amdgcn_readfirstlane(ie.w);
 
-        const float4 sv  = a.shiftVec[shift];
-        const float4 xqi = a.xq[ci * kClusterSize + ii];
-        const float  xi  = xqi.x + sv.x;
-        const float  yi  = xqi.y + sv.y;
-        const float  zi  = xqi.z + sv.z;
-        const float  qi  = ic.epsfac * xqi.w;
-        const int    tiOff = a.type[ci * kClusterSize + ii] * ntype;
+        IEntry I;
+        const float4 sv  = a.shiftVec[sh];
+        const float4 xqi = ldg(a.xq, static_cast<unsigned>(ci * kClusterSize + ii));
+        I.xi             = xqi.x + sv.x;
+        I.yi             = xqi.y + sv.y;
+        I.zi             = xqi.z + sv.z;
+        I.qi             = ic.epsfac * xqi.w;
+        I.tiOff          = ldg(a.type, static_cast<unsigned>(ci * kClusterSize + ii)) * ntype;
+        I.ci             = ci;
+        I.central        = (sh == kCentral);
 
         float fix = 0.0f, fiy = 0.0f, fiz = 0.0f;
         float vlj = 0.0f, vc = 0.0f;
-        if (kEnergy && central && jj == 0 && kk == 0)
+        if (kEnergy && I.central && jj == 0 && kk == 0)
         {
             // kernel_ref_outer.h:286-326, Vc -= qi * q_i * V(0)/2
-            vc -= qi * xqi.w * selfFac;
+            vc -= I.qi * xqi.w * selfFac;
         }
 
-        for (int jb = js; jb < je; jb += 4)
+        // software pipeline: j-entries two tiles ahead, j-atom data one ahead
+        const int ntile = (je - js + 3) >> 2;
+        auto entryOf    = [&](int t) {
+            const int j = js + 4 * t + kk;
+            return ldg(a.jentry, static_cast<unsigned>(j < je ? j : js));
+        };
+        uint2  eNext  = entryOf(0);
+        uint2  eNext2 = entryOf(1);
+        unsigned sNext = eNext.x * kClusterSize + jj;
+        float4 xNext  = ldg(a.xq, sNext);
+        int    tNext  = ldg(a.type, sNext);
+        for (int t = 0; t < ntile; t++)
         {
-            const int   jidx  = jb + kk;
-            const bool  valid = jidx < je;
-            const uint2 jent  = a.jentry[valid ? jidx : js];
-            const int   cj    = static_cast<int>(jent.x);
-            const int   sj    = cj * kClusterSize + jj;
-            const float4 xqj  = a.xq[sj];
-            const int   tj    = a.type[sj];
+            const uint2    jent = eNext;
+            const float4   xqj  = xNext;
+            const int      tj   = tNext;
+            const unsigned sj   = sNext;
+            eNext               = eNext2;
+            eNext2              = entryOf(t + 2);
+            sNext               = eNext.x * kClusterSize + jj;
+            xNext               = ldg(a.xq, sNext);
+            tNext               = ldg(a.type, sNext);
 
-            const float interact = static_cast<float>((jent.y >> (ii * 4 + jj)) & 1u);
-            float skipmask = (!valid || (central && cj == ci && jj <= ii)) ? 0.0f : 1.0f;
-
-            const float dx  = xi - xqj.x;
-            const float dy  = yi - xqj.y;
-            const float dz  = zi - xqj.z;
-            float       rsq = dx * dx + dy * dy + dz * dz;
-            skipmask        = (rsq >= rc2) ? 0.0f : skipmask;
-            rsq             = fmaxf(rsq, kMinRsq);
-            float rinv = __builtin_amdgcn_rsqf(rsq);
-            if (kEnergy)
+            const bool valid   = js + 4 * t + kk < je;
+            const bool special = jent.y != kMaskAll || (I.central && static_cast<int>(jent.x) == ci) || !valid;
+            if (__ballot(special) == 0ull)
             {
-                // one Newton-Raphson step: removes the systematic bias of the
-                // hardware estimate, which r^-12 would amplify in the energy sum
-                const float hx = 0.5f * rsq;
-                rinv           = rinv * fmaf(-hx * rinv, rinv, 1.5f);
-            }
-            rinv               = rinv * skipmask;
-            const float rinvsq = rinv * rinv;
-
-            const float2 c       = sNbfp[tiOff + tj];
-            const float  rinvsix = interact * rinvsq * rinvsq * rinvsq;
-            const float  FrLJ6   = c.x * rinvsix;
-            const float  FrLJ12  = c.y * rinvsix * rinvsix;
-            const float  frLJ    = FrLJ12 - FrLJ6;
-            if (kEnergy)
-            {
-                // (FrLJ12 + c12 cpot_rep)/12 - (FrLJ6 + c6 cpot_disp)/6 with the
-                // divisions as x*h + x*l (h + l = 1/12 or 1/6 to 2^-48), so the
-                // quotient is rounded once and without bias
-                const float v12 = fmaf(FrLJ12, kTwelfthHi, FrLJ12 * kTwelfthLo) + c.y * ic.cpot_rep12;
-                const float v6  = fmaf(FrLJ6, kSixthHi, FrLJ6 * kSixthLo) + c.x * ic.cpot_disp6;
-                const float VLJ = (v12 - v6) * interact * skipmask;
-                vlj += VLJ;
-            }
-
-            const float qq = skipmask * qi * xqj.w;
-            float       fcoul;
-            if (kElec == kElecRF)
-            {
-                fcoul = qq * (interact * rinv * rinvsq - k_rf2);
-                if (kEnergy)
-                {
-                    vc += qq * (interact * rinv + ic.k_rf * rsq - ic.c_rf);
-                }
+                pairTile<kElec, kEnergy, false>(I, jent, xqj, tj, true, sj, ii, jj, ic, sNbfp, sTab, tabN, a.fnb,
+                                                fix, fiy, fiz, vlj, vc);
             }
             else
             {
-                const float  rs    = rsq * rinv * ic.tab_scale;
-                // clamped so that no input (NaN, Inf) can index past the table
-                const int    ri    = min(static_cast<int>(rs), tabN - 2);
-                const float  frac  = rs - static_cast<float>(ri);
-                const float4 tab   = sTab[ri];
-                const float  fexcl = tab.x + frac * tab.y;
-                fcoul              = interact * rinvsq - fexcl;
-                if (kEnergy)
-                {
-                    vc += qq * (interact * (rinv - ic.sh_ewald) - (tab.z - halfsp * frac * (tab.x + fexcl)));
-                }
-                fcoul *= qq * rinv;
-            }
-
-            const float fscal = frLJ * rinvsq + fcoul;
-            const float fx    = fscal * dx;
-            const float fy    = fscal * dy;
-            const float fz    = fscal * dz;
-            fix += fx;
-            fiy += fy;
-            fiz += fz;
-            // j-force: sum over the four i-lanes of the quad, lane i = 0,1,2
-            // scatters component x,y,z
-            const float gx   = quadSum(fx);
-            const float gy   = quadSum(fy);
-            const float gz   = quadSum(fz);
-            const float comp = (ii == 0) ? gx : ((ii == 1) ? gy : gz);
-            if (valid && ii < 3)
-            {
-                unsafeAtomicAdd(reinterpret_cast<float*>(a.fnb + sj) + ii, -comp);
+                pairTile<kElec, kEnergy, true>(I, jent, xqj, tj, valid, sj, ii, jj, ic, sNbfp, sTab, tabN, a.fnb,
+                                               fix, fiy, fiz, vlj, vc);
             }
         }
 
@@ -237,13 +298,13 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
         }
         if (kShift)
         {
-            float sx = quadSum(fix);
-            float sy = quadSum(fiy);
-            float sz = quadSum(fiz);
+            const float sx = quadSum(fix);
+            const float sy = quadSum(fiy);
+            const float sz = quadSum(fiz);
             if (lane < 3)
             {
                 const float comp = (lane == 0) ? sx : ((lane == 1) ? sy : sz);
-                atomicAdd(sFsh + shift * 3 + lane, comp);
+                atomicAdd(sFsh + sh * 3 + lane, comp);
             }
         }
         if (kEnergy)
