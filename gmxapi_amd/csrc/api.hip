@@ -47,6 +47,14 @@ struct DevBuf
 {
     T*     p   = nullptr;
     size_t cap = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    void swap(DevBuf& o)
+    {
+        std::swap(p, o.p);
+        std::swap(cap, o.cap);
+    }
     void   ensure(size_t n)
     {
         if (n <= cap && p)
@@ -226,6 +234,7 @@ struct nbx_context
     int              numCU      = 256;
     int              elec       = kElecEwaldTable;
     InteractionConst ic{};
+    double           selfCoef = 0.0;
     float            rlist = 1.0f, rcoulomb = 1.0f;
 
     // parameters
@@ -254,11 +263,16 @@ struct nbx_context
     int            nci = 0, ncj = 0;
     DevBuf<int>    countJ, countI, offJ, offI, totals;
     DevBuf<int4>   ientry;
-    DevBuf<uint2>  jentry;
+    DevBuf<uint2>  jentry, jentryTmp;
+    // kernel work groups
+    int            groupSize = 0, ngroups = 0, nchunk = 0, windowSlots = 0, nwork = 0;
+    DevBuf<int>    unionCount, chunkCount, workCount, overflow, unionOff, chunkOff, workOff, unionList, groupStats;
+    DevBuf<int4>   chunks, groups;
+    DevBuf<uint2>  work;
 
     // outputs
     int             numBlocks = 0;
-    DevBuf<double>  energyPartial, energy, fshift, virialPartial, virial;
+    DevBuf<double>  energyChunk, energyPartial, energy, fshift, virialPartial, virial;
     DevBuf<float>   fshiftPartial, fOut;
     bool            haveEnergy = false, haveShift = false, haveVirial = false;
 };
@@ -334,6 +348,98 @@ const float* deviceCoordinates(nbx_context* c, const float* x, int onDevice)
     c->x.ensure(3 * static_cast<size_t>(c->natoms));
     check(hipMemcpyAsync(c->x.p, x, sizeof(float) * 3 * c->natoms, hipMemcpyHostToDevice, c->stream), "upload x");
     return c->x.p;
+}
+
+// Groups of consecutive i-clusters for the kernel's LDS j-force window
+// (groups.hip): the largest group size whose j-entries fit the LDS sort and
+// whose window fits the kernel's LDS budget.
+void buildWorkGroups(nbx_context* c)
+{
+    const int nc = c->ncluster;
+    c->groupStats.ensure(9);
+    check(hipMemsetAsync(c->groupStats.p, 0, sizeof(int) * 9, c->stream), "clear stats");
+    launchGroupSpans(c->stream, nc, c->offJ.p, c->groupStats.p);
+    int spans[4] = { 0, 0, 0, 0 };
+    check(hipMemcpyAsync(spans, c->groupStats.p, sizeof(spans), hipMemcpyDeviceToHost, c->stream), "read spans");
+    check(hipStreamSynchronize(c->stream), "group sync");
+    c->jentryTmp.ensure(c->ncj);
+    for (int k = 0; k < 4; k++)
+    {
+        const int G = kGroupSizeMax >> k;
+        if (spans[k] > kGroupMaxJ && G > 1)
+        {
+            continue;
+        }
+        if (spans[k] > kGroupMaxJ)
+        {
+            throw InputError("a grid cluster has more than " + std::to_string(kGroupMaxJ)
+                             + " cluster-pair entries: the pair-list cut-off is too long for this kernel");
+        }
+        const int ng = (nc + G - 1) / G;
+        c->unionCount.ensure(ng + 1);
+        c->chunkCount.ensure(ng + 1);
+        c->workCount.ensure(ng + 1);
+        c->overflow.ensure(ng + 1);
+        c->unionOff.ensure(ng + 1);
+        c->chunkOff.ensure(ng + 1);
+        c->workOff.ensure(ng + 1);
+        GroupArgs ga{};
+        ga.ncluster     = nc;
+        ga.groupSize    = G;
+        ga.ngroups      = ng;
+        ga.chunkEntries = kChunkEntries;
+        ga.offI         = c->offI.p;
+        ga.offJ         = c->offJ.p;
+        ga.ientry       = c->ientry.p;
+        ga.jentryIn     = c->jentry.p;
+        ga.jentryOut    = c->jentryTmp.p;
+        ga.unionCount   = c->unionCount.p;
+        ga.chunkCount   = c->chunkCount.p;
+        ga.workCount    = c->workCount.p;
+        ga.overflow     = c->overflow.p;
+        launchBuildGroups(c->stream, ga, false);
+        check(hipMemsetAsync(c->groupStats.p + 4, 0, sizeof(int) * 2, c->stream), "clear stats");
+        launchMaxReduce(c->stream, ng, c->unionCount.p, c->groupStats.p + 4);
+        launchMaxReduce(c->stream, ng, c->overflow.p, c->groupStats.p + 5);
+        launchExclusiveScan(c->stream, ng, c->unionCount.p, c->unionOff.p, c->groupStats.p + 6);
+        launchExclusiveScan(c->stream, ng, c->chunkCount.p, c->chunkOff.p, c->groupStats.p + 7);
+        launchExclusiveScan(c->stream, ng, c->workCount.p, c->workOff.p, c->groupStats.p + 8);
+        int st[5];
+        check(hipMemcpyAsync(st, c->groupStats.p + 4, sizeof(st), hipMemcpyDeviceToHost, c->stream), "read stats");
+        check(hipStreamSynchronize(c->stream), "group sync");
+        const int maxUnion = st[0], overflow = st[1], unionTotal = st[2], chunkTotal = st[3], workTotal = st[4];
+        const int slots    = G + maxUnion;
+        if (overflow || (nonbondedLdsBytes(c->ntype, c->ic.tab_n, c->elec, slots) > kKernelLdsBudget && G > 1))
+        {
+            continue;
+        }
+        if (nonbondedLdsBytes(c->ntype, c->ic.tab_n, c->elec, slots) > kKernelLdsBudget)
+        {
+            throw InputError("the j-cluster window of one cluster does not fit in LDS: the pair-list cut-off is "
+                             "too long for this kernel");
+        }
+        c->unionList.ensure(std::max(unionTotal, 1));
+        c->chunks.ensure(std::max(chunkTotal, 1));
+        c->groups.ensure(ng);
+        c->work.ensure(std::max(workTotal, 1));
+        ga.unionOff  = c->unionOff.p;
+        ga.chunkOff  = c->chunkOff.p;
+        ga.workOff   = c->workOff.p;
+        ga.unionList = c->unionList.p;
+        ga.work      = c->work.p;
+        ga.chunks    = c->chunks.p;
+        ga.groups    = c->groups.p;
+        launchBuildGroups(c->stream, ga, true);
+        check(hipGetLastError(), "group kernels");
+        c->jentry.swap(c->jentryTmp);
+        c->groupSize   = G;
+        c->ngroups     = ng;
+        c->nchunk      = chunkTotal;
+        c->nwork       = workTotal;
+        c->windowSlots = slots;
+        return;
+    }
+    throw InputError("no work-group size fits the kernel's LDS budget");
 }
 
 } // namespace
@@ -450,6 +556,10 @@ int nbx_create(const nbx_options* options, nbx_context** context)
             ic.tab_n     = static_cast<int>(c->rcoulomb * ic.tab_scale) + 2;
             std::vector<float> fdv0 = ewaldTableFDV0(ic.tab_n, ic.tab_scale, ic.beta);
             ic.self_v0   = fdv0[2];
+            // the self energy sums one large term per atom against a pair sum
+            // of nearly the same size: V(0) = 2 beta / sqrt(pi) in double (of
+            // the float beta every pair term uses) keeps its rounding out
+            c->selfCoef = 0.5 * static_cast<double>(ic.epsfac) * vEwaldLr(ic.beta, 0.0);
             c->tabFDV0.ensure(ic.tab_n);
             check(hipMemcpy(c->tabFDV0.p, fdv0.data(), sizeof(float) * fdv0.size(), hipMemcpyHostToDevice), "upload table");
         }
@@ -459,6 +569,7 @@ int nbx_create(const nbx_options* options, nbx_context** context)
             ic.tab_scale = 1.0f;
             ic.tab_n     = 0;
             ic.self_v0   = ic.c_rf;
+            c->selfCoef  = 0.5 * static_cast<double>(ic.epsfac) * static_cast<double>(ic.c_rf);
         }
         c->shiftVec.ensure(kNumShifts);
     });
@@ -719,12 +830,25 @@ int nbx_put_on_grid(nbx_context* c, const float* box, const float* x, int xOnDev
         const int nslot = c->ncluster * kClusterSize;
         launchGridSortColumns(c->stream, g, c->x.p, c->atomCol.p, c->atomRank.p, c->cxy_na.p, c->cxy_ind.p, c->bucket.p,
                               c->atomIndices.p, c->cells.p);
-        c->xq.ensure(nslot);
-        c->nbtype.ensure(nslot);
+        // one extra cluster of far-away fillers after the real ones: the
+        // kernel points lanes past the end of a j-list at it
+        c->xq.ensure(nslot + kClusterSize);
+        c->nbtype.ensure(nslot + kClusterSize);
         c->bb.ensure(2 * static_cast<size_t>(c->ncluster));
-        c->fnb.ensure(nslot);
+        c->fnb.ensure(nslot + kClusterSize);
         launchGridFill(c->stream, c->ncluster, c->x.p, c->atomIndices.p, c->type.p, c->charge.p, c->ntype - 1, c->xq.p,
                        c->nbtype.p, c->bb.p);
+        {
+            static const float4 far[kClusterSize] = { { kFarAway, kFarAway, kFarAway, 0.0f },
+                                                      { kFarAway, kFarAway, kFarAway, 0.0f },
+                                                      { kFarAway, kFarAway, kFarAway, 0.0f },
+                                                      { kFarAway, kFarAway, kFarAway, 0.0f } };
+            const int pad[kClusterSize] = { c->ntype - 1, c->ntype - 1, c->ntype - 1, c->ntype - 1 };
+            check(hipMemcpyAsync(c->xq.p + nslot, far, sizeof(far), hipMemcpyHostToDevice, c->stream), "dummy x");
+            check(hipMemcpyAsync(c->nbtype.p + nslot, pad, sizeof(pad), hipMemcpyHostToDevice, c->stream),
+                  "dummy type");
+            check(hipStreamSynchronize(c->stream), "dummy sync");
+        }
         check(hipGetLastError(), "grid kernels");
         c->gridValid = true;
         c->listValid = false;
@@ -776,9 +900,14 @@ int nbx_construct_pairlist(nbx_context* c)
         a.jentry = c->jentry.p;
         launchSearch(c->stream, a, true);
         check(hipGetLastError(), "search kernels");
-        // kernel grid: enough work-groups to fill every CU several times over
-        c->numBlocks = std::max(1, std::min((c->nci + 3) / 4, 8 * c->numCU));
+        buildWorkGroups(c);
+        // kernel grid: as many resident work-groups as LDS allows, at most
+        // eight per CU (4 wavefronts each)
+        const int lds      = nonbondedLdsBytes(c->ntype, c->ic.tab_n, c->elec, c->windowSlots);
+        const int perCU    = std::max(1, std::min(8, (160 * 1024) / lds));
+        c->numBlocks       = std::max(1, std::min(c->ngroups, perCU * c->numCU));
         c->energyPartial.ensure(2 * static_cast<size_t>(c->numBlocks));
+        c->energyChunk.ensure(2 * static_cast<size_t>(std::max(c->nchunk, 1)));
         c->fshiftPartial.ensure(static_cast<size_t>(c->numBlocks) * kNumShifts * 3);
         c->energy.ensure(2);
         c->fshift.ensure(kNumShifts * 3);
@@ -817,9 +946,14 @@ int nbx_dispatch_kernel(nbx_context* c, unsigned flags)
         const bool shift  = (flags & (NBX_COMPUTE_SHIFT_FORCES | NBX_COMPUTE_VIRIAL)) != 0;
         launchClearF4(c->stream, c->fnb.p, c->ncluster * kClusterSize);
         KernelArgs a{};
-        a.ientry        = c->ientry.p;
-        a.nci           = c->nci;
-        a.jentry        = c->jentry.p;
+        a.chunks        = c->chunks.p;
+        a.groups        = c->groups.p;
+        a.ngroups       = c->ngroups;
+        a.groupSize     = c->groupSize;
+        a.ncluster      = c->ncluster;
+        a.windowSlots   = c->windowSlots;
+        a.unionList     = c->unionList.p;
+        a.work          = c->work.p;
         a.xq            = c->xq.p;
         a.type          = c->nbtype.p;
         a.nbfp          = c->nbfp.p;
@@ -827,7 +961,9 @@ int nbx_dispatch_kernel(nbx_context* c, unsigned flags)
         a.tabFDV0       = c->tabFDV0.p;
         a.shiftVec      = c->shiftVec.p;
         a.ic            = c->ic;
+        a.selfCoef      = c->selfCoef;
         a.fnb           = c->fnb.p;
+        a.energyChunk   = c->energyChunk.p;
         a.energyPartial = c->energyPartial.p;
         a.fshiftPartial = c->fshiftPartial.p;
         launchNonbonded(c->stream, a, c->elec, energy, shift, c->numBlocks);
@@ -1045,6 +1181,14 @@ int nbx_get_pairlist(nbx_context* c, int* iEntries, uint32_t* jEntries)
                   "read jentries");
         }
         check(hipStreamSynchronize(c->stream), "sync");
+        if (jEntries)
+        {
+            // the kernel keeps its LDS window slot in the upper 16 bits
+            for (int k = 0; k < c->ncj; k++)
+            {
+                jEntries[2 * k + 1] &= 0xffffu;
+            }
+        }
     });
 }
 
@@ -1081,6 +1225,49 @@ int64_t nbx_list_pairs(nbx_context* c, double r, int64_t* out, int64_t cap)
         }
     });
     return rc == NBX_SUCCESS ? result : -rc;
+}
+
+int nbx_get_work_groups(nbx_context* c, int* info, int* groups, int* chunks, int* unionList, uint32_t* workOut)
+{
+    if (!c || !info)
+    {
+        return fail(NBX_ERROR_INPUT, "NULL argument");
+    }
+    return guarded([&] {
+        requireList(c);
+        setDevice(c);
+        info[0] = c->groupSize;
+        info[1] = c->ngroups;
+        info[2] = c->nchunk;
+        info[3] = c->windowSlots;
+        info[4] = c->nwork;
+        if (groups)
+        {
+            check(hipMemcpyAsync(groups, c->groups.p, sizeof(int4) * c->ngroups, hipMemcpyDeviceToHost, c->stream),
+                  "read groups");
+        }
+        if (chunks)
+        {
+            check(hipMemcpyAsync(chunks, c->chunks.p, sizeof(int4) * c->nchunk, hipMemcpyDeviceToHost, c->stream),
+                  "read chunks");
+        }
+        if (unionList)
+        {
+            int4 last;
+            check(hipMemcpyAsync(&last, c->groups.p + (c->ngroups - 1), sizeof(int4), hipMemcpyDeviceToHost, c->stream),
+                  "read groups");
+            check(hipStreamSynchronize(c->stream), "sync");
+            check(hipMemcpyAsync(unionList, c->unionList.p, sizeof(int) * (last.z + last.w), hipMemcpyDeviceToHost,
+                                 c->stream),
+                  "read union");
+        }
+        if (workOut)
+        {
+            check(hipMemcpyAsync(workOut, c->work.p, sizeof(uint2) * c->nwork, hipMemcpyDeviceToHost, c->stream),
+                  "read work list");
+        }
+        check(hipStreamSynchronize(c->stream), "sync");
+    });
 }
 
 int nbx_get_interaction_constants(nbx_context* c, double* out)

@@ -10,20 +10,29 @@
 //   Ewald (or RF) correction; the self term -1/2 q_i^2 epsfac V(0) is added
 //   once per central i-entry; shift forces are the sum of i-forces per shift.
 //
-// Mapping to the hardware: a wavefront owns one i-entry (4 i-atoms, one
-// shift) and walks its j-list four j-clusters at a time, so one instruction
-// covers 4 i x 4 j x 4 j-clusters = 64 pairs; lane = i + 4*j + 16*k.
-//   * j coordinates+charge arrive as one float4 load per lane (16 atoms x 16 B
-//     per instruction, contiguous per cluster), types as one int load;
-//   * the i-force stays in registers for the whole entry and is reduced over
-//     the 16 lanes that share i with cross-lane shuffles once per entry;
-//   * the j-force of each tile is summed over the 4 i-lanes of a quad with two
-//     DPP adds (quad_perm) and scattered with one float atomic per component,
-//     lane i = 0,1,2 carrying x,y,z;
-//   * LJ parameters and the Ewald F/D/V table sit in LDS;
-//   * energies accumulate per lane in float over one entry, then in double;
-//     per-workgroup double partials are summed by a separate kernel, which
-//     keeps the energies deterministic.
+// Mapping to the hardware.
+//   * A workgroup of 4 wavefronts owns a group of G consecutive i-clusters
+//     (z-neighbours in one grid column) and the union of their j-clusters.
+//     Every j-force goes into an LDS window (one float3 per atom per
+//     distinct j-cluster); the window is flushed with one global atomic per
+//     atom component when the group is done, which cuts global atomics to
+//     about a third of a per-tile scatter.
+//   * Wavefronts take work chunks of the group (pieces of i-entries) from an
+//     LDS counter.  In a chunk the four i-atoms are wave-uniform (scalar
+//     loads) and each lane owns one j-atom: lane = j + 4*k covers 16
+//     j-clusters, and loops over the four i-atoms, so one instruction does
+//     64 pairs and a tile of 16 cluster pairs is 256 pairs.  The lane keeps
+//     its j-force in registers for the tile (no cross-lane reduction) and
+//     adds it to the window with three LDS atomics; the i-forces stay in
+//     registers for the chunk and are reduce-scattered across the wave once.
+//   * j-entries are loaded two tiles ahead, j-atom data one tile ahead.
+//   * Masked j-entries come first in every i-entry, so a chunk's leading
+//     tiles run a loop with exclusion and diagonal checks and the remaining
+//     tiles a loop with none; lanes past the end of a list read a dummy
+//     cluster of far-away fillers instead of carrying a validity mask.
+//   * LJ parameters and the Ewald F/D/V table are in LDS; energies are
+//     summed per lane in float over a chunk, then in double; per-workgroup
+//     double partials are summed by a separate kernel in a fixed order.
 // Pair arithmetic is FP32 on the vector ALUs; no matrix instructions are
 // used (the pair interaction is not a dense contraction).
 #include "nbx_internal.h"
@@ -44,22 +53,6 @@ constexpr float kTwelfthLo = static_cast<float>(1.0 / 12.0 - static_cast<double>
 constexpr float kSixthHi   = static_cast<float>(1.0 / 6.0);
 constexpr float kSixthLo   = static_cast<float>(1.0 / 6.0 - static_cast<double>(static_cast<float>(1.0 / 6.0)));
 
-__device__ __forceinline__ float dppQuadXor1(float v)
-{
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));
-}
-
-__device__ __forceinline__ float dppQuadXor2(float v)
-{
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, true));
-}
-
-__device__ __forceinline__ float quadSum(float v)
-{
-    v = v + dppQuadXor1(v);
-    return v + dppQuadXor2(v);
-}
-
 // 32-bit byte offset from a wave-uniform base: lets the compiler use the
 // scalar-base + vector-offset form of global loads
 template<typename T>
@@ -68,40 +61,46 @@ __device__ __forceinline__ T ldg(const T* base, unsigned index)
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + index * static_cast<unsigned>(sizeof(T)));
 }
 
-// Per i-entry constants of one lane (i-atom ii of the entry's cluster).
-struct IEntry
+// readfirstlane moves 32-bit integers: floats go through their bit pattern
+__device__ __forceinline__ float uniformFloat(float v)
 {
-    float xi, yi, zi, qi;
-    int   tiOff;
-    int   ci;
-    bool  central;
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+// Per chunk, wave-uniform (scalar registers): the four i-atoms as stored
+// and their LJ table row offsets.  The periodic shift is applied to the
+// j-atom instead (x_i + s - x_j = x_i - (x_j - s)), once per tile, so the
+// i-data never has to leave the scalar unit.
+struct IAtoms
+{
+    float x[4], y[4], z[4], q[4];
+    int   tOff[4];
 };
 
-// One 4 i x 4 j x 4 j-cluster tile.  kChecks: exclusion bits, the central
-// diagonal (j <= i skipped) and lanes past the end of the j-list; without it
-// the tile is known to be full and free of masks.
-template<int kElec, bool kEnergy, bool kChecks>
-__device__ __forceinline__ void pairTile(const IEntry& I, const uint2 jent, const float4 xqj, const int tj,
-                                         const bool valid, const unsigned sj, const int ii, const int jj,
-                                         const InteractionConst& ic, const float2* __restrict__ sNbfp,
-                                         const float4* __restrict__ sTab, const int tabN, float4* __restrict__ fnb,
-                                         float& fix, float& fiy, float& fiz, float& vlj, float& vc)
+// One j-atom (this lane) against i-atom kI.  kChecks adds the exclusion
+// bit and the central diagonal.
+template<int kElec, bool kEnergy, bool kChecks, int kI>
+__device__ __forceinline__ void pairI(const IAtoms& I, const float4 xqj, const int tj, const unsigned mbits,
+                                      const bool diag, const int jj, const InteractionConst& ic,
+                                      const float2* __restrict__ sNbfp, const float4* __restrict__ sTab,
+                                      const int tabN, float* fi, float& fjx, float& fjy, float& fjz, float& vlj,
+                                      float& vc)
 {
-    const float dx  = I.xi - xqj.x;
-    const float dy  = I.yi - xqj.y;
-    const float dz  = I.zi - xqj.z;
-    float       rsq = dx * dx + dy * dy + dz * dz;
-    float interact  = 1.0f;
-    float skipmask;
+    // xqj: the j-atom minus the shift, w = epsfac * q_j
+    const float dx       = I.x[kI] - xqj.x;
+    const float dy       = I.y[kI] - xqj.y;
+    const float dz       = I.z[kI] - xqj.z;
+    float       rsq      = dx * dx + dy * dy + dz * dz;
+    float       interact = 1.0f;
+    bool        skip;
     if (kChecks)
     {
-        interact = static_cast<float>((jent.y >> (ii * 4 + jj)) & 1u);
-        const bool skip = !valid || (I.central && static_cast<int>(jent.x) == I.ci && jj <= ii) || rsq >= ic.rcoulomb2;
-        skipmask        = skip ? 0.0f : 1.0f;
+        interact = static_cast<float>((mbits >> (4 * kI)) & 1u);
+        skip     = (diag && jj <= kI) || !(rsq < ic.rcoulomb2);
     }
     else
     {
-        skipmask = (rsq >= ic.rcoulomb2) ? 0.0f : 1.0f;
+        skip = !(rsq < ic.rcoulomb2);
     }
     rsq        = fmaxf(rsq, kMinRsq);
     float rinv = __builtin_amdgcn_rsqf(rsq);
@@ -112,25 +111,27 @@ __device__ __forceinline__ void pairTile(const IEntry& I, const uint2 jent, cons
         const float hx = 0.5f * rsq;
         rinv           = rinv * fmaf(-hx * rinv, rinv, 1.5f);
     }
-    rinv               = rinv * skipmask;
+    rinv               = skip ? 0.0f : rinv;
     const float rinvsq = rinv * rinv;
 
-    const float2 c       = sNbfp[I.tiOff + tj];
-    const float  rinv6   = kChecks ? interact * rinvsq * rinvsq * rinvsq : rinvsq * rinvsq * rinvsq;
-    const float  FrLJ6   = c.x * rinv6;
-    const float  FrLJ12  = c.y * rinv6 * rinv6;
-    const float  frLJ    = FrLJ12 - FrLJ6;
+    const float2 c      = sNbfp[I.tOff[kI] + tj];
+    const float  rinv6  = kChecks ? interact * rinvsq * rinvsq * rinvsq : rinvsq * rinvsq * rinvsq;
+    const float  FrLJ6  = c.x * rinv6;
+    const float  FrLJ12 = c.y * rinv6 * rinv6;
+    const float  frLJ   = FrLJ12 - FrLJ6;
+    const float  smask  = skip ? 0.0f : 1.0f;
     if (kEnergy)
     {
-        // (FrLJ12 + c12 cpot_rep)/12 - (FrLJ6 + c6 cpot_disp)/6 with the
-        // divisions as x*h + x*l (h + l = 1/12 or 1/6 to 2^-48), so the
-        // quotient is rounded once and without bias
+        // (FrLJ12 + c12 cpot_rep)/12 - (FrLJ6 + c6 cpot_disp)/6, each quotient
+        // as x*h + x*l (h + l = 1/12 or 1/6 to 2^-48): rounded once, unbiased
         const float v12 = fmaf(FrLJ12, kTwelfthHi, FrLJ12 * kTwelfthLo) + c.y * ic.cpot_rep12;
         const float v6  = fmaf(FrLJ6, kSixthHi, FrLJ6 * kSixthLo) + c.x * ic.cpot_disp6;
-        vlj += kChecks ? (v12 - v6) * interact * skipmask : (v12 - v6) * skipmask;
+        vlj += kChecks ? (v12 - v6) * interact * smask : (v12 - v6) * smask;
     }
 
-    const float qq = skipmask * I.qi * xqj.w;
+    // beyond the cut-off rinv = 0 zeroes the Ewald force; energies and the
+    // reaction-field constant term need qq = 0 there as well
+    const float qq = (kEnergy || kElec == kElecRF) ? smask * I.q[kI] * xqj.w : I.q[kI] * xqj.w;
     float       fcoul;
     if (kElec == kElecRF)
     {
@@ -161,32 +162,158 @@ __device__ __forceinline__ void pairTile(const IEntry& I, const uint2 jent, cons
     const float fx    = fscal * dx;
     const float fy    = fscal * dy;
     const float fz    = fscal * dz;
-    fix += fx;
-    fiy += fy;
-    fiz += fz;
-    // j-force: sum over the four i-lanes of the quad; lane i = 0,1,2 scatters
-    // component x,y,z
-    const float gx   = quadSum(fx);
-    const float gy   = quadSum(fy);
-    const float gz   = quadSum(fz);
-    const float comp = (ii == 0) ? gx : ((ii == 1) ? gy : gz);
-    if ((!kChecks || valid) && ii < 3)
+    fi[3 * kI + 0] += fx;
+    fi[3 * kI + 1] += fy;
+    fi[3 * kI + 2] += fz;
+    fjx -= fx;
+    fjy -= fy;
+    fjz -= fz;
+}
+
+template<int kElec, bool kEnergy, bool kChecks>
+__device__ __forceinline__ void tile4(const IAtoms& I, const float4 xqj, const int tj, const unsigned mbits,
+                                      const bool diag, const int jj, const InteractionConst& ic,
+                                      const float2* __restrict__ sNbfp, const float4* __restrict__ sTab,
+                                      const int tabN, float* fi, float& fjx, float& fjy, float& fjz, float& vlj,
+                                      float& vc)
+{
+    // barriers between the i-atoms keep the scheduler from interleaving all
+    // four pair evaluations, which costs registers (occupancy) for little overlap
+    pairI<kElec, kEnergy, kChecks, 0>(I, xqj, tj, mbits, diag, jj, ic, sNbfp, sTab, tabN, fi, fjx, fjy, fjz, vlj, vc);
+    __builtin_amdgcn_sched_barrier(0);
+    pairI<kElec, kEnergy, kChecks, 1>(I, xqj, tj, mbits, diag, jj, ic, sNbfp, sTab, tabN, fi, fjx, fjy, fjz, vlj, vc);
+    __builtin_amdgcn_sched_barrier(0);
+    pairI<kElec, kEnergy, kChecks, 2>(I, xqj, tj, mbits, diag, jj, ic, sNbfp, sTab, tabN, fi, fjx, fjy, fjz, vlj, vc);
+    __builtin_amdgcn_sched_barrier(0);
+    pairI<kElec, kEnergy, kChecks, 3>(I, xqj, tj, mbits, diag, jj, ic, sNbfp, sTab, tabN, fi, fjx, fjy, fjz, vlj, vc);
+}
+
+// One tile: 16 j-entries x 4 i-atoms, this lane's j-atom against the four
+// i-atoms.  The j-force goes to the entry's LDS window slot; energies are
+// summed in float over the tile's four pairs, then in double.
+template<int kElec, bool kEnergy>
+__device__ __forceinline__ void tile(const IAtoms& I, const uint2 e, const float4 xj, const int tj, const float4 sv,
+                                     const bool checks, const bool central, const int ci, const int jj,
+                                     const InteractionConst& ic, const float2* __restrict__ sNbfp,
+                                     const float4* __restrict__ sTab, const int tabN, float* __restrict__ sWin,
+                                     float* fi, double& eVdw, double& eCoul)
+{
+    // the j-atom minus the shift, w = epsfac * q_j
+    const float4 xqj = make_float4(xj.x - sv.x, xj.y - sv.y, xj.z - sv.z, ic.epsfac * xj.w);
+    float        fjx = 0.0f, fjy = 0.0f, fjz = 0.0f, vlj = 0.0f, vc = 0.0f;
+    if (checks)
     {
-        unsafeAtomicAdd(reinterpret_cast<float*>(reinterpret_cast<char*>(fnb) + sj * 16u) + ii, -comp);
+        const bool diag = central && static_cast<int>(e.x) == ci;
+        tile4<kElec, kEnergy, true>(I, xqj, tj, (e.y & 0xffffu) >> jj, diag, jj, ic, sNbfp, sTab, tabN, fi, fjx, fjy,
+                                    fjz, vlj, vc);
     }
+    else
+    {
+        tile4<kElec, kEnergy, false>(I, xqj, tj, 0u, false, jj, ic, sNbfp, sTab, tabN, fi, fjx, fjy, fjz, vlj, vc);
+    }
+    float* w = sWin + (e.y >> 16) * 12 + jj * 3;
+    atomicAdd(w + 0, fjx);
+    atomicAdd(w + 1, fjy);
+    atomicAdd(w + 2, fjz);
+    if (kEnergy)
+    {
+        eVdw += static_cast<double>(vlj);
+        eCoul += static_cast<double>(vc);
+    }
+}
+
+// The tiles of one chunk.  Work entries are read two tiles ahead and j-atom
+// data one tile ahead, in a loop unrolled by two so that no loaded value is
+// copied between registers (a copy would wait for its load).  Chunks are
+// padded to whole tiles with a dummy far-away cluster, and reads past the
+// last tile are clamped to it, so no lane needs a bounds check.
+template<int kElec, bool kEnergy>
+__device__ __forceinline__ void chunkTiles(const KernelArgs& a, const IAtoms& I, const float4 sv, const int ci,
+                                           const bool central, const unsigned ws, const int nt, const int nspecial,
+                                           const int jj, const int kk, const InteractionConst& ic,
+                                           const float2* __restrict__ sNbfp, const float4* __restrict__ sTab,
+                                           const int tabN, float* __restrict__ sWin, float* fi, double& eVdw,
+                                           double& eCoul)
+{
+    auto entryAt = [&](int t) { return ldg(a.work, ws + static_cast<unsigned>(kTileJ * min(t, nt - 1) + kk)); };
+    uint2  eA = entryAt(0);
+    uint2  eB = entryAt(1);
+    float4 xA = ldg(a.xq, eA.x * kClusterSize + jj);
+    int    tA = ldg(a.type, eA.x * kClusterSize + jj);
+    for (int t = 0; t < nt; t += 2)
+    {
+        const float4 xB  = ldg(a.xq, eB.x * kClusterSize + jj);
+        const int    tB  = ldg(a.type, eB.x * kClusterSize + jj);
+        const uint2  eA2 = entryAt(t + 2);
+        tile<kElec, kEnergy>(I, eA, xA, tA, sv, t < nspecial, central, ci, jj, ic, sNbfp, sTab, tabN, sWin, fi, eVdw,
+                             eCoul);
+        if (t + 1 >= nt)
+        {
+            break;
+        }
+        const uint2 eB2 = entryAt(t + 3);
+        xA              = ldg(a.xq, eA2.x * kClusterSize + jj);
+        tA              = ldg(a.type, eA2.x * kClusterSize + jj);
+        tile<kElec, kEnergy>(I, eB, xB, tB, sv, t + 1 < nspecial, central, ci, jj, ic, sNbfp, sTab, tabN, sWin, fi,
+                             eVdw, eCoul);
+        eA = eA2;
+        eB = eB2;
+    }
+}
+
+// One butterfly step: lanes with the mask bit set keep b and send a, the
+// others keep a and send b; each adds what its partner sent.
+__device__ __forceinline__ float butterfly(float a, float b, bool hi, int m)
+{
+    const float keep = hi ? b : a;
+    const float send = hi ? a : b;
+    return keep + __shfl_xor(send, m, kWave);
+}
+
+// Reduce-scatter of the 12 i-force components over the wavefront: afterwards
+// lane L holds the wave sum of component L >> 2 (lanes 48..63 hold zeros).
+// Written on named scalars: an indexed array here lets the compiler fold the
+// lane-dependent selects into a dynamically indexed register array, which it
+// expands into long compare/select chains.
+__device__ __forceinline__ float waveReduceScatter12(const float* f, int lane)
+{
+    const bool h32 = (lane & 32) != 0;
+    const bool h16 = (lane & 16) != 0;
+    const bool h8  = (lane & 8) != 0;
+    const bool h4  = (lane & 4) != 0;
+    // component k + 8 for k >= 4 is zero
+    const float a0 = butterfly(f[0], f[8], h32, 32);
+    const float a1 = butterfly(f[1], f[9], h32, 32);
+    const float a2 = butterfly(f[2], f[10], h32, 32);
+    const float a3 = butterfly(f[3], f[11], h32, 32);
+    const float a4 = butterfly(f[4], 0.0f, h32, 32);
+    const float a5 = butterfly(f[5], 0.0f, h32, 32);
+    const float a6 = butterfly(f[6], 0.0f, h32, 32);
+    const float a7 = butterfly(f[7], 0.0f, h32, 32);
+    const float b0 = butterfly(a0, a4, h16, 16);
+    const float b1 = butterfly(a1, a5, h16, 16);
+    const float b2 = butterfly(a2, a6, h16, 16);
+    const float b3 = butterfly(a3, a7, h16, 16);
+    const float c0 = butterfly(b0, b2, h8, 8);
+    const float c1 = butterfly(b1, b3, h8, 8);
+    float       r  = butterfly(c0, c1, h4, 4);
+    r += __shfl_xor(r, 2, kWave);
+    r += __shfl_xor(r, 1, kWave);
+    return r;
 }
 
 template<int kElec, bool kEnergy, bool kShift>
 __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
 {
     extern __shared__ float4 smem4[];
+    __shared__ int sNext;
     const int ntype  = a.ntype;
     const int ntype2 = ntype * ntype;
     float2*   sNbfp  = reinterpret_cast<float2*>(smem4);
     float4*   sTab   = smem4 + ((ntype2 + 1) / 2);
     const int tabN   = (kElec == kElecEwaldTable) ? a.ic.tab_n : 0;
-    float*    sFsh   = reinterpret_cast<float*>(sTab + tabN);
-    double*   sEner  = reinterpret_cast<double*>(sFsh + (kShift ? kNumShifts * 3 + 1 : 0));
+    float*    sWin   = reinterpret_cast<float*>(sTab + tabN);
+    float*    sFsh   = sWin + a.windowSlots * 12;
 
     for (int t = threadIdx.x; t < ntype2; t += kThreads)
     {
@@ -199,144 +326,172 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
             sTab[t] = a.tabFDV0[t];
         }
     }
-    if (kShift)
+    for (int t = threadIdx.x; t < kNumShifts * 3; t += kThreads)
     {
-        for (int t = threadIdx.x; t < kNumShifts * 3; t += kThreads)
-        {
-            sFsh[t] = 0.0f;
-        }
+        sFsh[t] = 0.0f;
     }
-    __syncthreads();
 
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
-    const int ii   = lane & 3;
-    const int jj   = (lane >> 2) & 3;
-    const int kk   = lane >> 4;
+    const int jj   = lane & 3;
+    const int kk   = lane >> 2;
 
     const InteractionConst ic = a.ic;
-    const float selfFac       = 0.5f * ic.self_v0;
+    const int              G  = a.groupSize;
 
-    double eVdw = 0.0, eCoul = 0.0;
+    // this block's energies (lane 0 of wave 0), summed group by group in a
+    // fixed order
+    double blockVdw = 0.0, blockCoul = 0.0;
 
-    for (int e = blockIdx.x * kWavesBlock + wave; e < a.nci; e += gridDim.x * kWavesBlock)
+    for (int g = blockIdx.x; g < a.ngroups; g += gridDim.x)
     {
-        const int4 ie = a.ientry[__builtin_amdgcn_readfirstlane(e)];
-        const int  ci = __builtin_amdgcn_readfirstlane(ie.x);
-        const int  sh = __builtin_amdgcn_readfirstlane(ie.y);
-        const int  js = __builtin_amdgcn_readfirstlane(ie.z);
-        const int  je = __builtin_amdgcn_readfirstlane(ie.w);
-
-        IEntry I;
-        const float4 sv  = a.shiftVec[sh];
-        const float4 xqi = ldg(a.xq, static_cast<unsigned>(ci * kClusterSize + ii));
-        I.xi             = xqi.x + sv.x;
-        I.yi             = xqi.y + sv.y;
-        I.zi             = xqi.z + sv.z;
-        I.qi             = ic.epsfac * xqi.w;
-        I.tiOff          = ldg(a.type, static_cast<unsigned>(ci * kClusterSize + ii)) * ntype;
-        I.ci             = ci;
-        I.central        = (sh == kCentral);
-
-        float fix = 0.0f, fiy = 0.0f, fiz = 0.0f;
-        float vlj = 0.0f, vc = 0.0f;
-        if (kEnergy && I.central && jj == 0 && kk == 0)
+        const int4 grp    = a.groups[g]; // chunk range, union offset, union size
+        const int  c0     = g * G;
+        const int  nslots = G + grp.w;
+        for (int t = threadIdx.x; t < nslots * 12; t += kThreads)
         {
-            // kernel_ref_outer.h:286-326, Vc -= qi * q_i * V(0)/2
-            vc -= I.qi * xqi.w * selfFac;
+            sWin[t] = 0.0f;
         }
-
-        // software pipeline: j-entries two tiles ahead, j-atom data one ahead
-        const int ntile = (je - js + 3) >> 2;
-        auto entryOf    = [&](int t) {
-            const int j = js + 4 * t + kk;
-            return ldg(a.jentry, static_cast<unsigned>(j < je ? j : js));
-        };
-        uint2  eNext  = entryOf(0);
-        uint2  eNext2 = entryOf(1);
-        unsigned sNext = eNext.x * kClusterSize + jj;
-        float4 xNext  = ldg(a.xq, sNext);
-        int    tNext  = ldg(a.type, sNext);
-        for (int t = 0; t < ntile; t++)
+        if (threadIdx.x == 0)
         {
-            const uint2    jent = eNext;
-            const float4   xqj  = xNext;
-            const int      tj   = tNext;
-            const unsigned sj   = sNext;
-            eNext               = eNext2;
-            eNext2              = entryOf(t + 2);
-            sNext               = eNext.x * kClusterSize + jj;
-            xNext               = ldg(a.xq, sNext);
-            tNext               = ldg(a.type, sNext);
-
-            const bool valid   = js + 4 * t + kk < je;
-            const bool special = jent.y != kMaskAll || (I.central && static_cast<int>(jent.x) == ci) || !valid;
-            if (__ballot(special) == 0ull)
-            {
-                pairTile<kElec, kEnergy, false>(I, jent, xqj, tj, true, sj, ii, jj, ic, sNbfp, sTab, tabN, a.fnb,
-                                                fix, fiy, fiz, vlj, vc);
-            }
-            else
-            {
-                pairTile<kElec, kEnergy, true>(I, jent, xqj, tj, valid, sj, ii, jj, ic, sNbfp, sTab, tabN, a.fnb,
-                                               fix, fiy, fiz, vlj, vc);
-            }
+            sNext = grp.x;
         }
+        __syncthreads();
 
-        // i-force: reduce over the 16 lanes sharing ii (lane bits 2..5)
+        for (;;)
+        {
+            int c = 0;
+            if (lane == 0)
+            {
+                c = atomicAdd(&sNext, 1);
+            }
+            c = __builtin_amdgcn_readfirstlane(__shfl(c, 0, kWave));
+            if (c >= grp.y)
+            {
+                break;
+            }
+            const int4 ch      = a.chunks[c];
+            const int  ci      = __builtin_amdgcn_readfirstlane(ch.x);
+            const int  flags   = __builtin_amdgcn_readfirstlane(ch.y);
+            const int  sh      = flags & 0xff;
+            const bool self    = ((flags >> 8) & 1) != 0;
+            const int  nt      = __builtin_amdgcn_readfirstlane(ch.w);
+            const int  nspec   = flags >> 9;
+            const bool central = (sh == kCentral);
+
+            IAtoms       I;
+            const float4 sv    = a.shiftVec[sh];
+            float        qSelf = 0.0f;
 #pragma unroll
-        for (int m = 4; m < kWave; m *= 2)
-        {
-            fix += __shfl_xor(fix, m, kWave);
-            fiy += __shfl_xor(fiy, m, kWave);
-            fiz += __shfl_xor(fiz, m, kWave);
-        }
-        if (kk == 0 && jj < 3)
-        {
-            const float comp = (jj == 0) ? fix : ((jj == 1) ? fiy : fiz);
-            unsafeAtomicAdd(reinterpret_cast<float*>(a.fnb + ci * kClusterSize + ii) + jj, comp);
-        }
-        if (kShift)
-        {
-            const float sx = quadSum(fix);
-            const float sy = quadSum(fiy);
-            const float sz = quadSum(fiz);
-            if (lane < 3)
+            for (int i = 0; i < 4; i++)
             {
-                const float comp = (lane == 0) ? sx : ((lane == 1) ? sy : sz);
-                atomicAdd(sFsh + sh * 3 + lane, comp);
+                // a wave-uniform load, moved to scalar registers
+                const float4 p = a.xq[ci * kClusterSize + i];
+                I.x[i]         = uniformFloat(p.x);
+                I.y[i]         = uniformFloat(p.y);
+                I.z[i]         = uniformFloat(p.z);
+                I.q[i]         = uniformFloat(p.w);
+                I.tOff[i]      = __builtin_amdgcn_readfirstlane(a.type[ci * kClusterSize + i]) * ntype;
+                qSelf          = (lane == i) ? p.w : qSelf;
+            }
+            float fi[12];
+#pragma unroll
+            for (int k = 0; k < 12; k++)
+            {
+                fi[k] = 0.0f;
+            }
+            double eVdw = 0.0, eCoul = 0.0;
+            if (kEnergy && self && lane < 4)
+            {
+                // kernel_ref_outer.h:286-326, Vc -= 1/2 epsfac q_i^2 V(0)
+                eCoul = -a.selfCoef * static_cast<double>(qSelf) * static_cast<double>(qSelf);
+            }
+            chunkTiles<kElec, kEnergy>(a, I, sv, ci, central, static_cast<unsigned>(__builtin_amdgcn_readfirstlane(ch.z)),
+                                       nt, nspec, jj, kk, ic, sNbfp, sTab, tabN, sWin, fi, eVdw, eCoul);
+
+            // i-forces: lane L ends up with component (L >> 2) = 3*i + d
+            const float r   = waveReduceScatter12(fi, lane);
+            const int   idx = lane >> 2;
+            if ((lane & 3) == 0 && idx < 12)
+            {
+                atomicAdd(sWin + (ci - c0) * 12 + idx, r);
+            }
+            if (kEnergy)
+            {
+                // the chunk's energies, summed over the wave in a fixed order:
+                // which wave takes which chunk varies from run to run
+#pragma unroll
+                for (int m = 1; m < kWave; m *= 2)
+                {
+                    eVdw += __shfl_xor(eVdw, m, kWave);
+                    eCoul += __shfl_xor(eCoul, m, kWave);
+                }
+                if (lane == 0)
+                {
+                    a.energyChunk[2 * c]     = eVdw;
+                    a.energyChunk[2 * c + 1] = eCoul;
+                }
+            }
+            if (kShift)
+            {
+                // sum over the four i-atoms of component d: values d, 3+d, 6+d, 9+d
+                const int   d  = lane < 3 ? lane : 0;
+                const float s0 = __shfl(r, 4 * d, kWave);
+                const float s1 = __shfl(r, 4 * (3 + d), kWave);
+                const float s2 = __shfl(r, 4 * (6 + d), kWave);
+                const float s3 = __shfl(r, 4 * (9 + d), kWave);
+                if (lane < 3)
+                {
+                    atomicAdd(sFsh + sh * 3 + lane, (s0 + s1) + (s2 + s3));
+                }
             }
         }
-        if (kEnergy)
+        __syncthreads();
+
+        if (kEnergy && wave == 0)
         {
-            eVdw += static_cast<double>(vlj);
-            eCoul += static_cast<double>(vc);
+            // the group's chunk energies, in chunk order (written by this
+            // block before the barrier)
+            double v = 0.0, q = 0.0;
+            for (int k = grp.x + lane; k < grp.y; k += kWave)
+            {
+                v += a.energyChunk[2 * k];
+                q += a.energyChunk[2 * k + 1];
+            }
+#pragma unroll
+            for (int m = 1; m < kWave; m *= 2)
+            {
+                v += __shfl_xor(v, m, kWave);
+                q += __shfl_xor(q, m, kWave);
+            }
+            blockVdw += v;
+            blockCoul += q;
         }
+
+        // flush the window: one atomic per nonzero atom component
+        for (int t = threadIdx.x; t < nslots * 12; t += kThreads)
+        {
+            const float v = sWin[t];
+            if (v != 0.0f)
+            {
+                const int slot = t / 12;
+                const int r    = t - slot * 12;
+                const int atom = r / 3;
+                const int comp = r - atom * 3;
+                const int cl   = slot < G ? c0 + slot : a.unionList[grp.z + slot - G];
+                if (cl < a.ncluster)
+                {
+                    unsafeAtomicAdd(reinterpret_cast<float*>(a.fnb + cl * kClusterSize + atom) + comp, v);
+                }
+            }
+        }
+        __syncthreads();
     }
 
-    if (kEnergy)
+    if (kEnergy && threadIdx.x == 0)
     {
-#pragma unroll
-        for (int m = 1; m < kWave; m *= 2)
-        {
-            eVdw += __shfl_xor(eVdw, m, kWave);
-            eCoul += __shfl_xor(eCoul, m, kWave);
-        }
-        if (lane == 0)
-        {
-            sEner[2 * wave]     = eVdw;
-            sEner[2 * wave + 1] = eCoul;
-        }
-    }
-    __syncthreads();
-    if (kEnergy && threadIdx.x < 2)
-    {
-        double s = 0.0;
-        for (int w = 0; w < kWavesBlock; w++)
-        {
-            s += sEner[2 * w + threadIdx.x];
-        }
-        a.energyPartial[2 * blockIdx.x + threadIdx.x] = s;
+        a.energyPartial[2 * blockIdx.x]     = blockVdw;
+        a.energyPartial[2 * blockIdx.x + 1] = blockCoul;
     }
     if (kShift)
     {
@@ -370,26 +525,25 @@ void launchElec(hipStream_t s, const KernelArgs& a, bool energy, bool shiftForce
 
 } // namespace
 
-int nonbondedLdsBytes(int ntype, int tabN, int elec, bool /*energy*/)
+int nonbondedLdsBytes(int ntype, int tabN, int elec, int windowSlots)
 {
     int bytes = ((ntype * ntype + 1) / 2) * 16;
     if (elec == kElecEwaldTable)
     {
         bytes += tabN * 16;
     }
-    bytes += (kNumShifts * 3 + 1) * 4;
-    bytes = (bytes + 15) & ~15;
-    bytes += kWavesBlock * 2 * 8;
+    bytes += windowSlots * 12 * 4;
+    bytes += ((kNumShifts * 3 + 4) / 4) * 16;
     return bytes;
 }
 
 void launchNonbonded(hipStream_t s, const KernelArgs& a, int elec, bool energy, bool shiftForces, int numBlocks)
 {
-    if (a.nci == 0 || numBlocks == 0)
+    if (a.ngroups == 0 || numBlocks == 0)
     {
         return;
     }
-    const int lds = nonbondedLdsBytes(a.ntype, a.ic.tab_n, elec, energy);
+    const int lds = nonbondedLdsBytes(a.ntype, a.ic.tab_n, elec, a.windowSlots);
     if (elec == kElecRF)
     {
         launchElec<kElecRF>(s, a, energy, shiftForces, numBlocks, lds);

@@ -7,6 +7,9 @@
 //   atoms     xq[nslot] float4 {x, y, z, q}, type[nslot] int
 //   list      ientry[nci] int4 {ci, shift, cj_start, cj_end},
 //             jentry[ncj] uint2 {cj, exclusion mask (bit 4*i + j)}
+//   work      chunks int4, work[] uint2 {cj, window slot << 16 | mask}: the
+//             list regrouped for the kernel, chunks padded to 16 entries
+//             with a dummy far-away cluster (index ncluster)
 //   forces    fnb[nslot] float4 accumulation buffer, f[natoms*3] output
 //   params    nbfp[(T+1)^2] float2 {6*C6, 12*C12}, Ewald FDV0 table float4,
 //             shift vectors float4[45]
@@ -31,6 +34,15 @@ constexpr float kFarAway     = -1000000.0f;
 constexpr int kMaxTypes = 63;
 // the per-column rank sort is quadratic in the column population
 constexpr int kMaxColumnAtoms = 65536;
+// kernel work groups: at most this many consecutive i-clusters share an LDS
+// j-force window; a group's j-entries are sorted in LDS (at most kGroupMaxJ)
+constexpr int kGroupSizeMax = 8;
+constexpr int kGroupMaxJ    = 4096;
+// j-clusters per kernel tile, j-entries per work chunk (a multiple of it)
+constexpr int kTileJ        = 16;
+constexpr int kChunkEntries = 64;
+// LDS a nonbonded workgroup may use
+constexpr int kKernelLdsBudget = 64 * 1024;
 // rsq floor for float (pairlist.h:146): r > 6.2e-4 nm
 constexpr float kMinRsq = 3.82e-07f;
 
@@ -103,12 +115,45 @@ struct SearchArgs
 void launchSearch(hipStream_t s, const SearchArgs& a, bool fill);
 void launchExclusiveScan(hipStream_t s, int n, const int* in, int* out, int* total);
 
+// -- groups.hip
+struct GroupArgs
+{
+    int          ncluster;
+    int          groupSize;
+    int          ngroups;
+    int          chunkEntries;
+    const int*   offI;      // per cluster: first i-entry (ncluster + 1)
+    const int*   offJ;      // per cluster: first j-entry (ncluster + 1)
+    const int4*  ientry;    // search output {ci, shift, cj_start, cj_end}
+    const uint2* jentryIn;  // search output {cj, mask}
+    uint2*       jentryOut; // the same entries, masked entries first
+    int*         unionCount; // count pass, per group
+    int*         chunkCount; // count pass, per group
+    int*         workCount;  // count pass, per group: padded work entries
+    int*         overflow;   // count pass, per group
+    const int*   unionOff;   // fill pass: exclusive scan of unionCount
+    const int*   chunkOff;   // fill pass: exclusive scan of chunkCount
+    const int*   workOff;    // fill pass: exclusive scan of workCount
+    int*         unionList;
+    uint2*       work;       // {cj, slot << 16 | mask}, each chunk padded to whole tiles
+    int4*        chunks;     // {ci, shift | self << 8 | special << 9, work_start, tiles}
+    int4*        groups;     // {chunk_start, chunk_end, union_offset, union_size}
+};
+void launchGroupSpans(hipStream_t s, int ncluster, const int* offJ, int* maxSpan);
+void launchBuildGroups(hipStream_t s, const GroupArgs& a, bool fill);
+void launchMaxReduce(hipStream_t s, int n, const int* v, int* out);
+
 // -- kernel.hip
 struct KernelArgs
 {
-    const int4*   ientry;
-    int           nci;
-    const uint2*  jentry;
+    const int4*   chunks;
+    const int4*   groups;
+    int           ngroups;
+    int           groupSize;
+    int           ncluster;
+    int           windowSlots; // LDS window capacity in clusters (G + max union)
+    const int*    unionList;
+    const uint2*  work;
     const float4* xq;
     const int*    type;
     const float2* nbfp;
@@ -116,12 +161,14 @@ struct KernelArgs
     const float4* tabFDV0;
     const float4* shiftVec;
     InteractionConst ic;
+    double   selfCoef; // epsfac * V(0) / 2 in double (V(0) from the float beta)
     float4*  fnb;
+    double*  energyChunk;   // [number of chunks][2]
     double*  energyPartial; // [gridDim.x][2]
     float*   fshiftPartial; // [gridDim.x][45*3]
 };
 void launchNonbonded(hipStream_t s, const KernelArgs& a, int elec, bool energy, bool shiftForces, int numBlocks);
-int  nonbondedLdsBytes(int ntype, int tabN, int elec, bool energy);
+int  nonbondedLdsBytes(int ntype, int tabN, int elec, int windowSlots);
 
 // -- buffers.hip
 void launchClearF4(hipStream_t s, float4* p, int n);

@@ -109,6 +109,7 @@ def library() -> C.CDLL:
     lib.nbx_list_pairs.argtypes = [vp, d, vp, C.c_int64]
     lib.nbx_list_pairs.restype = C.c_int64
     lib.nbx_get_interaction_constants.argtypes = [vp, vp]
+    lib.nbx_get_work_groups.argtypes = [vp, vp, vp, vp, vp, vp]
     _lib = lib
     return lib
 
@@ -281,6 +282,24 @@ class Context:
         if m < 0:
             _raise(int(-m), "nbx_list_pairs")
         return np.sort(out[:n])
+
+    def work_groups(self) -> dict:
+        """The kernel's work groups: see nbx_get_work_groups in include/nbx.h."""
+        info = np.zeros(5, np.int32)
+        self._check(self._lib.nbx_get_work_groups(self._h, _ptr(info), None, None, None, None),
+                    "nbx_get_work_groups")
+        G, ng, nch, slots, nwork = (int(v) for v in info)
+        groups = np.zeros((max(ng, 1), 4), np.int32)
+        chunks = np.zeros((max(nch, 1), 4), np.int32)
+        self._check(self._lib.nbx_get_work_groups(self._h, _ptr(info), _ptr(groups), _ptr(chunks), None, None),
+                    "nbx_get_work_groups")
+        total = int(groups[ng - 1, 2] + groups[ng - 1, 3]) if ng else 0
+        union = np.zeros(max(total, 1), np.int32)
+        work = np.zeros((max(nwork, 1), 2), np.uint32)
+        self._check(self._lib.nbx_get_work_groups(self._h, _ptr(info), None, None, _ptr(union), _ptr(work)),
+                    "nbx_get_work_groups")
+        return dict(group_size=G, num_groups=ng, num_chunks=nch, window_slots=slots, groups=groups[:ng],
+                    chunks=chunks[:nch], union=union[:total], work=work[:nwork])
 
     def interaction_constants(self) -> dict:
         out = np.zeros(9)

