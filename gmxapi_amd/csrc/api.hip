@@ -272,7 +272,7 @@ struct nbx_context
 
     // outputs
     int             numBlocks = 0;
-    DevBuf<double>  energyChunk, energyPartial, energy, fshift, virialPartial, virial;
+    DevBuf<double>  energyPartial, energy, fshift, virialPartial, virial;
     DevBuf<float>   fshiftPartial, fOut;
     bool            haveEnergy = false, haveShift = false, haveVirial = false;
 };
@@ -907,7 +907,6 @@ int nbx_construct_pairlist(nbx_context* c)
         const int perCU    = std::max(1, std::min(8, (160 * 1024) / lds));
         c->numBlocks       = std::max(1, std::min(c->ngroups, perCU * c->numCU));
         c->energyPartial.ensure(2 * static_cast<size_t>(c->numBlocks));
-        c->energyChunk.ensure(2 * static_cast<size_t>(std::max(c->nchunk, 1)));
         c->fshiftPartial.ensure(static_cast<size_t>(c->numBlocks) * kNumShifts * 3);
         c->energy.ensure(2);
         c->fshift.ensure(kNumShifts * 3);
@@ -963,7 +962,6 @@ int nbx_dispatch_kernel(nbx_context* c, unsigned flags)
         a.ic            = c->ic;
         a.selfCoef      = c->selfCoef;
         a.fnb           = c->fnb.p;
-        a.energyChunk   = c->energyChunk.p;
         a.energyPartial = c->energyPartial.p;
         a.fshiftPartial = c->fshiftPartial.p;
         launchNonbonded(c->stream, a, c->elec, energy, shift, c->numBlocks);

@@ -261,42 +261,25 @@ __device__ __forceinline__ void chunkTiles(const KernelArgs& a, const IAtoms& I,
     }
 }
 
-// One butterfly step: lanes with the mask bit set keep b and send a, the
-// others keep a and send b; each adds what its partner sent.
-__device__ __forceinline__ float butterfly(float a, float b, bool hi, int m)
+// Reduce-scatter of 16 per-lane values over the wavefront: afterwards lane L
+// holds the wave sum of value (L >> 2).
+__device__ __forceinline__ float waveReduceScatter16(float* v, int lane)
 {
-    const float keep = hi ? b : a;
-    const float send = hi ? a : b;
-    return keep + __shfl_xor(send, m, kWave);
-}
-
-// Reduce-scatter of the 12 i-force components over the wavefront: afterwards
-// lane L holds the wave sum of component L >> 2 (lanes 48..63 hold zeros).
-// Written on named scalars: an indexed array here lets the compiler fold the
-// lane-dependent selects into a dynamically indexed register array, which it
-// expands into long compare/select chains.
-__device__ __forceinline__ float waveReduceScatter12(const float* f, int lane)
-{
-    const bool h32 = (lane & 32) != 0;
-    const bool h16 = (lane & 16) != 0;
-    const bool h8  = (lane & 8) != 0;
-    const bool h4  = (lane & 4) != 0;
-    // component k + 8 for k >= 4 is zero
-    const float a0 = butterfly(f[0], f[8], h32, 32);
-    const float a1 = butterfly(f[1], f[9], h32, 32);
-    const float a2 = butterfly(f[2], f[10], h32, 32);
-    const float a3 = butterfly(f[3], f[11], h32, 32);
-    const float a4 = butterfly(f[4], 0.0f, h32, 32);
-    const float a5 = butterfly(f[5], 0.0f, h32, 32);
-    const float a6 = butterfly(f[6], 0.0f, h32, 32);
-    const float a7 = butterfly(f[7], 0.0f, h32, 32);
-    const float b0 = butterfly(a0, a4, h16, 16);
-    const float b1 = butterfly(a1, a5, h16, 16);
-    const float b2 = butterfly(a2, a6, h16, 16);
-    const float b3 = butterfly(a3, a7, h16, 16);
-    const float c0 = butterfly(b0, b2, h8, 8);
-    const float c1 = butterfly(b1, b3, h8, 8);
-    float       r  = butterfly(c0, c1, h4, 4);
+#pragma unroll
+    for (int step = 0; step < 4; step++)
+    {
+        const int  m    = 32 >> step;
+        const int  half = 8 >> step;
+        const bool hi   = (lane & m) != 0;
+#pragma unroll
+        for (int k = 0; k < half; k++)
+        {
+            const float keep = hi ? v[k + half] : v[k];
+            const float send = hi ? v[k] : v[k + half];
+            v[k]             = keep + __shfl_xor(send, m, kWave);
+        }
+    }
+    float r = v[0];
     r += __shfl_xor(r, 2, kWave);
     r += __shfl_xor(r, 1, kWave);
     return r;
@@ -314,6 +297,7 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
     const int tabN   = (kElec == kElecEwaldTable) ? a.ic.tab_n : 0;
     float*    sWin   = reinterpret_cast<float*>(sTab + tabN);
     float*    sFsh   = sWin + a.windowSlots * 12;
+    double*   sEner  = reinterpret_cast<double*>(reinterpret_cast<float4*>(sFsh) + (kNumShifts * 3 + 4) / 4);
 
     for (int t = threadIdx.x; t < ntype2; t += kThreads)
     {
@@ -339,9 +323,7 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
     const InteractionConst ic = a.ic;
     const int              G  = a.groupSize;
 
-    // this block's energies (lane 0 of wave 0), summed group by group in a
-    // fixed order
-    double blockVdw = 0.0, blockCoul = 0.0;
+    double eVdw = 0.0, eCoul = 0.0;
 
     for (int g = blockIdx.x; g < a.ngroups; g += gridDim.x)
     {
@@ -400,37 +382,31 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
             {
                 fi[k] = 0.0f;
             }
-            double eVdw = 0.0, eCoul = 0.0;
             if (kEnergy && self && lane < 4)
             {
                 // kernel_ref_outer.h:286-326, Vc -= 1/2 epsfac q_i^2 V(0)
-                eCoul = -a.selfCoef * static_cast<double>(qSelf) * static_cast<double>(qSelf);
+                eCoul -= a.selfCoef * static_cast<double>(qSelf) * static_cast<double>(qSelf);
             }
             chunkTiles<kElec, kEnergy>(a, I, sv, ci, central, static_cast<unsigned>(__builtin_amdgcn_readfirstlane(ch.z)),
                                        nt, nspec, jj, kk, ic, sNbfp, sTab, tabN, sWin, fi, eVdw, eCoul);
 
             // i-forces: lane L ends up with component (L >> 2) = 3*i + d
-            const float r   = waveReduceScatter12(fi, lane);
+            float v16[16];
+#pragma unroll
+            for (int k = 0; k < 12; k++)
+            {
+                v16[k] = fi[k];
+            }
+#pragma unroll
+            for (int k = 12; k < 16; k++)
+            {
+                v16[k] = 0.0f;
+            }
+            const float r   = waveReduceScatter16(v16, lane);
             const int   idx = lane >> 2;
             if ((lane & 3) == 0 && idx < 12)
             {
                 atomicAdd(sWin + (ci - c0) * 12 + idx, r);
-            }
-            if (kEnergy)
-            {
-                // the chunk's energies, summed over the wave in a fixed order:
-                // which wave takes which chunk varies from run to run
-#pragma unroll
-                for (int m = 1; m < kWave; m *= 2)
-                {
-                    eVdw += __shfl_xor(eVdw, m, kWave);
-                    eCoul += __shfl_xor(eCoul, m, kWave);
-                }
-                if (lane == 0)
-                {
-                    a.energyChunk[2 * c]     = eVdw;
-                    a.energyChunk[2 * c + 1] = eCoul;
-                }
             }
             if (kShift)
             {
@@ -447,26 +423,6 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
             }
         }
         __syncthreads();
-
-        if (kEnergy && wave == 0)
-        {
-            // the group's chunk energies, in chunk order (written by this
-            // block before the barrier)
-            double v = 0.0, q = 0.0;
-            for (int k = grp.x + lane; k < grp.y; k += kWave)
-            {
-                v += a.energyChunk[2 * k];
-                q += a.energyChunk[2 * k + 1];
-            }
-#pragma unroll
-            for (int m = 1; m < kWave; m *= 2)
-            {
-                v += __shfl_xor(v, m, kWave);
-                q += __shfl_xor(q, m, kWave);
-            }
-            blockVdw += v;
-            blockCoul += q;
-        }
 
         // flush the window: one atomic per nonzero atom component
         for (int t = threadIdx.x; t < nslots * 12; t += kThreads)
@@ -488,10 +444,29 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
         __syncthreads();
     }
 
-    if (kEnergy && threadIdx.x == 0)
+    if (kEnergy)
     {
-        a.energyPartial[2 * blockIdx.x]     = blockVdw;
-        a.energyPartial[2 * blockIdx.x + 1] = blockCoul;
+#pragma unroll
+        for (int m = 1; m < kWave; m *= 2)
+        {
+            eVdw += __shfl_xor(eVdw, m, kWave);
+            eCoul += __shfl_xor(eCoul, m, kWave);
+        }
+        if (lane == 0)
+        {
+            sEner[2 * wave]     = eVdw;
+            sEner[2 * wave + 1] = eCoul;
+        }
+    }
+    __syncthreads();
+    if (kEnergy && threadIdx.x < 2)
+    {
+        double s = 0.0;
+        for (int w = 0; w < kWavesBlock; w++)
+        {
+            s += sEner[2 * w + threadIdx.x];
+        }
+        a.energyPartial[2 * blockIdx.x + threadIdx.x] = s;
     }
     if (kShift)
     {
@@ -534,6 +509,7 @@ int nonbondedLdsBytes(int ntype, int tabN, int elec, int windowSlots)
     }
     bytes += windowSlots * 12 * 4;
     bytes += ((kNumShifts * 3 + 4) / 4) * 16;
+    bytes += kWavesBlock * 2 * 8;
     return bytes;
 }
 

@@ -163,7 +163,6 @@ struct KernelArgs
     InteractionConst ic;
     double   selfCoef; // epsfac * V(0) / 2 in double (V(0) from the float beta)
     float4*  fnb;
-    double*  energyChunk;   // [number of chunks][2]
     double*  energyPartial; // [gridDim.x][2]
     float*   fshiftPartial; // [gridDim.x][45*3]
 };
