@@ -1,10 +1,13 @@
-"""Builds libnbx.so, the HIP library behind the C ABI in include/nbx.h.
+"""Builds libnbx.so, the HIP library behind the C ABI in include/nbx.h, and
+the nblib-compatible C++ layer over it.
 
 The library is compiled in-tree for gfx950 with hipcc: every .hip file under
 gmxapi_amd/csrc is compiled (device and host code) and linked into
 gmxapi_amd/lib/libnbx.so.  Packed FP32 instructions are disabled
 (-packed-fp32-ops); float atomics map to the hardware instructions
-(-munsafe-fp-atomics).
+(-munsafe-fp-atomics).  gmxapi_amd/nblib/nblib.cpp is compiled with g++ into
+gmxapi_amd/lib/libnblib_amd.so (linked to libnbx.so, found through
+$ORIGIN), and tests/cpp/test_nblib.cpp into gmxapi_amd/lib/test_nblib.
 """
 from __future__ import annotations
 
@@ -31,6 +34,42 @@ def library_path() -> str:
     return os.path.join(LIBDIR, LIBNAME)
 
 
+def nblib_path() -> str:
+    return os.path.join(LIBDIR, "libnblib_amd.so")
+
+
+def nblib_test_path() -> str:
+    return os.path.join(LIBDIR, "test_nblib")
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd))
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    errs = "\n".join(l for l in (res.stderr or "").splitlines() if "packed-fp32-ops" not in l)
+    if res.returncode != 0:
+        raise RuntimeError(f"{cmd[0]} failed ({res.returncode}):\n{errs}")
+    if verbose and errs.strip():
+        print(errs)
+
+
+def build_nblib(verbose: bool = False) -> str:
+    """Compile the C++ nblib layer and its test program (needs libnbx.so)."""
+    root = os.path.dirname(HERE)
+    inc = ["-I", os.path.join(root, "include"), "-I", os.path.join(HERE, "nblib")]
+    rpath = "-Wl,-rpath,$ORIGIN"
+    lib = nblib_path()
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall"] + inc
+         + [os.path.join(HERE, "nblib", "nblib.cpp"), "-L", LIBDIR, "-lnbx", rpath, "-o", lib + ".tmp"], verbose)
+    os.replace(lib + ".tmp", lib)
+    exe = nblib_test_path()
+    _run(["g++", "-O2", "-std=c++17", "-Wall"] + inc
+         + [os.path.join(root, "tests", "cpp", "test_nblib.cpp"), "-L", LIBDIR, "-lnblib_amd", "-lnbx", rpath,
+            "-o", exe + ".tmp"], verbose)
+    os.replace(exe + ".tmp", exe)
+    return lib
+
+
 def build(verbose: bool = False) -> str:
     """Compile the library; returns its path."""
     os.makedirs(LIBDIR, exist_ok=True)
@@ -51,6 +90,7 @@ def build(verbose: bool = False) -> str:
     if verbose and errs.strip():
         print(errs)
     os.replace(tmp, out)
+    build_nblib(verbose)
     return out
 
 

@@ -1,0 +1,326 @@
+// nblib-compatible host API over the HIP nonbonded engine; see nblib.h.
+#include "nblib.h"
+
+#include <cmath>
+#include <utility>
+
+#include "nbx.h"
+
+namespace nblib
+{
+
+namespace
+{
+
+// nbx status -> the exception nblib would throw for the same condition
+void check(int rc, const char* what)
+{
+    if (rc == NBX_SUCCESS)
+    {
+        return;
+    }
+    const std::string msg = std::string(what) + ": " + nbx_last_error();
+    if (rc == NBX_ERROR_INPUT || rc == NBX_ERROR_STATE)
+    {
+        throw InputException(msg);
+    }
+    throw std::runtime_error(msg);
+}
+
+bool finite(const std::vector<Vec3>& v)
+{
+    for (const Vec3& a : v)
+    {
+        if (!std::isfinite(a[0]) || !std::isfinite(a[1]) || !std::isfinite(a[2]))
+        {
+            return false;
+        }
+    }
+    return true;
+}
+
+} // namespace
+
+Box::Box(real l) : Box(l, l, l) {}
+
+// box.cpp:52-63
+Box::Box(real x, real y, real z)
+{
+    if (!std::isfinite(x) || !std::isfinite(y) || !std::isfinite(z))
+    {
+        throw InputException("Cannot have NaN or Inf box length.");
+    }
+    m_[0][0] = x;
+    m_[1][1] = y;
+    m_[2][2] = z;
+}
+
+void NonBondedInteractionMap::setInteractions(const std::string& a, const std::string& b, real c6, real c12)
+{
+    map_[{ a, b }] = { c6, c12 };
+    map_[{ b, a }] = { c6, c12 };
+}
+
+real NonBondedInteractionMap::getC6(const std::string& a, const std::string& b) const
+{
+    auto it = map_.find({ a, b });
+    if (it == map_.end())
+    {
+        throw InputException("No non-bonded interaction between " + a + " and " + b);
+    }
+    return it->second.first;
+}
+
+real NonBondedInteractionMap::getC12(const std::string& a, const std::string& b) const
+{
+    auto it = map_.find({ a, b });
+    if (it == map_.end())
+    {
+        throw InputException("No non-bonded interaction between " + a + " and " + b);
+    }
+    return it->second.second;
+}
+
+// interactions.cpp:83-93: only the geometric rule is defined
+static real combine(real v, real w, CombinationRule rule)
+{
+    if (rule != CombinationRule::Geometric)
+    {
+        throw InputException("unknown LJ Combination rule specified\n");
+    }
+    return std::sqrt(v * w);
+}
+
+// interactions.cpp:97-140
+ParticleTypesInteractions& ParticleTypesInteractions::add(const std::string& name, real c6, real c12)
+{
+    auto ins = single_.insert({ name, { c6, c12 } });
+    if (!ins.second && (ins.first->second.first != c6 || ins.first->second.second != c12))
+    {
+        throw InputException("Attempting to add nonbonded interaction parameters for particle type " + name
+                             + " twice");
+    }
+    return *this;
+}
+
+ParticleTypesInteractions& ParticleTypesInteractions::add(const std::string& a, const std::string& b, real c6, real c12)
+{
+    auto ins = pairs_.insert({ { a, b }, { c6, c12 } });
+    pairs_.insert({ { b, a }, { c6, c12 } });
+    if (!ins.second && (ins.first->second.first != c6 || ins.first->second.second != c12))
+    {
+        throw InputException("Attempting to add nonbonded interaction parameters between the particle types "
+                             + a + " " + b + " twice");
+    }
+    return *this;
+}
+
+// interactions.cpp:142-190
+NonBondedInteractionMap ParticleTypesInteractions::generateTable() const
+{
+    NonBondedInteractionMap m;
+    for (const auto& p1 : single_)
+    {
+        for (const auto& p2 : single_)
+        {
+            m.setInteractions(p1.first, p2.first, combine(p1.second.first, p2.second.first, rule_),
+                              combine(p1.second.second, p2.second.second, rule_));
+        }
+    }
+    for (const auto& p : pairs_)
+    {
+        m.setInteractions(p.first.first, p.first.second, p.second.first, p.second.second);
+    }
+    return m;
+}
+
+Topology::Topology(std::vector<ParticleType> particleTypes, std::vector<int> typeIdOfAllParticles,
+                   std::vector<real> charges, ExclusionLists exclusions, NonBondedInteractionMap interactions,
+                   CombinationRule rule) :
+    types_(std::move(particleTypes)),
+    typeIds_(std::move(typeIdOfAllParticles)),
+    charges_(std::move(charges)),
+    exclusions_(std::move(exclusions)),
+    interactions_(std::move(interactions)),
+    rule_(rule)
+{
+    const int n = numParticles();
+    if (static_cast<int>(charges_.size()) != n || exclusions_.numLists() != n)
+    {
+        throw InputException("Topology: types, charges and exclusion lists must have one entry per particle");
+    }
+    for (int t : typeIds_)
+    {
+        if (t < 0 || t >= static_cast<int>(types_.size()))
+        {
+            throw InputException("Topology: particle type id out of range");
+        }
+    }
+    // every type pair must have parameters (TopologyBuilder generates them)
+    for (const ParticleType& a : types_)
+    {
+        for (const ParticleType& b : types_)
+        {
+            interactions_.getC6(a.name(), b.name());
+        }
+    }
+}
+
+// simulationstate.cpp:70-84
+SimulationState::SimulationState(const std::vector<Vec3>& coordinates, const std::vector<Vec3>& velocities,
+                                 const std::vector<Vec3>& forces, Box box, Topology topology) :
+    x_(coordinates), v_(velocities), f_(forces), box_(box), topology_(std::move(topology))
+{
+    if (!finite(x_))
+    {
+        throw InputException("Input coordinates has at least one NaN");
+    }
+    if (!finite(v_))
+    {
+        throw InputException("Input velocities has at least one NaN");
+    }
+}
+
+struct ForceCalculator::Impl
+{
+    nbx_context* ctx   = nullptr;
+    unsigned     flags = NBX_COMPUTE_FORCES;
+    int          n     = 0;
+    ~Impl()
+    {
+        if (ctx)
+        {
+            nbx_destroy(ctx);
+        }
+    }
+};
+
+// GmxSetupDirector::setupGmxForceCalculator (gmxsetup.cpp:306-325)
+ForceCalculator::ForceCalculator(const SimulationState& system, const NBKernelOptions& options) :
+    impl_(std::make_unique<Impl>())
+{
+    const Topology& top = system.topology();
+    nbx_options     o;
+    nbx_default_options(&o);
+    switch (options.coulombType)
+    {
+        case CoulombType::Pme: o.coulomb_type = NBX_COULOMB_EWALD; break;
+        case CoulombType::Cutoff: o.coulomb_type = NBX_COULOMB_CUTOFF; break;
+        case CoulombType::ReactionField: o.coulomb_type = NBX_COULOMB_REACTION_FIELD; break;
+        default: throw InputException("Unsupported electrostatic interaction");
+    }
+    // the plain-C kernel always tabulates the Ewald correction
+    // (gmxsetup.cpp:117-128); the others follow useTabulatedEwaldCorr
+    const bool table   = options.nbnxmSimd == SimdKernels::SimdNo || options.useTabulatedEwaldCorr;
+    o.ewald_correction = table ? NBX_EWALD_TABLE : NBX_EWALD_ANALYTICAL;
+    o.pairlist_cutoff  = options.pairlistCutoff;
+    o.device           = options.device;
+    check(nbx_create(&o, &impl_->ctx), "ForceCalculator");
+
+    const auto& types = top.getParticleTypes();
+    const int   nt    = static_cast<int>(types.size());
+    const auto& map   = top.getNonBondedInteractionMap();
+    std::vector<double> c6(nt * nt), c12(nt * nt);
+    for (int a = 0; a < nt; a++)
+    {
+        for (int b = 0; b < nt; b++)
+        {
+            c6[a * nt + b]  = map.getC6(types[a].name(), types[b].name());
+            c12[a * nt + b] = map.getC12(types[a].name(), types[b].name());
+        }
+    }
+    check(nbx_set_nonbonded_parameters(impl_->ctx, nt, c6.data(), c12.data()), "setNonBondedParameters");
+    const ExclusionLists& ex = top.getGmxExclusions();
+    impl_->n                 = top.numParticles();
+    check(nbx_set_topology(impl_->ctx, impl_->n, top.getParticleTypeIdOfAllParticles().data(),
+                           top.getCharges().data(), ex.offsets.data(), ex.indices.data()),
+          "setAtomProperties");
+    if (options.computeVirialAndEnergy)
+    {
+        impl_->flags |= NBX_COMPUTE_ENERGY | NBX_COMPUTE_VIRIAL;
+    }
+    const std::vector<Vec3>& x = system.coordinates();
+    std::vector<int>         info(x.size(), 0);
+    updatePairList(ArrayRef<const int>(info.data(), info.size()),
+                   ArrayRef<Vec3>(const_cast<Vec3*>(x.data()), x.size()), system.box());
+}
+
+ForceCalculator::~ForceCalculator()                                = default;
+ForceCalculator::ForceCalculator(ForceCalculator&&) noexcept            = default;
+ForceCalculator& ForceCalculator::operator=(ForceCalculator&&) noexcept = default;
+
+void ForceCalculator::compute(ArrayRef<const Vec3> coordinates, ArrayRef<Vec3> forces)
+{
+    if (static_cast<int>(coordinates.size()) != impl_->n || static_cast<int>(forces.size()) != impl_->n)
+    {
+        throw InputException("compute: coordinate and force arrays must hold one entry per particle");
+    }
+    static_assert(sizeof(Vec3) == 3 * sizeof(float), "Vec3 must be a packed float triplet");
+    check(nbx_compute(impl_->ctx, &coordinates.data()->v[0], &forces.data()->v[0], impl_->flags, 0), "compute");
+}
+
+// gmxcalculator.cpp:85-103 (rectangular boxes only) and constructPairlist
+void ForceCalculator::updatePairList(ArrayRef<const int> particleInfoAllVdW, ArrayRef<Vec3> coordinates,
+                                     const Box& box)
+{
+    (void)particleInfoAllVdW; // every particle has LJ and charge on this path (gmxsetup.cpp:132-141)
+    const auto& m = box.legacyMatrix();
+    if (m[0][1] != 0 || m[0][2] != 0 || m[1][0] != 0 || m[1][2] != 0 || m[2][0] != 0 || m[2][1] != 0)
+    {
+        throw InputException("Only rectangular unit-cells are supported here");
+    }
+    if (static_cast<int>(coordinates.size()) != impl_->n)
+    {
+        throw InputException("updatePairList: one coordinate per particle expected");
+    }
+    const float diag[3] = { m[0][0], m[1][1], m[2][2] };
+    check(nbx_put_on_grid(impl_->ctx, diag, &coordinates.data()->v[0], 0), "setParticlesOnGrid");
+    check(nbx_construct_pairlist(impl_->ctx), "constructPairlist");
+}
+
+std::array<double, 2> ForceCalculator::energies() const
+{
+    std::array<double, 2> e{};
+    check(nbx_get_energies(impl_->ctx, &e[0], &e[1]), "energies");
+    return e;
+}
+
+std::array<double, 9> ForceCalculator::virial() const
+{
+    std::array<double, 9> v{};
+    check(nbx_get_virial(impl_->ctx, v.data()), "virial");
+    return v;
+}
+
+// integrator.cpp:53-75
+LeapFrog::LeapFrog(const Topology& topology, const Box& box) : box_(box)
+{
+    inverseMasses_.resize(topology.numParticles());
+    for (int i = 0; i < topology.numParticles(); i++)
+    {
+        const int t       = topology.getParticleTypeIdOfAllParticles()[i];
+        inverseMasses_[i] = 1.0f / topology.getParticleTypes()[t].mass();
+    }
+}
+
+void LeapFrog::integrate(real dt, ArrayRef<Vec3> x, ArrayRef<Vec3> v, ArrayRef<const Vec3> f)
+{
+    const auto& m = box_.legacyMatrix();
+    for (std::size_t i = 0; i < x.size(); i++)
+    {
+        for (int d = 0; d < 3; d++)
+        {
+            v[i][d] += f[i][d] * dt * inverseMasses_[i];
+            x[i][d] += v[i][d] * dt;
+            // put_atoms_in_box for a rectangular box
+            const real l = m[d][d];
+            x[i][d] -= l * std::floor(x[i][d] / l);
+            if (x[i][d] >= l)
+            {
+                x[i][d] -= l;
+            }
+        }
+    }
+}
+
+} // namespace nblib
