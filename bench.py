@@ -85,6 +85,8 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="water288k", choices=["water288k", "water9k", "chains150k"])
+    ap.add_argument("--ewald", default="table", choices=["table", "analytical"],
+                    help="Ewald correction: the reference's spline table or the analytical form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -113,7 +115,7 @@ def main():
     # below run on it, so the events bracket exactly the library's work
     stream = torch.cuda.Stream(device=dev)
     torch.cuda.set_stream(stream)
-    ctx = nbx.Context(rlist=s.rlist, rcoulomb=s.rcoulomb, device=0)
+    ctx = nbx.Context(rlist=s.rlist, rcoulomb=s.rcoulomb, ewald_correction=args.ewald, device=0)
     ctx.set_stream(stream.cuda_stream)
     m6, m12 = nbx.nblib_c6c12_matrices(s.c6, s.c12)
     ctx.set_nonbonded_parameters(m6, m12)
@@ -153,6 +155,9 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    # the library records HIP events on its stream around each launch of the
+    # cluster-pair kernel during the timed steps
+    ctx.set_kernel_timing(args.steps)
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     start.record(stream)
     for _ in range(args.steps):
@@ -160,22 +165,15 @@ def main():
     stop.record(stream)
     torch.cuda.synchronize()
     ms_per_step = start.elapsed_time(stop) / args.steps
+    kernel_ms, launches = ctx.kernel_timing()
+    assert launches == args.steps, launches
 
-    # kernel alone (dispatch = clear nbat f + cluster-pair kernel + FP64 reduction of partials)
-    kstart, kstop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    kstart.record(stream)
-    for _ in range(args.steps):
-        ctx.dispatch_kernel(nbx.COMPUTE_FORCES | nbx.COMPUTE_ENERGY | nbx.COMPUTE_SHIFT_FORCES)
-    kstop.record(stream)
-    torch.cuda.synchronize()
-    kernel_ms = kstart.elapsed_time(kstop) / args.steps
-    fstart, fstop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    fstart.record(stream)
+    # the forces-only kernel (no energies, no shift forces), same way
+    ctx.set_kernel_timing(args.steps)
     for _ in range(args.steps):
         ctx.dispatch_kernel(nbx.COMPUTE_FORCES)
-    fstop.record(stream)
-    torch.cuda.synchronize()
-    kernel_f_ms = fstart.elapsed_time(fstop) / args.steps
+    kernel_f_ms, _ = ctx.kernel_timing()
+    ctx.set_kernel_timing(0)
     step()
     torch.cuda.synchronize()
     vlj, vc = ctx.energies()
@@ -198,8 +196,9 @@ def main():
             "workload": {"water288k": "water-288k: 96,000 rigid 3-site waters, 14.2 nm cubic box",
                          "water9k": "water-9k: 3,000 rigid 3-site waters, 4.48 nm cubic box",
                          "chains150k": "chains-150k: 15,000 10-atom chains, 20 LJ types, 11.45 nm box"}[
-                args.workload] + f", rc {s.rcoulomb} nm, rlist {s.rlist} nm, Ewald rtol 1e-5 (table), "
-                "LJ potential shift; step = x convert + F/E/shift-force kernel + f reduce + FP64 virial",
+                args.workload] + f", rc {s.rcoulomb} nm, rlist {s.rlist} nm, Ewald rtol 1e-5 ({args.ewald} "
+                "correction), LJ potential shift; step = x convert + F/E/shift-force kernel + FP64 self/exclusion "
+                "energies + f reduce + FP64 virial",
             "natoms": s.natoms,
             "useful_pairs": useful,
             "computed_pairs": computed_pairs,
@@ -217,7 +216,9 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "frac": achieved / PEAK_FP32_TFLOPS,
             "traffic": None,
-            "kernel": "nonbondedKernel<EwaldTable, energy, shift>",
+            "kernel": "nonbondedKernel<%s, energy, shift>" % ("EwaldTable" if args.ewald == "table"
+                                                               else "EwaldAnalytical"),
+            "kernel_ms": kernel_ms,
             "compute_unit": "FP32 vector ALU (pair math is not a dense contraction; peak is MI355X FP32 157.3 TFLOP/s)",
             "flops_per_pair": FLOPS_PER_PAIR_VF,
             "pairs_per_launch": computed_pairs,

@@ -134,12 +134,15 @@ double vEwaldLr(double beta, double r)
 }
 
 // generateEwaldCorrectionTables (tables/forcetable.cpp:144-298): quadratic
-// spline of erf(beta r)/r; returns FDV0 quadruplets {F_i, F_i+1 - F_i, V_i, 0}.
-std::vector<float> ewaldTableFDV0(int numPoints, double tableScaling, float beta)
+// spline of erf(beta r)/r; returns FDV0 quadruplets {F_i, F_i+1 - F_i, V_i, 0}
+// stored in Real (float: the mixed-precision build's table; double: the
+// double build's, used for the excluded-pair energies).
+template<typename Real>
+std::vector<Real> ewaldTableFDV0(int numPoints, double tableScaling, float beta)
 {
-    std::vector<float> tf(numPoints, 0.0f), tv(numPoints, 0.0f);
+    std::vector<Real> tf(numPoints, Real(0)), tv(numPoints, Real(0));
     const double dx       = 1 / tableScaling;
-    const float  tabMax   = static_cast<float>(FLT_MAX * 0.0001);
+    const Real   tabMax   = static_cast<Real>(FLT_MAX * 0.0001);
     bool         outRange = false;
     int          iIn      = numPoints;
     double       vIn = 0, dc = 0;
@@ -158,7 +161,7 @@ std::vector<float> ewaldTableFDV0(int numPoints, double tableScaling, float beta
         {
             vi = vIn - dc * (i - iIn) * dx;
         }
-        tv[i] = static_cast<float>(vi);
+        tv[i] = static_cast<Real>(vi);
         if (i == 0)
         {
             continue;
@@ -175,7 +178,7 @@ std::vector<float> ewaldTableFDV0(int numPoints, double tableScaling, float beta
         }
         if (i == numPoints - 1)
         {
-            tf[i] = static_cast<float>(-dc);
+            tf[i] = static_cast<Real>(-dc);
         }
         else
         {
@@ -195,10 +198,10 @@ std::vector<float> ewaldTableFDV0(int numPoints, double tableScaling, float beta
                 dc = dcNew;
             }
         }
-        tf[i - 1] = static_cast<float>(-0.5 * dc);
+        tf[i - 1] = static_cast<Real>(-0.5 * dc);
     }
     tf[0] *= 2;
-    std::vector<float> fdv0(4 * numPoints);
+    std::vector<Real> fdv0(4 * numPoints);
     for (int i = 0; i < numPoints - 1; i++)
     {
         fdv0[4 * i]     = tf[i];
@@ -231,6 +234,9 @@ struct nbx_context
     int              device     = 0;
     hipStream_t      ownStream  = nullptr;
     hipStream_t      stream     = nullptr;
+    // optional HIP events around the pair-kernel launches (benchmarking)
+    std::vector<hipEvent_t> timingEvents; // start, stop pairs
+    int                     timingRecorded = 0;
     int              numCU      = 256;
     int              elec       = kElecEwaldTable;
     InteractionConst ic{};
@@ -242,6 +248,7 @@ struct nbx_context
     int            ntype     = 0; // incl. padding
     DevBuf<float2> nbfp;
     DevBuf<float4> tabFDV0;
+    DevBuf<double> tabFDV0d;
     DevBuf<float4> shiftVec;
 
     // topology
@@ -273,6 +280,8 @@ struct nbx_context
     // outputs
     int             numBlocks = 0;
     DevBuf<double>  energyPartial, energy, fshift, virialPartial, virial;
+    DevBuf<int>     groupCounter;
+    DevBuf<double>  exclPartial;
     DevBuf<float>   fshiftPartial, fOut;
     bool            haveEnergy = false, haveShift = false, haveVirial = false;
 };
@@ -363,14 +372,17 @@ void buildWorkGroups(nbx_context* c)
     check(hipMemcpyAsync(spans, c->groupStats.p, sizeof(spans), hipMemcpyDeviceToHost, c->stream), "read spans");
     check(hipStreamSynchronize(c->stream), "group sync");
     c->jentryTmp.ensure(c->ncj);
-    for (int k = 0; k < 4; k++)
+    // the largest group size whose j-force windows (one per wavefront) fit
+    // the LDS budget of four resident workgroups per CU; failing that, two
+    for (int k = 0; k < 8; k++)
     {
-        const int G = kGroupSizeMax >> k;
-        if (spans[k] > kGroupMaxJ && G > 1)
+        const int G      = kGroupSizeMax >> (k & 3);
+        const int budget = k < 4 ? kKernelLdsBudget : kKernelLdsBudgetMax;
+        if (spans[k & 3] > kGroupMaxJ && G > 1)
         {
             continue;
         }
-        if (spans[k] > kGroupMaxJ)
+        if (spans[k & 3] > kGroupMaxJ)
         {
             throw InputError("a grid cluster has more than " + std::to_string(kGroupMaxJ)
                              + " cluster-pair entries: the pair-list cut-off is too long for this kernel");
@@ -408,15 +420,11 @@ void buildWorkGroups(nbx_context* c)
         check(hipMemcpyAsync(st, c->groupStats.p + 4, sizeof(st), hipMemcpyDeviceToHost, c->stream), "read stats");
         check(hipStreamSynchronize(c->stream), "group sync");
         const int maxUnion = st[0], overflow = st[1], unionTotal = st[2], chunkTotal = st[3], workTotal = st[4];
-        const int slots    = G + maxUnion;
-        if (overflow || (nonbondedLdsBytes(c->ntype, c->ic.tab_n, c->elec, slots) > kKernelLdsBudget && G > 1))
+        // own clusters, the union, and the padding entries' spare slot
+        const int slots = G + maxUnion + 1;
+        if (overflow || nonbondedLdsBytes(c->ntype, c->ic.tab_n, c->elec, slots) > budget)
         {
             continue;
-        }
-        if (nonbondedLdsBytes(c->ntype, c->ic.tab_n, c->elec, slots) > kKernelLdsBudget)
-        {
-            throw InputError("the j-cluster window of one cluster does not fit in LDS: the pair-list cut-off is "
-                             "too long for this kernel");
         }
         c->unionList.ensure(std::max(unionTotal, 1));
         c->chunks.ensure(std::max(chunkTotal, 1));
@@ -439,7 +447,8 @@ void buildWorkGroups(nbx_context* c)
         c->windowSlots = slots;
         return;
     }
-    throw InputError("no work-group size fits the kernel's LDS budget");
+    throw InputError("the j-cluster window of one cluster does not fit in LDS: the pair-list cut-off is too long, or "
+                     "there are too many atom types, for this kernel");
 }
 
 } // namespace
@@ -496,10 +505,6 @@ int nbx_create(const nbx_options* options, nbx_context** context)
         {
             throw InputError("unknown Ewald correction type");
         }
-        if (o.ewald_correction == NBX_EWALD_ANALYTICAL && o.coulomb_type == NBX_COULOMB_EWALD)
-        {
-            throw InputError("the analytical Ewald correction is not available in this build; use NBX_EWALD_TABLE");
-        }
         int ndev = 0;
         check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
         if (o.device < 0 || o.device >= ndev)
@@ -545,16 +550,25 @@ int nbx_create(const nbx_options* options, nbx_context** context)
         }
         if (o.coulomb_type == NBX_COULOMB_EWALD)
         {
-            c->elec      = kElecEwaldTable;
+            c->elec      = o.ewald_correction == NBX_EWALD_ANALYTICAL ? kElecEwaldAna : kElecEwaldTable;
             ic.beta      = ewaldCoeff(c->rcoulomb, static_cast<float>(o.ewald_rtol));
             if (!(ic.beta > 0))
             {
                 throw InputError("Ewald coefficient should be > 0");
             }
+            ic.beta2 = ic.beta * ic.beta;
+            ic.beta3 = ic.beta2 * ic.beta;
+            if (c->elec == kElecEwaldAna && ic.beta * c->rcoulomb > 3.6f)
+            {
+                // the fitted range of the analytical correction
+                throw InputError("the analytical Ewald correction covers beta * rc <= 3.6 (ewald_rtol >= 3e-7); "
+                                 "use the table correction");
+            }
             ic.sh_ewald  = o.ewald_potential_shift ? std::erfc(ic.beta * c->rcoulomb) / c->rcoulomb : 0.0f;
             ic.tab_scale = ewaldTableScale(ic.beta, c->rcoulomb);
             ic.tab_n     = static_cast<int>(c->rcoulomb * ic.tab_scale) + 2;
-            std::vector<float> fdv0 = ewaldTableFDV0(ic.tab_n, ic.tab_scale, ic.beta);
+            std::vector<float>  fdv0  = ewaldTableFDV0<float>(ic.tab_n, ic.tab_scale, ic.beta);
+            std::vector<double> fdv0d = ewaldTableFDV0<double>(ic.tab_n, ic.tab_scale, ic.beta);
             ic.self_v0   = fdv0[2];
             // the self energy sums one large term per atom against a pair sum
             // of nearly the same size: V(0) = 2 beta / sqrt(pi) in double (of
@@ -562,6 +576,9 @@ int nbx_create(const nbx_options* options, nbx_context** context)
             c->selfCoef = 0.5 * static_cast<double>(ic.epsfac) * vEwaldLr(ic.beta, 0.0);
             c->tabFDV0.ensure(ic.tab_n);
             check(hipMemcpy(c->tabFDV0.p, fdv0.data(), sizeof(float) * fdv0.size(), hipMemcpyHostToDevice), "upload table");
+            c->tabFDV0d.ensure(fdv0d.size());
+            check(hipMemcpy(c->tabFDV0d.p, fdv0d.data(), sizeof(double) * fdv0d.size(), hipMemcpyHostToDevice),
+                  "upload table");
         }
         else
         {
@@ -594,8 +611,56 @@ int nbx_destroy(nbx_context* c)
         (void)hipStreamSynchronize(c->ownStream);
         (void)hipStreamDestroy(c->ownStream);
     }
+    for (hipEvent_t e : c->timingEvents)
+    {
+        (void)hipEventDestroy(e);
+    }
     delete c;
     return NBX_SUCCESS;
+}
+
+int nbx_set_kernel_timing(nbx_context* c, int maxLaunches)
+{
+    if (!c || maxLaunches < 0)
+    {
+        return fail(NBX_ERROR_INPUT, "nbx_set_kernel_timing: NULL context or negative count");
+    }
+    return guarded([&] {
+        setDevice(c);
+        check(hipStreamSynchronize(c->stream), "sync");
+        for (hipEvent_t e : c->timingEvents)
+        {
+            check(hipEventDestroy(e), "hipEventDestroy");
+        }
+        c->timingEvents.assign(2 * static_cast<size_t>(maxLaunches), nullptr);
+        for (hipEvent_t& e : c->timingEvents)
+        {
+            check(hipEventCreate(&e), "hipEventCreate");
+        }
+        c->timingRecorded = 0;
+    });
+}
+
+int nbx_get_kernel_timing(nbx_context* c, double* averageMs, int* launches)
+{
+    if (!c || !averageMs || !launches)
+    {
+        return fail(NBX_ERROR_INPUT, "NULL argument");
+    }
+    return guarded([&] {
+        setDevice(c);
+        double total = 0.0;
+        for (int k = 0; k < c->timingRecorded; k++)
+        {
+            check(hipEventSynchronize(c->timingEvents[2 * k + 1]), "hipEventSynchronize");
+            float ms = 0.0f;
+            check(hipEventElapsedTime(&ms, c->timingEvents[2 * k], c->timingEvents[2 * k + 1]), "hipEventElapsedTime");
+            total += ms;
+        }
+        *launches  = c->timingRecorded;
+        *averageMs = c->timingRecorded > 0 ? total / c->timingRecorded : 0.0;
+        c->timingRecorded = 0;
+    });
 }
 
 int nbx_set_stream(nbx_context* c, void* stream)
@@ -901,12 +966,15 @@ int nbx_construct_pairlist(nbx_context* c)
         launchSearch(c->stream, a, true);
         check(hipGetLastError(), "search kernels");
         buildWorkGroups(c);
-        // kernel grid: as many resident work-groups as LDS allows, at most
-        // eight per CU (4 wavefronts each)
+        // kernel grid: the workgroups (4 wavefronts each) that can be resident
+        // at once, at most four per CU; their wavefronts claim groups from a
+        // counter until none are left
         const int lds      = nonbondedLdsBytes(c->ntype, c->ic.tab_n, c->elec, c->windowSlots);
-        const int perCU    = std::max(1, std::min(8, (160 * 1024) / lds));
-        c->numBlocks       = std::max(1, std::min(c->ngroups, perCU * c->numCU));
-        c->energyPartial.ensure(2 * static_cast<size_t>(c->numBlocks));
+        const int perCU    = std::max(1, std::min(4, (160 * 1024) / lds));
+        c->numBlocks       = std::max(1, std::min((c->ngroups + kKernelWaves - 1) / kKernelWaves, perCU * c->numCU));
+        c->energyPartial.ensure(2 * static_cast<size_t>(std::max(c->ngroups, 1)));
+        c->groupCounter.ensure(1);
+        c->exclPartial.ensure(kExclMaxBlocks);
         c->fshiftPartial.ensure(static_cast<size_t>(c->numBlocks) * kNumShifts * 3);
         c->energy.ensure(2);
         c->fshift.ensure(kNumShifts * 3);
@@ -943,7 +1011,7 @@ int nbx_dispatch_kernel(nbx_context* c, unsigned flags)
         setDevice(c);
         const bool energy = (flags & NBX_COMPUTE_ENERGY) != 0;
         const bool shift  = (flags & (NBX_COMPUTE_SHIFT_FORCES | NBX_COMPUTE_VIRIAL)) != 0;
-        launchClearF4(c->stream, c->fnb.p, c->ncluster * kClusterSize);
+        launchClearF4(c->stream, c->fnb.p, c->ncluster * kClusterSize, c->groupCounter.p);
         KernelArgs a{};
         a.chunks        = c->chunks.p;
         a.groups        = c->groups.p;
@@ -960,15 +1028,46 @@ int nbx_dispatch_kernel(nbx_context* c, unsigned flags)
         a.tabFDV0       = c->tabFDV0.p;
         a.shiftVec      = c->shiftVec.p;
         a.ic            = c->ic;
-        a.selfCoef      = c->selfCoef;
+        a.groupCounter  = c->groupCounter.p;
         a.fnb           = c->fnb.p;
         a.energyPartial = c->energyPartial.p;
         a.fshiftPartial = c->fshiftPartial.p;
+        const bool timed = 2 * static_cast<size_t>(c->timingRecorded) < c->timingEvents.size();
+        if (timed)
+        {
+            check(hipEventRecord(c->timingEvents[2 * c->timingRecorded], c->stream), "hipEventRecord");
+        }
         launchNonbonded(c->stream, a, c->elec, energy, shift, c->numBlocks);
+        if (timed)
+        {
+            check(hipEventRecord(c->timingEvents[2 * c->timingRecorded + 1], c->stream), "hipEventRecord");
+            c->timingRecorded++;
+        }
+        int numExcl = 0;
+        if (energy)
+        {
+            ExclusionArgs e{};
+            e.natoms   = c->natoms;
+            e.cells    = c->cells.p;
+            e.xq       = c->xq.p;
+            e.exclOff  = c->exclOff.p;
+            e.exclIdx  = c->exclIdx.p;
+            e.box[0]   = c->g.box[0];
+            e.box[1]   = c->g.box[1];
+            e.box[2]   = c->g.box[2];
+            e.elec     = c->elec;
+            e.ic       = c->ic;
+            e.rc2      = static_cast<double>(c->ic.rcoulomb2);
+            e.epsfacD  = static_cast<double>(c->ic.epsfac);
+            e.selfCoef = c->selfCoef;
+            e.tabD     = c->tabFDV0d.p;
+            e.partial  = c->exclPartial.p;
+            numExcl    = launchExclusionEnergy(c->stream, e);
+        }
         if (energy || shift)
         {
-            launchFinalize(c->stream, c->numBlocks, c->energyPartial.p, c->fshiftPartial.p, shift, c->energy.p,
-                           c->fshift.p);
+            launchFinalize(c->stream, energy ? c->ngroups : 0, c->energyPartial.p, numExcl, c->exclPartial.p,
+                           c->numBlocks, c->fshiftPartial.p, shift, c->energy.p, c->fshift.p);
         }
         check(hipGetLastError(), "nonbonded kernel");
         c->haveEnergy = energy;

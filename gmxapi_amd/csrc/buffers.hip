@@ -12,7 +12,15 @@
 //               atomdata.cpp:1471-1487) summed in double in a fixed order
 //   virial      -1/2 sum x (x) f - 1/2 sum shift (x) fshift in double
 //               (calcvir.cpp:103-148, sim_util.cpp:157-177)
+//   exclusion   the Coulomb energy terms the pair kernel leaves out, in
+//   energies    double: the self energy -1/2 epsfac q_i^2 V(0) of every atom
+//               (kernel_ref_outer.h:286-326) and, for every excluded pair
+//               inside the cut-off, -epsfac q_i q_j V_corr(r) (Ewald) or
+//               epsfac q_i q_j (k_rf r^2 - c_rf) (reaction field)
+//               (kernel_ref_inner.h: excluded pairs keep the correction)
 // All are HBM-bandwidth bound: one coalesced pass over the arrays.
+#include <algorithm>
+
 #include "nbx_internal.h"
 
 namespace nbx
@@ -21,12 +29,18 @@ namespace nbx
 namespace
 {
 
-__global__ void clearF4Kernel(float4* __restrict__ p, int n)
+// also resets the kernel's work-group counter (a fill kernel rather than a
+// memset, so that the step can be captured in a graph)
+__global__ void clearF4Kernel(float4* __restrict__ p, int n, int* __restrict__ counter)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n)
     {
         p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    if (counter && i == 0)
+    {
+        *counter = 0;
     }
 }
 
@@ -65,26 +79,37 @@ __global__ void reduceFKernel(int natoms, const int* __restrict__ cells, const f
 
 constexpr int kFinalizeThreads = 256;
 
-__global__ __launch_bounds__(kFinalizeThreads) void finalizeKernel(int numPartials, const double* __restrict__ ePart,
+__global__ __launch_bounds__(kFinalizeThreads) void finalizeKernel(int numEnergyPartials,
+                                                                   const double* __restrict__ ePart,
+                                                                   int numExclPartials,
+                                                                   const double* __restrict__ exclPart,
+                                                                   int numShiftPartials,
                                                                    const float* __restrict__ fshPart, int doShift,
                                                                    double* __restrict__ eOut,
                                                                    double* __restrict__ fshOut)
 {
-    // block 0: energies, blocks 1..: one shift-force component each
+    // blocks 0, 1: the energies, blocks 2..: one shift-force component each
     __shared__ double s[kFinalizeThreads];
     const int comp = blockIdx.x;
     double    acc  = 0.0;
     if (comp < 2)
     {
-        for (int p = threadIdx.x; p < numPartials; p += kFinalizeThreads)
+        for (int p = threadIdx.x; p < numEnergyPartials; p += kFinalizeThreads)
         {
             acc += ePart[2 * p + comp];
+        }
+        if (comp == 1)
+        {
+            for (int p = threadIdx.x; p < numExclPartials; p += kFinalizeThreads)
+            {
+                acc += exclPart[p];
+            }
         }
     }
     else
     {
         const int k = comp - 2;
-        for (int p = threadIdx.x; p < numPartials; p += kFinalizeThreads)
+        for (int p = threadIdx.x; p < numShiftPartials; p += kFinalizeThreads)
         {
             acc += static_cast<double>(fshPart[p * kNumShifts * 3 + k]);
         }
@@ -111,6 +136,86 @@ __global__ __launch_bounds__(kFinalizeThreads) void finalizeKernel(int numPartia
         }
     }
     (void)doShift;
+}
+
+constexpr int kExclThreads = 256;
+
+// V_corr(r) = erf(beta r)/r in double: the double build's table (the pair
+// kernel's interpolation formula) or the closed form
+__device__ double ewaldCorrectionD(double r, const ExclusionArgs& a)
+{
+    if (a.elec == kElecEwaldTable)
+    {
+        const double rs    = r * static_cast<double>(a.ic.tab_scale);
+        const int    ri    = min(static_cast<int>(rs), a.ic.tab_n - 2);
+        const double frac  = rs - static_cast<double>(ri);
+        const double f0    = a.tabD[4 * ri];
+        const double fexcl = f0 + frac * a.tabD[4 * ri + 1];
+        return a.tabD[4 * ri + 2] - (0.5 / static_cast<double>(a.ic.tab_scale)) * frac * (f0 + fexcl);
+    }
+    return erf(static_cast<double>(a.ic.beta) * r) / r;
+}
+
+__global__ __launch_bounds__(kExclThreads) void exclusionEnergyKernel(const ExclusionArgs a)
+{
+    __shared__ double s[kExclThreads / 64];
+    double            e  = 0.0;
+    const double      L[3] = { a.box[0], a.box[1], a.box[2] };
+    for (int i = blockIdx.x * kExclThreads + threadIdx.x; i < a.natoms; i += gridDim.x * kExclThreads)
+    {
+        const float4 pi = a.xq[a.cells[i]];
+        const double qi = pi.w;
+        e -= a.selfCoef * qi * qi;
+        for (int k = a.exclOff[i]; k < a.exclOff[i + 1]; k++)
+        {
+            const int j = a.exclIdx[k];
+            if (j <= i)
+            {
+                continue;
+            }
+            const float4 pj = a.xq[a.cells[j]];
+            double       d[3] = { static_cast<double>(pi.x) - pj.x, static_cast<double>(pi.y) - pj.y,
+                            static_cast<double>(pi.z) - pj.z };
+            double       r2   = 0.0;
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+            {
+                d[c] -= L[c] * rint(d[c] / L[c]);
+                r2 += d[c] * d[c];
+            }
+            if (r2 < a.rc2)
+            {
+                const double qq = a.epsfacD * qi * static_cast<double>(pj.w);
+                if (a.elec == kElecRF)
+                {
+                    e += qq * (static_cast<double>(a.ic.k_rf) * r2 - static_cast<double>(a.ic.c_rf));
+                }
+                else
+                {
+                    e -= qq * ewaldCorrectionD(sqrt(r2), a);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 1; m < 64; m *= 2)
+    {
+        e += __shfl_xor(e, m, 64);
+    }
+    if ((threadIdx.x & 63) == 0)
+    {
+        s[threadIdx.x / 64] = e;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        double t = 0.0;
+        for (int w = 0; w < kExclThreads / 64; w++)
+        {
+            t += s[w];
+        }
+        a.partial[blockIdx.x] = t;
+    }
 }
 
 constexpr int kVirThreads = 256;
@@ -294,11 +399,11 @@ __global__ void expandPairsKernel(const int4* __restrict__ ientry, int nci, cons
 
 } // namespace
 
-void launchClearF4(hipStream_t s, float4* p, int n)
+void launchClearF4(hipStream_t s, float4* p, int n, int* counter)
 {
     if (n > 0)
     {
-        clearF4Kernel<<<(n + 255) / 256, 256, 0, s>>>(p, n);
+        clearF4Kernel<<<(n + 255) / 256, 256, 0, s>>>(p, n, counter);
     }
 }
 
@@ -318,12 +423,21 @@ void launchReduceF(hipStream_t s, int natoms, const int* cells, const float4* fn
     }
 }
 
-void launchFinalize(hipStream_t s, int numPartials, const double* energyPartial, const float* fshiftPartial,
-                    bool shiftForces, double* energyOut, double* fshiftOut)
+void launchFinalize(hipStream_t s, int numEnergyPartials, const double* energyPartial, int numExclPartials,
+                    const double* exclPartial, int numShiftPartials, const float* fshiftPartial, bool shiftForces,
+                    double* energyOut, double* fshiftOut)
 {
     const int blocks = shiftForces ? 2 + kNumShifts * 3 : 2;
-    finalizeKernel<<<blocks, kFinalizeThreads, 0, s>>>(numPartials, energyPartial, fshiftPartial, shiftForces ? 1 : 0,
-                                                       energyOut, fshiftOut);
+    finalizeKernel<<<blocks, kFinalizeThreads, 0, s>>>(numEnergyPartials, energyPartial, numExclPartials, exclPartial,
+                                                       numShiftPartials, fshiftPartial, shiftForces ? 1 : 0, energyOut,
+                                                       fshiftOut);
+}
+
+int launchExclusionEnergy(hipStream_t s, const ExclusionArgs& a)
+{
+    const int blocks = std::max(1, std::min(kExclMaxBlocks, (a.natoms + kExclThreads - 1) / kExclThreads));
+    exclusionEnergyKernel<<<blocks, kExclThreads, 0, s>>>(a);
+    return blocks;
 }
 
 void launchVirial(hipStream_t s, int natoms, const float* x, const float* f, const double* fshift,

@@ -13,8 +13,10 @@
 //   * the work list: every i-entry's j-entries as {cj, slot << 16 | mask}
 //     (slot = the j-cluster's window slot; the exclusion mask uses bits
 //     0..15, bit 4*i + j), padded to a whole number of kTileJ-entry tiles
-//     with a dummy entry (cluster ncluster, far-away fillers, own slot), so
-//     the kernel reads whole tiles without bounds checks,
+//     with a dummy entry (cluster ncluster, far-away fillers) whose slot is
+//     the group's spare slot G + U, which the kernel never flushes, so the
+//     kernel reads whole tiles without bounds checks and no two lanes of a
+//     tile share a live slot,
 //   * work chunks: i-entries cut into pieces of at most kChunkEntries
 //     j-entries, {ci, shift | self << 8 | special << 9, work start, tiles};
 //     self marks the piece that carries the cluster's Ewald/RF self energy
@@ -231,13 +233,13 @@ __global__ __launch_bounds__(kGroupThreads) void buildGroupsKernel(GroupArgs a)
                 nmasked = written;
             }
         }
-        // pad to whole tiles with the dummy cluster, in the entry's own slot
+        // pad to whole tiles with the dummy cluster, in the spare slot
         const int n    = je - js;
         const int npad = (n + kTileJ - 1) / kTileJ * kTileJ;
         if (n + lane < npad)
         {
             a.work[wb + n + lane] = make_uint2(static_cast<unsigned>(a.ncluster),
-                                               (static_cast<unsigned>(ci - c0) << 16) | 0xffffu);
+                                               (static_cast<unsigned>(a.groupSize + U) << 16) | 0xffffu);
         }
         if (lane == 0)
         {

@@ -9,32 +9,46 @@
 //   with potential-shift energies; excluded pairs inside the cut-off keep the
 //   Ewald (or RF) correction; the self term -1/2 q_i^2 epsfac V(0) is added
 //   once per central i-entry; shift forces are the sum of i-forces per shift.
+// The Ewald correction is either the reference's quadratic-spline table
+// (kElecEwaldTable, what the plain-C kernel uses) or an analytical form
+// (kElecEwaldAna, as the reference's SIMD kernels use: the force and
+// potential corrections of simd_math.h:1609-1720) evaluated with this
+// project's own rational fits of the closed forms (fit_ewald_correction.py).
 //
 // Mapping to the hardware.
-//   * A workgroup of 4 wavefronts owns a group of G consecutive i-clusters
-//     (z-neighbours in one grid column) and the union of their j-clusters.
-//     Every j-force goes into an LDS window (one float3 per atom per
-//     distinct j-cluster); the window is flushed with one global atomic per
-//     atom component when the group is done, which cuts global atomics to
-//     about a third of a per-tile scatter.
-//   * Wavefronts take work chunks of the group (pieces of i-entries) from an
-//     LDS counter.  In a chunk the four i-atoms are wave-uniform (scalar
-//     loads) and each lane owns one j-atom: lane = j + 4*k covers 16
-//     j-clusters, and loops over the four i-atoms, so one instruction does
-//     64 pairs and a tile of 16 cluster pairs is 256 pairs.  The lane keeps
-//     its j-force in registers for the tile (no cross-lane reduction) and
-//     adds it to the window with three LDS atomics; the i-forces stay in
-//     registers for the chunk and are reduce-scattered across the wave once.
-//   * j-entries are loaded two tiles ahead, j-atom data one tile ahead.
-//   * Masked j-entries come first in every i-entry, so a chunk's leading
-//     tiles run a loop with exclusion and diagonal checks and the remaining
-//     tiles a loop with none; lanes past the end of a list read a dummy
-//     cluster of far-away fillers instead of carrying a validity mask.
-//   * LJ parameters and the Ewald F/D/V table are in LDS; energies are
-//     summed per lane in float over a chunk, then in double; per-workgroup
-//     double partials are summed by a separate kernel in a fixed order.
-// Pair arithmetic is FP32 on the vector ALUs; no matrix instructions are
-// used (the pair interaction is not a dense contraction).
+//   * Every wavefront works alone on a group of G consecutive i-clusters
+//     (z-neighbours in one grid column, which share most j-clusters), claimed
+//     from a global counter.  It owns a private LDS window with one float3
+//     per atom of each distinct j-cluster of the group and of its own G
+//     clusters: j-forces are added to it with plain read-modify-write (no
+//     atomics, no barriers), and the window is flushed to the cluster force
+//     buffer with one global atomic per nonzero atom component when the group
+//     is done.
+//   * In a tile the four i-atoms are wave-uniform (scalar registers) and lane
+//     l owns j-atom l & 3 of j-entry l >> 2: 16 j-clusters per tile, four
+//     i-atoms per lane, so one VALU instruction evaluates 64 pairs.  The
+//     periodic shift is applied to the j-atom (x_i + s - x_j = x_i - (x_j -
+//     s)), so the i-data never leave the scalar unit.  A lane keeps its
+//     j-force for the tile in registers; i-forces stay in 12 registers for an
+//     i-entry and are reduce-scattered across the wave once per entry.
+//   * A group's tiles are one contiguous run of the work list (each i-entry
+//     padded to 16 entries with a far-away dummy cluster), walked by one
+//     software pipeline across i-entry boundaries: work entries are loaded
+//     two tiles ahead and j-atom data one tile ahead, in a loop unrolled by
+//     two so that no loaded value is copied between registers.
+//   * Masked entries come first in every i-entry: its leading tiles run with
+//     exclusion and diagonal checks, the rest without.
+//   * LJ parameters, shift vectors, the Ewald table (table flavour) and each
+//     group's i-atoms are staged in LDS.
+//   * Energies: float per lane over the four pairs of a tile, then double;
+//     each group's sum goes to its own slot and the finalize kernel sums them
+//     in group order, so energies are bitwise reproducible.  The self energy
+//     and the energies of excluded pairs inside the cut-off (large terms that
+//     nearly cancel) are left to a separate double-precision pass over the
+//     exclusion lists (buffers.hip); here excluded pairs contribute forces
+//     only.
+// Pair arithmetic is FP32 on the vector ALUs; no matrix instructions are used
+// (the pair interaction is not a dense contraction).
 #include "nbx_internal.h"
 
 namespace nbx
@@ -44,7 +58,7 @@ namespace
 {
 
 constexpr int kWave       = 64;
-constexpr int kWavesBlock = 4;
+constexpr int kWavesBlock = kKernelWaves;
 constexpr int kThreads    = kWave * kWavesBlock;
 
 // 1/12 and 1/6 split into a float and its float remainder
@@ -52,6 +66,21 @@ constexpr float kTwelfthHi = static_cast<float>(1.0 / 12.0);
 constexpr float kTwelfthLo = static_cast<float>(1.0 / 12.0 - static_cast<double>(static_cast<float>(1.0 / 12.0)));
 constexpr float kSixthHi   = static_cast<float>(1.0 / 6.0);
 constexpr float kSixthLo   = static_cast<float>(1.0 / 6.0 - static_cast<double>(static_cast<float>(1.0 / 6.0)));
+
+// Analytical Ewald correction with s = (beta r)^2, fitted by
+// fit_ewald_correction.py (relative error below 1e-7 in exact arithmetic for
+// s <= 12.96, about 4e-7 in float evaluation):
+//   force      F_corr(r) / r = beta^3 P_g(s) / Q_g(s),
+//              closed form beta^3 (erf z - 2 z exp(-z^2) / sqrt(pi)) / z^3
+//   potential  V_corr(r)     = beta P_h(s) / Q_h(s), closed form beta erf(z) / z
+constexpr float kGp0 = 0.7522528767585754f, kGp1 = -0.01339578628540039f, kGp2 = 0.016685180366039276f,
+                kGp3 = 0.0003194923046976328f, kGp4 = 3.5602472053142264e-05f, kGp5 = -3.4331239362472843e-07f;
+constexpr float kGq1 = 0.5821957588195801f, kGq2 = 0.15719348192214966f, kGq3 = 0.025581449270248413f,
+                kGq4 = 0.002643152140080929f, kGq5 = 0.0001805289793992415f;
+constexpr float kHp0 = 1.128378987312317f, kHp1 = 0.2956092357635498f, kHp2 = 0.0626194030046463f,
+                kHp3 = 0.0073672509752213955f, kHp4 = 0.00019465858349576592f, kHp5 = -1.2356190381979104e-06f;
+constexpr float kHq1 = 0.595305323600769f, kHq2 = 0.15395577251911163f, kHq3 = 0.022073965519666672f,
+                kHq4 = 0.0017358317272737622f;
 
 // 32-bit byte offset from a wave-uniform base: lets the compiler use the
 // scalar-base + vector-offset form of global loads
@@ -67,29 +96,38 @@ __device__ __forceinline__ float uniformFloat(float v)
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
-// Per chunk, wave-uniform (scalar registers): the four i-atoms as stored
-// and their LJ table row offsets.  The periodic shift is applied to the
-// j-atom instead (x_i + s - x_j = x_i - (x_j - s)), once per tile, so the
-// i-data never has to leave the scalar unit.
+__device__ __forceinline__ int uniformInt(int v)
+{
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+// The four i-atoms of the current i-entry, wave-uniform (scalar registers):
+// coordinates and charge as stored, and LJ table row offsets.
 struct IAtoms
 {
     float x[4], y[4], z[4], q[4];
     int   tOff[4];
 };
 
+// This lane's j-atom for one tile: coordinates minus the shift, w = epsfac
+// q_j, and the LJ type.
+struct JAtom
+{
+    float x, y, z, w;
+    int   type;
+};
+
 // One j-atom (this lane) against i-atom kI.  kChecks adds the exclusion
 // bit and the central diagonal.
 template<int kElec, bool kEnergy, bool kChecks, int kI>
-__device__ __forceinline__ void pairI(const IAtoms& I, const float4 xqj, const int tj, const unsigned mbits,
-                                      const bool diag, const int jj, const InteractionConst& ic,
-                                      const float2* __restrict__ sNbfp, const float4* __restrict__ sTab,
-                                      const int tabN, float* fi, float& fjx, float& fjy, float& fjz, float& vlj,
-                                      float& vc)
+__device__ __forceinline__ void pairI(const IAtoms& I, const JAtom& J, const unsigned mbits, const bool diag,
+                                      const int jj, const InteractionConst& ic, const float2* __restrict__ sNbfp,
+                                      const float4* __restrict__ sTab, float* fi, float& fjx, float& fjy,
+                                      float& fjz, float& vlj, float& vc)
 {
-    // xqj: the j-atom minus the shift, w = epsfac * q_j
-    const float dx       = I.x[kI] - xqj.x;
-    const float dy       = I.y[kI] - xqj.y;
-    const float dz       = I.z[kI] - xqj.z;
+    const float dx       = I.x[kI] - J.x;
+    const float dy       = I.y[kI] - J.y;
+    const float dz       = I.z[kI] - J.z;
     float       rsq      = dx * dx + dy * dy + dz * dz;
     float       interact = 1.0f;
     bool        skip;
@@ -114,7 +152,7 @@ __device__ __forceinline__ void pairI(const IAtoms& I, const float4 xqj, const i
     rinv               = skip ? 0.0f : rinv;
     const float rinvsq = rinv * rinv;
 
-    const float2 c      = sNbfp[I.tOff[kI] + tj];
+    const float2 c      = sNbfp[I.tOff[kI] + J.type];
     const float  rinv6  = kChecks ? interact * rinvsq * rinvsq * rinvsq : rinvsq * rinvsq * rinvsq;
     const float  FrLJ6  = c.x * rinv6;
     const float  FrLJ12 = c.y * rinv6 * rinv6;
@@ -129,33 +167,54 @@ __device__ __forceinline__ void pairI(const IAtoms& I, const float4 xqj, const i
         vlj += kChecks ? (v12 - v6) * interact * smask : (v12 - v6) * smask;
     }
 
-    // beyond the cut-off rinv = 0 zeroes the Ewald force; energies and the
-    // reaction-field constant term need qq = 0 there as well
-    const float qq = (kEnergy || kElec == kElecRF) ? smask * I.q[kI] * xqj.w : I.q[kI] * xqj.w;
+    // beyond the cut-off rinv = 0 zeroes the table's Ewald force; energies,
+    // the reaction-field constant and the analytical correction need qq = 0
+    // there as well
+    const bool  maskQ = kEnergy || kElec != kElecEwaldTable;
+    const float qq    = maskQ ? smask * I.q[kI] * J.w : I.q[kI] * J.w;
     float       fcoul;
+    float       vcorr = 0.0f;
     if (kElec == kElecRF)
     {
         fcoul = qq * (interact * rinv * rinvsq - 2.0f * ic.k_rf);
         if (kEnergy)
         {
-            vc += qq * (interact * rinv + ic.k_rf * rsq - ic.c_rf);
+            // excluded pairs (interact = 0) are left to the exclusion pass
+            vc += interact * qq * (rinv + ic.k_rf * rsq - ic.c_rf);
+        }
+    }
+    else if (kElec == kElecEwaldTable)
+    {
+        const float  rs    = rsq * rinv * ic.tab_scale;
+        // clamped so that no input (NaN, Inf) can index past the table
+        const int    ri    = min(static_cast<int>(rs), ic.tab_n - 2);
+        const float  frac  = rs - static_cast<float>(ri);
+        const float4 tab   = sTab[ri];
+        const float  fexcl = tab.x + frac * tab.y;
+        fcoul              = (interact * rinvsq - fexcl) * (qq * rinv);
+        if (kEnergy)
+        {
+            const float halfsp = 0.5f / ic.tab_scale;
+            vcorr              = tab.z - halfsp * frac * (tab.x + fexcl);
         }
     }
     else
     {
-        const float  rs    = rsq * rinv * ic.tab_scale;
-        // clamped so that no input (NaN, Inf) can index past the table
-        const int    ri    = min(static_cast<int>(rs), tabN - 2);
-        const float  frac  = rs - static_cast<float>(ri);
-        const float4 tab   = sTab[ri];
-        const float  fexcl = tab.x + frac * tab.y;
-        fcoul              = interact * rinvsq - fexcl;
+        const float s  = rsq * ic.beta2;
+        const float pg = fmaf(fmaf(fmaf(fmaf(fmaf(kGp5, s, kGp4), s, kGp3), s, kGp2), s, kGp1), s, kGp0);
+        const float qg = fmaf(fmaf(fmaf(fmaf(fmaf(kGq5, s, kGq4), s, kGq3), s, kGq2), s, kGq1), s, 1.0f);
+        fcoul          = qq * (interact * rinv * rinvsq - ic.beta3 * pg * __builtin_amdgcn_rcpf(qg));
         if (kEnergy)
         {
-            const float halfsp = 0.5f / ic.tab_scale;
-            vc += qq * (interact * (rinv - ic.sh_ewald) - (tab.z - halfsp * frac * (tab.x + fexcl)));
+            const float ph = fmaf(fmaf(fmaf(fmaf(fmaf(kHp5, s, kHp4), s, kHp3), s, kHp2), s, kHp1), s, kHp0);
+            const float qh = fmaf(fmaf(fmaf(fmaf(kHq4, s, kHq3), s, kHq2), s, kHq1), s, 1.0f);
+            vcorr          = ic.beta * ph * __builtin_amdgcn_rcpf(qh);
         }
-        fcoul *= qq * rinv;
+    }
+    if (kEnergy && kElec != kElecRF)
+    {
+        // excluded pairs (interact = 0) are left to the exclusion pass
+        vc += kChecks ? interact * qq * ((rinv - ic.sh_ewald) - vcorr) : qq * ((rinv - ic.sh_ewald) - vcorr);
     }
 
     const float fscal = frLJ * rinvsq + fcoul;
@@ -171,149 +230,121 @@ __device__ __forceinline__ void pairI(const IAtoms& I, const float4 xqj, const i
 }
 
 template<int kElec, bool kEnergy, bool kChecks>
-__device__ __forceinline__ void tile4(const IAtoms& I, const float4 xqj, const int tj, const unsigned mbits,
-                                      const bool diag, const int jj, const InteractionConst& ic,
-                                      const float2* __restrict__ sNbfp, const float4* __restrict__ sTab,
-                                      const int tabN, float* fi, float& fjx, float& fjy, float& fjz, float& vlj,
-                                      float& vc)
+__device__ __forceinline__ void tile4(const IAtoms& I, const JAtom& J, const unsigned mbits, const bool diag,
+                                      const int jj, const InteractionConst& ic, const float2* __restrict__ sNbfp,
+                                      const float4* __restrict__ sTab, float* fi, float& fjx, float& fjy,
+                                      float& fjz, float& vlj, float& vc)
 {
     // barriers between the i-atoms keep the scheduler from interleaving all
     // four pair evaluations, which costs registers (occupancy) for little overlap
-    pairI<kElec, kEnergy, kChecks, 0>(I, xqj, tj, mbits, diag, jj, ic, sNbfp, sTab, tabN, fi, fjx, fjy, fjz, vlj, vc);
+    pairI<kElec, kEnergy, kChecks, 0>(I, J, mbits, diag, jj, ic, sNbfp, sTab, fi, fjx, fjy, fjz, vlj, vc);
     __builtin_amdgcn_sched_barrier(0);
-    pairI<kElec, kEnergy, kChecks, 1>(I, xqj, tj, mbits, diag, jj, ic, sNbfp, sTab, tabN, fi, fjx, fjy, fjz, vlj, vc);
+    pairI<kElec, kEnergy, kChecks, 1>(I, J, mbits, diag, jj, ic, sNbfp, sTab, fi, fjx, fjy, fjz, vlj, vc);
     __builtin_amdgcn_sched_barrier(0);
-    pairI<kElec, kEnergy, kChecks, 2>(I, xqj, tj, mbits, diag, jj, ic, sNbfp, sTab, tabN, fi, fjx, fjy, fjz, vlj, vc);
+    pairI<kElec, kEnergy, kChecks, 2>(I, J, mbits, diag, jj, ic, sNbfp, sTab, fi, fjx, fjy, fjz, vlj, vc);
     __builtin_amdgcn_sched_barrier(0);
-    pairI<kElec, kEnergy, kChecks, 3>(I, xqj, tj, mbits, diag, jj, ic, sNbfp, sTab, tabN, fi, fjx, fjy, fjz, vlj, vc);
+    pairI<kElec, kEnergy, kChecks, 3>(I, J, mbits, diag, jj, ic, sNbfp, sTab, fi, fjx, fjy, fjz, vlj, vc);
 }
 
-// One tile: 16 j-entries x 4 i-atoms, this lane's j-atom against the four
-// i-atoms.  The j-force goes to the entry's LDS window slot; energies are
-// summed in float over the tile's four pairs, then in double.
-template<int kElec, bool kEnergy>
-__device__ __forceinline__ void tile(const IAtoms& I, const uint2 e, const float4 xj, const int tj, const float4 sv,
-                                     const bool checks, const bool central, const int ci, const int jj,
-                                     const InteractionConst& ic, const float2* __restrict__ sNbfp,
-                                     const float4* __restrict__ sTab, const int tabN, float* __restrict__ sWin,
-                                     float* fi, double& eVdw, double& eCoul)
+// One butterfly step: lanes with the mask bit set keep b and send a, the
+// others keep a and send b; each adds what its partner sent.
+__device__ __forceinline__ float butterfly(float a, float b, bool hi, int m)
 {
-    // the j-atom minus the shift, w = epsfac * q_j
-    const float4 xqj = make_float4(xj.x - sv.x, xj.y - sv.y, xj.z - sv.z, ic.epsfac * xj.w);
-    float        fjx = 0.0f, fjy = 0.0f, fjz = 0.0f, vlj = 0.0f, vc = 0.0f;
-    if (checks)
-    {
-        const bool diag = central && static_cast<int>(e.x) == ci;
-        tile4<kElec, kEnergy, true>(I, xqj, tj, (e.y & 0xffffu) >> jj, diag, jj, ic, sNbfp, sTab, tabN, fi, fjx, fjy,
-                                    fjz, vlj, vc);
-    }
-    else
-    {
-        tile4<kElec, kEnergy, false>(I, xqj, tj, 0u, false, jj, ic, sNbfp, sTab, tabN, fi, fjx, fjy, fjz, vlj, vc);
-    }
-    float* w = sWin + (e.y >> 16) * 12 + jj * 3;
-    atomicAdd(w + 0, fjx);
-    atomicAdd(w + 1, fjy);
-    atomicAdd(w + 2, fjz);
-    if (kEnergy)
-    {
-        eVdw += static_cast<double>(vlj);
-        eCoul += static_cast<double>(vc);
-    }
+    const float keep = hi ? b : a;
+    const float send = hi ? a : b;
+    return keep + __shfl_xor(send, m, kWave);
 }
 
-// The tiles of one chunk.  Work entries are read two tiles ahead and j-atom
-// data one tile ahead, in a loop unrolled by two so that no loaded value is
-// copied between registers (a copy would wait for its load).  Chunks are
-// padded to whole tiles with a dummy far-away cluster, and reads past the
-// last tile are clamped to it, so no lane needs a bounds check.
-template<int kElec, bool kEnergy>
-__device__ __forceinline__ void chunkTiles(const KernelArgs& a, const IAtoms& I, const float4 sv, const int ci,
-                                           const bool central, const unsigned ws, const int nt, const int nspecial,
-                                           const int jj, const int kk, const InteractionConst& ic,
-                                           const float2* __restrict__ sNbfp, const float4* __restrict__ sTab,
-                                           const int tabN, float* __restrict__ sWin, float* fi, double& eVdw,
-                                           double& eCoul)
+// Reduce-scatter of the 12 i-force components over the wavefront: afterwards
+// lane L holds the wave sum of component L >> 2 (lanes 48..63 hold zeros).
+// Written on named scalars: with an indexed array the compiler folds the
+// lane-dependent selects into a dynamically indexed register array and
+// expands that into long compare/select chains.
+__device__ __forceinline__ float waveReduceScatter12(const float* f, int lane)
 {
-    auto entryAt = [&](int t) { return ldg(a.work, ws + static_cast<unsigned>(kTileJ * min(t, nt - 1) + kk)); };
-    uint2  eA = entryAt(0);
-    uint2  eB = entryAt(1);
-    float4 xA = ldg(a.xq, eA.x * kClusterSize + jj);
-    int    tA = ldg(a.type, eA.x * kClusterSize + jj);
-    for (int t = 0; t < nt; t += 2)
-    {
-        const float4 xB  = ldg(a.xq, eB.x * kClusterSize + jj);
-        const int    tB  = ldg(a.type, eB.x * kClusterSize + jj);
-        const uint2  eA2 = entryAt(t + 2);
-        tile<kElec, kEnergy>(I, eA, xA, tA, sv, t < nspecial, central, ci, jj, ic, sNbfp, sTab, tabN, sWin, fi, eVdw,
-                             eCoul);
-        if (t + 1 >= nt)
-        {
-            break;
-        }
-        const uint2 eB2 = entryAt(t + 3);
-        xA              = ldg(a.xq, eA2.x * kClusterSize + jj);
-        tA              = ldg(a.type, eA2.x * kClusterSize + jj);
-        tile<kElec, kEnergy>(I, eB, xB, tB, sv, t + 1 < nspecial, central, ci, jj, ic, sNbfp, sTab, tabN, sWin, fi,
-                             eVdw, eCoul);
-        eA = eA2;
-        eB = eB2;
-    }
-}
-
-// Reduce-scatter of 16 per-lane values over the wavefront: afterwards lane L
-// holds the wave sum of value (L >> 2).
-__device__ __forceinline__ float waveReduceScatter16(float* v, int lane)
-{
-#pragma unroll
-    for (int step = 0; step < 4; step++)
-    {
-        const int  m    = 32 >> step;
-        const int  half = 8 >> step;
-        const bool hi   = (lane & m) != 0;
-#pragma unroll
-        for (int k = 0; k < half; k++)
-        {
-            const float keep = hi ? v[k + half] : v[k];
-            const float send = hi ? v[k] : v[k + half];
-            v[k]             = keep + __shfl_xor(send, m, kWave);
-        }
-    }
-    float r = v[0];
+    const bool  h32 = (lane & 32) != 0;
+    const bool  h16 = (lane & 16) != 0;
+    const bool  h8  = (lane & 8) != 0;
+    const bool  h4  = (lane & 4) != 0;
+    // component k + 8 for k >= 4 is zero
+    const float a0  = butterfly(f[0], f[8], h32, 32);
+    const float a1  = butterfly(f[1], f[9], h32, 32);
+    const float a2  = butterfly(f[2], f[10], h32, 32);
+    const float a3  = butterfly(f[3], f[11], h32, 32);
+    const float a4  = butterfly(f[4], 0.0f, h32, 32);
+    const float a5  = butterfly(f[5], 0.0f, h32, 32);
+    const float a6  = butterfly(f[6], 0.0f, h32, 32);
+    const float a7  = butterfly(f[7], 0.0f, h32, 32);
+    const float b0  = butterfly(a0, a4, h16, 16);
+    const float b1  = butterfly(a1, a5, h16, 16);
+    const float b2  = butterfly(a2, a6, h16, 16);
+    const float b3  = butterfly(a3, a7, h16, 16);
+    const float c0  = butterfly(b0, b2, h8, 8);
+    const float c1  = butterfly(b1, b3, h8, 8);
+    float       r   = butterfly(c0, c1, h4, 4);
     r += __shfl_xor(r, 2, kWave);
     r += __shfl_xor(r, 1, kWave);
     return r;
 }
 
+__device__ __forceinline__ double waveSumD(double v)
+{
+#pragma unroll
+    for (int m = 1; m < kWave; m *= 2)
+    {
+        v += __shfl_xor(v, m, kWave);
+    }
+    return v;
+}
+
+// LDS layout of a workgroup, in float4 units
+struct LdsLayout
+{
+    int nbfp, tab, shift, fsh, ix, it, win;
+    __host__ __device__ LdsLayout(int ntype, int tabN, int windowSlots)
+    {
+        nbfp  = 0;
+        tab   = nbfp + (ntype * ntype + 1) / 2;
+        shift = tab + tabN;
+        fsh   = shift + kNumShifts;
+        ix    = fsh + (kNumShifts * 3 + 3) / 4;
+        it    = ix + kWavesBlock * kGroupSizeMax * kClusterSize;
+        win   = it + kWavesBlock * kGroupSizeMax;
+        end_  = win + kWavesBlock * windowSlots * 3;
+    }
+    int end_;
+};
+
 template<int kElec, bool kEnergy, bool kShift>
 __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
 {
     extern __shared__ float4 smem4[];
-    __shared__ int sNext;
-    const int ntype  = a.ntype;
-    const int ntype2 = ntype * ntype;
-    float2*   sNbfp  = reinterpret_cast<float2*>(smem4);
-    float4*   sTab   = smem4 + ((ntype2 + 1) / 2);
-    const int tabN   = (kElec == kElecEwaldTable) ? a.ic.tab_n : 0;
-    float*    sWin   = reinterpret_cast<float*>(sTab + tabN);
-    float*    sFsh   = sWin + a.windowSlots * 12;
-    double*   sEner  = reinterpret_cast<double*>(reinterpret_cast<float4*>(sFsh) + (kNumShifts * 3 + 4) / 4);
+    const int       ntype  = a.ntype;
+    const int       ntype2 = ntype * ntype;
+    const int       tabN   = (kElec == kElecEwaldTable) ? a.ic.tab_n : 0;
+    const LdsLayout L(ntype, tabN, a.windowSlots);
+    float2*         sNbfp  = reinterpret_cast<float2*>(smem4 + L.nbfp);
+    float4*         sTab   = smem4 + L.tab;
+    float4*         sShift = smem4 + L.shift;
+    float*          sFsh   = reinterpret_cast<float*>(smem4 + L.fsh);
 
     for (int t = threadIdx.x; t < ntype2; t += kThreads)
     {
         sNbfp[t] = a.nbfp[t];
     }
-    if (kElec == kElecEwaldTable)
+    for (int t = threadIdx.x; t < tabN; t += kThreads)
     {
-        for (int t = threadIdx.x; t < tabN; t += kThreads)
-        {
-            sTab[t] = a.tabFDV0[t];
-        }
+        sTab[t] = a.tabFDV0[t];
+    }
+    for (int t = threadIdx.x; t < kNumShifts; t += kThreads)
+    {
+        sShift[t] = a.shiftVec[t];
     }
     for (int t = threadIdx.x; t < kNumShifts * 3; t += kThreads)
     {
         sFsh[t] = 0.0f;
     }
+    __syncthreads();
 
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
@@ -322,91 +353,91 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
 
     const InteractionConst ic = a.ic;
     const int              G  = a.groupSize;
+    // this wavefront's i-atom cache and j-force window
+    float4* ix  = smem4 + L.ix + wave * kGroupSizeMax * kClusterSize;
+    int*    it  = reinterpret_cast<int*>(smem4 + L.it) + wave * kGroupSizeMax * kClusterSize;
+    float*  win = reinterpret_cast<float*>(smem4 + L.win) + wave * a.windowSlots * 12;
 
-    double eVdw = 0.0, eCoul = 0.0;
-
-    for (int g = blockIdx.x; g < a.ngroups; g += gridDim.x)
+    for (;;)
     {
-        const int4 grp    = a.groups[g]; // chunk range, union offset, union size
-        const int  c0     = g * G;
-        const int  nslots = G + grp.w;
-        for (int t = threadIdx.x; t < nslots * 12; t += kThreads)
+        int g = 0;
+        if (lane == 0)
         {
-            sWin[t] = 0.0f;
+            g = atomicAdd(a.groupCounter, 1);
         }
-        if (threadIdx.x == 0)
+        g = uniformInt(__shfl(g, 0, kWave));
+        if (g >= a.ngroups)
         {
-            sNext = grp.x;
+            break;
         }
-        __syncthreads();
+        const int4 grp  = a.groups[g]; // chunk range, union offset, union size
+        const int  ch0  = uniformInt(grp.x);
+        const int  ch1  = uniformInt(grp.y);
+        const int  uoff = uniformInt(grp.z);
+        const int  U    = uniformInt(grp.w);
+        const int  c0   = g * G;
+        const int  nOwn = min(G, a.ncluster - c0);
 
-        for (;;)
+        // clear the window (own slots, union slots, the padding entries'
+        // spare slot) and cache the group's i-atoms; this wavefront's LDS
+        // accesses complete in order, so no barrier is needed
+        for (int t = lane; t < (G + U + 1) * 3; t += kWave)
         {
-            int c = 0;
-            if (lane == 0)
-            {
-                c = atomicAdd(&sNext, 1);
-            }
-            c = __builtin_amdgcn_readfirstlane(__shfl(c, 0, kWave));
-            if (c >= grp.y)
-            {
-                break;
-            }
-            const int4 ch      = a.chunks[c];
-            const int  ci      = __builtin_amdgcn_readfirstlane(ch.x);
-            const int  flags   = __builtin_amdgcn_readfirstlane(ch.y);
-            const int  sh      = flags & 0xff;
-            const bool self    = ((flags >> 8) & 1) != 0;
-            const int  nt      = __builtin_amdgcn_readfirstlane(ch.w);
-            const int  nspec   = flags >> 9;
-            const bool central = (sh == kCentral);
+            reinterpret_cast<float4*>(win)[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        if (lane < nOwn * kClusterSize)
+        {
+            ix[lane] = a.xq[c0 * kClusterSize + lane];
+            it[lane] = a.type[c0 * kClusterSize + lane] * ntype;
+        }
 
-            IAtoms       I;
-            const float4 sv    = a.shiftVec[sh];
-            float        qSelf = 0.0f;
+        // the group's tiles: work[W0, W0 + 16 T)
+        const int4     cFirst = a.chunks[ch0];
+        const int4     cLast  = a.chunks[ch1 - 1];
+        const unsigned W0     = static_cast<unsigned>(uniformInt(cFirst.z));
+        const int      T      = (uniformInt(cLast.z) - static_cast<int>(W0)) / kTileJ + uniformInt(cLast.w);
+
+        double eVdw = 0.0, eCoul = 0.0;
+        IAtoms I;
+        float4 sv;
+        float  fi[12];
+        int    ci = 0, sh = 0, nspec = 0, chStart = 0, chEnd = 0, chIdx = ch0;
+        bool   central = false;
+
+        auto beginChunk = [&](const int4 cd, int t) {
+            ci              = uniformInt(cd.x);
+            const int flags = uniformInt(cd.y);
+            sh              = flags & 0xff;
+            nspec           = flags >> 9;
+            central         = (sh == kCentral);
+            chStart         = t;
+            chEnd           = t + uniformInt(cd.w);
+            const int lc    = (ci - c0) * kClusterSize;
 #pragma unroll
             for (int i = 0; i < 4; i++)
             {
-                // a wave-uniform load, moved to scalar registers
-                const float4 p = a.xq[ci * kClusterSize + i];
+                const float4 p = ix[lc + i];
                 I.x[i]         = uniformFloat(p.x);
                 I.y[i]         = uniformFloat(p.y);
                 I.z[i]         = uniformFloat(p.z);
                 I.q[i]         = uniformFloat(p.w);
-                I.tOff[i]      = __builtin_amdgcn_readfirstlane(a.type[ci * kClusterSize + i]) * ntype;
-                qSelf          = (lane == i) ? p.w : qSelf;
+                I.tOff[i]      = uniformInt(it[lc + i]);
             }
-            float fi[12];
+            const float4 s = sShift[sh];
+            sv             = make_float4(uniformFloat(s.x), uniformFloat(s.y), uniformFloat(s.z), 0.0f);
 #pragma unroll
             for (int k = 0; k < 12; k++)
             {
                 fi[k] = 0.0f;
             }
-            if (kEnergy && self && lane < 4)
-            {
-                // kernel_ref_outer.h:286-326, Vc -= 1/2 epsfac q_i^2 V(0)
-                eCoul -= a.selfCoef * static_cast<double>(qSelf) * static_cast<double>(qSelf);
-            }
-            chunkTiles<kElec, kEnergy>(a, I, sv, ci, central, static_cast<unsigned>(__builtin_amdgcn_readfirstlane(ch.z)),
-                                       nt, nspec, jj, kk, ic, sNbfp, sTab, tabN, sWin, fi, eVdw, eCoul);
-
+        };
+        auto endChunk = [&]() {
             // i-forces: lane L ends up with component (L >> 2) = 3*i + d
-            float v16[16];
-#pragma unroll
-            for (int k = 0; k < 12; k++)
-            {
-                v16[k] = fi[k];
-            }
-#pragma unroll
-            for (int k = 12; k < 16; k++)
-            {
-                v16[k] = 0.0f;
-            }
-            const float r   = waveReduceScatter16(v16, lane);
+            const float r   = waveReduceScatter12(fi, lane);
             const int   idx = lane >> 2;
             if ((lane & 3) == 0 && idx < 12)
             {
-                atomicAdd(sWin + (ci - c0) * 12 + idx, r);
+                win[(ci - c0) * 12 + idx] += r;
             }
             if (kShift)
             {
@@ -421,55 +452,107 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
                     atomicAdd(sFsh + sh * 3 + lane, (s0 + s1) + (s2 + s3));
                 }
             }
+        };
+        auto doTile = [&](const uint2 e, const float4 xj, const int tj, const bool checked) {
+            const JAtom J{ xj.x - sv.x, xj.y - sv.y, xj.z - sv.z, ic.epsfac * xj.w, tj };
+            float       fjx = 0.0f, fjy = 0.0f, fjz = 0.0f, vlj = 0.0f, vc = 0.0f;
+            if (checked)
+            {
+                const bool diag = central && static_cast<int>(e.x) == ci;
+                tile4<kElec, kEnergy, true>(I, J, (e.y & 0xffffu) >> jj, diag, jj, ic, sNbfp, sTab, fi, fjx, fjy,
+                                            fjz, vlj, vc);
+            }
+            else
+            {
+                tile4<kElec, kEnergy, false>(I, J, 0u, false, jj, ic, sNbfp, sTab, fi, fjx, fjy, fjz, vlj, vc);
+            }
+            // the 16 entries of a tile have distinct slots (padding entries
+            // share the spare slot, adding zeros): plain read-modify-write
+            float* w = win + (e.y >> 16) * 12 + jj * 3;
+            w[0] += fjx;
+            w[1] += fjy;
+            w[2] += fjz;
+            if (kEnergy)
+            {
+                eVdw += static_cast<double>(vlj);
+                eCoul += static_cast<double>(vc);
+            }
+        };
+
+        // reads past the group's last tile are clamped to it
+        auto entryAt = [&](int t) {
+            return ldg(a.work, W0 + static_cast<unsigned>(kTileJ * min(t, T - 1) + kk));
+        };
+        uint2  eA     = entryAt(0);
+        uint2  eB     = entryAt(1);
+        float4 xA     = ldg(a.xq, eA.x * kClusterSize + jj);
+        int    tA     = ldg(a.type, eA.x * kClusterSize + jj);
+        int4   cdNext = a.chunks[min(ch0 + 1, ch1 - 1)];
+        beginChunk(cFirst, 0);
+        for (int t = 0; t < T; t += 2)
+        {
+            const float4 xB  = ldg(a.xq, eB.x * kClusterSize + jj);
+            const int    tB  = ldg(a.type, eB.x * kClusterSize + jj);
+            const uint2  eA2 = entryAt(t + 2);
+            if (t == chEnd)
+            {
+                endChunk();
+                chIdx++;
+                beginChunk(cdNext, t);
+                cdNext = a.chunks[min(chIdx + 1, ch1 - 1)];
+            }
+            doTile(eA, xA, tA, t - chStart < nspec);
+            if (t + 1 >= T)
+            {
+                break;
+            }
+            const uint2 eB2 = entryAt(t + 3);
+            xA              = ldg(a.xq, eA2.x * kClusterSize + jj);
+            tA              = ldg(a.type, eA2.x * kClusterSize + jj);
+            if (t + 1 == chEnd)
+            {
+                endChunk();
+                chIdx++;
+                beginChunk(cdNext, t + 1);
+                cdNext = a.chunks[min(chIdx + 1, ch1 - 1)];
+            }
+            doTile(eB, xB, tB, t + 1 - chStart < nspec);
+            eA = eA2;
+            eB = eB2;
         }
-        __syncthreads();
+        endChunk();
+
+        if (kEnergy)
+        {
+            eVdw  = waveSumD(eVdw);
+            eCoul = waveSumD(eCoul);
+            if (lane == 0)
+            {
+                a.energyPartial[2 * g]     = eVdw;
+                a.energyPartial[2 * g + 1] = eCoul;
+            }
+        }
 
         // flush the window: one atomic per nonzero atom component
-        for (int t = threadIdx.x; t < nslots * 12; t += kThreads)
+        for (int t = lane; t < (G + U) * 12; t += kWave)
         {
-            const float v = sWin[t];
+            const float v = win[t];
             if (v != 0.0f)
             {
                 const int slot = t / 12;
                 const int r    = t - slot * 12;
-                const int atom = r / 3;
-                const int comp = r - atom * 3;
-                const int cl   = slot < G ? c0 + slot : a.unionList[grp.z + slot - G];
+                const int cl   = slot < G ? c0 + slot : a.unionList[uoff + slot - G];
                 if (cl < a.ncluster)
                 {
-                    unsafeAtomicAdd(reinterpret_cast<float*>(a.fnb + cl * kClusterSize + atom) + comp, v);
+                    unsafeAtomicAdd(reinterpret_cast<float*>(a.fnb + cl * kClusterSize + r / 3) + (r % 3), v);
                 }
             }
         }
-        __syncthreads();
     }
 
-    if (kEnergy)
-    {
-#pragma unroll
-        for (int m = 1; m < kWave; m *= 2)
-        {
-            eVdw += __shfl_xor(eVdw, m, kWave);
-            eCoul += __shfl_xor(eCoul, m, kWave);
-        }
-        if (lane == 0)
-        {
-            sEner[2 * wave]     = eVdw;
-            sEner[2 * wave + 1] = eCoul;
-        }
-    }
-    __syncthreads();
-    if (kEnergy && threadIdx.x < 2)
-    {
-        double s = 0.0;
-        for (int w = 0; w < kWavesBlock; w++)
-        {
-            s += sEner[2 * w + threadIdx.x];
-        }
-        a.energyPartial[2 * blockIdx.x + threadIdx.x] = s;
-    }
     if (kShift)
     {
+        __syncthreads();
         for (int t = threadIdx.x; t < kNumShifts * 3; t += kThreads)
         {
             a.fshiftPartial[blockIdx.x * kNumShifts * 3 + t] = sFsh[t];
@@ -502,15 +585,8 @@ void launchElec(hipStream_t s, const KernelArgs& a, bool energy, bool shiftForce
 
 int nonbondedLdsBytes(int ntype, int tabN, int elec, int windowSlots)
 {
-    int bytes = ((ntype * ntype + 1) / 2) * 16;
-    if (elec == kElecEwaldTable)
-    {
-        bytes += tabN * 16;
-    }
-    bytes += windowSlots * 12 * 4;
-    bytes += ((kNumShifts * 3 + 4) / 4) * 16;
-    bytes += kWavesBlock * 2 * 8;
-    return bytes;
+    const LdsLayout L(ntype, elec == kElecEwaldTable ? tabN : 0, windowSlots);
+    return L.end_ * 16;
 }
 
 void launchNonbonded(hipStream_t s, const KernelArgs& a, int elec, bool energy, bool shiftForces, int numBlocks)
@@ -523,6 +599,10 @@ void launchNonbonded(hipStream_t s, const KernelArgs& a, int elec, bool energy, 
     if (elec == kElecRF)
     {
         launchElec<kElecRF>(s, a, energy, shiftForces, numBlocks, lds);
+    }
+    else if (elec == kElecEwaldAna)
+    {
+        launchElec<kElecEwaldAna>(s, a, energy, shiftForces, numBlocks, lds);
     }
     else
     {

@@ -34,15 +34,21 @@ constexpr float kFarAway     = -1000000.0f;
 constexpr int kMaxTypes = 63;
 // the per-column rank sort is quadratic in the column population
 constexpr int kMaxColumnAtoms = 65536;
-// kernel work groups: at most this many consecutive i-clusters share an LDS
-// j-force window; a group's j-entries are sorted in LDS (at most kGroupMaxJ)
+// kernel work groups: at most this many consecutive i-clusters share a
+// wavefront's LDS j-force window; a group's j-entries are sorted in LDS by
+// the group pass (at most kGroupMaxJ)
 constexpr int kGroupSizeMax = 8;
 constexpr int kGroupMaxJ    = 4096;
-// j-clusters per kernel tile, j-entries per work chunk (a multiple of it)
+// j-clusters per kernel tile, j-entries per work chunk (a multiple of it;
+// large enough that a chunk is a whole i-entry)
 constexpr int kTileJ        = 16;
-constexpr int kChunkEntries = 64;
-// LDS a nonbonded workgroup may use
-constexpr int kKernelLdsBudget = 64 * 1024;
+constexpr int kChunkEntries = 1 << 20;
+// wavefronts per nonbonded workgroup, and the LDS a workgroup may use for
+// four resident workgroups per CU (16 wavefronts), or at most 64 KiB when
+// the lists are too long for that
+constexpr int kKernelWaves        = 4;
+constexpr int kKernelLdsBudget    = 40 * 1024;
+constexpr int kKernelLdsBudgetMax = 64 * 1024;
 // rsq floor for float (pairlist.h:146): r > 6.2e-4 nm
 constexpr float kMinRsq = 3.82e-07f;
 
@@ -70,6 +76,8 @@ struct InteractionConst
     float tab_scale;
     int   tab_n;
     float self_v0; // V(0) of the Ewald correction (or c_rf for RF)
+    float beta2;   // beta^2 and beta^3 for the analytical correction
+    float beta3;
 };
 
 struct GridDims
@@ -161,20 +169,40 @@ struct KernelArgs
     const float4* tabFDV0;
     const float4* shiftVec;
     InteractionConst ic;
-    double   selfCoef; // epsfac * V(0) / 2 in double (V(0) from the float beta)
+    int*     groupCounter; // work-group claim counter, zeroed before launch
     float4*  fnb;
-    double*  energyPartial; // [gridDim.x][2]
+    double*  energyPartial; // [ngroups][2]
     float*   fshiftPartial; // [gridDim.x][45*3]
 };
 void launchNonbonded(hipStream_t s, const KernelArgs& a, int elec, bool energy, bool shiftForces, int numBlocks);
 int  nonbondedLdsBytes(int ntype, int tabN, int elec, int windowSlots);
 
 // -- buffers.hip
-void launchClearF4(hipStream_t s, float4* p, int n);
+void launchClearF4(hipStream_t s, float4* p, int n, int* counter);
 void launchConvertX(hipStream_t s, int nslot, const int* atomIndices, const float* x, float4* xq);
 void launchReduceF(hipStream_t s, int natoms, const int* cells, const float4* fnb, float* f);
-void launchFinalize(hipStream_t s, int numPartials, const double* energyPartial, const float* fshiftPartial,
-                    bool shiftForces, double* energyOut, double* fshiftOut);
+void launchFinalize(hipStream_t s, int numEnergyPartials, const double* energyPartial, int numExclPartials,
+                    const double* exclPartial, int numShiftPartials, const float* fshiftPartial, bool shiftForces,
+                    double* energyOut, double* fshiftOut);
+// the self energy and excluded-pair Coulomb energies in double
+struct ExclusionArgs
+{
+    int              natoms;
+    const int*       cells;
+    const float4*    xq;
+    const int*       exclOff;
+    const int*       exclIdx;
+    float            box[3];
+    int              elec;
+    InteractionConst ic;
+    double           rc2;      // the float rcoulomb^2, as double
+    double           epsfacD;  // the float epsfac, as double
+    double           selfCoef; // epsfac V(0) / 2
+    const double*    tabD;     // FDV0 table in double (table flavour)
+    double*          partial;  // one per block
+};
+constexpr int kExclMaxBlocks = 1024;
+int launchExclusionEnergy(hipStream_t s, const ExclusionArgs& a);
 void launchVirial(hipStream_t s, int natoms, const float* x, const float* f, const double* fshift, const float4* shiftVec,
                   double* virialPartial, int numBlocks, double* virialOut);
 void launchExpandPairs(hipStream_t s, const int4* ientry, int nci, const uint2* jentry, const float4* xq,

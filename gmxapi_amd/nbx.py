@@ -110,6 +110,8 @@ def library() -> C.CDLL:
     lib.nbx_list_pairs.restype = C.c_int64
     lib.nbx_get_interaction_constants.argtypes = [vp, vp]
     lib.nbx_get_work_groups.argtypes = [vp, vp, vp, vp, vp, vp]
+    lib.nbx_set_kernel_timing.argtypes = [vp, C.c_int]
+    lib.nbx_get_kernel_timing.argtypes = [vp, vp, vp]
     _lib = lib
     return lib
 
@@ -300,6 +302,18 @@ class Context:
                     "nbx_get_work_groups")
         return dict(group_size=G, num_groups=ng, num_chunks=nch, window_slots=slots, groups=groups[:ng],
                     chunks=chunks[:nch], union=union[:total], work=work[:nwork])
+
+    def set_kernel_timing(self, max_launches: int):
+        """Record HIP events around the next max_launches pair-kernel launches."""
+        self._check(self._lib.nbx_set_kernel_timing(self._h, int(max_launches)), "nbx_set_kernel_timing")
+
+    def kernel_timing(self):
+        """(average ms, launches) of the recorded pair-kernel launches."""
+        avg = C.c_double(0.0)
+        n = C.c_int(0)
+        self._check(self._lib.nbx_get_kernel_timing(self._h, C.byref(avg), C.byref(n)),
+                    "nbx_get_kernel_timing")
+        return avg.value, n.value
 
     def interaction_constants(self) -> dict:
         out = np.zeros(9)

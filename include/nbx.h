@@ -197,6 +197,16 @@ int64_t nbx_list_pairs(nbx_context* context, double r, int64_t* out, int64_t cap
 int nbx_get_work_groups(nbx_context* context, int* info, int* groups, int* chunks, int* union_list,
                         uint32_t* work);
 
+/* Records HIP events on the context's stream around each of the next
+ * max_launches launches of the cluster-pair kernel (0 stops recording); no
+ * counterpart in the reference, which times its CPU kernels with cycle
+ * counters (nbnxm/benchmark/bench_setup.cpp:300-330).  For benchmarking. */
+int nbx_set_kernel_timing(nbx_context* context, int max_launches);
+
+/* Average duration in ms of the recorded pair-kernel launches and their
+ * number; waits for them, then starts a new record. */
+int nbx_get_kernel_timing(nbx_context* context, double* average_ms, int* launches);
+
 /* beta, table scale, table size, epsfac, k_rf, c_rf, sh_ewald, LJ
  * dispersion and repulsion potential shifts (interaction_const_t). */
 int nbx_get_interaction_constants(nbx_context* context, double* out9);

@@ -285,6 +285,18 @@ void deviceTests(const std::vector<TestSystem>& systems)
         std::vector<Vec3> small(2);
         expectThrowsInput([&] { calc.compute(state.coordinates(), small); }, "force array size");
     });
+    run("AnalyticalEwaldDefaultOptions", [&] {
+        // nblib's default options: Pme with the analytical correction
+        SimulationState state = spc.state();
+        NBKernelOptions o;
+        o.pairlistCutoff = spc.rc;
+        ForceCalculator calc(state, o);
+        calc.compute(state.coordinates(), state.forces());
+        for (const Vec3& f : state.forces())
+        {
+            expect(std::isfinite(f[0]) && std::isfinite(f[1]) && std::isfinite(f[2]), "finite forces");
+        }
+    });
 }
 
 } // namespace

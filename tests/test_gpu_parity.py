@@ -162,6 +162,50 @@ def test_forces_energies_water288k():
     np.testing.assert_allclose(ctx.virial(), r64["virial"], rtol=0, atol=1e-5 * np.abs(r64["virial"]).max())
 
 
+@pytest.mark.parametrize("name,s", _systems()[1:])
+def test_analytical_ewald_against_exact_oracle(name, s):
+    # the analytical correction against the FP64 oracle with exact erfc
+    # (what the reference's SIMD kernels approximate, simd_math.h:1609-1720)
+    print(f"[analytical] {name}", flush=True)
+    ctx = nbx.context_for(s, ewald_correction="analytical")
+    f = ctx.compute(s.x, energy=True, virial=True)
+    vlj, vc = ctx.energies()
+    r64 = oracle_for(s, "f64", ewald_exact=True).compute(s.x)
+    r32 = oracle_for(s, "f32").compute(s.x)
+    ferr = rms(f - r64["f"]) / rms(r64["f"])
+    e64 = r64["Vvdw"] + r64["Vc"]
+    print(f"  force rms err {ferr:.2e}; Vlj {vlj:.6f} ({r64['Vvdw']:.6f}) Vc {vc:.6f} ({r64['Vc']:.6f})", flush=True)
+    assert ferr <= FORCE_TOL_RMS
+    for got, key in ((vlj, "Vvdw"), (vc, "Vc")):
+        scale = max(abs(r64[key]), abs(e64))
+        assert abs(got - r64[key]) <= _energy_bar(r64[key], r32[key], scale), (key, got, r64[key])
+
+
+def test_analytical_ewald_config0(water3000):
+    print("[analytical] config0 9000 atoms", flush=True)
+    s = water3000
+    ctx = nbx.context_for(s, ewald_correction="analytical")
+    f = ctx.compute(s.x, energy=True)
+    vlj, vc = ctx.energies()
+    r64 = oracle_for(s, "f64", ewald_exact=True).compute(s.x)
+    ferr = rms(f - r64["f"]) / rms(r64["f"])
+    print(f"  force rms err {ferr:.2e}; Vc {vc:.6f} ({r64['Vc']:.6f})", flush=True)
+    assert ferr <= FORCE_TOL_RMS
+    assert abs(vlj - r64["Vvdw"]) <= ENERGY_TOL_REL * abs(r64["Vvdw"] + r64["Vc"])
+    assert abs(vc - r64["Vc"]) <= ENERGY_TOL_REL * abs(r64["Vc"])
+
+
+def test_energies_are_bitwise_reproducible():
+    # each group's energy is summed by one wavefront and the group sums are
+    # added in group order, whatever wavefront took which group
+    s = S.config(0)
+    ctx = nbx.context_for(s)
+    ctx.compute(s.x, energy=True)
+    e1 = ctx.energies()
+    ctx.compute(s.x, energy=True)
+    assert ctx.energies() == e1
+
+
 @pytest.mark.parametrize("coulomb", ["cutoff", "reaction-field"])
 def test_reaction_field_flavours(coulomb):
     s = S.water_box(300, 2.1, seed=5)
@@ -259,7 +303,7 @@ def test_work_list_covers_the_pair_list(name, s):
         for c in range(ch0, ch1):
             ci, flags, ws, tiles = (int(v) for v in wg["chunks"][c])
             sh, is_self, special = flags & 0xFF, (flags >> 8) & 1, flags >> 9
-            assert g * G <= ci < (g + 1) * G and 1 <= tiles <= 4 and special <= tiles
+            assert g * G <= ci < (g + 1) * G and tiles >= 1 and special <= tiles
             if is_self:
                 assert sh == 22
                 self_chunks.append(ci)
@@ -267,7 +311,8 @@ def test_work_list_covers_the_pair_list(name, s):
                 cj, w = int(wg["work"][k, 0]), int(wg["work"][k, 1])
                 slot, mask = w >> 16, w & 0xFFFF
                 if cj == nc:
-                    assert slot == ci - g * G and mask == 0xFFFF
+                    # padding: the far-away dummy cluster in the spare slot
+                    assert slot == G + len(union) and mask == 0xFFFF
                     continue
                 assert (g * G + slot if slot < G else union[slot - G]) == cj
                 if mask != 0xFFFF:
