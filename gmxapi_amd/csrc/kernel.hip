@@ -79,6 +79,7 @@ constexpr float kGq1 = 0.5821957588195801f, kGq2 = 0.15719348192214966f, kGq3 = 
                 kGq4 = 0.002643152140080929f, kGq5 = 0.0001805289793992415f;
 constexpr float kHp0 = 1.128378987312317f, kHp1 = 0.2956092357635498f, kHp2 = 0.0626194030046463f,
                 kHp3 = 0.0073672509752213955f, kHp4 = 0.00019465858349576592f, kHp5 = -1.2356190381979104e-06f;
+constexpr float kAnaMaxS = 3.6f * 3.6f;
 constexpr float kHq1 = 0.595305323600769f, kHq2 = 0.15395577251911163f, kHq3 = 0.022073965519666672f,
                 kHq4 = 0.0017358317272737622f;
 
@@ -200,7 +201,10 @@ __device__ __forceinline__ void pairI(const IAtoms& I, const JAtom& J, const uns
     }
     else
     {
-        const float s  = rsq * ic.beta2;
+        // clamped to the fitted range: pairs beyond the cut-off (qq = 0) may
+        // be far away (the padding cluster), where s^5 would overflow to inf
+        // and inf / inf is NaN
+        const float s  = fminf(rsq * ic.beta2, kAnaMaxS);
         const float pg = fmaf(fmaf(fmaf(fmaf(fmaf(kGp5, s, kGp4), s, kGp3), s, kGp2), s, kGp1), s, kGp0);
         const float qg = fmaf(fmaf(fmaf(fmaf(fmaf(kGq5, s, kGq4), s, kGq3), s, kGq2), s, kGq1), s, 1.0f);
         fcoul          = qq * (interact * rinv * rinvsq - ic.beta3 * pg * __builtin_amdgcn_rcpf(qg));
