@@ -3,13 +3,15 @@ correction used by kernel.hip (NBX_EWALD_ANALYTICAL).
 
 With z = beta r and s = z^2, the kernel needs
     g(z) = (erf z - 2 z / sqrt(pi) exp(-z^2)) / z^3   (force: F_corr / r = beta^3 g)
-    h(z) = erf(z) / z                                 (potential: V_corr = beta h)
-on s in [0, S_MAX] (z up to 3.6, which covers beta rc for ewald_rtol down to
-about 3e-7; the library rejects the analytical flavour beyond it).  Both
-are even and smooth, so each is fitted as P(s)/Q(s) with Q(0) = 1 by
-iteratively reweighted linear least squares on the relative error
-(Loeb's linearisation, then Lawson-style reweighting toward equal ripple),
-against 40-digit values from mpmath.  Run: python3 fit_ewald_correction.py
+    E(z) = erfc(z) exp(z^2)                           (energy: erfc(beta r) / r
+                                                       = exp(-s) E(z) / r)
+for z up to 3.6, which covers beta rc for ewald_rtol down to about 3e-7 (the
+library rejects the analytical flavour beyond it).  The energy uses erfc
+directly rather than 1/r - erf(beta r)/r, which would cancel.  g is even
+and fitted as P(s)/Q(s); E is fitted as P(z)/Q(z); both with Q(0) = 1, by
+iteratively reweighted linear least squares on the relative error (Loeb's
+linearisation, then Lawson-style reweighting toward equal ripple), against
+40-digit values from mpmath.  Run: python3 fit_ewald_correction.py
 """
 import mpmath as mp
 import numpy as np
@@ -25,15 +27,14 @@ def g(s):
     return (mp.erf(z) - 2 * z / mp.sqrt(mp.pi) * mp.exp(-z * z)) / z ** 3
 
 
-def h(s):
-    z = mp.sqrt(mp.mpf(s))
-    if z == 0:
-        return 2 / mp.sqrt(mp.pi)
-    return mp.erf(z) / z
+def e_of_s(s):
+    # E as a function of the fit variable; fitted in z = sqrt(s) below
+    z = mp.mpf(s)
+    return mp.erfc(z) * mp.exp(z * z)
 
 
-def fit(fun, m, n, iters=40, nodes=3000):
-    s = (S_MAX / 2) * (1 - np.cos(np.pi * (np.arange(nodes) + 0.5) / nodes))
+def fit(fun, m, n, iters=40, nodes=3000, x_max=S_MAX):
+    s = (x_max / 2) * (1 - np.cos(np.pi * (np.arange(nodes) + 0.5) / nodes))
     f = np.array([float(fun(v)) for v in s])
     w = np.ones(nodes)
     for it in range(iters):
@@ -52,8 +53,8 @@ def fit(fun, m, n, iters=40, nodes=3000):
 
 
 if __name__ == "__main__":
-    for name, fun, (m, n) in (("g", g, (5, 5)), ("h", h, (5, 4))):
-        p, q, e = fit(fun, m, n)
+    for name, fun, (m, n), x_max in (("g(s)", g, (5, 5), S_MAX), ("E(z)", e_of_s, (4, 4), 3.6)):
+        p, q, e = fit(fun, m, n, x_max=x_max)
         print(f"{name}: P degree {m}, Q degree {n}, max relative error {e:.2e} (exact arithmetic)")
         print("  P:", ", ".join(f"{float(np.float32(v))!r}f" for v in p))
         print("  Q:", ", ".join(f"{float(np.float32(v))!r}f" for v in q[1:]))

@@ -67,21 +67,24 @@ constexpr float kTwelfthLo = static_cast<float>(1.0 / 12.0 - static_cast<double>
 constexpr float kSixthHi   = static_cast<float>(1.0 / 6.0);
 constexpr float kSixthLo   = static_cast<float>(1.0 / 6.0 - static_cast<double>(static_cast<float>(1.0 / 6.0)));
 
-// Analytical Ewald correction with s = (beta r)^2, fitted by
-// fit_ewald_correction.py (relative error below 1e-7 in exact arithmetic for
-// s <= 12.96, about 4e-7 in float evaluation):
-//   force      F_corr(r) / r = beta^3 P_g(s) / Q_g(s),
-//              closed form beta^3 (erf z - 2 z exp(-z^2) / sqrt(pi)) / z^3
-//   potential  V_corr(r)     = beta P_h(s) / Q_h(s), closed form beta erf(z) / z
+// Analytical Ewald correction, fitted by fit_ewald_correction.py (relative
+// error below 1e-7 in exact arithmetic, about 4e-7 in float evaluation):
+//   force   F_corr(r) / r = beta^3 P_g(s) / Q_g(s), s = (beta r)^2,
+//           closed form beta^3 (erf z - 2 z exp(-z^2) / sqrt(pi)) / z^3
+//   energy  erfc(beta r) / r = exp(-s) P_e(z) / Q_e(z) / r, z = beta r,
+//           closed form erfc(z) exp(z^2) for P_e / Q_e (the energy of a
+//           non-excluded pair, 1/r - erf(beta r)/r, taken without the
+//           cancellation of the two terms)
 constexpr float kGp0 = 0.7522528767585754f, kGp1 = -0.01339578628540039f, kGp2 = 0.016685180366039276f,
                 kGp3 = 0.0003194923046976328f, kGp4 = 3.5602472053142264e-05f, kGp5 = -3.4331239362472843e-07f;
 constexpr float kGq1 = 0.5821957588195801f, kGq2 = 0.15719348192214966f, kGq3 = 0.025581449270248413f,
                 kGq4 = 0.002643152140080929f, kGq5 = 0.0001805289793992415f;
-constexpr float kHp0 = 1.128378987312317f, kHp1 = 0.2956092357635498f, kHp2 = 0.0626194030046463f,
-                kHp3 = 0.0073672509752213955f, kHp4 = 0.00019465858349576592f, kHp5 = -1.2356190381979104e-06f;
+constexpr float kEp0 = 1.0f, kEp1 = 0.8535063862800598f, kEp2 = 0.35605478286743164f, kEp3 = 0.059257231652736664f,
+                kEp4 = 1.4358666703628842e-05f;
+constexpr float kEq1 = 1.9818848371505737f, kEq2 = 1.5923840999603271f, kEq3 = 0.6263630390167236f,
+                kEq4 = 0.10554185509681702f;
 constexpr float kAnaMaxS = 3.6f * 3.6f;
-constexpr float kHq1 = 0.595305323600769f, kHq2 = 0.15395577251911163f, kHq3 = 0.022073965519666672f,
-                kHq4 = 0.0017358317272737622f;
+constexpr float kLog2e   = 1.4426950408889634f;
 
 // 32-bit byte offset from a wave-uniform base: lets the compiler use the
 // scalar-base + vector-offset form of global loads
@@ -174,7 +177,7 @@ __device__ __forceinline__ void pairI(const IAtoms& I, const JAtom& J, const uns
     const bool  maskQ = kEnergy || kElec != kElecEwaldTable;
     const float qq    = maskQ ? smask * I.q[kI] * J.w : I.q[kI] * J.w;
     float       fcoul;
-    float       vcorr = 0.0f;
+    float       vpair = 0.0f; // the pair's Coulomb energy over qq (Ewald flavours)
     if (kElec == kElecRF)
     {
         fcoul = qq * (interact * rinv * rinvsq - 2.0f * ic.k_rf);
@@ -196,7 +199,7 @@ __device__ __forceinline__ void pairI(const IAtoms& I, const JAtom& J, const uns
         if (kEnergy)
         {
             const float halfsp = 0.5f / ic.tab_scale;
-            vcorr              = tab.z - halfsp * frac * (tab.x + fexcl);
+            vpair              = (rinv - ic.sh_ewald) - (tab.z - halfsp * frac * (tab.x + fexcl));
         }
     }
     else
@@ -210,15 +213,17 @@ __device__ __forceinline__ void pairI(const IAtoms& I, const JAtom& J, const uns
         fcoul          = qq * (interact * rinv * rinvsq - ic.beta3 * pg * __builtin_amdgcn_rcpf(qg));
         if (kEnergy)
         {
-            const float ph = fmaf(fmaf(fmaf(fmaf(fmaf(kHp5, s, kHp4), s, kHp3), s, kHp2), s, kHp1), s, kHp0);
-            const float qh = fmaf(fmaf(fmaf(fmaf(kHq4, s, kHq3), s, kHq2), s, kHq1), s, 1.0f);
-            vcorr          = ic.beta * ph * __builtin_amdgcn_rcpf(qh);
+            // erfc(beta r)/r directly: 1/r - erf(beta r)/r would cancel
+            const float z  = sqrtf(s);
+            const float pe = fmaf(fmaf(fmaf(fmaf(kEp4, z, kEp3), z, kEp2), z, kEp1), z, kEp0);
+            const float qe = fmaf(fmaf(fmaf(fmaf(kEq4, z, kEq3), z, kEq2), z, kEq1), z, 1.0f);
+            vpair = rinv * __builtin_amdgcn_exp2f(-s * kLog2e) * pe * __builtin_amdgcn_rcpf(qe) - ic.sh_ewald;
         }
     }
     if (kEnergy && kElec != kElecRF)
     {
         // excluded pairs (interact = 0) are left to the exclusion pass
-        vc += kChecks ? interact * qq * ((rinv - ic.sh_ewald) - vcorr) : qq * ((rinv - ic.sh_ewald) - vcorr);
+        vc += kChecks ? interact * qq * vpair : qq * vpair;
     }
 
     const float fscal = frLJ * rinvsq + fcoul;
