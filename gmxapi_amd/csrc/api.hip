@@ -9,8 +9,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/nbx.h"
-#include "nbx_internal.h"
+#include "context.h"
 
 using namespace nbx;
 
@@ -19,68 +18,15 @@ namespace
 
 thread_local std::string g_lastError;
 
-struct InputError : std::runtime_error
-{
-    using std::runtime_error::runtime_error;
-};
+} // namespace
 
-struct StateError : std::runtime_error
+std::string& nbx::lastError()
 {
-    using std::runtime_error::runtime_error;
-};
-
-struct DeviceError : std::runtime_error
-{
-    using std::runtime_error::runtime_error;
-};
-
-void check(hipError_t e, const char* what)
-{
-    if (e != hipSuccess)
-    {
-        throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
-    }
+    return g_lastError;
 }
 
-template<typename T>
-struct DevBuf
+namespace
 {
-    T*     p   = nullptr;
-    size_t cap = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    void swap(DevBuf& o)
-    {
-        std::swap(p, o.p);
-        std::swap(cap, o.cap);
-    }
-    void   ensure(size_t n)
-    {
-        if (n <= cap && p)
-        {
-            return;
-        }
-        if (p)
-        {
-            check(hipFree(p), "hipFree");
-            p = nullptr;
-        }
-        size_t want = std::max<size_t>(n, 1);
-        check(hipMalloc(reinterpret_cast<void**>(&p), want * sizeof(T)), "hipMalloc");
-        cap = want;
-    }
-    void release()
-    {
-        if (p)
-        {
-            (void)hipFree(p);
-        }
-        p   = nullptr;
-        cap = 0;
-    }
-    ~DevBuf() { release(); }
-};
 
 // ---------------------------------------------------------------------------
 // Interaction constants, restated from the reference in single precision
@@ -228,99 +174,9 @@ double one4PiEps0()
 
 } // namespace
 
-struct nbx_context
-{
-    nbx_options      opt{};
-    int              device     = 0;
-    hipStream_t      ownStream  = nullptr;
-    hipStream_t      stream     = nullptr;
-    // optional HIP events around the pair-kernel launches (benchmarking)
-    std::vector<hipEvent_t> timingEvents; // start, stop pairs
-    int                     timingRecorded = 0;
-    int              numCU      = 256;
-    int              elec       = kElecEwaldTable;
-    InteractionConst ic{};
-    double           selfCoef = 0.0;
-    float            rlist = 1.0f, rcoulomb = 1.0f;
-
-    // parameters
-    int            ntypeUser = -1;
-    int            ntype     = 0; // incl. padding
-    DevBuf<float2> nbfp;
-    DevBuf<float4> tabFDV0;
-    DevBuf<double> tabFDV0d;
-    DevBuf<float4> shiftVec;
-
-    // topology
-    int         natoms = -1;
-    DevBuf<int> type;
-    DevBuf<float> charge;
-    DevBuf<int> exclOff, exclIdx;
-
-    // grid
-    GridDims    g{};
-    bool        gridValid = false;
-    int         ncluster  = 0;
-    DevBuf<float> x;          // device copy of coordinates in atom order
-    DevBuf<int>   cxy_na, cxy_ind, atomCol, atomRank, bucket, atomIndices, cells, nbtype, counter;
-    DevBuf<float4> xq, bb, fnb;
-
-    // list
-    bool           listValid = false;
-    int            nci = 0, ncj = 0;
-    DevBuf<int>    countJ, countI, offJ, offI, totals;
-    DevBuf<int4>   ientry;
-    DevBuf<uint2>  jentry, jentryTmp;
-    // kernel work groups
-    int            groupSize = 0, ngroups = 0, nchunk = 0, windowSlots = 0, nwork = 0;
-    DevBuf<int>    unionCount, chunkCount, workCount, overflow, unionOff, chunkOff, workOff, unionList, groupStats;
-    DevBuf<int4>   chunks, groups;
-    DevBuf<uint2>  work;
-
-    // outputs
-    int             numBlocks = 0;
-    DevBuf<double>  energyPartial, energy, fshift, virialPartial, virial;
-    DevBuf<int>     groupCounter;
-    DevBuf<double>  exclPartial;
-    DevBuf<float>   fshiftPartial, fOut;
-    bool            haveEnergy = false, haveShift = false, haveVirial = false;
-};
 
 namespace
 {
-
-int fail(int code, const std::string& msg)
-{
-    g_lastError = msg;
-    return code;
-}
-
-template<typename F>
-int guarded(F&& f)
-{
-    try
-    {
-        f();
-        g_lastError.clear();
-        return NBX_SUCCESS;
-    }
-    catch (const InputError& e)
-    {
-        return fail(NBX_ERROR_INPUT, e.what());
-    }
-    catch (const StateError& e)
-    {
-        return fail(NBX_ERROR_STATE, e.what());
-    }
-    catch (const DeviceError& e)
-    {
-        return fail(NBX_ERROR_DEVICE, e.what());
-    }
-    catch (const std::exception& e)
-    {
-        return fail(NBX_ERROR_DEVICE, e.what());
-    }
-}
 
 void setDevice(nbx_context* c)
 {
@@ -429,6 +285,7 @@ void buildWorkGroups(nbx_context* c)
         c->unionList.ensure(std::max(unionTotal, 1));
         c->chunks.ensure(std::max(chunkTotal, 1));
         c->groups.ensure(ng);
+        c->groupInfo.ensure(ng);
         c->work.ensure(std::max(workTotal, 1));
         ga.unionOff  = c->unionOff.p;
         ga.chunkOff  = c->chunkOff.p;
@@ -437,6 +294,7 @@ void buildWorkGroups(nbx_context* c)
         ga.work      = c->work.p;
         ga.chunks    = c->chunks.p;
         ga.groups    = c->groups.p;
+        ga.groupInfo = c->groupInfo.p;
         launchBuildGroups(c->stream, ga, true);
         check(hipGetLastError(), "group kernels");
         c->jentry.swap(c->jentryTmp);
@@ -1015,6 +873,7 @@ int nbx_dispatch_kernel(nbx_context* c, unsigned flags)
         KernelArgs a{};
         a.chunks        = c->chunks.p;
         a.groups        = c->groups.p;
+        a.groupInfo     = c->groupInfo.p;
         a.ngroups       = c->ngroups;
         a.groupSize     = c->groupSize;
         a.ncluster      = c->ncluster;

@@ -1,0 +1,176 @@
+// Host-side internals shared by the C ABI translation units (api.hip,
+// batch.hip): error types, the device buffer holder and the context.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/nbx.h"
+#include "nbx_internal.h"
+
+namespace nbx
+{
+
+// the message behind nbx_last_error() (per thread)
+std::string& lastError();
+
+struct InputError : std::runtime_error
+{
+    using std::runtime_error::runtime_error;
+};
+
+struct StateError : std::runtime_error
+{
+    using std::runtime_error::runtime_error;
+};
+
+struct DeviceError : std::runtime_error
+{
+    using std::runtime_error::runtime_error;
+};
+
+inline void check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess)
+    {
+        throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+
+template<typename T>
+struct DevBuf
+{
+    T*     p   = nullptr;
+    size_t cap = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    void swap(DevBuf& o)
+    {
+        std::swap(p, o.p);
+        std::swap(cap, o.cap);
+    }
+    void   ensure(size_t n)
+    {
+        if (n <= cap && p)
+        {
+            return;
+        }
+        if (p)
+        {
+            check(hipFree(p), "hipFree");
+            p = nullptr;
+        }
+        size_t want = std::max<size_t>(n, 1);
+        check(hipMalloc(reinterpret_cast<void**>(&p), want * sizeof(T)), "hipMalloc");
+        cap = want;
+    }
+    void release()
+    {
+        if (p)
+        {
+            (void)hipFree(p);
+        }
+        p   = nullptr;
+        cap = 0;
+    }
+    ~DevBuf() { release(); }
+};
+
+inline int fail(int code, const std::string& msg)
+{
+    lastError() = msg;
+    return code;
+}
+
+template<typename F>
+int guarded(F&& f)
+{
+    try
+    {
+        f();
+        lastError().clear();
+        return NBX_SUCCESS;
+    }
+    catch (const InputError& e)
+    {
+        return fail(NBX_ERROR_INPUT, e.what());
+    }
+    catch (const StateError& e)
+    {
+        return fail(NBX_ERROR_STATE, e.what());
+    }
+    catch (const DeviceError& e)
+    {
+        return fail(NBX_ERROR_DEVICE, e.what());
+    }
+    catch (const std::exception& e)
+    {
+        return fail(NBX_ERROR_DEVICE, e.what());
+    }
+}
+
+} // namespace nbx
+
+// one system on one device (the state behind an nbx_context handle)
+struct nbx_context
+{
+    nbx_options      opt{};
+    int              device     = 0;
+    hipStream_t      ownStream  = nullptr;
+    hipStream_t      stream     = nullptr;
+    // optional HIP events around the pair-kernel launches (benchmarking)
+    std::vector<hipEvent_t> timingEvents; // start, stop pairs
+    int                     timingRecorded = 0;
+    int              numCU      = 256;
+    int              elec       = nbx::kElecEwaldTable;
+    nbx::InteractionConst ic{};
+    double           selfCoef = 0.0;
+    float            rlist = 1.0f, rcoulomb = 1.0f;
+
+    // parameters
+    int            ntypeUser = -1;
+    int            ntype     = 0; // incl. padding
+    nbx::DevBuf<float2> nbfp;
+    nbx::DevBuf<float4> tabFDV0;
+    nbx::DevBuf<double> tabFDV0d;
+    nbx::DevBuf<float4> shiftVec;
+
+    // topology
+    int         natoms = -1;
+    nbx::DevBuf<int> type;
+    nbx::DevBuf<float> charge;
+    nbx::DevBuf<int> exclOff, exclIdx;
+
+    // grid
+    nbx::GridDims g{};
+    bool        gridValid = false;
+    int         ncluster  = 0;
+    nbx::DevBuf<float> x;          // device copy of coordinates in atom order
+    nbx::DevBuf<int>   cxy_na, cxy_ind, atomCol, atomRank, bucket, atomIndices, cells, nbtype, counter;
+    nbx::DevBuf<float4> xq, bb, fnb;
+
+    // list
+    bool           listValid = false;
+    int            nci = 0, ncj = 0;
+    nbx::DevBuf<int>    countJ, countI, offJ, offI, totals;
+    nbx::DevBuf<int4>   ientry;
+    nbx::DevBuf<uint2>  jentry, jentryTmp;
+    // kernel work groups
+    int            groupSize = 0, ngroups = 0, nchunk = 0, windowSlots = 0, nwork = 0;
+    nbx::DevBuf<int>    unionCount, chunkCount, workCount, overflow, unionOff, chunkOff, workOff, unionList, groupStats;
+    nbx::DevBuf<int4>   chunks, groups, groupInfo;
+    nbx::DevBuf<uint2>  work;
+
+    // outputs
+    int             numBlocks = 0;
+    nbx::DevBuf<double>  energyPartial, energy, fshift, virialPartial, virial;
+    nbx::DevBuf<int>     groupCounter;
+    nbx::DevBuf<double>  exclPartial;
+    nbx::DevBuf<float>   fshiftPartial, fOut;
+    bool            haveEnergy = false, haveShift = false, haveVirial = false;
+};

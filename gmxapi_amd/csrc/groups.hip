@@ -165,7 +165,8 @@ __global__ __launch_bounds__(kGroupThreads) void buildGroupsKernel(GroupArgs a)
     const int coff = a.chunkOff[g];
     if (threadIdx.x == 0)
     {
-        a.groups[g] = make_int4(coff, coff + nchunk, uoff, U);
+        a.groups[g]    = make_int4(coff, coff + nchunk, uoff, U);
+        a.groupInfo[g] = make_int4(c0, (c1 - c0) | (a.groupSize << 16), 0, 0);
     }
     // per i-entry: masked-first partition, slots, work list, chunks (one
     // wave per entry)

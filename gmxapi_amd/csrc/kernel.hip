@@ -38,8 +38,10 @@
 //     two so that no loaded value is copied between registers.
 //   * Masked entries come first in every i-entry: its leading tiles run with
 //     exclusion and diagonal checks, the rest without.
-//   * LJ parameters, shift vectors, the Ewald table (table flavour) and each
-//     group's i-atoms are staged in LDS.
+//   * LJ parameters, the Ewald table (table flavour) and each group's
+//     i-atoms are staged in LDS.  A group's first cluster, size and shift
+//     table come from its groupInfo record, so one launch can cover groups of
+//     several independent systems (the batched ensemble).
 //   * Energies: float per lane over the four pairs of a tile, then double;
 //     each group's sum goes to its own slot and the finalize kernel sums them
 //     in group order, so energies are bitwise reproducible.  The self energy
@@ -309,13 +311,12 @@ __device__ __forceinline__ double waveSumD(double v)
 // LDS layout of a workgroup, in float4 units
 struct LdsLayout
 {
-    int nbfp, tab, shift, fsh, ix, it, win;
+    int nbfp, tab, fsh, ix, it, win;
     __host__ __device__ LdsLayout(int ntype, int tabN, int windowSlots)
     {
         nbfp  = 0;
         tab   = nbfp + (ntype * ntype + 1) / 2;
-        shift = tab + tabN;
-        fsh   = shift + kNumShifts;
+        fsh   = tab + tabN;
         ix    = fsh + (kNumShifts * 3 + 3) / 4;
         it    = ix + kWavesBlock * kGroupSizeMax * kClusterSize;
         win   = it + kWavesBlock * kGroupSizeMax;
@@ -334,7 +335,6 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
     const LdsLayout L(ntype, tabN, a.windowSlots);
     float2*         sNbfp  = reinterpret_cast<float2*>(smem4 + L.nbfp);
     float4*         sTab   = smem4 + L.tab;
-    float4*         sShift = smem4 + L.shift;
     float*          sFsh   = reinterpret_cast<float*>(smem4 + L.fsh);
 
     for (int t = threadIdx.x; t < ntype2; t += kThreads)
@@ -344,10 +344,6 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
     for (int t = threadIdx.x; t < tabN; t += kThreads)
     {
         sTab[t] = a.tabFDV0[t];
-    }
-    for (int t = threadIdx.x; t < kNumShifts; t += kThreads)
-    {
-        sShift[t] = a.shiftVec[t];
     }
     for (int t = threadIdx.x; t < kNumShifts * 3; t += kThreads)
     {
@@ -361,7 +357,6 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
     const int kk   = lane >> 2;
 
     const InteractionConst ic = a.ic;
-    const int              G  = a.groupSize;
     // this wavefront's i-atom cache and j-force window
     float4* ix  = smem4 + L.ix + wave * kGroupSizeMax * kClusterSize;
     int*    it  = reinterpret_cast<int*>(smem4 + L.it) + wave * kGroupSizeMax * kClusterSize;
@@ -384,8 +379,11 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
         const int  ch1  = uniformInt(grp.y);
         const int  uoff = uniformInt(grp.z);
         const int  U    = uniformInt(grp.w);
-        const int  c0   = g * G;
-        const int  nOwn = min(G, a.ncluster - c0);
+        const int4 gi   = a.groupInfo[g]; // first own cluster, own count | G << 16, shift table
+        const int  c0   = uniformInt(gi.x);
+        const int  nOwn = uniformInt(gi.y) & 0xffff;
+        const int  G    = uniformInt(gi.y) >> 16;
+        const int  shb  = uniformInt(gi.z);
 
         // clear the window (own slots, union slots, the padding entries'
         // spare slot) and cache the group's i-atoms; this wavefront's LDS
@@ -432,7 +430,7 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
                 I.q[i]         = uniformFloat(p.w);
                 I.tOff[i]      = uniformInt(it[lc + i]);
             }
-            const float4 s = sShift[sh];
+            const float4 s = a.shiftVec[shb + sh];
             sv             = make_float4(uniformFloat(s.x), uniformFloat(s.y), uniformFloat(s.z), 0.0f);
 #pragma unroll
             for (int k = 0; k < 12; k++)
@@ -550,8 +548,8 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
             {
                 const int slot = t / 12;
                 const int r    = t - slot * 12;
-                const int cl   = slot < G ? c0 + slot : a.unionList[uoff + slot - G];
-                if (cl < a.ncluster)
+                const int cl   = slot < G ? (slot < nOwn ? c0 + slot : -1) : a.unionList[uoff + slot - G];
+                if (cl >= 0)
                 {
                     unsafeAtomicAdd(reinterpret_cast<float*>(a.fnb + cl * kClusterSize + r / 3) + (r % 3), v);
                 }

@@ -146,6 +146,7 @@ struct GroupArgs
     uint2*       work;       // {cj, slot << 16 | mask}, each chunk padded to whole tiles
     int4*        chunks;     // {ci, shift | self << 8 | special << 9, work_start, tiles}
     int4*        groups;     // {chunk_start, chunk_end, union_offset, union_size}
+    int4*        groupInfo;  // {first own cluster, own count | G << 16, shift-table offset, 0}
 };
 void launchGroupSpans(hipStream_t s, int ncluster, const int* offJ, int* maxSpan);
 void launchBuildGroups(hipStream_t s, const GroupArgs& a, bool fill);
@@ -156,6 +157,7 @@ struct KernelArgs
 {
     const int4*   chunks;
     const int4*   groups;
+    const int4*   groupInfo;
     int           ngroups;
     int           groupSize;
     int           ncluster;
@@ -167,7 +169,7 @@ struct KernelArgs
     const float2* nbfp;
     int           ntype; // including the padding type
     const float4* tabFDV0;
-    const float4* shiftVec;
+    const float4* shiftVec; // 45 per system; groupInfo.z selects the table
     InteractionConst ic;
     int*     groupCounter; // work-group claim counter, zeroed before launch
     float4*  fnb;
