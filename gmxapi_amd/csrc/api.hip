@@ -230,10 +230,15 @@ void buildWorkGroups(nbx_context* c)
     c->jentryTmp.ensure(c->ncj);
     // the largest group size whose j-force windows (one per wavefront) fit
     // the LDS budget of four resident workgroups per CU; failing that, two
+    const int forced = c->opt.kernel_group_size;
     for (int k = 0; k < 8; k++)
     {
         const int G      = kGroupSizeMax >> (k & 3);
         const int budget = k < 4 ? kKernelLdsBudget : kKernelLdsBudgetMax;
+        if (forced > 0 && (G != forced || k < 4))
+        {
+            continue; // a chosen size may use the larger budget
+        }
         if (spans[k & 3] > kGroupMaxJ && G > 1)
         {
             continue;
@@ -333,6 +338,8 @@ void nbx_default_options(nbx_options* o)
     o->epsilon_rf            = 1.0;
     o->ewald_potential_shift = 0;
     o->device                = 0;
+    o->kernel_group_size     = 0;
+    o->kernel_blocks_per_cu  = 0;
 }
 
 int nbx_create(const nbx_options* options, nbx_context** context)
@@ -362,6 +369,15 @@ int nbx_create(const nbx_options* options, nbx_context** context)
         if (o.ewald_correction != NBX_EWALD_TABLE && o.ewald_correction != NBX_EWALD_ANALYTICAL)
         {
             throw InputError("unknown Ewald correction type");
+        }
+        if (o.kernel_group_size != 0 && o.kernel_group_size != 1 && o.kernel_group_size != 2
+            && o.kernel_group_size != 4 && o.kernel_group_size != 8)
+        {
+            throw InputError("kernel_group_size must be 0, 1, 2, 4 or 8");
+        }
+        if (o.kernel_blocks_per_cu < 0 || o.kernel_blocks_per_cu > 8)
+        {
+            throw InputError("kernel_blocks_per_cu must be in 0..8");
         }
         int ndev = 0;
         check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
@@ -828,7 +844,8 @@ int nbx_construct_pairlist(nbx_context* c)
         // at once, at most four per CU; their wavefronts claim groups from a
         // counter until none are left
         const int lds      = nonbondedLdsBytes(c->ntype, c->ic.tab_n, c->elec, c->windowSlots);
-        const int perCU    = std::max(1, std::min(4, (160 * 1024) / lds));
+        const int maxPerCU = c->opt.kernel_blocks_per_cu > 0 ? c->opt.kernel_blocks_per_cu : 4;
+        const int perCU    = std::max(1, std::min(maxPerCU, (160 * 1024) / lds));
         c->numBlocks       = std::max(1, std::min((c->ngroups + kKernelWaves - 1) / kKernelWaves, perCU * c->numCU));
         c->energyPartial.ensure(2 * static_cast<size_t>(std::max(c->ngroups, 1)));
         c->groupCounter.ensure(1);

@@ -159,13 +159,15 @@ __device__ double ewaldCorrectionD(double r, const ExclusionArgs& a)
 __global__ __launch_bounds__(kExclThreads) void exclusionEnergyKernel(const ExclusionArgs a)
 {
     __shared__ double s[kExclThreads / 64];
-    double            e  = 0.0;
-    const double      L[3] = { a.box[0], a.box[1], a.box[2] };
+    double            e = 0.0;
     for (int i = blockIdx.x * kExclThreads + threadIdx.x; i < a.natoms; i += gridDim.x * kExclThreads)
     {
-        const float4 pi = a.xq[a.cells[i]];
-        const double qi = pi.w;
-        e -= a.selfCoef * qi * qi;
+        // the atom's system box (a batch keeps one per system)
+        const float* box = a.atomSystem ? a.boxes + 3 * a.atomSystem[i] : a.box;
+        const double L[3] = { box[0], box[1], box[2] };
+        const float4 pi   = a.xq[a.cells[i]];
+        const double qi   = pi.w;
+        double       ei   = -a.selfCoef * qi * qi;
         for (int k = a.exclOff[i]; k < a.exclOff[i + 1]; k++)
         {
             const int j = a.exclIdx[k];
@@ -188,14 +190,23 @@ __global__ __launch_bounds__(kExclThreads) void exclusionEnergyKernel(const Excl
                 const double qq = a.epsfacD * qi * static_cast<double>(pj.w);
                 if (a.elec == kElecRF)
                 {
-                    e += qq * (static_cast<double>(a.ic.k_rf) * r2 - static_cast<double>(a.ic.c_rf));
+                    ei += qq * (static_cast<double>(a.ic.k_rf) * r2 - static_cast<double>(a.ic.c_rf));
                 }
                 else
                 {
-                    e -= qq * ewaldCorrectionD(sqrt(r2), a);
+                    ei -= qq * ewaldCorrectionD(sqrt(r2), a);
                 }
             }
         }
+        if (a.atomEnergy)
+        {
+            a.atomEnergy[i] = ei;
+        }
+        e += ei;
+    }
+    if (a.atomEnergy)
+    {
+        return;
     }
 #pragma unroll
     for (int m = 1; m < 64; m *= 2)
@@ -215,6 +226,45 @@ __global__ __launch_bounds__(kExclThreads) void exclusionEnergyKernel(const Excl
             t += s[w];
         }
         a.partial[blockIdx.x] = t;
+    }
+}
+
+// Per-system energies of a batch: block r sums system r's group energies
+// and per-atom exclusion energies in a fixed order.
+__global__ __launch_bounds__(kFinalizeThreads) void batchFinalizeKernel(const int* __restrict__ groupOff,
+                                                                        const double* __restrict__ ePart,
+                                                                        const int* __restrict__ atomOff,
+                                                                        const double* __restrict__ atomEnergy,
+                                                                        double* __restrict__ eOut)
+{
+    __shared__ double s[2][kFinalizeThreads];
+    const int r = blockIdx.x;
+    double    v = 0.0, c = 0.0;
+    for (int g = groupOff[r] + threadIdx.x; g < groupOff[r + 1]; g += kFinalizeThreads)
+    {
+        v += ePart[2 * g];
+        c += ePart[2 * g + 1];
+    }
+    for (int i = atomOff[r] + threadIdx.x; i < atomOff[r + 1]; i += kFinalizeThreads)
+    {
+        c += atomEnergy[i];
+    }
+    s[0][threadIdx.x] = v;
+    s[1][threadIdx.x] = c;
+    __syncthreads();
+    for (int off = kFinalizeThreads / 2; off > 0; off >>= 1)
+    {
+        if (threadIdx.x < off)
+        {
+            s[0][threadIdx.x] += s[0][threadIdx.x + off];
+            s[1][threadIdx.x] += s[1][threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        eOut[2 * r]     = s[0][0];
+        eOut[2 * r + 1] = s[1][0];
     }
 }
 
@@ -431,6 +481,16 @@ void launchFinalize(hipStream_t s, int numEnergyPartials, const double* energyPa
     finalizeKernel<<<blocks, kFinalizeThreads, 0, s>>>(numEnergyPartials, energyPartial, numExclPartials, exclPartial,
                                                        numShiftPartials, fshiftPartial, shiftForces ? 1 : 0, energyOut,
                                                        fshiftOut);
+}
+
+void launchBatchFinalize(hipStream_t s, int numSystems, const int* groupOff, const double* energyPartial,
+                         const int* atomOff, const double* atomEnergy, double* energyOut)
+{
+    if (numSystems > 0)
+    {
+        batchFinalizeKernel<<<numSystems, kFinalizeThreads, 0, s>>>(groupOff, energyPartial, atomOff, atomEnergy,
+                                                                    energyOut);
+    }
 }
 
 int launchExclusionEnergy(hipStream_t s, const ExclusionArgs& a)

@@ -166,7 +166,7 @@ __global__ __launch_bounds__(kGroupThreads) void buildGroupsKernel(GroupArgs a)
     if (threadIdx.x == 0)
     {
         a.groups[g]    = make_int4(coff, coff + nchunk, uoff, U);
-        a.groupInfo[g] = make_int4(c0, (c1 - c0) | (a.groupSize << 16), 0, 0);
+        a.groupInfo[g] = make_int4(c0, (c1 - c0) | (a.groupSize << 16), 0, g);
     }
     // per i-entry: masked-first partition, slots, work list, chunks (one
     // wave per entry)

@@ -379,7 +379,7 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
         const int  ch1  = uniformInt(grp.y);
         const int  uoff = uniformInt(grp.z);
         const int  U    = uniformInt(grp.w);
-        const int4 gi   = a.groupInfo[g]; // first own cluster, own count | G << 16, shift table
+        const int4 gi   = a.groupInfo[g]; // first own cluster, own count | G << 16, shift table, energy slot
         const int  c0   = uniformInt(gi.x);
         const int  nOwn = uniformInt(gi.y) & 0xffff;
         const int  G    = uniformInt(gi.y) >> 16;
@@ -535,8 +535,10 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
             eCoul = waveSumD(eCoul);
             if (lane == 0)
             {
-                a.energyPartial[2 * g]     = eVdw;
-                a.energyPartial[2 * g + 1] = eCoul;
+                // the group's energy slot (its index in build order)
+                const int slot             = uniformInt(gi.w);
+                a.energyPartial[2 * slot]     = eVdw;
+                a.energyPartial[2 * slot + 1] = eCoul;
             }
         }
 

@@ -146,7 +146,7 @@ struct GroupArgs
     uint2*       work;       // {cj, slot << 16 | mask}, each chunk padded to whole tiles
     int4*        chunks;     // {ci, shift | self << 8 | special << 9, work_start, tiles}
     int4*        groups;     // {chunk_start, chunk_end, union_offset, union_size}
-    int4*        groupInfo;  // {first own cluster, own count | G << 16, shift-table offset, 0}
+    int4*        groupInfo;  // {first own cluster, own count | G << 16, shift-table offset, energy slot}
 };
 void launchGroupSpans(hipStream_t s, int ncluster, const int* offJ, int* maxSpan);
 void launchBuildGroups(hipStream_t s, const GroupArgs& a, bool fill);
@@ -202,9 +202,16 @@ struct ExclusionArgs
     double           selfCoef; // epsfac V(0) / 2
     const double*    tabD;     // FDV0 table in double (table flavour)
     double*          partial;  // one per block
+    // batches: the atom's system (boxes[3 * system]) and per-atom energies
+    // instead of block partials; nullptr for a single system
+    const int*       atomSystem;
+    const float*     boxes;
+    double*          atomEnergy;
 };
 constexpr int kExclMaxBlocks = 1024;
 int launchExclusionEnergy(hipStream_t s, const ExclusionArgs& a);
+void launchBatchFinalize(hipStream_t s, int numSystems, const int* groupOff, const double* energyPartial,
+                         const int* atomOff, const double* atomEnergy, double* energyOut);
 void launchVirial(hipStream_t s, int natoms, const float* x, const float* f, const double* fshift, const float4* shiftVec,
                   double* virialPartial, int numBlocks, double* virialOut);
 void launchExpandPairs(hipStream_t s, const int4* ientry, int nci, const uint2* jentry, const float4* xq,

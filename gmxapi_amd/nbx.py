@@ -56,7 +56,26 @@ class NbxLibraryMissing(NbxError, ImportError):
 class _Options(C.Structure):
     _fields_ = [("coulomb_type", C.c_int), ("ewald_correction", C.c_int), ("pairlist_cutoff", C.c_double),
                 ("coulomb_cutoff", C.c_double), ("ewald_rtol", C.c_double), ("epsilon_r", C.c_double),
-                ("epsilon_rf", C.c_double), ("ewald_potential_shift", C.c_int), ("device", C.c_int)]
+                ("epsilon_rf", C.c_double), ("ewald_potential_shift", C.c_int), ("device", C.c_int),
+                ("kernel_group_size", C.c_int), ("kernel_blocks_per_cu", C.c_int)]
+
+
+def _options(coulomb="ewald", ewald_correction="table", rlist=1.0, rcoulomb=None, ewald_rtol=1e-5, epsilon_r=1.0,
+             epsilon_rf=1.0, ewald_potential_shift=False, device=0, kernel_group_size=0, kernel_blocks_per_cu=0):
+    o = _Options()
+    library().nbx_default_options(C.byref(o))
+    o.coulomb_type = COULOMB[coulomb]
+    o.ewald_correction = EWALD_CORRECTION[ewald_correction]
+    o.pairlist_cutoff = rlist
+    o.coulomb_cutoff = 0.0 if rcoulomb is None else rcoulomb
+    o.ewald_rtol = ewald_rtol
+    o.epsilon_r = epsilon_r
+    o.epsilon_rf = epsilon_rf
+    o.ewald_potential_shift = 1 if ewald_potential_shift else 0
+    o.device = device
+    o.kernel_group_size = kernel_group_size
+    o.kernel_blocks_per_cu = kernel_blocks_per_cu
+    return o
 
 
 class _Info(C.Structure):
@@ -111,6 +130,18 @@ def library() -> C.CDLL:
     lib.nbx_get_interaction_constants.argtypes = [vp, vp]
     lib.nbx_get_work_groups.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.nbx_set_kernel_timing.argtypes = [vp, C.c_int]
+    lib.nbx_batch_create.argtypes = [vp, C.c_int, vp, vp, vp]
+    lib.nbx_batch_destroy.argtypes = [vp]
+    lib.nbx_batch_set_stream.argtypes = [vp, vp]
+    lib.nbx_batch_add_system.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp, vp]
+    lib.nbx_batch_construct_pairlists.argtypes = [vp]
+    lib.nbx_batch_get_info.argtypes = [vp, vp]
+    lib.nbx_batch_compute.argtypes = [vp, vp, vp, C.c_uint]
+    lib.nbx_batch_capture_graph.argtypes = [vp, vp, vp, C.c_uint]
+    lib.nbx_batch_launch_graph.argtypes = [vp]
+    lib.nbx_batch_get_energies.argtypes = [vp, vp]
+    lib.nbx_batch_autotune.argtypes = [vp, vp, vp, C.c_uint, C.c_int, vp]
+    lib.nbx_batch_get_system.argtypes = [vp, C.c_int, vp]
     lib.nbx_get_kernel_timing.argtypes = [vp, vp, vp]
     _lib = lib
     return lib
@@ -135,31 +166,34 @@ class Context:
 
     def __init__(self, coulomb: str = "ewald", ewald_correction: str = "table", rlist: float = 1.0,
                  rcoulomb: float | None = None, ewald_rtol: float = 1e-5, epsilon_r: float = 1.0,
-                 epsilon_rf: float = 1.0, ewald_potential_shift: bool = False, device: int = 0):
+                 epsilon_rf: float = 1.0, ewald_potential_shift: bool = False, device: int = 0,
+                 kernel_group_size: int = 0, kernel_blocks_per_cu: int = 0):
         lib = library()
         self._lib = lib
-        o = _Options()
-        lib.nbx_default_options(C.byref(o))
-        o.coulomb_type = COULOMB[coulomb]
-        o.ewald_correction = EWALD_CORRECTION[ewald_correction]
-        o.pairlist_cutoff = rlist
-        o.coulomb_cutoff = 0.0 if rcoulomb is None else rcoulomb
-        o.ewald_rtol = ewald_rtol
-        o.epsilon_r = epsilon_r
-        o.epsilon_rf = epsilon_rf
-        o.ewald_potential_shift = 1 if ewald_potential_shift else 0
-        o.device = device
+        o = _options(coulomb, ewald_correction, rlist, rcoulomb, ewald_rtol, epsilon_r, epsilon_rf,
+                     ewald_potential_shift, device, kernel_group_size, kernel_blocks_per_cu)
         h = C.c_void_p()
         rc = lib.nbx_create(C.byref(o), C.byref(h))
         if rc != NBX_SUCCESS:
             _raise(rc, "nbx_create")
         self._h = h
+        self._owner = None
         self.natoms = 0
 
+    @classmethod
+    def _view(cls, handle, owner, natoms):
+        """A non-owning Context on a handle that `owner` (a Batch) keeps alive."""
+        obj = cls.__new__(cls)
+        obj._lib = library()
+        obj._h = C.c_void_p(handle)
+        obj._owner = owner
+        obj.natoms = natoms
+        return obj
+
     def close(self):
-        if getattr(self, "_h", None):
+        if getattr(self, "_h", None) and getattr(self, "_owner", None) is None:
             self._lib.nbx_destroy(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         try:
@@ -330,6 +364,101 @@ def nblib_c6c12_matrices(c6_type, c12_type):
     m6 = np.sqrt((c6[:, None] * c6[None, :]).astype(np.float32)).astype(np.float64)
     m12 = np.sqrt((c12[:, None] * c12[None, :]).astype(np.float32)).astype(np.float64)
     return m6, m12
+
+
+class Batch:
+    """A batched ensemble (include/nbx.h nbx_batch_*): independent systems
+    sharing options and LJ parameters, computed by one kernel launch."""
+
+    def __init__(self, c6_matrix, c12_matrix, **options):
+        lib = library()
+        self._lib = lib
+        o = _options(**options)
+        c6 = np.ascontiguousarray(c6_matrix, np.float64)
+        c12 = np.ascontiguousarray(c12_matrix, np.float64)
+        h = C.c_void_p()
+        rc = lib.nbx_batch_create(C.byref(o), c6.shape[0], _ptr(c6), _ptr(c12), C.byref(h))
+        if rc != NBX_SUCCESS:
+            _raise(rc, "nbx_batch_create")
+        self._h = h
+        self.natoms = []
+
+    def _check(self, rc, what):
+        if rc != NBX_SUCCESS:
+            _raise(rc, what)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.nbx_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_stream(self, stream_ptr: int):
+        self._check(self._lib.nbx_batch_set_stream(self._h, C.c_void_p(stream_ptr)), "nbx_batch_set_stream")
+
+    def add_system(self, types, charges, excl_off, excl_idx, box, x) -> int:
+        types = np.ascontiguousarray(types, np.int32)
+        charges = np.ascontiguousarray(charges, np.float32)
+        eo = np.ascontiguousarray(excl_off, np.int32)
+        ei = np.ascontiguousarray(excl_idx, np.int32)
+        box = np.ascontiguousarray(box, np.float32)
+        x = np.ascontiguousarray(x, np.float32).reshape(-1)
+        idx = C.c_int(-1)
+        self._check(self._lib.nbx_batch_add_system(self._h, len(types), _ptr(types), _ptr(charges), _ptr(eo),
+                                                   _ptr(ei), _ptr(box), _ptr(x), C.byref(idx)),
+                    "nbx_batch_add_system")
+        self.natoms.append(len(types))
+        return idx.value
+
+    def construct_pairlists(self):
+        self._check(self._lib.nbx_batch_construct_pairlists(self._h), "nbx_batch_construct_pairlists")
+
+    def info(self) -> dict:
+        out = np.zeros(7, np.int64)
+        self._check(self._lib.nbx_batch_get_info(self._h, _ptr(out)), "nbx_batch_get_info")
+        keys = ["num_systems", "num_atoms", "num_clusters", "num_j_entries", "num_groups", "group_size",
+                "blocks_per_cu"]
+        return dict(zip(keys, (int(v) for v in out)))
+
+    def compute_device(self, x_ptr: int, f_ptr: int, flags: int = COMPUTE_FORCES):
+        self._check(self._lib.nbx_batch_compute(self._h, C.c_void_p(x_ptr), C.c_void_p(f_ptr), flags),
+                    "nbx_batch_compute")
+
+    def capture_graph(self, x_ptr: int, f_ptr: int, flags: int = COMPUTE_FORCES):
+        self._check(self._lib.nbx_batch_capture_graph(self._h, C.c_void_p(x_ptr), C.c_void_p(f_ptr), flags),
+                    "nbx_batch_capture_graph")
+
+    def launch_graph(self):
+        self._check(self._lib.nbx_batch_launch_graph(self._h), "nbx_batch_launch_graph")
+
+    def energies(self) -> np.ndarray:
+        out = np.zeros((len(self.natoms), 2))
+        self._check(self._lib.nbx_batch_get_energies(self._h, _ptr(out)), "nbx_batch_get_energies")
+        return out
+
+    def autotune(self, x_ptr: int, f_ptr: int, flags: int = COMPUTE_FORCES, repeats: int = 5):
+        out = np.zeros(3, np.int32)
+        self._check(self._lib.nbx_batch_autotune(self._h, C.c_void_p(x_ptr), C.c_void_p(f_ptr), flags, repeats,
+                                                 _ptr(out)), "nbx_batch_autotune")
+        return dict(group_size=int(out[0]), blocks_per_cu=int(out[1]), heavy_first=bool(out[2]))
+
+    def system(self, index: int) -> "Context":
+        h = C.c_void_p()
+        self._check(self._lib.nbx_batch_get_system(self._h, index, C.byref(h)), "nbx_batch_get_system")
+        return Context._view(h.value, self, self.natoms[index])
+
+
+def batch_for(systems, **options) -> Batch:
+    """A Batch holding gmxapi_amd.systems.System objects that share LJ
+    parameters (set up as nblib would set up each one)."""
+    m6, m12 = nblib_c6c12_matrices(systems[0].c6, systems[0].c12)
+    b = Batch(m6, m12, rlist=systems[0].rlist, rcoulomb=systems[0].rcoulomb, **options)
+    for s in systems:
+        b.add_system(s.types, s.charges, s.excl_off, s.excl_idx, s.box, s.x)
+    b.construct_pairlists()
+    return b
 
 
 def context_for(system, coulomb="ewald", ewald_correction="table", device=0, potential_shift=False) -> Context:

@@ -244,6 +244,18 @@ def small_water_box(n_molecules: int = 216, seed: int = WATER_SEED + 7, box_leng
     return water_box(n_molecules, L, seed=seed)
 
 
+def ensemble(n_replicas: int = 64, seed: int = WATER_SEED + 3, min_molecules: int = 1000,
+             max_molecules: int = 16000, density: float = 33.4) -> list[System]:
+    """The batched-ensemble workload: water boxes built like the 3,000-water
+    box, with molecule counts drawn log-uniformly in [min, max] (3k-48k
+    atoms), each box cubic at `density` molecules/nm^3; replica r uses seed
+    seed + 1000 r."""
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(math.log(min_molecules), math.log(max_molecules), size=n_replicas))
+    counts = np.round(counts).astype(int)
+    return [water_box(int(n), None, seed=seed + 1000 * r, density=density) for r, n in enumerate(counts)]
+
+
 def config(index: int) -> System:
     """The benchmark workloads: 0 = 3,000 waters in 4.48 nm, 1 = 96,000
     waters in 14.2 nm, 2 = 150,000-atom chains in 11.45 nm."""

@@ -73,6 +73,10 @@ typedef struct nbx_options
     double epsilon_rf;            /* 1 */
     int    ewald_potential_shift; /* 0: sh_ewald = 0 as nblib; 1: mdrun's shift */
     int    device;                /* HIP device ordinal */
+    int    kernel_group_size;     /* i-clusters per kernel work group: 0 = largest
+                                     that fits LDS, else 1, 2, 4 or 8 (tuning) */
+    int    kernel_blocks_per_cu;  /* resident kernel workgroups per CU: 0 = as many
+                                     as LDS allows up to 4 (tuning) */
 } nbx_options;
 
 typedef struct nbx_pairlist_info
@@ -210,6 +214,63 @@ int nbx_get_kernel_timing(nbx_context* context, double* average_ms, int* launche
 /* beta, table scale, table size, epsfac, k_rf, c_rf, sh_ewald, LJ
  * dispersion and repulsion potential shifts (interaction_const_t). */
 int nbx_get_interaction_constants(nbx_context* context, double* out9);
+
+/* ------------------------------------------------------------------------
+ * Batched ensemble: many independent systems (replicas of uneven size) that
+ * share the options and LJ parameters, computed by one launch of the pair
+ * kernel over all their work groups.  The reference runs an ensemble as one
+ * simulation per member (gmxapi's ensemble operations,
+ * python_packaging/src/gmxapi/operation.py, simulation/mdrun.py:162-200);
+ * nblib computes one system per ForceCalculator.  Coordinates and forces of
+ * the whole batch are device arrays with the systems' atoms concatenated in
+ * the order they were added.  Shift forces and the virial are per-system
+ * calls (nbx_compute on a single context); the batch computes forces and
+ * energies.
+ * ------------------------------------------------------------------------ */
+typedef struct nbx_batch nbx_batch;
+
+/* c6 and c12 as for nbx_set_nonbonded_parameters. */
+int nbx_batch_create(const nbx_options* options, int num_types, const double* c6, const double* c12,
+                     nbx_batch** batch);
+int nbx_batch_destroy(nbx_batch* batch);
+int nbx_batch_set_stream(nbx_batch* batch, void* stream);
+
+/* Adds one system (host arrays, as nbx_set_topology and nbx_put_on_grid
+ * take them) and returns its index. */
+int nbx_batch_add_system(nbx_batch* batch, int num_atoms, const int* types, const float* charges,
+                         const int* exclusion_offsets, const int* exclusion_indices, const float* box,
+                         const float* x, int* index);
+
+/* Grids and searches every system (as nbx_put_on_grid + nbx_construct_pairlist)
+ * and concatenates their kernel work into one launch. */
+int nbx_batch_construct_pairlists(nbx_batch* batch);
+
+/* {systems, atoms, clusters, cluster pairs (j-entries), kernel work groups,
+ * group size, workgroups per CU} */
+int nbx_batch_get_info(nbx_batch* batch, int64_t* info7);
+
+/* Forces (and with NBX_COMPUTE_ENERGY energies) of every system: x and f are
+ * device arrays of 3 * total atoms floats; f is overwritten.  Queued on the
+ * batch's stream. */
+int nbx_batch_compute(nbx_batch* batch, const float* x, float* f, unsigned flags);
+
+/* Records one nbx_batch_compute with these arguments into a HIP graph
+ * (replaces any earlier one), and replays it. */
+int nbx_batch_capture_graph(nbx_batch* batch, const float* x, float* f, unsigned flags);
+int nbx_batch_launch_graph(nbx_batch* batch);
+
+/* {LJ, Coulomb} per system (2 * systems doubles) of the last compute with
+ * energies.  Synchronises. */
+int nbx_batch_get_energies(nbx_batch* batch, double* energies);
+
+/* Times the candidate kernel schedules (work-group size, workgroups per CU,
+ * group order) on this batch with `repeats` launches each and keeps the
+ * fastest; chosen[3] = {group size, workgroups per CU, heavy groups first}. */
+int nbx_batch_autotune(nbx_batch* batch, const float* x, float* f, unsigned flags, int repeats, int* chosen3);
+
+/* The per-system contexts (for sequential reference launches and checks);
+ * owned by the batch. */
+int nbx_batch_get_system(nbx_batch* batch, int index, nbx_context** context);
 
 #ifdef __cplusplus
 }

@@ -107,3 +107,15 @@ def test_list_invariants():
                         continue
                     ia, ja = ai[c * 4 + i], ai[cj[k] * 4 + j]
                     assert (ia, ja) in excl
+
+
+def test_ensemble_generator():
+    # the batched-ensemble workload: counts log-uniform in [1000, 16000],
+    # boxes at 33.4 molecules/nm^3, all larger than twice the cut-off
+    from gmxapi_amd import systems as S
+    reps = S.ensemble(64)
+    n = np.array([s.natoms // 3 for s in reps])
+    assert len(reps) == 64 and n.min() >= 1000 and n.max() <= 16000
+    for s in reps[:4]:
+        assert abs(s.natoms / 3 / s.box[0] ** 3 - 33.4) < 1e-6 * 33.4
+        assert s.box[0] > 2 * s.rlist
