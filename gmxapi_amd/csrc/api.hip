@@ -228,22 +228,23 @@ void buildWorkGroups(nbx_context* c)
     check(hipMemcpyAsync(spans, c->groupStats.p, sizeof(spans), hipMemcpyDeviceToHost, c->stream), "read spans");
     check(hipStreamSynchronize(c->stream), "group sync");
     c->jentryTmp.ensure(c->ncj);
-    // the largest group size whose j-force windows (one per wavefront) fit
-    // the LDS budget of four resident workgroups per CU; failing that, two
+    // the largest group size (or the chosen one) whose j-force windows, one
+    // per wavefront, fit the workgroup's LDS budget: larger groups share more
+    // j-clusters per flush, which pays more than a third resident workgroup
     const int forced = c->opt.kernel_group_size;
-    for (int k = 0; k < 8; k++)
+    for (int k = 0; k < 4; k++)
     {
-        const int G      = kGroupSizeMax >> (k & 3);
-        const int budget = k < 4 ? kKernelLdsBudget : kKernelLdsBudgetMax;
-        if (forced > 0 && (G != forced || k < 4))
-        {
-            continue; // a chosen size may use the larger budget
-        }
-        if (spans[k & 3] > kGroupMaxJ && G > 1)
+        const int G      = kGroupSizeMax >> k;
+        const int budget = kKernelLdsBudget;
+        if (forced > 0 && G != forced)
         {
             continue;
         }
-        if (spans[k & 3] > kGroupMaxJ)
+        if (spans[k] > kGroupMaxJ && G > 1)
+        {
+            continue;
+        }
+        if (spans[k] > kGroupMaxJ)
         {
             throw InputError("a grid cluster has more than " + std::to_string(kGroupMaxJ)
                              + " cluster-pair entries: the pair-list cut-off is too long for this kernel");
@@ -281,11 +282,37 @@ void buildWorkGroups(nbx_context* c)
         check(hipMemcpyAsync(st, c->groupStats.p + 4, sizeof(st), hipMemcpyDeviceToHost, c->stream), "read stats");
         check(hipStreamSynchronize(c->stream), "group sync");
         const int maxUnion = st[0], overflow = st[1], unionTotal = st[2], chunkTotal = st[3], workTotal = st[4];
-        // own clusters, the union, and the padding entries' spare slot
-        const int slots = G + maxUnion + 1;
-        if (overflow || nonbondedLdsBytes(c->ntype, c->ic.tab_n, c->elec, slots) > budget)
+        if (overflow)
         {
             continue;
+        }
+        // window capacity: own clusters, the union, and the padding entries'
+        // spare slot; when the largest union does not fit the budget, the
+        // window holds the first union clusters of each group and the rest
+        // take direct global atomics, which is accepted while that is at most
+        // 2% of the union entries
+        int slots = G + maxUnion + 1;
+        if (nonbondedLdsBytes(c->ntype, c->ic.tab_n, c->elec, slots) > budget)
+        {
+            while (slots > G + 2 && nonbondedLdsBytes(c->ntype, c->ic.tab_n, c->elec, slots) > budget)
+            {
+                slots--;
+            }
+            if (nonbondedLdsBytes(c->ntype, c->ic.tab_n, c->elec, slots) > budget)
+            {
+                continue;
+            }
+            std::vector<int> counts(ng);
+            check(hipMemcpy(counts.data(), c->unionCount.p, sizeof(int) * ng, hipMemcpyDeviceToHost), "read unions");
+            long long spilled = 0;
+            for (int u : counts)
+            {
+                spilled += std::max(0, u - (slots - 1 - G));
+            }
+            if (spilled * 50 > unionTotal)
+            {
+                continue;
+            }
         }
         c->unionList.ensure(std::max(unionTotal, 1));
         c->chunks.ensure(std::max(chunkTotal, 1));
@@ -300,6 +327,7 @@ void buildWorkGroups(nbx_context* c)
         ga.chunks    = c->chunks.p;
         ga.groups    = c->groups.p;
         ga.groupInfo = c->groupInfo.p;
+        ga.windowCap = slots;
         launchBuildGroups(c->stream, ga, true);
         check(hipGetLastError(), "group kernels");
         c->jentry.swap(c->jentryTmp);
@@ -841,14 +869,18 @@ int nbx_construct_pairlist(nbx_context* c)
         check(hipGetLastError(), "search kernels");
         buildWorkGroups(c);
         // kernel grid: the workgroups (4 wavefronts each) that can be resident
-        // at once, at most four per CU; their wavefronts claim groups from a
-        // counter until none are left
+        // at once, at most four per CU; every wavefront gets a contiguous run
+        // of chunks of about equal cost
         const int lds      = nonbondedLdsBytes(c->ntype, c->ic.tab_n, c->elec, c->windowSlots);
         const int maxPerCU = c->opt.kernel_blocks_per_cu > 0 ? c->opt.kernel_blocks_per_cu : 4;
         const int perCU    = std::max(1, std::min(maxPerCU, (160 * 1024) / lds));
-        c->numBlocks       = std::max(1, std::min((c->ngroups + kKernelWaves - 1) / kKernelWaves, perCU * c->numCU));
-        c->energyPartial.ensure(2 * static_cast<size_t>(std::max(c->ngroups, 1)));
-        c->groupCounter.ensure(1);
+        c->numBlocks       = std::max(1, std::min((c->nchunk + kKernelWaves - 1) / kKernelWaves, perCU * c->numCU));
+        const int numWaves = c->numBlocks * kKernelWaves;
+        c->waveStart.ensure(numWaves + 1);
+        launchWaveSchedule(c->stream, numWaves, c->nchunk, c->chunks.p, c->nwork, kChunkWeight, c->ngroups,
+                           c->groups.p, c->waveStart.p);
+        check(hipGetLastError(), "schedule kernel");
+        c->energyPartial.ensure(2 * static_cast<size_t>(c->ngroups + numWaves));
         c->exclPartial.ensure(kExclMaxBlocks);
         c->fshiftPartial.ensure(static_cast<size_t>(c->numBlocks) * kNumShifts * 3);
         c->energy.ensure(2);
@@ -886,7 +918,9 @@ int nbx_dispatch_kernel(nbx_context* c, unsigned flags)
         setDevice(c);
         const bool energy = (flags & NBX_COMPUTE_ENERGY) != 0;
         const bool shift  = (flags & (NBX_COMPUTE_SHIFT_FORCES | NBX_COMPUTE_VIRIAL)) != 0;
-        launchClearF4(c->stream, c->fnb.p, c->ncluster * kClusterSize, c->groupCounter.p);
+        const int numSlots = c->ngroups + c->numBlocks * kKernelWaves;
+        launchClearF4(c->stream, c->fnb.p, c->ncluster * kClusterSize, energy ? c->energyPartial.p : nullptr,
+                      2 * numSlots);
         KernelArgs a{};
         a.chunks        = c->chunks.p;
         a.groups        = c->groups.p;
@@ -904,7 +938,7 @@ int nbx_dispatch_kernel(nbx_context* c, unsigned flags)
         a.tabFDV0       = c->tabFDV0.p;
         a.shiftVec      = c->shiftVec.p;
         a.ic            = c->ic;
-        a.groupCounter  = c->groupCounter.p;
+        a.waveStart     = c->waveStart.p;
         a.fnb           = c->fnb.p;
         a.energyPartial = c->energyPartial.p;
         a.fshiftPartial = c->fshiftPartial.p;
@@ -942,7 +976,7 @@ int nbx_dispatch_kernel(nbx_context* c, unsigned flags)
         }
         if (energy || shift)
         {
-            launchFinalize(c->stream, energy ? c->ngroups : 0, c->energyPartial.p, numExcl, c->exclPartial.p,
+            launchFinalize(c->stream, energy ? numSlots : 0, c->energyPartial.p, numExcl, c->exclPartial.p,
                            c->numBlocks, c->fshiftPartial.p, shift, c->energy.p, c->fshift.p);
         }
         check(hipGetLastError(), "nonbonded kernel");

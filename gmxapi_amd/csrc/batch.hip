@@ -8,8 +8,10 @@
 // offsets, all padding entries point at one shared far-away cluster, and
 // every work group carries its system's shift-vector table and energy slot
 // in its groupInfo record.  One launch then covers every group of every
-// system; the wavefronts claim groups from one counter, so small systems do
-// not leave the GPU idle.  Energies are summed per system in a fixed order.
+// system: the kernel's static schedule cuts the concatenated chunk list into
+// runs of equal cost, so small systems do not leave the GPU idle and one
+// wavefront may pass from one system to the next.  Energies are summed per
+// system in a fixed order.
 // A step (coordinate conversion, force clearing, pair kernel, exclusion
 // energies, per-system sums, force reduction) is launch-bound for small
 // systems and can be recorded once into a HIP graph and replayed.
@@ -32,13 +34,13 @@ struct nbx_batch
     hipStream_t               ownStream = nullptr, stream = nullptr;
 
     bool built = false;
-    int  totAtoms = 0, totClusters = 0, ngroups = 0, windowSlots = 0, numBlocks = 0;
-    int  groupSize = 0, blocksPerCU = 0, heavyFirst = 0;
+    int  totAtoms = 0, totClusters = 0, ngroups = 0, nchunk = 0, nwork = 0, windowSlots = 0, numBlocks = 0;
+    int  groupSize = 0, blocksPerCU = 0, chunkWeight = kChunkWeight;
     long long ncj = 0;
 
     DevBuf<float4> xq, fnb, shiftVec;
-    DevBuf<int>    nbtype, atomIndices, cells, exclOff, exclIdx, unionList, atomSystem, groupOff, atomOff,
-            groupCounter;
+    DevBuf<int>    nbtype, atomIndices, cells, exclOff, exclIdx, unionList, atomSystem, groupOff, atomOff, slotOff;
+    DevBuf<int2>   waveStart;
     DevBuf<float>  boxes;
     DevBuf<uint2>  work;
     DevBuf<int4>   chunks, groups, groupInfo;
@@ -105,9 +107,31 @@ void upload(DevBuf<T>& d, const std::vector<T>& h)
     }
 }
 
+// The kernel grid (blocksPerCU workgroups per CU, at most what LDS allows)
+// and its static schedule, with chunkWeight work entries per chunk.
+void scheduleBatch(nbx_batch* b, int blocksPerCU, int chunkWeight)
+{
+    nbx_context* c0    = b->sys[0];
+    const int    lds   = nonbondedLdsBytes(c0->ntype, c0->ic.tab_n, c0->elec, b->windowSlots);
+    const int    maxBpc = blocksPerCU > 0 ? blocksPerCU : 4;
+    b->blocksPerCU     = std::max(1, std::min(maxBpc, (160 * 1024) / lds));
+    b->numBlocks       = std::max(1, std::min((b->nchunk + kKernelWaves - 1) / kKernelWaves, b->blocksPerCU * b->numCU));
+    b->chunkWeight     = chunkWeight;
+    const int numWaves = b->numBlocks * kKernelWaves;
+    const int R        = static_cast<int>(b->sys.size());
+    b->waveStart.ensure(numWaves + 1);
+    b->slotOff.ensure(R + 1);
+    b->energyPartial.ensure(2 * static_cast<size_t>(b->ngroups + numWaves));
+    launchWaveSchedule(b->stream, numWaves, b->nchunk, b->chunks.p, b->nwork, chunkWeight, b->ngroups, b->groups.p,
+                       b->waveStart.p);
+    launchSystemSlots(b->stream, R, b->groupOff.p, b->ngroups, b->groups.p, numWaves, b->waveStart.p, b->slotOff.p);
+    check(hipGetLastError(), "schedule kernels");
+    check(hipStreamSynchronize(b->stream), "schedule sync");
+}
+
 // Regroups every system with group size G (0 = each system's own choice)
 // and concatenates the kernel inputs.
-void buildBatch(nbx_batch* b, int G, int blocksPerCU, int heavyFirst)
+void buildBatch(nbx_batch* b, int G, int blocksPerCU, int chunkWeight)
 {
     b->dropGraph();
     const int R = static_cast<int>(b->sys.size());
@@ -212,31 +236,6 @@ void buildBatch(nbx_batch* b, int G, int blocksPerCU, int heavyFirst)
         hAtomOff.push_back(atomOffset);
     }
 
-    // optionally the heaviest groups (most tiles) first: the last claims of
-    // the dynamic schedule are then the cheap ones
-    if (heavyFirst)
-    {
-        std::vector<long long> cost(hGroups.size(), 0);
-        for (size_t g = 0; g < hGroups.size(); g++)
-        {
-            for (int k = hGroups[g].x; k < hGroups[g].y; k++)
-            {
-                cost[g] += hChunks[k].w;
-            }
-        }
-        std::vector<int> order(hGroups.size());
-        std::iota(order.begin(), order.end(), 0);
-        std::stable_sort(order.begin(), order.end(), [&](int p, int q) { return cost[p] > cost[q]; });
-        std::vector<int4> g2, i2;
-        for (int k : order)
-        {
-            g2.push_back(hGroups[k]);
-            i2.push_back(hInfo[k]);
-        }
-        hGroups.swap(g2);
-        hInfo.swap(i2);
-    }
-
     upload(b->xq, hXq);
     upload(b->nbtype, hType);
     upload(b->atomIndices, hAi);
@@ -254,25 +253,20 @@ void buildBatch(nbx_batch* b, int G, int blocksPerCU, int heavyFirst)
     upload(b->atomOff, hAtomOff);
     upload(b->boxes, b->boxHost);
     b->fnb.ensure(4 * static_cast<size_t>(totClusters) + 4);
-    b->energyPartial.ensure(2 * hGroups.size());
     b->atomEnergy.ensure(std::max(atomOffset, 1));
     b->energy.ensure(2 * static_cast<size_t>(R));
-    b->groupCounter.ensure(1);
 
-    nbx_context* c0   = b->sys[0];
-    b->totAtoms       = atomOffset;
-    b->totClusters    = totClusters;
-    b->ngroups        = static_cast<int>(hGroups.size());
-    b->windowSlots    = maxSlots;
-    b->ncj            = ncj;
-    b->groupSize      = G > 0 ? G : c0->groupSize;
-    b->heavyFirst     = heavyFirst;
-    const int lds     = nonbondedLdsBytes(c0->ntype, c0->ic.tab_n, c0->elec, maxSlots);
-    const int maxBpc  = blocksPerCU > 0 ? blocksPerCU : 4;
-    b->blocksPerCU    = std::max(1, std::min(maxBpc, (160 * 1024) / lds));
-    b->numBlocks      = std::max(1, std::min((b->ngroups + kKernelWaves - 1) / kKernelWaves, b->blocksPerCU * b->numCU));
-    b->built          = true;
-    b->haveEnergy     = false;
+    b->totAtoms    = atomOffset;
+    b->totClusters = totClusters;
+    b->ngroups     = static_cast<int>(hGroups.size());
+    b->nchunk      = static_cast<int>(hChunks.size());
+    b->nwork       = static_cast<int>(hWork.size());
+    b->windowSlots = maxSlots;
+    b->ncj         = ncj;
+    b->groupSize   = G > 0 ? G : b->sys[0]->groupSize;
+    scheduleBatch(b, blocksPerCU, chunkWeight);
+    b->built      = true;
+    b->haveEnergy = false;
 }
 
 void enqueueCompute(nbx_batch* b, const float* x, float* f, unsigned flags)
@@ -280,7 +274,8 @@ void enqueueCompute(nbx_batch* b, const float* x, float* f, unsigned flags)
     nbx_context* c0     = b->sys[0];
     const bool   energy = (flags & NBX_COMPUTE_ENERGY) != 0;
     launchConvertX(b->stream, 4 * b->totClusters, b->atomIndices.p, x, b->xq.p);
-    launchClearF4(b->stream, b->fnb.p, 4 * b->totClusters, b->groupCounter.p);
+    const int numSlots = b->ngroups + b->numBlocks * kKernelWaves;
+    launchClearF4(b->stream, b->fnb.p, 4 * b->totClusters, energy ? b->energyPartial.p : nullptr, 2 * numSlots);
     KernelArgs a{};
     a.chunks       = b->chunks.p;
     a.groups       = b->groups.p;
@@ -298,7 +293,7 @@ void enqueueCompute(nbx_batch* b, const float* x, float* f, unsigned flags)
     a.tabFDV0      = c0->tabFDV0.p;
     a.shiftVec     = b->shiftVec.p;
     a.ic           = c0->ic;
-    a.groupCounter = b->groupCounter.p;
+    a.waveStart    = b->waveStart.p;
     a.fnb          = b->fnb.p;
     a.energyPartial = b->energyPartial.p;
     a.fshiftPartial = nullptr;
@@ -321,7 +316,7 @@ void enqueueCompute(nbx_batch* b, const float* x, float* f, unsigned flags)
         e.boxes      = b->boxes.p;
         e.atomEnergy = b->atomEnergy.p;
         launchExclusionEnergy(b->stream, e);
-        launchBatchFinalize(b->stream, static_cast<int>(b->sys.size()), b->groupOff.p, b->energyPartial.p,
+        launchBatchFinalize(b->stream, static_cast<int>(b->sys.size()), b->slotOff.p, b->energyPartial.p,
                             b->atomOff.p, b->atomEnergy.p, b->energy.p);
     }
     launchReduceF(b->stream, b->totAtoms, b->cells.p, b->fnb.p, f);
@@ -452,7 +447,7 @@ int nbx_batch_construct_pairlists(nbx_batch* b)
     {
         return fail(NBX_ERROR_INPUT, "NULL batch");
     }
-    return guarded([&] { buildBatch(b, b->opt.kernel_group_size, b->opt.kernel_blocks_per_cu, 0); });
+    return guarded([&] { buildBatch(b, b->opt.kernel_group_size, b->opt.kernel_blocks_per_cu, kChunkWeight); });
 }
 
 int nbx_batch_get_info(nbx_batch* b, int64_t* info)
@@ -559,24 +554,23 @@ int nbx_batch_autotune(nbx_batch* b, const float* x, float* f, unsigned flags, i
         check(hipEventCreate(&e0), "hipEventCreate");
         check(hipEventCreate(&e1), "hipEventCreate");
         float best = -1.0f;
-        int   bestG = 0, bestBpc = 0, bestOrder = 0;
+        int   bestG = 0, bestBpc = 0, bestWeight = 0;
         for (int G : { 8, 4, 2, 1 })
         {
-            for (int order = 0; order < 2; order++)
+            try
             {
-                try
+                buildBatch(b, G, 4, kChunkWeight);
+            }
+            catch (const InputError&)
+            {
+                continue; // this group size does not fit: try the next
+            }
+            const int fitBpc = b->blocksPerCU;
+            for (int bpc = fitBpc; bpc >= 2; bpc--)
+            {
+                for (int weight : { kChunkWeight, 4 * kChunkWeight })
                 {
-                    buildBatch(b, G, 4, order);
-                }
-                catch (const InputError&)
-                {
-                    break; // this group size does not fit: try the next
-                }
-                const int fitBpc = b->blocksPerCU;
-                for (int bpc = fitBpc; bpc >= 2; bpc--)
-                {
-                    b->blocksPerCU = bpc;
-                    b->numBlocks   = std::max(1, std::min((b->ngroups + kKernelWaves - 1) / kKernelWaves, bpc * b->numCU));
+                    scheduleBatch(b, bpc, weight);
                     enqueueCompute(b, x, f, flags); // warm-up
                     check(hipEventRecord(e0, b->stream), "hipEventRecord");
                     for (int k = 0; k < repeats; k++)
@@ -589,10 +583,10 @@ int nbx_batch_autotune(nbx_batch* b, const float* x, float* f, unsigned flags, i
                     check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
                     if (best < 0 || ms < best)
                     {
-                        best      = ms;
-                        bestG     = G;
-                        bestBpc   = bpc;
-                        bestOrder = order;
+                        best       = ms;
+                        bestG      = G;
+                        bestBpc    = b->blocksPerCU;
+                        bestWeight = weight;
                     }
                 }
             }
@@ -603,12 +597,12 @@ int nbx_batch_autotune(nbx_batch* b, const float* x, float* f, unsigned flags, i
         {
             throw InputError("no kernel schedule fits this batch");
         }
-        buildBatch(b, bestG, bestBpc, bestOrder);
+        buildBatch(b, bestG, bestBpc, bestWeight);
         if (chosen)
         {
             chosen[0] = bestG;
             chosen[1] = b->blocksPerCU;
-            chosen[2] = bestOrder;
+            chosen[2] = bestWeight;
         }
     });
 }

@@ -31,16 +31,16 @@ namespace
 
 // also resets the kernel's work-group counter (a fill kernel rather than a
 // memset, so that the step can be captured in a graph)
-__global__ void clearF4Kernel(float4* __restrict__ p, int n, int* __restrict__ counter)
+__global__ void clearF4Kernel(float4* __restrict__ p, int n, double* __restrict__ d, int nd)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n)
     {
         p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
-    if (counter && i == 0)
+    if (i < nd)
     {
-        *counter = 0;
+        d[i] = 0.0;
     }
 }
 
@@ -231,7 +231,7 @@ __global__ __launch_bounds__(kExclThreads) void exclusionEnergyKernel(const Excl
 
 // Per-system energies of a batch: block r sums system r's group energies
 // and per-atom exclusion energies in a fixed order.
-__global__ __launch_bounds__(kFinalizeThreads) void batchFinalizeKernel(const int* __restrict__ groupOff,
+__global__ __launch_bounds__(kFinalizeThreads) void batchFinalizeKernel(const int* __restrict__ slotOff,
                                                                         const double* __restrict__ ePart,
                                                                         const int* __restrict__ atomOff,
                                                                         const double* __restrict__ atomEnergy,
@@ -240,7 +240,7 @@ __global__ __launch_bounds__(kFinalizeThreads) void batchFinalizeKernel(const in
     __shared__ double s[2][kFinalizeThreads];
     const int r = blockIdx.x;
     double    v = 0.0, c = 0.0;
-    for (int g = groupOff[r] + threadIdx.x; g < groupOff[r + 1]; g += kFinalizeThreads)
+    for (int g = slotOff[r] + threadIdx.x; g < slotOff[r + 1]; g += kFinalizeThreads)
     {
         v += ePart[2 * g];
         c += ePart[2 * g + 1];
@@ -449,11 +449,12 @@ __global__ void expandPairsKernel(const int4* __restrict__ ientry, int nci, cons
 
 } // namespace
 
-void launchClearF4(hipStream_t s, float4* p, int n, int* counter)
+void launchClearF4(hipStream_t s, float4* p, int n, double* d, int nd)
 {
-    if (n > 0)
+    const int m = std::max(n, d ? nd : 0);
+    if (m > 0)
     {
-        clearF4Kernel<<<(n + 255) / 256, 256, 0, s>>>(p, n, counter);
+        clearF4Kernel<<<(m + 255) / 256, 256, 0, s>>>(p, n, d, d ? nd : 0);
     }
 }
 
@@ -483,12 +484,12 @@ void launchFinalize(hipStream_t s, int numEnergyPartials, const double* energyPa
                                                        fshiftOut);
 }
 
-void launchBatchFinalize(hipStream_t s, int numSystems, const int* groupOff, const double* energyPartial,
+void launchBatchFinalize(hipStream_t s, int numSystems, const int* slotOff, const double* energyPartial,
                          const int* atomOff, const double* atomEnergy, double* energyOut)
 {
     if (numSystems > 0)
     {
-        batchFinalizeKernel<<<numSystems, kFinalizeThreads, 0, s>>>(groupOff, energyPartial, atomOff, atomEnergy,
+        batchFinalizeKernel<<<numSystems, kFinalizeThreads, 0, s>>>(slotOff, energyPartial, atomOff, atomEnergy,
                                                                     energyOut);
     }
 }

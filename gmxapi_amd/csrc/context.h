@@ -169,7 +169,7 @@ struct nbx_context
     // outputs
     int             numBlocks = 0;
     nbx::DevBuf<double>  energyPartial, energy, fshift, virialPartial, virial;
-    nbx::DevBuf<int>     groupCounter;
+    nbx::DevBuf<int2>    waveStart; // static schedule: each wavefront's first chunk and its group
     nbx::DevBuf<double>  exclPartial;
     nbx::DevBuf<float>   fshiftPartial, fOut;
     bool            haveEnergy = false, haveShift = false, haveVirial = false;

@@ -134,6 +134,8 @@ __global__ __launch_bounds__(kGroupThreads) void buildGroupsKernel(GroupArgs a)
     }
     __syncthreads();
     const int U      = sUnion;
+    // union clusters that get a window slot (the rest overflow)
+    const int Uw     = kFill ? min(U, a.windowCap - 1 - a.groupSize) : U;
     const int nchunk = sChunks;
     if (!kFill)
     {
@@ -165,7 +167,7 @@ __global__ __launch_bounds__(kGroupThreads) void buildGroupsKernel(GroupArgs a)
     const int coff = a.chunkOff[g];
     if (threadIdx.x == 0)
     {
-        a.groups[g]    = make_int4(coff, coff + nchunk, uoff, U);
+        a.groups[g]    = make_int4(coff, coff + nchunk, uoff, Uw);
         a.groupInfo[g] = make_int4(c0, (c1 - c0) | (a.groupSize << 16), 0, g);
     }
     // per i-entry: masked-first partition, slots, work list, chunks (one
@@ -221,7 +223,7 @@ __global__ __launch_bounds__(kGroupThreads) void buildGroupsKernel(GroupArgs a)
                                 r = m;
                             }
                         }
-                        slot = a.groupSize + l;
+                        slot = l < Uw ? a.groupSize + l : static_cast<int>(kSlotOverflow);
                     }
                     a.jentryOut[js + written + before] = ent;
                     a.work[wb + written + before] =
@@ -240,7 +242,7 @@ __global__ __launch_bounds__(kGroupThreads) void buildGroupsKernel(GroupArgs a)
         if (n + lane < npad)
         {
             a.work[wb + n + lane] = make_uint2(static_cast<unsigned>(a.ncluster),
-                                               (static_cast<unsigned>(a.groupSize + U) << 16) | 0xffffu);
+                                               (static_cast<unsigned>(a.groupSize + Uw) << 16) | 0xffffu);
         }
         if (lane == 0)
         {
@@ -296,7 +298,100 @@ __global__ void maxReduceKernel(int n, const int* __restrict__ v, int* __restric
     }
 }
 
+__global__ void waveScheduleKernel(int numWaves, int nchunk, const int4* __restrict__ chunks, int nwork,
+                                   int chunkWeight, int ngroups, const int4* __restrict__ groups, int2* __restrict__ out)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w > numWaves)
+    {
+        return;
+    }
+    // cost before chunk k: its work offset plus chunkWeight per earlier chunk
+    const long long total  = static_cast<long long>(nwork) + static_cast<long long>(chunkWeight) * nchunk;
+    const long long target = total * w / numWaves;
+    int             lo = 0, hi = nchunk;
+    while (lo < hi)
+    {
+        const int m = (lo + hi) / 2;
+        if (chunks[m].z + static_cast<long long>(chunkWeight) * m < target)
+        {
+            lo = m + 1;
+        }
+        else
+        {
+            hi = m;
+        }
+    }
+    if (w == numWaves)
+    {
+        lo = nchunk;
+    }
+    // the group holding chunk lo: the first whose chunk range ends after it
+    int glo = 0, ghi = ngroups;
+    while (glo < ghi)
+    {
+        const int m = (glo + ghi) / 2;
+        if (groups[m].y <= lo)
+        {
+            glo = m + 1;
+        }
+        else
+        {
+            ghi = m;
+        }
+    }
+    out[w] = make_int2(lo, glo);
+}
+
+__global__ void systemSlotKernel(int numSystems, const int* __restrict__ groupOff, int ngroups,
+                                 const int4* __restrict__ groups, int numWaves, const int2* __restrict__ waveStart,
+                                 int* __restrict__ slotOff)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > numSystems)
+    {
+        return;
+    }
+    const int ga = groupOff[k];
+    if (ga >= ngroups)
+    {
+        slotOff[k] = ngroups + numWaves;
+        return;
+    }
+    // the wavefront whose run holds the group's first chunk position: the
+    // last one starting at or before it
+    const int x  = groups[ga].x;
+    int       lo = 0, hi = numWaves - 1;
+    while (lo < hi)
+    {
+        const int m = (lo + hi + 1) / 2;
+        if (waveStart[m].x <= x)
+        {
+            lo = m;
+        }
+        else
+        {
+            hi = m - 1;
+        }
+    }
+    slotOff[k] = ga + lo;
+}
+
 } // namespace
+
+void launchWaveSchedule(hipStream_t s, int numWaves, int nchunk, const int4* chunks, int nwork, int chunkWeight,
+                        int ngroups, const int4* groups, int2* out)
+{
+    waveScheduleKernel<<<(numWaves + 256) / 256, 256, 0, s>>>(numWaves, nchunk, chunks, nwork, chunkWeight, ngroups,
+                                                             groups, out);
+}
+
+void launchSystemSlots(hipStream_t s, int numSystems, const int* groupOff, int ngroups, const int4* groups,
+                       int numWaves, const int2* waveStart, int* slotOff)
+{
+    systemSlotKernel<<<(numSystems + 256) / 256, 256, 0, s>>>(numSystems, groupOff, ngroups, groups, numWaves,
+                                                             waveStart, slotOff);
+}
 
 void launchGroupSpans(hipStream_t s, int ncluster, const int* offJ, int* maxSpan)
 {

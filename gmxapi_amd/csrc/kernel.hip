@@ -16,14 +16,15 @@
 // project's own rational fits of the closed forms (fit_ewald_correction.py).
 //
 // Mapping to the hardware.
-//   * Every wavefront works alone on a group of G consecutive i-clusters
-//     (z-neighbours in one grid column, which share most j-clusters), claimed
-//     from a global counter.  It owns a private LDS window with one float3
-//     per atom of each distinct j-cluster of the group and of its own G
-//     clusters: j-forces are added to it with plain read-modify-write (no
-//     atomics, no barriers), and the window is flushed to the cluster force
-//     buffer with one global atomic per nonzero atom component when the group
-//     is done.
+//   * The i-clusters are cut into groups of G consecutive clusters
+//     (z-neighbours in one grid column, which share most j-clusters), and
+//     every wavefront gets a contiguous run of groups, balanced by tiles when
+//     the list is built.  A wavefront works alone on one group at a time with
+//     a private LDS window holding one float3 per atom of each distinct
+//     j-cluster of the group and of its own G clusters: j-forces are added to
+//     it with plain read-modify-write (no atomics, no barriers), and the
+//     window is flushed to the cluster force buffer with one global atomic per
+//     nonzero atom component when the wavefront moves to its next group.
 //   * In a tile the four i-atoms are wave-uniform (scalar registers) and lane
 //     l owns j-atom l & 3 of j-entry l >> 2: 16 j-clusters per tile, four
 //     i-atoms per lane, so one VALU instruction evaluates 64 pairs.  The
@@ -31,11 +32,11 @@
 //     s)), so the i-data never leave the scalar unit.  A lane keeps its
 //     j-force for the tile in registers; i-forces stay in 12 registers for an
 //     i-entry and are reduce-scattered across the wave once per entry.
-//   * A group's tiles are one contiguous run of the work list (each i-entry
-//     padded to 16 entries with a far-away dummy cluster), walked by one
-//     software pipeline across i-entry boundaries: work entries are loaded
-//     two tiles ahead and j-atom data one tile ahead, in a loop unrolled by
-//     two so that no loaded value is copied between registers.
+//   * A wavefront's groups are one contiguous run of the work list (each
+//     i-entry padded to 16 entries with a far-away dummy cluster), walked by
+//     one software pipeline across i-entry and group boundaries: work entries
+//     are loaded two tiles ahead and j-atom data one tile ahead, in a loop
+//     unrolled by two so that no loaded value is copied between registers.
 //   * Masked entries come first in every i-entry: its leading tiles run with
 //     exclusion and diagonal checks, the rest without.
 //   * LJ parameters, the Ewald table (table flavour) and each group's
@@ -311,7 +312,7 @@ __device__ __forceinline__ double waveSumD(double v)
 // LDS layout of a workgroup, in float4 units
 struct LdsLayout
 {
-    int nbfp, tab, fsh, ix, it, win;
+    int nbfp, tab, fsh, ix, it, uc, win;
     __host__ __device__ LdsLayout(int ntype, int tabN, int windowSlots)
     {
         nbfp  = 0;
@@ -319,7 +320,8 @@ struct LdsLayout
         fsh   = tab + tabN;
         ix    = fsh + (kNumShifts * 3 + 3) / 4;
         it    = ix + kWavesBlock * kGroupSizeMax * kClusterSize;
-        win   = it + kWavesBlock * kGroupSizeMax;
+        uc    = it + kWavesBlock * kGroupSizeMax;
+        win   = uc + kWavesBlock * ((windowSlots + 3) / 4);
         end_  = win + kWavesBlock * windowSlots * 3;
     }
     int end_;
@@ -361,56 +363,101 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
     float4* ix  = smem4 + L.ix + wave * kGroupSizeMax * kClusterSize;
     int*    it  = reinterpret_cast<int*>(smem4 + L.it) + wave * kGroupSizeMax * kClusterSize;
     float*  win = reinterpret_cast<float*>(smem4 + L.win) + wave * a.windowSlots * 12;
+    // the group's union list (cluster of each window slot beyond the own ones)
+    int*    uc  = reinterpret_cast<int*>(smem4 + L.uc) + wave * ((a.windowSlots + 3) / 4) * 4;
 
-    for (;;)
+    // this wavefront's contiguous run of chunks (balanced by cost when the
+    // list was built); chunks and work entries are stored in group order, so
+    // one software pipeline runs over the whole run, across i-entry and group
+    // boundaries
+    const int  wid    = blockIdx.x * kWavesBlock + wave;
+    const int2 ws     = a.waveStart[wid];
+    const int  chAll0 = uniformInt(ws.x);
+    const int  chAll1 = uniformInt(a.waveStart[wid + 1].x);
+    if (chAll0 < chAll1)
     {
-        int g = 0;
-        if (lane == 0)
-        {
-            g = atomicAdd(a.groupCounter, 1);
-        }
-        g = uniformInt(__shfl(g, 0, kWave));
-        if (g >= a.ngroups)
-        {
-            break;
-        }
-        const int4 grp  = a.groups[g]; // chunk range, union offset, union size
-        const int  ch0  = uniformInt(grp.x);
-        const int  ch1  = uniformInt(grp.y);
-        const int  uoff = uniformInt(grp.z);
-        const int  U    = uniformInt(grp.w);
-        const int4 gi   = a.groupInfo[g]; // first own cluster, own count | G << 16, shift table, energy slot
-        const int  c0   = uniformInt(gi.x);
-        const int  nOwn = uniformInt(gi.y) & 0xffff;
-        const int  G    = uniformInt(gi.y) >> 16;
-        const int  shb  = uniformInt(gi.z);
-
-        // clear the window (own slots, union slots, the padding entries'
-        // spare slot) and cache the group's i-atoms; this wavefront's LDS
-        // accesses complete in order, so no barrier is needed
-        for (int t = lane; t < (G + U + 1) * 3; t += kWave)
-        {
-            reinterpret_cast<float4*>(win)[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        }
-        if (lane < nOwn * kClusterSize)
-        {
-            ix[lane] = a.xq[c0 * kClusterSize + lane];
-            it[lane] = a.type[c0 * kClusterSize + lane] * ntype;
-        }
-
-        // the group's tiles: work[W0, W0 + 16 T)
-        const int4     cFirst = a.chunks[ch0];
-        const int4     cLast  = a.chunks[ch1 - 1];
+        const int4     cFirst = a.chunks[chAll0];
+        const int4     cLast  = a.chunks[chAll1 - 1];
         const unsigned W0     = static_cast<unsigned>(uniformInt(cFirst.z));
         const int      T      = (uniformInt(cLast.z) - static_cast<int>(W0)) / kTileJ + uniformInt(cLast.w);
 
-        double eVdw = 0.0, eCoul = 0.0;
+        // current group (the run starts inside the group holding its first
+        // chunk); wavefront w writes the energies of group g to slot g + w
+        int g = uniformInt(ws.y), chGrpEnd = 0, uoff = 0, U = 0, c0 = 0, nOwn = 0, G = 0, shb = 0;
+        // current chunk
         IAtoms I;
         float4 sv;
         float  fi[12];
-        int    ci = 0, sh = 0, nspec = 0, chStart = 0, chEnd = 0, chIdx = ch0;
+        int    ci = 0, sh = 0, nspec = 0, chStart = 0, chEnd = 0, chIdx = chAll0;
         bool   central = false;
+        double eVdw = 0.0, eCoul = 0.0;
 
+        auto beginGroup = [&](int gg) {
+            const int4 grp = a.groups[gg]; // chunk range, union offset, windowed union size
+            const int4 gi  = a.groupInfo[gg]; // first own cluster, own count | G << 16, shift table
+            chGrpEnd       = uniformInt(grp.y);
+            uoff           = uniformInt(grp.z);
+            U              = uniformInt(grp.w);
+            c0             = uniformInt(gi.x);
+            nOwn           = uniformInt(gi.y) & 0xffff;
+            G              = uniformInt(gi.y) >> 16;
+            shb            = uniformInt(gi.z);
+            // clear the window (own slots, union slots, the padding entries'
+            // spare slot), cache the group's i-atoms and union list; this
+            // wavefront's LDS accesses complete in order, so no barrier
+            for (int t = lane; t < (G + U + 1) * 3; t += kWave)
+            {
+                reinterpret_cast<float4*>(win)[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+            if (lane < nOwn * kClusterSize)
+            {
+                ix[lane] = a.xq[c0 * kClusterSize + lane];
+                it[lane] = a.type[c0 * kClusterSize + lane] * ntype;
+            }
+            for (int t = lane; t < U; t += kWave)
+            {
+                uc[t] = a.unionList[uoff + t];
+            }
+        };
+        auto endGroup = [&]() {
+            if (kEnergy)
+            {
+                const double v = waveSumD(eVdw);
+                const double c = waveSumD(eCoul);
+                if (lane == 0)
+                {
+                    a.energyPartial[2 * (g + wid)]     = v;
+                    a.energyPartial[2 * (g + wid) + 1] = c;
+                }
+                eVdw  = 0.0;
+                eCoul = 0.0;
+            }
+            // flush the window: a lane per window atom, one global atomic per
+            // nonzero component
+            for (int t = lane; t < (G + U) * kClusterSize; t += kWave)
+            {
+                const int    slot = t >> 2;
+                const float* wv   = win + t * 3;
+                const float  vx = wv[0], vy = wv[1], vz = wv[2];
+                const int    cl   = slot < G ? (slot < nOwn ? c0 + slot : -1) : uc[slot - G];
+                if (cl >= 0)
+                {
+                    float* dst = reinterpret_cast<float*>(a.fnb + cl * kClusterSize + (t & 3));
+                    if (vx != 0.0f)
+                    {
+                        unsafeAtomicAdd(dst, vx);
+                    }
+                    if (vy != 0.0f)
+                    {
+                        unsafeAtomicAdd(dst + 1, vy);
+                    }
+                    if (vz != 0.0f)
+                    {
+                        unsafeAtomicAdd(dst + 2, vz);
+                    }
+                }
+            }
+        };
         auto beginChunk = [&](const int4 cd, int t) {
             ci              = uniformInt(cd.x);
             const int flags = uniformInt(cd.y);
@@ -474,27 +521,55 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
                 tile4<kElec, kEnergy, false>(I, J, 0u, false, jj, ic, sNbfp, sTab, fi, fjx, fjy, fjz, vlj, vc);
             }
             // the 16 entries of a tile have distinct slots (padding entries
-            // share the spare slot, adding zeros): plain read-modify-write
-            float* w = win + (e.y >> 16) * 12 + jj * 3;
-            w[0] += fjx;
-            w[1] += fjy;
-            w[2] += fjz;
+            // share the spare slot, adding zeros): plain read-modify-write;
+            // clusters that did not fit the window take global atomics
+            const unsigned slot = e.y >> 16;
+            if (slot != kSlotOverflow)
+            {
+                float* w = win + slot * 12 + jj * 3;
+                w[0] += fjx;
+                w[1] += fjy;
+                w[2] += fjz;
+            }
+            else
+            {
+                float* dst = reinterpret_cast<float*>(a.fnb + e.x * kClusterSize + jj);
+                unsafeAtomicAdd(dst, fjx);
+                unsafeAtomicAdd(dst + 1, fjy);
+                unsafeAtomicAdd(dst + 2, fjz);
+            }
             if (kEnergy)
             {
                 eVdw += static_cast<double>(vlj);
                 eCoul += static_cast<double>(vc);
             }
         };
+        // at a chunk boundary: the next chunk, and the next group holding
+        // chunks when the chunk was its group's last (groups without
+        // i-entries in between are passed through)
+        int4 cdNext  = a.chunks[min(chIdx + 1, chAll1 - 1)];
+        auto advance = [&](int t) {
+            endChunk();
+            chIdx++;
+            while (chIdx == chGrpEnd)
+            {
+                endGroup();
+                g++;
+                beginGroup(g);
+            }
+            beginChunk(cdNext, t);
+            cdNext = a.chunks[min(chIdx + 1, chAll1 - 1)];
+        };
 
-        // reads past the group's last tile are clamped to it
+        // reads past the last tile are clamped to it
         auto entryAt = [&](int t) {
             return ldg(a.work, W0 + static_cast<unsigned>(kTileJ * min(t, T - 1) + kk));
         };
-        uint2  eA     = entryAt(0);
-        uint2  eB     = entryAt(1);
-        float4 xA     = ldg(a.xq, eA.x * kClusterSize + jj);
-        int    tA     = ldg(a.type, eA.x * kClusterSize + jj);
-        int4   cdNext = a.chunks[min(ch0 + 1, ch1 - 1)];
+        uint2  eA = entryAt(0);
+        uint2  eB = entryAt(1);
+        float4 xA = ldg(a.xq, eA.x * kClusterSize + jj);
+        int    tA = ldg(a.type, eA.x * kClusterSize + jj);
+        beginGroup(g);
         beginChunk(cFirst, 0);
         for (int t = 0; t < T; t += 2)
         {
@@ -503,10 +578,7 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
             const uint2  eA2 = entryAt(t + 2);
             if (t == chEnd)
             {
-                endChunk();
-                chIdx++;
-                beginChunk(cdNext, t);
-                cdNext = a.chunks[min(chIdx + 1, ch1 - 1)];
+                advance(t);
             }
             doTile(eA, xA, tA, t - chStart < nspec);
             if (t + 1 >= T)
@@ -518,45 +590,14 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
             tA              = ldg(a.type, eA2.x * kClusterSize + jj);
             if (t + 1 == chEnd)
             {
-                endChunk();
-                chIdx++;
-                beginChunk(cdNext, t + 1);
-                cdNext = a.chunks[min(chIdx + 1, ch1 - 1)];
+                advance(t + 1);
             }
             doTile(eB, xB, tB, t + 1 - chStart < nspec);
             eA = eA2;
             eB = eB2;
         }
         endChunk();
-
-        if (kEnergy)
-        {
-            eVdw  = waveSumD(eVdw);
-            eCoul = waveSumD(eCoul);
-            if (lane == 0)
-            {
-                // the group's energy slot (its index in build order)
-                const int slot             = uniformInt(gi.w);
-                a.energyPartial[2 * slot]     = eVdw;
-                a.energyPartial[2 * slot + 1] = eCoul;
-            }
-        }
-
-        // flush the window: one atomic per nonzero atom component
-        for (int t = lane; t < (G + U) * 12; t += kWave)
-        {
-            const float v = win[t];
-            if (v != 0.0f)
-            {
-                const int slot = t / 12;
-                const int r    = t - slot * 12;
-                const int cl   = slot < G ? (slot < nOwn ? c0 + slot : -1) : a.unionList[uoff + slot - G];
-                if (cl >= 0)
-                {
-                    unsafeAtomicAdd(reinterpret_cast<float*>(a.fnb + cl * kClusterSize + r / 3) + (r % 3), v);
-                }
-            }
-        }
+        endGroup();
     }
 
     if (kShift)

@@ -43,12 +43,13 @@ constexpr int kGroupMaxJ    = 4096;
 // large enough that a chunk is a whole i-entry)
 constexpr int kTileJ        = 16;
 constexpr int kChunkEntries = 1 << 20;
-// wavefronts per nonbonded workgroup, and the LDS a workgroup may use for
-// four resident workgroups per CU (16 wavefronts), or at most 64 KiB when
-// the lists are too long for that
-constexpr int kKernelWaves        = 4;
-constexpr int kKernelLdsBudget    = 40 * 1024;
-constexpr int kKernelLdsBudgetMax = 64 * 1024;
+// cost of one chunk (i-entry) in work entries for the static schedule: its
+// i-data set-up and i-force reduction
+constexpr int kChunkWeight = 8;
+// wavefronts per nonbonded workgroup, and the LDS a workgroup may use (two
+// resident workgroups per CU, eight wavefronts)
+constexpr int kKernelWaves     = 4;
+constexpr int kKernelLdsBudget = 64 * 1024;
 // rsq floor for float (pairlist.h:146): r > 6.2e-4 nm
 constexpr float kMinRsq = 3.82e-07f;
 
@@ -146,11 +147,25 @@ struct GroupArgs
     uint2*       work;       // {cj, slot << 16 | mask}, each chunk padded to whole tiles
     int4*        chunks;     // {ci, shift | self << 8 | special << 9, work_start, tiles}
     int4*        groups;     // {chunk_start, chunk_end, union_offset, union_size}
-    int4*        groupInfo;  // {first own cluster, own count | G << 16, shift-table offset, energy slot}
+    int4*        groupInfo;  // {first own cluster, own count | G << 16, shift-table offset, group index}
+    int          windowCap;  // fill pass: window slots (own, union, spare); union beyond -> kSlotOverflow
 };
+// window slot of a j-entry whose cluster did not fit the group's window:
+// the kernel adds its forces with global atomics
+constexpr unsigned kSlotOverflow = 0xffffu;
 void launchGroupSpans(hipStream_t s, int ncluster, const int* offJ, int* maxSpan);
 void launchBuildGroups(hipStream_t s, const GroupArgs& a, bool fill);
 void launchMaxReduce(hipStream_t s, int n, const int* v, int* out);
+// Static schedule of the pair kernel: the chunk list (in group order) cut into
+// numWaves contiguous runs of about equal cost (work entries plus
+// chunkWeight per chunk).  out[w] = {first chunk, group of that chunk} for
+// w in [0, numWaves], out[numWaves] = {nchunk, ngroups}.
+void launchWaveSchedule(hipStream_t s, int numWaves, int nchunk, const int4* chunks, int nwork, int chunkWeight,
+                        int ngroups, const int4* groups, int2* out);
+// Energy-slot range of each system of a batch: slotOff[k] for k in [0, R]
+// (the kernel's slot g + w of the first chunk of the system's first group).
+void launchSystemSlots(hipStream_t s, int numSystems, const int* groupOff, int ngroups, const int4* groups,
+                       int numWaves, const int2* waveStart, int* slotOff);
 
 // -- kernel.hip
 struct KernelArgs
@@ -171,16 +186,19 @@ struct KernelArgs
     const float4* tabFDV0;
     const float4* shiftVec; // 45 per system; groupInfo.z selects the table
     InteractionConst ic;
-    int*     groupCounter; // work-group claim counter, zeroed before launch
+    const int2* waveStart; // [4 * gridDim.x + 1]: each wavefront's first chunk and its group
     float4*  fnb;
-    double*  energyPartial; // [ngroups][2]
+    // [ngroups + 4 * gridDim.x][2], zeroed before the launch: the energies of
+    // wavefront w in group g go to slot g + w
+    double*  energyPartial;
     float*   fshiftPartial; // [gridDim.x][45*3]
 };
 void launchNonbonded(hipStream_t s, const KernelArgs& a, int elec, bool energy, bool shiftForces, int numBlocks);
 int  nonbondedLdsBytes(int ntype, int tabN, int elec, int windowSlots);
 
 // -- buffers.hip
-void launchClearF4(hipStream_t s, float4* p, int n, int* counter);
+// zeroes n float4 and, when d is not NULL, nd doubles
+void launchClearF4(hipStream_t s, float4* p, int n, double* d, int nd);
 void launchConvertX(hipStream_t s, int nslot, const int* atomIndices, const float* x, float4* xq);
 void launchReduceF(hipStream_t s, int natoms, const int* cells, const float4* fnb, float* f);
 void launchFinalize(hipStream_t s, int numEnergyPartials, const double* energyPartial, int numExclPartials,
@@ -210,7 +228,7 @@ struct ExclusionArgs
 };
 constexpr int kExclMaxBlocks = 1024;
 int launchExclusionEnergy(hipStream_t s, const ExclusionArgs& a);
-void launchBatchFinalize(hipStream_t s, int numSystems, const int* groupOff, const double* energyPartial,
+void launchBatchFinalize(hipStream_t s, int numSystems, const int* slotOff, const double* energyPartial,
                          const int* atomOff, const double* atomEnergy, double* energyOut);
 void launchVirial(hipStream_t s, int natoms, const float* x, const float* f, const double* fshift, const float4* shiftVec,
                   double* virialPartial, int numBlocks, double* virialOut);

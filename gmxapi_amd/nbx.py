@@ -442,7 +442,7 @@ class Batch:
         out = np.zeros(3, np.int32)
         self._check(self._lib.nbx_batch_autotune(self._h, C.c_void_p(x_ptr), C.c_void_p(f_ptr), flags, repeats,
                                                  _ptr(out)), "nbx_batch_autotune")
-        return dict(group_size=int(out[0]), blocks_per_cu=int(out[1]), heavy_first=bool(out[2]))
+        return dict(group_size=int(out[0]), blocks_per_cu=int(out[1]), entry_weight=int(out[2]))
 
     def system(self, index: int) -> "Context":
         h = C.c_void_p()
@@ -461,11 +461,13 @@ def batch_for(systems, **options) -> Batch:
     return b
 
 
-def context_for(system, coulomb="ewald", ewald_correction="table", device=0, potential_shift=False) -> Context:
+def context_for(system, coulomb="ewald", ewald_correction="table", device=0, potential_shift=False,
+                kernel_group_size=0, kernel_blocks_per_cu=0) -> Context:
     """Context set up on a gmxapi_amd.systems.System the way nblib sets up its
     calculator: parameters, topology, grid, pair list."""
     ctx = Context(coulomb=coulomb, ewald_correction=ewald_correction, rlist=system.rlist,
-                  rcoulomb=system.rcoulomb, ewald_potential_shift=potential_shift, device=device)
+                  rcoulomb=system.rcoulomb, ewald_potential_shift=potential_shift, device=device,
+                  kernel_group_size=kernel_group_size, kernel_blocks_per_cu=kernel_blocks_per_cu)
     m6, m12 = nblib_c6c12_matrices(system.c6, system.c12)
     ctx.set_nonbonded_parameters(m6, m12)
     ctx.set_topology(system.types, system.charges, system.excl_off, system.excl_idx)

@@ -264,8 +264,9 @@ int nbx_batch_launch_graph(nbx_batch* batch);
 int nbx_batch_get_energies(nbx_batch* batch, double* energies);
 
 /* Times the candidate kernel schedules (work-group size, workgroups per CU,
- * group order) on this batch with `repeats` launches each and keeps the
- * fastest; chosen[3] = {group size, workgroups per CU, heavy groups first}. */
+ * cost weight of an i-entry in the static schedule, in pair-list entries) on
+ * this batch with `repeats` launches each and keeps the fastest;
+ * chosen[3] = {group size, workgroups per CU, i-entry weight}. */
 int nbx_batch_autotune(nbx_batch* batch, const float* x, float* f, unsigned flags, int repeats, int* chosen3);
 
 /* The per-system contexts (for sequential reference launches and checks);

@@ -98,14 +98,43 @@ def info_clusters(ctx):
     return ctx.info()["num_clusters"]
 
 
-def _energy_bar(e64, e32, scale):
-    """Energy tolerance: 1e-6 relative, widened on small boxes to twice the
-    error of the FP32 reference kernel itself.  Float coordinate differences
-    carry ~1e-7 relative rounding that r^-12 and the Ewald self/exclusion
-    cancellation amplify; on a few hundred molecules that alone exceeds 1e-6
-    of the total in the reference's own FP32 kernel (the 9,000- and
-    288,000-atom boxes are held to the plain 1e-6)."""
-    return max(ENERGY_TOL_REL * scale, 2.0 * abs(e32 - e64))
+_NOISE = {}
+
+
+def _fp32_noise(s, coulomb="ewald", ewald_exact=False, copies=6):
+    """RMS deviation from FP64 of the reference FP32 kernel (the oracle's
+    single-precision restatement) over rigid translations of the system.
+
+    A translation leaves every energy unchanged in exact arithmetic (the FP64
+    oracle agrees to ~1e-10) but moves every coordinate's rounding, so these
+    deviations are samples of the FP32 kernel's rounding noise on this box.
+    On boxes of a few hundred molecules that noise is above 1e-6 of the total
+    (float coordinate differences and shift vectors carry ~1e-7 relative
+    rounding that r^-12 and the Coulomb sum amplify), and one sample of it
+    can land arbitrarily close to zero."""
+    key = (id(s), coulomb, ewald_exact)
+    if key not in _NOISE:
+        rng = np.random.default_rng(17)
+        box = np.asarray(s.box, np.float64)
+        dev = {"Vvdw": [], "Vc": []}
+        for k in range(copies):
+            t = S.System(**{f: getattr(s, f) for f in s.__dataclass_fields__})
+            x = np.mod(s.x + (0.0 if k == 0 else rng.uniform(0.0, 1.0, 3) * box), box)
+            x[x.astype(np.float32) >= box.astype(np.float32)] = 0.0  # stay inside [0, L) in float
+            t.x = x
+            r64 = oracle_for(t, "f64", coulomb=coulomb, ewald_exact=ewald_exact).compute(t.x)
+            r32 = oracle_for(t, "f32", coulomb=coulomb).compute(t.x)
+            for e in dev:
+                dev[e].append(r32[e] - r64[e])
+        _NOISE[key] = {e: float(np.sqrt(np.mean(np.square(v)))) for e, v in dev.items()}
+    return _NOISE[key]
+
+
+def _energy_bar(s, key, scale, coulomb="ewald", ewald_exact=False):
+    """Energy tolerance: 1e-6 relative, or on small boxes three times the
+    FP32 reference kernel's own RMS rounding noise there, whichever is larger
+    (the 9,000- and 288,000-atom boxes are held to the plain 1e-6)."""
+    return max(ENERGY_TOL_REL * scale, 3.0 * _fp32_noise(s, coulomb, ewald_exact)[key])
 
 
 def _compare(s, ctx, coulomb="ewald", widen_energy=True):
@@ -123,7 +152,7 @@ def _compare(s, ctx, coulomb="ewald", widen_energy=True):
     assert ferr32 <= FORCE_TOL_RMS
     for got, key in ((vlj, "Vvdw"), (vc, "Vc")):
         scale = max(abs(r64[key]), abs(e64))
-        bar = _energy_bar(r64[key], r32[key], scale) if widen_energy else ENERGY_TOL_REL * scale
+        bar = _energy_bar(s, key, scale, coulomb) if widen_energy else ENERGY_TOL_REL * scale
         assert abs(got - r64[key]) <= bar, (key, got, r64[key], bar)
     vir = ctx.virial()
     np.testing.assert_allclose(vir, r64["virial"], rtol=0, atol=1e-4 * np.abs(r64["virial"]).max())
@@ -171,14 +200,13 @@ def test_analytical_ewald_against_exact_oracle(name, s):
     f = ctx.compute(s.x, energy=True, virial=True)
     vlj, vc = ctx.energies()
     r64 = oracle_for(s, "f64", ewald_exact=True).compute(s.x)
-    r32 = oracle_for(s, "f32").compute(s.x)
     ferr = rms(f - r64["f"]) / rms(r64["f"])
     e64 = r64["Vvdw"] + r64["Vc"]
     print(f"  force rms err {ferr:.2e}; Vlj {vlj:.6f} ({r64['Vvdw']:.6f}) Vc {vc:.6f} ({r64['Vc']:.6f})", flush=True)
     assert ferr <= FORCE_TOL_RMS
     for got, key in ((vlj, "Vvdw"), (vc, "Vc")):
         scale = max(abs(r64[key]), abs(e64))
-        assert abs(got - r64[key]) <= _energy_bar(r64[key], r32[key], scale), (key, got, r64[key])
+        assert abs(got - r64[key]) <= _energy_bar(s, key, scale, ewald_exact=True), (key, got, r64[key])
 
 
 def test_analytical_ewald_config0(water3000):
@@ -196,8 +224,8 @@ def test_analytical_ewald_config0(water3000):
 
 
 def test_energies_are_bitwise_reproducible():
-    # each group's energy is summed by one wavefront and the group sums are
-    # added in group order, whatever wavefront took which group
+    # the static schedule fixes which wavefront sums which tiles, and the
+    # per-wavefront group sums are added in a fixed order
     s = S.config(0)
     ctx = nbx.context_for(s)
     ctx.compute(s.x, energy=True)
@@ -314,12 +342,22 @@ def test_work_list_covers_the_pair_list(name, s):
                     # padding: the far-away dummy cluster in the spare slot
                     assert slot == G + len(union) and mask == 0xFFFF
                     continue
-                assert (g * G + slot if slot < G else union[slot - G]) == cj
+                if slot == 0xFFFF:
+                    # a union cluster beyond the window's capacity
+                    assert not (g * G <= cj < (g + 1) * G)
+                else:
+                    assert (g * G + slot if slot < G else union[slot - G]) == cj
                 if mask != 0xFFFF:
                     assert (k - ws) // 16 < special
                 got.append((ci, sh, cj, mask))
     assert sorted(got) == want
     assert sorted(self_chunks) == list(range(nc))
+    # groups, chunks and work entries are laid out back to back in group
+    # order: the kernel's static schedule walks them as one stream
+    gr, ch = wg["groups"], wg["chunks"]
+    assert gr[0, 0] == 0 and np.all(gr[1:, 0] == gr[:-1, 1]) and gr[-1, 1] == len(ch)
+    assert ch[0, 2] == 0 and np.all(ch[1:, 2] == ch[:-1, 2] + 16 * ch[:-1, 3])
+    assert ch[-1, 2] + 16 * ch[-1, 3] == len(wg["work"])
     for c, sh, a, b in ie:
         masked = (je[a:b, 1] & 0xFFFF) != 0xFFFF
         assert not np.any(np.diff(masked.astype(int)) > 0), "masked entries must lead"
