@@ -230,21 +230,26 @@ void buildWorkGroups(nbx_context* c)
     c->jentryTmp.ensure(c->ncj);
     // the largest group size (or the chosen one) whose j-force windows, one
     // per wavefront, fit the workgroup's LDS budget: larger groups share more
-    // j-clusters per flush, which pays more than a third resident workgroup
+    // j-clusters per flush, which pays more than a third resident workgroup.
+    // When no size fits with at most 2% of the union entries spilled to
+    // global atomics (long cut-offs, many types), the last pass takes the
+    // smallest (or the chosen) size with whatever spill its window leaves.
     const int forced = c->opt.kernel_group_size;
-    for (int k = 0; k < 4; k++)
+    for (int k = 0; k < 5; k++)
     {
-        const int G      = kGroupSizeMax >> k;
-        const int budget = kKernelLdsBudget;
-        if (forced > 0 && G != forced)
+        const bool fallback = (k == 4);
+        const int  G        = fallback ? (forced > 0 ? forced : 1) : kGroupSizeMax >> k;
+        const int  si       = G == 8 ? 0 : G == 4 ? 1 : G == 2 ? 2 : 3; // spans[] index of G
+        const int  budget   = kKernelLdsBudget;
+        if (!fallback && forced > 0 && G != forced)
         {
             continue;
         }
-        if (spans[k] > kGroupMaxJ && G > 1)
+        if (spans[si] > kGroupMaxJ && G > 1)
         {
             continue;
         }
-        if (spans[k] > kGroupMaxJ)
+        if (spans[si] > kGroupMaxJ)
         {
             throw InputError("a grid cluster has more than " + std::to_string(kGroupMaxJ)
                              + " cluster-pair entries: the pair-list cut-off is too long for this kernel");
@@ -290,7 +295,7 @@ void buildWorkGroups(nbx_context* c)
         // spare slot; when the largest union does not fit the budget, the
         // window holds the first union clusters of each group and the rest
         // take direct global atomics, which is accepted while that is at most
-        // 2% of the union entries
+        // 2% of the union entries (any share in the last pass)
         int slots = G + maxUnion + 1;
         if (nonbondedLdsBytes(c->ntype, c->ic.tab_n, c->elec, slots) > budget)
         {
@@ -309,7 +314,7 @@ void buildWorkGroups(nbx_context* c)
             {
                 spilled += std::max(0, u - (slots - 1 - G));
             }
-            if (spilled * 50 > unionTotal)
+            if (!fallback && spilled * 50 > unionTotal)
             {
                 continue;
             }
@@ -610,12 +615,19 @@ int nbx_set_nonbonded_parameters(nbx_context* c, int numTypes, const double* c6,
                 nb[i * t + j] = make_float2(static_cast<float>(a) * 6.0f, static_cast<float>(b) * 12.0f);
             }
         }
+        // the stream's queued kernels may still read the old table
+        check(hipStreamSynchronize(c->stream), "sync");
         c->nbfp.ensure(nb.size());
         check(hipMemcpy(c->nbfp.p, nb.data(), sizeof(float2) * nb.size(), hipMemcpyHostToDevice), "upload nbfp");
+        // new values for the same types leave the grid and the pair list
+        // valid (they depend on positions only); a new type count does not
+        if (t != c->ntype)
+        {
+            c->gridValid = false;
+            c->listValid = false;
+        }
         c->ntypeUser = numTypes;
         c->ntype     = t;
-        c->gridValid = false;
-        c->listValid = false;
     });
 }
 
@@ -1050,6 +1062,147 @@ int nbx_compute(nbx_context* c, const float* x, float* f, unsigned flags, int on
         if (!onDevice)
         {
             check(hipMemcpyAsync(f, fd, sizeof(float) * 3 * c->natoms, hipMemcpyDeviceToHost, c->stream), "download f");
+            check(hipStreamSynchronize(c->stream), "sync");
+        }
+    });
+}
+
+int nbx_set_charges(nbx_context* c, const float* charges, int onDevice)
+{
+    if (!c)
+    {
+        return fail(NBX_ERROR_INPUT, "NULL context");
+    }
+    return guarded([&] {
+        requireGrid(c);
+        if (!charges)
+        {
+            throw InputError("charges must not be NULL");
+        }
+        setDevice(c);
+        if (onDevice)
+        {
+            check(hipMemcpyAsync(c->charge.p, charges, sizeof(float) * c->natoms, hipMemcpyDeviceToDevice, c->stream),
+                  "copy q");
+        }
+        else
+        {
+            for (int a = 0; a < c->natoms; a++)
+            {
+                if (!std::isfinite(charges[a]))
+                {
+                    throw InputError("non-finite charge");
+                }
+            }
+            check(hipMemcpyAsync(c->charge.p, charges, sizeof(float) * c->natoms, hipMemcpyHostToDevice, c->stream),
+                  "upload q");
+        }
+        launchSetCharges(c->stream, c->ncluster * kClusterSize, c->atomIndices.p, c->charge.p, c->xq.p);
+        check(hipGetLastError(), "set charges");
+        if (!onDevice)
+        {
+            check(hipStreamSynchronize(c->stream), "sync");
+        }
+    });
+}
+
+int nbx_compute_adjoint(nbx_context* c, const float* x, const float* g, float* gradX, double* gradQ, double* gradC6,
+                        double* gradC12, int onDevice)
+{
+    if (!c)
+    {
+        return fail(NBX_ERROR_INPUT, "NULL context");
+    }
+    return guarded([&] {
+        requireList(c);
+        if (!g)
+        {
+            throw InputError("the force cotangent g must not be NULL");
+        }
+        if (c->ntype > kAdjointMaxTypes)
+        {
+            throw InputError("the adjoint supports at most " + std::to_string(kAdjointMaxTypes - 1) + " particle types");
+        }
+        const int lds = adjointLdsBytes(c->ntype, c->ic.tab_n, c->elec, c->windowSlots);
+        if (lds > kKernelLdsBudget)
+        {
+            throw InputError("the adjoint's j-cluster window does not fit in LDS: choose a smaller kernel_group_size");
+        }
+        setDevice(c);
+        const size_t na    = static_cast<size_t>(c->natoms);
+        const int    nslot = c->ncluster * kClusterSize;
+        const float* xd    = deviceCoordinates(c, x, onDevice);
+        launchConvertX(c->stream, nslot, c->atomIndices.p, xd, c->xq.p);
+        const float* gd = g;
+        if (!onDevice)
+        {
+            c->gIn.ensure(3 * na);
+            check(hipMemcpyAsync(c->gIn.p, g, sizeof(float) * 3 * na, hipMemcpyHostToDevice, c->stream), "upload g");
+            gd = c->gIn.p;
+        }
+        c->gq.ensure(nslot + kClusterSize);
+        launchConvertG(c->stream, nslot, c->atomIndices.p, gd, c->gq.p);
+        c->gradXnb.ensure(nslot + kClusterSize);
+        c->gradQnb.ensure(nslot + kClusterSize);
+        const int    blocks = adjointBlocks(c->numBlocks);
+        const size_t np     = 2 * static_cast<size_t>(c->ntype) * c->ntype;
+        c->paramPartial.ensure(blocks * np);
+        const int tu    = c->ntypeUser;
+        float*    gxOut = gradX;
+        double*   gqOut = gradQ;
+        double*   g6Out = gradC6;
+        double*   g12Out = gradC12;
+        if (!onDevice)
+        {
+            c->gradXOut.ensure(3 * na);
+            c->gradQOut.ensure(na);
+            c->paramOut.ensure(2 * static_cast<size_t>(tu) * tu);
+            gxOut  = gradX ? c->gradXOut.p : nullptr;
+            gqOut  = gradQ ? c->gradQOut.p : nullptr;
+            g6Out  = gradC6 ? c->paramOut.p : nullptr;
+            g12Out = gradC12 ? c->paramOut.p + static_cast<size_t>(tu) * tu : nullptr;
+        }
+        AdjointArgs a{};
+        a.chunks       = c->chunks.p;
+        a.groups       = c->groups.p;
+        a.groupInfo    = c->groupInfo.p;
+        a.ncluster     = c->ncluster;
+        a.windowSlots  = c->windowSlots;
+        a.unionList    = c->unionList.p;
+        a.work         = c->work.p;
+        a.xq           = c->xq.p;
+        a.gq           = c->gq.p;
+        a.type         = c->nbtype.p;
+        a.nbfp         = c->nbfp.p;
+        a.ntype        = c->ntype;
+        a.tabFDV0      = c->tabFDV0.p;
+        a.shiftVec     = c->shiftVec.p;
+        a.ic           = c->ic;
+        a.waveStart    = c->waveStart.p;
+        a.gradX        = c->gradXnb.p;
+        a.gradQ        = c->gradQnb.p;
+        a.paramPartial = c->paramPartial.p;
+        launchAdjoint(c->stream, a, c->elec, c->numBlocks, c->natoms, c->cells.p, gxOut, gqOut, tu, g6Out, g12Out);
+        check(hipGetLastError(), "adjoint kernels");
+        if (!onDevice && gradX)
+        {
+            check(hipMemcpyAsync(gradX, gxOut, sizeof(float) * 3 * na, hipMemcpyDeviceToHost, c->stream), "download");
+        }
+        if (!onDevice && gradQ)
+        {
+            check(hipMemcpyAsync(gradQ, gqOut, sizeof(double) * na, hipMemcpyDeviceToHost, c->stream), "download");
+        }
+        const size_t nt = static_cast<size_t>(tu) * tu;
+        if (!onDevice && gradC6)
+        {
+            check(hipMemcpyAsync(gradC6, g6Out, sizeof(double) * nt, hipMemcpyDeviceToHost, c->stream), "download");
+        }
+        if (!onDevice && gradC12)
+        {
+            check(hipMemcpyAsync(gradC12, g12Out, sizeof(double) * nt, hipMemcpyDeviceToHost, c->stream), "download");
+        }
+        if (!onDevice)
+        {
             check(hipStreamSynchronize(c->stream), "sync");
         }
     });

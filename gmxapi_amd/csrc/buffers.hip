@@ -63,6 +63,20 @@ __global__ void convertXKernel(int nslot, const int* __restrict__ atomIndices, c
     }
 }
 
+__global__ void setChargesKernel(int nslot, const int* __restrict__ atomIndices, const float* __restrict__ q,
+                                 float4* __restrict__ xq)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nslot)
+    {
+        const int a = atomIndices[s];
+        if (a >= 0)
+        {
+            xq[s].w = q[a];
+        }
+    }
+}
+
 __global__ void reduceFKernel(int natoms, const int* __restrict__ cells, const float4* __restrict__ fnb,
                               float* __restrict__ f)
 {
@@ -463,6 +477,14 @@ void launchConvertX(hipStream_t s, int nslot, const int* atomIndices, const floa
     if (nslot > 0)
     {
         convertXKernel<<<(nslot + 255) / 256, 256, 0, s>>>(nslot, atomIndices, x, xq);
+    }
+}
+
+void launchSetCharges(hipStream_t s, int nslot, const int* atomIndices, const float* q, float4* xq)
+{
+    if (nslot > 0)
+    {
+        setChargesKernel<<<(nslot + 255) / 256, 256, 0, s>>>(nslot, atomIndices, q, xq);
     }
 }
 

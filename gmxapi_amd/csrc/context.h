@@ -172,5 +172,9 @@ struct nbx_context
     nbx::DevBuf<int2>    waveStart; // static schedule: each wavefront's first chunk and its group
     nbx::DevBuf<double>  exclPartial;
     nbx::DevBuf<float>   fshiftPartial, fOut;
+    // adjoint
+    nbx::DevBuf<float4>  gq, gradXnb;
+    nbx::DevBuf<double>  gradQnb, paramPartial, paramOut, gradQOut;
+    nbx::DevBuf<float>   gIn, gradXOut;
     bool            haveEnergy = false, haveShift = false, haveVirial = false;
 };

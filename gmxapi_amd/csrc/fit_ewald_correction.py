@@ -1,5 +1,5 @@
 """Generates the rational approximations of the analytical Ewald
-correction used by kernel.hip (NBX_EWALD_ANALYTICAL).
+correction used by the pair kernels (coefficients in kernel_common.h; NBX_EWALD_ANALYTICAL).
 
 With z = beta r and s = z^2, the kernel needs
     g(z) = (erf z - 2 z / sqrt(pi) exp(-z^2)) / z^3   (force: F_corr / r = beta^3 g)

@@ -196,10 +196,49 @@ struct KernelArgs
 void launchNonbonded(hipStream_t s, const KernelArgs& a, int elec, bool energy, bool shiftForces, int numBlocks);
 int  nonbondedLdsBytes(int ntype, int tabN, int elec, int windowSlots);
 
+// -- adjoint.hip
+struct AdjointArgs
+{
+    const int4*   chunks;
+    const int4*   groups;
+    const int4*   groupInfo;
+    int           ncluster;
+    int           windowSlots;
+    const int*    unionList;
+    const uint2*  work;
+    const float4* xq;
+    const float4* gq; // dL/dF per cluster-layout atom (w unused)
+    const int*    type;
+    const float2* nbfp;
+    int           ntype;
+    const float4* tabFDV0;
+    const float4* shiftVec;
+    InteractionConst ic;
+    const int2* waveStart; // the forward kernel's static schedule
+    float4*     gradX;     // dL/dx per cluster-layout atom, zeroed by launchAdjoint
+    double*     gradQ;     // dL/dq per cluster-layout atom, zeroed by launchAdjoint
+    double*     paramPartial; // [adjoint workgroups][ntype^2][2]
+};
+// the adjoint's type-pair accumulators live in LDS: at most this many types
+// (including the padding type)
+constexpr int kAdjointMaxTypes = 32;
+int  adjointLdsBytes(int ntype, int tabN, int elec, int windowSlots);
+int  adjointBlocks(int forwardBlocks);
+// launches the adjoint kernel on the forward kernel's schedule, then writes
+// dL/dx (natoms x 3 floats) and dL/dq (natoms doubles) in atom order (either
+// may be NULL) and dL/dC6, dL/dC12 as numTypes x numTypes matrices (the
+// padding type dropped; either may be NULL, both NULL skip the C6/C12 sums)
+void launchAdjoint(hipStream_t s, const AdjointArgs& a, int elec, int forwardBlocks, int natoms, const int* cells,
+                   float* gradXOut, double* gradQOut, int numTypes, double* gradC6Out, double* gradC12Out);
+// g into the cluster layout: gq must hold nslot + 4 entries (the padding cluster)
+void launchConvertG(hipStream_t s, int nslot, const int* atomIndices, const float* g, float4* gq);
+
 // -- buffers.hip
 // zeroes n float4 and, when d is not NULL, nd doubles
 void launchClearF4(hipStream_t s, float4* p, int n, double* d, int nd);
 void launchConvertX(hipStream_t s, int nslot, const int* atomIndices, const float* x, float4* xq);
+// charges (atom order) into the w component of the cluster layout
+void launchSetCharges(hipStream_t s, int nslot, const int* atomIndices, const float* q, float4* xq);
 void launchReduceF(hipStream_t s, int natoms, const int* cells, const float4* fnb, float* f);
 void launchFinalize(hipStream_t s, int numEnergyPartials, const double* energyPartial, int numExclPartials,
                     const double* exclPartial, int numShiftPartials, const float* fshiftPartial, bool shiftForces,

@@ -94,6 +94,19 @@ def header_symbols() -> list[str]:
     return sorted(set(re.findall(r"\b(nbx_[a-z0-9_]+)\s*\(", text)))
 
 
+def _init_torch_device_runtime():
+    """PyTorch wheels bring their own HIP runtime.  In one process the two
+    runtimes work side by side only when torch's opens the device first (the
+    other order leaves torch without devices), so when torch is installed
+    and sees a GPU, its runtime is initialised before this library loads."""
+    try:
+        import torch
+    except ImportError:
+        return
+    if torch.cuda.is_available():
+        torch.cuda.init()
+
+
 def library() -> C.CDLL:
     """Load libnbx.so (built in-tree by gmxapi_amd.build)."""
     global _lib
@@ -102,6 +115,7 @@ def library() -> C.CDLL:
     path = _build.library_path()
     if not os.path.exists(path):
         raise NbxLibraryMissing(f"{path} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    _init_torch_device_runtime()
     lib = C.CDLL(path)
     vp, i, u, d = C.c_void_p, C.c_int, C.c_uint, C.c_double
     lib.nbx_last_error.restype = C.c_char_p
@@ -143,6 +157,8 @@ def library() -> C.CDLL:
     lib.nbx_batch_autotune.argtypes = [vp, vp, vp, C.c_uint, C.c_int, vp]
     lib.nbx_batch_get_system.argtypes = [vp, C.c_int, vp]
     lib.nbx_get_kernel_timing.argtypes = [vp, vp, vp]
+    lib.nbx_set_charges.argtypes = [vp, vp, i]
+    lib.nbx_compute_adjoint.argtypes = [vp, vp, vp, vp, vp, vp, vp, i]
     _lib = lib
     return lib
 
@@ -219,6 +235,7 @@ class Context:
         t = c6.shape[0]
         self._check(self._lib.nbx_set_nonbonded_parameters(self._h, t, _ptr(c6), _ptr(c12)),
                     "nbx_set_nonbonded_parameters")
+        self.num_types = t
 
     def set_topology(self, types, charges, excl_off, excl_idx):
         t = np.ascontiguousarray(types, np.int32)
@@ -254,6 +271,35 @@ class Context:
     def compute_device(self, x_ptr: int, f_ptr: int, flags: int = COMPUTE_FORCES):
         """Asynchronous compute on device buffers of the context's device."""
         self._check(self._lib.nbx_compute(self._h, x_ptr, f_ptr, flags, 1), "nbx_compute")
+
+    def set_charges(self, charges):
+        """New per-atom charges; the grid and pair list stay valid."""
+        q = np.ascontiguousarray(charges, np.float32)
+        self._check(self._lib.nbx_set_charges(self._h, _ptr(q), 0), "nbx_set_charges")
+
+    def set_charges_device(self, q_ptr: int):
+        self._check(self._lib.nbx_set_charges(self._h, q_ptr, 1), "nbx_set_charges")
+
+    def compute_adjoint(self, x, g):
+        """dL/dx (N, 3), dL/dq (N,) and dL/dC6, dL/dC12 (T, T) for a
+        force cotangent g = dL/dF (host arrays, synchronous)."""
+        xa = np.ascontiguousarray(x, np.float32).reshape(-1)
+        ga = np.ascontiguousarray(g, np.float32).reshape(-1)
+        gx = np.empty_like(xa)
+        gq = np.empty(len(xa) // 3, np.float64)
+        t = self.num_types
+        g6 = np.empty((t, t), np.float64)
+        g12 = np.empty((t, t), np.float64)
+        self._check(self._lib.nbx_compute_adjoint(self._h, _ptr(xa), _ptr(ga), _ptr(gx), _ptr(gq), _ptr(g6),
+                                                  _ptr(g12), 0), "nbx_compute_adjoint")
+        return gx.reshape(-1, 3), gq, g6, g12
+
+    def compute_adjoint_device(self, x_ptr: int, g_ptr: int, grad_x_ptr: int | None, grad_q_ptr: int | None,
+                               grad_c6_ptr: int | None = None, grad_c12_ptr: int | None = None):
+        """Device-pointer adjoint, asynchronous on the context's stream:
+        grad_c6/grad_c12 are (T, T) float64 device arrays (or None)."""
+        self._check(self._lib.nbx_compute_adjoint(self._h, x_ptr, g_ptr, grad_x_ptr, grad_q_ptr, grad_c6_ptr,
+                                                  grad_c12_ptr, 1), "nbx_compute_adjoint")
 
     def convert_coordinates_device(self, x_ptr: int):
         self._check(self._lib.nbx_convert_coordinates(self._h, x_ptr, 1), "nbx_convert_coordinates")

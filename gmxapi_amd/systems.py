@@ -258,11 +258,15 @@ def ensemble(n_replicas: int = 64, seed: int = WATER_SEED + 3, min_molecules: in
 
 def config(index: int) -> System:
     """The benchmark workloads: 0 = 3,000 waters in 4.48 nm, 1 = 96,000
-    waters in 14.2 nm, 2 = 150,000-atom chains in 11.45 nm."""
+    waters in 14.2 nm, 2 = 150,000-atom chains in 11.45 nm, 4 = the
+    force-matching system, 24,000 atoms of chains with 16 LJ types in
+    6.214 nm (3 is the batched ensemble, see `ensemble`)."""
     if index == 0:
         return water_box(3000, 4.48, seed=WATER_SEED + 0)
     if index == 1:
         return water_box(96000, 14.2, seed=WATER_SEED + 1)
     if index == 2:
         return hetero_chains()
+    if index == 4:
+        return hetero_chains(24_000, 6.214, ntypes=16, seed=WATER_SEED + 4)
     raise ValueError(f"unknown config {index}")

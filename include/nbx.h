@@ -113,7 +113,8 @@ int nbx_set_stream(nbx_context* context, void* stream);
 /* NbvSetupUtil::setNonBondedParameters (gmxsetup.cpp:143-166) and
  * nbnxn_atomdata_init's parameter table (atomdata.cpp:440-620): c6 and c12
  * are num_types x num_types matrices (row-major, unscaled C6/C12); the
- * kernel stores {6*C6, 12*C12} plus a zero padding type for fillers. */
+ * kernel stores {6*C6, 12*C12} plus a zero padding type for fillers.  New
+ * values for the same number of types keep the grid and pair list. */
 int nbx_set_nonbonded_parameters(nbx_context* context, int num_types, const double* c6, const double* c12);
 
 /* The Topology inputs nblib feeds to nbnxm (topology.h:72-121): per-atom
@@ -153,6 +154,28 @@ int nbx_add_nbat_f_to_f(nbx_context* context, float* f, int f_on_device);
  * calcvir.cpp:103-148).  Device pointers are used asynchronously on the
  * context's stream; host pointers make the call synchronous. */
 int nbx_compute(nbx_context* context, const float* x, float* f, unsigned flags, int on_device);
+
+/* nonbonded_verlet_t::setAtomProperties (nbnxm.h:268-272; charges into the
+ * atom data, atomdata.cpp:784-836) for new per-atom charges without
+ * re-gridding: the grid and pair list stay valid. */
+int nbx_set_charges(nbx_context* context, const float* charges, int on_device);
+
+/* Adjoint of the forces (no counterpart in the reference, whose kernels
+ * are forward only): for a loss L(F) with g = dL/dF (num_atoms x 3, atom
+ * order), at coordinates x and on the current pair list (recomputed pair by
+ * pair, nothing stored from the forward call):
+ *   grad_x   dL/dx, num_atoms x 3 floats (the Hessian-vector product H g);
+ *   grad_q   dL/dq, num_atoms doubles;
+ *   grad_c6, grad_c12  dL/dC6 and dL/dC12 for the matrices given to
+ *            nbx_set_nonbonded_parameters, num_types^2 doubles each, every
+ *            entry taken as independent (for a symmetric parameter
+ *            C_ij = C_ji, i != j, the derivative is grad[i][j] + grad[j][i]).
+ * Any output may be NULL (no C6/C12 output skips their accumulation).  All
+ * pointers are device pointers, used asynchronously on the context's
+ * stream, when on_device is set; host pointers make the call synchronous.
+ * At most 31 particle types. */
+int nbx_compute_adjoint(nbx_context* context, const float* x, const float* g, float* grad_x, double* grad_q,
+                        double* grad_c6, double* grad_c12, int on_device);
 
 /* Energies of the last compute with NBX_COMPUTE_ENERGY (gmx_enerdata_t
  * terms reduced by reduce_energies_over_lists, kernel_common.cpp:109-133):

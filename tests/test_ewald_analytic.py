@@ -1,6 +1,6 @@
 """Pins the analytical Ewald correction used by the HIP kernel
 (NBX_EWALD_ANALYTICAL): the rational fits P(s)/Q(s), s = (beta r)^2, whose
-coefficients are in gmxapi_amd/csrc/kernel.hip, evaluated in float32 the way
+coefficients are in gmxapi_amd/csrc/kernel_common.h, evaluated in float32 the way
 the kernel does (Horner with fused multiply-adds, then a reciprocal), against
 the closed forms the reference's SIMD kernels approximate
 (src/gromacs/simd/tests/simd_math.cpp:938-977):
@@ -14,7 +14,7 @@ import re
 
 import numpy as np
 
-SRC = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "gmxapi_amd", "csrc", "kernel.hip")
+SRC = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "gmxapi_amd", "csrc", "kernel_common.h")
 
 
 def _coefficients():
