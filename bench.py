@@ -21,6 +21,15 @@ per pair (107 for Ewald + LJ with energies, gmxlib/nrnb.cpp:106) times the
 pairs the kernel computes (16 per cluster pair in its list), divided by the
 kernel's duration measured with HIP events on its stream.
 
+The same line carries the metric's two other parts, measured in the same
+run: "batched" - the 64-replica water ensemble (replica sizes log-uniform in
+1,000-16,000 molecules) as one batched launch, eager and replayed from a
+captured HIP graph, against the same 64 systems launched one after another
+(aggregate useful pairs/s); "fwd_bwd" - the 24,000-atom, 16-type
+force-matching system: the forward forces call against the adjoint call
+that returns dL/dx, dL/dq, dL/dC6 and dL/dC12 (the backward of the
+torch.autograd.Function).
+
 One GPU only: --gpus above 1 exits with a message.
 """
 import argparse
@@ -81,6 +90,108 @@ def cpu_baseline(system, budget_s=30.0):
     return sorted(times)[len(times) // 2], t_setup, len(times)
 
 
+def _time(fn, stream, reps, warm=3):
+    import torch
+
+    for _ in range(warm):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def bench_ensemble(ewald, stream, reps=20):
+    """64 water boxes: sequential per-system launches vs one batched launch
+    (eager and HIP-graph replay), forces + energies."""
+    import numpy as np
+    import torch
+
+    from gmxapi_amd import nbx, systems
+
+    reps_ = systems.ensemble(64)
+    b = nbx.batch_for(reps_, ewald_correction=ewald)
+    b.set_stream(stream.cuda_stream)
+    x = np.concatenate([r.x.astype(np.float32).reshape(-1) for r in reps_])
+    xd = torch.from_numpy(x).to("cuda")
+    fd = torch.zeros_like(xd)
+    flags = nbx.COMPUTE_FORCES | nbx.COMPUTE_ENERGY
+    offs = np.cumsum([0] + [r.natoms for r in reps_])
+    ctxs = [b.system(k) for k in range(len(reps_))]
+    useful = sum(c.count_pairs(r.rcoulomb) - 3 * (r.natoms // 3) for c, r in zip(ctxs, reps_))
+
+    def sequential():
+        for k, c in enumerate(ctxs):
+            c.compute_device(xd.data_ptr() + 12 * int(offs[k]), fd.data_ptr() + 12 * int(offs[k]), flags)
+
+    seq_ms = _time(sequential, stream, reps)
+    chosen = b.autotune(xd.data_ptr(), fd.data_ptr(), flags, repeats=5)
+    eager_ms = _time(lambda: b.compute_device(xd.data_ptr(), fd.data_ptr(), flags), stream, reps)
+    b.capture_graph(xd.data_ptr(), fd.data_ptr(), flags)
+    graph_ms = _time(b.launch_graph, stream, reps)
+    return {
+        "workload": "64 water boxes (config-0 recipe), 1,000-16,000 molecules each, log-uniform",
+        "natoms": int(offs[-1]),
+        "useful_pairs": int(useful),
+        "sequential_ms": seq_ms,
+        "batched_ms": eager_ms,
+        "graph_ms": graph_ms,
+        "pairs_per_s_sequential": useful / (seq_ms * 1e-3),
+        "pairs_per_s_graph": useful / (graph_ms * 1e-3),
+        "speedup_graph_vs_sequential": seq_ms / graph_ms,
+        "speedup_batched_vs_sequential": seq_ms / eager_ms,
+        "autotuned": chosen,
+        "step": "F + E for all 64 systems (x convert, pair kernel, FP64 exclusion energies, per-system sums, f reduce)",
+    }
+
+
+def bench_forward_backward(ewald, stream, reps=20):
+    """The 24,000-atom, 16-type force-matching system: forward forces vs the
+    adjoint returning dL/dx, dL/dq, dL/dC6, dL/dC12."""
+    import numpy as np
+    import torch
+
+    from gmxapi_amd import nbx, systems
+
+    s = systems.config(4)
+    ctx = nbx.context_for(s, ewald_correction=ewald)
+    ctx.set_stream(stream.cuda_stream)
+    dev = torch.device("cuda:0")
+    x = torch.tensor(s.x, dtype=torch.float32, device=dev).contiguous()
+    f = torch.empty_like(x)
+    g = torch.tensor(np.random.default_rng(1).normal(size=s.x.shape), dtype=torch.float32, device=dev)
+    gx = torch.empty_like(x)
+    gq = torch.empty(s.natoms, dtype=torch.float64, device=dev)
+    g6 = torch.empty((s.ntypes, s.ntypes), dtype=torch.float64, device=dev)
+    g12 = torch.empty_like(g6)
+    useful = ctx.count_pairs(s.rcoulomb) - excluded_pairs_within(s, s.rcoulomb)
+
+    def fwd():
+        ctx.compute_device(x.data_ptr(), f.data_ptr(), nbx.COMPUTE_FORCES)
+
+    def bwd():
+        ctx.compute_adjoint_device(x.data_ptr(), g.data_ptr(), gx.data_ptr(), gq.data_ptr(), g6.data_ptr(),
+                                   g12.data_ptr())
+
+    fwd_ms = _time(fwd, stream, reps)
+    bwd_ms = _time(bwd, stream, reps)
+    return {
+        "workload": f"force matching: {s.natoms} atoms, 10-atom chains, 16 LJ types, {s.box[0]} nm box, "
+                    f"rc {s.rcoulomb} nm, rlist {s.rlist} nm",
+        "useful_pairs": int(useful),
+        "fwd_ms": fwd_ms,
+        "bwd_ms": bwd_ms,
+        "fwd_plus_bwd_over_fwd": (fwd_ms + bwd_ms) / fwd_ms,
+        "pairs_per_s_fwd": useful / (fwd_ms * 1e-3),
+        "pairs_per_s_fwd_bwd": useful / ((fwd_ms + bwd_ms) * 1e-3),
+        "bwd_outputs": "dL/dx (N x 3), dL/dq (N, FP64), dL/dC6 and dL/dC12 (16 x 16, FP64)",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +201,7 @@ def main():
     ap.add_argument("--ewald", default="analytical", choices=["table", "analytical"],
                     help="Ewald correction: the reference's spline table or the analytical form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the batched and forward+backward parts")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -226,6 +338,11 @@ def main():
             "pairs_per_launch": computed_pairs,
         },
     }
+    if not args.no_extras:
+        t0 = time.perf_counter()
+        out["batched"] = bench_ensemble(args.ewald, stream)
+        out["fwd_bwd"] = bench_forward_backward(args.ewald, stream)
+        log(f"[bench] batched and forward+backward parts in {time.perf_counter() - t0:.1f} s")
     if not args.no_cpu_baseline:
         t_step, t_setup, nrep = cpu_baseline(s)
         out["cpu_baseline"] = {
