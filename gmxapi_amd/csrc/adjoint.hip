@@ -254,7 +254,7 @@ __global__ __launch_bounds__(kAdjThreads) void adjointKernel(const AdjointArgs a
         const unsigned W0     = static_cast<unsigned>(uniformInt(cFirst.z));
         const int      T      = (uniformInt(cLast.z) - static_cast<int>(W0)) / kTileJ + uniformInt(cLast.w);
 
-        int    g = uniformInt(ws.y), chGrpEnd = 0, uoff = 0, U = 0, c0 = 0, nOwn = 0, G = 0, shb = 0;
+        int    g = uniformInt(ws.y), chGrpEnd = 0, uoff = 0, U = 0, c0 = 0, nOwn = 0, G = 0;
         IAtoms I;
         IGrad  Gi;
         float4 sv;
@@ -271,7 +271,6 @@ __global__ __launch_bounds__(kAdjThreads) void adjointKernel(const AdjointArgs a
             c0             = uniformInt(gi.x);
             nOwn           = uniformInt(gi.y) & 0xffff;
             G              = uniformInt(gi.y) >> 16;
-            shb            = uniformInt(gi.z);
             for (int t = lane; t < (G + U + 1) * 4; t += kWave)
             {
                 reinterpret_cast<float4*>(win)[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -317,7 +316,7 @@ __global__ __launch_bounds__(kAdjThreads) void adjointKernel(const AdjointArgs a
                 }
             }
         };
-        auto beginChunk = [&](const int4 cd, int t) {
+        auto beginChunk = [&](const int4 cd, const float4 sf, int t) {
             ci              = uniformInt(cd.x);
             const int flags = uniformInt(cd.y);
             sh              = flags & 0xff;
@@ -340,8 +339,7 @@ __global__ __launch_bounds__(kAdjThreads) void adjointKernel(const AdjointArgs a
                 Gi.y[i]        = uniformFloat(q.y);
                 Gi.z[i]        = uniformFloat(q.z);
             }
-            const float4 sf = a.shiftVec[shb + sh];
-            sv              = make_float4(uniformFloat(sf.x), uniformFloat(sf.y), uniformFloat(sf.z), 0.0f);
+            sv = make_float4(uniformFloat(sf.x), uniformFloat(sf.y), uniformFloat(sf.z), 0.0f);
 #pragma unroll
             for (int k = 0; k < 16; k++)
             {
@@ -389,8 +387,9 @@ __global__ __launch_bounds__(kAdjThreads) void adjointKernel(const AdjointArgs a
                 unsafeAtomicAdd(a.gradQ + k, static_cast<double>(aj.w));
             }
         };
-        int4 cdNext  = a.chunks[min(chIdx + 1, chAll1 - 1)];
-        auto advance = [&](int t) {
+        int4   cdNext = a.chunks[min(chIdx + 1, chAll1 - 1)];
+        float4 svNext = a.chunkShift[min(chIdx + 1, chAll1 - 1)];
+        auto   advance = [&](int t) {
             endChunk();
             chIdx++;
             while (chIdx == chGrpEnd)
@@ -399,8 +398,9 @@ __global__ __launch_bounds__(kAdjThreads) void adjointKernel(const AdjointArgs a
                 g++;
                 beginGroup(g);
             }
-            beginChunk(cdNext, t);
+            beginChunk(cdNext, svNext, t);
             cdNext = a.chunks[min(chIdx + 1, chAll1 - 1)];
+            svNext = a.chunkShift[min(chIdx + 1, chAll1 - 1)];
         };
 
         // the forward kernel's pipeline: work entries two tiles ahead, j-atom
@@ -414,7 +414,7 @@ __global__ __launch_bounds__(kAdjThreads) void adjointKernel(const AdjointArgs a
         int    tA = ldg(a.type, eA.x * kClusterSize + jj);
         float4 gA = ldg(a.gq, eA.x * kClusterSize + jj);
         beginGroup(g);
-        beginChunk(cFirst, 0);
+        beginChunk(cFirst, a.chunkShift[chAll0], 0);
         for (int t = 0; t < T; t += 2)
         {
             const float4 xB  = ldg(a.xq, eB.x * kClusterSize + jj);

@@ -334,8 +334,11 @@ void buildWorkGroups(nbx_context* c)
         ga.groupInfo = c->groupInfo.p;
         ga.windowCap = slots;
         launchBuildGroups(c->stream, ga, true);
+        c->chunkShift.ensure(std::max(chunkTotal, 1));
+        launchChunkShifts(c->stream, chunkTotal, c->chunks.p, c->shiftVec.p, c->chunkShift.p);
         check(hipGetLastError(), "group kernels");
         c->jentry.swap(c->jentryTmp);
+        buildOccurrences(c->stream, c->gather, ng, c->groups.p, c->groupInfo.p, c->unionList.p, nc);
         c->groupSize   = G;
         c->ngroups     = ng;
         c->nchunk      = chunkTotal;
@@ -892,6 +895,7 @@ int nbx_construct_pairlist(nbx_context* c)
         launchWaveSchedule(c->stream, numWaves, c->nchunk, c->chunks.p, c->nwork, kChunkWeight, c->ngroups,
                            c->groups.p, c->waveStart.p);
         check(hipGetLastError(), "schedule kernel");
+        buildSegments(c->stream, c->gather, c->ngroups, c->groups.p, c->groupInfo.p, numWaves, c->waveStart.p);
         c->energyPartial.ensure(2 * static_cast<size_t>(c->ngroups + numWaves));
         c->exclPartial.ensure(kExclMaxBlocks);
         c->fshiftPartial.ensure(static_cast<size_t>(c->numBlocks) * kNumShifts * 3);
@@ -948,9 +952,11 @@ int nbx_dispatch_kernel(nbx_context* c, unsigned flags)
         a.nbfp          = c->nbfp.p;
         a.ntype         = c->ntype;
         a.tabFDV0       = c->tabFDV0.p;
-        a.shiftVec      = c->shiftVec.p;
+        a.chunkShift    = c->chunkShift.p;
         a.ic            = c->ic;
         a.waveStart     = c->waveStart.p;
+        a.segInfo       = c->gather.segInfo.p;
+        a.staging       = c->gather.staging.p;
         a.fnb           = c->fnb.p;
         a.energyPartial = c->energyPartial.p;
         a.fshiftPartial = c->fshiftPartial.p;
@@ -965,6 +971,8 @@ int nbx_dispatch_kernel(nbx_context* c, unsigned flags)
             check(hipEventRecord(c->timingEvents[2 * c->timingRecorded + 1], c->stream), "hipEventRecord");
             c->timingRecorded++;
         }
+        launchGatherForces(c->stream, c->ncluster, c->gather.occOff.p, c->gather.occ.p, c->gather.segInfo.p,
+                           c->gather.staging.p, c->fnb.p);
         int numExcl = 0;
         if (energy)
         {
@@ -1176,7 +1184,7 @@ int nbx_compute_adjoint(nbx_context* c, const float* x, const float* g, float* g
         a.nbfp         = c->nbfp.p;
         a.ntype        = c->ntype;
         a.tabFDV0      = c->tabFDV0.p;
-        a.shiftVec     = c->shiftVec.p;
+        a.chunkShift   = c->chunkShift.p;
         a.ic           = c->ic;
         a.waveStart    = c->waveStart.p;
         a.gradX        = c->gradXnb.p;

@@ -38,10 +38,11 @@ struct nbx_batch
     int  groupSize = 0, blocksPerCU = 0, chunkWeight = kChunkWeight;
     long long ncj = 0;
 
-    DevBuf<float4> xq, fnb, shiftVec;
+    DevBuf<float4> xq, fnb, shiftVec, chunkShift;
     DevBuf<int>    nbtype, atomIndices, cells, exclOff, exclIdx, unionList, atomSystem, groupOff, atomOff, slotOff;
     DevBuf<int2>   waveStart;
     DevBuf<float>  boxes;
+    WindowGather   gather;
     DevBuf<uint2>  work;
     DevBuf<int4>   chunks, groups, groupInfo;
     DevBuf<double> energyPartial, atomEnergy, energy;
@@ -126,6 +127,7 @@ void scheduleBatch(nbx_batch* b, int blocksPerCU, int chunkWeight)
                        b->waveStart.p);
     launchSystemSlots(b->stream, R, b->groupOff.p, b->ngroups, b->groups.p, numWaves, b->waveStart.p, b->slotOff.p);
     check(hipGetLastError(), "schedule kernels");
+    buildSegments(b->stream, b->gather, b->ngroups, b->groups.p, b->groupInfo.p, numWaves, b->waveStart.p);
     check(hipStreamSynchronize(b->stream), "schedule sync");
 }
 
@@ -150,7 +152,7 @@ void buildBatch(nbx_batch* b, int G, int blocksPerCU, int chunkWeight)
     std::vector<int>    hType, hAi, hCells, hExOff{ 0 }, hExIdx, hUnion, hAtomSys, hGroupOff{ 0 }, hAtomOff{ 0 };
     std::vector<uint2>  hWork;
     std::vector<int4>   hChunks, hGroups, hInfo;
-    std::vector<float4> hShift;
+    std::vector<float4> hShift, hChunkShift;
     int                 clusterOff = 0, atomOffset = 0, maxSlots = 0;
     long long           ncj        = 0;
     int                 totClusters = 0;
@@ -174,6 +176,7 @@ void buildBatch(nbx_batch* b, int G, int blocksPerCU, int chunkWeight)
         std::vector<int>    ce = download(c->cells.p, na);
         std::vector<uint2>  wk = download(c->work.p, c->nwork);
         std::vector<int4>   ch = download(c->chunks.p, c->nchunk);
+        std::vector<float4> cs = download(c->chunkShift.p, c->nchunk);
         std::vector<int4>   gr = download(c->groups.p, c->ngroups);
         std::vector<int4>   gi = download(c->groupInfo.p, c->ngroups);
         const int4          last = gr.back();
@@ -219,6 +222,7 @@ void buildBatch(nbx_batch* b, int G, int blocksPerCU, int chunkWeight)
             hUnion.push_back(v + clusterOff);
         }
         hShift.insert(hShift.end(), sv.begin(), sv.end());
+        hChunkShift.insert(hChunkShift.end(), cs.begin(), cs.end());
         for (int k = 0; k < na; k++)
         {
             hExOff.push_back(hExOff.back() + (eo[k + 1] - eo[k]));
@@ -246,6 +250,7 @@ void buildBatch(nbx_batch* b, int G, int blocksPerCU, int chunkWeight)
     upload(b->groupInfo, hInfo);
     upload(b->unionList, hUnion);
     upload(b->shiftVec, hShift);
+    upload(b->chunkShift, hChunkShift);
     upload(b->exclOff, hExOff);
     upload(b->exclIdx, hExIdx);
     upload(b->atomSystem, hAtomSys);
@@ -264,6 +269,7 @@ void buildBatch(nbx_batch* b, int G, int blocksPerCU, int chunkWeight)
     b->windowSlots = maxSlots;
     b->ncj         = ncj;
     b->groupSize   = G > 0 ? G : b->sys[0]->groupSize;
+    buildOccurrences(b->stream, b->gather, b->ngroups, b->groups.p, b->groupInfo.p, b->unionList.p, totClusters);
     scheduleBatch(b, blocksPerCU, chunkWeight);
     b->built      = true;
     b->haveEnergy = false;
@@ -291,13 +297,17 @@ void enqueueCompute(nbx_batch* b, const float* x, float* f, unsigned flags)
     a.nbfp         = c0->nbfp.p;
     a.ntype        = c0->ntype;
     a.tabFDV0      = c0->tabFDV0.p;
-    a.shiftVec     = b->shiftVec.p;
+    a.chunkShift   = b->chunkShift.p;
     a.ic           = c0->ic;
     a.waveStart    = b->waveStart.p;
+    a.segInfo      = b->gather.segInfo.p;
+    a.staging      = b->gather.staging.p;
     a.fnb          = b->fnb.p;
     a.energyPartial = b->energyPartial.p;
     a.fshiftPartial = nullptr;
     launchNonbonded(b->stream, a, c0->elec, energy, false, b->numBlocks);
+    launchGatherForces(b->stream, b->totClusters, b->gather.occOff.p, b->gather.occ.p, b->gather.segInfo.p,
+                       b->gather.staging.p, b->fnb.p);
     if (energy)
     {
         ExclusionArgs e{};

@@ -63,6 +63,40 @@ __global__ void convertXKernel(int nslot, const int* __restrict__ atomIndices, c
     }
 }
 
+// Forces of each cluster atom from the staged windows that hold it, added
+// in (group, wavefront) order, then added to the cluster force buffer (which
+// holds the contributions of clusters that spilled out of a window).
+__global__ void gatherForcesKernel(int nslot, const int* __restrict__ occOff, const int2* __restrict__ occ,
+                                   const int4* __restrict__ segInfo, const float4* __restrict__ staging,
+                                   float4* __restrict__ fnb)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nslot)
+    {
+        return;
+    }
+    const int c = k >> 2;
+    const int a = k & 3;
+    float     fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    for (int o = occOff[c]; o < occOff[c + 1]; o++)
+    {
+        const int2 e  = occ[o];
+        const int4 si = segInfo[e.x];
+        for (int w = 0; w < si.z; w++)
+        {
+            const float* p = reinterpret_cast<const float*>(staging + si.x + w * si.w) + e.y * 12 + a * 3;
+            fx += p[0];
+            fy += p[1];
+            fz += p[2];
+        }
+    }
+    float4 f = fnb[k];
+    f.x += fx;
+    f.y += fy;
+    f.z += fz;
+    fnb[k] = f;
+}
+
 __global__ void setChargesKernel(int nslot, const int* __restrict__ atomIndices, const float* __restrict__ q,
                                  float4* __restrict__ xq)
 {
@@ -477,6 +511,16 @@ void launchConvertX(hipStream_t s, int nslot, const int* atomIndices, const floa
     if (nslot > 0)
     {
         convertXKernel<<<(nslot + 255) / 256, 256, 0, s>>>(nslot, atomIndices, x, xq);
+    }
+}
+
+void launchGatherForces(hipStream_t s, int ncluster, const int* occOff, const int2* occ, const int4* segInfo,
+                        const float4* staging, float4* fnb)
+{
+    const int nslot = ncluster * kClusterSize;
+    if (nslot > 0)
+    {
+        gatherForcesKernel<<<(nslot + 255) / 256, 256, 0, s>>>(nslot, occOff, occ, segInfo, staging, fnb);
     }
 }
 

@@ -81,6 +81,59 @@ struct DevBuf
     ~DevBuf() { release(); }
 };
 
+// The pair kernel's staged j-force windows and the lists that gather them
+// (one per context, one per batch).
+struct WindowGather
+{
+    DevBuf<int>    occCount, occOff, segSize, segBase;
+    DevBuf<int2>   occ;
+    DevBuf<int4>   segInfo;
+    DevBuf<float4> staging;
+};
+
+// Occurrence lists of the clusters in the groups' windows (once per list).
+inline void buildOccurrences(hipStream_t s, WindowGather& w, int ngroups, const int4* groups, const int4* groupInfo,
+                             const int* unionList, int ncluster)
+{
+    w.occCount.ensure(ncluster + 1);
+    w.occOff.ensure(ncluster + 1);
+    if (ncluster == 0)
+    {
+        check(hipMemsetAsync(w.occOff.p, 0, sizeof(int), s), "clear occurrences");
+        w.occ.ensure(1);
+        return;
+    }
+    check(hipMemsetAsync(w.occCount.p, 0, sizeof(int) * ncluster, s), "clear occurrences");
+    launchOccurrences(s, ngroups, groups, groupInfo, unionList, ncluster, w.occCount.p, nullptr, nullptr, false);
+    launchExclusiveScan(s, ncluster, w.occCount.p, w.occOff.p, w.occOff.p + ncluster);
+    int total = 0;
+    check(hipMemcpyAsync(&total, w.occOff.p + ncluster, sizeof(int), hipMemcpyDeviceToHost, s), "read occurrences");
+    check(hipStreamSynchronize(s), "occurrence sync");
+    w.occ.ensure(std::max(total, 1));
+    check(hipMemsetAsync(w.occCount.p, 0, sizeof(int) * ncluster, s), "clear occurrences");
+    launchOccurrences(s, ngroups, groups, groupInfo, unionList, ncluster, w.occCount.p, w.occOff.p, w.occ.p, true);
+    check(hipGetLastError(), "occurrence kernels");
+}
+
+// Staging layout of a schedule (whenever the schedule changes).
+inline void buildSegments(hipStream_t s, WindowGather& w, int ngroups, const int4* groups, const int4* groupInfo,
+                          int numWaves, const int2* waveStart)
+{
+    w.segSize.ensure(ngroups + 1);
+    w.segBase.ensure(ngroups + 1);
+    w.segInfo.ensure(std::max(ngroups, 1));
+    int total = 0;
+    if (ngroups > 0)
+    {
+        launchSegments(s, ngroups, groups, groupInfo, numWaves, waveStart, w.segSize.p, w.segBase.p,
+                       w.segBase.p + ngroups, w.segInfo.p);
+        check(hipMemcpyAsync(&total, w.segBase.p + ngroups, sizeof(int), hipMemcpyDeviceToHost, s), "read segments");
+        check(hipStreamSynchronize(s), "segment sync");
+    }
+    w.staging.ensure(std::max(total, 1));
+    check(hipGetLastError(), "segment kernels");
+}
+
 inline int fail(int code, const std::string& msg)
 {
     lastError() = msg;
@@ -164,12 +217,14 @@ struct nbx_context
     int            groupSize = 0, ngroups = 0, nchunk = 0, windowSlots = 0, nwork = 0;
     nbx::DevBuf<int>    unionCount, chunkCount, workCount, overflow, unionOff, chunkOff, workOff, unionList, groupStats;
     nbx::DevBuf<int4>   chunks, groups, groupInfo;
+    nbx::DevBuf<float4> chunkShift;
     nbx::DevBuf<uint2>  work;
 
     // outputs
     int             numBlocks = 0;
     nbx::DevBuf<double>  energyPartial, energy, fshift, virialPartial, virial;
     nbx::DevBuf<int2>    waveStart; // static schedule: each wavefront's first chunk and its group
+    nbx::WindowGather    gather;
     nbx::DevBuf<double>  exclPartial;
     nbx::DevBuf<float>   fshiftPartial, fOut;
     // adjoint

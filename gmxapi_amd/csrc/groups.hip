@@ -298,22 +298,123 @@ __global__ void maxReduceKernel(int n, const int* __restrict__ v, int* __restric
     }
 }
 
-__global__ void waveScheduleKernel(int numWaves, int nchunk, const int4* __restrict__ chunks, int nwork,
-                                   int chunkWeight, int ngroups, const int4* __restrict__ groups, int2* __restrict__ out)
+// Occurrences of every cluster in the groups' windows: its own slot in its
+// group and its slots in other groups' windowed unions.  One block per group.
+template<bool kFill>
+__global__ void occurrenceKernel(int ngroups, const int4* __restrict__ groups, const int4* __restrict__ groupInfo,
+                                 const int* __restrict__ unionList, int* __restrict__ count,
+                                 const int* __restrict__ occOff, int2* __restrict__ occ)
 {
-    const int w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w > numWaves)
+    const int  g    = blockIdx.x;
+    const int4 grp  = groups[g];
+    const int4 gi   = groupInfo[g];
+    const int  nOwn = gi.y & 0xffff;
+    const int  G    = gi.y >> 16;
+    for (int t = threadIdx.x; t < nOwn + grp.w; t += blockDim.x)
+    {
+        const int c = t < nOwn ? gi.x + t : unionList[grp.z + t - nOwn];
+        const int k = atomicAdd(&count[c], 1);
+        if (kFill)
+        {
+            occ[occOff[c] + k] = make_int2(g, t < nOwn ? t : G + (t - nOwn));
+        }
+    }
+}
+
+// each cluster's occurrences in (group, slot) order, so that the force
+// gather adds them in a fixed order
+__global__ void occurrenceSortKernel(int ncluster, const int* __restrict__ occOff, int2* __restrict__ occ)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncluster)
     {
         return;
     }
-    // cost before chunk k: its work offset plus chunkWeight per earlier chunk
-    const long long total  = static_cast<long long>(nwork) + static_cast<long long>(chunkWeight) * nchunk;
-    const long long target = total * w / numWaves;
-    int             lo = 0, hi = nchunk;
+    const int b = occOff[c], e = occOff[c + 1];
+    for (int i = b + 1; i < e; i++)
+    {
+        const int2 v = occ[i];
+        int        j = i - 1;
+        while (j >= b && (occ[j].x > v.x || (occ[j].x == v.x && occ[j].y > v.y)))
+        {
+            occ[j + 1] = occ[j];
+            j--;
+        }
+        occ[j + 1] = v;
+    }
+}
+
+// the last wavefront whose run starts at or before chunk x
+__device__ int waveOfChunk(int numWaves, const int2* __restrict__ waveStart, int x)
+{
+    int lo = 0, hi = numWaves - 1;
+    while (lo < hi)
+    {
+        const int m = (lo + hi + 1) / 2;
+        if (waveStart[m].x <= x)
+        {
+            lo = m;
+        }
+        else
+        {
+            hi = m - 1;
+        }
+    }
+    return lo;
+}
+
+// Per group: the wavefronts that run its chunks (first, count) and its
+// window size in float4 (3 per cluster slot); size = their product.
+__global__ void segmentKernel(int ngroups, const int4* __restrict__ groups, const int4* __restrict__ groupInfo,
+                              int numWaves, const int2* __restrict__ waveStart, int* __restrict__ size,
+                              int4* __restrict__ segInfo)
+{
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= ngroups)
+    {
+        return;
+    }
+    const int4 grp = groups[h];
+    const int  win = ((groupInfo[h].y >> 16) + grp.w) * 3;
+    if (grp.x >= grp.y)
+    {
+        size[h]    = 0;
+        segInfo[h] = make_int4(0, 0, 0, win);
+        return;
+    }
+    const int wf = waveOfChunk(numWaves, waveStart, grp.x);
+    const int wl = waveOfChunk(numWaves, waveStart, grp.y - 1);
+    size[h]      = (wl - wf + 1) * win;
+    segInfo[h]   = make_int4(0, wf, wl - wf + 1, win);
+}
+
+__global__ void segmentBaseKernel(int ngroups, const int* __restrict__ base, int4* __restrict__ segInfo)
+{
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h < ngroups)
+    {
+        segInfo[h].x = base[h];
+    }
+}
+
+__global__ void chunkShiftKernel(int nchunk, const int4* __restrict__ chunks, const float4* __restrict__ shiftVec,
+                                 float4* __restrict__ out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nchunk)
+    {
+        out[k] = shiftVec[chunks[k].y & 0xff];
+    }
+}
+
+// the group holding chunk k: the first whose chunk range ends after it
+__device__ int groupOfChunk(int ngroups, const int4* __restrict__ groups, int k)
+{
+    int lo = 0, hi = ngroups;
     while (lo < hi)
     {
         const int m = (lo + hi) / 2;
-        if (chunks[m].z + static_cast<long long>(chunkWeight) * m < target)
+        if (groups[m].y <= k)
         {
             lo = m + 1;
         }
@@ -322,25 +423,71 @@ __global__ void waveScheduleKernel(int numWaves, int nchunk, const int4* __restr
             hi = m;
         }
     }
-    if (w == numWaves)
+    return lo;
+}
+
+// One workgroup.  Wavefront w's target start is the first chunk whose cost
+// offset (work offset plus chunkWeight per earlier chunk) reaches w/numWaves
+// of the total; equal targets are merged, so the runs are non-empty and in
+// order, and the wavefronts left over (more wavefronts than distinct
+// starts) get empty runs at the end.  A group's chunks are then run by
+// consecutive wavefronts, each of which writes its staging segment.
+__global__ __launch_bounds__(1024) void waveScheduleKernel(int numWaves, int nchunk, const int4* __restrict__ chunks,
+                                                          int nwork, int chunkWeight, int ngroups,
+                                                          const int4* __restrict__ groups, int2* __restrict__ out)
+{
+    __shared__ int sLo[1024];
+    __shared__ int sFlag[1024];
+    const long long total = static_cast<long long>(nwork) + static_cast<long long>(chunkWeight) * nchunk;
+    int             carry = 0, prevLo = -1;
+    for (int base = 0; base < numWaves; base += 1024)
     {
-        lo = nchunk;
+        const int w  = base + threadIdx.x;
+        int       lo = nchunk;
+        if (w < numWaves)
+        {
+            const long long target = total * w / numWaves;
+            int             hi     = nchunk;
+            lo                     = 0;
+            while (lo < hi)
+            {
+                const int m = (lo + hi) / 2;
+                if (chunks[m].z + static_cast<long long>(chunkWeight) * m < target)
+                {
+                    lo = m + 1;
+                }
+                else
+                {
+                    hi = m;
+                }
+            }
+        }
+        sLo[threadIdx.x] = lo;
+        __syncthreads();
+        const int prev = threadIdx.x > 0 ? sLo[threadIdx.x - 1] : prevLo;
+        const int flag = (w < numWaves && lo < nchunk && lo != prev) ? 1 : 0;
+        sFlag[threadIdx.x] = flag;
+        __syncthreads();
+        for (int off = 1; off < 1024; off *= 2)
+        {
+            const int t = threadIdx.x >= off ? sFlag[threadIdx.x - off] : 0;
+            __syncthreads();
+            sFlag[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (flag)
+        {
+            const int r = carry + sFlag[threadIdx.x] - 1;
+            out[r]      = make_int2(lo, groupOfChunk(ngroups, groups, lo));
+        }
+        carry += sFlag[1023];
+        prevLo = sLo[1023];
+        __syncthreads();
     }
-    // the group holding chunk lo: the first whose chunk range ends after it
-    int glo = 0, ghi = ngroups;
-    while (glo < ghi)
+    for (int r = carry + threadIdx.x; r <= numWaves; r += blockDim.x)
     {
-        const int m = (glo + ghi) / 2;
-        if (groups[m].y <= lo)
-        {
-            glo = m + 1;
-        }
-        else
-        {
-            ghi = m;
-        }
+        out[r] = make_int2(nchunk, ngroups);
     }
-    out[w] = make_int2(lo, glo);
 }
 
 __global__ void systemSlotKernel(int numSystems, const int* __restrict__ groupOff, int ngroups,
@@ -379,11 +526,46 @@ __global__ void systemSlotKernel(int numSystems, const int* __restrict__ groupOf
 
 } // namespace
 
+void launchOccurrences(hipStream_t s, int ngroups, const int4* groups, const int4* groupInfo, const int* unionList,
+                       int ncluster, int* count, int* occOff, int2* occ, bool fill)
+{
+    if (ngroups == 0)
+    {
+        return;
+    }
+    if (!fill)
+    {
+        occurrenceKernel<false><<<ngroups, 256, 0, s>>>(ngroups, groups, groupInfo, unionList, count, nullptr, nullptr);
+        return;
+    }
+    occurrenceKernel<true><<<ngroups, 256, 0, s>>>(ngroups, groups, groupInfo, unionList, count, occOff, occ);
+    occurrenceSortKernel<<<(ncluster + 255) / 256, 256, 0, s>>>(ncluster, occOff, occ);
+}
+
+void launchSegments(hipStream_t s, int ngroups, const int4* groups, const int4* groupInfo, int numWaves,
+                    const int2* waveStart, int* size, int* base, int* total, int4* segInfo)
+{
+    if (ngroups == 0)
+    {
+        return;
+    }
+    segmentKernel<<<(ngroups + 255) / 256, 256, 0, s>>>(ngroups, groups, groupInfo, numWaves, waveStart, size, segInfo);
+    launchExclusiveScan(s, ngroups, size, base, total);
+    segmentBaseKernel<<<(ngroups + 255) / 256, 256, 0, s>>>(ngroups, base, segInfo);
+}
+
+void launchChunkShifts(hipStream_t s, int nchunk, const int4* chunks, const float4* shiftVec, float4* out)
+{
+    if (nchunk > 0)
+    {
+        chunkShiftKernel<<<(nchunk + 255) / 256, 256, 0, s>>>(nchunk, chunks, shiftVec, out);
+    }
+}
+
 void launchWaveSchedule(hipStream_t s, int numWaves, int nchunk, const int4* chunks, int nwork, int chunkWeight,
                         int ngroups, const int4* groups, int2* out)
 {
-    waveScheduleKernel<<<(numWaves + 256) / 256, 256, 0, s>>>(numWaves, nchunk, chunks, nwork, chunkWeight, ngroups,
-                                                             groups, out);
+    waveScheduleKernel<<<1, 1024, 0, s>>>(numWaves, nchunk, chunks, nwork, chunkWeight, ngroups, groups, out);
 }
 
 void launchSystemSlots(hipStream_t s, int numSystems, const int* groupOff, int ngroups, const int4* groups,

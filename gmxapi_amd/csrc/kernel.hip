@@ -24,9 +24,11 @@
 //     wavefronts).  A wavefront works alone on one group at a time with a
 //     private LDS window holding one float3 per atom of each distinct
 //     j-cluster of the group and of its own G clusters: j-forces are added to
-//     it with plain read-modify-write (no atomics, no barriers), and the
-//     window is flushed to the cluster force buffer with one global atomic per
-//     nonzero atom component when the wavefront moves to its next group.
+//     it with plain read-modify-write (no atomics, no barriers).  When the
+//     wavefront moves to its next group the window is written with plain
+//     coalesced stores to its own staging segment, and a gather kernel
+//     (buffers.hip) adds each cluster's staged copies in a fixed order after
+//     the launch: no float atomics, and bitwise reproducible forces.
 //   * In a tile the four i-atoms are wave-uniform (scalar registers) and lane
 //     l owns j-atom l & 3 of j-entry l >> 2: 16 j-clusters per tile, four
 //     i-atoms per lane, so one VALU instruction evaluates 64 pairs.  The
@@ -158,8 +160,10 @@ __device__ __forceinline__ void pairI(const IAtoms& I, const JAtom& J, const uns
         fcoul          = qq * (interact * rinv * rinvsq - ic.beta3 * pg * __builtin_amdgcn_rcpf(qg));
         if (kEnergy)
         {
-            // erfc(beta r)/r directly: 1/r - erf(beta r)/r would cancel
-            const float z  = sqrtf(s);
+            // erfc(beta r)/r directly: 1/r - erf(beta r)/r would cancel;
+            // z = beta r from rsq / sqrt(rsq) (0 beyond the cut-off, where
+            // qq = 0)
+            const float z  = ic.beta * (rsq * rinv);
             const float pe = fmaf(fmaf(fmaf(fmaf(kEp4, z, kEp3), z, kEp2), z, kEp1), z, kEp0);
             const float qe = fmaf(fmaf(fmaf(fmaf(kEq4, z, kEq3), z, kEq2), z, kEq1), z, 1.0f);
             vpair = rinv * __builtin_amdgcn_exp2f(-s * kLog2e) * pe * __builtin_amdgcn_rcpf(qe) - ic.sh_ewald;
@@ -203,7 +207,7 @@ __device__ __forceinline__ void tile4(const IAtoms& I, const JAtom& J, const uns
 // LDS layout of a workgroup, in float4 units
 struct LdsLayout
 {
-    int nbfp, tab, fsh, ix, it, uc, win;
+    int nbfp, tab, fsh, ix, it, win;
     __host__ __device__ LdsLayout(int ntype, int tabN, int windowSlots)
     {
         nbfp  = 0;
@@ -211,8 +215,7 @@ struct LdsLayout
         fsh   = tab + tabN;
         ix    = fsh + (kNumShifts * 3 + 3) / 4;
         it    = ix + kWavesBlock * kGroupSizeMax * kClusterSize;
-        uc    = it + kWavesBlock * kGroupSizeMax;
-        win   = uc + kWavesBlock * ((windowSlots + 3) / 4);
+        win   = it + kWavesBlock * kGroupSizeMax;
         end_  = win + kWavesBlock * windowSlots * 3;
     }
     int end_;
@@ -254,8 +257,6 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
     float4* ix  = smem4 + L.ix + wave * kGroupSizeMax * kClusterSize;
     int*    it  = reinterpret_cast<int*>(smem4 + L.it) + wave * kGroupSizeMax * kClusterSize;
     float*  win = reinterpret_cast<float*>(smem4 + L.win) + wave * a.windowSlots * 12;
-    // the group's union list (cluster of each window slot beyond the own ones)
-    int*    uc  = reinterpret_cast<int*>(smem4 + L.uc) + wave * ((a.windowSlots + 3) / 4) * 4;
 
     // this wavefront's contiguous run of chunks (balanced by cost when the
     // list was built); chunks and work entries are stored in group order, so
@@ -274,7 +275,7 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
 
         // current group (the run starts inside the group holding its first
         // chunk); wavefront w writes the energies of group g to slot g + w
-        int g = uniformInt(ws.y), chGrpEnd = 0, uoff = 0, U = 0, c0 = 0, nOwn = 0, G = 0, shb = 0;
+        int g = uniformInt(ws.y), chGrpEnd = 0, U = 0, c0 = 0, nOwn = 0, G = 0, stage = -1;
         // current chunk
         IAtoms I;
         float4 sv;
@@ -285,17 +286,18 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
 
         auto beginGroup = [&](int gg) {
             const int4 grp = a.groups[gg]; // chunk range, union offset, windowed union size
-            const int4 gi  = a.groupInfo[gg]; // first own cluster, own count | G << 16, shift table
+            const int4 gi  = a.groupInfo[gg]; // first own cluster, own count | G << 16
+            const int4 si  = a.segInfo[gg];   // staging: first float4, first wavefront, wavefronts, size
             chGrpEnd       = uniformInt(grp.y);
-            uoff           = uniformInt(grp.z);
             U              = uniformInt(grp.w);
             c0             = uniformInt(gi.x);
             nOwn           = uniformInt(gi.y) & 0xffff;
             G              = uniformInt(gi.y) >> 16;
-            shb            = uniformInt(gi.z);
+            // this wavefront's staging segment (none for a group without chunks)
+            stage = uniformInt(si.z) > 0 ? uniformInt(si.x) + (wid - uniformInt(si.y)) * uniformInt(si.w) : -1;
             // clear the window (own slots, union slots, the padding entries'
-            // spare slot), cache the group's i-atoms and union list; this
-            // wavefront's LDS accesses complete in order, so no barrier
+            // spare slot) and cache the group's i-atoms; this wavefront's LDS
+            // accesses complete in order, so no barrier
             for (int t = lane; t < (G + U + 1) * 3; t += kWave)
             {
                 reinterpret_cast<float4*>(win)[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -304,10 +306,6 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
             {
                 ix[lane] = a.xq[c0 * kClusterSize + lane];
                 it[lane] = a.type[c0 * kClusterSize + lane] * ntype;
-            }
-            for (int t = lane; t < U; t += kWave)
-            {
-                uc[t] = a.unionList[uoff + t];
             }
         };
         auto endGroup = [&]() {
@@ -323,33 +321,19 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
                 eVdw  = 0.0;
                 eCoul = 0.0;
             }
-            // flush the window: a lane per window atom, one global atomic per
-            // nonzero component
-            for (int t = lane; t < (G + U) * kClusterSize; t += kWave)
+            // stage the window (own and union slots, 3 float4 each) with
+            // plain coalesced stores; the gather kernel adds the staged
+            // windows of every cluster after the launch
+            if (stage >= 0)
             {
-                const int    slot = t >> 2;
-                const float* wv   = win + t * 3;
-                const float  vx = wv[0], vy = wv[1], vz = wv[2];
-                const int    cl   = slot < G ? (slot < nOwn ? c0 + slot : -1) : uc[slot - G];
-                if (cl >= 0)
+                const float4* w4 = reinterpret_cast<const float4*>(win);
+                for (int t = lane; t < (G + U) * 3; t += kWave)
                 {
-                    float* dst = reinterpret_cast<float*>(a.fnb + cl * kClusterSize + (t & 3));
-                    if (vx != 0.0f)
-                    {
-                        unsafeAtomicAdd(dst, vx);
-                    }
-                    if (vy != 0.0f)
-                    {
-                        unsafeAtomicAdd(dst + 1, vy);
-                    }
-                    if (vz != 0.0f)
-                    {
-                        unsafeAtomicAdd(dst + 2, vz);
-                    }
+                    a.staging[stage + t] = w4[t];
                 }
             }
         };
-        auto beginChunk = [&](const int4 cd, int t) {
+        auto beginChunk = [&](const int4 cd, const float4 s, int t) {
             ci              = uniformInt(cd.x);
             const int flags = uniformInt(cd.y);
             sh              = flags & 0xff;
@@ -368,8 +352,7 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
                 I.q[i]         = uniformFloat(p.w);
                 I.tOff[i]      = uniformInt(it[lc + i]);
             }
-            const float4 s = a.shiftVec[shb + sh];
-            sv             = make_float4(uniformFloat(s.x), uniformFloat(s.y), uniformFloat(s.z), 0.0f);
+            sv = make_float4(uniformFloat(s.x), uniformFloat(s.y), uniformFloat(s.z), 0.0f);
 #pragma unroll
             for (int k = 0; k < 12; k++)
             {
@@ -438,8 +421,10 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
         // at a chunk boundary: the next chunk, and the next group holding
         // chunks when the chunk was its group's last (groups without
         // i-entries in between are passed through)
-        int4 cdNext  = a.chunks[min(chIdx + 1, chAll1 - 1)];
-        auto advance = [&](int t) {
+        // the next chunk's descriptor and shift vector are loaded one chunk ahead
+        int4   cdNext = a.chunks[min(chIdx + 1, chAll1 - 1)];
+        float4 svNext = a.chunkShift[min(chIdx + 1, chAll1 - 1)];
+        auto   advance = [&](int t) {
             endChunk();
             chIdx++;
             while (chIdx == chGrpEnd)
@@ -448,8 +433,9 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
                 g++;
                 beginGroup(g);
             }
-            beginChunk(cdNext, t);
+            beginChunk(cdNext, svNext, t);
             cdNext = a.chunks[min(chIdx + 1, chAll1 - 1)];
+            svNext = a.chunkShift[min(chIdx + 1, chAll1 - 1)];
         };
 
         // reads past the last tile are clamped to it
@@ -461,7 +447,7 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
         float4 xA = ldg(a.xq, eA.x * kClusterSize + jj);
         int    tA = ldg(a.type, eA.x * kClusterSize + jj);
         beginGroup(g);
-        beginChunk(cFirst, 0);
+        beginChunk(cFirst, a.chunkShift[chAll0], 0);
         for (int t = 0; t < T; t += 2)
         {
             const float4 xB  = ldg(a.xq, eB.x * kClusterSize + jj);

@@ -156,10 +156,27 @@ constexpr unsigned kSlotOverflow = 0xffffu;
 void launchGroupSpans(hipStream_t s, int ncluster, const int* offJ, int* maxSpan);
 void launchBuildGroups(hipStream_t s, const GroupArgs& a, bool fill);
 void launchMaxReduce(hipStream_t s, int n, const int* v, int* out);
+// Occurrences of every cluster in the groups' j-force windows (own slot and
+// windowed-union slots) as a CSR list of {group, slot}, sorted per cluster.
+// Call once with fill = false (count must be zeroed; it returns the counts),
+// scan the counts into occOff[ncluster + 1], zero count, call with fill = true.
+void launchOccurrences(hipStream_t s, int ngroups, const int4* groups, const int4* groupInfo, const int* unionList,
+                       int ncluster, int* count, int* occOff, int2* occ, bool fill);
+// The pair kernel's window staging layout for a schedule: per group
+// segInfo = {first float4 of its segments, first wavefront, wavefronts,
+// window float4}; wavefront w of the group writes its window at
+// x + (w - y) * w_size.  total receives the staging size in float4.
+void launchSegments(hipStream_t s, int ngroups, const int4* groups, const int4* groupInfo, int numWaves,
+                    const int2* waveStart, int* size, int* base, int* total, int4* segInfo);
+// Per chunk, the shift vector of its i-entry (shiftVec[chunk shift index]),
+// so that the pair kernel loads it together with the chunk descriptor.
+void launchChunkShifts(hipStream_t s, int nchunk, const int4* chunks, const float4* shiftVec, float4* out);
 // Static schedule of the pair kernel: the chunk list (in group order) cut into
 // numWaves contiguous runs of about equal cost (work entries plus
 // chunkWeight per chunk).  out[w] = {first chunk, group of that chunk} for
-// w in [0, numWaves], out[numWaves] = {nchunk, ngroups}.
+// w in [0, numWaves], out[numWaves] = {nchunk, ngroups}; runs are non-empty
+// except for the trailing wavefronts when there are more wavefronts than
+// distinct starts.
 void launchWaveSchedule(hipStream_t s, int numWaves, int nchunk, const int4* chunks, int nwork, int chunkWeight,
                         int ngroups, const int4* groups, int2* out);
 // Energy-slot range of each system of a batch: slotOff[k] for k in [0, R]
@@ -184,9 +201,11 @@ struct KernelArgs
     const float2* nbfp;
     int           ntype; // including the padding type
     const float4* tabFDV0;
-    const float4* shiftVec; // 45 per system; groupInfo.z selects the table
+    const float4* chunkShift; // per chunk: its i-entry's shift vector
     InteractionConst ic;
     const int2* waveStart; // [4 * gridDim.x + 1]: each wavefront's first chunk and its group
+    const int4* segInfo;   // per group: window staging layout (launchSegments)
+    float4*     staging;   // the wavefronts' j-force windows, gathered after the kernel
     float4*  fnb;
     // [ngroups + 4 * gridDim.x][2], zeroed before the launch: the energies of
     // wavefront w in group g go to slot g + w
@@ -212,7 +231,7 @@ struct AdjointArgs
     const float2* nbfp;
     int           ntype;
     const float4* tabFDV0;
-    const float4* shiftVec;
+    const float4* chunkShift;
     InteractionConst ic;
     const int2* waveStart; // the forward kernel's static schedule
     float4*     gradX;     // dL/dx per cluster-layout atom, zeroed by launchAdjoint
@@ -237,6 +256,9 @@ void launchConvertG(hipStream_t s, int nslot, const int* atomIndices, const floa
 // zeroes n float4 and, when d is not NULL, nd doubles
 void launchClearF4(hipStream_t s, float4* p, int n, double* d, int nd);
 void launchConvertX(hipStream_t s, int nslot, const int* atomIndices, const float* x, float4* xq);
+// after the pair kernel: the staged windows into the cluster force buffer
+void launchGatherForces(hipStream_t s, int ncluster, const int* occOff, const int2* occ, const int4* segInfo,
+                        const float4* staging, float4* fnb);
 // charges (atom order) into the w component of the cluster layout
 void launchSetCharges(hipStream_t s, int nslot, const int* atomIndices, const float* q, float4* xq);
 void launchReduceF(hipStream_t s, int natoms, const int* cells, const float4* fnb, float* f);
