@@ -895,7 +895,8 @@ int nbx_construct_pairlist(nbx_context* c)
         launchWaveSchedule(c->stream, numWaves, c->nchunk, c->chunks.p, c->nwork, kChunkWeight, c->ngroups,
                            c->groups.p, c->waveStart.p);
         check(hipGetLastError(), "schedule kernel");
-        buildSegments(c->stream, c->gather, c->ngroups, c->groups.p, c->groupInfo.p, numWaves, c->waveStart.p);
+        buildSegments(c->stream, c->gather, c->ngroups, c->groups.p, c->groupInfo.p, numWaves, c->waveStart.p,
+                      c->ncluster);
         c->energyPartial.ensure(2 * static_cast<size_t>(c->ngroups + numWaves));
         c->exclPartial.ensure(kExclMaxBlocks);
         c->fshiftPartial.ensure(static_cast<size_t>(c->numBlocks) * kNumShifts * 3);
@@ -971,8 +972,8 @@ int nbx_dispatch_kernel(nbx_context* c, unsigned flags)
             check(hipEventRecord(c->timingEvents[2 * c->timingRecorded + 1], c->stream), "hipEventRecord");
             c->timingRecorded++;
         }
-        launchGatherForces(c->stream, c->ncluster, c->gather.occOff.p, c->gather.occ.p, c->gather.segInfo.p,
-                           c->gather.staging.p, c->fnb.p);
+        launchGatherForces(c->stream, c->ncluster, c->gather.gposOff.p, c->gather.gpos.p, c->gather.staging.p,
+                           c->fnb.p);
         int numExcl = 0;
         if (energy)
         {

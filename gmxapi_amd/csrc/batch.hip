@@ -127,7 +127,8 @@ void scheduleBatch(nbx_batch* b, int blocksPerCU, int chunkWeight)
                        b->waveStart.p);
     launchSystemSlots(b->stream, R, b->groupOff.p, b->ngroups, b->groups.p, numWaves, b->waveStart.p, b->slotOff.p);
     check(hipGetLastError(), "schedule kernels");
-    buildSegments(b->stream, b->gather, b->ngroups, b->groups.p, b->groupInfo.p, numWaves, b->waveStart.p);
+    buildSegments(b->stream, b->gather, b->ngroups, b->groups.p, b->groupInfo.p, numWaves, b->waveStart.p,
+                  b->totClusters);
     check(hipStreamSynchronize(b->stream), "schedule sync");
 }
 
@@ -306,8 +307,8 @@ void enqueueCompute(nbx_batch* b, const float* x, float* f, unsigned flags)
     a.energyPartial = b->energyPartial.p;
     a.fshiftPartial = nullptr;
     launchNonbonded(b->stream, a, c0->elec, energy, false, b->numBlocks);
-    launchGatherForces(b->stream, b->totClusters, b->gather.occOff.p, b->gather.occ.p, b->gather.segInfo.p,
-                       b->gather.staging.p, b->fnb.p);
+    launchGatherForces(b->stream, b->totClusters, b->gather.gposOff.p, b->gather.gpos.p, b->gather.staging.p,
+                       b->fnb.p);
     if (energy)
     {
         ExclusionArgs e{};

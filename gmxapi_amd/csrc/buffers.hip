@@ -21,7 +21,7 @@
 // All are HBM-bandwidth bound: one coalesced pass over the arrays.
 #include <algorithm>
 
-#include "nbx_internal.h"
+#include "kernel_common.h"
 
 namespace nbx
 {
@@ -63,38 +63,55 @@ __global__ void convertXKernel(int nslot, const int* __restrict__ atomIndices, c
     }
 }
 
-// Forces of each cluster atom from the staged windows that hold it, added
-// in (group, wavefront) order, then added to the cluster force buffer (which
-// holds the contributions of clusters that spilled out of a window).
-__global__ void gatherForcesKernel(int nslot, const int* __restrict__ occOff, const int2* __restrict__ occ,
-                                   const int4* __restrict__ segInfo, const float4* __restrict__ staging,
-                                   float4* __restrict__ fnb)
+// Forces of each cluster from the staged windows that hold it: a wavefront
+// per cluster, lane l adding copies l, l + 64, ... (12 floats each), then a
+// fixed butterfly over the lanes, so the sum order is the same every step;
+// added to the cluster force buffer (which holds the contributions of
+// clusters that spilled out of a window).
+__global__ __launch_bounds__(256) void gatherForcesKernel(int ncluster, const int* __restrict__ gposOff,
+                                                          const int* __restrict__ gpos,
+                                                          const float4* __restrict__ staging,
+                                                          float4* __restrict__ fnb)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nslot)
+    const int lane = threadIdx.x & 63;
+    const int c    = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (c >= ncluster)
     {
         return;
     }
-    const int c = k >> 2;
-    const int a = k & 3;
-    float     fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    for (int o = occOff[c]; o < occOff[c + 1]; o++)
+    float f[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++)
     {
-        const int2 e  = occ[o];
-        const int4 si = segInfo[e.x];
-        for (int w = 0; w < si.z; w++)
-        {
-            const float* p = reinterpret_cast<const float*>(staging + si.x + w * si.w) + e.y * 12 + a * 3;
-            fx += p[0];
-            fy += p[1];
-            fz += p[2];
-        }
+        f[k] = 0.0f;
     }
-    float4 f = fnb[k];
-    f.x += fx;
-    f.y += fy;
-    f.z += fz;
-    fnb[k] = f;
+    for (int i = gposOff[c] + lane; i < gposOff[c + 1]; i += 64)
+    {
+        const int    p  = gpos[i];
+        const float4 v0 = staging[p];
+        const float4 v1 = staging[p + 1];
+        const float4 v2 = staging[p + 2];
+        f[0] += v0.x;
+        f[1] += v0.y;
+        f[2] += v0.z;
+        f[3] += v0.w;
+        f[4] += v1.x;
+        f[5] += v1.y;
+        f[6] += v1.z;
+        f[7] += v1.w;
+        f[8] += v2.x;
+        f[9] += v2.y;
+        f[10] += v2.z;
+        f[11] += v2.w;
+    }
+    // lane L ends up with component L >> 2 = 3 * atom + d
+    const float r   = waveReduceScatter12(f, lane);
+    const int   idx = lane >> 2;
+    if ((lane & 3) == 0 && idx < 12)
+    {
+        float* dst = reinterpret_cast<float*>(fnb + c * kClusterSize + idx / 3) + idx % 3;
+        *dst += r;
+    }
 }
 
 __global__ void setChargesKernel(int nslot, const int* __restrict__ atomIndices, const float* __restrict__ q,
@@ -514,13 +531,12 @@ void launchConvertX(hipStream_t s, int nslot, const int* atomIndices, const floa
     }
 }
 
-void launchGatherForces(hipStream_t s, int ncluster, const int* occOff, const int2* occ, const int4* segInfo,
-                        const float4* staging, float4* fnb)
+void launchGatherForces(hipStream_t s, int ncluster, const int* gposOff, const int* gpos, const float4* staging,
+                        float4* fnb)
 {
-    const int nslot = ncluster * kClusterSize;
-    if (nslot > 0)
+    if (ncluster > 0)
     {
-        gatherForcesKernel<<<(nslot + 255) / 256, 256, 0, s>>>(nslot, occOff, occ, segInfo, staging, fnb);
+        gatherForcesKernel<<<(ncluster + 3) / 4, 256, 0, s>>>(ncluster, gposOff, gpos, staging, fnb);
     }
 }
 

@@ -85,7 +85,7 @@ struct DevBuf
 // (one per context, one per batch).
 struct WindowGather
 {
-    DevBuf<int>    occCount, occOff, segSize, segBase;
+    DevBuf<int>    occCount, occOff, segSize, segBase, gposCount, gposOff, gpos;
     DevBuf<int2>   occ;
     DevBuf<int4>   segInfo;
     DevBuf<float4> staging;
@@ -117,7 +117,7 @@ inline void buildOccurrences(hipStream_t s, WindowGather& w, int ngroups, const 
 
 // Staging layout of a schedule (whenever the schedule changes).
 inline void buildSegments(hipStream_t s, WindowGather& w, int ngroups, const int4* groups, const int4* groupInfo,
-                          int numWaves, const int2* waveStart)
+                          int numWaves, const int2* waveStart, int ncluster)
 {
     w.segSize.ensure(ngroups + 1);
     w.segBase.ensure(ngroups + 1);
@@ -131,6 +131,23 @@ inline void buildSegments(hipStream_t s, WindowGather& w, int ngroups, const int
         check(hipStreamSynchronize(s), "segment sync");
     }
     w.staging.ensure(std::max(total, 1));
+    // the staging positions each cluster's gather reads
+    w.gposCount.ensure(ncluster + 1);
+    w.gposOff.ensure(ncluster + 1);
+    int npos = 0;
+    if (ncluster > 0)
+    {
+        launchGatherList(s, ncluster, w.occOff.p, w.occ.p, w.segInfo.p, w.gposCount.p, nullptr, nullptr, false);
+        launchExclusiveScan(s, ncluster, w.gposCount.p, w.gposOff.p, w.gposOff.p + ncluster);
+        check(hipMemcpyAsync(&npos, w.gposOff.p + ncluster, sizeof(int), hipMemcpyDeviceToHost, s), "read gather");
+        check(hipStreamSynchronize(s), "gather sync");
+    }
+    else
+    {
+        check(hipMemsetAsync(w.gposOff.p, 0, sizeof(int), s), "clear gather");
+    }
+    w.gpos.ensure(std::max(npos, 1));
+    launchGatherList(s, ncluster, w.occOff.p, w.occ.p, w.segInfo.p, w.gposCount.p, w.gposOff.p, w.gpos.p, true);
     check(hipGetLastError(), "segment kernels");
 }
 

@@ -322,25 +322,96 @@ __global__ void occurrenceKernel(int ngroups, const int4* __restrict__ groups, c
 }
 
 // each cluster's occurrences in (group, slot) order, so that the force
-// gather adds them in a fixed order
-__global__ void occurrenceSortKernel(int ncluster, const int* __restrict__ occOff, int2* __restrict__ occ)
+// gather adds them in a fixed order: a wavefront per cluster sorts up to 64
+// entries with a bitonic network across its lanes (longer lists, which do
+// not occur at the supported cut-offs and densities, are sorted by one lane)
+__global__ __launch_bounds__(256) void occurrenceSortKernel(int ncluster, const int* __restrict__ occOff,
+                                                            int2* __restrict__ occ)
+{
+    const int lane = threadIdx.x & 63;
+    const int c    = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (c >= ncluster)
+    {
+        return;
+    }
+    const int b = occOff[c], n = occOff[c + 1] - b;
+    if (n > 64)
+    {
+        if (lane == 0)
+        {
+            for (int i = b + 1; i < b + n; i++)
+            {
+                const int2 v = occ[i];
+                int        j = i - 1;
+                while (j >= b && (occ[j].x > v.x || (occ[j].x == v.x && occ[j].y > v.y)))
+                {
+                    occ[j + 1] = occ[j];
+                    j--;
+                }
+                occ[j + 1] = v;
+            }
+        }
+        return;
+    }
+    // keys (group, slot) as one 64-bit value; unused lanes sort last
+    unsigned hi = 0xffffffffu, lo = 0xffffffffu;
+    if (lane < n)
+    {
+        const int2 v = occ[b + lane];
+        hi           = static_cast<unsigned>(v.x);
+        lo           = static_cast<unsigned>(v.y);
+    }
+    for (int k = 2; k <= 64; k <<= 1)
+    {
+        for (int j = k >> 1; j > 0; j >>= 1)
+        {
+            const unsigned ohi   = __shfl_xor(hi, j, 64);
+            const unsigned olo   = __shfl_xor(lo, j, 64);
+            const bool     less  = ohi < hi || (ohi == hi && olo < lo); // other < mine
+            const bool     keepMin = ((lane & j) == 0) == ((lane & k) == 0);
+            if (keepMin == less)
+            {
+                hi = ohi;
+                lo = olo;
+            }
+        }
+    }
+    if (lane < n)
+    {
+        occ[b + lane] = make_int2(static_cast<int>(hi), static_cast<int>(lo));
+    }
+}
+
+// Per cluster, the staging positions (float4 index of its window slot) of
+// all its occurrences, one per wavefront segment of the occurrence's group,
+// in occurrence order.
+template<bool kFill>
+__global__ void gatherListKernel(int ncluster, const int* __restrict__ occOff, const int2* __restrict__ occ,
+                                 const int4* __restrict__ segInfo, int* __restrict__ count,
+                                 const int* __restrict__ gposOff, int* __restrict__ gpos)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncluster)
     {
         return;
     }
-    const int b = occOff[c], e = occOff[c + 1];
-    for (int i = b + 1; i < e; i++)
+    int n = kFill ? gposOff[c] : 0;
+    for (int o = occOff[c]; o < occOff[c + 1]; o++)
     {
-        const int2 v = occ[i];
-        int        j = i - 1;
-        while (j >= b && (occ[j].x > v.x || (occ[j].x == v.x && occ[j].y > v.y)))
+        const int2 e  = occ[o];
+        const int4 si = segInfo[e.x];
+        for (int w = 0; w < si.z; w++)
         {
-            occ[j + 1] = occ[j];
-            j--;
+            if (kFill)
+            {
+                gpos[n] = si.x + w * si.w + e.y * 3;
+            }
+            n++;
         }
-        occ[j + 1] = v;
+    }
+    if (!kFill)
+    {
+        count[c] = n;
     }
 }
 
@@ -539,7 +610,7 @@ void launchOccurrences(hipStream_t s, int ngroups, const int4* groups, const int
         return;
     }
     occurrenceKernel<true><<<ngroups, 256, 0, s>>>(ngroups, groups, groupInfo, unionList, count, occOff, occ);
-    occurrenceSortKernel<<<(ncluster + 255) / 256, 256, 0, s>>>(ncluster, occOff, occ);
+    occurrenceSortKernel<<<(ncluster + 3) / 4, 256, 0, s>>>(ncluster, occOff, occ);
 }
 
 void launchSegments(hipStream_t s, int ngroups, const int4* groups, const int4* groupInfo, int numWaves,
@@ -552,6 +623,25 @@ void launchSegments(hipStream_t s, int ngroups, const int4* groups, const int4* 
     segmentKernel<<<(ngroups + 255) / 256, 256, 0, s>>>(ngroups, groups, groupInfo, numWaves, waveStart, size, segInfo);
     launchExclusiveScan(s, ngroups, size, base, total);
     segmentBaseKernel<<<(ngroups + 255) / 256, 256, 0, s>>>(ngroups, base, segInfo);
+}
+
+void launchGatherList(hipStream_t s, int ncluster, const int* occOff, const int2* occ, const int4* segInfo,
+                      int* count, const int* gposOff, int* gpos, bool fill)
+{
+    if (ncluster == 0)
+    {
+        return;
+    }
+    if (fill)
+    {
+        gatherListKernel<true><<<(ncluster + 255) / 256, 256, 0, s>>>(ncluster, occOff, occ, segInfo, count, gposOff,
+                                                                      gpos);
+    }
+    else
+    {
+        gatherListKernel<false><<<(ncluster + 255) / 256, 256, 0, s>>>(ncluster, occOff, occ, segInfo, count, gposOff,
+                                                                       gpos);
+    }
 }
 
 void launchChunkShifts(hipStream_t s, int nchunk, const int4* chunks, const float4* shiftVec, float4* out)

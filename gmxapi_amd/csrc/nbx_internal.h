@@ -168,6 +168,11 @@ void launchOccurrences(hipStream_t s, int ngroups, const int4* groups, const int
 // x + (w - y) * w_size.  total receives the staging size in float4.
 void launchSegments(hipStream_t s, int ngroups, const int4* groups, const int4* groupInfo, int numWaves,
                     const int2* waveStart, int* size, int* base, int* total, int4* segInfo);
+// Per cluster, the staging positions of all its window copies (CSR:
+// gposOff[ncluster + 1], gpos): count pass (fill = false) into count, scan,
+// then fill.
+void launchGatherList(hipStream_t s, int ncluster, const int* occOff, const int2* occ, const int4* segInfo,
+                      int* count, const int* gposOff, int* gpos, bool fill);
 // Per chunk, the shift vector of its i-entry (shiftVec[chunk shift index]),
 // so that the pair kernel loads it together with the chunk descriptor.
 void launchChunkShifts(hipStream_t s, int nchunk, const int4* chunks, const float4* shiftVec, float4* out);
@@ -257,8 +262,8 @@ void launchConvertG(hipStream_t s, int nslot, const int* atomIndices, const floa
 void launchClearF4(hipStream_t s, float4* p, int n, double* d, int nd);
 void launchConvertX(hipStream_t s, int nslot, const int* atomIndices, const float* x, float4* xq);
 // after the pair kernel: the staged windows into the cluster force buffer
-void launchGatherForces(hipStream_t s, int ncluster, const int* occOff, const int2* occ, const int4* segInfo,
-                        const float4* staging, float4* fnb);
+void launchGatherForces(hipStream_t s, int ncluster, const int* gposOff, const int* gpos, const float4* staging,
+                        float4* fnb);
 // charges (atom order) into the w component of the cluster layout
 void launchSetCharges(hipStream_t s, int nslot, const int* atomIndices, const float* q, float4* xq);
 void launchReduceF(hipStream_t s, int natoms, const int* cells, const float4* fnb, float* f);
