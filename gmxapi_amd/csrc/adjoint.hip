@@ -21,7 +21,7 @@
 // are reduce-scattered once per entry.  dL/dq leaves the window into a
 // double buffer; the C6/C12 gradients are summed per workgroup in LDS in
 // double (ds_add_f64) and the workgroup sums are added in a fixed order.
-// Two wavefronts per workgroup keep the window within 64 KiB of LDS.
+// Two wavefronts per workgroup keep the float4 window of a workgroup small.
 #include "kernel_common.h"
 
 namespace nbx
@@ -537,20 +537,32 @@ __global__ void clearAdjointKernel(int n, float4* __restrict__ gradX, double* __
     }
 }
 
+// dynamic LDS beyond 64 KiB is requested explicitly (a CU has 160 KiB)
+template<int kElec, bool kParams>
+void launchOne(hipStream_t s, const AdjointArgs& a, int numBlocks, int lds)
+{
+    if (lds > 64 * 1024)
+    {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&adjointKernel<kElec, kParams>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    }
+    adjointKernel<kElec, kParams><<<numBlocks, kAdjThreads, lds, s>>>(a);
+}
+
 template<bool kParams>
 void launchElec(hipStream_t s, const AdjointArgs& a, int elec, int numBlocks, int lds)
 {
     if (elec == kElecRF)
     {
-        adjointKernel<kElecRF, kParams><<<numBlocks, kAdjThreads, lds, s>>>(a);
+        launchOne<kElecRF, kParams>(s, a, numBlocks, lds);
     }
     else if (elec == kElecEwaldAna)
     {
-        adjointKernel<kElecEwaldAna, kParams><<<numBlocks, kAdjThreads, lds, s>>>(a);
+        launchOne<kElecEwaldAna, kParams>(s, a, numBlocks, lds);
     }
     else
     {
-        adjointKernel<kElecEwaldTable, kParams><<<numBlocks, kAdjThreads, lds, s>>>(a);
+        launchOne<kElecEwaldTable, kParams>(s, a, numBlocks, lds);
     }
 }
 

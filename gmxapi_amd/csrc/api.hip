@@ -1133,7 +1133,7 @@ int nbx_compute_adjoint(nbx_context* c, const float* x, const float* g, float* g
             throw InputError("the adjoint supports at most " + std::to_string(kAdjointMaxTypes - 1) + " particle types");
         }
         const int lds = adjointLdsBytes(c->ntype, c->ic.tab_n, c->elec, c->windowSlots);
-        if (lds > kKernelLdsBudget)
+        if (lds > kAdjointLdsBudget)
         {
             throw InputError("the adjoint's j-cluster window does not fit in LDS: choose a smaller kernel_group_size");
         }

@@ -142,7 +142,30 @@ __global__ void reduceFKernel(int natoms, const int* __restrict__ cells, const f
     f[3 * a + 2]   = v.z;
 }
 
-constexpr int kFinalizeThreads = 256;
+constexpr int kFinalizeThreads = 1024;
+
+// Sum of v[k * stride + comp] over k in [0, n) by this thread's share of a
+// workgroup, with four independent accumulators (the loads of one pass are
+// in flight together); the order is fixed, so the result is reproducible.
+template<typename T>
+__device__ __forceinline__ double strideSum(const T* __restrict__ v, int n, int stride, int comp)
+{
+    double    a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    const int t  = kFinalizeThreads;
+    int       k  = threadIdx.x;
+    for (; k + 3 * t < n; k += 4 * t)
+    {
+        a0 += static_cast<double>(v[static_cast<size_t>(k) * stride + comp]);
+        a1 += static_cast<double>(v[static_cast<size_t>(k + t) * stride + comp]);
+        a2 += static_cast<double>(v[static_cast<size_t>(k + 2 * t) * stride + comp]);
+        a3 += static_cast<double>(v[static_cast<size_t>(k + 3 * t) * stride + comp]);
+    }
+    for (; k < n; k += t)
+    {
+        a0 += static_cast<double>(v[static_cast<size_t>(k) * stride + comp]);
+    }
+    return (a0 + a1) + (a2 + a3);
+}
 
 __global__ __launch_bounds__(kFinalizeThreads) void finalizeKernel(int numEnergyPartials,
                                                                    const double* __restrict__ ePart,
@@ -159,25 +182,15 @@ __global__ __launch_bounds__(kFinalizeThreads) void finalizeKernel(int numEnergy
     double    acc  = 0.0;
     if (comp < 2)
     {
-        for (int p = threadIdx.x; p < numEnergyPartials; p += kFinalizeThreads)
-        {
-            acc += ePart[2 * p + comp];
-        }
+        acc = strideSum(ePart, numEnergyPartials, 2, comp);
         if (comp == 1)
         {
-            for (int p = threadIdx.x; p < numExclPartials; p += kFinalizeThreads)
-            {
-                acc += exclPart[p];
-            }
+            acc += strideSum(exclPart, numExclPartials, 1, 0);
         }
     }
     else
     {
-        const int k = comp - 2;
-        for (int p = threadIdx.x; p < numShiftPartials; p += kFinalizeThreads)
-        {
-            acc += static_cast<double>(fshPart[p * kNumShifts * 3 + k]);
-        }
+        acc = strideSum(fshPart, numShiftPartials, kNumShifts * 3, comp - 2);
     }
     s[threadIdx.x] = acc;
     __syncthreads();

@@ -322,22 +322,56 @@ __global__ void occurrenceKernel(int ngroups, const int4* __restrict__ groups, c
 }
 
 // each cluster's occurrences in (group, slot) order, so that the force
-// gather adds them in a fixed order: a wavefront per cluster sorts up to 64
-// entries with a bitonic network across its lanes (longer lists, which do
-// not occur at the supported cut-offs and densities, are sorted by one lane)
-__global__ __launch_bounds__(256) void occurrenceSortKernel(int ncluster, const int* __restrict__ occOff,
-                                                            int2* __restrict__ occ)
+// gather adds them in a fixed order.  One workgroup per cluster: up to 64
+// entries are sorted by its first wavefront with a bitonic network across
+// the lanes, up to kSortMax by the workgroup with a bitonic network in LDS,
+// longer lists (not expected at supported cut-offs) by one thread.
+constexpr int kSortMax = 1024;
+
+__global__ __launch_bounds__(256) void occurrenceSortKernel(const int* __restrict__ occOff, int2* __restrict__ occ)
 {
-    const int lane = threadIdx.x & 63;
-    const int c    = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (c >= ncluster)
+    __shared__ unsigned long long key[kSortMax];
+    const int c = blockIdx.x;
+    const int b = occOff[c], n = occOff[c + 1] - b;
+    if (n <= 64)
     {
+        if (threadIdx.x >= 64)
+        {
+            return;
+        }
+        const int lane = threadIdx.x;
+        // keys (group, slot) as one 64-bit value; unused lanes sort last
+        unsigned hi = 0xffffffffu, lo = 0xffffffffu;
+        if (lane < n)
+        {
+            const int2 v = occ[b + lane];
+            hi           = static_cast<unsigned>(v.x);
+            lo           = static_cast<unsigned>(v.y);
+        }
+        for (int k = 2; k <= 64; k <<= 1)
+        {
+            for (int j = k >> 1; j > 0; j >>= 1)
+            {
+                const unsigned ohi     = __shfl_xor(hi, j, 64);
+                const unsigned olo     = __shfl_xor(lo, j, 64);
+                const bool     less    = ohi < hi || (ohi == hi && olo < lo); // other < mine
+                const bool     keepMin = ((lane & j) == 0) == ((lane & k) == 0);
+                if (keepMin == less)
+                {
+                    hi = ohi;
+                    lo = olo;
+                }
+            }
+        }
+        if (lane < n)
+        {
+            occ[b + lane] = make_int2(static_cast<int>(hi), static_cast<int>(lo));
+        }
         return;
     }
-    const int b = occOff[c], n = occOff[c + 1] - b;
-    if (n > 64)
+    if (n > kSortMax)
     {
-        if (lane == 0)
+        if (threadIdx.x == 0)
         {
             for (int i = b + 1; i < b + n; i++)
             {
@@ -353,32 +387,47 @@ __global__ __launch_bounds__(256) void occurrenceSortKernel(int ncluster, const 
         }
         return;
     }
-    // keys (group, slot) as one 64-bit value; unused lanes sort last
-    unsigned hi = 0xffffffffu, lo = 0xffffffffu;
-    if (lane < n)
+    int m = 128;
+    while (m < n)
     {
-        const int2 v = occ[b + lane];
-        hi           = static_cast<unsigned>(v.x);
-        lo           = static_cast<unsigned>(v.y);
+        m <<= 1;
     }
-    for (int k = 2; k <= 64; k <<= 1)
+    for (int i = threadIdx.x; i < m; i += blockDim.x)
+    {
+        if (i < n)
+        {
+            const int2 v = occ[b + i];
+            key[i] = (static_cast<unsigned long long>(static_cast<unsigned>(v.x)) << 32) | static_cast<unsigned>(v.y);
+        }
+        else
+        {
+            key[i] = ~0ull;
+        }
+    }
+    __syncthreads();
+    for (int k = 2; k <= m; k <<= 1)
     {
         for (int j = k >> 1; j > 0; j >>= 1)
         {
-            const unsigned ohi   = __shfl_xor(hi, j, 64);
-            const unsigned olo   = __shfl_xor(lo, j, 64);
-            const bool     less  = ohi < hi || (ohi == hi && olo < lo); // other < mine
-            const bool     keepMin = ((lane & j) == 0) == ((lane & k) == 0);
-            if (keepMin == less)
+            for (int i = threadIdx.x; i < m; i += blockDim.x)
             {
-                hi = ohi;
-                lo = olo;
+                const int l = i ^ j;
+                if (l > i)
+                {
+                    const unsigned long long x = key[i], y = key[l];
+                    if ((x > y) == ((i & k) == 0))
+                    {
+                        key[i] = y;
+                        key[l] = x;
+                    }
+                }
             }
+            __syncthreads();
         }
     }
-    if (lane < n)
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
     {
-        occ[b + lane] = make_int2(static_cast<int>(hi), static_cast<int>(lo));
+        occ[b + i] = make_int2(static_cast<int>(key[i] >> 32), static_cast<int>(key[i] & 0xffffffffu));
     }
 }
 
@@ -610,7 +659,7 @@ void launchOccurrences(hipStream_t s, int ngroups, const int4* groups, const int
         return;
     }
     occurrenceKernel<true><<<ngroups, 256, 0, s>>>(ngroups, groups, groupInfo, unionList, count, occOff, occ);
-    occurrenceSortKernel<<<(ncluster + 3) / 4, 256, 0, s>>>(ncluster, occOff, occ);
+    occurrenceSortKernel<<<ncluster, 256, 0, s>>>(occOff, occ);
 }
 
 void launchSegments(hipStream_t s, int ngroups, const int4* groups, const int4* groupInfo, int numWaves,

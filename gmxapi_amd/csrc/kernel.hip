@@ -487,24 +487,36 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
     }
 }
 
+// dynamic LDS beyond 64 KiB is requested explicitly (a CU has 160 KiB)
+template<int kElec, bool kEnergy, bool kShift>
+void launchOne(hipStream_t s, const KernelArgs& a, int numBlocks, int lds)
+{
+    if (lds > 64 * 1024)
+    {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&nonbondedKernel<kElec, kEnergy, kShift>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    }
+    nonbondedKernel<kElec, kEnergy, kShift><<<numBlocks, kThreads, lds, s>>>(a);
+}
+
 template<int kElec>
 void launchElec(hipStream_t s, const KernelArgs& a, bool energy, bool shiftForces, int numBlocks, int lds)
 {
     if (energy && shiftForces)
     {
-        nonbondedKernel<kElec, true, true><<<numBlocks, kThreads, lds, s>>>(a);
+        launchOne<kElec, true, true>(s, a, numBlocks, lds);
     }
     else if (energy)
     {
-        nonbondedKernel<kElec, true, false><<<numBlocks, kThreads, lds, s>>>(a);
+        launchOne<kElec, true, false>(s, a, numBlocks, lds);
     }
     else if (shiftForces)
     {
-        nonbondedKernel<kElec, false, true><<<numBlocks, kThreads, lds, s>>>(a);
+        launchOne<kElec, false, true>(s, a, numBlocks, lds);
     }
     else
     {
-        nonbondedKernel<kElec, false, false><<<numBlocks, kThreads, lds, s>>>(a);
+        launchOne<kElec, false, false>(s, a, numBlocks, lds);
     }
 }
 

@@ -47,9 +47,11 @@ constexpr int kChunkEntries = 1 << 20;
 // i-data set-up and i-force reduction
 constexpr int kChunkWeight = 8;
 // wavefronts per nonbonded workgroup, and the LDS a workgroup may use (two
-// resident workgroups per CU, eight wavefronts)
-constexpr int kKernelWaves     = 4;
-constexpr int kKernelLdsBudget = 64 * 1024;
+// resident workgroups per CU of 160 KiB, eight wavefronts); the adjoint may
+// use a CU's whole LDS
+constexpr int kKernelWaves      = 4;
+constexpr int kKernelLdsBudget  = 80 * 1024;
+constexpr int kAdjointLdsBudget = 160 * 1024;
 // rsq floor for float (pairlist.h:146): r > 6.2e-4 nm
 constexpr float kMinRsq = 3.82e-07f;
 
