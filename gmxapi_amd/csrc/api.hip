@@ -924,6 +924,88 @@ int nbx_convert_coordinates(nbx_context* c, const float* x, int xOnDevice)
     });
 }
 
+// One step on the context's stream: set-up (coordinates when xd is given,
+// clears), the pair kernel, the force gather (into the cluster buffer, or
+// with fOut straight into the caller's atom-order forces), the atom pass
+// (exclusion energies, virial) and the final sums.
+static void enqueueStep(nbx_context* c, unsigned flags, const float* xd, float* fOut)
+{
+    const bool energy = (flags & NBX_COMPUTE_ENERGY) != 0;
+    const bool shift  = (flags & (NBX_COMPUTE_SHIFT_FORCES | NBX_COMPUTE_VIRIAL)) != 0;
+    // the virial needs the atom-order forces and coordinates of this step
+    const bool virial = (flags & NBX_COMPUTE_VIRIAL) != 0 && fOut && xd;
+    const int numSlots = c->ngroups + c->numBlocks * kKernelWaves;
+    launchPrepareStep(c->stream, c->ncluster * kClusterSize, c->atomIndices.p, xd, c->xq.p, c->fnb.p,
+                      energy ? c->energyPartial.p : nullptr, 2 * numSlots);
+    KernelArgs a{};
+    a.chunks        = c->chunks.p;
+    a.groups        = c->groups.p;
+    a.groupInfo     = c->groupInfo.p;
+    a.ngroups       = c->ngroups;
+    a.groupSize     = c->groupSize;
+    a.ncluster      = c->ncluster;
+    a.windowSlots   = c->windowSlots;
+    a.unionList     = c->unionList.p;
+    a.work          = c->work.p;
+    a.xq            = c->xq.p;
+    a.type          = c->nbtype.p;
+    a.nbfp          = c->nbfp.p;
+    a.ntype         = c->ntype;
+    a.tabFDV0       = c->tabFDV0.p;
+    a.chunkShift    = c->chunkShift.p;
+    a.ic            = c->ic;
+    a.waveStart     = c->waveStart.p;
+    a.segInfo       = c->gather.segInfo.p;
+    a.staging       = c->gather.staging.p;
+    a.fnb           = c->fnb.p;
+    a.energyPartial = c->energyPartial.p;
+    a.fshiftPartial = c->fshiftPartial.p;
+    const bool timed = 2 * static_cast<size_t>(c->timingRecorded) < c->timingEvents.size();
+    if (timed)
+    {
+        check(hipEventRecord(c->timingEvents[2 * c->timingRecorded], c->stream), "hipEventRecord");
+    }
+    launchNonbonded(c->stream, a, c->elec, energy, shift, c->numBlocks);
+    if (timed)
+    {
+        check(hipEventRecord(c->timingEvents[2 * c->timingRecorded + 1], c->stream), "hipEventRecord");
+        c->timingRecorded++;
+    }
+    launchGatherForces(c->stream, c->ncluster, c->gather.gposOff.p, c->gather.gpos.p, c->gather.staging.p,
+                       c->fnb.p, c->atomIndices.p, fOut);
+    int numExcl = 0;
+    if (energy || virial)
+    {
+        ExclusionArgs e{};
+        e.natoms   = c->natoms;
+        e.cells    = c->cells.p;
+        e.xq       = c->xq.p;
+        e.exclOff  = c->exclOff.p;
+        e.exclIdx  = c->exclIdx.p;
+        e.box[0]   = c->g.box[0];
+        e.box[1]   = c->g.box[1];
+        e.box[2]   = c->g.box[2];
+        e.elec     = c->elec;
+        e.ic       = c->ic;
+        e.rc2      = static_cast<double>(c->ic.rcoulomb2);
+        e.epsfacD  = static_cast<double>(c->ic.epsfac);
+        e.selfCoef = c->selfCoef;
+        e.tabD     = c->tabFDV0d.p;
+        e.partial  = c->exclPartial.p;
+        numExcl    = launchAtomPass(c->stream, e, energy, xd, fOut, virial ? c->virialPartial.p : nullptr);
+    }
+    if (energy || shift || virial)
+    {
+        launchFinalize(c->stream, energy ? numSlots : 0, c->energyPartial.p, energy ? numExcl : 0,
+                       c->exclPartial.p, c->numBlocks, c->fshiftPartial.p, shift, c->energy.p, c->fshift.p,
+                       virial ? numExcl : 0, c->virialPartial.p, c->shiftVec.p, virial ? c->virial.p : nullptr);
+    }
+    check(hipGetLastError(), "nonbonded kernel");
+    c->haveEnergy = energy;
+    c->haveShift  = shift;
+    c->haveVirial = virial;
+}
+
 int nbx_dispatch_kernel(nbx_context* c, unsigned flags)
 {
     if (!c)
@@ -933,77 +1015,7 @@ int nbx_dispatch_kernel(nbx_context* c, unsigned flags)
     return guarded([&] {
         requireList(c);
         setDevice(c);
-        const bool energy = (flags & NBX_COMPUTE_ENERGY) != 0;
-        const bool shift  = (flags & (NBX_COMPUTE_SHIFT_FORCES | NBX_COMPUTE_VIRIAL)) != 0;
-        const int numSlots = c->ngroups + c->numBlocks * kKernelWaves;
-        launchClearF4(c->stream, c->fnb.p, c->ncluster * kClusterSize, energy ? c->energyPartial.p : nullptr,
-                      2 * numSlots);
-        KernelArgs a{};
-        a.chunks        = c->chunks.p;
-        a.groups        = c->groups.p;
-        a.groupInfo     = c->groupInfo.p;
-        a.ngroups       = c->ngroups;
-        a.groupSize     = c->groupSize;
-        a.ncluster      = c->ncluster;
-        a.windowSlots   = c->windowSlots;
-        a.unionList     = c->unionList.p;
-        a.work          = c->work.p;
-        a.xq            = c->xq.p;
-        a.type          = c->nbtype.p;
-        a.nbfp          = c->nbfp.p;
-        a.ntype         = c->ntype;
-        a.tabFDV0       = c->tabFDV0.p;
-        a.chunkShift    = c->chunkShift.p;
-        a.ic            = c->ic;
-        a.waveStart     = c->waveStart.p;
-        a.segInfo       = c->gather.segInfo.p;
-        a.staging       = c->gather.staging.p;
-        a.fnb           = c->fnb.p;
-        a.energyPartial = c->energyPartial.p;
-        a.fshiftPartial = c->fshiftPartial.p;
-        const bool timed = 2 * static_cast<size_t>(c->timingRecorded) < c->timingEvents.size();
-        if (timed)
-        {
-            check(hipEventRecord(c->timingEvents[2 * c->timingRecorded], c->stream), "hipEventRecord");
-        }
-        launchNonbonded(c->stream, a, c->elec, energy, shift, c->numBlocks);
-        if (timed)
-        {
-            check(hipEventRecord(c->timingEvents[2 * c->timingRecorded + 1], c->stream), "hipEventRecord");
-            c->timingRecorded++;
-        }
-        launchGatherForces(c->stream, c->ncluster, c->gather.gposOff.p, c->gather.gpos.p, c->gather.staging.p,
-                           c->fnb.p);
-        int numExcl = 0;
-        if (energy)
-        {
-            ExclusionArgs e{};
-            e.natoms   = c->natoms;
-            e.cells    = c->cells.p;
-            e.xq       = c->xq.p;
-            e.exclOff  = c->exclOff.p;
-            e.exclIdx  = c->exclIdx.p;
-            e.box[0]   = c->g.box[0];
-            e.box[1]   = c->g.box[1];
-            e.box[2]   = c->g.box[2];
-            e.elec     = c->elec;
-            e.ic       = c->ic;
-            e.rc2      = static_cast<double>(c->ic.rcoulomb2);
-            e.epsfacD  = static_cast<double>(c->ic.epsfac);
-            e.selfCoef = c->selfCoef;
-            e.tabD     = c->tabFDV0d.p;
-            e.partial  = c->exclPartial.p;
-            numExcl    = launchExclusionEnergy(c->stream, e);
-        }
-        if (energy || shift)
-        {
-            launchFinalize(c->stream, energy ? numSlots : 0, c->energyPartial.p, numExcl, c->exclPartial.p,
-                           c->numBlocks, c->fshiftPartial.p, shift, c->energy.p, c->fshift.p);
-        }
-        check(hipGetLastError(), "nonbonded kernel");
-        c->haveEnergy = energy;
-        c->haveShift  = shift;
-        c->haveVirial = false;
+        enqueueStep(c, flags & ~static_cast<unsigned>(NBX_COMPUTE_VIRIAL), nullptr, nullptr);
     });
 }
 
@@ -1049,25 +1061,13 @@ int nbx_compute(nbx_context* c, const float* x, float* f, unsigned flags, int on
         }
         setDevice(c);
         const float* xd = deviceCoordinates(c, x, onDevice);
-        launchConvertX(c->stream, c->ncluster * kClusterSize, c->atomIndices.p, xd, c->xq.p);
-        if (nbx_dispatch_kernel(c, flags) != NBX_SUCCESS)
-        {
-            throw DeviceError(g_lastError);
-        }
-        float* fd = f;
+        float*       fd = f;
         if (!onDevice)
         {
             c->fOut.ensure(3 * static_cast<size_t>(c->natoms));
             fd = c->fOut.p;
         }
-        launchReduceF(c->stream, c->natoms, c->cells.p, c->fnb.p, fd);
-        if (flags & NBX_COMPUTE_VIRIAL)
-        {
-            const int vb = std::max(1, std::min(256, (c->natoms + 1023) / 1024));
-            launchVirial(c->stream, c->natoms, xd, fd, c->fshift.p, c->shiftVec.p, c->virialPartial.p, vb, c->virial.p);
-            c->haveVirial = true;
-        }
-        check(hipGetLastError(), "compute");
+        enqueueStep(c, flags, xd, fd);
         if (!onDevice)
         {
             check(hipMemcpyAsync(f, fd, sizeof(float) * 3 * c->natoms, hipMemcpyDeviceToHost, c->stream), "download f");

@@ -280,9 +280,9 @@ void enqueueCompute(nbx_batch* b, const float* x, float* f, unsigned flags)
 {
     nbx_context* c0     = b->sys[0];
     const bool   energy = (flags & NBX_COMPUTE_ENERGY) != 0;
-    launchConvertX(b->stream, 4 * b->totClusters, b->atomIndices.p, x, b->xq.p);
     const int numSlots = b->ngroups + b->numBlocks * kKernelWaves;
-    launchClearF4(b->stream, b->fnb.p, 4 * b->totClusters, energy ? b->energyPartial.p : nullptr, 2 * numSlots);
+    launchPrepareStep(b->stream, 4 * b->totClusters, b->atomIndices.p, x, b->xq.p, b->fnb.p,
+                      energy ? b->energyPartial.p : nullptr, 2 * numSlots);
     KernelArgs a{};
     a.chunks       = b->chunks.p;
     a.groups       = b->groups.p;
@@ -308,7 +308,7 @@ void enqueueCompute(nbx_batch* b, const float* x, float* f, unsigned flags)
     a.fshiftPartial = nullptr;
     launchNonbonded(b->stream, a, c0->elec, energy, false, b->numBlocks);
     launchGatherForces(b->stream, b->totClusters, b->gather.gposOff.p, b->gather.gpos.p, b->gather.staging.p,
-                       b->fnb.p);
+                       b->fnb.p, b->atomIndices.p, f);
     if (energy)
     {
         ExclusionArgs e{};
@@ -330,7 +330,6 @@ void enqueueCompute(nbx_batch* b, const float* x, float* f, unsigned flags)
         launchBatchFinalize(b->stream, static_cast<int>(b->sys.size()), b->slotOff.p, b->energyPartial.p,
                             b->atomOff.p, b->atomEnergy.p, b->energy.p);
     }
-    launchReduceF(b->stream, b->totAtoms, b->cells.p, b->fnb.p, f);
 }
 
 void requireBuilt(const nbx_batch* b)

@@ -44,6 +44,34 @@ __global__ void clearF4Kernel(float4* __restrict__ p, int n, double* __restrict_
     }
 }
 
+// A step's set-up in one launch: coordinates into the cluster layout (when
+// x is given), the cluster force buffer and the energy slots cleared.
+__global__ void prepareStepKernel(int nslot, const int* __restrict__ atomIndices, const float* __restrict__ x,
+                                  float4* __restrict__ xq, float4* __restrict__ fnb, double* __restrict__ d, int nd)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nslot)
+    {
+        fnb[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (x)
+        {
+            const int a = atomIndices[i];
+            if (a >= 0)
+            {
+                float4 v = xq[i];
+                v.x      = x[3 * a + 0];
+                v.y      = x[3 * a + 1];
+                v.z      = x[3 * a + 2];
+                xq[i]    = v;
+            }
+        }
+    }
+    if (i < nd)
+    {
+        d[i] = 0.0;
+    }
+}
+
 __global__ void convertXKernel(int nslot, const int* __restrict__ atomIndices, const float* __restrict__ x,
                                float4* __restrict__ xq)
 {
@@ -68,10 +96,12 @@ __global__ void convertXKernel(int nslot, const int* __restrict__ atomIndices, c
 // fixed butterfly over the lanes, so the sum order is the same every step;
 // added to the cluster force buffer (which holds the contributions of
 // clusters that spilled out of a window).
+template<bool kToAtoms>
 __global__ __launch_bounds__(256) void gatherForcesKernel(int ncluster, const int* __restrict__ gposOff,
                                                           const int* __restrict__ gpos,
                                                           const float4* __restrict__ staging,
-                                                          float4* __restrict__ fnb)
+                                                          float4* __restrict__ fnb,
+                                                          const int* __restrict__ atomIndices, float* __restrict__ fOut)
 {
     const int lane = threadIdx.x & 63;
     const int c    = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -110,7 +140,19 @@ __global__ __launch_bounds__(256) void gatherForcesKernel(int ncluster, const in
     if ((lane & 3) == 0 && idx < 12)
     {
         float* dst = reinterpret_cast<float*>(fnb + c * kClusterSize + idx / 3) + idx % 3;
-        *dst += r;
+        if (kToAtoms)
+        {
+            // straight to the caller's atom-order forces (nbat-f -> f)
+            const int a = atomIndices[c * kClusterSize + idx / 3];
+            if (a >= 0)
+            {
+                fOut[3 * a + idx % 3] = *dst + r;
+            }
+        }
+        else
+        {
+            *dst += r;
+        }
     }
 }
 
@@ -174,13 +216,31 @@ __global__ __launch_bounds__(kFinalizeThreads) void finalizeKernel(int numEnergy
                                                                    int numShiftPartials,
                                                                    const float* __restrict__ fshPart, int doShift,
                                                                    double* __restrict__ eOut,
-                                                                   double* __restrict__ fshOut)
+                                                                   double* __restrict__ fshOut, int numVirPartials,
+                                                                   const double* __restrict__ virPart,
+                                                                   const float4* __restrict__ shiftVec,
+                                                                   double* __restrict__ virOut)
 {
-    // blocks 0, 1: the energies, blocks 2..: one shift-force component each
+    // blocks 0, 1: the energies; 2 .. 2 + 135 (with shift forces): one
+    // shift-force component each; the last 9 (with the virial): one virial
+    // component each, -1/2 sum x (x) f plus -1/2 sum_s s (x) fshift_s
     __shared__ double s[kFinalizeThreads];
     const int comp = blockIdx.x;
     double    acc  = 0.0;
-    if (comp < 2)
+    const int vir0 = doShift ? 2 + kNumShifts * 3 : 2;
+    if (virOut && comp >= vir0)
+    {
+        const int k = comp - vir0, d1 = k / 3, d2 = k % 3;
+        acc         = strideSum(virPart, numVirPartials, 9, k);
+        for (int t = threadIdx.x; t < numShiftPartials * kNumShifts; t += kFinalizeThreads)
+        {
+            const int    sh = t % kNumShifts;
+            const float4 sv = shiftVec[sh];
+            const double sd = d1 == 0 ? sv.x : (d1 == 1 ? sv.y : sv.z);
+            acc -= 0.5 * sd * static_cast<double>(fshPart[static_cast<size_t>(t / kNumShifts) * kNumShifts * 3 + sh * 3 + d2]);
+        }
+    }
+    else if (comp < 2)
     {
         acc = strideSum(ePart, numEnergyPartials, 2, comp);
         if (comp == 1)
@@ -204,7 +264,11 @@ __global__ __launch_bounds__(kFinalizeThreads) void finalizeKernel(int numEnergy
     }
     if (threadIdx.x == 0)
     {
-        if (comp < 2)
+        if (virOut && comp >= vir0)
+        {
+            virOut[comp - vir0] = s[0];
+        }
+        else if (comp < 2)
         {
             eOut[comp] = s[0];
         }
@@ -234,48 +298,56 @@ __device__ double ewaldCorrectionD(double r, const ExclusionArgs& a)
     return erf(static_cast<double>(a.ic.beta) * r) / r;
 }
 
+// Atom i's self energy and the Coulomb energies of its excluded partners
+// j > i inside the cut-off, in double.
+__device__ double atomExclusionEnergy(const ExclusionArgs& a, int i)
+{
+    // the atom's system box (a batch keeps one per system)
+    const float* box = a.atomSystem ? a.boxes + 3 * a.atomSystem[i] : a.box;
+    const double L[3] = { box[0], box[1], box[2] };
+    const float4 pi   = a.xq[a.cells[i]];
+    const double qi   = pi.w;
+    double       ei   = -a.selfCoef * qi * qi;
+    for (int k = a.exclOff[i]; k < a.exclOff[i + 1]; k++)
+    {
+        const int j = a.exclIdx[k];
+        if (j <= i)
+        {
+            continue;
+        }
+        const float4 pj = a.xq[a.cells[j]];
+        double       d[3] = { static_cast<double>(pi.x) - pj.x, static_cast<double>(pi.y) - pj.y,
+                        static_cast<double>(pi.z) - pj.z };
+        double       r2   = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+        {
+            d[c] -= L[c] * rint(d[c] / L[c]);
+            r2 += d[c] * d[c];
+        }
+        if (r2 < a.rc2)
+        {
+            const double qq = a.epsfacD * qi * static_cast<double>(pj.w);
+            if (a.elec == kElecRF)
+            {
+                ei += qq * (static_cast<double>(a.ic.k_rf) * r2 - static_cast<double>(a.ic.c_rf));
+            }
+            else
+            {
+                ei -= qq * ewaldCorrectionD(sqrt(r2), a);
+            }
+        }
+    }
+    return ei;
+}
+
 __global__ __launch_bounds__(kExclThreads) void exclusionEnergyKernel(const ExclusionArgs a)
 {
     __shared__ double s[kExclThreads / 64];
     double            e = 0.0;
     for (int i = blockIdx.x * kExclThreads + threadIdx.x; i < a.natoms; i += gridDim.x * kExclThreads)
     {
-        // the atom's system box (a batch keeps one per system)
-        const float* box = a.atomSystem ? a.boxes + 3 * a.atomSystem[i] : a.box;
-        const double L[3] = { box[0], box[1], box[2] };
-        const float4 pi   = a.xq[a.cells[i]];
-        const double qi   = pi.w;
-        double       ei   = -a.selfCoef * qi * qi;
-        for (int k = a.exclOff[i]; k < a.exclOff[i + 1]; k++)
-        {
-            const int j = a.exclIdx[k];
-            if (j <= i)
-            {
-                continue;
-            }
-            const float4 pj = a.xq[a.cells[j]];
-            double       d[3] = { static_cast<double>(pi.x) - pj.x, static_cast<double>(pi.y) - pj.y,
-                            static_cast<double>(pi.z) - pj.z };
-            double       r2   = 0.0;
-#pragma unroll
-            for (int c = 0; c < 3; c++)
-            {
-                d[c] -= L[c] * rint(d[c] / L[c]);
-                r2 += d[c] * d[c];
-            }
-            if (r2 < a.rc2)
-            {
-                const double qq = a.epsfacD * qi * static_cast<double>(pj.w);
-                if (a.elec == kElecRF)
-                {
-                    ei += qq * (static_cast<double>(a.ic.k_rf) * r2 - static_cast<double>(a.ic.c_rf));
-                }
-                else
-                {
-                    ei -= qq * ewaldCorrectionD(sqrt(r2), a);
-                }
-            }
-        }
+        const double ei = atomExclusionEnergy(a, i);
         if (a.atomEnergy)
         {
             a.atomEnergy[i] = ei;
@@ -343,6 +415,76 @@ __global__ __launch_bounds__(kFinalizeThreads) void batchFinalizeKernel(const in
     {
         eOut[2 * r]     = s[0][0];
         eOut[2 * r + 1] = s[1][0];
+    }
+}
+
+// One pass over the atoms after the forces are final: exclusion and self
+// energies (per-workgroup sums into a.partial) and the virial's -1/2 x (x) f
+// term (per-workgroup sums into virPart[9 * workgroup + k]), in double.
+template<bool kEnergy, bool kVirial>
+__global__ __launch_bounds__(kExclThreads) void atomPassKernel(const ExclusionArgs a, const float* __restrict__ x,
+                                                               const float* __restrict__ f,
+                                                               double* __restrict__ virPart)
+{
+    __shared__ double s[10][kExclThreads / 64];
+    double            v[10] = { 0, 0, 0, 0, 0, 0, 0, 0, 0, 0 }; // virial 0..8, energy 9
+    for (int i = blockIdx.x * kExclThreads + threadIdx.x; i < a.natoms; i += gridDim.x * kExclThreads)
+    {
+        if (kEnergy)
+        {
+            v[9] += atomExclusionEnergy(a, i);
+        }
+        if (kVirial)
+        {
+            const double xa[3] = { x[3 * i], x[3 * i + 1], x[3 * i + 2] };
+            const double fa[3] = { f[3 * i], f[3 * i + 1], f[3 * i + 2] };
+#pragma unroll
+            for (int d1 = 0; d1 < 3; d1++)
+            {
+#pragma unroll
+                for (int d2 = 0; d2 < 3; d2++)
+                {
+                    v[d1 * 3 + d2] -= 0.5 * xa[d1] * fa[d2];
+                }
+            }
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x / 64;
+#pragma unroll
+    for (int k = 0; k < 10; k++)
+    {
+        if ((k < 9 && kVirial) || (k == 9 && kEnergy))
+        {
+            for (int m = 1; m < 64; m *= 2)
+            {
+                v[k] += __shfl_xor(v[k], m, 64);
+            }
+            if (lane == 0)
+            {
+                s[k][wave] = v[k];
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 10)
+    {
+        const int k = threadIdx.x;
+        if ((k < 9 && kVirial) || (k == 9 && kEnergy))
+        {
+            double t = 0.0;
+            for (int w = 0; w < kExclThreads / 64; w++)
+            {
+                t += s[k][w];
+            }
+            if (k == 9)
+            {
+                a.partial[blockIdx.x] = t;
+            }
+            else
+            {
+                virPart[blockIdx.x * 9 + k] = t;
+            }
+        }
     }
 }
 
@@ -536,6 +678,16 @@ void launchClearF4(hipStream_t s, float4* p, int n, double* d, int nd)
     }
 }
 
+void launchPrepareStep(hipStream_t s, int nslot, const int* atomIndices, const float* x, float4* xq, float4* fnb,
+                       double* d, int nd)
+{
+    const int m = std::max(nslot, d ? nd : 0);
+    if (m > 0)
+    {
+        prepareStepKernel<<<(m + 255) / 256, 256, 0, s>>>(nslot, atomIndices, x, xq, fnb, d, d ? nd : 0);
+    }
+}
+
 void launchConvertX(hipStream_t s, int nslot, const int* atomIndices, const float* x, float4* xq)
 {
     if (nslot > 0)
@@ -545,11 +697,21 @@ void launchConvertX(hipStream_t s, int nslot, const int* atomIndices, const floa
 }
 
 void launchGatherForces(hipStream_t s, int ncluster, const int* gposOff, const int* gpos, const float4* staging,
-                        float4* fnb)
+                        float4* fnb, const int* atomIndices, float* f)
 {
-    if (ncluster > 0)
+    if (ncluster == 0)
     {
-        gatherForcesKernel<<<(ncluster + 3) / 4, 256, 0, s>>>(ncluster, gposOff, gpos, staging, fnb);
+        return;
+    }
+    if (f)
+    {
+        gatherForcesKernel<true><<<(ncluster + 3) / 4, 256, 0, s>>>(ncluster, gposOff, gpos, staging, fnb,
+                                                                     atomIndices, f);
+    }
+    else
+    {
+        gatherForcesKernel<false><<<(ncluster + 3) / 4, 256, 0, s>>>(ncluster, gposOff, gpos, staging, fnb, nullptr,
+                                                                      nullptr);
     }
 }
 
@@ -571,12 +733,32 @@ void launchReduceF(hipStream_t s, int natoms, const int* cells, const float4* fn
 
 void launchFinalize(hipStream_t s, int numEnergyPartials, const double* energyPartial, int numExclPartials,
                     const double* exclPartial, int numShiftPartials, const float* fshiftPartial, bool shiftForces,
-                    double* energyOut, double* fshiftOut)
+                    double* energyOut, double* fshiftOut, int numVirPartials, const double* virPartial,
+                    const float4* shiftVec, double* virialOut)
 {
-    const int blocks = shiftForces ? 2 + kNumShifts * 3 : 2;
+    const int blocks = (shiftForces ? 2 + kNumShifts * 3 : 2) + (virialOut ? 9 : 0);
     finalizeKernel<<<blocks, kFinalizeThreads, 0, s>>>(numEnergyPartials, energyPartial, numExclPartials, exclPartial,
                                                        numShiftPartials, fshiftPartial, shiftForces ? 1 : 0, energyOut,
-                                                       fshiftOut);
+                                                       fshiftOut, numVirPartials, virPartial, shiftVec, virialOut);
+}
+
+int launchAtomPass(hipStream_t s, const ExclusionArgs& a, bool energy, const float* x, const float* f,
+                   double* virialPartial)
+{
+    const int blocks = std::max(1, std::min(kExclMaxBlocks, (a.natoms + kExclThreads - 1) / kExclThreads));
+    if (energy && virialPartial)
+    {
+        atomPassKernel<true, true><<<blocks, kExclThreads, 0, s>>>(a, x, f, virialPartial);
+    }
+    else if (energy)
+    {
+        atomPassKernel<true, false><<<blocks, kExclThreads, 0, s>>>(a, x, f, virialPartial);
+    }
+    else if (virialPartial)
+    {
+        atomPassKernel<false, true><<<blocks, kExclThreads, 0, s>>>(a, x, f, virialPartial);
+    }
+    return blocks;
 }
 
 void launchBatchFinalize(hipStream_t s, int numSystems, const int* slotOff, const double* energyPartial,

@@ -263,15 +263,23 @@ void launchConvertG(hipStream_t s, int nslot, const int* atomIndices, const floa
 // zeroes n float4 and, when d is not NULL, nd doubles
 void launchClearF4(hipStream_t s, float4* p, int n, double* d, int nd);
 void launchConvertX(hipStream_t s, int nslot, const int* atomIndices, const float* x, float4* xq);
-// after the pair kernel: the staged windows into the cluster force buffer
+// after the pair kernel: the staged windows added to the cluster force
+// buffer, or (f given) written with it to the atom-order forces f
 void launchGatherForces(hipStream_t s, int ncluster, const int* gposOff, const int* gpos, const float4* staging,
-                        float4* fnb);
+                        float4* fnb, const int* atomIndices, float* f);
+// coordinates into the cluster layout (x may be NULL), cluster forces and
+// nd doubles at d cleared, in one launch
+void launchPrepareStep(hipStream_t s, int nslot, const int* atomIndices, const float* x, float4* xq, float4* fnb,
+                       double* d, int nd);
 // charges (atom order) into the w component of the cluster layout
 void launchSetCharges(hipStream_t s, int nslot, const int* atomIndices, const float* q, float4* xq);
 void launchReduceF(hipStream_t s, int natoms, const int* cells, const float4* fnb, float* f);
+// energies, shift forces and (virialOut given) the virial from their
+// partial sums, in fixed order
 void launchFinalize(hipStream_t s, int numEnergyPartials, const double* energyPartial, int numExclPartials,
                     const double* exclPartial, int numShiftPartials, const float* fshiftPartial, bool shiftForces,
-                    double* energyOut, double* fshiftOut);
+                    double* energyOut, double* fshiftOut, int numVirPartials, const double* virPartial,
+                    const float4* shiftVec, double* virialOut);
 // the self energy and excluded-pair Coulomb energies in double
 struct ExclusionArgs
 {
@@ -296,6 +304,10 @@ struct ExclusionArgs
 };
 constexpr int kExclMaxBlocks = 1024;
 int launchExclusionEnergy(hipStream_t s, const ExclusionArgs& a);
+// exclusion energies (energy) and virial partials (virialPartial given) in
+// one pass over the atoms; returns the number of workgroup partials
+int launchAtomPass(hipStream_t s, const ExclusionArgs& a, bool energy, const float* x, const float* f,
+                   double* virialPartial);
 void launchBatchFinalize(hipStream_t s, int numSystems, const int* slotOff, const double* energyPartial,
                          const int* atomOff, const double* atomEnergy, double* energyOut);
 void launchVirial(hipStream_t s, int natoms, const float* x, const float* f, const double* fshift, const float4* shiftVec,
