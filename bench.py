@@ -90,6 +90,19 @@ def cpu_baseline(system, budget_s=30.0):
     return sorted(times)[len(times) // 2], t_setup, len(times)
 
 
+def measured_traffic(workload, ewald):
+    """HBM bytes per launch of the F+E+shift kernel from the committed
+    rocprofv3 counter summary (profiles/nonbonded_<ewald>_<workload>_pmc.md,
+    a line 'traffic_bytes_per_launch: N'), or None."""
+    path = os.path.join(ROOT, "profiles", f"nonbonded_{ewald}_{workload}_pmc.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(float(line.split(":", 1)[1]))
+    return None
+
+
 def _time(fn, stream, reps, warm=3):
     import torch
 
@@ -149,9 +162,11 @@ def bench_ensemble(ewald, stream, reps=20):
     }
 
 
-def bench_forward_backward(ewald, stream, reps=20):
+def bench_forward_backward(stream, reps=20, ewald="analytical"):
     """The 24,000-atom, 16-type force-matching system: forward forces vs the
-    adjoint returning dL/dx, dL/dq, dL/dC6, dL/dC12."""
+    adjoint returning dL/dx, dL/dq, dL/dC6, dL/dC12.  The analytical Ewald
+    correction is the one to differentiate: its force is a smooth function of
+    r, while the table's is piecewise linear."""
     import numpy as np
     import torch
 
@@ -181,7 +196,7 @@ def bench_forward_backward(ewald, stream, reps=20):
     bwd_ms = _time(bwd, stream, reps)
     return {
         "workload": f"force matching: {s.natoms} atoms, 10-atom chains, 16 LJ types, {s.box[0]} nm box, "
-                    f"rc {s.rcoulomb} nm, rlist {s.rlist} nm",
+                    f"rc {s.rcoulomb} nm, rlist {s.rlist} nm, {ewald} Ewald correction",
         "useful_pairs": int(useful),
         "fwd_ms": fwd_ms,
         "bwd_ms": bwd_ms,
@@ -202,6 +217,9 @@ def main():
                     help="Ewald correction: the reference's spline table or the analytical form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the batched and forward+backward parts")
+    ap.add_argument("--profile-copy", action="store_true",
+                    help="also run one 64 MiB device-to-device copy: the reference that scales rocprofv3's "
+                         "FETCH_SIZE / WRITE_SIZE counters to bytes")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -262,6 +280,12 @@ def main():
         f"useful pairs {useful} (estimate {s.useful_pairs_estimate():.0f}), computed {computed_pairs}")
 
     flags = nbx.COMPUTE_FORCES | nbx.COMPUTE_ENERGY | nbx.COMPUTE_SHIFT_FORCES | nbx.COMPUTE_VIRIAL
+    if args.profile_copy:
+        src = torch.ones(16 * 1024 * 1024, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     def step():
         ctx.compute_device(x_d.data_ptr(), f_d.data_ptr(), flags)
@@ -329,7 +353,7 @@ def main():
             "achieved": achieved,
             "peak": PEAK_FP32_TFLOPS,
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None,
+            "traffic": measured_traffic(args.workload, args.ewald),
             "kernel": "nonbondedKernel<%s, energy, shift>" % ("EwaldTable" if args.ewald == "table"
                                                                else "EwaldAnalytical"),
             "kernel_ms": kernel_ms,
@@ -341,7 +365,7 @@ def main():
     if not args.no_extras:
         t0 = time.perf_counter()
         out["batched"] = bench_ensemble(args.ewald, stream)
-        out["fwd_bwd"] = bench_forward_backward(args.ewald, stream)
+        out["fwd_bwd"] = bench_forward_backward(stream)
         log(f"[bench] batched and forward+backward parts in {time.perf_counter() - t0:.1f} s")
     if not args.no_cpu_baseline:
         t_step, t_setup, nrep = cpu_baseline(s)

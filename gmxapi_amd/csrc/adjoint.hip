@@ -18,9 +18,11 @@
 // Mapping: as in the forward kernel, four wave-uniform i-atoms per tile and
 // one j-atom per lane.  A wavefront's LDS window holds {dL/dx, dL/dq} per
 // j-cluster atom (float4); i-atom sums live in 16 registers per i-entry and
-// are reduce-scattered once per entry.  dL/dq leaves the window into a
-// double buffer; the C6/C12 gradients are summed per workgroup in LDS in
-// double (ds_add_f64) and the workgroup sums are added in a fixed order.
+// are reduce-scattered once per entry.  Windows are staged with plain stores
+// and gathered per cluster after the launch, as in the forward path (dL/dq
+// summed in double there); the C6/C12 gradients are summed per workgroup in
+// LDS in double (ds_add_f64) and the workgroup sums are added in a fixed
+// order.
 // Two wavefronts per workgroup keep the float4 window of a workgroup small.
 #include "kernel_common.h"
 
@@ -36,7 +38,7 @@ constexpr int kAdjThreads = kWave * kAdjWaves;
 // LDS layout of an adjoint workgroup, in float4 units
 struct AdjLayout
 {
-    int nbfp, tab, ix, ig, it, uc, win, acc, end_;
+    int nbfp, tab, ix, ig, it, win, acc, end_;
     __host__ __device__ AdjLayout(int ntype, int tabN, int windowSlots)
     {
         nbfp = 0;
@@ -44,8 +46,7 @@ struct AdjLayout
         ix   = tab + tabN;
         ig   = ix + kAdjWaves * kGroupSizeMax * kClusterSize;
         it   = ig + kAdjWaves * kGroupSizeMax * kClusterSize;
-        uc   = it + kAdjWaves * kGroupSizeMax;
-        win  = uc + kAdjWaves * ((windowSlots + 3) / 4);
+        win  = it + kAdjWaves * kGroupSizeMax;
         acc  = win + kAdjWaves * windowSlots * 4;
         end_ = acc + ntype * ntype; // {dC6, dC12} per type pair as two doubles
     }
@@ -238,7 +239,6 @@ __global__ __launch_bounds__(kAdjThreads) void adjointKernel(const AdjointArgs a
     float4* ix  = smem4 + L.ix + wave * kGroupSizeMax * kClusterSize;
     float4* ig  = smem4 + L.ig + wave * kGroupSizeMax * kClusterSize;
     int*    it  = reinterpret_cast<int*>(smem4 + L.it) + wave * kGroupSizeMax * kClusterSize;
-    int*    uc  = reinterpret_cast<int*>(smem4 + L.uc) + wave * ((a.windowSlots + 3) / 4) * 4;
     float*  win = reinterpret_cast<float*>(smem4 + L.win) + wave * a.windowSlots * 16;
 
     // the wavefront's run of chunks, as in the forward kernel (the same
@@ -254,7 +254,7 @@ __global__ __launch_bounds__(kAdjThreads) void adjointKernel(const AdjointArgs a
         const unsigned W0     = static_cast<unsigned>(uniformInt(cFirst.z));
         const int      T      = (uniformInt(cLast.z) - static_cast<int>(W0)) / kTileJ + uniformInt(cLast.w);
 
-        int    g = uniformInt(ws.y), chGrpEnd = 0, uoff = 0, U = 0, c0 = 0, nOwn = 0, G = 0;
+        int    g = uniformInt(ws.y), chGrpEnd = 0, U = 0, c0 = 0, nOwn = 0, G = 0, stage = -1;
         IAtoms I;
         IGrad  Gi;
         float4 sv;
@@ -265,9 +265,10 @@ __global__ __launch_bounds__(kAdjThreads) void adjointKernel(const AdjointArgs a
         auto beginGroup = [&](int gg) {
             const int4 grp = a.groups[gg];
             const int4 gi  = a.groupInfo[gg];
+            const int4 si  = a.segInfo[gg];
             chGrpEnd       = uniformInt(grp.y);
-            uoff           = uniformInt(grp.z);
             U              = uniformInt(grp.w);
+            stage = uniformInt(si.z) > 0 ? uniformInt(si.x) + (wid - uniformInt(si.y)) * uniformInt(si.w) : -1;
             c0             = uniformInt(gi.x);
             nOwn           = uniformInt(gi.y) & 0xffff;
             G              = uniformInt(gi.y) >> 16;
@@ -281,38 +282,16 @@ __global__ __launch_bounds__(kAdjThreads) void adjointKernel(const AdjointArgs a
                 ig[lane] = a.gq[c0 * kClusterSize + lane];
                 it[lane] = a.type[c0 * kClusterSize + lane] * ntype;
             }
-            for (int t = lane; t < U; t += kWave)
-            {
-                uc[t] = a.unionList[uoff + t];
-            }
         };
         auto endGroup = [&]() {
-            // a lane per window atom: dL/dx by float atomics, dL/dq by double
-            for (int t = lane; t < (G + U) * kClusterSize; t += kWave)
+            // stage the window (4 float4 per cluster slot) with plain stores;
+            // the adjoint's gather adds each cluster's copies in a fixed order
+            if (stage >= 0)
             {
-                const int    slot = t >> 2;
-                const float4 v    = reinterpret_cast<const float4*>(win)[t];
-                const int    cl   = slot < G ? (slot < nOwn ? c0 + slot : -1) : uc[slot - G];
-                if (cl >= 0)
+                const float4* w4 = reinterpret_cast<const float4*>(win);
+                for (int t = lane; t < (G + U) * 4; t += kWave)
                 {
-                    const int k   = cl * kClusterSize + (t & 3);
-                    float*    dst = reinterpret_cast<float*>(a.gradX + k);
-                    if (v.x != 0.0f)
-                    {
-                        unsafeAtomicAdd(dst, v.x);
-                    }
-                    if (v.y != 0.0f)
-                    {
-                        unsafeAtomicAdd(dst + 1, v.y);
-                    }
-                    if (v.z != 0.0f)
-                    {
-                        unsafeAtomicAdd(dst + 2, v.z);
-                    }
-                    if (v.w != 0.0f)
-                    {
-                        unsafeAtomicAdd(a.gradQ + k, static_cast<double>(v.w));
-                    }
+                    a.staging[stage + t] = w4[t];
                 }
             }
         };
@@ -487,35 +466,79 @@ __global__ __launch_bounds__(kWave) void adjointParamKernel(int numBlocks, int n
     }
 }
 
-// Cluster layout to atom order: dL/dx (float) and dL/dq (double)
-__global__ void adjointGatherKernel(int natoms, const int* __restrict__ cells, const float4* __restrict__ gradX,
-                                    const double* __restrict__ gradQ, float* __restrict__ outX,
-                                    double* __restrict__ outQ)
+// The staged adjoint windows of each cluster (a wavefront per cluster, as
+// the force gather): dL/dx summed in float with a fixed butterfly, dL/dq in
+// double; plus the cluster-layout buffers, which hold the contributions of
+// clusters that spilled out of a window, written to the atom-order outputs
+// (either may be NULL).
+__global__ __launch_bounds__(256) void adjointGatherStagedKernel(int ncluster, const int* __restrict__ gposOff,
+                                                                 const int* __restrict__ gpos,
+                                                                 const float4* __restrict__ staging,
+                                                                 const float4* __restrict__ gradX,
+                                                                 const double* __restrict__ gradQ,
+                                                                 const int* __restrict__ atomIndices,
+                                                                 float* __restrict__ outX, double* __restrict__ outQ)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= natoms)
+    const int lane = threadIdx.x & 63;
+    const int c    = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (c >= ncluster)
     {
         return;
     }
-    const int    c = cells[i];
-    const float4 v = gradX[c];
-    if (outX)
+    float  f[16];
+    double q[4] = { 0.0, 0.0, 0.0, 0.0 };
+#pragma unroll
+    for (int k = 0; k < 16; k++)
     {
-        outX[3 * i + 0] = v.x;
-        outX[3 * i + 1] = v.y;
-        outX[3 * i + 2] = v.z;
+        f[k] = 0.0f;
     }
-    if (outQ)
+    for (int i = gposOff[c] + lane; i < gposOff[c + 1]; i += kWave)
     {
-        outQ[i] = gradQ[c];
+        const int p = gpos[i];
+#pragma unroll
+        for (int at = 0; at < 4; at++)
+        {
+            const float4 v = staging[p + at];
+            f[4 * at + 0] += v.x;
+            f[4 * at + 1] += v.y;
+            f[4 * at + 2] += v.z;
+            q[at] += static_cast<double>(v.w);
+        }
+    }
+    // lane L ends up with component L >> 2 = 4 * atom + {x, y, z, q}
+    const float r   = waveReduceScatter16(f, lane);
+    const int   idx = lane >> 2;
+    if (outX && (lane & 3) == 0 && (idx & 3) < 3)
+    {
+        const int k = c * kClusterSize + (idx >> 2);
+        const int a = atomIndices[k];
+        if (a >= 0)
+        {
+            outX[3 * a + (idx & 3)] = reinterpret_cast<const float*>(gradX + k)[idx & 3] + r;
+        }
+    }
+#pragma unroll
+    for (int at = 0; at < 4; at++)
+    {
+        const double t = waveSumD(q[at]);
+        if (outQ && lane == at)
+        {
+            const int k = c * kClusterSize + at;
+            const int a = atomIndices[k];
+            if (a >= 0)
+            {
+                outQ[a] = gradQ[k] + t;
+            }
+        }
     }
 }
 
-// g (atom order, 3 floats per atom) into the cluster layout; filler slots
-// and the far-away padding cluster after the last one (slots nslot ..
-// nslot + 3, read by padding work entries) get zeros
-__global__ void convertGKernel(int nslot, const int* __restrict__ atomIndices, const float* __restrict__ g,
-                               float4* __restrict__ gq)
+// A backward step's set-up in one launch: coordinates (when x is given) and
+// the cotangent g into the cluster layout (fillers and the padding cluster
+// after the last one get g = 0), the spill accumulators cleared.
+__global__ void adjointPrepareKernel(int nslot, const int* __restrict__ atomIndices, const float* __restrict__ x,
+                                     const float* __restrict__ g, float4* __restrict__ xq, float4* __restrict__ gq,
+                                     float4* __restrict__ gradX, double* __restrict__ gradQ)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslot + kClusterSize)
@@ -523,17 +546,19 @@ __global__ void convertGKernel(int nslot, const int* __restrict__ atomIndices, c
         return;
     }
     const int a = s < nslot ? atomIndices[s] : -1;
-    gq[s] = a >= 0 ? make_float4(g[3 * a + 0], g[3 * a + 1], g[3 * a + 2], 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-}
-
-// zeroes the cluster-layout gradient buffers
-__global__ void clearAdjointKernel(int n, float4* __restrict__ gradX, double* __restrict__ gradQ)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n)
+    if (x && a >= 0)
     {
-        gradX[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        gradQ[i] = 0.0;
+        float4 v = xq[s];
+        v.x      = x[3 * a + 0];
+        v.y      = x[3 * a + 1];
+        v.z      = x[3 * a + 2];
+        xq[s]    = v;
+    }
+    gq[s] = a >= 0 ? make_float4(g[3 * a + 0], g[3 * a + 1], g[3 * a + 2], 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (s < nslot)
+    {
+        gradX[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        gradQ[s] = 0.0;
     }
 }
 
@@ -579,15 +604,10 @@ int adjointBlocks(int forwardBlocks)
     return forwardBlocks * kKernelWaves / kAdjWaves;
 }
 
-void launchAdjoint(hipStream_t s, const AdjointArgs& a, int elec, int forwardBlocks, int natoms, const int* cells,
+void launchAdjoint(hipStream_t s, const AdjointArgs& a, int elec, int forwardBlocks, const int* atomIndices,
                    float* gradXOut, double* gradQOut, int numTypes, double* gradC6Out, double* gradC12Out)
 {
     const int numBlocks = adjointBlocks(forwardBlocks);
-    const int nslot     = a.ncluster * kClusterSize;
-    if (nslot > 0)
-    {
-        clearAdjointKernel<<<(nslot + 255) / 256, 256, 0, s>>>(nslot, a.gradX, a.gradQ);
-    }
     const int lds = adjointLdsBytes(a.ntype, a.ic.tab_n, elec, a.windowSlots);
     if (gradC6Out || gradC12Out)
     {
@@ -599,15 +619,19 @@ void launchAdjoint(hipStream_t s, const AdjointArgs& a, int elec, int forwardBlo
     {
         launchElec<false>(s, a, elec, numBlocks, lds);
     }
-    if (natoms > 0)
+    if (a.ncluster > 0 && (gradXOut || gradQOut))
     {
-        adjointGatherKernel<<<(natoms + 255) / 256, 256, 0, s>>>(natoms, cells, a.gradX, a.gradQ, gradXOut, gradQOut);
+        adjointGatherStagedKernel<<<(a.ncluster + 3) / 4, 256, 0, s>>>(a.ncluster, a.gposOff, a.gpos, a.staging,
+                                                                        a.gradX, a.gradQ, atomIndices, gradXOut,
+                                                                        gradQOut);
     }
 }
 
-void launchConvertG(hipStream_t s, int nslot, const int* atomIndices, const float* g, float4* gq)
+void launchAdjointPrepare(hipStream_t s, int nslot, const int* atomIndices, const float* x, const float* g, float4* xq,
+                          float4* gq, float4* gradX, double* gradQ)
 {
-    convertGKernel<<<(nslot + kClusterSize + 255) / 256, 256, 0, s>>>(nslot, atomIndices, g, gq);
+    adjointPrepareKernel<<<(nslot + kClusterSize + 255) / 256, 256, 0, s>>>(nslot, atomIndices, x, g, xq, gq, gradX,
+                                                                           gradQ);
 }
 
 } // namespace nbx

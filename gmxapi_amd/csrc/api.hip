@@ -217,9 +217,14 @@ const float* deviceCoordinates(nbx_context* c, const float* x, int onDevice)
 
 // Groups of consecutive i-clusters for the kernel's LDS j-force window
 // (groups.hip): the largest group size whose j-entries fit the LDS sort and
-// whose window fits the kernel's LDS budget.
-void buildWorkGroups(nbx_context* c)
+// whose window fits the kernel's LDS budget (the pair kernel's, or with
+// adjoint the adjoint kernel's, whose window holds 4 floats per atom).
+void buildWorkGroups(nbx_context* c, WorkLayout& L, bool adjoint)
 {
+    auto ldsBytes = [&](int slots) {
+        return adjoint ? adjointLdsBytes(c->ntype, c->ic.tab_n, c->elec, slots)
+                       : nonbondedLdsBytes(c->ntype, c->ic.tab_n, c->elec, slots);
+    };
     const int nc = c->ncluster;
     c->groupStats.ensure(9);
     check(hipMemsetAsync(c->groupStats.p, 0, sizeof(int) * 9, c->stream), "clear stats");
@@ -234,13 +239,13 @@ void buildWorkGroups(nbx_context* c)
     // When no size fits with at most 2% of the union entries spilled to
     // global atomics (long cut-offs, many types), the last pass takes the
     // smallest (or the chosen) size with whatever spill its window leaves.
-    const int forced = c->opt.kernel_group_size;
+    const int forced = adjoint ? 0 : c->opt.kernel_group_size;
     for (int k = 0; k < 5; k++)
     {
         const bool fallback = (k == 4);
         const int  G        = fallback ? (forced > 0 ? forced : 1) : kGroupSizeMax >> k;
         const int  si       = G == 8 ? 0 : G == 4 ? 1 : G == 2 ? 2 : 3; // spans[] index of G
-        const int  budget   = kKernelLdsBudget;
+        const int  budget   = adjoint ? kAdjointLayoutBudget : kKernelLdsBudget;
         if (!fallback && forced > 0 && G != forced)
         {
             continue;
@@ -255,13 +260,13 @@ void buildWorkGroups(nbx_context* c)
                              + " cluster-pair entries: the pair-list cut-off is too long for this kernel");
         }
         const int ng = (nc + G - 1) / G;
-        c->unionCount.ensure(ng + 1);
-        c->chunkCount.ensure(ng + 1);
-        c->workCount.ensure(ng + 1);
-        c->overflow.ensure(ng + 1);
-        c->unionOff.ensure(ng + 1);
-        c->chunkOff.ensure(ng + 1);
-        c->workOff.ensure(ng + 1);
+        L.unionCount.ensure(ng + 1);
+        L.chunkCount.ensure(ng + 1);
+        L.workCount.ensure(ng + 1);
+        L.overflow.ensure(ng + 1);
+        L.unionOff.ensure(ng + 1);
+        L.chunkOff.ensure(ng + 1);
+        L.workOff.ensure(ng + 1);
         GroupArgs ga{};
         ga.ncluster     = nc;
         ga.groupSize    = G;
@@ -272,17 +277,17 @@ void buildWorkGroups(nbx_context* c)
         ga.ientry       = c->ientry.p;
         ga.jentryIn     = c->jentry.p;
         ga.jentryOut    = c->jentryTmp.p;
-        ga.unionCount   = c->unionCount.p;
-        ga.chunkCount   = c->chunkCount.p;
-        ga.workCount    = c->workCount.p;
-        ga.overflow     = c->overflow.p;
+        ga.unionCount   = L.unionCount.p;
+        ga.chunkCount   = L.chunkCount.p;
+        ga.workCount    = L.workCount.p;
+        ga.overflow     = L.overflow.p;
         launchBuildGroups(c->stream, ga, false);
         check(hipMemsetAsync(c->groupStats.p + 4, 0, sizeof(int) * 2, c->stream), "clear stats");
-        launchMaxReduce(c->stream, ng, c->unionCount.p, c->groupStats.p + 4);
-        launchMaxReduce(c->stream, ng, c->overflow.p, c->groupStats.p + 5);
-        launchExclusiveScan(c->stream, ng, c->unionCount.p, c->unionOff.p, c->groupStats.p + 6);
-        launchExclusiveScan(c->stream, ng, c->chunkCount.p, c->chunkOff.p, c->groupStats.p + 7);
-        launchExclusiveScan(c->stream, ng, c->workCount.p, c->workOff.p, c->groupStats.p + 8);
+        launchMaxReduce(c->stream, ng, L.unionCount.p, c->groupStats.p + 4);
+        launchMaxReduce(c->stream, ng, L.overflow.p, c->groupStats.p + 5);
+        launchExclusiveScan(c->stream, ng, L.unionCount.p, L.unionOff.p, c->groupStats.p + 6);
+        launchExclusiveScan(c->stream, ng, L.chunkCount.p, L.chunkOff.p, c->groupStats.p + 7);
+        launchExclusiveScan(c->stream, ng, L.workCount.p, L.workOff.p, c->groupStats.p + 8);
         int st[5];
         check(hipMemcpyAsync(st, c->groupStats.p + 4, sizeof(st), hipMemcpyDeviceToHost, c->stream), "read stats");
         check(hipStreamSynchronize(c->stream), "group sync");
@@ -297,18 +302,18 @@ void buildWorkGroups(nbx_context* c)
         // take direct global atomics, which is accepted while that is at most
         // 2% of the union entries (any share in the last pass)
         int slots = G + maxUnion + 1;
-        if (nonbondedLdsBytes(c->ntype, c->ic.tab_n, c->elec, slots) > budget)
+        if (ldsBytes(slots) > budget)
         {
-            while (slots > G + 2 && nonbondedLdsBytes(c->ntype, c->ic.tab_n, c->elec, slots) > budget)
+            while (slots > G + 2 && ldsBytes(slots) > budget)
             {
                 slots--;
             }
-            if (nonbondedLdsBytes(c->ntype, c->ic.tab_n, c->elec, slots) > budget)
+            if (ldsBytes(slots) > budget)
             {
                 continue;
             }
             std::vector<int> counts(ng);
-            check(hipMemcpy(counts.data(), c->unionCount.p, sizeof(int) * ng, hipMemcpyDeviceToHost), "read unions");
+            check(hipMemcpy(counts.data(), L.unionCount.p, sizeof(int) * ng, hipMemcpyDeviceToHost), "read unions");
             long long spilled = 0;
             for (int u : counts)
             {
@@ -319,36 +324,68 @@ void buildWorkGroups(nbx_context* c)
                 continue;
             }
         }
-        c->unionList.ensure(std::max(unionTotal, 1));
-        c->chunks.ensure(std::max(chunkTotal, 1));
-        c->groups.ensure(ng);
-        c->groupInfo.ensure(ng);
-        c->work.ensure(std::max(workTotal, 1));
-        ga.unionOff  = c->unionOff.p;
-        ga.chunkOff  = c->chunkOff.p;
-        ga.workOff   = c->workOff.p;
-        ga.unionList = c->unionList.p;
-        ga.work      = c->work.p;
-        ga.chunks    = c->chunks.p;
-        ga.groups    = c->groups.p;
-        ga.groupInfo = c->groupInfo.p;
+        L.unionList.ensure(std::max(unionTotal, 1));
+        L.chunks.ensure(std::max(chunkTotal, 1));
+        L.groups.ensure(ng);
+        L.groupInfo.ensure(ng);
+        L.work.ensure(std::max(workTotal, 1));
+        ga.unionOff  = L.unionOff.p;
+        ga.chunkOff  = L.chunkOff.p;
+        ga.workOff   = L.workOff.p;
+        ga.unionList = L.unionList.p;
+        ga.work      = L.work.p;
+        ga.chunks    = L.chunks.p;
+        ga.groups    = L.groups.p;
+        ga.groupInfo = L.groupInfo.p;
         ga.windowCap = slots;
         launchBuildGroups(c->stream, ga, true);
-        c->chunkShift.ensure(std::max(chunkTotal, 1));
-        launchChunkShifts(c->stream, chunkTotal, c->chunks.p, c->shiftVec.p, c->chunkShift.p);
+        L.chunkShift.ensure(std::max(chunkTotal, 1));
+        launchChunkShifts(c->stream, chunkTotal, L.chunks.p, c->shiftVec.p, L.chunkShift.p);
         check(hipGetLastError(), "group kernels");
         c->jentry.swap(c->jentryTmp);
-        buildOccurrences(c->stream, c->gather, ng, c->groups.p, c->groupInfo.p, c->unionList.p, nc);
-        c->groupSize   = G;
-        c->ngroups     = ng;
-        c->nchunk      = chunkTotal;
-        c->nwork       = workTotal;
-        c->windowSlots = slots;
+        buildOccurrences(c->stream, L.gather, ng, L.groups.p, L.groupInfo.p, L.unionList.p, nc);
+        L.groupSize   = G;
+        L.ngroups     = ng;
+        L.nchunk      = chunkTotal;
+        L.nwork       = workTotal;
+        L.windowSlots = slots;
         return;
     }
     throw InputError("the j-cluster window of one cluster does not fit in LDS: the pair-list cut-off is too long, or "
                      "there are too many atom types, for this kernel");
 }
+// The layout's grid and static schedule (the workgroups that can be
+// resident, every wavefront a contiguous run of chunks of about equal cost)
+// and its window staging.  The pair kernel runs kKernelWaves wavefronts per
+// workgroup; the adjoint two (its numBlocks counts groups of four
+// wavefronts, as the schedule does).
+void scheduleLayout(nbx_context* c, WorkLayout& L, bool adjoint)
+{
+    int numWaves;
+    if (adjoint)
+    {
+        const int lds   = adjointLdsBytes(c->ntype, c->ic.tab_n, c->elec, L.windowSlots);
+        const int perCU = std::max(1, std::min(8, (160 * 1024) / lds)); // two-wavefront workgroups
+        numWaves        = std::max(kKernelWaves, std::min((L.nchunk + kKernelWaves - 1) / kKernelWaves * kKernelWaves,
+                                                          perCU * c->numCU * 2));
+        numWaves        = (numWaves + kKernelWaves - 1) / kKernelWaves * kKernelWaves;
+    }
+    else
+    {
+        const int lds      = nonbondedLdsBytes(c->ntype, c->ic.tab_n, c->elec, L.windowSlots);
+        const int maxPerCU = c->opt.kernel_blocks_per_cu > 0 ? c->opt.kernel_blocks_per_cu : 4;
+        const int perCU    = std::max(1, std::min(maxPerCU, (160 * 1024) / lds));
+        numWaves = std::max(1, std::min((L.nchunk + kKernelWaves - 1) / kKernelWaves, perCU * c->numCU)) * kKernelWaves;
+    }
+    L.numBlocks = numWaves / kKernelWaves;
+    L.waveStart.ensure(numWaves + 1);
+    launchWaveSchedule(c->stream, numWaves, L.nchunk, L.chunks.p, L.nwork, kChunkWeight, L.ngroups, L.groups.p,
+                       L.waveStart.p);
+    check(hipGetLastError(), "schedule kernel");
+    buildSegments(c->stream, L.gather, L.ngroups, L.groups.p, L.groupInfo.p, numWaves, L.waveStart.p, c->ncluster,
+                  adjoint ? 4 : 3);
+}
+
 
 } // namespace
 
@@ -882,29 +919,18 @@ int nbx_construct_pairlist(nbx_context* c)
         a.jentry = c->jentry.p;
         launchSearch(c->stream, a, true);
         check(hipGetLastError(), "search kernels");
-        buildWorkGroups(c);
-        // kernel grid: the workgroups (4 wavefronts each) that can be resident
-        // at once, at most four per CU; every wavefront gets a contiguous run
-        // of chunks of about equal cost
-        const int lds      = nonbondedLdsBytes(c->ntype, c->ic.tab_n, c->elec, c->windowSlots);
-        const int maxPerCU = c->opt.kernel_blocks_per_cu > 0 ? c->opt.kernel_blocks_per_cu : 4;
-        const int perCU    = std::max(1, std::min(maxPerCU, (160 * 1024) / lds));
-        c->numBlocks       = std::max(1, std::min((c->nchunk + kKernelWaves - 1) / kKernelWaves, perCU * c->numCU));
-        const int numWaves = c->numBlocks * kKernelWaves;
-        c->waveStart.ensure(numWaves + 1);
-        launchWaveSchedule(c->stream, numWaves, c->nchunk, c->chunks.p, c->nwork, kChunkWeight, c->ngroups,
-                           c->groups.p, c->waveStart.p);
-        check(hipGetLastError(), "schedule kernel");
-        buildSegments(c->stream, c->gather, c->ngroups, c->groups.p, c->groupInfo.p, numWaves, c->waveStart.p,
-                      c->ncluster);
-        c->energyPartial.ensure(2 * static_cast<size_t>(c->ngroups + numWaves));
+        buildWorkGroups(c, c->lay, false);
+        scheduleLayout(c, c->lay, false);
+        const int numWaves = c->lay.numBlocks * kKernelWaves;
+        c->energyPartial.ensure(2 * static_cast<size_t>(c->lay.ngroups + numWaves));
         c->exclPartial.ensure(kExclMaxBlocks);
-        c->fshiftPartial.ensure(static_cast<size_t>(c->numBlocks) * kNumShifts * 3);
+        c->fshiftPartial.ensure(static_cast<size_t>(c->lay.numBlocks) * kNumShifts * 3);
         c->energy.ensure(2);
         c->fshift.ensure(kNumShifts * 3);
         c->virialPartial.ensure(9 * 1024);
         c->virial.ensure(9);
-        c->listValid  = true;
+        c->listValid = true;
+        c->adjValid  = false;
         c->haveEnergy = c->haveShift = c->haveVirial = false;
     });
 }
@@ -934,29 +960,29 @@ static void enqueueStep(nbx_context* c, unsigned flags, const float* xd, float* 
     const bool shift  = (flags & (NBX_COMPUTE_SHIFT_FORCES | NBX_COMPUTE_VIRIAL)) != 0;
     // the virial needs the atom-order forces and coordinates of this step
     const bool virial = (flags & NBX_COMPUTE_VIRIAL) != 0 && fOut && xd;
-    const int numSlots = c->ngroups + c->numBlocks * kKernelWaves;
+    const int numSlots = c->lay.ngroups + c->lay.numBlocks * kKernelWaves;
     launchPrepareStep(c->stream, c->ncluster * kClusterSize, c->atomIndices.p, xd, c->xq.p, c->fnb.p,
                       energy ? c->energyPartial.p : nullptr, 2 * numSlots);
     KernelArgs a{};
-    a.chunks        = c->chunks.p;
-    a.groups        = c->groups.p;
-    a.groupInfo     = c->groupInfo.p;
-    a.ngroups       = c->ngroups;
-    a.groupSize     = c->groupSize;
+    a.chunks        = c->lay.chunks.p;
+    a.groups        = c->lay.groups.p;
+    a.groupInfo     = c->lay.groupInfo.p;
+    a.ngroups       = c->lay.ngroups;
+    a.groupSize     = c->lay.groupSize;
     a.ncluster      = c->ncluster;
-    a.windowSlots   = c->windowSlots;
-    a.unionList     = c->unionList.p;
-    a.work          = c->work.p;
+    a.windowSlots   = c->lay.windowSlots;
+    a.unionList     = c->lay.unionList.p;
+    a.work          = c->lay.work.p;
     a.xq            = c->xq.p;
     a.type          = c->nbtype.p;
     a.nbfp          = c->nbfp.p;
     a.ntype         = c->ntype;
     a.tabFDV0       = c->tabFDV0.p;
-    a.chunkShift    = c->chunkShift.p;
+    a.chunkShift    = c->lay.chunkShift.p;
     a.ic            = c->ic;
-    a.waveStart     = c->waveStart.p;
-    a.segInfo       = c->gather.segInfo.p;
-    a.staging       = c->gather.staging.p;
+    a.waveStart     = c->lay.waveStart.p;
+    a.segInfo       = c->lay.gather.segInfo.p;
+    a.staging       = c->lay.gather.staging.p;
     a.fnb           = c->fnb.p;
     a.energyPartial = c->energyPartial.p;
     a.fshiftPartial = c->fshiftPartial.p;
@@ -965,13 +991,13 @@ static void enqueueStep(nbx_context* c, unsigned flags, const float* xd, float* 
     {
         check(hipEventRecord(c->timingEvents[2 * c->timingRecorded], c->stream), "hipEventRecord");
     }
-    launchNonbonded(c->stream, a, c->elec, energy, shift, c->numBlocks);
+    launchNonbonded(c->stream, a, c->elec, energy, shift, c->lay.numBlocks);
     if (timed)
     {
         check(hipEventRecord(c->timingEvents[2 * c->timingRecorded + 1], c->stream), "hipEventRecord");
         c->timingRecorded++;
     }
-    launchGatherForces(c->stream, c->ncluster, c->gather.gposOff.p, c->gather.gpos.p, c->gather.staging.p,
+    launchGatherForces(c->stream, c->ncluster, c->lay.gather.gposOff.p, c->lay.gather.gpos.p, c->lay.gather.staging.p,
                        c->fnb.p, c->atomIndices.p, fOut);
     int numExcl = 0;
     if (energy || virial)
@@ -997,7 +1023,7 @@ static void enqueueStep(nbx_context* c, unsigned flags, const float* xd, float* 
     if (energy || shift || virial)
     {
         launchFinalize(c->stream, energy ? numSlots : 0, c->energyPartial.p, energy ? numExcl : 0,
-                       c->exclPartial.p, c->numBlocks, c->fshiftPartial.p, shift, c->energy.p, c->fshift.p,
+                       c->exclPartial.p, c->lay.numBlocks, c->fshiftPartial.p, shift, c->energy.p, c->fshift.p,
                        virial ? numExcl : 0, c->virialPartial.p, c->shiftVec.p, virial ? c->virial.p : nullptr);
     }
     check(hipGetLastError(), "nonbonded kernel");
@@ -1132,7 +1158,15 @@ int nbx_compute_adjoint(nbx_context* c, const float* x, const float* g, float* g
         {
             throw InputError("the adjoint supports at most " + std::to_string(kAdjointMaxTypes - 1) + " particle types");
         }
-        const int lds = adjointLdsBytes(c->ntype, c->ic.tab_n, c->elec, c->windowSlots);
+        setDevice(c);
+        if (!c->adjValid)
+        {
+            // the adjoint's own work layout: groups sized for its window
+            buildWorkGroups(c, c->adj, true);
+            scheduleLayout(c, c->adj, true);
+            c->adjValid = true;
+        }
+        const int lds = adjointLdsBytes(c->ntype, c->ic.tab_n, c->elec, c->adj.windowSlots);
         if (lds > kAdjointLdsBudget)
         {
             throw InputError("the adjoint's j-cluster window does not fit in LDS: choose a smaller kernel_group_size");
@@ -1141,8 +1175,7 @@ int nbx_compute_adjoint(nbx_context* c, const float* x, const float* g, float* g
         const size_t na    = static_cast<size_t>(c->natoms);
         const int    nslot = c->ncluster * kClusterSize;
         const float* xd    = deviceCoordinates(c, x, onDevice);
-        launchConvertX(c->stream, nslot, c->atomIndices.p, xd, c->xq.p);
-        const float* gd = g;
+        const float* gd    = g;
         if (!onDevice)
         {
             c->gIn.ensure(3 * na);
@@ -1150,10 +1183,10 @@ int nbx_compute_adjoint(nbx_context* c, const float* x, const float* g, float* g
             gd = c->gIn.p;
         }
         c->gq.ensure(nslot + kClusterSize);
-        launchConvertG(c->stream, nslot, c->atomIndices.p, gd, c->gq.p);
         c->gradXnb.ensure(nslot + kClusterSize);
         c->gradQnb.ensure(nslot + kClusterSize);
-        const int    blocks = adjointBlocks(c->numBlocks);
+        launchAdjointPrepare(c->stream, nslot, c->atomIndices.p, xd, gd, c->xq.p, c->gq.p, c->gradXnb.p, c->gradQnb.p);
+        const int    blocks = adjointBlocks(c->adj.numBlocks);
         const size_t np     = 2 * static_cast<size_t>(c->ntype) * c->ntype;
         c->paramPartial.ensure(blocks * np);
         const int tu    = c->ntypeUser;
@@ -1172,26 +1205,30 @@ int nbx_compute_adjoint(nbx_context* c, const float* x, const float* g, float* g
             g12Out = gradC12 ? c->paramOut.p + static_cast<size_t>(tu) * tu : nullptr;
         }
         AdjointArgs a{};
-        a.chunks       = c->chunks.p;
-        a.groups       = c->groups.p;
-        a.groupInfo    = c->groupInfo.p;
+        a.chunks       = c->adj.chunks.p;
+        a.groups       = c->adj.groups.p;
+        a.groupInfo    = c->adj.groupInfo.p;
         a.ncluster     = c->ncluster;
-        a.windowSlots  = c->windowSlots;
-        a.unionList    = c->unionList.p;
-        a.work         = c->work.p;
+        a.windowSlots  = c->adj.windowSlots;
+        a.unionList    = c->adj.unionList.p;
+        a.work         = c->adj.work.p;
         a.xq           = c->xq.p;
         a.gq           = c->gq.p;
         a.type         = c->nbtype.p;
         a.nbfp         = c->nbfp.p;
         a.ntype        = c->ntype;
         a.tabFDV0      = c->tabFDV0.p;
-        a.chunkShift   = c->chunkShift.p;
+        a.chunkShift   = c->adj.chunkShift.p;
         a.ic           = c->ic;
-        a.waveStart    = c->waveStart.p;
+        a.waveStart    = c->adj.waveStart.p;
+        a.segInfo      = c->adj.gather.segInfo.p;
+        a.staging      = c->adj.gather.staging.p;
+        a.gposOff      = c->adj.gather.gposOff.p;
+        a.gpos         = c->adj.gather.gpos.p;
         a.gradX        = c->gradXnb.p;
         a.gradQ        = c->gradQnb.p;
         a.paramPartial = c->paramPartial.p;
-        launchAdjoint(c->stream, a, c->elec, c->numBlocks, c->natoms, c->cells.p, gxOut, gqOut, tu, g6Out, g12Out);
+        launchAdjoint(c->stream, a, c->elec, c->adj.numBlocks, c->atomIndices.p, gxOut, gqOut, tu, g6Out, g12Out);
         check(hipGetLastError(), "adjoint kernels");
         if (!onDevice && gradX)
         {
@@ -1405,34 +1442,34 @@ int nbx_get_work_groups(nbx_context* c, int* info, int* groups, int* chunks, int
     return guarded([&] {
         requireList(c);
         setDevice(c);
-        info[0] = c->groupSize;
-        info[1] = c->ngroups;
-        info[2] = c->nchunk;
-        info[3] = c->windowSlots;
-        info[4] = c->nwork;
+        info[0] = c->lay.groupSize;
+        info[1] = c->lay.ngroups;
+        info[2] = c->lay.nchunk;
+        info[3] = c->lay.windowSlots;
+        info[4] = c->lay.nwork;
         if (groups)
         {
-            check(hipMemcpyAsync(groups, c->groups.p, sizeof(int4) * c->ngroups, hipMemcpyDeviceToHost, c->stream),
+            check(hipMemcpyAsync(groups, c->lay.groups.p, sizeof(int4) * c->lay.ngroups, hipMemcpyDeviceToHost, c->stream),
                   "read groups");
         }
         if (chunks)
         {
-            check(hipMemcpyAsync(chunks, c->chunks.p, sizeof(int4) * c->nchunk, hipMemcpyDeviceToHost, c->stream),
+            check(hipMemcpyAsync(chunks, c->lay.chunks.p, sizeof(int4) * c->lay.nchunk, hipMemcpyDeviceToHost, c->stream),
                   "read chunks");
         }
         if (unionList)
         {
             int4 last;
-            check(hipMemcpyAsync(&last, c->groups.p + (c->ngroups - 1), sizeof(int4), hipMemcpyDeviceToHost, c->stream),
+            check(hipMemcpyAsync(&last, c->lay.groups.p + (c->lay.ngroups - 1), sizeof(int4), hipMemcpyDeviceToHost, c->stream),
                   "read groups");
             check(hipStreamSynchronize(c->stream), "sync");
-            check(hipMemcpyAsync(unionList, c->unionList.p, sizeof(int) * (last.z + last.w), hipMemcpyDeviceToHost,
+            check(hipMemcpyAsync(unionList, c->lay.unionList.p, sizeof(int) * (last.z + last.w), hipMemcpyDeviceToHost,
                                  c->stream),
                   "read union");
         }
         if (workOut)
         {
-            check(hipMemcpyAsync(workOut, c->work.p, sizeof(uint2) * c->nwork, hipMemcpyDeviceToHost, c->stream),
+            check(hipMemcpyAsync(workOut, c->lay.work.p, sizeof(uint2) * c->lay.nwork, hipMemcpyDeviceToHost, c->stream),
                   "read work list");
         }
         check(hipStreamSynchronize(c->stream), "sync");

@@ -175,13 +175,13 @@ void buildBatch(nbx_batch* b, int G, int blocksPerCU, int chunkWeight)
         std::vector<int>    ty = download(c->nbtype.p, 4 * static_cast<size_t>(nc) + 4);
         std::vector<int>    ai = download(c->atomIndices.p, 4 * static_cast<size_t>(nc));
         std::vector<int>    ce = download(c->cells.p, na);
-        std::vector<uint2>  wk = download(c->work.p, c->nwork);
-        std::vector<int4>   ch = download(c->chunks.p, c->nchunk);
-        std::vector<float4> cs = download(c->chunkShift.p, c->nchunk);
-        std::vector<int4>   gr = download(c->groups.p, c->ngroups);
-        std::vector<int4>   gi = download(c->groupInfo.p, c->ngroups);
+        std::vector<uint2>  wk = download(c->lay.work.p, c->lay.nwork);
+        std::vector<int4>   ch = download(c->lay.chunks.p, c->lay.nchunk);
+        std::vector<float4> cs = download(c->lay.chunkShift.p, c->lay.nchunk);
+        std::vector<int4>   gr = download(c->lay.groups.p, c->lay.ngroups);
+        std::vector<int4>   gi = download(c->lay.groupInfo.p, c->lay.ngroups);
         const int4          last = gr.back();
-        std::vector<int>    un = download(c->unionList.p, last.z + last.w);
+        std::vector<int>    un = download(c->lay.unionList.p, last.z + last.w);
         std::vector<float4> sv = download(c->shiftVec.p, kNumShifts);
         std::vector<int>    eo = download(c->exclOff.p, na + 1);
         std::vector<int>    ei = download(c->exclIdx.p, eo[na]);
@@ -233,7 +233,7 @@ void buildBatch(nbx_batch* b, int G, int blocksPerCU, int chunkWeight)
         {
             hExIdx.push_back(v + atomOffset);
         }
-        maxSlots = std::max(maxSlots, c->windowSlots);
+        maxSlots = std::max(maxSlots, c->lay.windowSlots);
         ncj += c->ncj;
         clusterOff += nc;
         atomOffset += na;
@@ -269,7 +269,7 @@ void buildBatch(nbx_batch* b, int G, int blocksPerCU, int chunkWeight)
     b->nwork       = static_cast<int>(hWork.size());
     b->windowSlots = maxSlots;
     b->ncj         = ncj;
-    b->groupSize   = G > 0 ? G : b->sys[0]->groupSize;
+    b->groupSize   = G > 0 ? G : b->sys[0]->lay.groupSize;
     buildOccurrences(b->stream, b->gather, b->ngroups, b->groups.p, b->groupInfo.p, b->unionList.p, totClusters);
     scheduleBatch(b, blocksPerCU, chunkWeight);
     b->built      = true;

@@ -116,16 +116,20 @@ inline void buildOccurrences(hipStream_t s, WindowGather& w, int ngroups, const 
 }
 
 // Staging layout of a schedule (whenever the schedule changes).
+// slotF4: float4 per window cluster slot (3: forces, 4: the adjoint);
+// occurrences come from occ (the same groups' lists; w itself by default)
 inline void buildSegments(hipStream_t s, WindowGather& w, int ngroups, const int4* groups, const int4* groupInfo,
-                          int numWaves, const int2* waveStart, int ncluster)
+                          int numWaves, const int2* waveStart, int ncluster, int slotF4 = 3,
+                          const WindowGather* occ = nullptr)
 {
+    const WindowGather& o = occ ? *occ : w;
     w.segSize.ensure(ngroups + 1);
     w.segBase.ensure(ngroups + 1);
     w.segInfo.ensure(std::max(ngroups, 1));
     int total = 0;
     if (ngroups > 0)
     {
-        launchSegments(s, ngroups, groups, groupInfo, numWaves, waveStart, w.segSize.p, w.segBase.p,
+        launchSegments(s, ngroups, groups, groupInfo, numWaves, waveStart, slotF4, w.segSize.p, w.segBase.p,
                        w.segBase.p + ngroups, w.segInfo.p);
         check(hipMemcpyAsync(&total, w.segBase.p + ngroups, sizeof(int), hipMemcpyDeviceToHost, s), "read segments");
         check(hipStreamSynchronize(s), "segment sync");
@@ -137,7 +141,8 @@ inline void buildSegments(hipStream_t s, WindowGather& w, int ngroups, const int
     int npos = 0;
     if (ncluster > 0)
     {
-        launchGatherList(s, ncluster, w.occOff.p, w.occ.p, w.segInfo.p, w.gposCount.p, nullptr, nullptr, false);
+        launchGatherList(s, ncluster, o.occOff.p, o.occ.p, w.segInfo.p, slotF4, w.gposCount.p, nullptr, nullptr,
+                         false);
         launchExclusiveScan(s, ncluster, w.gposCount.p, w.gposOff.p, w.gposOff.p + ncluster);
         check(hipMemcpyAsync(&npos, w.gposOff.p + ncluster, sizeof(int), hipMemcpyDeviceToHost, s), "read gather");
         check(hipStreamSynchronize(s), "gather sync");
@@ -147,9 +152,24 @@ inline void buildSegments(hipStream_t s, WindowGather& w, int ngroups, const int
         check(hipMemsetAsync(w.gposOff.p, 0, sizeof(int), s), "clear gather");
     }
     w.gpos.ensure(std::max(npos, 1));
-    launchGatherList(s, ncluster, w.occOff.p, w.occ.p, w.segInfo.p, w.gposCount.p, w.gposOff.p, w.gpos.p, true);
+    launchGatherList(s, ncluster, o.occOff.p, o.occ.p, w.segInfo.p, slotF4, w.gposCount.p, w.gposOff.p, w.gpos.p,
+                     true);
     check(hipGetLastError(), "segment kernels");
 }
+
+// Work groups, chunks, work list, static schedule and window staging of one
+// kernel over the pair list (groups.hip).
+struct WorkLayout
+{
+    int            groupSize = 0, ngroups = 0, nchunk = 0, windowSlots = 0, nwork = 0;
+    int            numBlocks = 0; // workgroups of kKernelWaves wavefronts (the schedule's wavefronts / 4)
+    DevBuf<int>    unionCount, chunkCount, workCount, overflow, unionOff, chunkOff, workOff, unionList;
+    DevBuf<int4>   chunks, groups, groupInfo;
+    DevBuf<float4> chunkShift;
+    DevBuf<uint2>  work;
+    DevBuf<int2>   waveStart; // static schedule: each wavefront's first chunk and its group
+    WindowGather   gather;
+};
 
 inline int fail(int code, const std::string& msg)
 {
@@ -230,18 +250,16 @@ struct nbx_context
     nbx::DevBuf<int>    countJ, countI, offJ, offI, totals;
     nbx::DevBuf<int4>   ientry;
     nbx::DevBuf<uint2>  jentry, jentryTmp;
-    // kernel work groups
-    int            groupSize = 0, ngroups = 0, nchunk = 0, windowSlots = 0, nwork = 0;
-    nbx::DevBuf<int>    unionCount, chunkCount, workCount, overflow, unionOff, chunkOff, workOff, unionList, groupStats;
-    nbx::DevBuf<int4>   chunks, groups, groupInfo;
-    nbx::DevBuf<float4> chunkShift;
-    nbx::DevBuf<uint2>  work;
+    // kernel work layouts: the pair kernel's, and the adjoint's (its own
+    // group size, sized for its larger window per cluster slot; built on
+    // the first adjoint call after a search)
+    nbx::DevBuf<int>    groupStats;
+    nbx::WorkLayout     lay;
+    nbx::WorkLayout     adj;
+    bool                adjValid = false;
 
     // outputs
-    int             numBlocks = 0;
     nbx::DevBuf<double>  energyPartial, energy, fshift, virialPartial, virial;
-    nbx::DevBuf<int2>    waveStart; // static schedule: each wavefront's first chunk and its group
-    nbx::WindowGather    gather;
     nbx::DevBuf<double>  exclPartial;
     nbx::DevBuf<float>   fshiftPartial, fOut;
     // adjoint

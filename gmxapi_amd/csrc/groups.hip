@@ -436,7 +436,7 @@ __global__ __launch_bounds__(256) void occurrenceSortKernel(const int* __restric
 // in occurrence order.
 template<bool kFill>
 __global__ void gatherListKernel(int ncluster, const int* __restrict__ occOff, const int2* __restrict__ occ,
-                                 const int4* __restrict__ segInfo, int* __restrict__ count,
+                                 const int4* __restrict__ segInfo, int slotF4, int* __restrict__ count,
                                  const int* __restrict__ gposOff, int* __restrict__ gpos)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -453,7 +453,7 @@ __global__ void gatherListKernel(int ncluster, const int* __restrict__ occOff, c
         {
             if (kFill)
             {
-                gpos[n] = si.x + w * si.w + e.y * 3;
+                gpos[n] = si.x + w * si.w + e.y * slotF4;
             }
             n++;
         }
@@ -484,9 +484,10 @@ __device__ int waveOfChunk(int numWaves, const int2* __restrict__ waveStart, int
 }
 
 // Per group: the wavefronts that run its chunks (first, count) and its
-// window size in float4 (3 per cluster slot); size = their product.
+// window size in float4 (slotF4 per cluster slot: 3 for the forces, 4 for
+// the adjoint's {dL/dx, dL/dq}); size = their product.
 __global__ void segmentKernel(int ngroups, const int4* __restrict__ groups, const int4* __restrict__ groupInfo,
-                              int numWaves, const int2* __restrict__ waveStart, int* __restrict__ size,
+                              int numWaves, const int2* __restrict__ waveStart, int slotF4, int* __restrict__ size,
                               int4* __restrict__ segInfo)
 {
     const int h = blockIdx.x * blockDim.x + threadIdx.x;
@@ -495,7 +496,7 @@ __global__ void segmentKernel(int ngroups, const int4* __restrict__ groups, cons
         return;
     }
     const int4 grp = groups[h];
-    const int  win = ((groupInfo[h].y >> 16) + grp.w) * 3;
+    const int  win = ((groupInfo[h].y >> 16) + grp.w) * slotF4;
     if (grp.x >= grp.y)
     {
         size[h]    = 0;
@@ -663,19 +664,20 @@ void launchOccurrences(hipStream_t s, int ngroups, const int4* groups, const int
 }
 
 void launchSegments(hipStream_t s, int ngroups, const int4* groups, const int4* groupInfo, int numWaves,
-                    const int2* waveStart, int* size, int* base, int* total, int4* segInfo)
+                    const int2* waveStart, int slotF4, int* size, int* base, int* total, int4* segInfo)
 {
     if (ngroups == 0)
     {
         return;
     }
-    segmentKernel<<<(ngroups + 255) / 256, 256, 0, s>>>(ngroups, groups, groupInfo, numWaves, waveStart, size, segInfo);
+    segmentKernel<<<(ngroups + 255) / 256, 256, 0, s>>>(ngroups, groups, groupInfo, numWaves, waveStart, slotF4, size,
+                                                        segInfo);
     launchExclusiveScan(s, ngroups, size, base, total);
     segmentBaseKernel<<<(ngroups + 255) / 256, 256, 0, s>>>(ngroups, base, segInfo);
 }
 
 void launchGatherList(hipStream_t s, int ncluster, const int* occOff, const int2* occ, const int4* segInfo,
-                      int* count, const int* gposOff, int* gpos, bool fill)
+                      int slotF4, int* count, const int* gposOff, int* gpos, bool fill)
 {
     if (ncluster == 0)
     {
@@ -683,13 +685,13 @@ void launchGatherList(hipStream_t s, int ncluster, const int* occOff, const int2
     }
     if (fill)
     {
-        gatherListKernel<true><<<(ncluster + 255) / 256, 256, 0, s>>>(ncluster, occOff, occ, segInfo, count, gposOff,
-                                                                      gpos);
+        gatherListKernel<true><<<(ncluster + 255) / 256, 256, 0, s>>>(ncluster, occOff, occ, segInfo, slotF4, count,
+                                                                      gposOff, gpos);
     }
     else
     {
-        gatherListKernel<false><<<(ncluster + 255) / 256, 256, 0, s>>>(ncluster, occOff, occ, segInfo, count, gposOff,
-                                                                       gpos);
+        gatherListKernel<false><<<(ncluster + 255) / 256, 256, 0, s>>>(ncluster, occOff, occ, segInfo, slotF4, count,
+                                                                       gposOff, gpos);
     }
 }
 

@@ -52,6 +52,9 @@ constexpr int kChunkWeight = 8;
 constexpr int kKernelWaves      = 4;
 constexpr int kKernelLdsBudget  = 80 * 1024;
 constexpr int kAdjointLdsBudget = 160 * 1024;
+// LDS per two-wavefront adjoint workgroup that its group size is chosen for
+// (three resident workgroups per CU)
+constexpr int kAdjointLayoutBudget = 48 * 1024;
 // rsq floor for float (pairlist.h:146): r > 6.2e-4 nm
 constexpr float kMinRsq = 3.82e-07f;
 
@@ -166,15 +169,16 @@ void launchOccurrences(hipStream_t s, int ngroups, const int4* groups, const int
                        int ncluster, int* count, int* occOff, int2* occ, bool fill);
 // The pair kernel's window staging layout for a schedule: per group
 // segInfo = {first float4 of its segments, first wavefront, wavefronts,
-// window float4}; wavefront w of the group writes its window at
-// x + (w - y) * w_size.  total receives the staging size in float4.
+// window float4 (slotF4 per cluster slot)}; wavefront w of the group writes
+// its window at x + (w - y) * w_size.  total receives the staging size in
+// float4.
 void launchSegments(hipStream_t s, int ngroups, const int4* groups, const int4* groupInfo, int numWaves,
-                    const int2* waveStart, int* size, int* base, int* total, int4* segInfo);
+                    const int2* waveStart, int slotF4, int* size, int* base, int* total, int4* segInfo);
 // Per cluster, the staging positions of all its window copies (CSR:
 // gposOff[ncluster + 1], gpos): count pass (fill = false) into count, scan,
 // then fill.
 void launchGatherList(hipStream_t s, int ncluster, const int* occOff, const int2* occ, const int4* segInfo,
-                      int* count, const int* gposOff, int* gpos, bool fill);
+                      int slotF4, int* count, const int* gposOff, int* gpos, bool fill);
 // Per chunk, the shift vector of its i-entry (shiftVec[chunk shift index]),
 // so that the pair kernel loads it together with the chunk descriptor.
 void launchChunkShifts(hipStream_t s, int nchunk, const int4* chunks, const float4* shiftVec, float4* out);
@@ -241,8 +245,12 @@ struct AdjointArgs
     const float4* chunkShift;
     InteractionConst ic;
     const int2* waveStart; // the forward kernel's static schedule
-    float4*     gradX;     // dL/dx per cluster-layout atom, zeroed by launchAdjoint
-    double*     gradQ;     // dL/dq per cluster-layout atom, zeroed by launchAdjoint
+    const int4* segInfo;   // the adjoint's window staging layout (4 float4 per cluster slot)
+    float4*     staging;
+    const int*  gposOff;   // per cluster: the staging positions of its window copies
+    const int*  gpos;
+    float4*     gradX;     // dL/dx per cluster-layout atom, zeroed by launchAdjointPrepare
+    double*     gradQ;     // dL/dq per cluster-layout atom, zeroed by launchAdjointPrepare
     double*     paramPartial; // [adjoint workgroups][ntype^2][2]
 };
 // the adjoint's type-pair accumulators live in LDS: at most this many types
@@ -250,14 +258,16 @@ struct AdjointArgs
 constexpr int kAdjointMaxTypes = 32;
 int  adjointLdsBytes(int ntype, int tabN, int elec, int windowSlots);
 int  adjointBlocks(int forwardBlocks);
-// launches the adjoint kernel on the forward kernel's schedule, then writes
-// dL/dx (natoms x 3 floats) and dL/dq (natoms doubles) in atom order (either
-// may be NULL) and dL/dC6, dL/dC12 as numTypes x numTypes matrices (the
-// padding type dropped; either may be NULL, both NULL skip the C6/C12 sums)
-void launchAdjoint(hipStream_t s, const AdjointArgs& a, int elec, int forwardBlocks, int natoms, const int* cells,
+// launches the adjoint kernel on its layout's schedule, then writes dL/dx
+// (natoms x 3 floats) and dL/dq (natoms doubles) in atom order (either may
+// be NULL) and dL/dC6, dL/dC12 as numTypes x numTypes matrices (the padding
+// type dropped; either may be NULL, both NULL skip the C6/C12 sums)
+void launchAdjoint(hipStream_t s, const AdjointArgs& a, int elec, int forwardBlocks, const int* atomIndices,
                    float* gradXOut, double* gradQOut, int numTypes, double* gradC6Out, double* gradC12Out);
-// g into the cluster layout: gq must hold nslot + 4 entries (the padding cluster)
-void launchConvertG(hipStream_t s, int nslot, const int* atomIndices, const float* g, float4* gq);
+// coordinates (x may be NULL) and g into the cluster layout, spill
+// accumulators cleared: xq, gq hold nslot + 4 entries (the padding cluster)
+void launchAdjointPrepare(hipStream_t s, int nslot, const int* atomIndices, const float* x, const float* g, float4* xq,
+                          float4* gq, float4* gradX, double* gradQ);
 
 // -- buffers.hip
 // zeroes n float4 and, when d is not NULL, nd doubles
