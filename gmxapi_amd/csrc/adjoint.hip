@@ -176,34 +176,6 @@ __device__ __forceinline__ void adjointTile(const IAtoms& I, const IGrad& Gi, co
     adjointPairI<kElec, kParams, kChecks, 3>(I, Gi, J, gj, mbits, diag, jj, ic, sNbfp, sTab, sAcc, ai, aj);
 }
 
-// Reduce-scatter of 16 components (4 i-atoms x {x, y, z, q}) over the
-// wavefront: afterwards lane L holds the wave sum of component L >> 2.
-__device__ __forceinline__ float waveReduceScatter16(const float* f, int lane)
-{
-    const bool  h32 = (lane & 32) != 0;
-    const bool  h16 = (lane & 16) != 0;
-    const bool  h8  = (lane & 8) != 0;
-    const bool  h4  = (lane & 4) != 0;
-    const float a0  = butterfly(f[0], f[8], h32, 32);
-    const float a1  = butterfly(f[1], f[9], h32, 32);
-    const float a2  = butterfly(f[2], f[10], h32, 32);
-    const float a3  = butterfly(f[3], f[11], h32, 32);
-    const float a4  = butterfly(f[4], f[12], h32, 32);
-    const float a5  = butterfly(f[5], f[13], h32, 32);
-    const float a6  = butterfly(f[6], f[14], h32, 32);
-    const float a7  = butterfly(f[7], f[15], h32, 32);
-    const float b0  = butterfly(a0, a4, h16, 16);
-    const float b1  = butterfly(a1, a5, h16, 16);
-    const float b2  = butterfly(a2, a6, h16, 16);
-    const float b3  = butterfly(a3, a7, h16, 16);
-    const float c0  = butterfly(b0, b2, h8, 8);
-    const float c1  = butterfly(b1, b3, h8, 8);
-    float       r   = butterfly(c0, c1, h4, 4);
-    r += __shfl_xor(r, 2, kWave);
-    r += __shfl_xor(r, 1, kWave);
-    return r;
-}
-
 template<int kElec, bool kParams>
 __global__ __launch_bounds__(kAdjThreads) void adjointKernel(const AdjointArgs a)
 {

@@ -514,9 +514,15 @@ int nbx_create(const nbx_options* options, nbx_context** context)
             ic.sh_ewald  = o.ewald_potential_shift ? std::erfc(ic.beta * c->rcoulomb) / c->rcoulomb : 0.0f;
             ic.tab_scale = ewaldTableScale(ic.beta, c->rcoulomb);
             ic.tab_n     = static_cast<int>(c->rcoulomb * ic.tab_scale) + 2;
+            ic.tab_rs_max = std::nextafter(static_cast<float>(ic.tab_n - 1), 0.0f);
             std::vector<float>  fdv0  = ewaldTableFDV0<float>(ic.tab_n, ic.tab_scale, ic.beta);
             std::vector<double> fdv0d = ewaldTableFDV0<double>(ic.tab_n, ic.tab_scale, ic.beta);
             ic.self_v0   = fdv0[2];
+            // the pair kernel reads V + sh_ewald from the unused fourth entry
+            for (int i = 0; i < ic.tab_n; i++)
+            {
+                fdv0[4 * i + 3] = static_cast<float>(static_cast<double>(fdv0[4 * i + 2]) + ic.sh_ewald);
+            }
             // the self energy sums one large term per atom against a pair sum
             // of nearly the same size: V(0) = 2 beta / sqrt(pi) in double (of
             // the float beta every pair term uses) keeps its rounding out
@@ -532,6 +538,7 @@ int nbx_create(const nbx_options* options, nbx_context** context)
             c->elec      = kElecRF;
             ic.tab_scale = 1.0f;
             ic.tab_n     = 0;
+            ic.tab_rs_max = 0.0f;
             ic.self_v0   = ic.c_rf;
             c->selfCoef  = 0.5 * static_cast<double>(ic.epsfac) * static_cast<double>(ic.c_rf);
         }

@@ -69,7 +69,10 @@ constexpr int kWavesBlock = kKernelWaves;
 constexpr int kThreads    = kWave * kWavesBlock;
 
 // One j-atom (this lane) against i-atom kI.  kChecks adds the exclusion
-// bit and the central diagonal.
+// bit and the central diagonal.  vlj accumulates 12 times the LJ energy
+// (FrLJ12 - 2 FrLJ6 plus the shift: one rounding fewer than dividing each
+// pair by 12 and 6, and no bias from a rounded 1/12); the wavefront's double
+// sum is divided by 12 once per group.
 template<int kElec, bool kEnergy, bool kChecks, int kI>
 __device__ __forceinline__ void pairI(const IAtoms& I, const JAtom& J, const unsigned mbits, const bool diag,
                                       const int jj, const InteractionConst& ic, const float2* __restrict__ sNbfp,
@@ -96,64 +99,67 @@ __device__ __forceinline__ void pairI(const IAtoms& I, const JAtom& J, const uns
     if (kEnergy)
     {
         // one Newton-Raphson step: removes the systematic bias of the
-        // hardware estimate, which r^-12 would amplify in the energy sum
+        // hardware estimate, which r^-12 (and, through r, the tabulated
+        // potential) would carry into the energy sums
         const float hx = 0.5f * rsq;
         rinv           = rinv * fmaf(-hx * rinv, rinv, 1.5f);
     }
+    // r for the table position
+    const float r = rsq * rinv;
     rinv               = skip ? 0.0f : rinv;
     const float rinvsq = rinv * rinv;
+    // energy mask: 0 beyond the cut-off, on the diagonal and (with checks)
+    // for excluded pairs, whose energies the exclusion pass adds
+    const float emask = skip ? 0.0f : interact;
 
     const float2 c      = sNbfp[I.tOff[kI] + J.type];
     const float  rinv6  = kChecks ? interact * rinvsq * rinvsq * rinvsq : rinvsq * rinvsq * rinvsq;
     const float  FrLJ6  = c.x * rinv6;
-    const float  FrLJ12 = c.y * rinv6 * rinv6;
-    const float  frLJ   = FrLJ12 - FrLJ6;
-    const float  smask  = skip ? 0.0f : 1.0f;
+    const float  frLJ   = fmaf(c.y * rinv6, rinv6, -FrLJ6);
     if (kEnergy)
     {
-        // (FrLJ12 + c12 cpot_rep)/12 - (FrLJ6 + c6 cpot_disp)/6, each quotient
-        // as x*h + x*l (h + l = 1/12 or 1/6 to 2^-48): rounded once, unbiased
-        const float v12 = fmaf(FrLJ12, kTwelfthHi, FrLJ12 * kTwelfthLo) + c.y * ic.cpot_rep12;
-        const float v6  = fmaf(FrLJ6, kSixthHi, FrLJ6 * kSixthLo) + c.x * ic.cpot_disp6;
-        vlj += kChecks ? (v12 - v6) * interact * smask : (v12 - v6) * smask;
+        // 12 V = FrLJ12 - 2 FrLJ6 + c12 cpot_rep - 2 c6 cpot_disp
+        const float v12 = fmaf(c.x, -2.0f * ic.cpot_disp, fmaf(c.y, ic.cpot_rep, frLJ - FrLJ6));
+        vlj             = fmaf(v12, emask, vlj);
     }
 
-    // beyond the cut-off rinv = 0 zeroes the table's Ewald force; energies,
-    // the reaction-field constant and the analytical correction need qq = 0
-    // there as well
-    const bool  maskQ = kEnergy || kElec != kElecEwaldTable;
-    const float qq    = maskQ ? smask * I.q[kI] * J.w : I.q[kI] * J.w;
-    float       fcoul;
-    float       vpair = 0.0f; // the pair's Coulomb energy over qq (Ewald flavours)
+    float fcoul;
     if (kElec == kElecRF)
     {
-        fcoul = qq * (interact * rinv * rinvsq - 2.0f * ic.k_rf);
+        // the reaction-field constant needs qq = 0 beyond the cut-off
+        const float qq = (skip ? 0.0f : I.q[kI]) * J.w;
+        fcoul          = qq * (interact * rinv * rinvsq - 2.0f * ic.k_rf);
         if (kEnergy)
         {
-            // excluded pairs (interact = 0) are left to the exclusion pass
-            vc += interact * qq * (rinv + ic.k_rf * rsq - ic.c_rf);
+            vc = fmaf(interact * qq, rinv + ic.k_rf * rsq - ic.c_rf, vc);
         }
     }
     else if (kElec == kElecEwaldTable)
     {
-        const float  rs    = rsq * rinv * ic.tab_scale;
-        // clamped so that no input (NaN, Inf) can index past the table
-        const int    ri    = min(static_cast<int>(rs), ic.tab_n - 2);
-        const float  frac  = rs - static_cast<float>(ri);
+        // beyond the cut-off rinv = 0 zeroes the force; the index is clamped
+        // so that no input (NaN, Inf, the far-away padding cluster) can read
+        // past the table
+        const float  rs    = fminf(r * ic.tab_scale, ic.tab_rs_max);
+        const int    ri    = static_cast<int>(rs);
+        const float  frac  = __builtin_amdgcn_fractf(rs);
         const float4 tab   = sTab[ri];
-        const float  fexcl = tab.x + frac * tab.y;
-        fcoul              = (interact * rinvsq - fexcl) * (qq * rinv);
+        const float  fexcl = fmaf(frac, tab.y, tab.x);
+        const float  qq    = I.q[kI] * J.w;
+        fcoul              = (kChecks ? fmaf(interact, rinvsq, -fexcl) : rinvsq - fexcl) * (qq * rinv);
         if (kEnergy)
         {
-            const float halfsp = 0.5f / ic.tab_scale;
-            vpair              = (rinv - ic.sh_ewald) - (tab.z - halfsp * frac * (tab.x + fexcl));
+            // rinv - sh_ewald - V_tab, V_tab = V - (h/2) frac (F + fexcl);
+            // tab.w holds V + sh_ewald
+            const float vpair = fmaf((0.5f / ic.tab_scale) * frac, tab.x + fexcl, rinv - tab.w);
+            vc                = fmaf(qq * emask, vpair, vc);
         }
     }
     else
     {
-        // clamped to the fitted range: pairs beyond the cut-off (qq = 0) may
-        // be far away (the padding cluster), where s^5 would overflow to inf
-        // and inf / inf is NaN
+        // qq = 0 beyond the cut-off; the correction is clamped to the fitted
+        // range: pairs beyond the cut-off may be far away (the padding
+        // cluster), where s^5 would overflow to inf and inf / inf is NaN
+        const float qq = (skip ? 0.0f : I.q[kI]) * J.w;
         const float s  = fminf(rsq * ic.beta2, kAnaMaxS);
         const float pg = fmaf(fmaf(fmaf(fmaf(fmaf(kGp5, s, kGp4), s, kGp3), s, kGp2), s, kGp1), s, kGp0);
         const float qg = fmaf(fmaf(fmaf(fmaf(fmaf(kGq5, s, kGq4), s, kGq3), s, kGq2), s, kGq1), s, 1.0f);
@@ -166,25 +172,19 @@ __device__ __forceinline__ void pairI(const IAtoms& I, const JAtom& J, const uns
             const float z  = ic.beta * (rsq * rinv);
             const float pe = fmaf(fmaf(fmaf(fmaf(kEp4, z, kEp3), z, kEp2), z, kEp1), z, kEp0);
             const float qe = fmaf(fmaf(fmaf(fmaf(kEq4, z, kEq3), z, kEq2), z, kEq1), z, 1.0f);
-            vpair = rinv * __builtin_amdgcn_exp2f(-s * kLog2e) * pe * __builtin_amdgcn_rcpf(qe) - ic.sh_ewald;
+            const float vpair =
+                    rinv * __builtin_amdgcn_exp2f(-s * kLog2e) * pe * __builtin_amdgcn_rcpf(qe) - ic.sh_ewald;
+            vc = fmaf(kChecks ? interact * qq : qq, vpair, vc);
         }
     }
-    if (kEnergy && kElec != kElecRF)
-    {
-        // excluded pairs (interact = 0) are left to the exclusion pass
-        vc += kChecks ? interact * qq * vpair : qq * vpair;
-    }
 
-    const float fscal = frLJ * rinvsq + fcoul;
-    const float fx    = fscal * dx;
-    const float fy    = fscal * dy;
-    const float fz    = fscal * dz;
-    fi[3 * kI + 0] += fx;
-    fi[3 * kI + 1] += fy;
-    fi[3 * kI + 2] += fz;
-    fjx -= fx;
-    fjy -= fy;
-    fjz -= fz;
+    const float fscal = fmaf(frLJ, rinvsq, fcoul);
+    fi[3 * kI + 0]    = fmaf(fscal, dx, fi[3 * kI + 0]);
+    fi[3 * kI + 1]    = fmaf(fscal, dy, fi[3 * kI + 1]);
+    fi[3 * kI + 2]    = fmaf(fscal, dz, fi[3 * kI + 2]);
+    fjx               = fmaf(-fscal, dx, fjx);
+    fjy               = fmaf(-fscal, dy, fjy);
+    fjz               = fmaf(-fscal, dz, fjz);
 }
 
 template<int kElec, bool kEnergy, bool kChecks>
@@ -311,7 +311,7 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
         auto endGroup = [&]() {
             if (kEnergy)
             {
-                const double v = waveSumD(eVdw);
+                const double v = waveSumD(eVdw) * (1.0 / 12.0);
                 const double c = waveSumD(eCoul);
                 if (lane == 0)
                 {

@@ -72,45 +72,92 @@ struct JAtom
     int   type;
 };
 
-// One butterfly step: lanes with the mask bit set keep b and send a, the
-// others keep a and send b; each adds what its partner sent.
-__device__ __forceinline__ float butterfly(float a, float b, bool hi, int m)
+// Butterfly steps of a reduce-scatter: lanes with the step's lane bit set
+// keep b, the others keep a, and each adds its partner's copy of the value it
+// keeps.  Lane bits 5 and 4 use CDNA4's lane-swap instructions: swapping the
+// upper half of a with the lower half of b (bit 5), or the odd 16-lane rows
+// of a with the even rows of b (bit 4), leaves every lane with its kept value
+// in one register and the partner's in the other, so the step is the swap and
+// one add.  Bits 3 and 2 pair lane l with its mirror image in its row of 16
+// (l ^ 15) or half-row of 8 (l ^ 7), bits 1 and 0 with l ^ 2 and l ^ 1 (DPP
+// row_mirror, row_half_mirror and quad permutations): no LDS traffic.
+__device__ __forceinline__ float swapStep32(float a, float b)
+{
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
+__device__ __forceinline__ float swapStep16(float a, float b)
+{
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
+constexpr int kDppRowMirror     = 0x140;
+constexpr int kDppRowHalfMirror = 0x141;
+constexpr int kDppQuadXor2      = 0x4e; // quad_perm [2, 3, 0, 1]
+constexpr int kDppQuadXor1      = 0xb1; // quad_perm [1, 0, 3, 2]
+
+template<int kCtrl>
+__device__ __forceinline__ float dppMove(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kCtrl, 0xf, 0xf, false));
+}
+
+template<int kCtrl>
+__device__ __forceinline__ float dppStep(float a, float b, bool hi)
 {
     const float keep = hi ? b : a;
     const float send = hi ? a : b;
-    return keep + __shfl_xor(send, m, kWave);
+    return keep + dppMove<kCtrl>(send);
+}
+
+// The last steps of a reduce-scatter over 16 lane groups of four: c0..c3
+// hold the partial sums of four components after the steps on lane bits 5
+// and 4; afterwards lane L holds the wave sum of component (L >> 2) & 3 of
+// its row pair.
+__device__ __forceinline__ float quadTail(float c0, float c1, float c2, float c3, int lane)
+{
+    const float d0 = dppStep<kDppRowMirror>(c0, c2, (lane & 8) != 0);
+    const float d1 = dppStep<kDppRowMirror>(c1, c3, (lane & 8) != 0);
+    float       r  = dppStep<kDppRowHalfMirror>(d0, d1, (lane & 4) != 0);
+    r += dppMove<kDppQuadXor2>(r);
+    r += dppMove<kDppQuadXor1>(r);
+    return r;
 }
 
 // Reduce-scatter of the 12 i-force components over the wavefront: afterwards
 // lane L holds the wave sum of component L >> 2 (lanes 48..63 hold zeros).
-// Written on named scalars: with an indexed array the compiler folds the
-// lane-dependent selects into a dynamically indexed register array and
-// expands that into long compare/select chains.
+// The whole wavefront must be active.  Written on named scalars: with an
+// indexed array the compiler folds the lane-dependent selects into a
+// dynamically indexed register array and expands that into long
+// compare/select chains.
 __device__ __forceinline__ float waveReduceScatter12(const float* f, int lane)
 {
-    const bool  h32 = (lane & 32) != 0;
-    const bool  h16 = (lane & 16) != 0;
-    const bool  h8  = (lane & 8) != 0;
-    const bool  h4  = (lane & 4) != 0;
     // component k + 8 for k >= 4 is zero
-    const float a0  = butterfly(f[0], f[8], h32, 32);
-    const float a1  = butterfly(f[1], f[9], h32, 32);
-    const float a2  = butterfly(f[2], f[10], h32, 32);
-    const float a3  = butterfly(f[3], f[11], h32, 32);
-    const float a4  = butterfly(f[4], 0.0f, h32, 32);
-    const float a5  = butterfly(f[5], 0.0f, h32, 32);
-    const float a6  = butterfly(f[6], 0.0f, h32, 32);
-    const float a7  = butterfly(f[7], 0.0f, h32, 32);
-    const float b0  = butterfly(a0, a4, h16, 16);
-    const float b1  = butterfly(a1, a5, h16, 16);
-    const float b2  = butterfly(a2, a6, h16, 16);
-    const float b3  = butterfly(a3, a7, h16, 16);
-    const float c0  = butterfly(b0, b2, h8, 8);
-    const float c1  = butterfly(b1, b3, h8, 8);
-    float       r   = butterfly(c0, c1, h4, 4);
-    r += __shfl_xor(r, 2, kWave);
-    r += __shfl_xor(r, 1, kWave);
-    return r;
+    const float a0 = swapStep32(f[0], f[8]);
+    const float a1 = swapStep32(f[1], f[9]);
+    const float a2 = swapStep32(f[2], f[10]);
+    const float a3 = swapStep32(f[3], f[11]);
+    const float a4 = swapStep32(f[4], 0.0f);
+    const float a5 = swapStep32(f[5], 0.0f);
+    const float a6 = swapStep32(f[6], 0.0f);
+    const float a7 = swapStep32(f[7], 0.0f);
+    return quadTail(swapStep16(a0, a4), swapStep16(a1, a5), swapStep16(a2, a6), swapStep16(a3, a7), lane);
+}
+
+// The same for 16 components (4 i-atoms x {x, y, z, q}).
+__device__ __forceinline__ float waveReduceScatter16(const float* f, int lane)
+{
+    const float a0 = swapStep32(f[0], f[8]);
+    const float a1 = swapStep32(f[1], f[9]);
+    const float a2 = swapStep32(f[2], f[10]);
+    const float a3 = swapStep32(f[3], f[11]);
+    const float a4 = swapStep32(f[4], f[12]);
+    const float a5 = swapStep32(f[5], f[13]);
+    const float a6 = swapStep32(f[6], f[14]);
+    const float a7 = swapStep32(f[7], f[15]);
+    return quadTail(swapStep16(a0, a4), swapStep16(a1, a5), swapStep16(a2, a6), swapStep16(a3, a7), lane);
 }
 
 __device__ __forceinline__ double waveSumD(double v)

@@ -81,6 +81,7 @@ struct InteractionConst
     float cpot_rep12; // cpot_rep / 12, rounded once
     float tab_scale;
     int   tab_n;
+    float tab_rs_max; // largest float below tab_n - 1: clamps the scaled distance to the last interval
     float self_v0; // V(0) of the Ewald correction (or c_rf for RF)
     float beta2;   // beta^2 and beta^3 for the analytical correction
     float beta3;
