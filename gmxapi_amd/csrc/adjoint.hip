@@ -538,11 +538,7 @@ __global__ void adjointPrepareKernel(int nslot, const int* __restrict__ atomIndi
 template<int kElec, bool kParams>
 void launchOne(hipStream_t s, const AdjointArgs& a, int numBlocks, int lds)
 {
-    if (lds > 64 * 1024)
-    {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&adjointKernel<kElec, kParams>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-    }
+    (void)allowDynamicLds(reinterpret_cast<const void*>(&adjointKernel<kElec, kParams>), lds);
     adjointKernel<kElec, kParams><<<numBlocks, kAdjThreads, lds, s>>>(a);
 }
 

@@ -226,10 +226,13 @@ void buildWorkGroups(nbx_context* c, WorkLayout& L, bool adjoint)
                        : nonbondedLdsBytes(c->ntype, c->ic.tab_n, c->elec, slots);
     };
     const int nc = c->ncluster;
-    c->groupStats.ensure(9);
-    check(hipMemsetAsync(c->groupStats.p, 0, sizeof(int) * 9, c->stream), "clear stats");
+    // [0, kGroupSizeCount): largest j-entry span per candidate size, then
+    // max union, overflow flag and the three scan totals of one pass
+    constexpr int kStats = kGroupSizeCount + 5;
+    c->groupStats.ensure(kStats);
+    check(hipMemsetAsync(c->groupStats.p, 0, sizeof(int) * kStats, c->stream), "clear stats");
     launchGroupSpans(c->stream, nc, c->offJ.p, c->groupStats.p);
-    int spans[4] = { 0, 0, 0, 0 };
+    int spans[kGroupSizeCount] = {};
     check(hipMemcpyAsync(spans, c->groupStats.p, sizeof(spans), hipMemcpyDeviceToHost, c->stream), "read spans");
     check(hipStreamSynchronize(c->stream), "group sync");
     c->jentryTmp.ensure(c->ncj);
@@ -240,11 +243,15 @@ void buildWorkGroups(nbx_context* c, WorkLayout& L, bool adjoint)
     // global atomics (long cut-offs, many types), the last pass takes the
     // smallest (or the chosen) size with whatever spill its window leaves.
     const int forced = adjoint ? 0 : c->opt.kernel_group_size;
-    for (int k = 0; k < 5; k++)
+    for (int k = 0; k <= kGroupSizeCount; k++)
     {
-        const bool fallback = (k == 4);
+        const bool fallback = (k == kGroupSizeCount);
         const int  G        = fallback ? (forced > 0 ? forced : 1) : kGroupSizeMax >> k;
-        const int  si       = G == 8 ? 0 : G == 4 ? 1 : G == 2 ? 2 : 3; // spans[] index of G
+        int        si       = 0; // spans[] index of G
+        while ((kGroupSizeMax >> si) > G)
+        {
+            si++;
+        }
         const int  budget   = adjoint ? kAdjointLayoutBudget : kKernelLdsBudget;
         if (!fallback && forced > 0 && G != forced)
         {
@@ -282,14 +289,14 @@ void buildWorkGroups(nbx_context* c, WorkLayout& L, bool adjoint)
         ga.workCount    = L.workCount.p;
         ga.overflow     = L.overflow.p;
         launchBuildGroups(c->stream, ga, false);
-        check(hipMemsetAsync(c->groupStats.p + 4, 0, sizeof(int) * 2, c->stream), "clear stats");
-        launchMaxReduce(c->stream, ng, L.unionCount.p, c->groupStats.p + 4);
-        launchMaxReduce(c->stream, ng, L.overflow.p, c->groupStats.p + 5);
-        launchExclusiveScan(c->stream, ng, L.unionCount.p, L.unionOff.p, c->groupStats.p + 6);
-        launchExclusiveScan(c->stream, ng, L.chunkCount.p, L.chunkOff.p, c->groupStats.p + 7);
-        launchExclusiveScan(c->stream, ng, L.workCount.p, L.workOff.p, c->groupStats.p + 8);
+        check(hipMemsetAsync(c->groupStats.p + kGroupSizeCount, 0, sizeof(int) * 2, c->stream), "clear stats");
+        launchMaxReduce(c->stream, ng, L.unionCount.p, c->groupStats.p + kGroupSizeCount);
+        launchMaxReduce(c->stream, ng, L.overflow.p, c->groupStats.p + kGroupSizeCount + 1);
+        launchExclusiveScan(c->stream, ng, L.unionCount.p, L.unionOff.p, c->groupStats.p + kGroupSizeCount + 2);
+        launchExclusiveScan(c->stream, ng, L.chunkCount.p, L.chunkOff.p, c->groupStats.p + kGroupSizeCount + 3);
+        launchExclusiveScan(c->stream, ng, L.workCount.p, L.workOff.p, c->groupStats.p + kGroupSizeCount + 4);
         int st[5];
-        check(hipMemcpyAsync(st, c->groupStats.p + 4, sizeof(st), hipMemcpyDeviceToHost, c->stream), "read stats");
+        check(hipMemcpyAsync(st, c->groupStats.p + kGroupSizeCount, sizeof(st), hipMemcpyDeviceToHost, c->stream), "read stats");
         check(hipStreamSynchronize(c->stream), "group sync");
         const int maxUnion = st[0], overflow = st[1], unionTotal = st[2], chunkTotal = st[3], workTotal = st[4];
         if (overflow)
@@ -302,6 +309,7 @@ void buildWorkGroups(nbx_context* c, WorkLayout& L, bool adjoint)
         // take direct global atomics, which is accepted while that is at most
         // 2% of the union entries (any share in the last pass)
         int slots = G + maxUnion + 1;
+        L.spill   = false;
         if (ldsBytes(slots) > budget)
         {
             while (slots > G + 2 && ldsBytes(slots) > budget)
@@ -323,6 +331,7 @@ void buildWorkGroups(nbx_context* c, WorkLayout& L, bool adjoint)
             {
                 continue;
             }
+            L.spill = spilled > 0;
         }
         L.unionList.ensure(std::max(unionTotal, 1));
         L.chunks.ensure(std::max(chunkTotal, 1));
@@ -444,9 +453,9 @@ int nbx_create(const nbx_options* options, nbx_context** context)
             throw InputError("unknown Ewald correction type");
         }
         if (o.kernel_group_size != 0 && o.kernel_group_size != 1 && o.kernel_group_size != 2
-            && o.kernel_group_size != 4 && o.kernel_group_size != 8)
+            && o.kernel_group_size != 4 && o.kernel_group_size != 8 && o.kernel_group_size != 16)
         {
-            throw InputError("kernel_group_size must be 0, 1, 2, 4 or 8");
+            throw InputError("kernel_group_size must be 0, 1, 2, 4, 8 or 16");
         }
         if (o.kernel_blocks_per_cu < 0 || o.kernel_blocks_per_cu > 8)
         {
@@ -740,6 +749,26 @@ int nbx_set_topology(nbx_context* c, int numAtoms, const int* types, const float
         {
             check(hipMemcpy(c->exclIdx.p, exclIdx, sizeof(int) * nnz, hipMemcpyHostToDevice), "upload excl");
         }
+        // each excluded pair once, from the list of its lower-indexed atom
+        std::vector<int2> pairs;
+        for (int a = 0; a < numAtoms; a++)
+        {
+            for (int k = exclOff[a]; k < exclOff[a + 1]; k++)
+            {
+                if (exclIdx[k] > a)
+                {
+                    pairs.push_back(make_int2(a, exclIdx[k]));
+                }
+            }
+        }
+        c->numExclPairs = static_cast<int>(pairs.size());
+        c->exclPairs.ensure(std::max<size_t>(pairs.size(), 1));
+        c->exclPairSlots.ensure(std::max<size_t>(pairs.size(), 1));
+        if (!pairs.empty())
+        {
+            check(hipMemcpy(c->exclPairs.p, pairs.data(), sizeof(int2) * pairs.size(), hipMemcpyHostToDevice),
+                  "upload excluded pairs");
+        }
         c->gridValid = false;
         c->listValid = false;
     });
@@ -926,6 +955,7 @@ int nbx_construct_pairlist(nbx_context* c)
         a.jentry = c->jentry.p;
         launchSearch(c->stream, a, true);
         check(hipGetLastError(), "search kernels");
+        launchExclusionSlots(c->stream, c->numExclPairs, c->exclPairs.p, c->cells.p, c->exclPairSlots.p);
         buildWorkGroups(c, c->lay, false);
         scheduleLayout(c, c->lay, false);
         const int numWaves = c->lay.numBlocks * kKernelWaves;
@@ -934,7 +964,7 @@ int nbx_construct_pairlist(nbx_context* c)
         c->fshiftPartial.ensure(static_cast<size_t>(c->lay.numBlocks) * kNumShifts * 3);
         c->energy.ensure(2);
         c->fshift.ensure(kNumShifts * 3);
-        c->virialPartial.ensure(9 * 1024);
+        c->virialPartial.ensure(9 * static_cast<size_t>(kGatherMaxBlocks));
         c->virial.ensure(9);
         c->listValid = true;
         c->adjValid  = false;
@@ -967,9 +997,14 @@ static void enqueueStep(nbx_context* c, unsigned flags, const float* xd, float* 
     const bool shift  = (flags & (NBX_COMPUTE_SHIFT_FORCES | NBX_COMPUTE_VIRIAL)) != 0;
     // the virial needs the atom-order forces and coordinates of this step
     const bool virial = (flags & NBX_COMPUTE_VIRIAL) != 0 && fOut && xd;
-    const int numSlots = c->lay.ngroups + c->lay.numBlocks * kKernelWaves;
-    launchPrepareStep(c->stream, c->ncluster * kClusterSize, c->atomIndices.p, xd, c->xq.p, c->fnb.p,
-                      energy ? c->energyPartial.p : nullptr, 2 * numSlots);
+    // one energy slot per wavefront
+    const int numSlots = c->lay.numBlocks * kKernelWaves;
+    // the cluster force buffer holds the spills of windows that did not fit
+    // (and is the output when no atom-order forces are asked for); without
+    // either it stays zero and is neither cleared nor read
+    const bool useFnb = c->lay.spill || !fOut;
+    launchPrepareStep(c->stream, c->ncluster * kClusterSize, c->atomIndices.p, xd, c->xq.p,
+                      useFnb ? c->fnb.p : nullptr, energy ? c->energyPartial.p : nullptr, 2 * numSlots);
     KernelArgs a{};
     a.chunks        = c->lay.chunks.p;
     a.groups        = c->lay.groups.p;
@@ -992,46 +1027,47 @@ static void enqueueStep(nbx_context* c, unsigned flags, const float* xd, float* 
     a.staging       = c->lay.gather.staging.p;
     a.fnb           = c->fnb.p;
     a.energyPartial = c->energyPartial.p;
+    a.waveEnergy    = 1;
     a.fshiftPartial = c->fshiftPartial.p;
     const bool timed = 2 * static_cast<size_t>(c->timingRecorded) < c->timingEvents.size();
+    check(launchNonbonded(c->stream, a, c->elec, energy, shift, c->lay.numBlocks,
+                          timed ? c->timingEvents[2 * c->timingRecorded] : nullptr,
+                          timed ? c->timingEvents[2 * c->timingRecorded + 1] : nullptr),
+          "nonbonded kernel");
     if (timed)
     {
-        check(hipEventRecord(c->timingEvents[2 * c->timingRecorded], c->stream), "hipEventRecord");
-    }
-    launchNonbonded(c->stream, a, c->elec, energy, shift, c->lay.numBlocks);
-    if (timed)
-    {
-        check(hipEventRecord(c->timingEvents[2 * c->timingRecorded + 1], c->stream), "hipEventRecord");
         c->timingRecorded++;
     }
-    launchGatherForces(c->stream, c->ncluster, c->lay.gather.gposOff.p, c->lay.gather.gpos.p, c->lay.gather.staging.p,
-                       c->fnb.p, c->atomIndices.p, fOut);
-    int numExcl = 0;
-    if (energy || virial)
+    ExclusionArgs e{};
+    if (energy)
     {
-        ExclusionArgs e{};
-        e.natoms   = c->natoms;
-        e.cells    = c->cells.p;
-        e.xq       = c->xq.p;
-        e.exclOff  = c->exclOff.p;
-        e.exclIdx  = c->exclIdx.p;
-        e.box[0]   = c->g.box[0];
-        e.box[1]   = c->g.box[1];
-        e.box[2]   = c->g.box[2];
-        e.elec     = c->elec;
-        e.ic       = c->ic;
-        e.rc2      = static_cast<double>(c->ic.rcoulomb2);
-        e.epsfacD  = static_cast<double>(c->ic.epsfac);
-        e.selfCoef = c->selfCoef;
-        e.tabD     = c->tabFDV0d.p;
-        e.partial  = c->exclPartial.p;
-        numExcl    = launchAtomPass(c->stream, e, energy, xd, fOut, virial ? c->virialPartial.p : nullptr);
+        e.natoms    = c->natoms;
+        e.nslot     = c->ncluster * kClusterSize;
+        e.npairs    = c->numExclPairs;
+        e.pairSlots = c->exclPairSlots.p;
+        e.xq        = c->xq.p;
+        e.box[0]    = c->g.box[0];
+        e.box[1]    = c->g.box[1];
+        e.box[2]    = c->g.box[2];
+        e.elec      = c->elec;
+        e.ic        = c->ic;
+        e.rc2       = static_cast<double>(c->ic.rcoulomb2);
+        e.epsfacD   = static_cast<double>(c->ic.epsfac);
+        e.selfCoef  = c->selfCoef;
+        e.tabD      = c->tabFDV0d.p;
+        e.partial   = c->exclPartial.p;
     }
+    // the gather's workgroups also sum the virial's x (x) f term
+    const int numVir =
+            launchGatherForces(c->stream, c->ncluster, c->lay.gather.gposOff.p, c->lay.gather.gpos.p,
+                               c->lay.gather.staging.p, useFnb ? c->fnb.p : nullptr, c->atomIndices.p, fOut,
+                               c->xq.p, virial ? c->virialPartial.p : nullptr);
+    const int numExcl = energy ? launchExclusionPairEnergy(c->stream, e) : 0;
     if (energy || shift || virial)
     {
         launchFinalize(c->stream, energy ? numSlots : 0, c->energyPartial.p, energy ? numExcl : 0,
                        c->exclPartial.p, c->lay.numBlocks, c->fshiftPartial.p, shift, c->energy.p, c->fshift.p,
-                       virial ? numExcl : 0, c->virialPartial.p, c->shiftVec.p, virial ? c->virial.p : nullptr);
+                       virial ? numVir : 0, c->virialPartial.p, c->shiftVec.p, virial ? c->virial.p : nullptr);
     }
     check(hipGetLastError(), "nonbonded kernel");
     c->haveEnergy = energy;

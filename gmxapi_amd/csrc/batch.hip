@@ -306,9 +306,9 @@ void enqueueCompute(nbx_batch* b, const float* x, float* f, unsigned flags)
     a.fnb          = b->fnb.p;
     a.energyPartial = b->energyPartial.p;
     a.fshiftPartial = nullptr;
-    launchNonbonded(b->stream, a, c0->elec, energy, false, b->numBlocks);
+    check(launchNonbonded(b->stream, a, c0->elec, energy, false, b->numBlocks), "nonbonded kernel");
     launchGatherForces(b->stream, b->totClusters, b->gather.gposOff.p, b->gather.gpos.p, b->gather.staging.p,
-                       b->fnb.p, b->atomIndices.p, f);
+                       b->fnb.p, b->atomIndices.p, f, nullptr, nullptr);
     if (energy)
     {
         ExclusionArgs e{};
@@ -565,7 +565,7 @@ int nbx_batch_autotune(nbx_batch* b, const float* x, float* f, unsigned flags, i
         check(hipEventCreate(&e1), "hipEventCreate");
         float best = -1.0f;
         int   bestG = 0, bestBpc = 0, bestWeight = 0;
-        for (int G : { 8, 4, 2, 1 })
+        for (int G : { 16, 8, 4, 2, 1 })
         {
             try
             {

@@ -52,7 +52,10 @@ __global__ void prepareStepKernel(int nslot, const int* __restrict__ atomIndices
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < nslot)
     {
-        fnb[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (fnb)
+        {
+            fnb[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
         if (x)
         {
             const int a = atomIndices[i];
@@ -88,71 +91,6 @@ __global__ void convertXKernel(int nslot, const int* __restrict__ atomIndices, c
         v.y      = x[3 * a + 1];
         v.z      = x[3 * a + 2];
         xq[s]    = v;
-    }
-}
-
-// Forces of each cluster from the staged windows that hold it: a wavefront
-// per cluster, lane l adding copies l, l + 64, ... (12 floats each), then a
-// fixed butterfly over the lanes, so the sum order is the same every step;
-// added to the cluster force buffer (which holds the contributions of
-// clusters that spilled out of a window).
-template<bool kToAtoms>
-__global__ __launch_bounds__(256) void gatherForcesKernel(int ncluster, const int* __restrict__ gposOff,
-                                                          const int* __restrict__ gpos,
-                                                          const float4* __restrict__ staging,
-                                                          float4* __restrict__ fnb,
-                                                          const int* __restrict__ atomIndices, float* __restrict__ fOut)
-{
-    const int lane = threadIdx.x & 63;
-    const int c    = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (c >= ncluster)
-    {
-        return;
-    }
-    float f[12];
-#pragma unroll
-    for (int k = 0; k < 12; k++)
-    {
-        f[k] = 0.0f;
-    }
-    for (int i = gposOff[c] + lane; i < gposOff[c + 1]; i += 64)
-    {
-        const int    p  = gpos[i];
-        const float4 v0 = staging[p];
-        const float4 v1 = staging[p + 1];
-        const float4 v2 = staging[p + 2];
-        f[0] += v0.x;
-        f[1] += v0.y;
-        f[2] += v0.z;
-        f[3] += v0.w;
-        f[4] += v1.x;
-        f[5] += v1.y;
-        f[6] += v1.z;
-        f[7] += v1.w;
-        f[8] += v2.x;
-        f[9] += v2.y;
-        f[10] += v2.z;
-        f[11] += v2.w;
-    }
-    // lane L ends up with component L >> 2 = 3 * atom + d
-    const float r   = waveReduceScatter12(f, lane);
-    const int   idx = lane >> 2;
-    if ((lane & 3) == 0 && idx < 12)
-    {
-        float* dst = reinterpret_cast<float*>(fnb + c * kClusterSize + idx / 3) + idx % 3;
-        if (kToAtoms)
-        {
-            // straight to the caller's atom-order forces (nbat-f -> f)
-            const int a = atomIndices[c * kClusterSize + idx / 3];
-            if (a >= 0)
-            {
-                fOut[3 * a + idx % 3] = *dst + r;
-            }
-        }
-        else
-        {
-            *dst += r;
-        }
     }
 }
 
@@ -202,9 +140,18 @@ __device__ __forceinline__ double strideSum(const T* __restrict__ v, int n, int 
         a2 += static_cast<double>(v[static_cast<size_t>(k + 2 * t) * stride + comp]);
         a3 += static_cast<double>(v[static_cast<size_t>(k + 3 * t) * stride + comp]);
     }
-    for (; k < n; k += t)
+    // at most three left, loaded independently
+    if (k < n)
     {
         a0 += static_cast<double>(v[static_cast<size_t>(k) * stride + comp]);
+    }
+    if (k + t < n)
+    {
+        a1 += static_cast<double>(v[static_cast<size_t>(k + t) * stride + comp]);
+    }
+    if (k + 2 * t < n)
+    {
+        a2 += static_cast<double>(v[static_cast<size_t>(k + 2 * t) * stride + comp]);
     }
     return (a0 + a1) + (a2 + a3);
 }
@@ -224,7 +171,7 @@ __global__ __launch_bounds__(kFinalizeThreads) void finalizeKernel(int numEnergy
     // blocks 0, 1: the energies; 2 .. 2 + 135 (with shift forces): one
     // shift-force component each; the last 9 (with the virial): one virial
     // component each, -1/2 sum x (x) f plus -1/2 sum_s s (x) fshift_s
-    __shared__ double s[kFinalizeThreads];
+    __shared__ double s[kFinalizeThreads / 64];
     const int comp = blockIdx.x;
     double    acc  = 0.0;
     const int vir0 = doShift ? 2 + kNumShifts * 3 : 2;
@@ -232,12 +179,22 @@ __global__ __launch_bounds__(kFinalizeThreads) void finalizeKernel(int numEnergy
     {
         const int k = comp - vir0, d1 = k / 3, d2 = k % 3;
         acc         = strideSum(virPart, numVirPartials, 9, k);
-        for (int t = threadIdx.x; t < numShiftPartials * kNumShifts; t += kFinalizeThreads)
+        // a thread per shift-force partial, its shifts unrolled so that the
+        // loads are independent
+        static_assert(kNumShifts % 3 == 0, "shift loop unrolled by 3");
+        for (int b = threadIdx.x; b < numShiftPartials; b += kFinalizeThreads)
         {
-            const int    sh = t % kNumShifts;
-            const float4 sv = shiftVec[sh];
-            const double sd = d1 == 0 ? sv.x : (d1 == 1 ? sv.y : sv.z);
-            acc -= 0.5 * sd * static_cast<double>(fshPart[static_cast<size_t>(t / kNumShifts) * kNumShifts * 3 + sh * 3 + d2]);
+            const float* p  = fshPart + static_cast<size_t>(b) * kNumShifts * 3 + d2;
+            double       t0 = 0.0, t1 = 0.0, t2 = 0.0;
+#pragma unroll
+            for (int sh = 0; sh < kNumShifts; sh += 3)
+            {
+                const float4 s0 = shiftVec[sh], s1 = shiftVec[sh + 1], s2 = shiftVec[sh + 2];
+                t0 += static_cast<double>(d1 == 0 ? s0.x : (d1 == 1 ? s0.y : s0.z)) * p[3 * sh];
+                t1 += static_cast<double>(d1 == 0 ? s1.x : (d1 == 1 ? s1.y : s1.z)) * p[3 * sh + 3];
+                t2 += static_cast<double>(d1 == 0 ? s2.x : (d1 == 1 ? s2.y : s2.z)) * p[3 * sh + 6];
+            }
+            acc -= 0.5 * ((t0 + t1) + t2);
         }
     }
     else if (comp < 2)
@@ -252,29 +209,31 @@ __global__ __launch_bounds__(kFinalizeThreads) void finalizeKernel(int numEnergy
     {
         acc = strideSum(fshPart, numShiftPartials, kNumShifts * 3, comp - 2);
     }
-    s[threadIdx.x] = acc;
-    __syncthreads();
-    for (int off = kFinalizeThreads / 2; off > 0; off >>= 1)
+    // wavefront sums, then the wavefront sums in order
+    acc = waveSumD(acc);
+    if ((threadIdx.x & 63) == 0)
     {
-        if (threadIdx.x < off)
-        {
-            s[threadIdx.x] += s[threadIdx.x + off];
-        }
-        __syncthreads();
+        s[threadIdx.x / 64] = acc;
     }
+    __syncthreads();
     if (threadIdx.x == 0)
     {
+        double t = 0.0;
+        for (int w = 0; w < kFinalizeThreads / 64; w++)
+        {
+            t += s[w];
+        }
         if (virOut && comp >= vir0)
         {
-            virOut[comp - vir0] = s[0];
+            virOut[comp - vir0] = t;
         }
         else if (comp < 2)
         {
-            eOut[comp] = s[0];
+            eOut[comp] = t;
         }
         else
         {
-            fshOut[comp - 2] = s[0];
+            fshOut[comp - 2] = t;
         }
     }
     (void)doShift;
@@ -418,154 +377,167 @@ __global__ __launch_bounds__(kFinalizeThreads) void batchFinalizeKernel(const in
     }
 }
 
-// One pass over the atoms after the forces are final: exclusion and self
-// energies (per-workgroup sums into a.partial) and the virial's -1/2 x (x) f
-// term (per-workgroup sums into virPart[9 * workgroup + k]), in double.
-template<bool kEnergy, bool kVirial>
-__global__ __launch_bounds__(kExclThreads) void atomPassKernel(const ExclusionArgs a, const float* __restrict__ x,
-                                                               const float* __restrict__ f,
-                                                               double* __restrict__ virPart)
+// Item t of the single-system exclusion pass, in double: the self term
+// -selfCoef q^2 of cluster slot t (fillers carry q = 0) and the energy of
+// excluded pair t inside the cut-off (both atoms as cluster slots, each pair
+// once).  Every load depends only on t or on one list entry.
+__device__ double exclusionItemEnergy(const ExclusionArgs& a, int t)
 {
-    __shared__ double s[10][kExclThreads / 64];
-    double            v[10] = { 0, 0, 0, 0, 0, 0, 0, 0, 0, 0 }; // virial 0..8, energy 9
-    for (int i = blockIdx.x * kExclThreads + threadIdx.x; i < a.natoms; i += gridDim.x * kExclThreads)
+    double e = 0.0;
+    if (t < a.nslot)
     {
-        if (kEnergy)
-        {
-            v[9] += atomExclusionEnergy(a, i);
-        }
-        if (kVirial)
-        {
-            const double xa[3] = { x[3 * i], x[3 * i + 1], x[3 * i + 2] };
-            const double fa[3] = { f[3 * i], f[3 * i + 1], f[3 * i + 2] };
-#pragma unroll
-            for (int d1 = 0; d1 < 3; d1++)
-            {
-#pragma unroll
-                for (int d2 = 0; d2 < 3; d2++)
-                {
-                    v[d1 * 3 + d2] -= 0.5 * xa[d1] * fa[d2];
-                }
-            }
-        }
+        const double q = a.xq[t].w;
+        e -= a.selfCoef * q * q;
     }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x / 64;
+    if (t < a.npairs)
+    {
+        const int2   ps   = a.pairSlots[t];
+        const float4 pi   = a.xq[ps.x];
+        const float4 pj   = a.xq[ps.y];
+        double       d[3] = { static_cast<double>(pi.x) - pj.x, static_cast<double>(pi.y) - pj.y,
+                        static_cast<double>(pi.z) - pj.z };
+        double       r2   = 0.0;
 #pragma unroll
-    for (int k = 0; k < 10; k++)
-    {
-        if ((k < 9 && kVirial) || (k == 9 && kEnergy))
+        for (int c = 0; c < 3; c++)
         {
-            for (int m = 1; m < 64; m *= 2)
-            {
-                v[k] += __shfl_xor(v[k], m, 64);
-            }
-            if (lane == 0)
-            {
-                s[k][wave] = v[k];
-            }
+            const double L = a.box[c];
+            d[c] -= L * rint(d[c] / L);
+            r2 += d[c] * d[c];
         }
-    }
-    __syncthreads();
-    if (threadIdx.x < 10)
-    {
-        const int k = threadIdx.x;
-        if ((k < 9 && kVirial) || (k == 9 && kEnergy))
+        if (r2 < a.rc2)
         {
-            double t = 0.0;
-            for (int w = 0; w < kExclThreads / 64; w++)
+            const double qq = a.epsfacD * static_cast<double>(pi.w) * static_cast<double>(pj.w);
+            if (a.elec == kElecRF)
             {
-                t += s[k][w];
-            }
-            if (k == 9)
-            {
-                a.partial[blockIdx.x] = t;
+                e += qq * (static_cast<double>(a.ic.k_rf) * r2 - static_cast<double>(a.ic.c_rf));
             }
             else
             {
-                virPart[blockIdx.x * 9 + k] = t;
+                e -= qq * ewaldCorrectionD(sqrt(r2), a);
             }
         }
     }
+    return e;
 }
 
-constexpr int kVirThreads = 256;
+// Forces of each cluster from the staged windows that hold it: a wavefront
+// per cluster, lane l adding copies l, l + 64, ... (12 floats each), then a
+// fixed butterfly over the lanes, so the sum order is the same every step;
+// added to the cluster force buffer (which holds the contributions of
+// clusters that spilled out of a window; with atom-order output it may be
+// NULL when nothing spilled).  Wavefronts stride over the
+// clusters.  With virPart the -1/2 x (x) f part of the virial is summed in
+// double from the cluster-layout coordinates and the final forces, one
+// partial per workgroup in a fixed order.
+constexpr int kGatherThreads = 256;
 
-__global__ __launch_bounds__(kVirThreads) void virialPartialKernel(int natoms, const float* __restrict__ x,
-                                                                   const float* __restrict__ f,
-                                                                   double* __restrict__ part)
+template<bool kToAtoms>
+__global__ __launch_bounds__(kGatherThreads) void gatherForcesKernel(int ncluster, const int* __restrict__ gposOff,
+                                                                     const int* __restrict__ gpos,
+                                                                     const float4* __restrict__ staging,
+                                                                     float4* __restrict__ fnb,
+                                                                     const int* __restrict__ atomIndices,
+                                                                     float* __restrict__ fOut,
+                                                                     const float4* __restrict__ xq,
+                                                                     double* __restrict__ virPart)
 {
-    __shared__ double s[9][kVirThreads / 64];
-    double v[9] = { 0, 0, 0, 0, 0, 0, 0, 0, 0 };
-    for (int a = blockIdx.x * kVirThreads + threadIdx.x; a < natoms; a += gridDim.x * kVirThreads)
+    constexpr int kWaves = kGatherThreads / 64;
+    __shared__ double sVir[kWaves][12][3];
+    const int         lane = threadIdx.x & 63;
+    const int         wave = threadIdx.x >> 6;
+    const int         idx  = lane >> 2; // component 3 * atom + d of this lane's sum
+    double            vir[3] = { 0.0, 0.0, 0.0 };
+    for (int c = blockIdx.x * kWaves + wave; c < ncluster; c += gridDim.x * kWaves)
     {
-        const double xa[3] = { x[3 * a], x[3 * a + 1], x[3 * a + 2] };
-        const double fa[3] = { f[3 * a], f[3 * a + 1], f[3 * a + 2] };
+        float f[12];
 #pragma unroll
-        for (int d1 = 0; d1 < 3; d1++)
+        for (int k = 0; k < 12; k++)
         {
-#pragma unroll
-            for (int d2 = 0; d2 < 3; d2++)
+            f[k] = 0.0f;
+        }
+        for (int i = gposOff[c] + lane; i < gposOff[c + 1]; i += 64)
+        {
+            const int    p  = gpos[i];
+            const float4 v0 = staging[p];
+            const float4 v1 = staging[p + 1];
+            const float4 v2 = staging[p + 2];
+            f[0] += v0.x;
+            f[1] += v0.y;
+            f[2] += v0.z;
+            f[3] += v0.w;
+            f[4] += v1.x;
+            f[5] += v1.y;
+            f[6] += v1.z;
+            f[7] += v1.w;
+            f[8] += v2.x;
+            f[9] += v2.y;
+            f[10] += v2.z;
+            f[11] += v2.w;
+        }
+        // lane L ends up with component L >> 2 = 3 * atom + d
+        const float r = waveReduceScatter12(f, lane);
+        if ((lane & 3) == 0 && idx < 12)
+        {
+            const int slot = c * kClusterSize + idx / 3;
+            float*    dst  = fnb ? reinterpret_cast<float*>(fnb + slot) + idx % 3 : nullptr;
+            if (kToAtoms)
             {
-                v[d1 * 3 + d2] -= 0.5 * xa[d1] * fa[d2];
+                // straight to the caller's atom-order forces (nbat-f -> f)
+                const int a = atomIndices[slot];
+                if (a >= 0)
+                {
+                    const float fv       = (fnb ? *dst : 0.0f) + r;
+                    fOut[3 * a + idx % 3] = fv;
+                    if (virPart)
+                    {
+                        const float4 p = xq[slot];
+                        vir[0] += static_cast<double>(p.x) * fv;
+                        vir[1] += static_cast<double>(p.y) * fv;
+                        vir[2] += static_cast<double>(p.z) * fv;
+                    }
+                }
+            }
+            else
+            {
+                *dst += r;
             }
         }
     }
-#pragma unroll
-    for (int k = 0; k < 9; k++)
+    if (kToAtoms && virPart)
     {
-        for (int m = 1; m < 64; m *= 2)
+        if ((lane & 3) == 0 && idx < 12)
         {
-            v[k] += __shfl_xor(v[k], m, 64);
+            sVir[wave][idx][0] = vir[0];
+            sVir[wave][idx][1] = vir[1];
+            sVir[wave][idx][2] = vir[2];
         }
-    }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x / 64;
-    if (lane == 0)
-    {
-        for (int k = 0; k < 9; k++)
+        __syncthreads();
+        if (threadIdx.x < 9)
         {
-            s[k][wave] = v[k];
+            // component (d1, d2): x_d1 f_d2 over the workgroup's atoms
+            const int d1 = threadIdx.x / 3, d2 = threadIdx.x % 3;
+            double    t  = 0.0;
+            for (int w = 0; w < kWaves; w++)
+            {
+                for (int at = 0; at < kClusterSize; at++)
+                {
+                    t += sVir[w][3 * at + d2][d1];
+                }
+            }
+            virPart[blockIdx.x * 9 + threadIdx.x] = -0.5 * t;
         }
-    }
-    __syncthreads();
-    if (threadIdx.x < 9)
-    {
-        double t = 0;
-        for (int w = 0; w < kVirThreads / 64; w++)
-        {
-            t += s[threadIdx.x][w];
-        }
-        part[blockIdx.x * 9 + threadIdx.x] = t;
     }
 }
 
-// One wavefront per virial component: strided sum of the partials plus the
-// shift-force term, then a shuffle reduction.
-__global__ __launch_bounds__(64 * 9) void virialFinalKernel(int numPartials, const double* __restrict__ part,
-                                                            const double* __restrict__ fshift,
-                                                            const float4* __restrict__ shiftVec,
-                                                            double* __restrict__ out)
+// Excluded pairs as cluster slots, after each gridding.
+__global__ void exclusionSlotsKernel(int npairs, const int2* __restrict__ pairs, const int* __restrict__ cells,
+                                     int2* __restrict__ pairSlots)
 {
-    const int k    = threadIdx.x / 64;
-    const int lane = threadIdx.x & 63;
-    const int d1 = k / 3, d2 = k % 3;
-    double    t  = 0.0;
-    for (int p = lane; p < numPartials; p += 64)
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < npairs)
     {
-        t += part[p * 9 + k];
-    }
-    if (lane < kNumShifts)
-    {
-        const float4 sv  = shiftVec[lane];
-        const double svd = (d1 == 0) ? sv.x : ((d1 == 1) ? sv.y : sv.z);
-        t -= 0.5 * svd * fshift[lane * 3 + d2];
-    }
-    for (int m = 1; m < 64; m *= 2)
-    {
-        t += __shfl_xor(t, m, 64);
-    }
-    if (lane == 0)
-    {
-        out[k] = t;
+        const int2 p = pairs[t];
+        pairSlots[t] = make_int2(cells[p.x], cells[p.y]);
     }
 }
 
@@ -667,6 +639,34 @@ __global__ void expandPairsKernel(const int4* __restrict__ ientry, int nci, cons
     }
 }
 
+// Self energies and the energies of excluded pairs inside the cut-off for one
+// system (exclusionItemEnergy over the threads), one partial per workgroup.
+__global__ __launch_bounds__(kExclThreads) void exclusionPairEnergyKernel(const ExclusionArgs a)
+{
+    __shared__ double s[kExclThreads / 64];
+    double            e = 0.0;
+    const int         n = max(a.nslot, a.npairs);
+    for (int t = blockIdx.x * kExclThreads + threadIdx.x; t < n; t += gridDim.x * kExclThreads)
+    {
+        e += exclusionItemEnergy(a, t);
+    }
+    e = waveSumD(e);
+    if ((threadIdx.x & 63) == 0)
+    {
+        s[threadIdx.x / 64] = e;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        double t = 0.0;
+        for (int w = 0; w < kExclThreads / 64; w++)
+        {
+            t += s[w];
+        }
+        a.partial[blockIdx.x] = t;
+    }
+}
+
 } // namespace
 
 void launchClearF4(hipStream_t s, float4* p, int n, double* d, int nd)
@@ -696,23 +696,33 @@ void launchConvertX(hipStream_t s, int nslot, const int* atomIndices, const floa
     }
 }
 
-void launchGatherForces(hipStream_t s, int ncluster, const int* gposOff, const int* gpos, const float4* staging,
-                        float4* fnb, const int* atomIndices, float* f)
+int launchGatherForces(hipStream_t s, int ncluster, const int* gposOff, const int* gpos, const float4* staging,
+                       float4* fnb, const int* atomIndices, float* f, const float4* xq, double* virPart)
 {
     if (ncluster == 0)
     {
-        return;
+        return 0;
     }
+    const int blocks = std::min(kGatherMaxBlocks, (ncluster + 3) / 4);
     if (f)
     {
-        gatherForcesKernel<true><<<(ncluster + 3) / 4, 256, 0, s>>>(ncluster, gposOff, gpos, staging, fnb,
-                                                                     atomIndices, f);
+        gatherForcesKernel<true><<<blocks, kGatherThreads, 0, s>>>(ncluster, gposOff, gpos, staging, fnb, atomIndices,
+                                                                    f, xq, virPart);
     }
     else
     {
-        gatherForcesKernel<false><<<(ncluster + 3) / 4, 256, 0, s>>>(ncluster, gposOff, gpos, staging, fnb, nullptr,
-                                                                      nullptr);
+        gatherForcesKernel<false><<<blocks, kGatherThreads, 0, s>>>(ncluster, gposOff, gpos, staging, fnb, nullptr,
+                                                                     nullptr, nullptr, nullptr);
     }
+    return blocks;
+}
+
+int launchExclusionPairEnergy(hipStream_t s, const ExclusionArgs& a)
+{
+    const int n      = std::max(a.nslot, a.npairs);
+    const int blocks = std::max(1, std::min(kExclMaxBlocks, (n + kExclThreads - 1) / kExclThreads));
+    exclusionPairEnergyKernel<<<blocks, kExclThreads, 0, s>>>(a);
+    return blocks;
 }
 
 void launchSetCharges(hipStream_t s, int nslot, const int* atomIndices, const float* q, float4* xq)
@@ -742,23 +752,12 @@ void launchFinalize(hipStream_t s, int numEnergyPartials, const double* energyPa
                                                        fshiftOut, numVirPartials, virPartial, shiftVec, virialOut);
 }
 
-int launchAtomPass(hipStream_t s, const ExclusionArgs& a, bool energy, const float* x, const float* f,
-                   double* virialPartial)
+void launchExclusionSlots(hipStream_t s, int npairs, const int2* pairs, const int* cells, int2* pairSlots)
 {
-    const int blocks = std::max(1, std::min(kExclMaxBlocks, (a.natoms + kExclThreads - 1) / kExclThreads));
-    if (energy && virialPartial)
+    if (npairs > 0)
     {
-        atomPassKernel<true, true><<<blocks, kExclThreads, 0, s>>>(a, x, f, virialPartial);
+        exclusionSlotsKernel<<<(npairs + 255) / 256, 256, 0, s>>>(npairs, pairs, cells, pairSlots);
     }
-    else if (energy)
-    {
-        atomPassKernel<true, false><<<blocks, kExclThreads, 0, s>>>(a, x, f, virialPartial);
-    }
-    else if (virialPartial)
-    {
-        atomPassKernel<false, true><<<blocks, kExclThreads, 0, s>>>(a, x, f, virialPartial);
-    }
-    return blocks;
 }
 
 void launchBatchFinalize(hipStream_t s, int numSystems, const int* slotOff, const double* energyPartial,
@@ -778,12 +777,6 @@ int launchExclusionEnergy(hipStream_t s, const ExclusionArgs& a)
     return blocks;
 }
 
-void launchVirial(hipStream_t s, int natoms, const float* x, const float* f, const double* fshift,
-                  const float4* shiftVec, double* virialPartial, int numBlocks, double* virialOut)
-{
-    virialPartialKernel<<<numBlocks, kVirThreads, 0, s>>>(natoms, x, f, virialPartial);
-    virialFinalKernel<<<1, 64 * 9, 0, s>>>(numBlocks, virialPartial, fshift, shiftVec, virialOut);
-}
 
 void launchExpandPairs(hipStream_t s, const int4* ientry, int nci, const uint2* jentry, const float4* xq,
                        const int* atomIndices, const float4* shiftVec, float r2, unsigned long long* counter,

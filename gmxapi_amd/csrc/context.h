@@ -163,6 +163,7 @@ struct WorkLayout
 {
     int            groupSize = 0, ngroups = 0, nchunk = 0, windowSlots = 0, nwork = 0;
     int            numBlocks = 0; // workgroups of kKernelWaves wavefronts (the schedule's wavefronts / 4)
+    bool           spill     = false; // some union clusters take global atomics (no window slot)
     DevBuf<int>    unionCount, chunkCount, workCount, overflow, unionOff, chunkOff, workOff, unionList;
     DevBuf<int4>   chunks, groups, groupInfo;
     DevBuf<float4> chunkShift;
@@ -235,6 +236,10 @@ struct nbx_context
     nbx::DevBuf<int> type;
     nbx::DevBuf<float> charge;
     nbx::DevBuf<int> exclOff, exclIdx;
+    // excluded pairs (i < j as listed for i, duplicates kept), in atom
+    // indices and, after each search, as cluster slots
+    nbx::DevBuf<int2> exclPairs, exclPairSlots;
+    int               numExclPairs = 0;
 
     // grid
     nbx::GridDims g{};

@@ -270,7 +270,7 @@ __global__ void groupSpanKernel(int ncluster, const int* __restrict__ offJ, int*
         return;
     }
 #pragma unroll
-    for (int k = 0; k < 4; k++)
+    for (int k = 0; k < kGroupSizeCount; k++)
     {
         const int G = kGroupSizeMax >> k;
         if (c % G == 0)

@@ -57,6 +57,12 @@
 //     only.
 // Pair arithmetic is FP32 on the vector ALUs; no matrix instructions are used
 // (the pair interaction is not a dense contraction).
+#include <hip/hip_ext.h>
+
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include "kernel_common.h"
 
 namespace nbx
@@ -309,7 +315,7 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
             }
         };
         auto endGroup = [&]() {
-            if (kEnergy)
+            if (kEnergy && !a.waveEnergy)
             {
                 const double v = waveSumD(eVdw) * (1.0 / 12.0);
                 const double c = waveSumD(eCoul);
@@ -475,6 +481,16 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
         }
         endChunk();
         endGroup();
+        if (kEnergy && a.waveEnergy)
+        {
+            const double v = waveSumD(eVdw) * (1.0 / 12.0);
+            const double c = waveSumD(eCoul);
+            if (lane == 0)
+            {
+                a.energyPartial[2 * wid]     = v;
+                a.energyPartial[2 * wid + 1] = c;
+            }
+        }
     }
 
     if (kShift)
@@ -487,40 +503,75 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
     }
 }
 
-// dynamic LDS beyond 64 KiB is requested explicitly (a CU has 160 KiB)
+// dynamic LDS beyond 64 KiB is requested explicitly (a CU has 160 KiB).
+// With events, the launch itself records the kernel's start and end (the
+// dispatch packet's own timestamps): no marker packets in the stream, so
+// timing the kernel leaves the step's timeline unchanged.
 template<int kElec, bool kEnergy, bool kShift>
-void launchOne(hipStream_t s, const KernelArgs& a, int numBlocks, int lds)
+hipError_t launchOne(hipStream_t s, const KernelArgs& a, int numBlocks, int lds, hipEvent_t start, hipEvent_t stop)
 {
-    if (lds > 64 * 1024)
+    const void* fn = reinterpret_cast<const void*>(&nonbondedKernel<kElec, kEnergy, kShift>);
+    const hipError_t e = allowDynamicLds(fn, lds);
+    if (e != hipSuccess)
     {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&nonbondedKernel<kElec, kEnergy, kShift>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        return e;
+    }
+    if (start || stop)
+    {
+        void* args[] = { const_cast<KernelArgs*>(&a) };
+        return hipExtLaunchKernel(fn, dim3(numBlocks), dim3(kThreads), args, lds, s, start, stop, 0);
     }
     nonbondedKernel<kElec, kEnergy, kShift><<<numBlocks, kThreads, lds, s>>>(a);
+    return hipGetLastError();
 }
 
 template<int kElec>
-void launchElec(hipStream_t s, const KernelArgs& a, bool energy, bool shiftForces, int numBlocks, int lds)
+hipError_t launchElec(hipStream_t s, const KernelArgs& a, bool energy, bool shiftForces, int numBlocks, int lds,
+                      hipEvent_t start, hipEvent_t stop)
 {
     if (energy && shiftForces)
     {
-        launchOne<kElec, true, true>(s, a, numBlocks, lds);
+        return launchOne<kElec, true, true>(s, a, numBlocks, lds, start, stop);
     }
-    else if (energy)
+    if (energy)
     {
-        launchOne<kElec, true, false>(s, a, numBlocks, lds);
+        return launchOne<kElec, true, false>(s, a, numBlocks, lds, start, stop);
     }
-    else if (shiftForces)
+    if (shiftForces)
     {
-        launchOne<kElec, false, true>(s, a, numBlocks, lds);
+        return launchOne<kElec, false, true>(s, a, numBlocks, lds, start, stop);
     }
-    else
-    {
-        launchOne<kElec, false, false>(s, a, numBlocks, lds);
-    }
+    return launchOne<kElec, false, false>(s, a, numBlocks, lds, start, stop);
 }
 
 } // namespace
+
+hipError_t allowDynamicLds(const void* fn, int lds)
+{
+    if (lds <= 64 * 1024)
+    {
+        return hipSuccess;
+    }
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess)
+    {
+        return e;
+    }
+    static std::mutex                                 lock;
+    static std::map<std::pair<const void*, int>, int> allowed;
+    std::lock_guard<std::mutex>                       guard(lock);
+    int&                                              cur = allowed[{ fn, dev }];
+    if (lds > cur)
+    {
+        e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        if (e == hipSuccess)
+        {
+            cur = lds;
+        }
+    }
+    return e;
+}
 
 int nonbondedLdsBytes(int ntype, int tabN, int elec, int windowSlots)
 {
@@ -528,25 +579,23 @@ int nonbondedLdsBytes(int ntype, int tabN, int elec, int windowSlots)
     return L.end_ * 16;
 }
 
-void launchNonbonded(hipStream_t s, const KernelArgs& a, int elec, bool energy, bool shiftForces, int numBlocks)
+hipError_t launchNonbonded(hipStream_t s, const KernelArgs& a, int elec, bool energy, bool shiftForces, int numBlocks,
+                           hipEvent_t start, hipEvent_t stop)
 {
     if (a.ngroups == 0 || numBlocks == 0)
     {
-        return;
+        return hipSuccess;
     }
     const int lds = nonbondedLdsBytes(a.ntype, a.ic.tab_n, elec, a.windowSlots);
     if (elec == kElecRF)
     {
-        launchElec<kElecRF>(s, a, energy, shiftForces, numBlocks, lds);
+        return launchElec<kElecRF>(s, a, energy, shiftForces, numBlocks, lds, start, stop);
     }
-    else if (elec == kElecEwaldAna)
+    if (elec == kElecEwaldAna)
     {
-        launchElec<kElecEwaldAna>(s, a, energy, shiftForces, numBlocks, lds);
+        return launchElec<kElecEwaldAna>(s, a, energy, shiftForces, numBlocks, lds, start, stop);
     }
-    else
-    {
-        launchElec<kElecEwaldTable>(s, a, energy, shiftForces, numBlocks, lds);
-    }
+    return launchElec<kElecEwaldTable>(s, a, energy, shiftForces, numBlocks, lds, start, stop);
 }
 
 } // namespace nbx

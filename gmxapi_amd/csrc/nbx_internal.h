@@ -37,7 +37,9 @@ constexpr int kMaxColumnAtoms = 65536;
 // kernel work groups: at most this many consecutive i-clusters share a
 // wavefront's LDS j-force window; a group's j-entries are sorted in LDS by
 // the group pass (at most kGroupMaxJ)
-constexpr int kGroupSizeMax = 8;
+constexpr int kGroupSizeMax = 16;
+// candidate group sizes: kGroupSizeMax, kGroupSizeMax / 2, ..., 1
+constexpr int kGroupSizeCount = 5;
 constexpr int kGroupMaxJ    = 4096;
 // j-clusters per kernel tile, j-entries per work chunk (a multiple of it;
 // large enough that a chunk is a whole i-entry)
@@ -219,12 +221,22 @@ struct KernelArgs
     const int4* segInfo;   // per group: window staging layout (launchSegments)
     float4*     staging;   // the wavefronts' j-force windows, gathered after the kernel
     float4*  fnb;
-    // [ngroups + 4 * gridDim.x][2], zeroed before the launch: the energies of
-    // wavefront w in group g go to slot g + w
+    // zeroed before the launch: with waveEnergy, [4 * gridDim.x][2], the
+    // energies of wavefront w (its groups in order) in slot w; otherwise
+    // [ngroups + 4 * gridDim.x][2], those of wavefront w in group g in slot
+    // g + w (per-system sums of a batch)
     double*  energyPartial;
+    int      waveEnergy;
     float*   fshiftPartial; // [gridDim.x][45*3]
 };
-void launchNonbonded(hipStream_t s, const KernelArgs& a, int elec, bool energy, bool shiftForces, int numBlocks);
+// Lets kernel fn use lds bytes of dynamic LDS (above the 64 KiB default) on
+// the current device.  hipFuncSetAttribute is a host call that can hold up
+// the stream, so it runs once per kernel and device (and again only for a
+// larger size).
+hipError_t allowDynamicLds(const void* fn, int lds);
+// start/stop (either may be NULL) record the launch's own start and end
+hipError_t launchNonbonded(hipStream_t s, const KernelArgs& a, int elec, bool energy, bool shiftForces, int numBlocks,
+                           hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 int  nonbondedLdsBytes(int ntype, int tabN, int elec, int windowSlots);
 
 // -- adjoint.hip
@@ -275,9 +287,13 @@ void launchAdjointPrepare(hipStream_t s, int nslot, const int* atomIndices, cons
 void launchClearF4(hipStream_t s, float4* p, int n, double* d, int nd);
 void launchConvertX(hipStream_t s, int nslot, const int* atomIndices, const float* x, float4* xq);
 // after the pair kernel: the staged windows added to the cluster force
-// buffer, or (f given) written with it to the atom-order forces f
-void launchGatherForces(hipStream_t s, int ncluster, const int* gposOff, const int* gpos, const float4* staging,
-                        float4* fnb, const int* atomIndices, float* f);
+// buffer, or (f given) written with it to the atom-order forces f; with f
+// and virPart, also the -1/2 x (x) f virial term from the cluster-layout
+// coordinates xq, 9 doubles per workgroup.  Returns the workgroup count
+// (at most kGatherMaxBlocks).
+constexpr int kGatherMaxBlocks = 4096;
+int launchGatherForces(hipStream_t s, int ncluster, const int* gposOff, const int* gpos, const float4* staging,
+                       float4* fnb, const int* atomIndices, float* f, const float4* xq, double* virPart);
 // coordinates into the cluster layout (x may be NULL), cluster forces and
 // nd doubles at d cleared, in one launch
 void launchPrepareStep(hipStream_t s, int nslot, const int* atomIndices, const float* x, float4* xq, float4* fnb,
@@ -312,17 +328,21 @@ struct ExclusionArgs
     const int*       atomSystem;
     const float*     boxes;
     double*          atomEnergy;
+    // single system, pair form: nslot cluster slots (self terms) and npairs
+    // excluded pairs as cluster slots
+    int              nslot;
+    int              npairs;
+    const int2*      pairSlots;
 };
 constexpr int kExclMaxBlocks = 1024;
+// per-atom form (batches: per-atom energies or block partials)
 int launchExclusionEnergy(hipStream_t s, const ExclusionArgs& a);
-// exclusion energies (energy) and virial partials (virialPartial given) in
-// one pass over the atoms; returns the number of workgroup partials
-int launchAtomPass(hipStream_t s, const ExclusionArgs& a, bool energy, const float* x, const float* f,
-                   double* virialPartial);
+// pair form (one system): block partials; returns their count
+int launchExclusionPairEnergy(hipStream_t s, const ExclusionArgs& a);
+// the excluded pairs (atom indices, each pair once) as cluster slots
+void launchExclusionSlots(hipStream_t s, int npairs, const int2* pairs, const int* cells, int2* pairSlots);
 void launchBatchFinalize(hipStream_t s, int numSystems, const int* slotOff, const double* energyPartial,
                          const int* atomOff, const double* atomEnergy, double* energyOut);
-void launchVirial(hipStream_t s, int natoms, const float* x, const float* f, const double* fshift, const float4* shiftVec,
-                  double* virialPartial, int numBlocks, double* virialOut);
 void launchExpandPairs(hipStream_t s, const int4* ientry, int nci, const uint2* jentry, const float4* xq,
                        const int* atomIndices, const float4* shiftVec, float r2, unsigned long long* counter,
                        long long* out, long long cap);

@@ -74,7 +74,7 @@ typedef struct nbx_options
     int    ewald_potential_shift; /* 0: sh_ewald = 0 as nblib; 1: mdrun's shift */
     int    device;                /* HIP device ordinal */
     int    kernel_group_size;     /* i-clusters per kernel work group: 0 = largest
-                                     that fits LDS, else 1, 2, 4 or 8 (tuning) */
+                                     that fits LDS, else 1, 2, 4, 8 or 16 (tuning) */
     int    kernel_blocks_per_cu;  /* resident kernel workgroups per CU: 0 = as many
                                      as LDS allows up to 4 (tuning) */
 } nbx_options;
