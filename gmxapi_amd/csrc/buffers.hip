@@ -172,6 +172,7 @@ __global__ __launch_bounds__(kFinalizeThreads) void finalizeKernel(int numEnergy
     // shift-force component each; the last 9 (with the virial): one virial
     // component each, -1/2 sum x (x) f plus -1/2 sum_s s (x) fshift_s
     __shared__ double s[kFinalizeThreads / 64];
+    __shared__ float  sShift[kNumShifts];
     const int comp = blockIdx.x;
     double    acc  = 0.0;
     const int vir0 = doShift ? 2 + kNumShifts * 3 : 2;
@@ -179,23 +180,30 @@ __global__ __launch_bounds__(kFinalizeThreads) void finalizeKernel(int numEnergy
     {
         const int k = comp - vir0, d1 = k / 3, d2 = k % 3;
         acc         = strideSum(virPart, numVirPartials, 9, k);
-        // a thread per shift-force partial, its shifts unrolled so that the
-        // loads are independent
-        static_assert(kNumShifts % 3 == 0, "shift loop unrolled by 3");
-        for (int b = threadIdx.x; b < numShiftPartials; b += kFinalizeThreads)
+        // the shift-force partials as one array of elements i = 45 b + s
+        // (component d2 at 3 i + d2): consecutive threads read consecutive
+        // elements, eight independent loads per thread and round
+        if (threadIdx.x < kNumShifts)
         {
-            const float* p  = fshPart + static_cast<size_t>(b) * kNumShifts * 3 + d2;
-            double       t0 = 0.0, t1 = 0.0, t2 = 0.0;
-#pragma unroll
-            for (int sh = 0; sh < kNumShifts; sh += 3)
-            {
-                const float4 s0 = shiftVec[sh], s1 = shiftVec[sh + 1], s2 = shiftVec[sh + 2];
-                t0 += static_cast<double>(d1 == 0 ? s0.x : (d1 == 1 ? s0.y : s0.z)) * p[3 * sh];
-                t1 += static_cast<double>(d1 == 0 ? s1.x : (d1 == 1 ? s1.y : s1.z)) * p[3 * sh + 3];
-                t2 += static_cast<double>(d1 == 0 ? s2.x : (d1 == 1 ? s2.y : s2.z)) * p[3 * sh + 6];
-            }
-            acc -= 0.5 * ((t0 + t1) + t2);
+            const float4 sv       = shiftVec[threadIdx.x];
+            sShift[threadIdx.x]   = d1 == 0 ? sv.x : (d1 == 1 ? sv.y : sv.z);
         }
+        __syncthreads();
+        const int n    = numShiftPartials * kNumShifts;
+        double    t[8] = { 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0 };
+        for (int i0 = threadIdx.x; i0 < n; i0 += 8 * kFinalizeThreads)
+        {
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+            {
+                const int i = i0 + u * kFinalizeThreads;
+                if (i < n)
+                {
+                    t[u] += static_cast<double>(sShift[i % kNumShifts]) * fshPart[3 * static_cast<size_t>(i) + d2];
+                }
+            }
+        }
+        acc -= 0.5 * (((t[0] + t[1]) + (t[2] + t[3])) + ((t[4] + t[5]) + (t[6] + t[7])));
     }
     else if (comp < 2)
     {
