@@ -48,7 +48,8 @@ struct AdjLayout
         it   = ig + kAdjWaves * kGroupSizeMax * kClusterSize;
         win  = it + kAdjWaves * kGroupSizeMax;
         acc  = win + kAdjWaves * windowSlots * 4;
-        end_ = acc + ntype * ntype; // {dC6, dC12} per type pair as two doubles
+        // {dC6, dC12} per type pair as two doubles, in adjointAccCopies copies
+        end_ = acc + adjointAccCopies(ntype) * ntype * ntype;
     }
 };
 
@@ -56,6 +57,7 @@ struct AdjLayout
 struct IGrad
 {
     float x[4], y[4], z[4];
+    float qe[4]; // epsfac q_i
 };
 
 template<int kElec, bool kParams, bool kChecks, int kI>
@@ -63,7 +65,7 @@ __device__ __forceinline__ void adjointPairI(const IAtoms& I, const IGrad& Gi, c
                                              const unsigned mbits, const bool diag, const int jj,
                                              const InteractionConst& ic, const float2* __restrict__ sNbfp,
                                              const float4* __restrict__ sTab, double* __restrict__ sAcc,
-                                             float* ai, float4& aj)
+                                             const int accStride, const int accCopies, float* ai, float4& aj)
 {
     const float dx       = I.x[kI] - J.x;
     const float dy       = I.y[kI] - J.y;
@@ -94,8 +96,10 @@ __device__ __forceinline__ void adjointPairI(const IAtoms& I, const IGrad& Gi, c
     const float  rinv6 = interact * rinvsq * rinvsq * rinvsq;
     const float  r8    = rinv6 * rinvsq;
     const float  r14   = rinv6 * r8;
-    const float  sLJ   = c.y * r14 - c.x * r8;
-    const float  dsLJ  = (4.0f * c.x * r8 - 7.0f * c.y * r14) * rinvsq;
+    const float  a12   = c.y * r14;
+    const float  a6    = c.x * r8;
+    const float  sLJ   = a12 - a6;
+    const float  dsLJ  = fmaf(a12, -7.0f, 4.0f * a6) * rinvsq;
 
     // Coulomb per unit epsfac q_i q_j: h and h' = dh/d(r^2)
     float hC, dhC;
@@ -131,30 +135,30 @@ __device__ __forceinline__ void adjointPairI(const IAtoms& I, const IGrad& Gi, c
         hC              = smask * (interact * rinv3 - ic.beta3 * G);
         dhC             = smask * (-1.5f * interact * rinv3 * rinvsq - ic.beta3 * ic.beta2 * dG);
     }
-    const float qi  = I.q[kI] * ic.epsfac; // epsfac q_i
-    const float qq  = I.q[kI] * J.w;       // epsfac q_i q_j
-    const float s   = sLJ + qq * hC;
-    const float sp  = dsLJ + qq * dhC;
+    const float qq  = I.q[kI] * J.w; // epsfac q_i q_j
+    const float s   = fmaf(qq, hC, sLJ);
+    const float sp  = fmaf(qq, dhC, dsLJ);
     const float gdx = Gi.x[kI] - gj.x;
     const float gdy = Gi.y[kI] - gj.y;
     const float gdz = Gi.z[kI] - gj.z;
     const float u   = gdx * dx + gdy * dy + gdz * dz;
     const float t   = 2.0f * sp * u;
-    const float wx  = s * gdx + t * dx;
-    const float wy  = s * gdy + t * dy;
-    const float wz  = s * gdz + t * dz;
+    const float wx  = fmaf(s, gdx, t * dx);
+    const float wy  = fmaf(s, gdy, t * dy);
+    const float wz  = fmaf(s, gdz, t * dz);
     const float uh  = u * hC;
     ai[4 * kI + 0] += wx;
     ai[4 * kI + 1] += wy;
     ai[4 * kI + 2] += wz;
-    ai[4 * kI + 3] += uh * J.w;
+    ai[4 * kI + 3] = fmaf(uh, J.w, ai[4 * kI + 3]);
     aj.x -= wx;
     aj.y -= wy;
     aj.z -= wz;
-    aj.w += uh * qi;
+    aj.w = fmaf(uh, Gi.qe[kI], aj.w);
     if (kParams)
     {
-        double* a = sAcc + 2 * (I.tOff[kI] + J.type);
+        // lanes jj of each j-cluster spread over the accumulator copies
+        double* a = sAcc + 2 * (I.tOff[kI] + J.type) + accStride * (jj & (accCopies - 1));
         atomicAdd(a, static_cast<double>(-6.0f * u * r8));
         atomicAdd(a + 1, static_cast<double>(12.0f * u * r14));
     }
@@ -164,16 +168,20 @@ template<int kElec, bool kParams, bool kChecks>
 __device__ __forceinline__ void adjointTile(const IAtoms& I, const IGrad& Gi, const JAtom& J, const float4 gj,
                                             const unsigned mbits, const bool diag, const int jj,
                                             const InteractionConst& ic, const float2* __restrict__ sNbfp,
-                                            const float4* __restrict__ sTab, double* __restrict__ sAcc, float* ai,
-                                            float4& aj)
+                                            const float4* __restrict__ sTab, double* __restrict__ sAcc,
+                                            const int accStride, const int accCopies, float* ai, float4& aj)
 {
-    adjointPairI<kElec, kParams, kChecks, 0>(I, Gi, J, gj, mbits, diag, jj, ic, sNbfp, sTab, sAcc, ai, aj);
+    adjointPairI<kElec, kParams, kChecks, 0>(I, Gi, J, gj, mbits, diag, jj, ic, sNbfp, sTab, sAcc, accStride,
+                                             accCopies, ai, aj);
     __builtin_amdgcn_sched_barrier(0);
-    adjointPairI<kElec, kParams, kChecks, 1>(I, Gi, J, gj, mbits, diag, jj, ic, sNbfp, sTab, sAcc, ai, aj);
+    adjointPairI<kElec, kParams, kChecks, 1>(I, Gi, J, gj, mbits, diag, jj, ic, sNbfp, sTab, sAcc, accStride,
+                                             accCopies, ai, aj);
     __builtin_amdgcn_sched_barrier(0);
-    adjointPairI<kElec, kParams, kChecks, 2>(I, Gi, J, gj, mbits, diag, jj, ic, sNbfp, sTab, sAcc, ai, aj);
+    adjointPairI<kElec, kParams, kChecks, 2>(I, Gi, J, gj, mbits, diag, jj, ic, sNbfp, sTab, sAcc, accStride,
+                                             accCopies, ai, aj);
     __builtin_amdgcn_sched_barrier(0);
-    adjointPairI<kElec, kParams, kChecks, 3>(I, Gi, J, gj, mbits, diag, jj, ic, sNbfp, sTab, sAcc, ai, aj);
+    adjointPairI<kElec, kParams, kChecks, 3>(I, Gi, J, gj, mbits, diag, jj, ic, sNbfp, sTab, sAcc, accStride,
+                                             accCopies, ai, aj);
 }
 
 template<int kElec, bool kParams>
@@ -187,6 +195,8 @@ __global__ __launch_bounds__(kAdjThreads) void adjointKernel(const AdjointArgs a
     float2*         sNbfp = reinterpret_cast<float2*>(smem4 + L.nbfp);
     float4*         sTab  = smem4 + L.tab;
     double*         sAcc  = reinterpret_cast<double*>(smem4 + L.acc);
+    const int       accCopies = adjointAccCopies(ntype);
+    const int       accStride = 2 * ntype2; // doubles per accumulator copy
 
     for (int t = threadIdx.x; t < ntype2; t += kAdjThreads)
     {
@@ -196,7 +206,7 @@ __global__ __launch_bounds__(kAdjThreads) void adjointKernel(const AdjointArgs a
     {
         sTab[t] = a.tabFDV0[t];
     }
-    for (int t = threadIdx.x; t < 2 * ntype2; t += kAdjThreads)
+    for (int t = threadIdx.x; t < accCopies * accStride; t += kAdjThreads)
     {
         sAcc[t] = 0.0;
     }
@@ -289,6 +299,7 @@ __global__ __launch_bounds__(kAdjThreads) void adjointKernel(const AdjointArgs a
                 Gi.x[i]        = uniformFloat(q.x);
                 Gi.y[i]        = uniformFloat(q.y);
                 Gi.z[i]        = uniformFloat(q.z);
+                Gi.qe[i]       = uniformFloat(p.w * ic.epsfac);
             }
             sv = make_float4(uniformFloat(sf.x), uniformFloat(sf.y), uniformFloat(sf.z), 0.0f);
 #pragma unroll
@@ -311,11 +322,12 @@ __global__ __launch_bounds__(kAdjThreads) void adjointKernel(const AdjointArgs a
             {
                 const bool diag = central && static_cast<int>(e.x) == ci;
                 adjointTile<kElec, kParams, true>(I, Gi, J, gj, (e.y & 0xffffu) >> jj, diag, jj, ic, sNbfp, sTab,
-                                                  sAcc, ai, aj);
+                                                  sAcc, accStride, accCopies, ai, aj);
             }
             else
             {
-                adjointTile<kElec, kParams, false>(I, Gi, J, gj, 0u, false, jj, ic, sNbfp, sTab, sAcc, ai, aj);
+                adjointTile<kElec, kParams, false>(I, Gi, J, gj, 0u, false, jj, ic, sNbfp, sTab, sAcc, accStride,
+                                                   accCopies, ai, aj);
             }
             const unsigned slot = e.y >> 16;
             if (slot != kSlotOverflow)
@@ -401,9 +413,14 @@ __global__ __launch_bounds__(kAdjThreads) void adjointKernel(const AdjointArgs a
     if (kParams)
     {
         __syncthreads();
-        for (int t = threadIdx.x; t < 2 * ntype2; t += kAdjThreads)
+        for (int t = threadIdx.x; t < accStride; t += kAdjThreads)
         {
-            a.paramPartial[static_cast<size_t>(blockIdx.x) * 2 * ntype2 + t] = sAcc[t];
+            double v = sAcc[t];
+            for (int k = 1; k < accCopies; k++)
+            {
+                v += sAcc[k * accStride + t];
+            }
+            a.paramPartial[static_cast<size_t>(blockIdx.x) * accStride + t] = v;
         }
     }
 }

@@ -252,7 +252,7 @@ void buildWorkGroups(nbx_context* c, WorkLayout& L, bool adjoint)
         {
             si++;
         }
-        const int  budget   = adjoint ? kAdjointLayoutBudget : kKernelLdsBudget;
+        const int  budget   = adjoint ? adjointLayoutBudget(c->ntype) : kKernelLdsBudget;
         if (!fallback && forced > 0 && G != forced)
         {
             continue;

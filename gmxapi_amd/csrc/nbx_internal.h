@@ -57,6 +57,20 @@ constexpr int kAdjointLdsBudget = 160 * 1024;
 // LDS per two-wavefront adjoint workgroup that its group size is chosen for
 // (three resident workgroups per CU)
 constexpr int kAdjointLayoutBudget = 48 * 1024;
+// copies of the adjoint's per-workgroup C6/C12 accumulators in LDS (lanes
+// spread over them, fewer lanes per address in each LDS atomic); the layout
+// budget grows by the copies beyond the first
+#ifndef NBX_ADJ_ACC_COPIES
+#define NBX_ADJ_ACC_COPIES 2
+#endif
+__host__ __device__ inline int adjointAccCopies(int ntype)
+{
+    return ntype <= 24 ? NBX_ADJ_ACC_COPIES : 1;
+}
+__host__ __device__ inline int adjointLayoutBudget(int ntype)
+{
+    return kAdjointLayoutBudget + (adjointAccCopies(ntype) - 1) * ntype * ntype * 16;
+}
 // rsq floor for float (pairlist.h:146): r > 6.2e-4 nm
 constexpr float kMinRsq = 3.82e-07f;
 
