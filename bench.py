@@ -207,6 +207,54 @@ def bench_forward_backward(stream, reps=20, ewald="analytical"):
     }
 
 
+def bench_prune(stream, reps=10):
+    """Dynamic pruning on the 288k water box searched with a 0.1 nm buffer
+    (rlist 1.1 nm) and pruned back to the 1.0 nm cut-off: the search and the
+    prune (both including the rebuild of the kernel's work layout), and the
+    F+E+shift step on the list as searched and as pruned."""
+    import numpy as np
+    import torch
+
+    from gmxapi_amd import nbx, systems
+
+    s = systems.water_box(96000, 14.2, seed=systems.WATER_SEED + 1, rc=1.0, rlist=1.1)
+    ctx = nbx.context_for(s)
+    ctx.set_stream(stream.cuda_stream)
+    dev = torch.device("cuda:0")
+    x = torch.from_numpy(s.x.astype(np.float32).reshape(-1)).to(dev)
+    f = torch.empty_like(x)
+    flags = nbx.COMPUTE_FORCES | nbx.COMPUTE_ENERGY | nbx.COMPUTE_SHIFT_FORCES
+
+    def step():
+        ctx.compute_device(x.data_ptr(), f.data_ptr(), flags)
+
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record(stream)
+    ctx.construct_pairlist()
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    search_ms = ev[0].elapsed_time(ev[1])
+    outer_cj = ctx.info()["num_j_entries"]
+    outer_ms = _time(step, stream, reps)
+    ev[0].record(stream)
+    ctx.prune_pairlist(s.rcoulomb)
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    prune_ms = ev[0].elapsed_time(ev[1])
+    inner_cj = ctx.info()["num_j_entries"]
+    inner_ms = _time(step, stream, reps)
+    return {
+        "workload": "water-288k searched at rlist 1.1 nm, pruned to rc 1.0 nm (static coordinates)",
+        "search_ms": search_ms,
+        "prune_ms": prune_ms,
+        "cluster_pairs_searched": int(outer_cj),
+        "cluster_pairs_pruned": int(inner_cj),
+        "step_ms_searched_list": outer_ms,
+        "step_ms_pruned_list": inner_ms,
+        "note": "search and prune times include rebuilding the kernel's work layout and schedule",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -371,6 +419,7 @@ def main():
         t0 = time.perf_counter()
         out["batched"] = bench_ensemble(args.ewald, stream)
         out["fwd_bwd"] = bench_forward_backward(stream)
+        out["prune"] = bench_prune(stream)
         log(f"[bench] batched and forward+backward parts in {time.perf_counter() - t0:.1f} s")
     if not args.no_cpu_baseline:
         t_step, t_setup, nrep = cpu_baseline(s)

@@ -910,6 +910,25 @@ int nbx_put_on_grid(nbx_context* c, const float* box, const float* x, int xOnDev
     });
 }
 
+// After a search or a prune: the pair kernel's work layout and schedule and
+// the per-step output buffers sized for it.
+static void finishList(nbx_context* c)
+{
+    buildWorkGroups(c, c->lay, false);
+    scheduleLayout(c, c->lay, false);
+    const int numWaves = c->lay.numBlocks * kKernelWaves;
+    c->energyPartial.ensure(2 * static_cast<size_t>(c->lay.ngroups + numWaves));
+    c->exclPartial.ensure(kExclMaxBlocks);
+    c->fshiftPartial.ensure(static_cast<size_t>(c->lay.numBlocks) * kNumShifts * 3);
+    c->energy.ensure(2);
+    c->fshift.ensure(kNumShifts * 3);
+    c->virialPartial.ensure(9 * static_cast<size_t>(kGatherMaxBlocks));
+    c->virial.ensure(9);
+    c->listValid  = true;
+    c->adjValid   = false;
+    c->haveEnergy = c->haveShift = c->haveVirial = false;
+}
+
 int nbx_construct_pairlist(nbx_context* c)
 {
     if (!c)
@@ -956,19 +975,72 @@ int nbx_construct_pairlist(nbx_context* c)
         launchSearch(c->stream, a, true);
         check(hipGetLastError(), "search kernels");
         launchExclusionSlots(c->stream, c->numExclPairs, c->exclPairs.p, c->cells.p, c->exclPairSlots.p);
-        buildWorkGroups(c, c->lay, false);
-        scheduleLayout(c, c->lay, false);
-        const int numWaves = c->lay.numBlocks * kKernelWaves;
-        c->energyPartial.ensure(2 * static_cast<size_t>(c->lay.ngroups + numWaves));
-        c->exclPartial.ensure(kExclMaxBlocks);
-        c->fshiftPartial.ensure(static_cast<size_t>(c->lay.numBlocks) * kNumShifts * 3);
-        c->energy.ensure(2);
-        c->fshift.ensure(kNumShifts * 3);
-        c->virialPartial.ensure(9 * static_cast<size_t>(kGatherMaxBlocks));
-        c->virial.ensure(9);
-        c->listValid = true;
-        c->adjValid  = false;
-        c->haveEnergy = c->haveShift = c->haveVirial = false;
+        c->haveOuter = false;
+        finishList(c);
+    });
+}
+
+int nbx_prune_pairlist(nbx_context* c, double rlistInner)
+{
+    if (!c)
+    {
+        return fail(NBX_ERROR_INPUT, "NULL context");
+    }
+    return guarded([&] {
+        requireList(c);
+        if (!(rlistInner >= c->rcoulomb && rlistInner <= c->rlist))
+        {
+            throw InputError("the inner pair-list cut-off must lie between the interaction cut-off and rlist");
+        }
+        setDevice(c);
+        const int nc = c->ncluster;
+        if (!c->haveOuter)
+        {
+            // keep the list as searched: every prune starts from it
+            c->nciOuter = c->nci;
+            c->ncjOuter = c->ncj;
+            c->offIOuter.ensure(nc + 1);
+            c->ientryOuter.ensure(std::max(c->nci, 1));
+            c->jentryOuter.ensure(std::max(c->ncj, 1));
+            check(hipMemcpyAsync(c->offIOuter.p, c->offI.p, sizeof(int) * (nc + 1), hipMemcpyDeviceToDevice,
+                                 c->stream),
+                  "keep outer list");
+            check(hipMemcpyAsync(c->ientryOuter.p, c->ientry.p, sizeof(int4) * c->nci, hipMemcpyDeviceToDevice,
+                                 c->stream),
+                  "keep outer list");
+            check(hipMemcpyAsync(c->jentryOuter.p, c->jentry.p, sizeof(uint2) * c->ncj, hipMemcpyDeviceToDevice,
+                                 c->stream),
+                  "keep outer list");
+            c->haveOuter = true;
+        }
+        const float r = static_cast<float>(rlistInner);
+        PruneArgs   a{};
+        a.ncluster    = nc;
+        a.rlist2      = r * r;
+        a.xq          = c->xq.p;
+        a.shiftVec    = c->shiftVec.p;
+        a.offIOuter   = c->offIOuter.p;
+        a.ientryOuter = c->ientryOuter.p;
+        a.jentryOuter = c->jentryOuter.p;
+        a.countI      = c->countI.p;
+        a.countJ      = c->countJ.p;
+        launchPrune(c->stream, a, false);
+        launchExclusiveScan(c->stream, nc, c->countJ.p, c->offJ.p, c->totals.p);
+        launchExclusiveScan(c->stream, nc, c->countI.p, c->offI.p, c->totals.p + 1);
+        int tot[2] = { 0, 0 };
+        check(hipMemcpyAsync(tot, c->totals.p, sizeof(int) * 2, hipMemcpyDeviceToHost, c->stream), "read totals");
+        check(hipStreamSynchronize(c->stream), "prune sync");
+        c->ncj = tot[0];
+        c->nci = tot[1];
+        c->ientry.ensure(std::max(c->nci, 1));
+        c->jentry.ensure(std::max(c->ncj, 1));
+        a.offI   = c->offI.p;
+        a.offJ   = c->offJ.p;
+        a.ientry = c->ientry.p;
+        a.jentry = c->jentry.p;
+        launchPrune(c->stream, a, true);
+        check(hipGetLastError(), "prune kernels");
+        finishList(c);
     });
 }
 

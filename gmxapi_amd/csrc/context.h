@@ -255,6 +255,13 @@ struct nbx_context
     nbx::DevBuf<int>    countJ, countI, offJ, offI, totals;
     nbx::DevBuf<int4>   ientry;
     nbx::DevBuf<uint2>  jentry, jentryTmp;
+    // the list as searched, kept by the first prune after a search (every
+    // prune starts from it); the list above is then the pruned one
+    bool                haveOuter = false;
+    int                 nciOuter = 0, ncjOuter = 0;
+    nbx::DevBuf<int>    offIOuter;
+    nbx::DevBuf<int4>   ientryOuter;
+    nbx::DevBuf<uint2>  jentryOuter;
     // kernel work layouts: the pair kernel's, and the adjoint's (its own
     // group size, sized for its larger window per cluster slot; built on
     // the first adjoint call after a search)

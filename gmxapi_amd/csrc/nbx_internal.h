@@ -113,6 +113,27 @@ struct GridDims
     float invBoxZ;
 };
 
+// -- prune.hip: the outer list (as searched) pruned to rlist2 with the
+// current coordinates xq, as a count pass (countI, countJ per cluster) and,
+// after exclusive scans into offI, offJ, a fill pass
+struct PruneArgs
+{
+    int           ncluster;
+    float         rlist2;
+    const float4* xq;
+    const float4* shiftVec;
+    const int*    offIOuter;
+    const int4*   ientryOuter;
+    const uint2*  jentryOuter;
+    int*          countI;
+    int*          countJ;
+    const int*    offI;
+    const int*    offJ;
+    int4*         ientry;
+    uint2*        jentry;
+};
+void launchPrune(hipStream_t s, const PruneArgs& a, bool fill);
+
 // -- grid.hip
 void launchGridColumns(hipStream_t s, const GridDims& g, const float* x, int* atomCol, int* atomRank, int* cxy_na);
 void launchGridScan(hipStream_t s, int ncol, const int* cxy_na, int* cxy_ind, int* ncluster_out);

@@ -128,6 +128,7 @@ def library() -> C.CDLL:
     lib.nbx_set_topology.argtypes = [vp, i, vp, vp, vp, vp]
     lib.nbx_put_on_grid.argtypes = [vp, vp, vp, i]
     lib.nbx_construct_pairlist.argtypes = [vp]
+    lib.nbx_prune_pairlist.argtypes = [vp, d]
     lib.nbx_convert_coordinates.argtypes = [vp, vp, i]
     lib.nbx_dispatch_kernel.argtypes = [vp, u]
     lib.nbx_add_nbat_f_to_f.argtypes = [vp, vp, i]
@@ -259,6 +260,11 @@ class Context:
     def construct_pairlist(self):
         self._check(self._lib.nbx_construct_pairlist(self._h), "nbx_construct_pairlist")
 
+    def prune_pairlist(self, r_inner: float):
+        """Dynamic pruning (dispatchPruneKernelCpu, nbnxm.h:312-313): the list
+        as searched pruned to r_inner with the current coordinates."""
+        self._check(self._lib.nbx_prune_pairlist(self._h, float(r_inner)), "nbx_prune_pairlist")
+
     def compute(self, x, energy=False, shift_forces=False, virial=False) -> np.ndarray:
         """Forces in atom order (host arrays, synchronous)."""
         xa = np.ascontiguousarray(x, np.float32).reshape(-1)
@@ -300,6 +306,13 @@ class Context:
         grad_c6/grad_c12 are (T, T) float64 device arrays (or None)."""
         self._check(self._lib.nbx_compute_adjoint(self._h, x_ptr, g_ptr, grad_x_ptr, grad_q_ptr, grad_c6_ptr,
                                                   grad_c12_ptr, 1), "nbx_compute_adjoint")
+
+    def convert_coordinates(self, x):
+        """nonbonded_verlet_t::convertCoordinates (nbnxm.cpp:129-141) from a
+        host array (N x 3), synchronous."""
+        xx = np.ascontiguousarray(x, np.float32).reshape(-1)
+        self._check(self._lib.nbx_convert_coordinates(self._h, _ptr(xx), 0), "nbx_convert_coordinates")
+        self.synchronize()
 
     def convert_coordinates_device(self, x_ptr: int):
         self._check(self._lib.nbx_convert_coordinates(self._h, x_ptr, 1), "nbx_convert_coordinates")

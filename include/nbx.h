@@ -135,6 +135,16 @@ int nbx_put_on_grid(nbx_context* context, const float* box, const float* x, int 
  * local intra-grid list with full periodicity. */
 int nbx_construct_pairlist(nbx_context* context);
 
+/* nonbonded_verlet_t::dispatchPruneKernelCpu (nbnxm.h:312-313) with the
+ * plain-C prune kernel nbnxn_kernel_prune_ref (kernel_ref_prune.cpp:39-123):
+ * the list as searched (at rlist) pruned to the j-clusters with at least one
+ * atom pair closer than rlist_inner with the current coordinates (those of
+ * the last put_on_grid, convert_coordinates or compute), keeping their order
+ * and exclusion masks and dropping emptied i-entries.  Every prune starts
+ * from the list as searched; the next construct_pairlist replaces it.
+ * rcoulomb <= rlist_inner <= rlist. */
+int nbx_prune_pairlist(nbx_context* context, double rlist_inner);
+
 /* nonbonded_verlet_t::convertCoordinates (nbnxm.cpp:129-141). */
 int nbx_convert_coordinates(nbx_context* context, const float* x, int x_on_device);
 

@@ -318,6 +318,12 @@ typedef struct
     int*      ci; /* 4 per entry: ci, shift, cj_ind_start, cj_ind_end */
     int*      cj;
     unsigned* excl;
+    /* the list as searched (at rlist), kept by the first prune after a
+     * search so that every prune starts from it (nbl->ciOuter/cjOuter) */
+    int       haveOuter, nciO, ncjO;
+    int*      ciO;
+    int*      cjO;
+    unsigned* exclO;
     /* outputs */
     real*  fnb; /* ncells*4*3 */
     real   fshift[ORC_SHIFTS * 3];
@@ -403,6 +409,9 @@ void orc_destroy(orc_sys* s)
     free(s->ci);
     free(s->cj);
     free(s->excl);
+    free(s->ciO);
+    free(s->cjO);
+    free(s->exclO);
     free(s->fnb);
     free(s);
 }
@@ -1024,6 +1033,7 @@ int orc_construct_pairlist(orc_sys* s)
 {
     s->nci         = 0;
     s->ncj         = 0;
+    s->haveOuter   = 0;
     real  rlist    = s->rlist;
     real  rlist2   = rlist * rlist;
     /* boundingbox_only_distance2, pairlist.cpp:2841-2878 */
@@ -1228,6 +1238,73 @@ int orc_construct_pairlist(orc_sys* s)
                     }
                 }
             }
+        }
+    }
+    return s->nci;
+}
+
+/* nbnxn_kernel_prune_ref, kernel_ref_prune.cpp:39-123: the list as searched
+ * (kept on the first call after a search) pruned to the j-clusters with at
+ * least one atom pair, filler atoms included, whose squared distance (i-atom
+ * plus shift, minus j-atom, in the canonical rounding) is below
+ * rlistInner^2, using the current coordinates; entries keep their order and
+ * exclusion masks, i-entries left without j-clusters are dropped. */
+int orc_prune_pairlist(orc_sys* s, double rlistInner)
+{
+    if (!s->haveOuter)
+    {
+        s->nciO  = s->nci;
+        s->ncjO  = s->ncj;
+        s->ciO   = (int*)xrealloc(s->ciO, sizeof(int) * 4 * (s->nci + 1));
+        s->cjO   = (int*)xrealloc(s->cjO, sizeof(int) * (s->ncj + 1));
+        s->exclO = (unsigned*)xrealloc(s->exclO, sizeof(unsigned) * (s->ncj + 1));
+        memcpy(s->ciO, s->ci, sizeof(int) * 4 * s->nci);
+        memcpy(s->cjO, s->cj, sizeof(int) * s->ncj);
+        memcpy(s->exclO, s->excl, sizeof(unsigned) * s->ncj);
+        s->haveOuter = 1;
+    }
+    const real r   = (real)rlistInner;
+    const real r2  = r * r;
+    s->nci         = 0;
+    s->ncj         = 0;
+    for (int e = 0; e < s->nciO; e++)
+    {
+        const int*  ie = s->ciO + 4 * e;
+        const int   ci = ie[0], sh = ie[1];
+        const real* sv = s->shiftvec[sh];
+        real        xi[ORC_NC][3];
+        for (int i = 0; i < ORC_NC; i++)
+        {
+            for (int d = 0; d < 3; d++)
+            {
+                xi[i][d] = s->xnb[(ci * ORC_NC + i) * 3 + d] + sv[d];
+            }
+        }
+        add_ci(s, ci, sh);
+        for (int k = ie[2]; k < ie[3]; k++)
+        {
+            const int cj      = s->cjO[k];
+            int       inRange = 0;
+            for (int i = 0; i < ORC_NC && !inRange; i++)
+            {
+                for (int j = 0; j < ORC_NC; j++)
+                {
+                    if (pair_d2(xi[i], s->xnb + (cj * ORC_NC + j) * 3) < r2)
+                    {
+                        inRange = 1;
+                    }
+                }
+            }
+            if (inRange)
+            {
+                add_cj(s, cj, s->exclO[k]);
+            }
+        }
+        int* last = s->ci + 4 * (s->nci - 1);
+        last[3]   = s->ncj;
+        if (last[3] == last[2])
+        {
+            s->nci--;
         }
     }
     return s->nci;

@@ -68,6 +68,8 @@ def _lib(precision: str) -> C.CDLL:
     lib.orc_get_shiftvec.argtypes = [vp, vp]
     lib.orc_construct_pairlist.restype = C.c_int
     lib.orc_construct_pairlist.argtypes = [vp]
+    lib.orc_prune_pairlist.restype = C.c_int
+    lib.orc_prune_pairlist.argtypes = [vp, C.c_double]
     lib.orc_num_ci.restype = C.c_int
     lib.orc_num_ci.argtypes = [vp]
     lib.orc_num_cj.restype = C.c_int
@@ -255,6 +257,15 @@ class Oracle:
         ex = np.zeros(max(ncj, 1), np.uint32)
         self.lib.orc_get_list(self.h, _p(ci), _p(cj), _p(ex))
         return ci[:4 * nci].reshape(-1, 4), cj[:ncj], ex[:ncj]
+
+    def prune(self, r_inner: float, x=None) -> int:
+        """nbnxn_kernel_prune_ref (kernel_ref_prune.cpp:39-123): prune the
+        list as searched to r_inner with the current coordinates (x, atom
+        order, converted first when given). Returns the i-entry count."""
+        if x is not None:
+            xx = np.ascontiguousarray(x, self.dt).reshape(-1)
+            self.lib.orc_convert_coordinates(self.h, _p(xx))
+        return int(self.lib.orc_prune_pairlist(self.h, float(r_inner)))
 
     def shift_vectors(self) -> np.ndarray:
         out = np.zeros(45 * 3, self.dt)
