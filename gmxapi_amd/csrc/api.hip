@@ -1133,7 +1133,7 @@ static void enqueueStep(nbx_context* c, unsigned flags, const float* xd, float* 
     const int numVir =
             launchGatherForces(c->stream, c->ncluster, c->lay.gather.gposOff.p, c->lay.gather.gpos.p,
                                c->lay.gather.staging.p, useFnb ? c->fnb.p : nullptr, c->atomIndices.p, fOut,
-                               c->xq.p, virial ? c->virialPartial.p : nullptr);
+                               (flags & NBX_COMPUTE_ADD_FORCES) != 0, c->xq.p, virial ? c->virialPartial.p : nullptr);
     const int numExcl = energy ? launchExclusionPairEnergy(c->stream, e) : 0;
     if (energy || shift || virial)
     {
@@ -1207,6 +1207,11 @@ int nbx_compute(nbx_context* c, const float* x, float* f, unsigned flags, int on
         {
             c->fOut.ensure(3 * static_cast<size_t>(c->natoms));
             fd = c->fOut.p;
+            if (flags & NBX_COMPUTE_ADD_FORCES)
+            {
+                check(hipMemcpyAsync(fd, f, sizeof(float) * 3 * c->natoms, hipMemcpyHostToDevice, c->stream),
+                      "upload f");
+            }
         }
         enqueueStep(c, flags, xd, fd);
         if (!onDevice)

@@ -308,7 +308,7 @@ void enqueueCompute(nbx_batch* b, const float* x, float* f, unsigned flags)
     a.fshiftPartial = nullptr;
     check(launchNonbonded(b->stream, a, c0->elec, energy, false, b->numBlocks), "nonbonded kernel");
     launchGatherForces(b->stream, b->totClusters, b->gather.gposOff.p, b->gather.gpos.p, b->gather.staging.p,
-                       b->fnb.p, b->atomIndices.p, f, nullptr, nullptr);
+                       b->fnb.p, b->atomIndices.p, f, (flags & NBX_COMPUTE_ADD_FORCES) != 0, nullptr, nullptr);
     if (energy)
     {
         ExclusionArgs e{};

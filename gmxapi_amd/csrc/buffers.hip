@@ -445,7 +445,7 @@ __global__ __launch_bounds__(kGatherThreads) void gatherForcesKernel(int ncluste
                                                                      const float4* __restrict__ staging,
                                                                      float4* __restrict__ fnb,
                                                                      const int* __restrict__ atomIndices,
-                                                                     float* __restrict__ fOut,
+                                                                     float* __restrict__ fOut, int addToF,
                                                                      const float4* __restrict__ xq,
                                                                      double* __restrict__ virPart)
 {
@@ -494,8 +494,9 @@ __global__ __launch_bounds__(kGatherThreads) void gatherForcesKernel(int ncluste
                 const int a = atomIndices[slot];
                 if (a >= 0)
                 {
-                    const float fv       = (fnb ? *dst : 0.0f) + r;
-                    fOut[3 * a + idx % 3] = fv;
+                    const float fv = (fnb ? *dst : 0.0f) + r;
+                    float*      fo = fOut + 3 * a + idx % 3;
+                    *fo            = addToF ? *fo + fv : fv;
                     if (virPart)
                     {
                         const float4 p = xq[slot];
@@ -705,7 +706,7 @@ void launchConvertX(hipStream_t s, int nslot, const int* atomIndices, const floa
 }
 
 int launchGatherForces(hipStream_t s, int ncluster, const int* gposOff, const int* gpos, const float4* staging,
-                       float4* fnb, const int* atomIndices, float* f, const float4* xq, double* virPart)
+                       float4* fnb, const int* atomIndices, float* f, bool addToF, const float4* xq, double* virPart)
 {
     if (ncluster == 0)
     {
@@ -715,12 +716,12 @@ int launchGatherForces(hipStream_t s, int ncluster, const int* gposOff, const in
     if (f)
     {
         gatherForcesKernel<true><<<blocks, kGatherThreads, 0, s>>>(ncluster, gposOff, gpos, staging, fnb, atomIndices,
-                                                                    f, xq, virPart);
+                                                                    f, addToF ? 1 : 0, xq, virPart);
     }
     else
     {
         gatherForcesKernel<false><<<blocks, kGatherThreads, 0, s>>>(ncluster, gposOff, gpos, staging, fnb, nullptr,
-                                                                     nullptr, nullptr, nullptr);
+                                                                     nullptr, 0, nullptr, nullptr);
     }
     return blocks;
 }

@@ -322,13 +322,15 @@ void launchAdjointPrepare(hipStream_t s, int nslot, const int* atomIndices, cons
 void launchClearF4(hipStream_t s, float4* p, int n, double* d, int nd);
 void launchConvertX(hipStream_t s, int nslot, const int* atomIndices, const float* x, float4* xq);
 // after the pair kernel: the staged windows added to the cluster force
-// buffer, or (f given) written with it to the atom-order forces f; with f
+// buffer, or (f given) written with it to the atom-order forces f (added to
+// them with addToF); with f
 // and virPart, also the -1/2 x (x) f virial term from the cluster-layout
 // coordinates xq, 9 doubles per workgroup.  Returns the workgroup count
 // (at most kGatherMaxBlocks).
 constexpr int kGatherMaxBlocks = 4096;
 int launchGatherForces(hipStream_t s, int ncluster, const int* gposOff, const int* gpos, const float4* staging,
-                       float4* fnb, const int* atomIndices, float* f, const float4* xq, double* virPart);
+                       float4* fnb, const int* atomIndices, float* f, bool addToF, const float4* xq,
+                       double* virPart);
 // coordinates into the cluster layout (x may be NULL), cluster forces and
 // nd doubles at d cleared, in one launch
 void launchPrepareStep(hipStream_t s, int nslot, const int* atomIndices, const float* x, float4* xq, float4* fnb,

@@ -31,6 +31,7 @@ COMPUTE_FORCES = 1
 COMPUTE_ENERGY = 2
 COMPUTE_SHIFT_FORCES = 4
 COMPUTE_VIRIAL = 8
+COMPUTE_ADD_FORCES = 16  # add to f instead of overwriting it
 
 
 class NbxError(RuntimeError):

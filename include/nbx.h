@@ -56,6 +56,11 @@ extern "C" {
 #define NBX_COMPUTE_ENERGY 2u
 #define NBX_COMPUTE_SHIFT_FORCES 4u
 #define NBX_COMPUTE_VIRIAL 8u
+/* forces added to f instead of overwriting it: atomdata_add_nbat_f_to_f
+ * without nblib's zero fill (nbnxm.cpp:162-180), and on device buffers
+ * atomdata_add_nbat_f_to_f_gpu (nbnxm.h:345-359), for a caller that sums
+ * several force sources in one buffer */
+#define NBX_COMPUTE_ADD_FORCES 16u
 
 typedef struct nbx_context nbx_context;
 
@@ -159,7 +164,8 @@ int nbx_dispatch_kernel(nbx_context* context, unsigned flags);
 int nbx_add_nbat_f_to_f(nbx_context* context, float* f, int f_on_device);
 
 /* GmxForceCalculator::compute (gmxcalculator.cpp:70-83): convert
- * coordinates, dispatch the kernel, reduce forces into f (overwritten).
+ * coordinates, dispatch the kernel, reduce forces into f (overwritten, or
+ * with NBX_COMPUTE_ADD_FORCES added to what f holds).
  * With NBX_COMPUTE_VIRIAL the virial is also formed (calc_vir,
  * calcvir.cpp:103-148).  Device pointers are used asynchronously on the
  * context's stream; host pointers make the call synchronous. */

@@ -361,3 +361,30 @@ def test_work_list_covers_the_pair_list(name, s):
     for c, sh, a, b in ie:
         masked = (je[a:b, 1] & 0xFFFF) != 0xFFFF
         assert not np.any(np.diff(masked.astype(int)) > 0), "masked entries must lead"
+
+
+def test_add_forces_accumulates_on_device():
+    # NBX_COMPUTE_ADD_FORCES: f += nonbonded forces (atomdata_add_nbat_f_to_f
+    # into a buffer holding other force sources); forces are bitwise
+    # reproducible, so the sum is exact to the one float addition
+    import torch
+
+    print("[forces] add to an existing device force buffer", flush=True)
+    s = S.small_water_box(216, seed=11)
+    ctx = nbx.context_for(s)
+    dev = torch.device("cuda:0")
+    x = torch.tensor(s.x, dtype=torch.float32, device=dev).contiguous()
+    f_nb = torch.empty_like(x)
+    ctx.compute_device(x.data_ptr(), f_nb.data_ptr(), nbx.COMPUTE_FORCES)
+    f0 = torch.randn_like(x)
+    f = f0.clone()
+    ctx.compute_device(x.data_ptr(), f.data_ptr(), nbx.COMPUTE_FORCES | nbx.COMPUTE_ADD_FORCES)
+    torch.cuda.synchronize()
+    assert torch.equal(f, f0 + f_nb)
+    # host arrays too
+    h0 = np.random.default_rng(3).normal(size=s.x.shape).astype(np.float32)
+    h = h0.copy()
+    xs = np.ascontiguousarray(s.x, np.float32)
+    ctx._check(ctx._lib.nbx_compute(ctx._h, nbx._ptr(xs), nbx._ptr(h),
+                                    nbx.COMPUTE_FORCES | nbx.COMPUTE_ADD_FORCES, 0), "nbx_compute")
+    np.testing.assert_array_equal(h, h0 + f_nb.cpu().numpy())
