@@ -675,6 +675,19 @@ int nbx_set_nonbonded_parameters(nbx_context* c, int numTypes, const double* c6,
         check(hipStreamSynchronize(c->stream), "sync");
         c->nbfp.ensure(nb.size());
         check(hipMemcpy(c->nbfp.p, nb.data(), sizeof(float2) * nb.size(), hipMemcpyHostToDevice), "upload nbfp");
+        // per type: whether it has LJ with any type (water hydrogens do not;
+        // the pair kernel skips their LJ terms, which are exactly zero)
+        std::vector<int> lj(t, 0);
+        for (int i = 0; i < t; i++)
+        {
+            for (int j = 0; j < t; j++)
+            {
+                lj[i] |= (nb[i * t + j].x != 0.0f || nb[i * t + j].y != 0.0f) ? 1 : 0;
+            }
+        }
+        c->typeLJ.ensure(t);
+        check(hipMemcpy(c->typeLJ.p, lj.data(), sizeof(int) * t, hipMemcpyHostToDevice), "upload LJ flags");
+        c->anyTypeWithoutLJ = std::count(lj.begin(), lj.end() - 1, 0) > 0;
         // new values for the same types leave the grid and the pair list
         // valid (they depend on positions only); a new type count does not
         if (t != c->ntype)
@@ -1090,6 +1103,8 @@ static void enqueueStep(nbx_context* c, unsigned flags, const float* xd, float* 
     a.xq            = c->xq.p;
     a.type          = c->nbtype.p;
     a.nbfp          = c->nbfp.p;
+    a.typeLJ        = c->typeLJ.p;
+    a.skipLJ        = c->anyTypeWithoutLJ ? 1 : 0;
     a.ntype         = c->ntype;
     a.tabFDV0       = c->tabFDV0.p;
     a.chunkShift    = c->lay.chunkShift.p;

@@ -296,6 +296,8 @@ void enqueueCompute(nbx_batch* b, const float* x, float* f, unsigned flags)
     a.xq           = b->xq.p;
     a.type         = b->nbtype.p;
     a.nbfp         = c0->nbfp.p;
+    a.typeLJ       = c0->typeLJ.p;
+    a.skipLJ       = c0->anyTypeWithoutLJ ? 1 : 0;
     a.ntype        = c0->ntype;
     a.tabFDV0      = c0->tabFDV0.p;
     a.chunkShift   = b->chunkShift.p;

@@ -266,6 +266,8 @@ struct nbx_context
     // group size, sized for its larger window per cluster slot; built on
     // the first adjoint call after a search)
     nbx::DevBuf<int>    groupStats;
+    nbx::DevBuf<int>    typeLJ; // per type (padding type included): 1 if it has LJ with any type
+    bool                anyTypeWithoutLJ = false; // among the user's types
     nbx::WorkLayout     lay;
     nbx::WorkLayout     adj;
     bool                adjValid = false;

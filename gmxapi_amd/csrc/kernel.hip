@@ -79,7 +79,7 @@ constexpr int kThreads    = kWave * kWavesBlock;
 // (FrLJ12 - 2 FrLJ6 plus the shift: one rounding fewer than dividing each
 // pair by 12 and 6, and no bias from a rounded 1/12); the wavefront's double
 // sum is divided by 12 once per group.
-template<int kElec, bool kEnergy, bool kChecks, int kI>
+template<int kElec, bool kEnergy, bool kChecks, bool kSkipLJ, int kI>
 __device__ __forceinline__ void pairI(const IAtoms& I, const JAtom& J, const unsigned mbits, const bool diag,
                                       const int jj, const InteractionConst& ic, const float2* __restrict__ sNbfp,
                                       const float4* __restrict__ sTab, float* fi, float& fjx, float& fjy,
@@ -118,15 +118,21 @@ __device__ __forceinline__ void pairI(const IAtoms& I, const JAtom& J, const uns
     // for excluded pairs, whose energies the exclusion pass adds
     const float emask = skip ? 0.0f : interact;
 
-    const float2 c      = sNbfp[I.tOff[kI] + J.type];
-    const float  rinv6  = kChecks ? interact * rinvsq * rinvsq * rinvsq : rinvsq * rinvsq * rinvsq;
-    const float  FrLJ6  = c.x * rinv6;
-    const float  frLJ   = fmaf(c.y * rinv6, rinv6, -FrLJ6);
-    if (kEnergy)
+    // LJ, skipped (a wave-uniform branch) for an i-atom whose type has no
+    // LJ with any type, where every term is exactly zero
+    float frLJ = 0.0f;
+    if (!kSkipLJ || I.lj[kI])
     {
-        // 12 V = FrLJ12 - 2 FrLJ6 + c12 cpot_rep - 2 c6 cpot_disp
-        const float v12 = fmaf(c.x, -2.0f * ic.cpot_disp, fmaf(c.y, ic.cpot_rep, frLJ - FrLJ6));
-        vlj             = fmaf(v12, emask, vlj);
+        const float2 c     = sNbfp[I.tOff[kI] + J.type];
+        const float  rinv6 = kChecks ? interact * rinvsq * rinvsq * rinvsq : rinvsq * rinvsq * rinvsq;
+        const float  FrLJ6 = c.x * rinv6;
+        frLJ               = fmaf(c.y * rinv6, rinv6, -FrLJ6);
+        if (kEnergy)
+        {
+            // 12 V = FrLJ12 - 2 FrLJ6 + c12 cpot_rep - 2 c6 cpot_disp
+            const float v12 = fmaf(c.x, -2.0f * ic.cpot_disp, fmaf(c.y, ic.cpot_rep, frLJ - FrLJ6));
+            vlj             = fmaf(v12, emask, vlj);
+        }
     }
 
     float fcoul;
@@ -193,7 +199,7 @@ __device__ __forceinline__ void pairI(const IAtoms& I, const JAtom& J, const uns
     fjz               = fmaf(-fscal, dz, fjz);
 }
 
-template<int kElec, bool kEnergy, bool kChecks>
+template<int kElec, bool kEnergy, bool kChecks, bool kSkipLJ>
 __device__ __forceinline__ void tile4(const IAtoms& I, const JAtom& J, const unsigned mbits, const bool diag,
                                       const int jj, const InteractionConst& ic, const float2* __restrict__ sNbfp,
                                       const float4* __restrict__ sTab, float* fi, float& fjx, float& fjy,
@@ -201,13 +207,13 @@ __device__ __forceinline__ void tile4(const IAtoms& I, const JAtom& J, const uns
 {
     // barriers between the i-atoms keep the scheduler from interleaving all
     // four pair evaluations, which costs registers (occupancy) for little overlap
-    pairI<kElec, kEnergy, kChecks, 0>(I, J, mbits, diag, jj, ic, sNbfp, sTab, fi, fjx, fjy, fjz, vlj, vc);
+    pairI<kElec, kEnergy, kChecks, kSkipLJ, 0>(I, J, mbits, diag, jj, ic, sNbfp, sTab, fi, fjx, fjy, fjz, vlj, vc);
     __builtin_amdgcn_sched_barrier(0);
-    pairI<kElec, kEnergy, kChecks, 1>(I, J, mbits, diag, jj, ic, sNbfp, sTab, fi, fjx, fjy, fjz, vlj, vc);
+    pairI<kElec, kEnergy, kChecks, kSkipLJ, 1>(I, J, mbits, diag, jj, ic, sNbfp, sTab, fi, fjx, fjy, fjz, vlj, vc);
     __builtin_amdgcn_sched_barrier(0);
-    pairI<kElec, kEnergy, kChecks, 2>(I, J, mbits, diag, jj, ic, sNbfp, sTab, fi, fjx, fjy, fjz, vlj, vc);
+    pairI<kElec, kEnergy, kChecks, kSkipLJ, 2>(I, J, mbits, diag, jj, ic, sNbfp, sTab, fi, fjx, fjy, fjz, vlj, vc);
     __builtin_amdgcn_sched_barrier(0);
-    pairI<kElec, kEnergy, kChecks, 3>(I, J, mbits, diag, jj, ic, sNbfp, sTab, fi, fjx, fjy, fjz, vlj, vc);
+    pairI<kElec, kEnergy, kChecks, kSkipLJ, 3>(I, J, mbits, diag, jj, ic, sNbfp, sTab, fi, fjx, fjy, fjz, vlj, vc);
 }
 
 // LDS layout of a workgroup, in float4 units
@@ -227,7 +233,7 @@ struct LdsLayout
     int end_;
 };
 
-template<int kElec, bool kEnergy, bool kShift>
+template<int kElec, bool kEnergy, bool kShift, bool kSkipLJ>
 __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
 {
     extern __shared__ float4 smem4[];
@@ -310,8 +316,10 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
             }
             if (lane < nOwn * kClusterSize)
             {
-                ix[lane] = a.xq[c0 * kClusterSize + lane];
-                it[lane] = a.type[c0 * kClusterSize + lane] * ntype;
+                // type row offset, with bit 30 set for types that have LJ
+                const int ty = a.type[c0 * kClusterSize + lane];
+                ix[lane]     = a.xq[c0 * kClusterSize + lane];
+                it[lane]     = ty * ntype | (a.typeLJ[ty] << 30);
             }
         };
         auto endGroup = [&]() {
@@ -356,7 +364,9 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
                 I.y[i]         = uniformFloat(p.y);
                 I.z[i]         = uniformFloat(p.z);
                 I.q[i]         = uniformFloat(p.w);
-                I.tOff[i]      = uniformInt(it[lc + i]);
+                const int ti   = uniformInt(it[lc + i]);
+                I.tOff[i]      = ti & 0x3fffffff;
+                I.lj[i]        = ti >> 30;
             }
             sv = make_float4(uniformFloat(s.x), uniformFloat(s.y), uniformFloat(s.z), 0.0f);
 #pragma unroll
@@ -393,12 +403,12 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
             if (checked)
             {
                 const bool diag = central && static_cast<int>(e.x) == ci;
-                tile4<kElec, kEnergy, true>(I, J, (e.y & 0xffffu) >> jj, diag, jj, ic, sNbfp, sTab, fi, fjx, fjy,
+                tile4<kElec, kEnergy, true, kSkipLJ>(I, J, (e.y & 0xffffu) >> jj, diag, jj, ic, sNbfp, sTab, fi, fjx, fjy,
                                             fjz, vlj, vc);
             }
             else
             {
-                tile4<kElec, kEnergy, false>(I, J, 0u, false, jj, ic, sNbfp, sTab, fi, fjx, fjy, fjz, vlj, vc);
+                tile4<kElec, kEnergy, false, kSkipLJ>(I, J, 0u, false, jj, ic, sNbfp, sTab, fi, fjx, fjy, fjz, vlj, vc);
             }
             // the 16 entries of a tile have distinct slots (padding entries
             // share the spare slot, adding zeros): plain read-modify-write;
@@ -507,10 +517,10 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
 // With events, the launch itself records the kernel's start and end (the
 // dispatch packet's own timestamps): no marker packets in the stream, so
 // timing the kernel leaves the step's timeline unchanged.
-template<int kElec, bool kEnergy, bool kShift>
+template<int kElec, bool kEnergy, bool kShift, bool kSkipLJ>
 hipError_t launchOne(hipStream_t s, const KernelArgs& a, int numBlocks, int lds, hipEvent_t start, hipEvent_t stop)
 {
-    const void* fn = reinterpret_cast<const void*>(&nonbondedKernel<kElec, kEnergy, kShift>);
+    const void* fn = reinterpret_cast<const void*>(&nonbondedKernel<kElec, kEnergy, kShift, kSkipLJ>);
     const hipError_t e = allowDynamicLds(fn, lds);
     if (e != hipSuccess)
     {
@@ -521,27 +531,36 @@ hipError_t launchOne(hipStream_t s, const KernelArgs& a, int numBlocks, int lds,
         void* args[] = { const_cast<KernelArgs*>(&a) };
         return hipExtLaunchKernel(fn, dim3(numBlocks), dim3(kThreads), args, lds, s, start, stop, 0);
     }
-    nonbondedKernel<kElec, kEnergy, kShift><<<numBlocks, kThreads, lds, s>>>(a);
+    nonbondedKernel<kElec, kEnergy, kShift, kSkipLJ><<<numBlocks, kThreads, lds, s>>>(a);
     return hipGetLastError();
 }
 
+template<int kElec, bool kSkipLJ>
+hipError_t launchFlags(hipStream_t s, const KernelArgs& a, bool energy, bool shiftForces, int numBlocks, int lds,
+                       hipEvent_t start, hipEvent_t stop)
+{
+    if (energy && shiftForces)
+    {
+        return launchOne<kElec, true, true, kSkipLJ>(s, a, numBlocks, lds, start, stop);
+    }
+    if (energy)
+    {
+        return launchOne<kElec, true, false, kSkipLJ>(s, a, numBlocks, lds, start, stop);
+    }
+    if (shiftForces)
+    {
+        return launchOne<kElec, false, true, kSkipLJ>(s, a, numBlocks, lds, start, stop);
+    }
+    return launchOne<kElec, false, false, kSkipLJ>(s, a, numBlocks, lds, start, stop);
+}
+
+// the variant that tests each i-atom's LJ flag only when some type has no LJ
 template<int kElec>
 hipError_t launchElec(hipStream_t s, const KernelArgs& a, bool energy, bool shiftForces, int numBlocks, int lds,
                       hipEvent_t start, hipEvent_t stop)
 {
-    if (energy && shiftForces)
-    {
-        return launchOne<kElec, true, true>(s, a, numBlocks, lds, start, stop);
-    }
-    if (energy)
-    {
-        return launchOne<kElec, true, false>(s, a, numBlocks, lds, start, stop);
-    }
-    if (shiftForces)
-    {
-        return launchOne<kElec, false, true>(s, a, numBlocks, lds, start, stop);
-    }
-    return launchOne<kElec, false, false>(s, a, numBlocks, lds, start, stop);
+    return a.skipLJ ? launchFlags<kElec, true>(s, a, energy, shiftForces, numBlocks, lds, start, stop)
+                    : launchFlags<kElec, false>(s, a, energy, shiftForces, numBlocks, lds, start, stop);
 }
 
 } // namespace

@@ -62,6 +62,7 @@ struct IAtoms
 {
     float x[4], y[4], z[4], q[4];
     int   tOff[4];
+    int   lj[4]; // the pair kernel: whether the atom's type has LJ at all
 };
 
 // This lane's j-atom for one tile: coordinates minus the shift, w = epsfac

@@ -248,7 +248,9 @@ struct KernelArgs
     const float4* xq;
     const int*    type;
     const float2* nbfp;
-    int           ntype; // including the padding type
+    const int*    typeLJ; // per type: 1 if it has LJ with any type
+    int           skipLJ; // host: some type has no LJ (launch the variant that skips it)
+    int           ntype;  // including the padding type
     const float4* tabFDV0;
     const float4* chunkShift; // per chunk: its i-entry's shift vector
     InteractionConst ic;
