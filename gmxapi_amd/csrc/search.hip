@@ -320,36 +320,70 @@ __global__ __launch_bounds__(kWave* kSearchWaves) void searchKernel(SearchArgs a
 }
 
 // Exclusive scan over n ints in one workgroup; writes the total.
-__global__ __launch_bounds__(1024) void exclusiveScanKernel(int n, const int* __restrict__ in, int* __restrict__ out,
-                                                            int* __restrict__ total)
+// Exclusive scan of n ints by one workgroup (the lists scanned here hold
+// one entry per cluster or group, at most a few 100k): rounds of 8192
+// elements, each thread summing 8 consecutive ones, a wavefront shuffle scan
+// over the threads and a scan over the 16 wavefront totals; out[n] and
+// *total receive the sum.
+constexpr int kScanThreads = 1024;
+constexpr int kScanPer     = 8;
+
+__global__ __launch_bounds__(kScanThreads) void exclusiveScanKernel(int n, const int* __restrict__ in,
+                                                                   int* __restrict__ out, int* __restrict__ total)
 {
-    __shared__ int s[1024];
-    int carry = 0;
-    for (int base = 0; base < n; base += 1024)
+    __shared__ int sWave[kScanThreads / 64];
+    const int      lane  = threadIdx.x & 63;
+    const int      wave  = threadIdx.x >> 6;
+    int            carry = 0;
+    for (int base = 0; base < n; base += kScanThreads * kScanPer)
     {
-        const int i = base + threadIdx.x;
-        const int v = (i < n) ? in[i] : 0;
-        s[threadIdx.x] = v;
-        __syncthreads();
-        for (int off = 1; off < 1024; off *= 2)
+        const int first = base + threadIdx.x * kScanPer;
+        int       v[kScanPer];
+        int       sum = 0;
+#pragma unroll
+        for (int k = 0; k < kScanPer; k++)
         {
-            const int t = (threadIdx.x >= off) ? s[threadIdx.x - off] : 0;
-            __syncthreads();
-            s[threadIdx.x] += t;
-            __syncthreads();
+            v[k] = (first + k < n) ? in[first + k] : 0;
+            sum += v[k];
         }
-        if (i < n)
+        // inclusive scan of the thread sums within the wavefront
+        int incl = sum;
+#pragma unroll
+        for (int off = 1; off < 64; off *= 2)
         {
-            out[i] = carry + s[threadIdx.x] - v;
+            const int t = __shfl_up(incl, off, 64);
+            incl += (lane >= off) ? t : 0;
         }
-        const int blockTotal = s[1023];
+        if (lane == 63)
+        {
+            sWave[wave] = incl;
+        }
         __syncthreads();
-        carry += blockTotal;
+        int waveBase = 0, roundTotal = 0;
+#pragma unroll
+        for (int w = 0; w < kScanThreads / 64; w++)
+        {
+            const int t = sWave[w];
+            waveBase += (w < wave) ? t : 0;
+            roundTotal += t;
+        }
+        int run = carry + waveBase + incl - sum;
+#pragma unroll
+        for (int k = 0; k < kScanPer; k++)
+        {
+            if (first + k < n)
+            {
+                out[first + k] = run;
+            }
+            run += v[k];
+        }
+        carry += roundTotal;
+        __syncthreads(); // sWave is rewritten next round
     }
     if (threadIdx.x == 0)
     {
-        out[n]  = carry;
-        *total  = carry;
+        out[n] = carry;
+        *total = carry;
     }
 }
 
@@ -374,7 +408,7 @@ void launchSearch(hipStream_t s, const SearchArgs& a, bool fill)
 
 void launchExclusiveScan(hipStream_t s, int n, const int* in, int* out, int* total)
 {
-    exclusiveScanKernel<<<1, 1024, 0, s>>>(n, in, out, total);
+    exclusiveScanKernel<<<1, kScanThreads, 0, s>>>(n, in, out, total);
 }
 
 } // namespace nbx
