@@ -183,6 +183,55 @@ void setDevice(nbx_context* c)
     check(hipSetDevice(c->device), "hipSetDevice");
 }
 
+// interaction_const_t's LJ modifier constants (forcerec.cpp:780-818 and
+// 858-873): for the force switch, per power p (6: dispersion, 12:
+// repulsion) force/p = r^-(p+1) + c2 rsw^2 + c3 rsw^3 between rsw and rc,
+// with the potential shifted to 0 at rc (cpot); for the potential switch
+// sw = 1 + c3 rsw^3 + c4 rsw^4 + c5 rsw^5, which has sw(0) = 1, sw(rc - rsw)
+// = 0 and zero first and second derivatives at both ends.  Computed in
+// double from the float radii, rounded once.  The kernel's forms: the
+// force-switch force term rsw^2 r (c12' (c2r + c3r rsw) - c6' (c2d + c3d
+// rsw)) and 12 times its potential term rsw^3 (c6' (4 c2d + 3 c3d rsw) -
+// c12' (4 c2r + 3 c3r rsw)), with c6' = 6 C6 and c12' = 12 C12 as stored.
+void setVdwModifierConstants(InteractionConst& ic, int modifier, float rsw, float rc)
+{
+    ic.rvdw_switch = rsw;
+    ic.fsw_c2d = ic.fsw_c3d = ic.fsw_c2r = ic.fsw_c3r = 0.0f;
+    ic.fsw_e2d = ic.fsw_e3d = ic.fsw_e2r = ic.fsw_e3r = 0.0f;
+    ic.sw_c3 = ic.sw_c4 = ic.sw_c5 = ic.sw_f3 = ic.sw_f4 = ic.sw_f5 = 0.0f;
+    const double d = static_cast<double>(rc) - static_cast<double>(rsw);
+    if (modifier == NBX_VDW_FORCE_SWITCH)
+    {
+        const double rcd = rc, rswd = rsw;
+        auto c2 = [&](double p) { return ((p + 1) * rswd - (p + 4) * rcd) / (std::pow(rcd, p + 2) * d * d); };
+        auto c3 = [&](double p) { return -((p + 1) * rswd - (p + 3) * rcd) / (std::pow(rcd, p + 2) * d * d * d); };
+        auto cpot = [&](double p) {
+            return -std::pow(rcd, -p) + p * c2(p) / 3 * d * d * d + p * c3(p) / 4 * d * d * d * d;
+        };
+        ic.fsw_c2d   = static_cast<float>(c2(6));
+        ic.fsw_c3d   = static_cast<float>(c3(6));
+        ic.fsw_c2r   = static_cast<float>(c2(12));
+        ic.fsw_c3r   = static_cast<float>(c3(12));
+        ic.fsw_e2d   = static_cast<float>(4 * c2(6));
+        ic.fsw_e3d   = static_cast<float>(3 * c3(6));
+        ic.fsw_e2r   = static_cast<float>(4 * c2(12));
+        ic.fsw_e3r   = static_cast<float>(3 * c3(12));
+        ic.cpot_disp = static_cast<float>(cpot(6));
+        ic.cpot_rep  = static_cast<float>(cpot(12));
+    }
+    else if (modifier == NBX_VDW_POTENTIAL_SWITCH)
+    {
+        ic.sw_c3     = static_cast<float>(-10 / (d * d * d));
+        ic.sw_c4     = static_cast<float>(15 / (d * d * d * d));
+        ic.sw_c5     = static_cast<float>(-6 / (d * d * d * d * d));
+        ic.sw_f3     = static_cast<float>(3 * (-10 / (d * d * d)));
+        ic.sw_f4     = static_cast<float>(4 * (15 / (d * d * d * d)));
+        ic.sw_f5     = static_cast<float>(5 * (-6 / (d * d * d * d * d)));
+        ic.cpot_disp = 0.0f;
+        ic.cpot_rep  = 0.0f;
+    }
+}
+
 void requireGrid(const nbx_context* c)
 {
     if (!c->gridValid)
@@ -422,6 +471,8 @@ void nbx_default_options(nbx_options* o)
     o->device                = 0;
     o->kernel_group_size     = 0;
     o->kernel_blocks_per_cu  = 0;
+    o->vdw_modifier          = NBX_VDW_POTENTIAL_SHIFT;
+    o->rvdw_switch           = 0.0;
 }
 
 int nbx_create(const nbx_options* options, nbx_context** context)
@@ -461,6 +512,15 @@ int nbx_create(const nbx_options* options, nbx_context** context)
         {
             throw InputError("kernel_blocks_per_cu must be in 0..8");
         }
+        if (o.vdw_modifier != NBX_VDW_POTENTIAL_SHIFT && o.vdw_modifier != NBX_VDW_FORCE_SWITCH
+            && o.vdw_modifier != NBX_VDW_POTENTIAL_SWITCH)
+        {
+            throw InputError("unknown LJ modifier");
+        }
+        if (o.vdw_modifier != NBX_VDW_POTENTIAL_SHIFT && !(o.rvdw_switch >= 0 && o.rvdw_switch < rcd))
+        {
+            throw InputError("the LJ switch radius must be in [0, rcoulomb)");
+        }
         int ndev = 0;
         check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
         if (o.device < 0 || o.device >= ndev)
@@ -487,6 +547,7 @@ int nbx_create(const nbx_options* options, nbx_context** context)
         ic.rcoulomb2 = c->rcoulomb * c->rcoulomb;
         ic.cpot_disp = static_cast<float>(-1.0 / p6);
         ic.cpot_rep  = static_cast<float>(-1.0 / p12);
+        setVdwModifierConstants(ic, o.vdw_modifier, static_cast<float>(o.rvdw_switch), rvdw);
         ic.cpot_disp6 = static_cast<float>(static_cast<double>(ic.cpot_disp) / 6.0);
         ic.cpot_rep12 = static_cast<float>(static_cast<double>(ic.cpot_rep) / 12.0);
         const float epsr = static_cast<float>(o.epsilon_r);
@@ -1105,6 +1166,7 @@ static void enqueueStep(nbx_context* c, unsigned flags, const float* xd, float* 
     a.nbfp          = c->nbfp.p;
     a.typeLJ        = c->typeLJ.p;
     a.skipLJ        = c->anyTypeWithoutLJ ? 1 : 0;
+    a.vdwModifier   = c->opt.vdw_modifier;
     a.ntype         = c->ntype;
     a.tabFDV0       = c->tabFDV0.p;
     a.chunkShift    = c->lay.chunkShift.p;
@@ -1288,6 +1350,10 @@ int nbx_compute_adjoint(nbx_context* c, const float* x, const float* g, float* g
         if (!g)
         {
             throw InputError("the force cotangent g must not be NULL");
+        }
+        if (c->opt.vdw_modifier != NBX_VDW_POTENTIAL_SHIFT)
+        {
+            throw InputError("the adjoint supports the LJ potential shift only");
         }
         if (c->ntype > kAdjointMaxTypes)
         {

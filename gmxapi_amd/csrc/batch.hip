@@ -298,6 +298,7 @@ void enqueueCompute(nbx_batch* b, const float* x, float* f, unsigned flags)
     a.nbfp         = c0->nbfp.p;
     a.typeLJ       = c0->typeLJ.p;
     a.skipLJ       = c0->anyTypeWithoutLJ ? 1 : 0;
+    a.vdwModifier  = c0->opt.vdw_modifier;
     a.ntype        = c0->ntype;
     a.tabFDV0      = c0->tabFDV0.p;
     a.chunkShift   = b->chunkShift.p;

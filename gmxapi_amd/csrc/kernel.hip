@@ -63,6 +63,7 @@
 #include <mutex>
 #include <utility>
 
+#include "../../include/nbx.h"
 #include "kernel_common.h"
 
 namespace nbx
@@ -79,7 +80,7 @@ constexpr int kThreads    = kWave * kWavesBlock;
 // (FrLJ12 - 2 FrLJ6 plus the shift: one rounding fewer than dividing each
 // pair by 12 and 6, and no bias from a rounded 1/12); the wavefront's double
 // sum is divided by 12 once per group.
-template<int kElec, bool kEnergy, bool kChecks, bool kSkipLJ, int kI>
+template<int kElec, bool kEnergy, bool kChecks, int kLJ, int kI>
 __device__ __forceinline__ void pairI(const IAtoms& I, const JAtom& J, const unsigned mbits, const bool diag,
                                       const int jj, const InteractionConst& ic, const float2* __restrict__ sNbfp,
                                       const float4* __restrict__ sTab, float* fi, float& fjx, float& fjy,
@@ -121,13 +122,52 @@ __device__ __forceinline__ void pairI(const IAtoms& I, const JAtom& J, const uns
     // LJ, skipped (a wave-uniform branch) for an i-atom whose type has no
     // LJ with any type, where every term is exactly zero
     float frLJ = 0.0f;
-    if (!kSkipLJ || I.lj[kI])
+    if (kLJ == kLJShift || I.lj[kI])
     {
         const float2 c     = sNbfp[I.tOff[kI] + J.type];
         const float  rinv6 = kChecks ? interact * rinvsq * rinvsq * rinvsq : rinvsq * rinvsq * rinvsq;
         const float  FrLJ6 = c.x * rinv6;
         frLJ               = fmaf(c.y * rinv6, rinv6, -FrLJ6);
-        if (kEnergy)
+        if (kLJ == kLJForceSwitch || kLJ == kLJPotSwitch)
+        {
+            // the switches act on rsw = max(r - rvdw_switch, 0), with r from
+            // the masked 1/r: 0 beyond the cut-off and on the diagonal (and
+            // so for the far-away padding cluster); excluded pairs inside
+            // the cut-off keep their r, as in the plain-C kernel
+            const float rm  = rsq * rinv;
+            const float rsw = fmaxf(rm - ic.rvdw_switch, 0.0f);
+            const float rs2 = rsw * rsw;
+            if (kLJ == kLJForceSwitch)
+            {
+                if (kEnergy)
+                {
+                    // 12 V = FrLJ12 - 2 FrLJ6 + c12 cpot_rep - 2 c6 cpot_disp
+                    //        + rsw^3 (c6 (4 c2d + 3 c3d rsw) - c12 (4 c2r + 3 c3r rsw)),
+                    // from the unswitched frLJ = FrLJ12 - FrLJ6
+                    const float vsw = rs2 * rsw
+                                      * fmaf(c.x, fmaf(ic.fsw_e3d, rsw, ic.fsw_e2d),
+                                             -c.y * fmaf(ic.fsw_e3r, rsw, ic.fsw_e2r));
+                    const float v12 = fmaf(c.x, -2.0f * ic.cpot_disp, fmaf(c.y, ic.cpot_rep, frLJ - FrLJ6 + vsw));
+                    vlj             = fmaf(v12, emask, vlj);
+                }
+                frLJ = fmaf(rs2 * rm,
+                            fmaf(c.y, fmaf(ic.fsw_c3r, rsw, ic.fsw_c2r), -c.x * fmaf(ic.fsw_c3d, rsw, ic.fsw_c2d)),
+                            frLJ);
+            }
+            else
+            {
+                // V (masked, no shift) scaled by sw; F = F sw - r V dsw
+                const float v12 = (frLJ - FrLJ6) * (kChecks ? interact : 1.0f);
+                const float sw  = fmaf(rs2 * rsw, fmaf(fmaf(ic.sw_c5, rsw, ic.sw_c4), rsw, ic.sw_c3), 1.0f);
+                const float dsw = rs2 * fmaf(fmaf(ic.sw_f5, rsw, ic.sw_f4), rsw, ic.sw_f3);
+                frLJ            = fmaf(frLJ, sw, -rm * (v12 * (1.0f / 12.0f)) * dsw);
+                if (kEnergy)
+                {
+                    vlj = fmaf(v12 * sw, emask, vlj);
+                }
+            }
+        }
+        else if (kEnergy)
         {
             // 12 V = FrLJ12 - 2 FrLJ6 + c12 cpot_rep - 2 c6 cpot_disp
             const float v12 = fmaf(c.x, -2.0f * ic.cpot_disp, fmaf(c.y, ic.cpot_rep, frLJ - FrLJ6));
@@ -199,7 +239,7 @@ __device__ __forceinline__ void pairI(const IAtoms& I, const JAtom& J, const uns
     fjz               = fmaf(-fscal, dz, fjz);
 }
 
-template<int kElec, bool kEnergy, bool kChecks, bool kSkipLJ>
+template<int kElec, bool kEnergy, bool kChecks, int kLJ>
 __device__ __forceinline__ void tile4(const IAtoms& I, const JAtom& J, const unsigned mbits, const bool diag,
                                       const int jj, const InteractionConst& ic, const float2* __restrict__ sNbfp,
                                       const float4* __restrict__ sTab, float* fi, float& fjx, float& fjy,
@@ -207,13 +247,13 @@ __device__ __forceinline__ void tile4(const IAtoms& I, const JAtom& J, const uns
 {
     // barriers between the i-atoms keep the scheduler from interleaving all
     // four pair evaluations, which costs registers (occupancy) for little overlap
-    pairI<kElec, kEnergy, kChecks, kSkipLJ, 0>(I, J, mbits, diag, jj, ic, sNbfp, sTab, fi, fjx, fjy, fjz, vlj, vc);
+    pairI<kElec, kEnergy, kChecks, kLJ, 0>(I, J, mbits, diag, jj, ic, sNbfp, sTab, fi, fjx, fjy, fjz, vlj, vc);
     __builtin_amdgcn_sched_barrier(0);
-    pairI<kElec, kEnergy, kChecks, kSkipLJ, 1>(I, J, mbits, diag, jj, ic, sNbfp, sTab, fi, fjx, fjy, fjz, vlj, vc);
+    pairI<kElec, kEnergy, kChecks, kLJ, 1>(I, J, mbits, diag, jj, ic, sNbfp, sTab, fi, fjx, fjy, fjz, vlj, vc);
     __builtin_amdgcn_sched_barrier(0);
-    pairI<kElec, kEnergy, kChecks, kSkipLJ, 2>(I, J, mbits, diag, jj, ic, sNbfp, sTab, fi, fjx, fjy, fjz, vlj, vc);
+    pairI<kElec, kEnergy, kChecks, kLJ, 2>(I, J, mbits, diag, jj, ic, sNbfp, sTab, fi, fjx, fjy, fjz, vlj, vc);
     __builtin_amdgcn_sched_barrier(0);
-    pairI<kElec, kEnergy, kChecks, kSkipLJ, 3>(I, J, mbits, diag, jj, ic, sNbfp, sTab, fi, fjx, fjy, fjz, vlj, vc);
+    pairI<kElec, kEnergy, kChecks, kLJ, 3>(I, J, mbits, diag, jj, ic, sNbfp, sTab, fi, fjx, fjy, fjz, vlj, vc);
 }
 
 // LDS layout of a workgroup, in float4 units
@@ -233,7 +273,7 @@ struct LdsLayout
     int end_;
 };
 
-template<int kElec, bool kEnergy, bool kShift, bool kSkipLJ>
+template<int kElec, bool kEnergy, bool kShift, int kLJ>
 __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
 {
     extern __shared__ float4 smem4[];
@@ -403,12 +443,12 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
             if (checked)
             {
                 const bool diag = central && static_cast<int>(e.x) == ci;
-                tile4<kElec, kEnergy, true, kSkipLJ>(I, J, (e.y & 0xffffu) >> jj, diag, jj, ic, sNbfp, sTab, fi, fjx, fjy,
+                tile4<kElec, kEnergy, true, kLJ>(I, J, (e.y & 0xffffu) >> jj, diag, jj, ic, sNbfp, sTab, fi, fjx, fjy,
                                             fjz, vlj, vc);
             }
             else
             {
-                tile4<kElec, kEnergy, false, kSkipLJ>(I, J, 0u, false, jj, ic, sNbfp, sTab, fi, fjx, fjy, fjz, vlj, vc);
+                tile4<kElec, kEnergy, false, kLJ>(I, J, 0u, false, jj, ic, sNbfp, sTab, fi, fjx, fjy, fjz, vlj, vc);
             }
             // the 16 entries of a tile have distinct slots (padding entries
             // share the spare slot, adding zeros): plain read-modify-write;
@@ -517,10 +557,10 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
 // With events, the launch itself records the kernel's start and end (the
 // dispatch packet's own timestamps): no marker packets in the stream, so
 // timing the kernel leaves the step's timeline unchanged.
-template<int kElec, bool kEnergy, bool kShift, bool kSkipLJ>
+template<int kElec, bool kEnergy, bool kShift, int kLJ>
 hipError_t launchOne(hipStream_t s, const KernelArgs& a, int numBlocks, int lds, hipEvent_t start, hipEvent_t stop)
 {
-    const void* fn = reinterpret_cast<const void*>(&nonbondedKernel<kElec, kEnergy, kShift, kSkipLJ>);
+    const void* fn = reinterpret_cast<const void*>(&nonbondedKernel<kElec, kEnergy, kShift, kLJ>);
     const hipError_t e = allowDynamicLds(fn, lds);
     if (e != hipSuccess)
     {
@@ -531,36 +571,45 @@ hipError_t launchOne(hipStream_t s, const KernelArgs& a, int numBlocks, int lds,
         void* args[] = { const_cast<KernelArgs*>(&a) };
         return hipExtLaunchKernel(fn, dim3(numBlocks), dim3(kThreads), args, lds, s, start, stop, 0);
     }
-    nonbondedKernel<kElec, kEnergy, kShift, kSkipLJ><<<numBlocks, kThreads, lds, s>>>(a);
+    nonbondedKernel<kElec, kEnergy, kShift, kLJ><<<numBlocks, kThreads, lds, s>>>(a);
     return hipGetLastError();
 }
 
-template<int kElec, bool kSkipLJ>
+template<int kElec, int kLJ>
 hipError_t launchFlags(hipStream_t s, const KernelArgs& a, bool energy, bool shiftForces, int numBlocks, int lds,
                        hipEvent_t start, hipEvent_t stop)
 {
     if (energy && shiftForces)
     {
-        return launchOne<kElec, true, true, kSkipLJ>(s, a, numBlocks, lds, start, stop);
+        return launchOne<kElec, true, true, kLJ>(s, a, numBlocks, lds, start, stop);
     }
     if (energy)
     {
-        return launchOne<kElec, true, false, kSkipLJ>(s, a, numBlocks, lds, start, stop);
+        return launchOne<kElec, true, false, kLJ>(s, a, numBlocks, lds, start, stop);
     }
     if (shiftForces)
     {
-        return launchOne<kElec, false, true, kSkipLJ>(s, a, numBlocks, lds, start, stop);
+        return launchOne<kElec, false, true, kLJ>(s, a, numBlocks, lds, start, stop);
     }
-    return launchOne<kElec, false, false, kSkipLJ>(s, a, numBlocks, lds, start, stop);
+    return launchOne<kElec, false, false, kLJ>(s, a, numBlocks, lds, start, stop);
 }
 
-// the variant that tests each i-atom's LJ flag only when some type has no LJ
+// the LJ flavour: the switches, or the potential shift with the variant
+// that tests each i-atom's LJ flag only when some type has no LJ
 template<int kElec>
 hipError_t launchElec(hipStream_t s, const KernelArgs& a, bool energy, bool shiftForces, int numBlocks, int lds,
                       hipEvent_t start, hipEvent_t stop)
 {
-    return a.skipLJ ? launchFlags<kElec, true>(s, a, energy, shiftForces, numBlocks, lds, start, stop)
-                    : launchFlags<kElec, false>(s, a, energy, shiftForces, numBlocks, lds, start, stop);
+    if (a.vdwModifier == NBX_VDW_FORCE_SWITCH)
+    {
+        return launchFlags<kElec, kLJForceSwitch>(s, a, energy, shiftForces, numBlocks, lds, start, stop);
+    }
+    if (a.vdwModifier == NBX_VDW_POTENTIAL_SWITCH)
+    {
+        return launchFlags<kElec, kLJPotSwitch>(s, a, energy, shiftForces, numBlocks, lds, start, stop);
+    }
+    return a.skipLJ ? launchFlags<kElec, kLJShiftSkip>(s, a, energy, shiftForces, numBlocks, lds, start, stop)
+                    : launchFlags<kElec, kLJShift>(s, a, energy, shiftForces, numBlocks, lds, start, stop);
 }
 
 } // namespace

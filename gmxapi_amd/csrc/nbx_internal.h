@@ -82,6 +82,18 @@ enum Elec : int
     kElecEwaldAna   = 2, // analytical correction
 };
 
+// LJ flavours of the pair kernel: the potential shift with every type
+// having LJ, the potential shift testing each i-atom's LJ flag (a type with
+// no LJ contributes exact zeros, so its terms are skipped), and the force
+// and potential switches (which test the flag too)
+enum LJFlavour : int
+{
+    kLJShift       = 0,
+    kLJShiftSkip   = 1,
+    kLJForceSwitch = 2,
+    kLJPotSwitch   = 3,
+};
+
 struct InteractionConst
 {
     float rlist2;
@@ -101,6 +113,14 @@ struct InteractionConst
     float self_v0; // V(0) of the Ewald correction (or c_rf for RF)
     float beta2;   // beta^2 and beta^3 for the analytical correction
     float beta3;
+    // LJ modifiers: the switch radius; force switch: force coefficients
+    // c2, c3 and 12-times-energy coefficients 4 c2, 3 c3 for dispersion (d)
+    // and repulsion (r); potential switch: sw coefficients c3..c5 and its
+    // derivative's 3 c3, 4 c4, 5 c5
+    float rvdw_switch;
+    float fsw_c2d, fsw_c3d, fsw_c2r, fsw_c3r;
+    float fsw_e2d, fsw_e3d, fsw_e2r, fsw_e3r;
+    float sw_c3, sw_c4, sw_c5, sw_f3, sw_f4, sw_f5;
 };
 
 struct GridDims
@@ -250,6 +270,7 @@ struct KernelArgs
     const float2* nbfp;
     const int*    typeLJ; // per type: 1 if it has LJ with any type
     int           skipLJ; // host: some type has no LJ (launch the variant that skips it)
+    int           vdwModifier; // host: NBX_VDW_*
     int           ntype;  // including the padding type
     const float4* tabFDV0;
     const float4* chunkShift; // per chunk: its i-entry's shift vector

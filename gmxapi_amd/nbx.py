@@ -26,6 +26,9 @@ NBX_ERROR_STATE = 3
 
 COULOMB = {"ewald": 0, "pme": 0, "cutoff": 1, "reaction-field": 2, "rf": 2}
 EWALD_CORRECTION = {"table": 0, "analytical": 1}
+# LJ modifiers (interaction_const_t::vdw_modifier): nblib's potential shift,
+# and the force and potential switches from rvdw_switch to the cut-off
+VDW_MODIFIER = {"potential-shift": 0, "force-switch": 1, "potential-switch": 2}
 
 COMPUTE_FORCES = 1
 COMPUTE_ENERGY = 2
@@ -58,11 +61,13 @@ class _Options(C.Structure):
     _fields_ = [("coulomb_type", C.c_int), ("ewald_correction", C.c_int), ("pairlist_cutoff", C.c_double),
                 ("coulomb_cutoff", C.c_double), ("ewald_rtol", C.c_double), ("epsilon_r", C.c_double),
                 ("epsilon_rf", C.c_double), ("ewald_potential_shift", C.c_int), ("device", C.c_int),
-                ("kernel_group_size", C.c_int), ("kernel_blocks_per_cu", C.c_int)]
+                ("kernel_group_size", C.c_int), ("kernel_blocks_per_cu", C.c_int), ("vdw_modifier", C.c_int),
+                ("rvdw_switch", C.c_double)]
 
 
 def _options(coulomb="ewald", ewald_correction="table", rlist=1.0, rcoulomb=None, ewald_rtol=1e-5, epsilon_r=1.0,
-             epsilon_rf=1.0, ewald_potential_shift=False, device=0, kernel_group_size=0, kernel_blocks_per_cu=0):
+             epsilon_rf=1.0, ewald_potential_shift=False, device=0, kernel_group_size=0, kernel_blocks_per_cu=0,
+             vdw_modifier="potential-shift", rvdw_switch=0.0):
     o = _Options()
     library().nbx_default_options(C.byref(o))
     o.coulomb_type = COULOMB[coulomb]
@@ -76,6 +81,10 @@ def _options(coulomb="ewald", ewald_correction="table", rlist=1.0, rcoulomb=None
     o.device = device
     o.kernel_group_size = kernel_group_size
     o.kernel_blocks_per_cu = kernel_blocks_per_cu
+    if vdw_modifier not in VDW_MODIFIER:
+        raise NbxInputError(f"unknown LJ modifier {vdw_modifier!r}; one of {sorted(VDW_MODIFIER)}")
+    o.vdw_modifier = VDW_MODIFIER[vdw_modifier]
+    o.rvdw_switch = rvdw_switch
     return o
 
 
@@ -185,11 +194,13 @@ class Context:
     def __init__(self, coulomb: str = "ewald", ewald_correction: str = "table", rlist: float = 1.0,
                  rcoulomb: float | None = None, ewald_rtol: float = 1e-5, epsilon_r: float = 1.0,
                  epsilon_rf: float = 1.0, ewald_potential_shift: bool = False, device: int = 0,
-                 kernel_group_size: int = 0, kernel_blocks_per_cu: int = 0):
+                 kernel_group_size: int = 0, kernel_blocks_per_cu: int = 0,
+                 vdw_modifier: str = "potential-shift", rvdw_switch: float = 0.0):
         lib = library()
         self._lib = lib
         o = _options(coulomb, ewald_correction, rlist, rcoulomb, ewald_rtol, epsilon_r, epsilon_rf,
-                     ewald_potential_shift, device, kernel_group_size, kernel_blocks_per_cu)
+                     ewald_potential_shift, device, kernel_group_size, kernel_blocks_per_cu, vdw_modifier,
+                     rvdw_switch)
         h = C.c_void_p()
         rc = lib.nbx_create(C.byref(o), C.byref(h))
         if rc != NBX_SUCCESS:
@@ -426,6 +437,24 @@ def nblib_c6c12_matrices(c6_type, c12_type):
     return m6, m12
 
 
+def lorentz_berthelot_c6c12_matrices(sigma_type, epsilon_type):
+    """C6/C12 matrices by the Lorentz-Berthelot rule, sigma_ij = (sigma_i +
+    sigma_j) / 2 and epsilon_ij = sqrt(epsilon_i epsilon_j), C6 = 4 eps
+    sigma^6, C12 = 4 eps sigma^12 (the rule nbnxm recognises as ljcrLB,
+    atomdata.cpp:540-590; nblib declares it, kerneloptions.h:66-72, but its
+    combineNonbondedParameters throws for it, interactions.cpp:83-93).  The
+    pair kernel reads the full matrices, so any rule, or none, works."""
+    sig = np.asarray(sigma_type, np.float64)
+    eps = np.asarray(epsilon_type, np.float64)
+    if sig.shape != eps.shape or sig.ndim != 1:
+        raise NbxInputError("one sigma and one epsilon per particle type")
+    if np.any(sig < 0) or np.any(eps < 0):
+        raise NbxInputError("sigma and epsilon must be non-negative")
+    s6 = (0.5 * (sig[:, None] + sig[None, :])) ** 6
+    e = np.sqrt(eps[:, None] * eps[None, :])
+    return 4.0 * e * s6, 4.0 * e * s6 * s6
+
+
 class Batch:
     """A batched ensemble (include/nbx.h nbx_batch_*): independent systems
     sharing options and LJ parameters, computed by one kernel launch."""
@@ -522,12 +551,14 @@ def batch_for(systems, **options) -> Batch:
 
 
 def context_for(system, coulomb="ewald", ewald_correction="table", device=0, potential_shift=False,
-                kernel_group_size=0, kernel_blocks_per_cu=0) -> Context:
+                kernel_group_size=0, kernel_blocks_per_cu=0, vdw_modifier="potential-shift",
+                rvdw_switch=0.0) -> Context:
     """Context set up on a gmxapi_amd.systems.System the way nblib sets up its
     calculator: parameters, topology, grid, pair list."""
     ctx = Context(coulomb=coulomb, ewald_correction=ewald_correction, rlist=system.rlist,
                   rcoulomb=system.rcoulomb, ewald_potential_shift=potential_shift, device=device,
-                  kernel_group_size=kernel_group_size, kernel_blocks_per_cu=kernel_blocks_per_cu)
+                  kernel_group_size=kernel_group_size, kernel_blocks_per_cu=kernel_blocks_per_cu,
+                  vdw_modifier=vdw_modifier, rvdw_switch=rvdw_switch)
     m6, m12 = nblib_c6c12_matrices(system.c6, system.c12)
     ctx.set_nonbonded_parameters(m6, m12)
     ctx.set_topology(system.types, system.charges, system.excl_off, system.excl_idx)

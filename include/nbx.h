@@ -50,6 +50,16 @@ extern "C" {
 #define NBX_EWALD_TABLE 0
 #define NBX_EWALD_ANALYTICAL 1
 
+/* LJ modifier (interaction_const_t::vdw_modifier; constants as
+ * mdlib/forcerec.cpp:780-818,858-873 sets them; kernel terms as the
+ * plain-C kernel's LJ_FORCE_SWITCH / LJ_POT_SWITCH paths,
+ * nbnxm/kernels_reference/kernel_ref_inner.h:155-205).  nblib always uses
+ * the potential shift (gmxsetup.cpp:264-267); the switches act between
+ * rvdw_switch and the cut-off. */
+#define NBX_VDW_POTENTIAL_SHIFT 0
+#define NBX_VDW_FORCE_SWITCH 1
+#define NBX_VDW_POTENTIAL_SWITCH 2
+
 /* compute flags (gmx::StepWorkload computeForces / computeEnergy /
  * computeVirial, gmxsetup.cpp:214-224) */
 #define NBX_COMPUTE_FORCES 1u
@@ -82,6 +92,8 @@ typedef struct nbx_options
                                      that fits LDS, else 1, 2, 4, 8 or 16 (tuning) */
     int    kernel_blocks_per_cu;  /* resident kernel workgroups per CU: 0 = as many
                                      as LDS allows up to 4 (tuning) */
+    int    vdw_modifier;          /* NBX_VDW_*, default POTENTIAL_SHIFT (nblib) */
+    double rvdw_switch;           /* switch radius for the switches, nm, 0 <= rvdw_switch < rcoulomb */
 } nbx_options;
 
 typedef struct nbx_pairlist_info
@@ -189,7 +201,8 @@ int nbx_set_charges(nbx_context* context, const float* charges, int on_device);
  * Any output may be NULL (no C6/C12 output skips their accumulation).  All
  * pointers are device pointers, used asynchronously on the context's
  * stream, when on_device is set; host pointers make the call synchronous.
- * At most 31 particle types. */
+ * At most 31 particle types; the LJ potential shift only (a context made
+ * with another vdw_modifier gets NBX_ERROR_INPUT). */
 int nbx_compute_adjoint(nbx_context* context, const float* x, const float* g, float* grad_x, double* grad_q,
                         double* grad_c6, double* grad_c12, int on_device);
 

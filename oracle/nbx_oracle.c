@@ -285,6 +285,13 @@ typedef struct
     int    ewald_exact; /* 0 = reference spline table, 1 = exact closed form */
     real   rlist, rcoulomb;
     real   epsfac, k_rf, c_rf, beta, sh_ewald, cpot_disp, cpot_rep;
+    /* LJ modifier (interaction_const_t vdw_modifier): 0 potential shift,
+     * 1 force switch, 2 potential switch; rvdw_switch and the constants of
+     * forcerec.cpp:780-818 (dispersion_shift, repulsion_shift, vdw_switch) */
+    int    vdw_mod;
+    real   rvdw_switch;
+    real   fsw_c2d, fsw_c3d, fsw_c2r, fsw_c3r;
+    real   sw_c3, sw_c4, sw_c5;
     real   tab_scale;
     int    tab_n;
     real*  tabF;
@@ -384,6 +391,62 @@ orc_sys* orc_create(int coulomb, int ewald_exact, double rlist, double rcoulomb,
         orc_ewald_tables(s->tab_n, s->tab_scale, s->beta, s->tabF, s->tabV, s->tabFDV0);
     }
     return s;
+}
+
+/* The LJ modifier and its constants (init_interaction_const,
+ * forcerec.cpp:848-873; force_switch_constants :787-801,
+ * potential_switch_constants :803-818), in double from the real radii and
+ * rounded to real once, as the reference's real-valued pow expressions do.
+ * Returns -1 for an unknown modifier or a switch radius outside [0, rc). */
+int orc_set_vdw_modifier(orc_sys* s, int modifier, double rvdw_switch)
+{
+    double rc  = (double)s->rcoulomb;
+    double rsw = (double)(real)rvdw_switch;
+    double d   = rc - rsw;
+    if (modifier < 0 || modifier > 2 || (modifier != 0 && !(rsw >= 0 && rsw < rc)))
+    {
+        return -1;
+    }
+    s->vdw_mod     = modifier;
+    s->rvdw_switch = (real)rsw;
+    s->fsw_c2d = s->fsw_c3d = s->fsw_c2r = s->fsw_c3r = 0;
+    s->sw_c3 = s->sw_c4 = s->sw_c5 = 0;
+    if (modifier == 0)
+    {
+        real p3      = s->rcoulomb * (s->rcoulomb * s->rcoulomb);
+        real p6      = p3 * p3;
+        s->cpot_disp = (real)(-1.0 / p6);
+        s->cpot_rep  = (real)(-1.0 / (p6 * p6));
+    }
+    else if (modifier == 1)
+    {
+        /* force/p = r^-(p+1) + c2 rsw^2 + c3 rsw^3, potential shifted to 0 at rc */
+        double pw[2] = { 6.0, 12.0 };
+        real   c2[2], c3[2], cpot[2];
+        for (int k = 0; k < 2; k++)
+        {
+            double p = pw[k];
+            c2[k]    = (real)(((p + 1) * rsw - (p + 4) * rc) / (pow(rc, p + 2) * d * d));
+            c3[k]    = (real)(-((p + 1) * rsw - (p + 3) * rc) / (pow(rc, p + 2) * d * d * d));
+            cpot[k]  = (real)(-pow(rc, -p) + p * (double)c2[k] / 3 * d * d * d + p * (double)c3[k] / 4 * d * d * d * d);
+        }
+        s->fsw_c2d   = c2[0];
+        s->fsw_c3d   = c3[0];
+        s->fsw_c2r   = c2[1];
+        s->fsw_c3r   = c3[1];
+        s->cpot_disp = cpot[0];
+        s->cpot_rep  = cpot[1];
+    }
+    else
+    {
+        /* sw = 1 + c3 rsw^3 + c4 rsw^4 + c5 rsw^5: 1 at rvdw_switch, 0 at rc */
+        s->sw_c3     = (real)(-10 / (d * d * d));
+        s->sw_c4     = (real)(15 / (d * d * d * d));
+        s->sw_c5     = (real)(-6 / (d * d * d * d * d));
+        s->cpot_disp = 0;
+        s->cpot_rep  = 0;
+    }
+    return 0;
 }
 
 void orc_destroy(orc_sys* s)
@@ -1534,11 +1597,45 @@ void orc_kernel(orc_sys* s, int energy)
                     real FrLJ6    = c6 * rinvsix;
                     real FrLJ12   = c12 * rinvsix * rinvsix;
                     real frLJ     = FrLJ12 - FrLJ6;
+                    real VLJ      = 0;
+                    real r = 0, rsw = 0;
+                    if (energy || s->vdw_mod == 2)
+                    {
+                        VLJ = (FrLJ12 + c12 * s->cpot_rep) / 12 - (FrLJ6 + c6 * s->cpot_disp) / 6;
+                    }
+                    if (s->vdw_mod != 0)
+                    {
+                        /* switching from rvdw_switch (kernel_ref_inner.h:168-172):
+                         * r from the masked rinv */
+                        r   = rsq * rinv;
+                        rsw = r - s->rvdw_switch;
+                        rsw = (rsw >= 0 ? rsw : 0);
+                    }
+                    if (s->vdw_mod == 1)
+                    {
+                        /* kernel_ref_inner.h:174-183 */
+                        frLJ += -c6 * (s->fsw_c2d + s->fsw_c3d * rsw) * rsw * rsw * r
+                                + c12 * (s->fsw_c2r + s->fsw_c3r * rsw) * rsw * rsw * r;
+                        if (energy)
+                        {
+                            VLJ += -c6 * (-s->fsw_c2d / 3 - s->fsw_c3d / 4 * rsw) * rsw * rsw * rsw
+                                   + c12 * (-s->fsw_c2r / 3 - s->fsw_c3r / 4 * rsw) * rsw * rsw * rsw;
+                        }
+                    }
+                    /* masked after the force switch, before the potential
+                     * switch (kernel_ref_inner.h:185-191) */
+                    VLJ = VLJ * interact;
+                    if (s->vdw_mod == 2)
+                    {
+                        /* kernel_ref_inner.h:193-203, kernel_ref_outer.h:171-176 */
+                        real sw  = 1 + (s->sw_c3 + (s->sw_c4 + s->sw_c5 * rsw) * rsw) * rsw * rsw * rsw;
+                        real dsw = (3 * s->sw_c3 + (4 * s->sw_c4 + 5 * s->sw_c5 * rsw) * rsw) * rsw * rsw;
+                        frLJ     = frLJ * sw - r * VLJ * dsw;
+                        VLJ *= sw;
+                    }
                     if (energy)
                     {
-                        real VLJ = (FrLJ12 + c12 * s->cpot_rep) / 12 - (FrLJ6 + c6 * s->cpot_disp) / 6;
-                        VLJ      = VLJ * interact;
-                        VLJ      = VLJ * skipmask;
+                        VLJ = VLJ * skipmask;
                         Vvdw_ci += VLJ;
                     }
                     real qq = skipmask * qi[i] * q[aj];

@@ -43,6 +43,8 @@ def _lib(precision: str) -> C.CDLL:
     lib.orc_create.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                C.c_double, C.c_int]
     lib.orc_destroy.argtypes = [vp]
+    lib.orc_set_vdw_modifier.restype = C.c_int
+    lib.orc_set_vdw_modifier.argtypes = [vp, C.c_int, C.c_double]
     lib.orc_ewald_coeff.restype = real
     lib.orc_ewald_coeff.argtypes = [real, real]
     lib.orc_ewald_table_scale.restype = real
@@ -176,6 +178,10 @@ def brute_forces(x, box, types, q, nbfp, ntype, excl_off, excl_idx, coulomb: str
     return f.reshape(n, 3), (float(e[0]), float(e[1])), vir.reshape(3, 3)
 
 
+# interaction_const_t vdw_modifier values the oracle implements
+VDW_MODIFIER = {"potential-shift": 0, "force-switch": 1, "potential-switch": 2}
+
+
 class Oracle:
     """One nblib-style force calculator on the CPU.
 
@@ -184,7 +190,8 @@ class Oracle:
     """
 
     def __init__(self, precision="f64", coulomb="ewald", ewald_exact=False, rlist=1.0, rcoulomb=None,
-                 epsilon_r=1.0, epsilon_rf=1.0, ewald_rtol=1e-5, ewald_potential_shift=False):
+                 epsilon_r=1.0, epsilon_rf=1.0, ewald_rtol=1e-5, ewald_potential_shift=False,
+                 vdw_modifier="potential-shift", rvdw_switch=0.0):
         self.precision = precision
         self.lib = _lib(precision)
         self.dt = np.float32 if precision == "f32" else np.float64
@@ -193,6 +200,8 @@ class Oracle:
         self.h = self.lib.orc_create(1 if coulomb == "ewald" else 0, 1 if ewald_exact else 0, rlist, rc,
                                      epsilon_r, epsilon_rf, ewald_rtol, 1 if ewald_potential_shift else 0)
         self.natoms = 0
+        if self.lib.orc_set_vdw_modifier(self.h, VDW_MODIFIER[vdw_modifier], rvdw_switch) != 0:
+            raise ValueError(f"bad LJ modifier {vdw_modifier!r} / rvdw_switch {rvdw_switch}")
 
     def __del__(self):
         h = getattr(self, "h", None)

@@ -101,7 +101,7 @@ def info_clusters(ctx):
 _NOISE = {}
 
 
-def _fp32_noise(s, coulomb="ewald", ewald_exact=False, copies=6):
+def _fp32_noise(s, coulomb="ewald", ewald_exact=False, copies=6, vdw=("potential-shift", 0.0)):
     """RMS deviation from FP64 of the reference FP32 kernel (the oracle's
     single-precision restatement) over rigid translations of the system.
 
@@ -112,7 +112,7 @@ def _fp32_noise(s, coulomb="ewald", ewald_exact=False, copies=6):
     (float coordinate differences and shift vectors carry ~1e-7 relative
     rounding that r^-12 and the Coulomb sum amplify), and one sample of it
     can land arbitrarily close to zero."""
-    key = (id(s), coulomb, ewald_exact)
+    key = (id(s), coulomb, ewald_exact, vdw)
     if key not in _NOISE:
         rng = np.random.default_rng(17)
         box = np.asarray(s.box, np.float64)
@@ -122,26 +122,26 @@ def _fp32_noise(s, coulomb="ewald", ewald_exact=False, copies=6):
             x = np.mod(s.x + (0.0 if k == 0 else rng.uniform(0.0, 1.0, 3) * box), box)
             x[x.astype(np.float32) >= box.astype(np.float32)] = 0.0  # stay inside [0, L) in float
             t.x = x
-            r64 = oracle_for(t, "f64", coulomb=coulomb, ewald_exact=ewald_exact).compute(t.x)
-            r32 = oracle_for(t, "f32", coulomb=coulomb).compute(t.x)
+            r64 = oracle_for(t, "f64", coulomb=coulomb, ewald_exact=ewald_exact, vdw=vdw).compute(t.x)
+            r32 = oracle_for(t, "f32", coulomb=coulomb, vdw=vdw).compute(t.x)
             for e in dev:
                 dev[e].append(r32[e] - r64[e])
         _NOISE[key] = {e: float(np.sqrt(np.mean(np.square(v)))) for e, v in dev.items()}
     return _NOISE[key]
 
 
-def _energy_bar(s, key, scale, coulomb="ewald", ewald_exact=False):
+def _energy_bar(s, key, scale, coulomb="ewald", ewald_exact=False, vdw=("potential-shift", 0.0)):
     """Energy tolerance: 1e-6 relative, or on small boxes three times the
     FP32 reference kernel's own RMS rounding noise there, whichever is larger
     (the 9,000- and 288,000-atom boxes are held to the plain 1e-6)."""
-    return max(ENERGY_TOL_REL * scale, 3.0 * _fp32_noise(s, coulomb, ewald_exact)[key])
+    return max(ENERGY_TOL_REL * scale, 3.0 * _fp32_noise(s, coulomb, ewald_exact, vdw=vdw)[key])
 
 
-def _compare(s, ctx, coulomb="ewald", widen_energy=True):
+def _compare(s, ctx, coulomb="ewald", widen_energy=True, vdw=("potential-shift", 0.0)):
     f = ctx.compute(s.x, energy=True, virial=True)
     vlj, vc = ctx.energies()
-    r64 = oracle_for(s, "f64", coulomb=coulomb).compute(s.x)
-    r32 = oracle_for(s, "f32", coulomb=coulomb).compute(s.x)
+    r64 = oracle_for(s, "f64", coulomb=coulomb, vdw=vdw).compute(s.x)
+    r32 = oracle_for(s, "f32", coulomb=coulomb, vdw=vdw).compute(s.x)
     ferr64 = rms(f - r64["f"]) / rms(r64["f"])
     ferr32 = rms(f - r32["f"]) / rms(r64["f"])
     e64 = r64["Vvdw"] + r64["Vc"]
@@ -152,7 +152,7 @@ def _compare(s, ctx, coulomb="ewald", widen_energy=True):
     assert ferr32 <= FORCE_TOL_RMS
     for got, key in ((vlj, "Vvdw"), (vc, "Vc")):
         scale = max(abs(r64[key]), abs(e64))
-        bar = _energy_bar(s, key, scale, coulomb) if widen_energy else ENERGY_TOL_REL * scale
+        bar = _energy_bar(s, key, scale, coulomb, vdw=vdw) if widen_energy else ENERGY_TOL_REL * scale
         assert abs(got - r64[key]) <= bar, (key, got, r64[key], bar)
     vir = ctx.virial()
     np.testing.assert_allclose(vir, r64["virial"], rtol=0, atol=1e-4 * np.abs(r64["virial"]).max())
@@ -239,6 +239,64 @@ def test_reaction_field_flavours(coulomb):
     s = S.water_box(300, 2.1, seed=5)
     ctx = nbx.context_for(s, coulomb=coulomb)
     _compare(s, ctx, coulomb="rf")
+
+
+@pytest.mark.parametrize("modifier", ["force-switch", "potential-switch"])
+@pytest.mark.parametrize("name,s", _systems()[1:])
+def test_lj_modifiers(name, s, modifier):
+    # the plain-C kernel's LJ_FORCE_SWITCH / LJ_POT_SWITCH flavours
+    # (kernel_ref_inner.h:155-205) against the FP64 and FP32 oracle; the
+    # water box has types without LJ (the skip test), the chains 5 types and
+    # bonded exclusions
+    print(f"[lj modifier] {name} {modifier}", flush=True)
+    vdw = (modifier, 0.6)
+    ctx = nbx.context_for(s, vdw_modifier=modifier, rvdw_switch=vdw[1])
+    _compare(s, ctx, vdw=vdw)
+
+
+def test_lorentz_berthelot_parameters():
+    # the kernel reads the full C6/C12 matrices, so Lorentz-Berthelot
+    # parameters (nbnxm's ljcrLB, atomdata.cpp:540-590) need only the host's
+    # matrix builder; checked against the FP64 oracle on the same matrices
+    _, s = _systems()[2]
+    sig = (s.c12 / s.c6) ** (1.0 / 6.0)
+    eps = s.c6 ** 2 / (4.0 * s.c12)
+    m6, m12 = nbx.lorentz_berthelot_c6c12_matrices(sig, eps)
+    ctx = nbx.Context(rlist=s.rlist, rcoulomb=s.rcoulomb)
+    ctx.set_nonbonded_parameters(m6, m12)
+    ctx.set_topology(s.types, s.charges, s.excl_off, s.excl_idx)
+    ctx.put_on_grid(s.box, s.x)
+    ctx.construct_pairlist()
+    f = ctx.compute(s.x, energy=True)
+    vlj, vc = ctx.energies()
+    o = O.Oracle("f64", "ewald", rlist=s.rlist, rcoulomb=s.rcoulomb)
+    nbfp = np.stack([6.0 * m6, 12.0 * m12], axis=-1).reshape(-1)
+    o.setup(s.box, s.x, s.types, s.charges, nbfp, s.ntypes, s.excl_off, s.excl_idx)
+    r64 = o.compute(s.x)
+    geo = oracle_for(s, "f64").compute(s.x)
+    ferr = rms(f - r64["f"]) / rms(r64["f"])
+    print(f"  LB force rms err {ferr:.2e}; Vlj {vlj:.6f} ({r64['Vvdw']:.6f}, geometric {geo['Vvdw']:.6f})",
+          flush=True)
+    assert ferr <= FORCE_TOL_RMS
+    scale = abs(r64["Vvdw"]) + abs(r64["Vc"])
+    assert abs(vlj - r64["Vvdw"]) <= 1e-5 * scale
+    assert abs(vc - r64["Vc"]) <= 1e-5 * scale
+    # the rule matters on this system
+    assert abs(geo["Vvdw"] - r64["Vvdw"]) > 1e-3 * abs(r64["Vvdw"])
+
+
+def test_lj_modifier_option_errors():
+    s = _systems()[0][1]
+    with pytest.raises(nbx.NbxInputError):
+        nbx.context_for(s, vdw_modifier="force-switch", rvdw_switch=s.rcoulomb)
+    with pytest.raises(nbx.NbxInputError):
+        nbx.context_for(s, vdw_modifier="potential-switch", rvdw_switch=-0.1)
+    with pytest.raises(nbx.NbxInputError):
+        nbx.context_for(s, vdw_modifier="exact-cutoff")
+    # the adjoint covers the potential shift only
+    ctx = nbx.context_for(s, vdw_modifier="force-switch", rvdw_switch=0.5)
+    with pytest.raises(nbx.NbxInputError):
+        ctx.compute_adjoint(s.x, np.zeros_like(s.x))
 
 
 @pytest.mark.parametrize("name", ["nblib_argon.json", "nblib_spc_methanol.json"])
