@@ -60,6 +60,42 @@ float ewaldCoeff(float rc, float rtol)
     return beta;
 }
 
+// calc_ewaldcoeff_lj (ewald/ewald_utils.cpp:76-115): the same doubling and
+// bisection as ewaldCoeff on g(x) = exp(-x^2) (1 + x^2 + x^4 / 2), the
+// dispersion's share beyond x = beta rc
+float ljGridTail(float beta, float rc)
+{
+    const float x2 = (beta * rc) * (beta * rc);
+    return std::exp(-x2) * (1 + x2 + x2 * x2 / 2.0f);
+}
+
+float ewaldCoeffLJ(float rc, float rtol)
+{
+    float beta = 5, low, high;
+    int   n, i = 0;
+    do
+    {
+        i++;
+        beta *= 2;
+    } while (ljGridTail(beta, rc) > rtol);
+    n    = i + 60;
+    low  = 0;
+    high = beta;
+    for (i = 0; i < n; i++)
+    {
+        beta = (low + high) / 2;
+        if (ljGridTail(beta, rc) > rtol)
+        {
+            low = beta;
+        }
+        else
+        {
+            high = beta;
+        }
+    }
+    return beta;
+}
+
 // ewald_spline3_table_scale, Coulomb part (tables/forcetable.cpp:304-358)
 float ewaldTableScale(float beta, float rc)
 {
@@ -272,7 +308,7 @@ void buildWorkGroups(nbx_context* c, WorkLayout& L, bool adjoint)
 {
     auto ldsBytes = [&](int slots) {
         return adjoint ? adjointLdsBytes(c->ntype, c->ic.tab_n, c->elec, slots)
-                       : nonbondedLdsBytes(c->ntype, c->ic.tab_n, c->elec, slots);
+                       : nonbondedLdsBytes(c->ntype, c->ic.tab_n, c->elec, slots, c->opt.vdw_type == NBX_VDW_PME);
     };
     const int nc = c->ncluster;
     // [0, kGroupSizeCount): largest j-entry span per candidate size, then
@@ -430,7 +466,8 @@ void scheduleLayout(nbx_context* c, WorkLayout& L, bool adjoint)
     }
     else
     {
-        const int lds      = nonbondedLdsBytes(c->ntype, c->ic.tab_n, c->elec, L.windowSlots);
+        const int lds      = nonbondedLdsBytes(c->ntype, c->ic.tab_n, c->elec, L.windowSlots,
+                                               c->opt.vdw_type == NBX_VDW_PME);
         const int maxPerCU = c->opt.kernel_blocks_per_cu > 0 ? c->opt.kernel_blocks_per_cu : 4;
         const int perCU    = std::max(1, std::min(maxPerCU, (160 * 1024) / lds));
         numWaves = std::max(1, std::min((L.nchunk + kKernelWaves - 1) / kKernelWaves, perCU * c->numCU)) * kKernelWaves;
@@ -473,6 +510,9 @@ void nbx_default_options(nbx_options* o)
     o->kernel_blocks_per_cu  = 0;
     o->vdw_modifier          = NBX_VDW_POTENTIAL_SHIFT;
     o->rvdw_switch           = 0.0;
+    o->vdw_type              = NBX_VDW_CUTOFF;
+    o->ljpme_comb_rule       = NBX_LJPME_GEOMETRIC;
+    o->ewald_rtol_lj         = 1e-3;
 }
 
 int nbx_create(const nbx_options* options, nbx_context** context)
@@ -521,6 +561,25 @@ int nbx_create(const nbx_options* options, nbx_context** context)
         {
             throw InputError("the LJ switch radius must be in [0, rcoulomb)");
         }
+        if (o.vdw_type != NBX_VDW_CUTOFF && o.vdw_type != NBX_VDW_PME)
+        {
+            throw InputError("unknown LJ type");
+        }
+        if (o.vdw_type == NBX_VDW_PME)
+        {
+            if (o.vdw_modifier != NBX_VDW_POTENTIAL_SHIFT)
+            {
+                throw InputError("LJ-PME takes the potential-shift modifier");
+            }
+            if (o.ljpme_comb_rule != NBX_LJPME_GEOMETRIC && o.ljpme_comb_rule != NBX_LJPME_LORENTZ_BERTHELOT)
+            {
+                throw InputError("unknown LJ-PME combination rule");
+            }
+            if (!(o.ewald_rtol_lj > 0 && o.ewald_rtol_lj < 1))
+            {
+                throw InputError("ewald_rtol_lj must be in (0, 1)");
+            }
+        }
         int ndev = 0;
         check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
         if (o.device < 0 || o.device >= ndev)
@@ -548,6 +607,16 @@ int nbx_create(const nbx_options* options, nbx_context** context)
         ic.cpot_disp = static_cast<float>(-1.0 / p6);
         ic.cpot_rep  = static_cast<float>(-1.0 / p12);
         setVdwModifierConstants(ic, o.vdw_modifier, static_cast<float>(o.rvdw_switch), rvdw);
+        if (o.vdw_type == NBX_VDW_PME)
+        {
+            // forcerec.cpp:703-717 and the kernel's lje_coeff6_6
+            // (kernel_ref_outer.h:182-186), in float
+            const float betaLJ = ewaldCoeffLJ(rvdw, static_cast<float>(o.ewald_rtol_lj));
+            const float crc2   = (betaLJ * rvdw) * (betaLJ * rvdw);
+            ic.lje_coeff2      = betaLJ * betaLJ;
+            ic.lje_coeff6_6    = ic.lje_coeff2 * ic.lje_coeff2 * ic.lje_coeff2 / 6.0f;
+            ic.sh_lj_ewald     = (std::exp(-crc2) * (1 + crc2 + 0.5f * crc2 * crc2) - 1) / p6;
+        }
         ic.cpot_disp6 = static_cast<float>(static_cast<double>(ic.cpot_disp) / 6.0);
         ic.cpot_rep12 = static_cast<float>(static_cast<double>(ic.cpot_rep) / 12.0);
         const float epsr = static_cast<float>(o.epsilon_r);
@@ -749,6 +818,51 @@ int nbx_set_nonbonded_parameters(nbx_context* c, int numTypes, const double* c6,
         c->typeLJ.ensure(t);
         check(hipMemcpy(c->typeLJ.p, lj.data(), sizeof(int) * t, hipMemcpyHostToDevice), "upload LJ flags");
         c->anyTypeWithoutLJ = std::count(lj.begin(), lj.end() - 1, 0) > 0;
+        c->c6diag.assign(t, 0.0f);
+        for (int i = 0; i < t; i++)
+        {
+            c->c6diag[i] = nb[i * t + i].x;
+        }
+        if (c->opt.vdw_type == NBX_VDW_PME)
+        {
+            // the grid's per-type parameters from the nbfp diagonal
+            // (atomdata.cpp:288-330): geometric sqrt(6 C6_ii); LB
+            // 0.5 (c12 / c6)^(1/6) and sqrt(c6^2 / c12), which combine to
+            // 6 C6 of the LB sigma and epsilon; the padding type gets zeros
+            std::vector<float> ljc(2 * t, 0.0f);
+            for (int i = 0; i < t; i++)
+            {
+                const float c6i = nb[i * t + i].x, c12i = nb[i * t + i].y;
+                if (c->opt.ljpme_comb_rule == NBX_LJPME_GEOMETRIC)
+                {
+                    ljc[2 * i] = std::sqrt(c6i);
+                }
+                else if (c6i > 0 && c12i > 0)
+                {
+                    ljc[2 * i]     = 0.5f * static_cast<float>(std::pow(static_cast<double>(c12i / c6i), 1.0 / 6.0));
+                    ljc[2 * i + 1] = std::sqrt(c6i * c6i / c12i);
+                }
+            }
+            std::vector<float> grid(static_cast<size_t>(t) * t, 0.0f);
+            for (int i = 0; i < t; i++)
+            {
+                for (int j = 0; j < t; j++)
+                {
+                    if (c->opt.ljpme_comb_rule == NBX_LJPME_GEOMETRIC)
+                    {
+                        grid[i * t + j] = ljc[2 * i] * ljc[2 * j];
+                    }
+                    else
+                    {
+                        const float sg = ljc[2 * i] + ljc[2 * j], sg2 = sg * sg;
+                        grid[i * t + j] = ljc[2 * i + 1] * ljc[2 * j + 1] * sg2 * sg2 * sg2;
+                    }
+                }
+            }
+            c->c6grid.ensure(grid.size());
+            check(hipMemcpy(c->c6grid.p, grid.data(), sizeof(float) * grid.size(), hipMemcpyHostToDevice),
+                  "upload LJ-PME grid C6");
+        }
         // new values for the same types leave the grid and the pair list
         // valid (they depend on positions only); a new type count does not
         if (t != c->ntype)
@@ -787,6 +901,11 @@ int nbx_set_topology(nbx_context* c, int numAtoms, const int* types, const float
             {
                 throw InputError("non-finite charge");
             }
+        }
+        c->typeCount.assign(c->ntypeUser, 0);
+        for (int a = 0; a < numAtoms; a++)
+        {
+            c->typeCount[types[a]]++;
         }
         if (exclOff[0] != 0)
         {
@@ -1167,6 +1286,8 @@ static void enqueueStep(nbx_context* c, unsigned flags, const float* xd, float* 
     a.typeLJ        = c->typeLJ.p;
     a.skipLJ        = c->anyTypeWithoutLJ ? 1 : 0;
     a.vdwModifier   = c->opt.vdw_modifier;
+    a.ljPme         = c->opt.vdw_type == NBX_VDW_PME ? 1 : 0;
+    a.c6grid        = c->c6grid.p;
     a.ntype         = c->ntype;
     a.tabFDV0       = c->tabFDV0.p;
     a.chunkShift    = c->lay.chunkShift.p;
@@ -1351,9 +1472,9 @@ int nbx_compute_adjoint(nbx_context* c, const float* x, const float* g, float* g
         {
             throw InputError("the force cotangent g must not be NULL");
         }
-        if (c->opt.vdw_modifier != NBX_VDW_POTENTIAL_SHIFT)
+        if (c->opt.vdw_modifier != NBX_VDW_POTENTIAL_SHIFT || c->opt.vdw_type != NBX_VDW_CUTOFF)
         {
-            throw InputError("the adjoint supports the LJ potential shift only");
+            throw InputError("the adjoint supports the LJ cut-off with the potential shift only");
         }
         if (c->ntype > kAdjointMaxTypes)
         {
@@ -1470,6 +1591,16 @@ int nbx_get_energies(nbx_context* c, double* vlj, double* vc)
         double e[2];
         check(hipMemcpyAsync(e, c->energy.p, sizeof(e), hipMemcpyDeviceToHost, c->stream), "read energies");
         check(hipStreamSynchronize(c->stream), "sync");
+        if (c->opt.vdw_type == NBX_VDW_PME)
+        {
+            // LJ-PME self energy, 0.5 C6_ii beta^6 / 6 per atom
+            // (kernel_ref_outer.h:315-322): a constant of the topology
+            for (size_t t = 0; t < c->typeCount.size() && t < c->c6diag.size(); t++)
+            {
+                e[0] += static_cast<double>(c->typeCount[t]) * 0.5 * static_cast<double>(c->c6diag[t]) / 6.0
+                        * static_cast<double>(c->ic.lje_coeff6_6);
+            }
+        }
         if (vlj)
         {
             *vlj = e[0];

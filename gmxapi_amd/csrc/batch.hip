@@ -365,6 +365,10 @@ int nbx_batch_create(const nbx_options* options, int numTypes, const double* c6,
     auto* b = new nbx_batch();
     int   rc = guarded([&] {
         b->opt    = *options;
+        if (options->vdw_type != NBX_VDW_CUTOFF)
+        {
+            throw InputError("batched ensembles take the LJ cut-off (no LJ-PME)");
+        }
         b->ntypes = numTypes;
         b->c6.assign(c6, c6 + numTypes * numTypes);
         b->c12.assign(c12, c12 + numTypes * numTypes);

@@ -268,6 +268,11 @@ struct nbx_context
     nbx::DevBuf<int>    groupStats;
     nbx::DevBuf<int>    typeLJ; // per type (padding type included): 1 if it has LJ with any type
     bool                anyTypeWithoutLJ = false; // among the user's types
+    // LJ-PME: the grid's 6 C6 per type pair; the host's 6 C6 diagonal and
+    // the per-type atom counts give the self energy
+    nbx::DevBuf<float>  c6grid;
+    std::vector<float>  c6diag;
+    std::vector<long>   typeCount;
     nbx::WorkLayout     lay;
     nbx::WorkLayout     adj;
     bool                adjValid = false;

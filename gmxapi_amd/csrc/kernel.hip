@@ -83,7 +83,7 @@ constexpr int kThreads    = kWave * kWavesBlock;
 template<int kElec, bool kEnergy, bool kChecks, int kLJ, int kI>
 __device__ __forceinline__ void pairI(const IAtoms& I, const JAtom& J, const unsigned mbits, const bool diag,
                                       const int jj, const InteractionConst& ic, const float2* __restrict__ sNbfp,
-                                      const float4* __restrict__ sTab, float* fi, float& fjx, float& fjy,
+                                      const float* __restrict__ sC6g, const float4* __restrict__ sTab, float* fi, float& fjx, float& fjy,
                                       float& fjz, float& vlj, float& vc)
 {
     const float dx       = I.x[kI] - J.x;
@@ -173,6 +173,27 @@ __device__ __forceinline__ void pairI(const IAtoms& I, const JAtom& J, const uns
             const float v12 = fmaf(c.x, -2.0f * ic.cpot_disp, fmaf(c.y, ic.cpot_rep, frLJ - FrLJ6));
             vlj             = fmaf(v12, emask, vlj);
         }
+        if (kLJ == kLJEwald)
+        {
+            // LJ-PME: subtract the grid's dispersion, c6g (r^-6 - e^-x (r^-6
+            // (1 + x + x^2/2) + beta^6/6)) with x = beta^2 r^2, from r^-6
+            // without the exclusion mask (excluded pairs inside the cut-off
+            // keep the grid correction, as Ewald's Coulomb ones do); beyond
+            // the cut-off r^-6 = 0 and the force's 1/r^2 = 0
+            const float c6g     = sC6g[I.tOff[kI] + J.type];
+            const float rinv6nm = rinvsq * rinvsq * rinvsq;
+            const float cr2     = ic.lje_coeff2 * rsq;
+            const float expm    = __builtin_amdgcn_exp2f(-cr2 * kLog2e);
+            const float poly    = fmaf(0.5f * cr2, cr2, 1.0f + cr2);
+            frLJ                = fmaf(c6g, rinv6nm - expm * fmaf(rinv6nm, poly, ic.lje_coeff6_6), frLJ);
+            if (kEnergy)
+            {
+                // 12 (c6g / 6) (r^-6 (1 - e^-x poly) + sh_lj_ewald interact),
+                // masked by the cut-off and the diagonal only
+                const float vg = 2.0f * c6g * fmaf(rinv6nm, fmaf(-expm, poly, 1.0f), ic.sh_lj_ewald * interact);
+                vlj            = skip ? vlj : vlj + vg;
+            }
+        }
     }
 
     float fcoul;
@@ -242,28 +263,29 @@ __device__ __forceinline__ void pairI(const IAtoms& I, const JAtom& J, const uns
 template<int kElec, bool kEnergy, bool kChecks, int kLJ>
 __device__ __forceinline__ void tile4(const IAtoms& I, const JAtom& J, const unsigned mbits, const bool diag,
                                       const int jj, const InteractionConst& ic, const float2* __restrict__ sNbfp,
-                                      const float4* __restrict__ sTab, float* fi, float& fjx, float& fjy,
+                                      const float* __restrict__ sC6g, const float4* __restrict__ sTab, float* fi, float& fjx, float& fjy,
                                       float& fjz, float& vlj, float& vc)
 {
     // barriers between the i-atoms keep the scheduler from interleaving all
     // four pair evaluations, which costs registers (occupancy) for little overlap
-    pairI<kElec, kEnergy, kChecks, kLJ, 0>(I, J, mbits, diag, jj, ic, sNbfp, sTab, fi, fjx, fjy, fjz, vlj, vc);
+    pairI<kElec, kEnergy, kChecks, kLJ, 0>(I, J, mbits, diag, jj, ic, sNbfp, sC6g, sTab, fi, fjx, fjy, fjz, vlj, vc);
     __builtin_amdgcn_sched_barrier(0);
-    pairI<kElec, kEnergy, kChecks, kLJ, 1>(I, J, mbits, diag, jj, ic, sNbfp, sTab, fi, fjx, fjy, fjz, vlj, vc);
+    pairI<kElec, kEnergy, kChecks, kLJ, 1>(I, J, mbits, diag, jj, ic, sNbfp, sC6g, sTab, fi, fjx, fjy, fjz, vlj, vc);
     __builtin_amdgcn_sched_barrier(0);
-    pairI<kElec, kEnergy, kChecks, kLJ, 2>(I, J, mbits, diag, jj, ic, sNbfp, sTab, fi, fjx, fjy, fjz, vlj, vc);
+    pairI<kElec, kEnergy, kChecks, kLJ, 2>(I, J, mbits, diag, jj, ic, sNbfp, sC6g, sTab, fi, fjx, fjy, fjz, vlj, vc);
     __builtin_amdgcn_sched_barrier(0);
-    pairI<kElec, kEnergy, kChecks, kLJ, 3>(I, J, mbits, diag, jj, ic, sNbfp, sTab, fi, fjx, fjy, fjz, vlj, vc);
+    pairI<kElec, kEnergy, kChecks, kLJ, 3>(I, J, mbits, diag, jj, ic, sNbfp, sC6g, sTab, fi, fjx, fjy, fjz, vlj, vc);
 }
 
 // LDS layout of a workgroup, in float4 units
 struct LdsLayout
 {
-    int nbfp, tab, fsh, ix, it, win;
-    __host__ __device__ LdsLayout(int ntype, int tabN, int windowSlots)
+    int nbfp, grid, tab, fsh, ix, it, win;
+    __host__ __device__ LdsLayout(int ntype, int tabN, int windowSlots, bool ljPme)
     {
         nbfp  = 0;
-        tab   = nbfp + (ntype * ntype + 1) / 2;
+        grid  = nbfp + (ntype * ntype + 1) / 2;
+        tab   = grid + (ljPme ? (ntype * ntype + 3) / 4 : 0);
         fsh   = tab + tabN;
         ix    = fsh + (kNumShifts * 3 + 3) / 4;
         it    = ix + kWavesBlock * kGroupSizeMax * kClusterSize;
@@ -280,14 +302,22 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
     const int       ntype  = a.ntype;
     const int       ntype2 = ntype * ntype;
     const int       tabN   = (kElec == kElecEwaldTable) ? a.ic.tab_n : 0;
-    const LdsLayout L(ntype, tabN, a.windowSlots);
+    const LdsLayout L(ntype, tabN, a.windowSlots, kLJ == kLJEwald);
     float2*         sNbfp  = reinterpret_cast<float2*>(smem4 + L.nbfp);
+    float*          sC6g   = reinterpret_cast<float*>(smem4 + L.grid);
     float4*         sTab   = smem4 + L.tab;
     float*          sFsh   = reinterpret_cast<float*>(smem4 + L.fsh);
 
     for (int t = threadIdx.x; t < ntype2; t += kThreads)
     {
         sNbfp[t] = a.nbfp[t];
+    }
+    if (kLJ == kLJEwald)
+    {
+        for (int t = threadIdx.x; t < ntype2; t += kThreads)
+        {
+            sC6g[t] = a.c6grid[t];
+        }
     }
     for (int t = threadIdx.x; t < tabN; t += kThreads)
     {
@@ -443,12 +473,12 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
             if (checked)
             {
                 const bool diag = central && static_cast<int>(e.x) == ci;
-                tile4<kElec, kEnergy, true, kLJ>(I, J, (e.y & 0xffffu) >> jj, diag, jj, ic, sNbfp, sTab, fi, fjx, fjy,
+                tile4<kElec, kEnergy, true, kLJ>(I, J, (e.y & 0xffffu) >> jj, diag, jj, ic, sNbfp, sC6g, sTab, fi, fjx, fjy,
                                             fjz, vlj, vc);
             }
             else
             {
-                tile4<kElec, kEnergy, false, kLJ>(I, J, 0u, false, jj, ic, sNbfp, sTab, fi, fjx, fjy, fjz, vlj, vc);
+                tile4<kElec, kEnergy, false, kLJ>(I, J, 0u, false, jj, ic, sNbfp, sC6g, sTab, fi, fjx, fjy, fjz, vlj, vc);
             }
             // the 16 entries of a tile have distinct slots (padding entries
             // share the spare slot, adding zeros): plain read-modify-write;
@@ -600,6 +630,10 @@ template<int kElec>
 hipError_t launchElec(hipStream_t s, const KernelArgs& a, bool energy, bool shiftForces, int numBlocks, int lds,
                       hipEvent_t start, hipEvent_t stop)
 {
+    if (a.ljPme)
+    {
+        return launchFlags<kElec, kLJEwald>(s, a, energy, shiftForces, numBlocks, lds, start, stop);
+    }
     if (a.vdwModifier == NBX_VDW_FORCE_SWITCH)
     {
         return launchFlags<kElec, kLJForceSwitch>(s, a, energy, shiftForces, numBlocks, lds, start, stop);
@@ -641,9 +675,9 @@ hipError_t allowDynamicLds(const void* fn, int lds)
     return e;
 }
 
-int nonbondedLdsBytes(int ntype, int tabN, int elec, int windowSlots)
+int nonbondedLdsBytes(int ntype, int tabN, int elec, int windowSlots, bool ljPme)
 {
-    const LdsLayout L(ntype, elec == kElecEwaldTable ? tabN : 0, windowSlots);
+    const LdsLayout L(ntype, elec == kElecEwaldTable ? tabN : 0, windowSlots, ljPme);
     return L.end_ * 16;
 }
 
@@ -654,7 +688,7 @@ hipError_t launchNonbonded(hipStream_t s, const KernelArgs& a, int elec, bool en
     {
         return hipSuccess;
     }
-    const int lds = nonbondedLdsBytes(a.ntype, a.ic.tab_n, elec, a.windowSlots);
+    const int lds = nonbondedLdsBytes(a.ntype, a.ic.tab_n, elec, a.windowSlots, a.ljPme != 0);
     if (elec == kElecRF)
     {
         return launchElec<kElecRF>(s, a, energy, shiftForces, numBlocks, lds, start, stop);

@@ -92,6 +92,7 @@ enum LJFlavour : int
     kLJShiftSkip   = 1,
     kLJForceSwitch = 2,
     kLJPotSwitch   = 3,
+    kLJEwald       = 4, // LJ-PME real space (potential shift; tests the flag)
 };
 
 struct InteractionConst
@@ -121,6 +122,8 @@ struct InteractionConst
     float fsw_c2d, fsw_c3d, fsw_c2r, fsw_c3r;
     float fsw_e2d, fsw_e3d, fsw_e2r, fsw_e3r;
     float sw_c3, sw_c4, sw_c5, sw_f3, sw_f4, sw_f5;
+    // LJ-PME: beta_lj^2, beta_lj^6 / 6 and the grid part's potential shift
+    float lje_coeff2, lje_coeff6_6, sh_lj_ewald;
 };
 
 struct GridDims
@@ -270,7 +273,9 @@ struct KernelArgs
     const float2* nbfp;
     const int*    typeLJ; // per type: 1 if it has LJ with any type
     int           skipLJ; // host: some type has no LJ (launch the variant that skips it)
-    int           vdwModifier; // host: NBX_VDW_*
+    int           vdwModifier; // host: NBX_VDW_* modifier
+    int           ljPme;       // host: LJ-PME (c6grid is read)
+    const float*  c6grid;      // LJ-PME: the grid's 6 C6 per type pair
     int           ntype;  // including the padding type
     const float4* tabFDV0;
     const float4* chunkShift; // per chunk: its i-entry's shift vector
@@ -295,7 +300,7 @@ hipError_t allowDynamicLds(const void* fn, int lds);
 // start/stop (either may be NULL) record the launch's own start and end
 hipError_t launchNonbonded(hipStream_t s, const KernelArgs& a, int elec, bool energy, bool shiftForces, int numBlocks,
                            hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
-int  nonbondedLdsBytes(int ntype, int tabN, int elec, int windowSlots);
+int  nonbondedLdsBytes(int ntype, int tabN, int elec, int windowSlots, bool ljPme = false);
 
 // -- adjoint.hip
 struct AdjointArgs

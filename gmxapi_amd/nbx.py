@@ -29,6 +29,9 @@ EWALD_CORRECTION = {"table": 0, "analytical": 1}
 # LJ modifiers (interaction_const_t::vdw_modifier): nblib's potential shift,
 # and the force and potential switches from rvdw_switch to the cut-off
 VDW_MODIFIER = {"potential-shift": 0, "force-switch": 1, "potential-switch": 2}
+# LJ type (interaction_const_t::vdwtype) and LJ-PME's grid combination rule
+VDW_TYPE = {"cutoff": 0, "pme": 1}
+LJPME_COMB_RULE = {"geometric": 0, "lorentz-berthelot": 1}
 
 COMPUTE_FORCES = 1
 COMPUTE_ENERGY = 2
@@ -62,12 +65,14 @@ class _Options(C.Structure):
                 ("coulomb_cutoff", C.c_double), ("ewald_rtol", C.c_double), ("epsilon_r", C.c_double),
                 ("epsilon_rf", C.c_double), ("ewald_potential_shift", C.c_int), ("device", C.c_int),
                 ("kernel_group_size", C.c_int), ("kernel_blocks_per_cu", C.c_int), ("vdw_modifier", C.c_int),
-                ("rvdw_switch", C.c_double)]
+                ("rvdw_switch", C.c_double), ("vdw_type", C.c_int), ("ljpme_comb_rule", C.c_int),
+                ("ewald_rtol_lj", C.c_double)]
 
 
 def _options(coulomb="ewald", ewald_correction="table", rlist=1.0, rcoulomb=None, ewald_rtol=1e-5, epsilon_r=1.0,
              epsilon_rf=1.0, ewald_potential_shift=False, device=0, kernel_group_size=0, kernel_blocks_per_cu=0,
-             vdw_modifier="potential-shift", rvdw_switch=0.0):
+             vdw_modifier="potential-shift", rvdw_switch=0.0, vdw_type="cutoff", ljpme_comb_rule="geometric",
+             ewald_rtol_lj=1e-3):
     o = _Options()
     library().nbx_default_options(C.byref(o))
     o.coulomb_type = COULOMB[coulomb]
@@ -85,6 +90,13 @@ def _options(coulomb="ewald", ewald_correction="table", rlist=1.0, rcoulomb=None
         raise NbxInputError(f"unknown LJ modifier {vdw_modifier!r}; one of {sorted(VDW_MODIFIER)}")
     o.vdw_modifier = VDW_MODIFIER[vdw_modifier]
     o.rvdw_switch = rvdw_switch
+    if vdw_type not in VDW_TYPE:
+        raise NbxInputError(f"unknown LJ type {vdw_type!r}; one of {sorted(VDW_TYPE)}")
+    if ljpme_comb_rule not in LJPME_COMB_RULE:
+        raise NbxInputError(f"unknown LJ-PME combination rule {ljpme_comb_rule!r}")
+    o.vdw_type = VDW_TYPE[vdw_type]
+    o.ljpme_comb_rule = LJPME_COMB_RULE[ljpme_comb_rule]
+    o.ewald_rtol_lj = ewald_rtol_lj
     return o
 
 
@@ -195,12 +207,13 @@ class Context:
                  rcoulomb: float | None = None, ewald_rtol: float = 1e-5, epsilon_r: float = 1.0,
                  epsilon_rf: float = 1.0, ewald_potential_shift: bool = False, device: int = 0,
                  kernel_group_size: int = 0, kernel_blocks_per_cu: int = 0,
-                 vdw_modifier: str = "potential-shift", rvdw_switch: float = 0.0):
+                 vdw_modifier: str = "potential-shift", rvdw_switch: float = 0.0, vdw_type: str = "cutoff",
+                 ljpme_comb_rule: str = "geometric", ewald_rtol_lj: float = 1e-3):
         lib = library()
         self._lib = lib
         o = _options(coulomb, ewald_correction, rlist, rcoulomb, ewald_rtol, epsilon_r, epsilon_rf,
                      ewald_potential_shift, device, kernel_group_size, kernel_blocks_per_cu, vdw_modifier,
-                     rvdw_switch)
+                     rvdw_switch, vdw_type, ljpme_comb_rule, ewald_rtol_lj)
         h = C.c_void_p()
         rc = lib.nbx_create(C.byref(o), C.byref(h))
         if rc != NBX_SUCCESS:
@@ -552,13 +565,14 @@ def batch_for(systems, **options) -> Batch:
 
 def context_for(system, coulomb="ewald", ewald_correction="table", device=0, potential_shift=False,
                 kernel_group_size=0, kernel_blocks_per_cu=0, vdw_modifier="potential-shift",
-                rvdw_switch=0.0) -> Context:
+                rvdw_switch=0.0, vdw_type="cutoff", ljpme_comb_rule="geometric", ewald_rtol_lj=1e-3) -> Context:
     """Context set up on a gmxapi_amd.systems.System the way nblib sets up its
     calculator: parameters, topology, grid, pair list."""
     ctx = Context(coulomb=coulomb, ewald_correction=ewald_correction, rlist=system.rlist,
                   rcoulomb=system.rcoulomb, ewald_potential_shift=potential_shift, device=device,
                   kernel_group_size=kernel_group_size, kernel_blocks_per_cu=kernel_blocks_per_cu,
-                  vdw_modifier=vdw_modifier, rvdw_switch=rvdw_switch)
+                  vdw_modifier=vdw_modifier, rvdw_switch=rvdw_switch, vdw_type=vdw_type,
+                  ljpme_comb_rule=ljpme_comb_rule, ewald_rtol_lj=ewald_rtol_lj)
     m6, m12 = nblib_c6c12_matrices(system.c6, system.c12)
     ctx.set_nonbonded_parameters(m6, m12)
     ctx.set_topology(system.types, system.charges, system.excl_off, system.excl_idx)

@@ -60,6 +60,19 @@ extern "C" {
 #define NBX_VDW_FORCE_SWITCH 1
 #define NBX_VDW_POTENTIAL_SWITCH 2
 
+/* LJ type (interaction_const_t::vdwtype): a plain cut-off, or LJ-PME, whose
+ * real-space kernel subtracts the grid's dispersion part with the
+ * potential shift (kernel_ref_inner.h:207-248, the self term
+ * kernel_ref_outer.h:315-322, ewaldcoeff_lj and sh_lj_ewald
+ * forcerec.cpp:703-717).  The grid's C6 per type pair comes from the C6/C12
+ * diagonal by the geometric or the Lorentz-Berthelot rule (nbfp_comb,
+ * atomdata.cpp:288-330; the rule as nbnxm_setup.cpp:398-410 picks it).
+ * The mesh part itself is not computed here (as in nbnxm). */
+#define NBX_VDW_CUTOFF 0
+#define NBX_VDW_PME 1
+#define NBX_LJPME_GEOMETRIC 0
+#define NBX_LJPME_LORENTZ_BERTHELOT 1
+
 /* compute flags (gmx::StepWorkload computeForces / computeEnergy /
  * computeVirial, gmxsetup.cpp:214-224) */
 #define NBX_COMPUTE_FORCES 1u
@@ -94,6 +107,9 @@ typedef struct nbx_options
                                      as LDS allows up to 4 (tuning) */
     int    vdw_modifier;          /* NBX_VDW_*, default POTENTIAL_SHIFT (nblib) */
     double rvdw_switch;           /* switch radius for the switches, nm, 0 <= rvdw_switch < rcoulomb */
+    int    vdw_type;              /* NBX_VDW_CUTOFF (nblib) or NBX_VDW_PME (potential shift only) */
+    int    ljpme_comb_rule;       /* NBX_LJPME_*: the grid's combination rule, default GEOMETRIC */
+    double ewald_rtol_lj;         /* LJ-PME: g(beta_lj rc) = rtol, default 1e-3 (ewald-rtol-lj) */
 } nbx_options;
 
 typedef struct nbx_pairlist_info
@@ -201,8 +217,9 @@ int nbx_set_charges(nbx_context* context, const float* charges, int on_device);
  * Any output may be NULL (no C6/C12 output skips their accumulation).  All
  * pointers are device pointers, used asynchronously on the context's
  * stream, when on_device is set; host pointers make the call synchronous.
- * At most 31 particle types; the LJ potential shift only (a context made
- * with another vdw_modifier gets NBX_ERROR_INPUT). */
+ * At most 31 particle types; the LJ cut-off with the potential shift only
+ * (a context made with another vdw_modifier or LJ-PME gets
+ * NBX_ERROR_INPUT). */
 int nbx_compute_adjoint(nbx_context* context, const float* x, const float* g, float* grad_x, double* grad_q,
                         double* grad_c6, double* grad_c12, int on_device);
 

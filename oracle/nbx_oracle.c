@@ -292,6 +292,10 @@ typedef struct
     real   rvdw_switch;
     real   fsw_c2d, fsw_c3d, fsw_c2r, fsw_c3r;
     real   sw_c3, sw_c4, sw_c5;
+    /* LJ-PME real space (vdwtype evdwPME): grid combination rule 0
+     * geometric, 1 Lorentz-Berthelot; beta_lj^2, beta_lj^6 / 6, sh_lj_ewald */
+    int    lj_pme, ljpme_comb;
+    real   lje_coeff2, lje_coeff6_6, lje_vc;
     real   tab_scale;
     int    tab_n;
     real*  tabF;
@@ -447,6 +451,70 @@ int orc_set_vdw_modifier(orc_sys* s, int modifier, double rvdw_switch)
         s->cpot_rep  = 0;
     }
     return 0;
+}
+
+/* g(x) = exp(-x^2) (1 + x^2 + x^4 / 2) at x = beta rc
+ * (compute_lj_function, ewald/ewald_utils.cpp:76-85) */
+static real lj_grid_tail(real beta, real rc)
+{
+    real xrc  = beta * rc;
+    real xrc2 = xrc * xrc;
+    real xrc4 = xrc2 * xrc2;
+    return (real)(exp(-xrc2) * (1 + xrc2 + xrc4 / 2.0));
+}
+
+/* calc_ewaldcoeff_lj (ewald_utils.cpp:87-115): doubling, then bisection */
+real orc_ewald_coeff_lj(real rc, real rtol)
+{
+    real beta = 5, low, high;
+    int  n, i = 0;
+    do
+    {
+        i++;
+        beta *= 2.0;
+    } while (lj_grid_tail(beta, rc) > rtol);
+    n    = i + 60;
+    low  = 0;
+    high = beta;
+    for (i = 0; i < n; ++i)
+    {
+        beta = (low + high) / 2.0;
+        if (lj_grid_tail(beta, rc) > rtol)
+        {
+            low = beta;
+        }
+        else
+        {
+            high = beta;
+        }
+    }
+    return beta;
+}
+
+/* LJ-PME's real-space part with the potential shift (forcerec.cpp:703-717,
+ * kernel_ref_outer.h:182-189).  Returns -1 for an unknown rule or a
+ * modifier other than the potential shift. */
+int orc_set_lj_pme(orc_sys* s, int comb_rule, double rtol_lj)
+{
+    if ((comb_rule != 0 && comb_rule != 1) || s->vdw_mod != 0 || !(rtol_lj > 0 && rtol_lj < 1))
+    {
+        return -1;
+    }
+    real rc         = s->rcoulomb;
+    real beta       = orc_ewald_coeff_lj(rc, (real)rtol_lj);
+    real crc2       = (beta * rc) * (beta * rc);
+    real p3         = rc * (rc * rc);
+    s->lj_pme       = 1;
+    s->ljpme_comb   = comb_rule;
+    s->lje_coeff2   = beta * beta;
+    s->lje_coeff6_6 = (real)(s->lje_coeff2 * s->lje_coeff2 * s->lje_coeff2 / 6.0);
+    s->lje_vc       = (real)((exp(-crc2) * (1 + crc2 + 0.5 * crc2 * crc2) - 1) / (p3 * p3));
+    return 0;
+}
+
+real orc_lj_pme_beta(const orc_sys* s)
+{
+    return s->lj_pme ? (real)sqrt((double)s->lje_coeff2) : 0;
 }
 
 void orc_destroy(orc_sys* s)
@@ -1510,6 +1578,24 @@ void orc_kernel(orc_sys* s, int energy)
     const real halfsp   = (real)(0.5 / s->tab_scale);
     const int  tab      = (s->coulomb == 1 && !s->ewald_exact);
     const int  exact    = (s->coulomb == 1 && s->ewald_exact);
+    /* LJ-PME grid parameters per type from the nbfp diagonal (nbfp_comb,
+     * atomdata.cpp:288-330) */
+    real* ljc = (real*)calloc((size_t)(2 * s->numTypes + 2), sizeof(real));
+    for (int t = 0; s->lj_pme && t < s->numTypes; t++)
+    {
+        real c6  = s->nbfp[(t * s->numTypes + t) * 2];
+        real c12 = s->nbfp[(t * s->numTypes + t) * 2 + 1];
+        if (s->ljpme_comb == 0)
+        {
+            ljc[t * 2]     = (real)sqrt((double)c6);
+            ljc[t * 2 + 1] = (real)sqrt((double)c12);
+        }
+        else if (c6 > 0 && c12 > 0)
+        {
+            ljc[t * 2]     = (real)(0.5 * pow((double)(c12 / c6), 1.0 / 6.0));
+            ljc[t * 2 + 1] = (real)sqrt((double)(c6 * c6 / c12));
+        }
+    }
     for (int k = 0; k < nslot * 3; k++)
     {
         s->fnb[k] = 0;
@@ -1566,6 +1652,12 @@ void orc_kernel(orc_sys* s, int energy)
                 for (int i = 0; i < ORC_NC; i++)
                 {
                     s->Vc -= qi[i] * q[ci * ORC_NC + i] * Vc_sub_self;
+                    if (s->lj_pme)
+                    {
+                        /* LJ Ewald self interaction (kernel_ref_outer.h:315-322) */
+                        int t = s->nbtype[ci * ORC_NC + i];
+                        s->Vvdw += (real)0.5 * s->nbfp[(t * s->numTypes + t) * 2] / 6 * s->lje_coeff6_6;
+                    }
                 }
             }
         }
@@ -1632,6 +1724,34 @@ void orc_kernel(orc_sys* s, int energy)
                         real dsw = (3 * s->sw_c3 + (4 * s->sw_c4 + 5 * s->sw_c5 * rsw) * rsw) * rsw * rsw;
                         frLJ     = frLJ * sw - r * VLJ * dsw;
                         VLJ *= sw;
+                    }
+                    if (s->lj_pme)
+                    {
+                        /* kernel_ref_inner.h:207-248 */
+                        int  ti = s->nbtype[ai], tj = s->nbtype[aj];
+                        real c6grid;
+                        if (s->ljpme_comb == 0)
+                        {
+                            c6grid = ljc[ti * 2] * ljc[tj * 2];
+                        }
+                        else
+                        {
+                            real sigma   = ljc[ti * 2] + ljc[tj * 2];
+                            real epsilon = ljc[ti * 2 + 1] * ljc[tj * 2 + 1];
+                            real sigma2  = sigma * sigma;
+                            c6grid       = epsilon * sigma2 * sigma2 * sigma2;
+                        }
+                        /* r^-6 without the exclusion mask */
+                        real rinvsix_nm = rinvsq * rinvsq * rinvsq;
+                        real cr2        = s->lje_coeff2 * rsq;
+                        real expmcr2    = (real)exp(-(double)cr2);
+                        real poly       = 1 + cr2 + (real)0.5 * cr2 * cr2;
+                        frLJ += c6grid * (rinvsix_nm - expmcr2 * (rinvsix_nm * poly + s->lje_coeff6_6));
+                        if (energy)
+                        {
+                            real sh_mask = s->lje_vc * interact;
+                            VLJ += c6grid / 6 * (rinvsix_nm * (1 - expmcr2 * poly) + sh_mask);
+                        }
                     }
                     if (energy)
                     {
@@ -1720,6 +1840,7 @@ void orc_kernel(orc_sys* s, int energy)
             s->Vc += Vc_ci;
         }
     }
+    free(ljc);
 }
 
 /* add_nbat_f_to_f for one output buffer (atomdata.cpp:1181-1228), with the

@@ -45,6 +45,12 @@ def _lib(precision: str) -> C.CDLL:
     lib.orc_destroy.argtypes = [vp]
     lib.orc_set_vdw_modifier.restype = C.c_int
     lib.orc_set_vdw_modifier.argtypes = [vp, C.c_int, C.c_double]
+    lib.orc_set_lj_pme.restype = C.c_int
+    lib.orc_set_lj_pme.argtypes = [vp, C.c_int, C.c_double]
+    lib.orc_lj_pme_beta.restype = real
+    lib.orc_lj_pme_beta.argtypes = [vp]
+    lib.orc_ewald_coeff_lj.restype = real
+    lib.orc_ewald_coeff_lj.argtypes = [real, real]
     lib.orc_ewald_coeff.restype = real
     lib.orc_ewald_coeff.argtypes = [real, real]
     lib.orc_ewald_table_scale.restype = real
@@ -180,6 +186,7 @@ def brute_forces(x, box, types, q, nbfp, ntype, excl_off, excl_idx, coulomb: str
 
 # interaction_const_t vdw_modifier values the oracle implements
 VDW_MODIFIER = {"potential-shift": 0, "force-switch": 1, "potential-switch": 2}
+LJPME_COMB_RULE = {"geometric": 0, "lorentz-berthelot": 1}
 
 
 class Oracle:
@@ -191,7 +198,7 @@ class Oracle:
 
     def __init__(self, precision="f64", coulomb="ewald", ewald_exact=False, rlist=1.0, rcoulomb=None,
                  epsilon_r=1.0, epsilon_rf=1.0, ewald_rtol=1e-5, ewald_potential_shift=False,
-                 vdw_modifier="potential-shift", rvdw_switch=0.0):
+                 vdw_modifier="potential-shift", rvdw_switch=0.0, lj_pme=None, ewald_rtol_lj=1e-3):
         self.precision = precision
         self.lib = _lib(precision)
         self.dt = np.float32 if precision == "f32" else np.float64
@@ -202,12 +209,20 @@ class Oracle:
         self.natoms = 0
         if self.lib.orc_set_vdw_modifier(self.h, VDW_MODIFIER[vdw_modifier], rvdw_switch) != 0:
             raise ValueError(f"bad LJ modifier {vdw_modifier!r} / rvdw_switch {rvdw_switch}")
+        # lj_pme: None (LJ cut-off) or the grid's combination rule
+        if lj_pme is not None:
+            if self.lib.orc_set_lj_pme(self.h, LJPME_COMB_RULE[lj_pme], ewald_rtol_lj) != 0:
+                raise ValueError(f"bad LJ-PME set-up {lj_pme!r} / rtol {ewald_rtol_lj}")
 
     def __del__(self):
         h = getattr(self, "h", None)
         if h:
             self.lib.orc_destroy(h)
             self.h = None
+
+    def lj_pme_beta(self) -> float:
+        """ewaldcoeff_lj (0 without LJ-PME)."""
+        return float(self.lib.orc_lj_pme_beta(self.h))
 
     def constants(self) -> dict:
         out = np.zeros(9)

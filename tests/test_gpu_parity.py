@@ -101,7 +101,7 @@ def info_clusters(ctx):
 _NOISE = {}
 
 
-def _fp32_noise(s, coulomb="ewald", ewald_exact=False, copies=6, vdw=("potential-shift", 0.0)):
+def _fp32_noise(s, coulomb="ewald", ewald_exact=False, copies=6, vdw=("potential-shift", 0.0), lj_pme=None):
     """RMS deviation from FP64 of the reference FP32 kernel (the oracle's
     single-precision restatement) over rigid translations of the system.
 
@@ -112,7 +112,7 @@ def _fp32_noise(s, coulomb="ewald", ewald_exact=False, copies=6, vdw=("potential
     (float coordinate differences and shift vectors carry ~1e-7 relative
     rounding that r^-12 and the Coulomb sum amplify), and one sample of it
     can land arbitrarily close to zero."""
-    key = (id(s), coulomb, ewald_exact, vdw)
+    key = (id(s), coulomb, ewald_exact, vdw, lj_pme)
     if key not in _NOISE:
         rng = np.random.default_rng(17)
         box = np.asarray(s.box, np.float64)
@@ -122,26 +122,26 @@ def _fp32_noise(s, coulomb="ewald", ewald_exact=False, copies=6, vdw=("potential
             x = np.mod(s.x + (0.0 if k == 0 else rng.uniform(0.0, 1.0, 3) * box), box)
             x[x.astype(np.float32) >= box.astype(np.float32)] = 0.0  # stay inside [0, L) in float
             t.x = x
-            r64 = oracle_for(t, "f64", coulomb=coulomb, ewald_exact=ewald_exact, vdw=vdw).compute(t.x)
-            r32 = oracle_for(t, "f32", coulomb=coulomb, vdw=vdw).compute(t.x)
+            r64 = oracle_for(t, "f64", coulomb=coulomb, ewald_exact=ewald_exact, vdw=vdw, lj_pme=lj_pme).compute(t.x)
+            r32 = oracle_for(t, "f32", coulomb=coulomb, vdw=vdw, lj_pme=lj_pme).compute(t.x)
             for e in dev:
                 dev[e].append(r32[e] - r64[e])
         _NOISE[key] = {e: float(np.sqrt(np.mean(np.square(v)))) for e, v in dev.items()}
     return _NOISE[key]
 
 
-def _energy_bar(s, key, scale, coulomb="ewald", ewald_exact=False, vdw=("potential-shift", 0.0)):
+def _energy_bar(s, key, scale, coulomb="ewald", ewald_exact=False, vdw=("potential-shift", 0.0), lj_pme=None):
     """Energy tolerance: 1e-6 relative, or on small boxes three times the
     FP32 reference kernel's own RMS rounding noise there, whichever is larger
     (the 9,000- and 288,000-atom boxes are held to the plain 1e-6)."""
-    return max(ENERGY_TOL_REL * scale, 3.0 * _fp32_noise(s, coulomb, ewald_exact, vdw=vdw)[key])
+    return max(ENERGY_TOL_REL * scale, 3.0 * _fp32_noise(s, coulomb, ewald_exact, vdw=vdw, lj_pme=lj_pme)[key])
 
 
-def _compare(s, ctx, coulomb="ewald", widen_energy=True, vdw=("potential-shift", 0.0)):
+def _compare(s, ctx, coulomb="ewald", widen_energy=True, vdw=("potential-shift", 0.0), lj_pme=None):
     f = ctx.compute(s.x, energy=True, virial=True)
     vlj, vc = ctx.energies()
-    r64 = oracle_for(s, "f64", coulomb=coulomb, vdw=vdw).compute(s.x)
-    r32 = oracle_for(s, "f32", coulomb=coulomb, vdw=vdw).compute(s.x)
+    r64 = oracle_for(s, "f64", coulomb=coulomb, vdw=vdw, lj_pme=lj_pme).compute(s.x)
+    r32 = oracle_for(s, "f32", coulomb=coulomb, vdw=vdw, lj_pme=lj_pme).compute(s.x)
     ferr64 = rms(f - r64["f"]) / rms(r64["f"])
     ferr32 = rms(f - r32["f"]) / rms(r64["f"])
     e64 = r64["Vvdw"] + r64["Vc"]
@@ -152,7 +152,7 @@ def _compare(s, ctx, coulomb="ewald", widen_energy=True, vdw=("potential-shift",
     assert ferr32 <= FORCE_TOL_RMS
     for got, key in ((vlj, "Vvdw"), (vc, "Vc")):
         scale = max(abs(r64[key]), abs(e64))
-        bar = _energy_bar(s, key, scale, coulomb, vdw=vdw) if widen_energy else ENERGY_TOL_REL * scale
+        bar = _energy_bar(s, key, scale, coulomb, vdw=vdw, lj_pme=lj_pme) if widen_energy else ENERGY_TOL_REL * scale
         assert abs(got - r64[key]) <= bar, (key, got, r64[key], bar)
     vir = ctx.virial()
     np.testing.assert_allclose(vir, r64["virial"], rtol=0, atol=1e-4 * np.abs(r64["virial"]).max())
@@ -254,6 +254,17 @@ def test_lj_modifiers(name, s, modifier):
     _compare(s, ctx, vdw=vdw)
 
 
+@pytest.mark.parametrize("rule", ["geometric", "lorentz-berthelot"])
+@pytest.mark.parametrize("name,s", _systems()[1:])
+def test_lj_pme(name, s, rule):
+    # LJ-PME's real-space kernel (kernel_ref_inner.h:207-248) with the grid's
+    # C6 by either combination rule, self terms included, against the FP64
+    # and FP32 oracle
+    print(f"[lj-pme] {name} {rule}", flush=True)
+    ctx = nbx.context_for(s, vdw_type="pme", ljpme_comb_rule=rule)
+    _compare(s, ctx, lj_pme=rule)
+
+
 def test_lorentz_berthelot_parameters():
     # the kernel reads the full C6/C12 matrices, so Lorentz-Berthelot
     # parameters (nbnxm's ljcrLB, atomdata.cpp:540-590) need only the host's
@@ -293,10 +304,18 @@ def test_lj_modifier_option_errors():
         nbx.context_for(s, vdw_modifier="potential-switch", rvdw_switch=-0.1)
     with pytest.raises(nbx.NbxInputError):
         nbx.context_for(s, vdw_modifier="exact-cutoff")
-    # the adjoint covers the potential shift only
+    # the adjoint covers the LJ cut-off with the potential shift only
     ctx = nbx.context_for(s, vdw_modifier="force-switch", rvdw_switch=0.5)
     with pytest.raises(nbx.NbxInputError):
         ctx.compute_adjoint(s.x, np.zeros_like(s.x))
+    ctx = nbx.context_for(s, vdw_type="pme")
+    with pytest.raises(nbx.NbxInputError):
+        ctx.compute_adjoint(s.x, np.zeros_like(s.x))
+    # LJ-PME takes the potential shift; batches take the cut-off
+    with pytest.raises(nbx.NbxInputError):
+        nbx.context_for(s, vdw_type="pme", vdw_modifier="force-switch", rvdw_switch=0.5)
+    with pytest.raises(nbx.NbxInputError):
+        nbx.batch_for([s], vdw_type="pme")
 
 
 @pytest.mark.parametrize("name", ["nblib_argon.json", "nblib_spc_methanol.json"])
