@@ -255,6 +255,46 @@ def bench_prune(stream, reps=10):
     }
 
 
+def bench_flavours(s, stream, reps=10):
+    """The other LJ flavours on the same 288k water box: the force and
+    potential switches (rvdw_switch 0.8 nm) and LJ-PME's real-space part
+    (geometric grid rule), each with energies, shift forces and virial: the
+    step and its pair kernel (HIP events recorded by the launch)."""
+    import numpy as np
+    import torch
+
+    from gmxapi_amd import nbx
+
+    dev = torch.device("cuda:0")
+    x = torch.from_numpy(s.x.astype(np.float32).reshape(-1)).to(dev)
+    f = torch.empty_like(x)
+    flags = nbx.COMPUTE_FORCES | nbx.COMPUTE_ENERGY | nbx.COMPUTE_SHIFT_FORCES | nbx.COMPUTE_VIRIAL
+    out = {}
+    for name, opts in (("potential-shift", {}),
+                       ("force-switch", {"vdw_modifier": "force-switch", "rvdw_switch": 0.8}),
+                       ("potential-switch", {"vdw_modifier": "potential-switch", "rvdw_switch": 0.8}),
+                       ("lj-pme-geometric", {"vdw_type": "pme"})):
+        ctx = nbx.context_for(s, **opts)
+        ctx.set_stream(stream.cuda_stream)
+
+        def step():
+            ctx.compute_device(x.data_ptr(), f.data_ptr(), flags)
+
+        step_ms = _time(step, stream, reps)
+        ctx.set_kernel_timing(reps)
+        for _ in range(reps):
+            step()
+        kernel_ms, _ = ctx.kernel_timing()
+        ctx.set_kernel_timing(0)
+        out[name] = {"step_ms": step_ms, "kernel_ms": kernel_ms}
+        ctx.close()
+    base = out["potential-shift"]["kernel_ms"]
+    for v in out.values():
+        v["kernel_vs_potential_shift"] = v["kernel_ms"] / base
+    out["workload"] = "water-288k, F+E+shift+virial step per LJ flavour (table Ewald)"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -420,6 +460,7 @@ def main():
         out["batched"] = bench_ensemble(args.ewald, stream)
         out["fwd_bwd"] = bench_forward_backward(stream)
         out["prune"] = bench_prune(stream)
+        out["lj_flavours"] = bench_flavours(s, stream)
         log(f"[bench] batched and forward+backward parts in {time.perf_counter() - t0:.1f} s")
     if not args.no_cpu_baseline:
         t_step, t_setup, nrep = cpu_baseline(s)
