@@ -34,10 +34,10 @@ def _split(f, replicas):
     return out
 
 
-def _single_results(replicas):
+def _single_results(replicas, **options):
     res = []
     for s in replicas:
-        ctx = nbx.context_for(s)
+        ctx = nbx.context_for(s, **options)
         f = ctx.compute(s.x, energy=True)
         res.append((f, ctx.energies()))
     return res
@@ -56,6 +56,26 @@ def test_batch_matches_single_systems(replicas):
     for k, (f_ref, e_ref) in enumerate(_single_results(replicas)):
         assert rms(fs[k] - f_ref) <= 1e-6 * rms(f_ref)
         np.testing.assert_allclose(e[k], e_ref, rtol=1e-10)
+
+
+@pytest.mark.parametrize("modifier", ["force-switch", "potential-switch"])
+def test_batch_with_lj_switches_matches_single_systems(replicas, modifier):
+    # the batched launch takes the LJ switch flavours like a single context
+    import torch
+
+    opts = {"vdw_modifier": modifier, "rvdw_switch": 0.7}
+    b = nbx.batch_for(replicas, **opts)
+    xd, fd = _device_arrays(replicas)
+    b.compute_device(xd.data_ptr(), fd.data_ptr(), FLAGS)
+    torch.cuda.synchronize()
+    e = b.energies()
+    fs = _split(fd.cpu().numpy(), replicas)
+    plain = _single_results(replicas)
+    for k, (f_ref, e_ref) in enumerate(_single_results(replicas, **opts)):
+        assert rms(fs[k] - f_ref) <= 1e-6 * rms(f_ref)
+        np.testing.assert_allclose(e[k], e_ref, rtol=1e-10)
+        # the switch changes the LJ energy
+        assert abs(e_ref[0] - plain[k][1][0]) > 1e-4 * abs(plain[k][1][0])
 
 
 def test_graph_replays_match_eager(replicas):
