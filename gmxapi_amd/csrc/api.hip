@@ -1591,16 +1591,7 @@ int nbx_get_energies(nbx_context* c, double* vlj, double* vc)
         double e[2];
         check(hipMemcpyAsync(e, c->energy.p, sizeof(e), hipMemcpyDeviceToHost, c->stream), "read energies");
         check(hipStreamSynchronize(c->stream), "sync");
-        if (c->opt.vdw_type == NBX_VDW_PME)
-        {
-            // LJ-PME self energy, 0.5 C6_ii beta^6 / 6 per atom
-            // (kernel_ref_outer.h:315-322): a constant of the topology
-            for (size_t t = 0; t < c->typeCount.size() && t < c->c6diag.size(); t++)
-            {
-                e[0] += static_cast<double>(c->typeCount[t]) * 0.5 * static_cast<double>(c->c6diag[t]) / 6.0
-                        * static_cast<double>(c->ic.lje_coeff6_6);
-            }
-        }
+        e[0] += ljPmeSelfEnergy(c);
         if (vlj)
         {
             *vlj = e[0];

@@ -113,7 +113,8 @@ void upload(DevBuf<T>& d, const std::vector<T>& h)
 void scheduleBatch(nbx_batch* b, int blocksPerCU, int chunkWeight)
 {
     nbx_context* c0    = b->sys[0];
-    const int    lds   = nonbondedLdsBytes(c0->ntype, c0->ic.tab_n, c0->elec, b->windowSlots);
+    const int    lds   = nonbondedLdsBytes(c0->ntype, c0->ic.tab_n, c0->elec, b->windowSlots,
+                                           c0->opt.vdw_type == NBX_VDW_PME);
     const int    maxBpc = blocksPerCU > 0 ? blocksPerCU : 4;
     b->blocksPerCU     = std::max(1, std::min(maxBpc, (160 * 1024) / lds));
     b->numBlocks       = std::max(1, std::min((b->nchunk + kKernelWaves - 1) / kKernelWaves, b->blocksPerCU * b->numCU));
@@ -299,6 +300,8 @@ void enqueueCompute(nbx_batch* b, const float* x, float* f, unsigned flags)
     a.typeLJ       = c0->typeLJ.p;
     a.skipLJ       = c0->anyTypeWithoutLJ ? 1 : 0;
     a.vdwModifier  = c0->opt.vdw_modifier;
+    a.ljPme        = c0->opt.vdw_type == NBX_VDW_PME ? 1 : 0;
+    a.c6grid       = c0->c6grid.p;
     a.ntype        = c0->ntype;
     a.tabFDV0      = c0->tabFDV0.p;
     a.chunkShift   = b->chunkShift.p;
@@ -365,10 +368,6 @@ int nbx_batch_create(const nbx_options* options, int numTypes, const double* c6,
     auto* b = new nbx_batch();
     int   rc = guarded([&] {
         b->opt    = *options;
-        if (options->vdw_type != NBX_VDW_CUTOFF)
-        {
-            throw InputError("batched ensembles take the LJ cut-off (no LJ-PME)");
-        }
         b->ntypes = numTypes;
         b->c6.assign(c6, c6 + numTypes * numTypes);
         b->c12.assign(c12, c12 + numTypes * numTypes);
@@ -555,6 +554,10 @@ int nbx_batch_get_energies(nbx_batch* b, double* energies)
         check(hipStreamSynchronize(b->stream), "sync");
         check(hipMemcpy(energies, b->energy.p, 2 * b->sys.size() * sizeof(double), hipMemcpyDeviceToHost),
               "read energies");
+        for (size_t r = 0; r < b->sys.size(); r++)
+        {
+            energies[2 * r] += ljPmeSelfEnergy(b->sys[r]);
+        }
     });
 }
 

@@ -287,3 +287,21 @@ struct nbx_context
     nbx::DevBuf<float>   gIn, gradXOut;
     bool            haveEnergy = false, haveShift = false, haveVirial = false;
 };
+
+// LJ-PME self energy, 0.5 C6_ii beta_lj^6 / 6 per atom
+// (kernel_ref_outer.h:315-322): a constant of the topology and parameters,
+// added when the energies are read; 0 without LJ-PME
+inline double ljPmeSelfEnergy(const nbx_context* c)
+{
+    double e = 0.0;
+    if (c->opt.vdw_type != NBX_VDW_PME)
+    {
+        return e;
+    }
+    for (size_t t = 0; t < c->typeCount.size() && t < c->c6diag.size(); t++)
+    {
+        e += static_cast<double>(c->typeCount[t]) * 0.5 * static_cast<double>(c->c6diag[t]) / 6.0
+             * static_cast<double>(c->ic.lje_coeff6_6);
+    }
+    return e;
+}

@@ -58,12 +58,15 @@ def test_batch_matches_single_systems(replicas):
         np.testing.assert_allclose(e[k], e_ref, rtol=1e-10)
 
 
-@pytest.mark.parametrize("modifier", ["force-switch", "potential-switch"])
-def test_batch_with_lj_switches_matches_single_systems(replicas, modifier):
-    # the batched launch takes the LJ switch flavours like a single context
+@pytest.mark.parametrize("opts", [{"vdw_modifier": "force-switch", "rvdw_switch": 0.7},
+                                  {"vdw_modifier": "potential-switch", "rvdw_switch": 0.7},
+                                  {"vdw_type": "pme", "ljpme_comb_rule": "lorentz-berthelot"}],
+                         ids=["force-switch", "potential-switch", "lj-pme"])
+def test_batch_with_lj_flavours_matches_single_systems(replicas, opts):
+    # the batched launch takes the other LJ flavours like a single context
+    # (LJ-PME's self energies included per system)
     import torch
 
-    opts = {"vdw_modifier": modifier, "rvdw_switch": 0.7}
     b = nbx.batch_for(replicas, **opts)
     xd, fd = _device_arrays(replicas)
     b.compute_device(xd.data_ptr(), fd.data_ptr(), FLAGS)
@@ -74,7 +77,7 @@ def test_batch_with_lj_switches_matches_single_systems(replicas, modifier):
     for k, (f_ref, e_ref) in enumerate(_single_results(replicas, **opts)):
         assert rms(fs[k] - f_ref) <= 1e-6 * rms(f_ref)
         np.testing.assert_allclose(e[k], e_ref, rtol=1e-10)
-        # the switch changes the LJ energy
+        # the flavour changes the LJ energy
         assert abs(e_ref[0] - plain[k][1][0]) > 1e-4 * abs(plain[k][1][0])
 
 

@@ -311,11 +311,9 @@ def test_lj_modifier_option_errors():
     ctx = nbx.context_for(s, vdw_type="pme")
     with pytest.raises(nbx.NbxInputError):
         ctx.compute_adjoint(s.x, np.zeros_like(s.x))
-    # LJ-PME takes the potential shift; batches take the cut-off
+    # LJ-PME takes the potential shift
     with pytest.raises(nbx.NbxInputError):
         nbx.context_for(s, vdw_type="pme", vdw_modifier="force-switch", rvdw_switch=0.5)
-    with pytest.raises(nbx.NbxInputError):
-        nbx.batch_for([s], vdw_type="pme")
 
 
 @pytest.mark.parametrize("name", ["nblib_argon.json", "nblib_spc_methanol.json"])
