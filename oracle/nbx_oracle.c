@@ -26,11 +26,19 @@
  *                                       kernel_ref_inner.h:44-375
  *   src/gromacs/pbcutil/pbc.cpp:1187-1202, pbcutil/ishift.h:40-55
  *   src/gromacs/mdlib/calcvir.cpp:103-148             virial
+ *   src/gromacs/mdlib/forcerec.cpp:703-717,780-818,848-873  LJ modifiers, LJ-PME
+ *   src/gromacs/ewald/ewald_utils.cpp:76-115          calc_ewaldcoeff_lj
+ *   src/gromacs/nbnxm/atomdata.cpp:288-330            LJ-PME grid parameters
+ *   src/gromacs/nbnxm/kernels_reference/kernel_ref_prune.cpp:39-123  pruning
  *
  * Parity pins (see tests/test_oracle_*.py): nblib's Argon and SPC-methanol
  * golden forces (RF/cut-off path), closed forms of the Ewald correction,
  * brute-force O(N^2) pair sets and FP64 pair sums, central finite differences
- * of the energy, Newton's third law and the virial closed form.
+ * of the energy, Newton's third law and the virial closed form; for the LJ
+ * switches and LJ-PME, energy gradients against forces, zero potential and
+ * force at the cut-off, the plain LJ below the switch radius, the
+ * smoothstep's value at mid-range, and LJ-PME's two-atom closed form, tail
+ * condition and self-energy limit.
  */
 #include <float.h>
 #include <math.h>
