@@ -265,6 +265,26 @@ def test_lj_pme(name, s, rule):
     _compare(s, ctx, lj_pme=rule)
 
 
+@pytest.mark.parametrize("opts,oracle_opts", [
+    ({"vdw_type": "pme"}, {"lj_pme": "geometric"}),
+    ({"vdw_modifier": "force-switch", "rvdw_switch": 0.8}, {"vdw": ("force-switch", 0.8)}),
+], ids=["lj-pme", "force-switch"])
+def test_lj_flavours_water288k(opts, oracle_opts):
+    # the full-size water box with another LJ flavour: forces against the
+    # FP64 oracle, energies to 1e-6 of each component on its own
+    print(f"[lj flavour] water 288k atoms {opts}", flush=True)
+    s = S.config(1)
+    ctx = nbx.context_for(s, **opts)
+    f = ctx.compute(s.x, energy=True, virial=True)
+    vlj, vc = ctx.energies()
+    r64 = oracle_for(s, "f64", **oracle_opts).compute(s.x)
+    ferr = rms(f - r64["f"]) / rms(r64["f"])
+    print(f"  force rms err {ferr:.2e}; Vlj {vlj:.6f} ({r64['Vvdw']:.6f}) Vc {vc:.6f} ({r64['Vc']:.6f})", flush=True)
+    assert ferr <= FORCE_TOL_RMS
+    assert abs(vlj - r64["Vvdw"]) <= ENERGY_TOL_REL * abs(r64["Vvdw"])
+    assert abs(vc - r64["Vc"]) <= ENERGY_TOL_REL * abs(r64["Vc"])
+
+
 def test_lorentz_berthelot_parameters():
     # the kernel reads the full C6/C12 matrices, so Lorentz-Berthelot
     # parameters (nbnxm's ljcrLB, atomdata.cpp:540-590) need only the host's
