@@ -23,13 +23,25 @@ import torch
 from . import nbx
 
 
+def _sqrt_zero_grad_at_zero(c: torch.Tensor) -> torch.Tensor:
+    """sqrt(c) for c >= 0 whose gradient at c = 0 is taken as 0 instead of
+    inf (the sqrt is evaluated on 1 where c = 0 and the result masked)."""
+    pos = c > 0
+    return torch.sqrt(torch.where(pos, c, torch.ones_like(c))) * pos
+
+
 def lj_matrices(sigma: torch.Tensor, epsilon: torch.Tensor):
     """Per-type sigma, epsilon -> (C6, C12) type-pair matrices with nblib's
     geometric rule on C6 and C12 (api/nblib/interactions.cpp:83-93):
-    C_ij = sqrt(C_i C_j), C6_i = 4 eps_i sigma_i^6, C12_i = 4 eps_i sigma_i^12."""
-    c6 = 4.0 * epsilon * sigma ** 6
-    c12 = 4.0 * epsilon * sigma ** 12
-    return torch.sqrt(torch.outer(c6, c6)), torch.sqrt(torch.outer(c12, c12))
+    C_ij = sqrt(C_i C_j), C6_i = 4 eps_i sigma_i^6, C12_i = 4 eps_i sigma_i^12.
+
+    Written as the outer product of sqrt(C_i), so a type with epsilon = 0
+    (water hydrogens) gives zero C rows without a sqrt of 0 on the gradient
+    path: its dL/dsigma is 0 (C does not depend on it) and its dL/depsilon,
+    infinite at epsilon = 0 (d sqrt(eps)/d eps), is reported as 0."""
+    s6 = _sqrt_zero_grad_at_zero(4.0 * epsilon * sigma ** 6)
+    s12 = _sqrt_zero_grad_at_zero(4.0 * epsilon * sigma ** 12)
+    return torch.outer(s6, s6), torch.outer(s12, s12)
 
 
 def _apply_parameters(nb: nbx.Context, q: torch.Tensor, c6: torch.Tensor, c12: torch.Tensor):

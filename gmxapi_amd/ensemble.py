@@ -60,6 +60,7 @@ class NonbondedEnsemble:
             raise nbx.NbxInputError("an ensemble needs at least one member")
         if coordinates is not None and len(coordinates) != len(systems):
             raise nbx.NbxInputError("one coordinate array per ensemble member")
+        nbx.check_batch_members(systems)
         self.width = len(systems)
         self._systems = list(systems)
         self._coordinates = coordinates

@@ -552,9 +552,28 @@ class Batch:
         return Context._view(h.value, self, self.natoms[index])
 
 
+def check_batch_members(systems):
+    """Raise NbxInputError unless every system shares the first one's C6,
+    C12, rlist and rcoulomb (host-side check, no device needed)."""
+    systems = list(systems)
+    if not systems:
+        raise NbxInputError("a batch needs at least one system")
+    s0 = systems[0]
+    for k, s in enumerate(systems[1:], start=1):
+        if (np.shape(s.c6) != np.shape(s0.c6) or np.shape(s.c12) != np.shape(s0.c12)
+                or not np.array_equal(s.c6, s0.c6) or not np.array_equal(s.c12, s0.c12)):
+            raise NbxInputError(f"batch member {k} has LJ parameters that differ from member 0's")
+        if s.rlist != s0.rlist or s.rcoulomb != s0.rcoulomb:
+            raise NbxInputError(f"batch member {k} has cut-offs (rlist {s.rlist}, rcoulomb {s.rcoulomb}) that "
+                                f"differ from member 0's ({s0.rlist}, {s0.rcoulomb})")
+
+
 def batch_for(systems, **options) -> Batch:
     """A Batch holding gmxapi_amd.systems.System objects that share LJ
-    parameters (set up as nblib would set up each one)."""
+    parameters (set up as nblib would set up each one).  Every member must
+    have the first member's per-type C6/C12 and cut-offs: one launch uses one
+    parameter matrix and one set of interaction constants."""
+    check_batch_members(systems)
     m6, m12 = nblib_c6c12_matrices(systems[0].c6, systems[0].c12)
     b = Batch(m6, m12, rlist=systems[0].rlist, rcoulomb=systems[0].rcoulomb, **options)
     for s in systems:

@@ -162,7 +162,10 @@ def hetero_chains(natoms: int = 150_000, box_length: float = 11.45, ntypes: int 
     uniformly per atom.  Each chain is a random walk with bond length `bond`;
     steps are rejected when the new atom would come closer than `min_dist` to
     any earlier atom other than its bonded predecessor (a cell-list check),
-    which keeps the LJ forces finite.  1-2 and 1-3 pairs are excluded.
+    which keeps the LJ forces finite.  A chain whose walk finds no free step
+    in 1,000 tries is removed and started again from a new random head, so
+    no two atoms other than bonded neighbours are ever closer than
+    `min_dist`.  1-2 and 1-3 pairs are excluded.
     """
     if natoms % chain:
         raise ValueError("natoms must be a multiple of the chain length")
@@ -202,31 +205,49 @@ def hetero_chains(natoms: int = 150_000, box_length: float = 11.45, ntypes: int 
                         a = nxt[a]
         return False
 
+    def cell_key(p):
+        c = cell_of(p)
+        return (c[0] * ncell + c[1]) * ncell + c[2]
+
     def insert(a, p):
         x[a] = p
-        c = cell_of(p)
-        k = (c[0] * ncell + c[1]) * ncell + c[2]
+        k = cell_key(p)
         nxt[a] = head[k]
         head[k] = a
 
-    a = 0
-    nchain = natoms // chain
-    for ch in range(nchain):
+    def remove_last(a):
+        # atoms are inserted at the head of their cell list and only the
+        # chain being grown is removed, newest first, so `a` is its cell's head
+        k = cell_key(x[a])
+        head[k] = nxt[a]
+        nxt[a] = -1
+
+    def grow(a0):
+        """Grow one chain from atom a0; False when a step finds no free
+        position (the caller removes the chain and starts it elsewhere)."""
         while True:
             p = rng.uniform(0, L, 3)
             if not clashes(p, -1):
                 break
-        insert(a, p)
-        a += 1
-        for k in range(1, chain):
+        insert(a0, p)
+        for a in range(a0 + 1, a0 + chain):
             for _ in range(1000):
                 v = rng.normal(size=3)
                 p = x[a - 1] + bond * v / np.linalg.norm(v)
                 p -= L * np.floor(p / L)
                 if not clashes(p, a - 1):
+                    insert(a, p)
                     break
-            insert(a, p)
-            a += 1
+            else:
+                for b in range(a - 1, a0 - 1, -1):
+                    remove_last(b)
+                return False
+        return True
+
+    nchain = natoms // chain
+    for ch in range(nchain):
+        while not grow(ch * chain):
+            pass
     box = np.array([L, L, L])
     x = _wrap(x, box)
     groups = []

@@ -52,14 +52,16 @@ def pair_table(x, box, rc: float):
 
 def excluded_mask(i, j, excl_off, excl_idx) -> np.ndarray:
     """True where the pair (i, j) is in the exclusion lists."""
+    excl_off = np.asarray(excl_off, np.int64)
+    excl_idx = np.asarray(excl_idx, np.int64)
     n = len(excl_off) - 1
-    ex = set()
-    for a in range(n):
-        for b in excl_idx[excl_off[a]:excl_off[a + 1]]:
-            b = int(b)
-            if b != a:
-                ex.add((min(a, b), max(a, b)))
-    return np.fromiter(((int(a), int(b)) in ex for a, b in zip(i, j)), bool, count=len(i))
+    a = np.repeat(np.arange(n, dtype=np.int64), np.diff(excl_off))
+    b = excl_idx
+    keep = a != b
+    ekeys = np.unique(np.minimum(a, b)[keep] * n + np.maximum(a, b)[keep])
+    i = np.asarray(i, np.int64)
+    j = np.asarray(j, np.int64)
+    return np.isin(np.minimum(i, j) * n + np.maximum(i, j), ekeys)
 
 
 def forces_and_gradient(system, g, c6m, c12m, charges=None, coulomb="ewald", beta=None, k_rf=0.0, pairs=None):
