@@ -513,6 +513,7 @@ void nbx_default_options(nbx_options* o)
     o->vdw_type              = NBX_VDW_CUTOFF;
     o->ljpme_comb_rule       = NBX_LJPME_GEOMETRIC;
     o->ewald_rtol_lj         = 1e-3;
+    o->vdw_cutoff            = 0.0;
 }
 
 int nbx_create(const nbx_options* options, nbx_context** context)
@@ -557,9 +558,19 @@ int nbx_create(const nbx_options* options, nbx_context** context)
         {
             throw InputError("unknown LJ modifier");
         }
-        if (o.vdw_modifier != NBX_VDW_POTENTIAL_SHIFT && !(o.rvdw_switch >= 0 && o.rvdw_switch < rcd))
+        if (!(o.vdw_cutoff >= 0) || o.vdw_cutoff > rcd)
         {
-            throw InputError("the LJ switch radius must be in [0, rcoulomb)");
+            throw InputError("the LJ cut-off must be in (0, rcoulomb] (0 means rcoulomb)");
+        }
+        const double rvdwd = o.vdw_cutoff > 0 ? o.vdw_cutoff : rcd;
+        if (rvdwd < rcd && (o.coulomb_type != NBX_COULOMB_EWALD || o.vdw_type != NBX_VDW_CUTOFF))
+        {
+            // grompp's condition (readir.cpp:386-398)
+            throw InputError("rvdw < rcoulomb is supported with Ewald electrostatics and LJ cut-off only");
+        }
+        if (o.vdw_modifier != NBX_VDW_POTENTIAL_SHIFT && !(o.rvdw_switch >= 0 && o.rvdw_switch < rvdwd))
+        {
+            throw InputError("the LJ switch radius must be in [0, rvdw)");
         }
         if (o.vdw_type != NBX_VDW_CUTOFF && o.vdw_type != NBX_VDW_PME)
         {
@@ -597,13 +608,17 @@ int nbx_create(const nbx_options* options, nbx_context** context)
         // setupInteractionConst (gmxsetup.cpp:226-284), float arithmetic
         c->rlist    = static_cast<float>(o.pairlist_cutoff);
         c->rcoulomb = static_cast<float>(rcd);
-        const float rvdw = c->rcoulomb;
+        // interaction_const_t::rvdw: the LJ potential shift and switches
+        // act at rvdw (forcerec.cpp:780-818, 848-873)
+        const float rvdw = o.vdw_cutoff > 0 ? static_cast<float>(o.vdw_cutoff) : c->rcoulomb;
+        c->twinCut       = rvdw < c->rcoulomb;
         const float p3   = rvdw * (rvdw * rvdw);
         const float p6   = p3 * p3;
         const float p12  = p6 * p6;
         InteractionConst& ic = c->ic;
         ic.rlist2    = c->rlist * c->rlist;
         ic.rcoulomb2 = c->rcoulomb * c->rcoulomb;
+        ic.rvdw2     = rvdw * rvdw;
         ic.cpot_disp = static_cast<float>(-1.0 / p6);
         ic.cpot_rep  = static_cast<float>(-1.0 / p12);
         setVdwModifierConstants(ic, o.vdw_modifier, static_cast<float>(o.rvdw_switch), rvdw);
@@ -1287,6 +1302,7 @@ static void enqueueStep(nbx_context* c, unsigned flags, const float* xd, float* 
     a.skipLJ        = c->anyTypeWithoutLJ ? 1 : 0;
     a.vdwModifier   = c->opt.vdw_modifier;
     a.ljPme         = c->opt.vdw_type == NBX_VDW_PME ? 1 : 0;
+    a.twinCut       = c->twinCut ? 1 : 0;
     a.c6grid        = c->c6grid.p;
     a.ntype         = c->ntype;
     a.tabFDV0       = c->tabFDV0.p;
@@ -1472,9 +1488,9 @@ int nbx_compute_adjoint(nbx_context* c, const float* x, const float* g, float* g
         {
             throw InputError("the force cotangent g must not be NULL");
         }
-        if (c->opt.vdw_modifier != NBX_VDW_POTENTIAL_SHIFT || c->opt.vdw_type != NBX_VDW_CUTOFF)
+        if (c->opt.vdw_modifier != NBX_VDW_POTENTIAL_SHIFT || c->opt.vdw_type != NBX_VDW_CUTOFF || c->twinCut)
         {
-            throw InputError("the adjoint supports the LJ cut-off with the potential shift only");
+            throw InputError("the adjoint supports the LJ cut-off with the potential shift at rvdw = rcoulomb only");
         }
         if (c->ntype > kAdjointMaxTypes)
         {
@@ -1816,6 +1832,41 @@ int nbx_get_interaction_constants(nbx_context* c, double* out)
     out[7] = ic.cpot_disp;
     out[8] = ic.cpot_rep;
     return NBX_SUCCESS;
+}
+
+int nbx_get_vdw_constants(nbx_context* c, double* out)
+{
+    if (!c || !out)
+    {
+        return fail(NBX_ERROR_INPUT, "NULL argument");
+    }
+    const InteractionConst& ic = c->ic;
+    const double v[12] = { std::sqrt(static_cast<double>(ic.rvdw2)), ic.rvdw_switch, ic.fsw_c2d, ic.fsw_c3d,
+                           ic.fsw_c2r, ic.fsw_c3r, ic.sw_c3, ic.sw_c4, ic.sw_c5, ic.lje_coeff2, ic.lje_coeff6_6,
+                           ic.sh_lj_ewald };
+    for (int k = 0; k < 12; k++)
+    {
+        out[k] = v[k];
+    }
+    return NBX_SUCCESS;
+}
+
+int nbx_get_ewald_table(nbx_context* c, int* n, float* fdv0)
+{
+    if (!c || !n)
+    {
+        return fail(NBX_ERROR_INPUT, "NULL argument");
+    }
+    return guarded([&] {
+        const bool ewald = c->opt.coulomb_type == NBX_COULOMB_EWALD;
+        *n               = ewald ? c->ic.tab_n : 0;
+        if (fdv0 && ewald)
+        {
+            setDevice(c);
+            check(hipMemcpy(fdv0, c->tabFDV0.p, sizeof(float4) * c->ic.tab_n, hipMemcpyDeviceToHost),
+                  "download table");
+        }
+    });
 }
 
 } // extern "C"

@@ -301,6 +301,7 @@ void enqueueCompute(nbx_batch* b, const float* x, float* f, unsigned flags)
     a.skipLJ       = c0->anyTypeWithoutLJ ? 1 : 0;
     a.vdwModifier  = c0->opt.vdw_modifier;
     a.ljPme        = c0->opt.vdw_type == NBX_VDW_PME ? 1 : 0;
+    a.twinCut      = c0->twinCut ? 1 : 0;
     a.c6grid       = c0->c6grid.p;
     a.ntype        = c0->ntype;
     a.tabFDV0      = c0->tabFDV0.p;

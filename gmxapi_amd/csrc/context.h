@@ -222,6 +222,7 @@ struct nbx_context
     nbx::InteractionConst ic{};
     double           selfCoef = 0.0;
     float            rlist = 1.0f, rcoulomb = 1.0f;
+    bool             twinCut = false; // rvdw < rcoulomb
 
     // parameters
     int            ntypeUser = -1;

@@ -86,6 +86,8 @@ __device__ __forceinline__ void pairI(const IAtoms& I, const JAtom& J, const uns
                                       const float* __restrict__ sC6g, const float4* __restrict__ sTab, float* fi, float& fjx, float& fjy,
                                       float& fjz, float& vlj, float& vc)
 {
+    constexpr int  kFl   = kLJ & kLJFlavourMask;
+    constexpr bool kTwin = (kLJ & kLJTwin) != 0;
     const float dx       = I.x[kI] - J.x;
     const float dy       = I.y[kI] - J.y;
     const float dz       = I.z[kI] - J.z;
@@ -118,17 +120,21 @@ __device__ __forceinline__ void pairI(const IAtoms& I, const JAtom& J, const uns
     // energy mask: 0 beyond the cut-off, on the diagonal and (with checks)
     // for excluded pairs, whose energies the exclusion pass adds
     const float emask = skip ? 0.0f : interact;
+    // twin cut-off: LJ forces and energies only within rvdw (the floored
+    // r^2, as the reference compares it)
+    const bool  inVdw  = !kTwin || rsq < ic.rvdw2;
+    const float ljmask = kTwin ? (inVdw ? emask : 0.0f) : emask;
 
     // LJ, skipped (a wave-uniform branch) for an i-atom whose type has no
     // LJ with any type, where every term is exactly zero
     float frLJ = 0.0f;
-    if (kLJ == kLJShift || I.lj[kI])
+    if (kFl == kLJShift || I.lj[kI])
     {
         const float2 c     = sNbfp[I.tOff[kI] + J.type];
         const float  rinv6 = kChecks ? interact * rinvsq * rinvsq * rinvsq : rinvsq * rinvsq * rinvsq;
         const float  FrLJ6 = c.x * rinv6;
         frLJ               = fmaf(c.y * rinv6, rinv6, -FrLJ6);
-        if (kLJ == kLJForceSwitch || kLJ == kLJPotSwitch)
+        if (kFl == kLJForceSwitch || kFl == kLJPotSwitch)
         {
             // the switches act on rsw = max(r - rvdw_switch, 0), with r from
             // the masked 1/r: 0 beyond the cut-off and on the diagonal (and
@@ -137,7 +143,7 @@ __device__ __forceinline__ void pairI(const IAtoms& I, const JAtom& J, const uns
             const float rm  = rsq * rinv;
             const float rsw = fmaxf(rm - ic.rvdw_switch, 0.0f);
             const float rs2 = rsw * rsw;
-            if (kLJ == kLJForceSwitch)
+            if (kFl == kLJForceSwitch)
             {
                 if (kEnergy)
                 {
@@ -148,7 +154,7 @@ __device__ __forceinline__ void pairI(const IAtoms& I, const JAtom& J, const uns
                                       * fmaf(c.x, fmaf(ic.fsw_e3d, rsw, ic.fsw_e2d),
                                              -c.y * fmaf(ic.fsw_e3r, rsw, ic.fsw_e2r));
                     const float v12 = fmaf(c.x, -2.0f * ic.cpot_disp, fmaf(c.y, ic.cpot_rep, frLJ - FrLJ6 + vsw));
-                    vlj             = fmaf(v12, emask, vlj);
+                    vlj             = fmaf(v12, ljmask, vlj);
                 }
                 frLJ = fmaf(rs2 * rm,
                             fmaf(c.y, fmaf(ic.fsw_c3r, rsw, ic.fsw_c2r), -c.x * fmaf(ic.fsw_c3d, rsw, ic.fsw_c2d)),
@@ -163,7 +169,7 @@ __device__ __forceinline__ void pairI(const IAtoms& I, const JAtom& J, const uns
                 frLJ            = fmaf(frLJ, sw, -rm * (v12 * (1.0f / 12.0f)) * dsw);
                 if (kEnergy)
                 {
-                    vlj = fmaf(v12 * sw, emask, vlj);
+                    vlj = fmaf(v12 * sw, ljmask, vlj);
                 }
             }
         }
@@ -171,9 +177,13 @@ __device__ __forceinline__ void pairI(const IAtoms& I, const JAtom& J, const uns
         {
             // 12 V = FrLJ12 - 2 FrLJ6 + c12 cpot_rep - 2 c6 cpot_disp
             const float v12 = fmaf(c.x, -2.0f * ic.cpot_disp, fmaf(c.y, ic.cpot_rep, frLJ - FrLJ6));
-            vlj             = fmaf(v12, emask, vlj);
+            vlj             = fmaf(v12, ljmask, vlj);
         }
-        if (kLJ == kLJEwald)
+        if (kTwin)
+        {
+            frLJ = inVdw ? frLJ : 0.0f;
+        }
+        if (kFl == kLJEwald)
         {
             // LJ-PME: subtract the grid's dispersion, c6g (r^-6 - e^-x (r^-6
             // (1 + x + x^2/2) + beta^6/6)) with x = beta^2 r^2, from r^-6
@@ -630,6 +640,24 @@ template<int kElec>
 hipError_t launchElec(hipStream_t s, const KernelArgs& a, bool energy, bool shiftForces, int numBlocks, int lds,
                       hipEvent_t start, hipEvent_t stop)
 {
+    if constexpr (kElec != kElecRF)
+    {
+        // the twin cut-off (Ewald with an LJ cut-off shorter than rcoulomb)
+        if (a.twinCut)
+        {
+            if (a.vdwModifier == NBX_VDW_FORCE_SWITCH)
+            {
+                return launchFlags<kElec, kLJForceSwitch | kLJTwin>(s, a, energy, shiftForces, numBlocks, lds, start,
+                                                                    stop);
+            }
+            if (a.vdwModifier == NBX_VDW_POTENTIAL_SWITCH)
+            {
+                return launchFlags<kElec, kLJPotSwitch | kLJTwin>(s, a, energy, shiftForces, numBlocks, lds, start,
+                                                                  stop);
+            }
+            return launchFlags<kElec, kLJShiftSkip | kLJTwin>(s, a, energy, shiftForces, numBlocks, lds, start, stop);
+        }
+    }
     if (a.ljPme)
     {
         return launchFlags<kElec, kLJEwald>(s, a, energy, shiftForces, numBlocks, lds, start, stop);

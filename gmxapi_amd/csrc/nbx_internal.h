@@ -93,12 +93,17 @@ enum LJFlavour : int
     kLJForceSwitch = 2,
     kLJPotSwitch   = 3,
     kLJEwald       = 4, // LJ-PME real space (potential shift; tests the flag)
+    // flag bit on the flavour: the twin cut-off, LJ masked beyond rvdw
+    // (VDW_CUTOFF_CHECK, kernel_ref_inner.h:254-271)
+    kLJTwin        = 8,
+    kLJFlavourMask = 7,
 };
 
 struct InteractionConst
 {
     float rlist2;
     float rcoulomb2;
+    float rvdw2; // the LJ cut-off squared (below rcoulomb2 with the twin cut-off)
     float epsfac;
     float k_rf;
     float c_rf;
@@ -275,6 +280,7 @@ struct KernelArgs
     int           skipLJ; // host: some type has no LJ (launch the variant that skips it)
     int           vdwModifier; // host: NBX_VDW_* modifier
     int           ljPme;       // host: LJ-PME (c6grid is read)
+    int           twinCut;     // host: rvdw < rcoulomb (launch the twin-cut-off variant)
     const float*  c6grid;      // LJ-PME: the grid's 6 C6 per type pair
     int           ntype;  // including the padding type
     const float4* tabFDV0;

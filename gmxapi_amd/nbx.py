@@ -66,13 +66,13 @@ class _Options(C.Structure):
                 ("epsilon_rf", C.c_double), ("ewald_potential_shift", C.c_int), ("device", C.c_int),
                 ("kernel_group_size", C.c_int), ("kernel_blocks_per_cu", C.c_int), ("vdw_modifier", C.c_int),
                 ("rvdw_switch", C.c_double), ("vdw_type", C.c_int), ("ljpme_comb_rule", C.c_int),
-                ("ewald_rtol_lj", C.c_double)]
+                ("ewald_rtol_lj", C.c_double), ("vdw_cutoff", C.c_double)]
 
 
 def _options(coulomb="ewald", ewald_correction="table", rlist=1.0, rcoulomb=None, ewald_rtol=1e-5, epsilon_r=1.0,
              epsilon_rf=1.0, ewald_potential_shift=False, device=0, kernel_group_size=0, kernel_blocks_per_cu=0,
              vdw_modifier="potential-shift", rvdw_switch=0.0, vdw_type="cutoff", ljpme_comb_rule="geometric",
-             ewald_rtol_lj=1e-3):
+             ewald_rtol_lj=1e-3, vdw_cutoff=None):
     o = _Options()
     library().nbx_default_options(C.byref(o))
     o.coulomb_type = COULOMB[coulomb]
@@ -97,6 +97,8 @@ def _options(coulomb="ewald", ewald_correction="table", rlist=1.0, rcoulomb=None
     o.vdw_type = VDW_TYPE[vdw_type]
     o.ljpme_comb_rule = LJPME_COMB_RULE[ljpme_comb_rule]
     o.ewald_rtol_lj = ewald_rtol_lj
+    # rvdw: None or 0 means rcoulomb; below rcoulomb is the twin cut-off
+    o.vdw_cutoff = 0.0 if vdw_cutoff is None else vdw_cutoff
     return o
 
 
@@ -165,6 +167,8 @@ def library() -> C.CDLL:
     lib.nbx_list_pairs.argtypes = [vp, d, vp, C.c_int64]
     lib.nbx_list_pairs.restype = C.c_int64
     lib.nbx_get_interaction_constants.argtypes = [vp, vp]
+    lib.nbx_get_vdw_constants.argtypes = [vp, vp]
+    lib.nbx_get_ewald_table.argtypes = [vp, vp, vp]
     lib.nbx_get_work_groups.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.nbx_set_kernel_timing.argtypes = [vp, C.c_int]
     lib.nbx_batch_create.argtypes = [vp, C.c_int, vp, vp, vp]
@@ -208,12 +212,12 @@ class Context:
                  epsilon_rf: float = 1.0, ewald_potential_shift: bool = False, device: int = 0,
                  kernel_group_size: int = 0, kernel_blocks_per_cu: int = 0,
                  vdw_modifier: str = "potential-shift", rvdw_switch: float = 0.0, vdw_type: str = "cutoff",
-                 ljpme_comb_rule: str = "geometric", ewald_rtol_lj: float = 1e-3):
+                 ljpme_comb_rule: str = "geometric", ewald_rtol_lj: float = 1e-3, vdw_cutoff: float | None = None):
         lib = library()
         self._lib = lib
         o = _options(coulomb, ewald_correction, rlist, rcoulomb, ewald_rtol, epsilon_r, epsilon_rf,
                      ewald_potential_shift, device, kernel_group_size, kernel_blocks_per_cu, vdw_modifier,
-                     rvdw_switch, vdw_type, ljpme_comb_rule, ewald_rtol_lj)
+                     rvdw_switch, vdw_type, ljpme_comb_rule, ewald_rtol_lj, vdw_cutoff)
         h = C.c_void_p()
         rc = lib.nbx_create(C.byref(o), C.byref(h))
         if rc != NBX_SUCCESS:
@@ -439,6 +443,22 @@ class Context:
         keys = ["beta", "tab_scale", "tab_n", "epsfac", "k_rf", "c_rf", "sh_ewald", "cpot_disp", "cpot_rep"]
         return dict(zip(keys, out.tolist()))
 
+    def vdw_constants(self) -> dict:
+        out = np.zeros(12)
+        self._check(self._lib.nbx_get_vdw_constants(self._h, _ptr(out)), "nbx_get_vdw_constants")
+        keys = ["rvdw", "rvdw_switch", "fsw_c2d", "fsw_c3d", "fsw_c2r", "fsw_c3r", "sw_c3", "sw_c4", "sw_c5",
+                "lje_coeff2", "lje_coeff6_6", "sh_lj_ewald"]
+        return dict(zip(keys, out.tolist()))
+
+    def ewald_table(self) -> np.ndarray:
+        """The FDV0 quadruplets the kernel reads, (n, 4) float32."""
+        n = C.c_int(0)
+        self._check(self._lib.nbx_get_ewald_table(self._h, C.byref(n), None), "nbx_get_ewald_table")
+        out = np.zeros((max(n.value, 1), 4), np.float32)
+        if n.value:
+            self._check(self._lib.nbx_get_ewald_table(self._h, C.byref(n), _ptr(out)), "nbx_get_ewald_table")
+        return out[:n.value]
+
 
 def nblib_c6c12_matrices(c6_type, c12_type):
     """Geometric combination as nblib does it in float
@@ -584,14 +604,15 @@ def batch_for(systems, **options) -> Batch:
 
 def context_for(system, coulomb="ewald", ewald_correction="table", device=0, potential_shift=False,
                 kernel_group_size=0, kernel_blocks_per_cu=0, vdw_modifier="potential-shift",
-                rvdw_switch=0.0, vdw_type="cutoff", ljpme_comb_rule="geometric", ewald_rtol_lj=1e-3) -> Context:
+                rvdw_switch=0.0, vdw_type="cutoff", ljpme_comb_rule="geometric", ewald_rtol_lj=1e-3,
+                vdw_cutoff=None) -> Context:
     """Context set up on a gmxapi_amd.systems.System the way nblib sets up its
     calculator: parameters, topology, grid, pair list."""
     ctx = Context(coulomb=coulomb, ewald_correction=ewald_correction, rlist=system.rlist,
                   rcoulomb=system.rcoulomb, ewald_potential_shift=potential_shift, device=device,
                   kernel_group_size=kernel_group_size, kernel_blocks_per_cu=kernel_blocks_per_cu,
                   vdw_modifier=vdw_modifier, rvdw_switch=rvdw_switch, vdw_type=vdw_type,
-                  ljpme_comb_rule=ljpme_comb_rule, ewald_rtol_lj=ewald_rtol_lj)
+                  ljpme_comb_rule=ljpme_comb_rule, ewald_rtol_lj=ewald_rtol_lj, vdw_cutoff=vdw_cutoff)
     m6, m12 = nblib_c6c12_matrices(system.c6, system.c12)
     ctx.set_nonbonded_parameters(m6, m12)
     ctx.set_topology(system.types, system.charges, system.excl_off, system.excl_idx)

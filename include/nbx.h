@@ -95,7 +95,7 @@ typedef struct nbx_options
     int    coulomb_type;          /* NBX_COULOMB_*, default EWALD (nblib Pme) */
     int    ewald_correction;      /* NBX_EWALD_*, default TABLE */
     double pairlist_cutoff;       /* rlist, nm (nblib pairlistCutoff, 1.0) */
-    double coulomb_cutoff;        /* rcoulomb = rvdw <= rlist; 0 means rlist */
+    double coulomb_cutoff;        /* rcoulomb <= rlist; 0 means rlist */
     double ewald_rtol;            /* erfc(beta rc) = rtol, 1e-5 (gmxsetup.cpp:275) */
     double epsilon_r;             /* 1 (gmxsetup.cpp:251-252) */
     double epsilon_rf;            /* 1 */
@@ -110,6 +110,12 @@ typedef struct nbx_options
     int    vdw_type;              /* NBX_VDW_CUTOFF (nblib) or NBX_VDW_PME (potential shift only) */
     int    ljpme_comb_rule;       /* NBX_LJPME_*: the grid's combination rule, default GEOMETRIC */
     double ewald_rtol_lj;         /* LJ-PME: g(beta_lj rc) = rtol, default 1e-3 (ewald-rtol-lj) */
+    double vdw_cutoff;            /* rvdw <= rcoulomb; 0 means rcoulomb.  rvdw < rcoulomb is the
+                                     twin cut-off (the coulktTAB_TWIN / coulktEWALD_TWIN kernels,
+                                     kerneldispatch.cpp:175-192, VDW_CUTOFF_CHECK in
+                                     kernel_ref_inner.h:254-271): Ewald Coulomb with an LJ cut-off
+                                     only, as grompp allows (readir.cpp:386-398); the LJ
+                                     potential shift and switches then act at rvdw */
 } nbx_options;
 
 typedef struct nbx_pairlist_info
@@ -283,6 +289,20 @@ int nbx_get_kernel_timing(nbx_context* context, double* average_ms, int* launche
 /* beta, table scale, table size, epsfac, k_rf, c_rf, sh_ewald, LJ
  * dispersion and repulsion potential shifts (interaction_const_t). */
 int nbx_get_interaction_constants(nbx_context* context, double* out9);
+
+/* The LJ part of interaction_const_t as the kernels use it: rvdw, the switch
+ * radius, the force switch's dispersion and repulsion c2, c3
+ * (force_switch_constants, forcerec.cpp:787-801), the potential switch's c3,
+ * c4, c5 (potential_switch_constants :803-818), and LJ-PME's beta_lj^2,
+ * beta_lj^6 / 6 and sh_lj_ewald (forcerec.cpp:703-717): 12 values. */
+int nbx_get_vdw_constants(nbx_context* context, double* out12);
+
+/* The Ewald correction table as the kernel reads it (interaction_const_t's
+ * coulombEwaldTables->tableFDV0, generateEwaldCorrectionTables,
+ * tables/forcetable.cpp:144-298): *num_points quadruplets {F_i, F_i+1 - F_i,
+ * V_i, V_i + sh_ewald} in float (the reference leaves the fourth entry 0).
+ * With fdv0 NULL only *num_points is set (0 without Ewald). */
+int nbx_get_ewald_table(nbx_context* context, int* num_points, float* fdv0);
 
 /* ------------------------------------------------------------------------
  * Batched ensemble: many independent systems (replicas of uneven size) that

@@ -292,6 +292,8 @@ typedef struct
     int    coulomb; /* 0 = reaction-field / plain cut-off, 1 = Ewald */
     int    ewald_exact; /* 0 = reference spline table, 1 = exact closed form */
     real   rlist, rcoulomb;
+    /* interaction_const_t::rvdw: rcoulomb unless a twin cut-off is set */
+    real   rvdw;
     real   epsfac, k_rf, c_rf, beta, sh_ewald, cpot_disp, cpot_rep;
     /* LJ modifier (interaction_const_t vdw_modifier): 0 potential shift,
      * 1 force switch, 2 potential switch; rvdw_switch and the constants of
@@ -370,7 +372,8 @@ orc_sys* orc_create(int coulomb, int ewald_exact, double rlist, double rcoulomb,
     s->ewald_exact = ewald_exact;
     s->rlist       = (real)rlist;
     s->rcoulomb    = (real)rcoulomb;
-    real rvdw      = s->rcoulomb;
+    s->rvdw        = s->rcoulomb;
+    real rvdw      = s->rvdw;
     /* power6/power12 of math/functions.h:333-390 */
     real p3        = rvdw * (rvdw * rvdw);
     real p6        = p3 * p3;
@@ -405,6 +408,28 @@ orc_sys* orc_create(int coulomb, int ewald_exact, double rlist, double rcoulomb,
     return s;
 }
 
+/* The twin cut-off: interaction_const_t::rvdw below rcoulomb
+ * (kerneldispatch.cpp:175-192 picks the TAB_TWIN / EWALD_TWIN kernels, which
+ * mask LJ beyond rvdw: VDW_CUTOFF_CHECK, kernel_ref_inner.h:254-271), with
+ * the LJ potential shift at rvdw (forcerec.cpp:848-873).  grompp allows it
+ * with Ewald electrostatics and an LJ cut-off only (readir.cpp:386-398).
+ * Call before orc_set_vdw_modifier, which then switches at rvdw.  Returns
+ * -1 unless 0 < rvdw <= rcoulomb with Ewald and no LJ-PME. */
+int orc_set_vdw_cutoff(orc_sys* s, double rvdw)
+{
+    real r = (real)rvdw;
+    if (!(r > 0 && r <= s->rcoulomb) || (r < s->rcoulomb && s->coulomb != 1) || s->lj_pme || s->vdw_mod != 0)
+    {
+        return -1;
+    }
+    s->rvdw      = r;
+    real p3      = r * (r * r);
+    real p6      = p3 * p3;
+    s->cpot_disp = (real)(-1.0 / p6);
+    s->cpot_rep  = (real)(-1.0 / (p6 * p6));
+    return 0;
+}
+
 /* The LJ modifier and its constants (init_interaction_const,
  * forcerec.cpp:848-873; force_switch_constants :787-801,
  * potential_switch_constants :803-818), in double from the real radii and
@@ -412,7 +437,7 @@ orc_sys* orc_create(int coulomb, int ewald_exact, double rlist, double rcoulomb,
  * Returns -1 for an unknown modifier or a switch radius outside [0, rc). */
 int orc_set_vdw_modifier(orc_sys* s, int modifier, double rvdw_switch)
 {
-    double rc  = (double)s->rcoulomb;
+    double rc  = (double)s->rvdw;
     double rsw = (double)(real)rvdw_switch;
     double d   = rc - rsw;
     if (modifier < 0 || modifier > 2 || (modifier != 0 && !(rsw >= 0 && rsw < rc)))
@@ -425,7 +450,7 @@ int orc_set_vdw_modifier(orc_sys* s, int modifier, double rvdw_switch)
     s->sw_c3 = s->sw_c4 = s->sw_c5 = 0;
     if (modifier == 0)
     {
-        real p3      = s->rcoulomb * (s->rcoulomb * s->rcoulomb);
+        real p3      = s->rvdw * (s->rvdw * s->rvdw);
         real p6      = p3 * p3;
         s->cpot_disp = (real)(-1.0 / p6);
         s->cpot_rep  = (real)(-1.0 / (p6 * p6));
@@ -504,7 +529,8 @@ real orc_ewald_coeff_lj(real rc, real rtol)
  * modifier other than the potential shift. */
 int orc_set_lj_pme(orc_sys* s, int comb_rule, double rtol_lj)
 {
-    if ((comb_rule != 0 && comb_rule != 1) || s->vdw_mod != 0 || !(rtol_lj > 0 && rtol_lj < 1))
+    if ((comb_rule != 0 && comb_rule != 1) || s->vdw_mod != 0 || !(rtol_lj > 0 && rtol_lj < 1)
+        || s->rvdw != s->rcoulomb)
     {
         return -1;
     }
@@ -566,6 +592,18 @@ void orc_get_constants(const orc_sys* s, double* out)
     out[6] = s->sh_ewald;
     out[7] = s->cpot_disp;
     out[8] = s->cpot_rep;
+}
+
+/* rvdw, rvdw_switch, force-switch c2/c3 (dispersion, repulsion),
+ * potential-switch c3..c5, LJ-PME beta^2, beta^6/6, sh_lj_ewald */
+void orc_get_vdw_constants(const orc_sys* s, double* out)
+{
+    const real v[12] = { s->rvdw,   s->rvdw_switch, s->fsw_c2d,    s->fsw_c3d,      s->fsw_c2r, s->fsw_c3r,
+                         s->sw_c3, s->sw_c4,       s->sw_c5,      s->lje_coeff2,   s->lje_coeff6_6, s->lje_vc };
+    for (int k = 0; k < 12; k++)
+    {
+        out[k] = (double)v[k];
+    }
 }
 
 void orc_get_tables(const orc_sys* s, real* F, real* V, real* FDV0)
@@ -1580,6 +1618,8 @@ void orc_kernel(orc_sys* s, int energy)
 {
     const int  nslot    = s->ncells * ORC_NC;
     const real rcut2    = s->rcoulomb * s->rcoulomb;
+    const int  twin     = s->rvdw < s->rcoulomb;
+    const real rvdw2    = s->rvdw * s->rvdw;
     const int  ntype2   = s->numTypes * 2;
     const real facel    = s->epsfac;
     const real k_rf2    = 2 * s->k_rf;
@@ -1761,9 +1801,20 @@ void orc_kernel(orc_sys* s, int energy)
                             VLJ += c6grid / 6 * (rinvsix_nm * (1 - expmcr2 * poly) + sh_mask);
                         }
                     }
-                    if (energy)
+                    if (twin)
+                    {
+                        /* LJ only within rvdw (VDW_CUTOFF_CHECK,
+                         * kernel_ref_inner.h:254-265), on the floored r^2 */
+                        real skipmask_rvdw = (rsq < rvdw2) ? 1 : 0;
+                        frLJ *= skipmask_rvdw;
+                        VLJ *= skipmask_rvdw;
+                    }
+                    else
                     {
                         VLJ = VLJ * skipmask;
+                    }
+                    if (energy)
+                    {
                         Vvdw_ci += VLJ;
                     }
                     real qq = skipmask * qi[i] * q[aj];

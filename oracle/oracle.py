@@ -43,6 +43,8 @@ def _lib(precision: str) -> C.CDLL:
     lib.orc_create.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                C.c_double, C.c_int]
     lib.orc_destroy.argtypes = [vp]
+    lib.orc_set_vdw_cutoff.restype = C.c_int
+    lib.orc_set_vdw_cutoff.argtypes = [vp, C.c_double]
     lib.orc_set_vdw_modifier.restype = C.c_int
     lib.orc_set_vdw_modifier.argtypes = [vp, C.c_int, C.c_double]
     lib.orc_set_lj_pme.restype = C.c_int
@@ -65,6 +67,7 @@ def _lib(precision: str) -> C.CDLL:
     lib.orc_one_4pi_eps0.restype = C.c_double
     lib.orc_get_constants.argtypes = [vp, vp]
     lib.orc_get_tables.argtypes = [vp, vp, vp, vp]
+    lib.orc_get_vdw_constants.argtypes = [vp, vp]
     lib.orc_set_nbfp.argtypes = [vp, C.c_int, vp]
     lib.orc_set_exclusions.argtypes = [vp, C.c_int, vp, vp]
     lib.orc_put_on_grid.restype = C.c_int
@@ -187,6 +190,8 @@ def brute_forces(x, box, types, q, nbfp, ntype, excl_off, excl_idx, coulomb: str
 # interaction_const_t vdw_modifier values the oracle implements
 VDW_MODIFIER = {"potential-shift": 0, "force-switch": 1, "potential-switch": 2}
 LJPME_COMB_RULE = {"geometric": 0, "lorentz-berthelot": 1}
+VDW_CONSTANT_KEYS = ["rvdw", "rvdw_switch", "fsw_c2d", "fsw_c3d", "fsw_c2r", "fsw_c3r", "sw_c3", "sw_c4", "sw_c5",
+                     "lje_coeff2", "lje_coeff6_6", "sh_lj_ewald"]
 
 
 class Oracle:
@@ -198,7 +203,7 @@ class Oracle:
 
     def __init__(self, precision="f64", coulomb="ewald", ewald_exact=False, rlist=1.0, rcoulomb=None,
                  epsilon_r=1.0, epsilon_rf=1.0, ewald_rtol=1e-5, ewald_potential_shift=False,
-                 vdw_modifier="potential-shift", rvdw_switch=0.0, lj_pme=None, ewald_rtol_lj=1e-3):
+                 vdw_modifier="potential-shift", rvdw_switch=0.0, lj_pme=None, ewald_rtol_lj=1e-3, rvdw=None):
         self.precision = precision
         self.lib = _lib(precision)
         self.dt = np.float32 if precision == "f32" else np.float64
@@ -207,6 +212,9 @@ class Oracle:
         self.h = self.lib.orc_create(1 if coulomb == "ewald" else 0, 1 if ewald_exact else 0, rlist, rc,
                                      epsilon_r, epsilon_rf, ewald_rtol, 1 if ewald_potential_shift else 0)
         self.natoms = 0
+        # rvdw: None (= rcoulomb) or the twin cut-off's LJ radius
+        if rvdw is not None and self.lib.orc_set_vdw_cutoff(self.h, float(rvdw)) != 0:
+            raise ValueError(f"bad LJ cut-off {rvdw} (needs 0 < rvdw <= rcoulomb, Ewald, no LJ-PME)")
         if self.lib.orc_set_vdw_modifier(self.h, VDW_MODIFIER[vdw_modifier], rvdw_switch) != 0:
             raise ValueError(f"bad LJ modifier {vdw_modifier!r} / rvdw_switch {rvdw_switch}")
         # lj_pme: None (LJ cut-off) or the grid's combination rule
@@ -229,6 +237,11 @@ class Oracle:
         self.lib.orc_get_constants(self.h, _p(out))
         keys = ["beta", "tab_scale", "tab_n", "epsfac", "k_rf", "c_rf", "sh_ewald", "cpot_disp", "cpot_rep"]
         return dict(zip(keys, out.tolist()))
+
+    def vdw_constants(self) -> dict:
+        out = np.zeros(12)
+        self.lib.orc_get_vdw_constants(self.h, _p(out))
+        return dict(zip(VDW_CONSTANT_KEYS, out.tolist()))
 
     def tables(self):
         n = int(self.constants()["tab_n"])

@@ -23,13 +23,15 @@ def load_golden(name):
 
 
 def oracle_for(system, precision="f64", coulomb="ewald", ewald_exact=False, potential_shift=False,
-               rlist=None, rcoulomb=None, vdw=("potential-shift", 0.0), lj_pme=None):
+               rlist=None, rcoulomb=None, vdw=("potential-shift", 0.0), lj_pme=None, rvdw=None):
     """An oracle calculator set up on `system` the way nblib sets one up;
-    vdw = (LJ modifier, rvdw_switch); lj_pme = None or LJ-PME's grid rule."""
+    vdw = (LJ modifier, rvdw_switch); lj_pme = None or LJ-PME's grid rule;
+    rvdw = None (= rcoulomb) or the twin cut-off's LJ radius."""
     o = O.Oracle(precision, coulomb, ewald_exact=ewald_exact,
                  rlist=system.rlist if rlist is None else rlist,
                  rcoulomb=system.rcoulomb if rcoulomb is None else rcoulomb,
-                 ewald_potential_shift=potential_shift, vdw_modifier=vdw[0], rvdw_switch=vdw[1], lj_pme=lj_pme)
+                 ewald_potential_shift=potential_shift, vdw_modifier=vdw[0], rvdw_switch=vdw[1], lj_pme=lj_pme,
+                 rvdw=rvdw)
     nb = O.nblib_nbfp(system.c6, system.c12, precision)
     o.setup(system.box, system.x, system.types, system.charges, nb, system.ntypes, system.excl_off,
             system.excl_idx)

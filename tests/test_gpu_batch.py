@@ -60,8 +60,9 @@ def test_batch_matches_single_systems(replicas):
 
 @pytest.mark.parametrize("opts", [{"vdw_modifier": "force-switch", "rvdw_switch": 0.7},
                                   {"vdw_modifier": "potential-switch", "rvdw_switch": 0.7},
-                                  {"vdw_type": "pme", "ljpme_comb_rule": "lorentz-berthelot"}],
-                         ids=["force-switch", "potential-switch", "lj-pme"])
+                                  {"vdw_type": "pme", "ljpme_comb_rule": "lorentz-berthelot"},
+                                  {"vdw_cutoff": 0.7}],
+                         ids=["force-switch", "potential-switch", "lj-pme", "twin-cutoff"])
 def test_batch_with_lj_flavours_matches_single_systems(replicas, opts):
     # the batched launch takes the other LJ flavours like a single context
     # (LJ-PME's self energies included per system)

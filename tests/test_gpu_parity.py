@@ -101,32 +101,37 @@ def info_clusters(ctx):
 _NOISE = {}
 
 
-def _fp32_noise(s, coulomb="ewald", ewald_exact=False, copies=6, vdw=("potential-shift", 0.0), lj_pme=None):
+def _fp32_noise(s, coulomb="ewald", ewald_exact=False, copies=6, vdw=("potential-shift", 0.0), lj_pme=None,
+                rvdw=None):
     """The reference FP32 kernel's RMS energy rounding noise on this box
     (tests.helpers.fp32_energy_noise: the FP32 oracle's deviation from FP64
     over rigid translations).  On boxes of a few hundred molecules that noise
     is above 1e-6 of the total (float coordinate differences and shift
     vectors carry ~1e-7 relative rounding that r^-12 and the Coulomb sum
     amplify)."""
-    key = (id(s), coulomb, ewald_exact, vdw, lj_pme)
+    key = (id(s), coulomb, ewald_exact, vdw, lj_pme, rvdw)
     if key not in _NOISE:
         _NOISE[key] = fp32_energy_noise(s, copies, coulomb=coulomb, ewald_exact=ewald_exact, vdw=vdw,
-                                        lj_pme=lj_pme)
+                                        lj_pme=lj_pme, rvdw=rvdw)
     return _NOISE[key]
 
 
-def _energy_bar(s, key, scale, coulomb="ewald", ewald_exact=False, vdw=("potential-shift", 0.0), lj_pme=None):
+def _energy_bar(s, key, scale, coulomb="ewald", ewald_exact=False, vdw=("potential-shift", 0.0), lj_pme=None,
+                rvdw=None):
     """Energy tolerance: 1e-6 relative, or on small boxes three times the
     FP32 reference kernel's own RMS rounding noise there, whichever is larger
     (the 9,000- and 288,000-atom boxes are held to the plain 1e-6)."""
-    return max(ENERGY_TOL_REL * scale, 3.0 * _fp32_noise(s, coulomb, ewald_exact, vdw=vdw, lj_pme=lj_pme)[key])
+    return max(ENERGY_TOL_REL * scale,
+               3.0 * _fp32_noise(s, coulomb, ewald_exact, vdw=vdw, lj_pme=lj_pme, rvdw=rvdw)[key])
 
 
-def _compare(s, ctx, coulomb="ewald", widen_energy=True, vdw=("potential-shift", 0.0), lj_pme=None):
+def _compare(s, ctx, coulomb="ewald", widen_energy=True, vdw=("potential-shift", 0.0), lj_pme=None, rvdw=None,
+             ewald_exact=False):
     f = ctx.compute(s.x, energy=True, virial=True)
     vlj, vc = ctx.energies()
-    r64 = oracle_for(s, "f64", coulomb=coulomb, vdw=vdw, lj_pme=lj_pme).compute(s.x)
-    r32 = oracle_for(s, "f32", coulomb=coulomb, vdw=vdw, lj_pme=lj_pme).compute(s.x)
+    r64 = oracle_for(s, "f64", coulomb=coulomb, vdw=vdw, lj_pme=lj_pme, rvdw=rvdw, ewald_exact=ewald_exact).compute(s.x)
+    r32 = oracle_for(s, "f32", coulomb=coulomb, vdw=vdw, lj_pme=lj_pme, rvdw=rvdw,
+                     ewald_exact=ewald_exact).compute(s.x)
     ferr64 = rms(f - r64["f"]) / rms(r64["f"])
     ferr32 = rms(f - r32["f"]) / rms(r64["f"])
     e64 = r64["Vvdw"] + r64["Vc"]
@@ -137,7 +142,8 @@ def _compare(s, ctx, coulomb="ewald", widen_energy=True, vdw=("potential-shift",
     assert ferr32 <= FORCE_TOL_RMS
     for got, key in ((vlj, "Vvdw"), (vc, "Vc")):
         scale = max(abs(r64[key]), abs(e64))
-        bar = _energy_bar(s, key, scale, coulomb, vdw=vdw, lj_pme=lj_pme) if widen_energy else ENERGY_TOL_REL * scale
+        bar = (_energy_bar(s, key, scale, coulomb, ewald_exact, vdw=vdw, lj_pme=lj_pme, rvdw=rvdw) if widen_energy
+               else ENERGY_TOL_REL * scale)
         assert abs(got - r64[key]) <= bar, (key, got, r64[key], bar)
     vir = ctx.virial()
     np.testing.assert_allclose(vir, r64["virial"], rtol=0, atol=1e-4 * np.abs(r64["virial"]).max())
@@ -253,7 +259,8 @@ def test_lj_pme(name, s, rule):
 @pytest.mark.parametrize("opts,oracle_opts", [
     ({"vdw_type": "pme"}, {"lj_pme": "geometric"}),
     ({"vdw_modifier": "force-switch", "rvdw_switch": 0.8}, {"vdw": ("force-switch", 0.8)}),
-], ids=["lj-pme", "force-switch"])
+    ({"vdw_cutoff": 0.8}, {"rvdw": 0.8}),
+], ids=["lj-pme", "force-switch", "twin-cutoff"])
 def test_lj_flavours_water288k(opts, oracle_opts):
     # the full-size water box with another LJ flavour: forces against the
     # FP64 oracle, energies to 1e-6 of each component on its own
@@ -481,3 +488,66 @@ def test_add_forces_accumulates_on_device():
     ctx._check(ctx._lib.nbx_compute(ctx._h, nbx._ptr(xs), nbx._ptr(h),
                                     nbx.COMPUTE_FORCES | nbx.COMPUTE_ADD_FORCES, 0), "nbx_compute")
     np.testing.assert_array_equal(h, h0 + f_nb.cpu().numpy())
+
+
+@pytest.mark.parametrize("vdw", [("potential-shift", 0.0), ("force-switch", 0.5), ("potential-switch", 0.5)],
+                         ids=["shift", "force-switch", "potential-switch"])
+@pytest.mark.parametrize("correction", ["table", "analytical"])
+@pytest.mark.parametrize("name,s", _systems()[1:])
+def test_twin_cutoff(name, s, correction, vdw):
+    # rvdw < rcoulomb: the TAB_TWIN / EWALD_TWIN kernels with LJ masked
+    # beyond rvdw (kernel_ref_inner.h:254-271), shift and switches at rvdw
+    print(f"[twin cut-off] {name} rvdw 0.7 {correction} {vdw[0]}", flush=True)
+    ctx = nbx.context_for(s, ewald_correction=correction, vdw_cutoff=0.7, vdw_modifier=vdw[0],
+                          rvdw_switch=vdw[1])
+    _compare(s, ctx, vdw=vdw, rvdw=0.7, ewald_exact=correction == "analytical")
+
+
+def test_twin_cutoff_option_errors():
+    s = _systems()[0][1]
+    for opts in ({"coulomb": "reaction-field", "vdw_cutoff": 0.8}, {"vdw_type": "pme", "vdw_cutoff": 0.8},
+                 {"vdw_cutoff": 1.2}, {"vdw_cutoff": 0.8, "vdw_modifier": "force-switch", "rvdw_switch": 0.8}):
+        with pytest.raises(nbx.NbxInputError):
+            nbx.context_for(s, **opts)
+    ctx = nbx.context_for(s, vdw_cutoff=0.8)
+    with pytest.raises(nbx.NbxInputError):
+        ctx.compute_adjoint(s.x, np.zeros_like(s.x))
+
+
+@pytest.mark.parametrize("opts,oracle_opts", [
+    ({}, {}),
+    ({"coulomb": "reaction-field"}, {"coulomb": "rf"}),
+    ({"vdw_modifier": "force-switch", "rvdw_switch": 0.6}, {"vdw": ("force-switch", 0.6)}),
+    ({"vdw_modifier": "potential-switch", "rvdw_switch": 0.6}, {"vdw": ("potential-switch", 0.6)}),
+    ({"vdw_type": "pme"}, {"lj_pme": "geometric"}),
+    ({"vdw_cutoff": 0.8, "vdw_modifier": "force-switch", "rvdw_switch": 0.6},
+     {"rvdw": 0.8, "vdw": ("force-switch", 0.6)}),
+], ids=["ewald", "rf", "force-switch", "potential-switch", "lj-pme", "twin-force-switch"])
+def test_interaction_constants_match_oracle(opts, oracle_opts):
+    # interaction_const_t as the library computes it (float) against the
+    # oracle's FP32 restatement: beta, table scale and size, epsfac, RF
+    # constants, potential shifts, switch and LJ-PME constants, and the
+    # Ewald correction table the kernel reads
+    s = S.small_water_box(216, seed=11)
+    ctx = nbx.context_for(s, **opts)
+    o = oracle_for(s, "f32", **oracle_opts)
+    got, ref = ctx.interaction_constants(), o.constants()
+    for k in ref:
+        assert got[k] == pytest.approx(ref[k], rel=2e-7, abs=1e-30), (k, got[k], ref[k])
+    got, ref = ctx.vdw_constants(), o.vdw_constants()
+    for k in ref:
+        assert got[k] == pytest.approx(ref[k], rel=2e-6, abs=1e-30), (k, got[k], ref[k])
+    tab = ctx.ewald_table()
+    if oracle_opts.get("coulomb", "ewald") == "ewald":
+        F, V, fdv0 = o.tables()
+        fdv0 = fdv0.reshape(-1, 4)
+        assert tab.shape == fdv0.shape
+        np.testing.assert_array_equal(tab[:, :3], fdv0[:, :3])
+        # the fourth entry holds V + sh_ewald (sh_ewald = 0 on nblib's path)
+        np.testing.assert_array_equal(tab[:, 3], fdv0[:, 2] + np.float32(ref_sh(o)))
+    else:
+        assert tab.shape[0] == 0
+
+
+def ref_sh(o):
+    return o.constants()["sh_ewald"]
