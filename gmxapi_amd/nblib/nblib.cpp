@@ -1,7 +1,9 @@
 // nblib-compatible host API over the HIP nonbonded engine; see nblib.h.
 #include "nblib.h"
 
+#include <algorithm>
 #include <cmath>
+#include <set>
 #include <utility>
 
 #include "nbx.h"
@@ -164,6 +166,258 @@ Topology::Topology(std::vector<ParticleType> particleTypes, std::vector<int> typ
             interactions_.getC6(a.name(), b.name());
         }
     }
+}
+
+// interactions.cpp:192-220
+void ParticleTypesInteractions::merge(const ParticleTypesInteractions& other)
+{
+    for (const auto& p : other.single_)
+    {
+        add(p.first, p.second.first, p.second.second);
+    }
+    for (const auto& p : other.pairs_)
+    {
+        add(p.first.first, p.first.second, p.second.first, p.second.second);
+    }
+}
+
+int Topology::sequenceID(const MoleculeName& moleculeName, int moleculeNr, const ResidueName& residueName,
+                         const ParticleName& particleName) const
+{
+    auto it = sequence_.find({ moleculeName.value(), moleculeNr, residueName.value(), particleName.value() });
+    if (it == sequence_.end())
+    {
+        throw InputException("no particle " + particleName.value() + " in residue " + residueName.value()
+                             + " of molecule " + moleculeName.value() + " number " + std::to_string(moleculeNr));
+    }
+    return it->second;
+}
+
+// molecules.cpp:57-170
+Molecule::Molecule(MoleculeName moleculeName) : name_(std::move(moleculeName)) {}
+
+Molecule& Molecule::addParticle(const ParticleName& particleName, const ResidueName& residueName,
+                                const Charge& charge, const ParticleType& particleType)
+{
+    auto same = [&](const ParticleType& t) { return t.name() == particleType.name(); };
+    auto it   = std::find_if(particleTypes_.begin(), particleTypes_.end(), same);
+    if (it == particleTypes_.end())
+    {
+        particleTypes_.push_back(particleType);
+    }
+    else if (!(*it == particleType))
+    {
+        throw InputException("Differing ParticleTypes with identical names encountered in the same molecule.");
+    }
+    particles_.push_back(ParticleData{ particleName.value(), residueName.value(), particleType.name(), charge.value() });
+    // every particle excludes itself
+    const int id = static_cast<int>(particles_.size()) - 1;
+    exclusions_.emplace_back(id, id);
+    return *this;
+}
+
+Molecule& Molecule::addParticle(const ParticleName& particleName, const ResidueName& residueName,
+                                const ParticleType& particleType)
+{
+    return addParticle(particleName, residueName, Charge(0.0f), particleType);
+}
+
+Molecule& Molecule::addParticle(const ParticleName& particleName, const Charge& charge, const ParticleType& particleType)
+{
+    return addParticle(particleName, ResidueName(name_.value()), charge, particleType);
+}
+
+Molecule& Molecule::addParticle(const ParticleName& particleName, const ParticleType& particleType)
+{
+    return addParticle(particleName, ResidueName(name_.value()), Charge(0.0f), particleType);
+}
+
+void Molecule::addExclusion(int particleIndex, int particleIndexToExclude)
+{
+    const int n = numParticlesInMolecule();
+    if (particleIndex < 0 || particleIndex >= n || particleIndexToExclude < 0 || particleIndexToExclude >= n)
+    {
+        throw InputException("exclusion between particle indices outside the molecule");
+    }
+    if (particleIndex != particleIndexToExclude)
+    {
+        exclusions_.emplace_back(particleIndex, particleIndexToExclude);
+        exclusions_.emplace_back(particleIndexToExclude, particleIndex);
+    }
+}
+
+void Molecule::addExclusion(std::tuple<std::string, std::string> particle,
+                            std::tuple<std::string, std::string> particleToExclude)
+{
+    exclusionsByName_.emplace_back(std::get<0>(particle), std::get<1>(particle), std::get<0>(particleToExclude),
+                                   std::get<1>(particleToExclude));
+}
+
+void Molecule::addExclusion(const std::string& particleName, const std::string& particleNameToExclude)
+{
+    addExclusion(std::make_tuple(particleName, name_.value()), std::make_tuple(particleNameToExclude, name_.value()));
+}
+
+const ParticleType& Molecule::at(const std::string& particleTypeName) const
+{
+    for (const ParticleType& t : particleTypes_)
+    {
+        if (t.name() == particleTypeName)
+        {
+            return t;
+        }
+    }
+    throw std::out_of_range("no particle type " + particleTypeName + " in molecule " + name_.value());
+}
+
+ParticleName Molecule::particleName(int i) const
+{
+    return ParticleName(particles_.at(i).particleName_);
+}
+
+ResidueName Molecule::residueName(int i) const
+{
+    return ResidueName(particles_.at(i).residueName_);
+}
+
+std::unordered_map<std::string, ParticleType> Molecule::particleTypesMap() const
+{
+    std::unordered_map<std::string, ParticleType> m;
+    for (const ParticleType& t : particleTypes_)
+    {
+        m.emplace(t.name(), t);
+    }
+    return m;
+}
+
+std::vector<std::tuple<int, int>> Molecule::getExclusions() const
+{
+    // (particle name, residue name) -> index; the names of a molecule's
+    // particles identify them
+    std::map<std::pair<std::string, std::string>, int> index;
+    for (int i = 0; i < numParticlesInMolecule(); i++)
+    {
+        index.emplace(std::make_pair(particles_[i].particleName_, particles_[i].residueName_), i);
+    }
+    auto lookup = [&](const std::string& particle, const std::string& residue) {
+        auto it = index.find({ particle, residue });
+        if (it == index.end())
+        {
+            throw InputException("Particle " + particle + " in residue " + residue
+                                 + " not found in list of particles");
+        }
+        return it->second;
+    };
+    std::set<std::tuple<int, int>> all(exclusions_.begin(), exclusions_.end());
+    for (const auto& t : exclusionsByName_)
+    {
+        const int a = lookup(std::get<0>(t), std::get<1>(t));
+        const int b = lookup(std::get<2>(t), std::get<3>(t));
+        all.emplace(a, b);
+        all.emplace(b, a);
+    }
+    return std::vector<std::tuple<int, int>>(all.begin(), all.end());
+}
+
+// topology.cpp:175-204
+TopologyBuilder& TopologyBuilder::addMolecule(const Molecule& molecule, int nMolecules)
+{
+    if (nMolecules < 0)
+    {
+        throw InputException("cannot add a negative number of molecules");
+    }
+    for (const ParticleData& p : molecule.particleData())
+    {
+        const ParticleType& t    = molecule.at(p.particleTypeName_);
+        auto                same = [&](const ParticleType& u) { return u.name() == t.name(); };
+        auto                it   = std::find_if(particleTypes_.begin(), particleTypes_.end(), same);
+        if (it == particleTypes_.end())
+        {
+            particleTypes_.push_back(t);
+        }
+        else if (!(*it == t))
+        {
+            throw InputException("Differing ParticleTypes with identical names encountered");
+        }
+    }
+    molecules_.emplace_back(molecule, nMolecules);
+    numParticles_ += nMolecules * molecule.numParticlesInMolecule();
+    return *this;
+}
+
+void TopologyBuilder::addParticleTypesInteractions(const ParticleTypesInteractions& particleTypesInteractions)
+{
+    interactions_.merge(particleTypesInteractions);
+}
+
+// topology.cpp:58-174: per-particle charges and type ids, the exclusion
+// lists of every copy of every molecule offset to global indices, the
+// interaction table, and the check that every type pair has parameters
+Topology TopologyBuilder::buildTopology()
+{
+    std::vector<int>  typeIds;
+    std::vector<real> charges;
+    typeIds.reserve(numParticles_);
+    charges.reserve(numParticles_);
+    std::map<std::string, int> typeId;
+    for (size_t t = 0; t < particleTypes_.size(); t++)
+    {
+        typeId[particleTypes_[t].name()] = static_cast<int>(t);
+    }
+    ExclusionLists                                                         ex;
+    std::map<std::tuple<std::string, int, std::string, std::string>, int> sequence;
+    std::map<std::string, int>                                             copiesSoFar;
+    int                                                                    offset = 0;
+    for (const auto& mn : molecules_)
+    {
+        const Molecule&                         mol   = std::get<0>(mn);
+        const int                               count = std::get<1>(mn);
+        const int                               n     = mol.numParticlesInMolecule();
+        const std::vector<std::tuple<int, int>> pairs = mol.getExclusions();
+        const std::vector<ParticleData>         data  = mol.particleData();
+        // per particle of the molecule: its excluded partners, sorted
+        std::vector<std::vector<int>> perParticle(n);
+        for (const auto& pr : pairs)
+        {
+            perParticle[std::get<0>(pr)].push_back(std::get<1>(pr));
+        }
+        int& copy = copiesSoFar[mol.name().value()];
+        for (int m = 0; m < count; m++, copy++)
+        {
+            for (int i = 0; i < n; i++)
+            {
+                typeIds.push_back(typeId.at(data[i].particleTypeName_));
+                charges.push_back(data[i].charge_);
+                for (int j : perParticle[i])
+                {
+                    ex.indices.push_back(offset + j);
+                }
+                ex.offsets.push_back(static_cast<int>(ex.indices.size()));
+                sequence[{ mol.name().value(), copy, data[i].residueName_, data[i].particleName_ }] = offset + i;
+            }
+            offset += n;
+        }
+    }
+    NonBondedInteractionMap table = interactions_.generateTable();
+    for (const ParticleType& a : particleTypes_)
+    {
+        for (const ParticleType& b : particleTypes_)
+        {
+            try
+            {
+                table.getC6(a.name(), b.name());
+            }
+            catch (const InputException&)
+            {
+                throw InputException("Missing nonbonded interaction parameters for pair " + a.name() + " "
+                                     + b.name());
+            }
+        }
+    }
+    Topology top(particleTypes_, std::move(typeIds), std::move(charges), std::move(ex), std::move(table),
+                 interactions_.getCombinationRule());
+    top.sequence_ = std::move(sequence);
+    return top;
 }
 
 // simulationstate.cpp:70-84

@@ -10,10 +10,16 @@
 // nblib::InputException.  The force backend is the HIP library; there is no
 // CPU fallback, so constructing a ForceCalculator without a device throws.
 //
-// Differences, all additive: the topology is built directly from arrays
-// rather than through Molecule/TopologyBuilder (which only produce these
-// arrays), ForceCalculator exposes energies and the virial of the last
-// compute, and NBKernelOptions carries the HIP device ordinal.
+// Topologies are built as in the reference, through Molecule
+// (molecules.h:80-170) and TopologyBuilder (topology.h:133-175) with the
+// strong types of the reference's call sites (ParticleName, Charge,
+// MoleculeName, ParticleTypeName, Mass, C6, C12); a Topology can also be
+// made directly from the arrays the builder produces.  Type ids are given
+// in order of first appearance over the added molecules (the reference
+// numbers them in its hash map's iteration order, which only permutes the
+// ids).  Other differences, all additive: ForceCalculator exposes energies
+// and the virial of the last compute, and NBKernelOptions carries the HIP
+// device ordinal.
 #pragma once
 
 #include <array>
@@ -22,6 +28,8 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <tuple>
+#include <unordered_map>
 #include <vector>
 
 namespace nblib
@@ -60,6 +68,35 @@ private:
     T*          data_ = nullptr;
     std::size_t size_ = 0;
 };
+
+//! A value of type T tagged by Phantom, explicit to construct
+//! (nblib/util/user.h's StrongType): names and quantities cannot be mixed up
+template<typename T, typename Phantom>
+class StrongType
+{
+public:
+    StrongType() = default;
+    template<typename U>
+    explicit StrongType(U&& v) : value_(std::forward<U>(v))
+    {
+    }
+    const T& value() const { return value_; }
+    operator const T&() const { return value_; }
+    bool operator==(const StrongType& o) const { return value_ == o.value_; }
+    bool operator<(const StrongType& o) const { return value_ < o.value_; }
+
+private:
+    T value_{};
+};
+
+using ParticleTypeName = StrongType<std::string, struct ParticleTypeNameParameter>;
+using Mass             = StrongType<real, struct MassParameter>;
+using C6               = StrongType<real, struct C6Parameter>;
+using C12              = StrongType<real, struct C12Parameter>;
+using ParticleName     = StrongType<std::string, struct ParticleNameParameter>;
+using Charge           = StrongType<real, struct ChargeParameter>;
+using ResidueName      = StrongType<std::string, struct ResidueNameParameter>;
+using MoleculeName     = StrongType<std::string, struct MoleculeNameParameter>;
 
 //! Invalid user input (api/nblib/exception.h)
 class InputException : public std::runtime_error
@@ -129,9 +166,12 @@ struct NBKernelOptions
 class ParticleType
 {
 public:
+    ParticleType(ParticleTypeName name, Mass mass) : name_(name.value()), mass_(mass.value()) {}
     ParticleType(std::string name, real mass) : name_(std::move(name)), mass_(mass) {}
     const std::string& name() const { return name_; }
     real               mass() const { return mass_; }
+    //! particletype.cpp: same name and mass
+    bool operator==(const ParticleType& o) const { return name_ == o.name_ && mass_ == o.mass_; }
 
 private:
     std::string name_;
@@ -158,7 +198,20 @@ public:
     explicit ParticleTypesInteractions(CombinationRule rule = CombinationRule::Geometric) : rule_(rule) {}
     ParticleTypesInteractions& add(const std::string& name, real c6, real c12);
     ParticleTypesInteractions& add(const std::string& a, const std::string& b, real c6, real c12);
-    NonBondedInteractionMap    generateTable() const;
+    //! the reference's signatures (interactions.h:110-118)
+    ParticleTypesInteractions& add(const ParticleTypeName& name, C6 c6, C12 c12)
+    {
+        return add(name.value(), c6.value(), c12.value());
+    }
+    ParticleTypesInteractions& add(const ParticleTypeName& a, const ParticleTypeName& b, C6 c6, C12 c12)
+    {
+        return add(a.value(), b.value(), c6.value(), c12.value());
+    }
+    NonBondedInteractionMap generateTable() const;
+    CombinationRule         getCombinationRule() const { return rule_; }
+    //! interactions.cpp:192-220: the other's parameters join these (same rule;
+    //! conflicting duplicates throw)
+    void merge(const ParticleTypesInteractions& other);
 
 private:
     CombinationRule                                                      rule_;
@@ -190,13 +243,91 @@ public:
     const NonBondedInteractionMap& getNonBondedInteractionMap() const { return interactions_; }
     CombinationRule                getCombinationRule() const { return rule_; }
 
+    //! topology.cpp:264-268 (ParticleSequencer): the global index of particle
+    //! particleName in residue residueName of the moleculeNr-th molecule
+    //! called moleculeName (topologies from TopologyBuilder only)
+    int sequenceID(const MoleculeName& moleculeName, int moleculeNr, const ResidueName& residueName,
+                   const ParticleName& particleName) const;
+
 private:
+    friend class TopologyBuilder;
     std::vector<ParticleType> types_;
     std::vector<int>          typeIds_;
     std::vector<real>         charges_;
     ExclusionLists            exclusions_;
     NonBondedInteractionMap   interactions_;
     CombinationRule           rule_;
+    std::map<std::tuple<std::string, int, std::string, std::string>, int> sequence_;
+};
+
+//! molecules.h:70-76
+struct ParticleData
+{
+    std::string particleName_;
+    std::string residueName_;
+    std::string particleTypeName_;
+    real        charge_;
+};
+
+//! molecules.h:80-170: a molecule type, its particles and the exclusions
+//! between them (each particle excludes itself)
+class Molecule
+{
+public:
+    explicit Molecule(MoleculeName moleculeName);
+
+    Molecule& addParticle(const ParticleName& particleName, const ResidueName& residueName, const Charge& charge,
+                          const ParticleType& particleType);
+    //! charge 0
+    Molecule& addParticle(const ParticleName& particleName, const ResidueName& residueName,
+                          const ParticleType& particleType);
+    //! residue named after the molecule
+    Molecule& addParticle(const ParticleName& particleName, const Charge& charge, const ParticleType& particleType);
+    //! residue named after the molecule, charge 0
+    Molecule& addParticle(const ParticleName& particleName, const ParticleType& particleType);
+
+    //! both orders; a particle with itself is already excluded
+    void addExclusion(int particleIndex, int particleIndexToExclude);
+    //! by (particle name, residue name), resolved when the topology is built
+    void addExclusion(std::tuple<std::string, std::string> particle,
+                      std::tuple<std::string, std::string> particleToExclude);
+    //! by particle name within the residue named after the molecule
+    void addExclusion(const std::string& particleName, const std::string& particleNameToExclude);
+
+    int                 numParticlesInMolecule() const { return static_cast<int>(particles_.size()); }
+    const ParticleType& at(const std::string& particleTypeName) const;
+    //! sorted, without duplicates, both orders, self pairs included
+    std::vector<std::tuple<int, int>>             getExclusions() const;
+    ParticleName                                  particleName(int i) const;
+    ResidueName                                   residueName(int i) const;
+    std::vector<ParticleData>                     particleData() const { return particles_; }
+    std::unordered_map<std::string, ParticleType> particleTypesMap() const;
+    MoleculeName                                  name() const { return name_; }
+
+private:
+    MoleculeName                      name_;
+    std::vector<ParticleData>         particles_;
+    std::vector<ParticleType>         particleTypes_; // distinct, in order of first use
+    std::vector<std::tuple<int, int>> exclusions_;
+    std::vector<std::tuple<std::string, std::string, std::string, std::string>> exclusionsByName_;
+};
+
+//! topology.h:133-175, topology.cpp:58-217
+class TopologyBuilder
+{
+public:
+    TopologyBuilder() = default;
+    //! nMolecules copies of molecule, appended in order
+    TopologyBuilder& addMolecule(const Molecule& molecule, int nMolecules);
+    void             addParticleTypesInteractions(const ParticleTypesInteractions& particleTypesInteractions);
+    //! the topology of every molecule added; throws if a type pair has no parameters
+    Topology buildTopology();
+
+private:
+    std::vector<std::tuple<Molecule, int>> molecules_;
+    std::vector<ParticleType>              particleTypes_; // distinct, in order of first appearance
+    ParticleTypesInteractions              interactions_;
+    int                                    numParticles_ = 0;
 };
 
 //! simulationstate.h:66-97

@@ -14,6 +14,7 @@
 #include <fstream>
 #include <functional>
 #include <iostream>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -140,6 +141,83 @@ TestSystem load(const std::string& path)
     return s;
 }
 
+// The reference's test systems (api/nblib/tests/testsystems.cpp:50-335),
+// built the way it builds them: a particle library, Molecule with
+// exclusions by name, TopologyBuilder.
+struct ParticleLibrary
+{
+    // testsystems.cpp:53-82
+    std::map<std::string, ParticleType> types{
+        { "Ow", ParticleType(ParticleTypeName("Ow"), Mass(15.99940f)) },
+        { "H", ParticleType(ParticleTypeName("H"), Mass(1.008f)) },
+        { "OMet", ParticleType(ParticleTypeName("OMet"), Mass(15.999f)) },
+        { "CMet", ParticleType(ParticleTypeName("CMet"), Mass(15.035f)) },
+        { "Ar", ParticleType(ParticleTypeName("Ar"), Mass(39.94800f)) },
+    };
+    std::map<std::string, real> c6{ { "Ow", 0.0026173456f }, { "H", 0.0f }, { "OMet", 0.0022619536f },
+                                    { "CMet", 0.0088755241f }, { "Ar", 0.0062647225f } };
+    std::map<std::string, real> c12{ { "Ow", 2.634129e-06f }, { "H", 0.0f }, { "OMet", 1.505529e-06f },
+                                     { "CMet", 2.0852922e-05f }, { "Ar", 9.847044e-06f } };
+    const ParticleType& type(const std::string& n) const { return types.at(n); }
+    ParticleTypesInteractions interactions(const std::vector<std::string>& names) const
+    {
+        ParticleTypesInteractions p;
+        for (const std::string& n : names)
+        {
+            p.add(ParticleTypeName(n), C6(c6.at(n)), C12(c12.at(n)));
+        }
+        return p;
+    }
+};
+
+// testsystems.cpp:104-121 and 130-160
+Molecule waterMolecule(const ParticleLibrary& plib)
+{
+    Molecule water(MoleculeName("SOL"));
+    water.addParticle(ParticleName("Oxygen"), Charge(-0.82f), plib.type("Ow"));
+    water.addParticle(ParticleName("H1"), Charge(+0.41f), plib.type("H"));
+    water.addParticle(ParticleName("H2"), Charge(+0.41f), plib.type("H"));
+    water.addExclusion("H1", "Oxygen");
+    water.addExclusion("H2", "Oxygen");
+    water.addExclusion("H1", "H2");
+    return water;
+}
+
+Molecule methanolMolecule(const ParticleLibrary& plib)
+{
+    Molecule methanol(MoleculeName("MeOH"));
+    methanol.addParticle(ParticleName("Me1"), Charge(+0.176f), plib.type("CMet"));
+    methanol.addParticle(ParticleName("O2"), Charge(-0.574f), plib.type("OMet"));
+    methanol.addParticle(ParticleName("H3"), Charge(+0.398f), plib.type("H"));
+    methanol.addExclusion("Me1", "O2");
+    methanol.addExclusion("Me1", "H3");
+    methanol.addExclusion("H3", "O2");
+    return methanol;
+}
+
+// SpcMethanolTopologyBuilder::buildTopology (testsystems.cpp:188-208)
+Topology spcMethanolTopology(int numWater, int numMethanol)
+{
+    ParticleLibrary plib;
+    TopologyBuilder builder;
+    builder.addMolecule(methanolMolecule(plib), numMethanol);
+    builder.addMolecule(waterMolecule(plib), numWater);
+    builder.addParticleTypesInteractions(plib.interactions({ "Ow", "H", "OMet", "CMet" }));
+    return builder.buildTopology();
+}
+
+// ArgonTopologyBuilder (testsystems.cpp:225-245)
+Topology argonTopology(int numParticles)
+{
+    ParticleLibrary plib;
+    Molecule        argon(MoleculeName("AR"));
+    argon.addParticle(ParticleName("AR"), plib.type("Ar"));
+    TopologyBuilder builder;
+    builder.addMolecule(argon, numParticles);
+    builder.addParticleTypesInteractions(plib.interactions({ "Ar" }));
+    return builder.buildTopology();
+}
+
 NBKernelOptions plainCutoff(real rc)
 {
     NBKernelOptions o;
@@ -179,6 +257,66 @@ void hostTests()
         ex.offsets = { 0, 1 };
         expectThrowsInput([&] { Topology t({ ParticleType("A", 1) }, { 1 }, { 0.0f }, ex, p.generateTable()); },
                           "type id");
+    });
+    run("TopologyBuilderSpcMethanol", [] {
+        // the layout the reference's builder produces: methanol then water,
+        // charges per particle, each particle excluding its molecule
+        const Topology t = spcMethanolTopology(1, 1);
+        expect(t.numParticles() == 6, "6 particles");
+        const std::vector<real> q{ 0.176f, -0.574f, 0.398f, -0.82f, 0.41f, 0.41f };
+        expect(t.getCharges() == q, "charges");
+        const ExclusionLists& ex = t.getGmxExclusions();
+        const std::vector<int> off{ 0, 3, 6, 9, 12, 15, 18 };
+        const std::vector<int> idx{ 0, 1, 2, 0, 1, 2, 0, 1, 2, 3, 4, 5, 3, 4, 5, 3, 4, 5 };
+        expect(ex.offsets == off && ex.indices == idx, "exclusions");
+        const std::vector<std::string> names{ "CMet", "OMet", "H", "Ow", "H", "H" };
+        for (int a = 0; a < 6; a++)
+        {
+            const std::string& n = t.getParticleTypes()[t.getParticleTypeIdOfAllParticles()[a]].name();
+            expect(n == names[a], "type of particle " + std::to_string(a) + " is " + n);
+        }
+        expect(t.getParticleTypes().size() == 4, "4 distinct types");
+        // geometric combination of the library's C6 (interactions.cpp:83-93)
+        const real c6 = t.getNonBondedInteractionMap().getC6("Ow", "CMet");
+        expect(std::fabs(c6 - std::sqrt(0.0026173456f * 0.0088755241f)) <= 1e-9f, "combined C6");
+        expect(t.sequenceID(MoleculeName("SOL"), 0, ResidueName("SOL"), ParticleName("H2")) == 5, "sequenceID");
+        expect(t.sequenceID(MoleculeName("MeOH"), 0, ResidueName("MeOH"), ParticleName("O2")) == 1, "sequenceID");
+        const Topology two = spcMethanolTopology(2, 1);
+        expect(two.numParticles() == 9 && two.getGmxExclusions().indices[21] == 6, "second water offset");
+        expect(two.sequenceID(MoleculeName("SOL"), 1, ResidueName("SOL"), ParticleName("Oxygen")) == 6,
+               "second water");
+    });
+    run("TopologyBuilderArgon", [] {
+        const Topology t = argonTopology(12);
+        expect(t.numParticles() == 12 && t.getParticleTypes().size() == 1, "12 argon atoms of one type");
+        for (int a = 0; a < 12; a++)
+        {
+            const ExclusionLists& ex = t.getGmxExclusions();
+            expect(ex.offsets[a + 1] - ex.offsets[a] == 1 && ex.indices[ex.offsets[a]] == a, "self exclusion");
+            expect(t.getCharges()[a] == 0.0f, "charge 0");
+        }
+    });
+    run("MoleculeInputErrors", [] {
+        ParticleLibrary plib;
+        Molecule        m(MoleculeName("X"));
+        m.addParticle(ParticleName("A"), plib.type("Ow"));
+        // same type name, different mass
+        expectThrowsInput([&] { m.addParticle(ParticleName("B"), ParticleType(ParticleTypeName("Ow"), Mass(1.0f))); },
+                          "differing particle types");
+        m.addExclusion("A", "Nope");
+        expectThrowsInput([&] { m.getExclusions(); }, "exclusion by unknown name");
+        expectThrowsInput([&] { m.addExclusion(0, 3); }, "exclusion index out of range");
+        // a type without parameters
+        TopologyBuilder b;
+        Molecule        w = waterMolecule(plib);
+        b.addMolecule(w, 2);
+        b.addParticleTypesInteractions(plib.interactions({ "Ow" }));
+        expectThrowsInput([&] { b.buildTopology(); }, "missing interaction parameters");
+        TopologyBuilder b2;
+        Molecule        other(MoleculeName("Y"));
+        other.addParticle(ParticleName("O"), ParticleType(ParticleTypeName("Ow"), Mass(2.0f)));
+        b2.addMolecule(w, 1);
+        expectThrowsInput([&] { b2.addMolecule(other, 1); }, "differing types across molecules");
     });
     run("SimulationStateRejectsNaN", [] {
         ParticleTypesInteractions p;
@@ -223,6 +361,36 @@ void deviceTests(const std::vector<TestSystem>& systems)
     for (const TestSystem& sys : systems)
     {
         run(sys.name + "ForcesAreCorrect", [&] { forcesAreCorrect(sys); });
+    }
+    // the same systems with the topology from Molecule/TopologyBuilder
+    // (testsystems.cpp:244-335) instead of the arrays
+    for (const TestSystem& sys : systems)
+    {
+        run(sys.name + "BuiltWithTopologyBuilderForcesAreCorrect", [&] {
+            const Topology top = sys.name == "Argon" ? argonTopology(static_cast<int>(sys.x.size()))
+                                                     : spcMethanolTopology(1, 1);
+            std::vector<Vec3> zeros(sys.x.size());
+            SimulationState   state(sys.x, zeros, zeros, Box(sys.box), top);
+            ForceCalculator   calc(state, plainCutoff(sys.rc));
+            ArrayRef<Vec3>    forces(state.forces());
+            calc.compute(state.coordinates(), forces);
+            real fmax = 0;
+            for (const Vec3& f : sys.fRef)
+            {
+                for (int d = 0; d < 3; d++)
+                {
+                    fmax = std::max(fmax, std::fabs(f[d]));
+                }
+            }
+            for (size_t i = 0; i < forces.size(); i++)
+            {
+                for (int d = 0; d < 3; d++)
+                {
+                    expect(std::fabs(forces[i][d] - sys.fRef[i][d]) <= 5e-5f * fmax,
+                           "force " + std::to_string(i) + "." + std::to_string(d));
+                }
+            }
+        });
     }
     const TestSystem& spc = systems.back();
     run("ExpectedNumberOfForces", [&] {
