@@ -207,11 +207,16 @@ def bench_forward_backward(stream, reps=20, ewald="analytical"):
     }
 
 
-def bench_prune(stream, reps=10):
+def bench_prune(stream, reps=20, parts=4):
     """Dynamic pruning on the 288k water box searched with a 0.1 nm buffer
-    (rlist 1.1 nm) and pruned back to the 1.0 nm cut-off: the search and the
-    prune (both including the rebuild of the kernel's work layout), and the
-    F+E+shift step on the list as searched and as pruned."""
+    (rlist 1.1 nm) and pruned back to the 1.0 nm cut-off (static
+    coordinates): the search (with the kernel's work layout); the prune inside
+    the work layout (nbx_prune_rolling: whole layout, and one of `parts`
+    rolling parts); the F+E+shift step on the layout as searched and as
+    pruned; and the step loop with one rolling part pruned before every step
+    (the prune amortised as in an MD loop with rolling pruning).  The full
+    re-listing prune (nbx_prune_pairlist, which rebuilds the layout) is
+    timed for comparison."""
     import numpy as np
     import torch
 
@@ -236,22 +241,46 @@ def bench_prune(stream, reps=10):
     search_ms = ev[0].elapsed_time(ev[1])
     outer_cj = ctx.info()["num_j_entries"]
     outer_ms = _time(step, stream, reps)
+    tiles_outer = int(ctx.work_groups()["chunks"][:, 3].sum())
+    prune_all_ms = _time(lambda: ctx.prune_rolling(s.rcoulomb), stream, reps)
+    tiles_inner = int(ctx.work_groups()["chunks"][:, 3].sum())
+    inner_ms = _time(step, stream, reps)
+    part = [0]
+
+    def prune_part():
+        ctx.prune_rolling(s.rcoulomb, part[0] % parts, parts)
+        part[0] += 1
+
+    prune_part_ms = _time(prune_part, stream, reps)
+
+    def rolling_step():
+        prune_part()
+        step()
+
+    rolling_ms = _time(rolling_step, stream, reps)
+    # the re-listing prune (the list itself pruned, layout rebuilt)
     ev[0].record(stream)
     ctx.prune_pairlist(s.rcoulomb)
     ev[1].record(stream)
     torch.cuda.synchronize()
-    prune_ms = ev[0].elapsed_time(ev[1])
+    relist_ms = ev[0].elapsed_time(ev[1])
     inner_cj = ctx.info()["num_j_entries"]
-    inner_ms = _time(step, stream, reps)
     return {
         "workload": "water-288k searched at rlist 1.1 nm, pruned to rc 1.0 nm (static coordinates)",
         "search_ms": search_ms,
-        "prune_ms": prune_ms,
         "cluster_pairs_searched": int(outer_cj),
         "cluster_pairs_pruned": int(inner_cj),
+        "kernel_tiles_searched": tiles_outer,
+        "kernel_tiles_pruned": tiles_inner,
+        "prune_layout_ms": prune_all_ms,
+        "prune_layout_part_ms": prune_part_ms,
+        "rolling_parts": parts,
         "step_ms_searched_list": outer_ms,
-        "step_ms_pruned_list": inner_ms,
-        "note": "search and prune times include rebuilding the kernel's work layout and schedule",
+        "step_ms_pruned_layout": inner_ms,
+        "step_ms_rolling_prune_amortised": rolling_ms,
+        "prune_relist_ms": relist_ms,
+        "note": "prune_layout: nbx_prune_rolling over the whole layout; rolling step: one of the parts pruned "
+                "before each step; prune_relist (nbx_prune_pairlist) includes rebuilding the work layout",
     }
 
 
@@ -441,7 +470,7 @@ def main():
             "energies_kj_mol": {"lj": vlj, "coulomb_real_space": vc},
         },
         "roofline": {
-            "bound": "mfma",
+            "bound": "valu",
             "unit": "TFLOP/s",
             "achieved": achieved,
             "peak": PEAK_FP32_TFLOPS,

@@ -1134,6 +1134,7 @@ static void finishList(nbx_context* c)
     c->virial.ensure(9);
     c->listValid  = true;
     c->adjValid   = false;
+    c->layOuterValid = false;
     c->haveEnergy = c->haveShift = c->haveVirial = false;
 }
 
@@ -1249,6 +1250,75 @@ int nbx_prune_pairlist(nbx_context* c, double rlistInner)
         launchPrune(c->stream, a, true);
         check(hipGetLastError(), "prune kernels");
         finishList(c);
+    });
+}
+
+int nbx_prune_rolling(nbx_context* c, double rlistInner, int part, int numParts)
+{
+    if (!c)
+    {
+        return fail(NBX_ERROR_INPUT, "NULL context");
+    }
+    return guarded([&] {
+        requireList(c);
+        if (!(rlistInner >= c->rcoulomb && rlistInner <= c->rlist))
+        {
+            throw InputError("the inner pair-list cut-off must lie between the interaction cut-off and rlist");
+        }
+        if (numParts < 1 || part < 0 || part >= numParts)
+        {
+            throw InputError("the rolling prune part must lie in [0, num_parts) with num_parts >= 1");
+        }
+        setDevice(c);
+        WorkLayout& L        = c->lay;
+        const int   numWaves = L.numBlocks * kKernelWaves;
+        LayoutPruneArgs a{};
+        a.nchunk    = L.nchunk;
+        a.numWaves  = numWaves;
+        a.ngroups   = L.ngroups;
+        a.part      = part;
+        a.numParts  = numParts;
+        a.groupSize = L.groupSize;
+        a.ncluster  = c->ncluster;
+        const float r = static_cast<float>(rlistInner);
+        a.rlist2      = r * r;
+        if (!c->layOuterValid)
+        {
+            // keep the layout as built: every prune starts from it
+            c->chunksOuter.ensure(std::max(L.nchunk, 1));
+            c->workOuter.ensure(std::max(L.nwork, 1));
+            c->chunkWave.ensure(std::max(L.nchunk, 1));
+            c->chunkGroup.ensure(std::max(L.nchunk, 1));
+            c->keptCount.ensure(std::max(L.nchunk, 1));
+            c->keptMasked.ensure(std::max(L.nchunk, 1));
+            c->tileKeep.ensure(std::max(L.nwork / kTileJ, 1));
+            check(hipMemcpyAsync(c->chunksOuter.p, L.chunks.p, sizeof(int4) * L.nchunk, hipMemcpyDeviceToDevice,
+                                 c->stream),
+                  "keep outer layout");
+            check(hipMemcpyAsync(c->workOuter.p, L.work.p, sizeof(uint2) * L.nwork, hipMemcpyDeviceToDevice,
+                                 c->stream),
+                  "keep outer layout");
+        }
+        a.xq          = c->xq.p;
+        a.chunkShift  = L.chunkShift.p;
+        a.waveStart   = L.waveStart.p;
+        a.groups      = L.groups.p;
+        a.chunksOuter = c->chunksOuter.p;
+        a.workOuter   = c->workOuter.p;
+        a.chunkWave   = c->chunkWave.p;
+        a.chunkGroup  = c->chunkGroup.p;
+        a.keptCount   = c->keptCount.p;
+        a.keptMasked  = c->keptMasked.p;
+        a.tileKeep    = c->tileKeep.p;
+        a.chunks      = L.chunks.p;
+        a.work        = L.work.p;
+        if (!c->layOuterValid)
+        {
+            launchLayoutPruneSetup(c->stream, a);
+            c->layOuterValid = true;
+        }
+        launchLayoutPrune(c->stream, a);
+        check(hipGetLastError(), "rolling prune kernels");
     });
 }
 

@@ -275,6 +275,14 @@ struct nbx_context
     std::vector<float>  c6diag;
     std::vector<long>   typeCount;
     nbx::WorkLayout     lay;
+    // pruning inside the pair kernel's layout: its chunks and work entries
+    // as built (the outer layout), the chunk -> wavefront and group maps,
+    // and per-prune scratch; valid until the layout is rebuilt
+    bool                        layOuterValid = false;
+    nbx::DevBuf<int4>           chunksOuter;
+    nbx::DevBuf<uint2>          workOuter;
+    nbx::DevBuf<int>            chunkWave, chunkGroup, keptCount, keptMasked;
+    nbx::DevBuf<unsigned short> tileKeep;
     nbx::WorkLayout     adj;
     bool                adjValid = false;
 

@@ -162,6 +162,39 @@ struct PruneArgs
 };
 void launchPrune(hipStream_t s, const PruneArgs& a, bool fill);
 
+// Pruning inside the pair kernel's work layout (prune.hip): the layout's
+// chunks and work entries as built from the searched list are kept (the
+// outer layout); a prune rewrites, for the chunks of the wavefronts in the
+// selected part, the kept entries compacted at the front of each
+// wavefront's region of the work list, every chunk padded to whole tiles
+// (at least one), with its tile and masked-tile counts.  Groups, window
+// slots, the static schedule and the staging layout stay as they are.
+struct LayoutPruneArgs
+{
+    int            nchunk;
+    int            numWaves;
+    int            ngroups;
+    int            part, numParts;  // wavefronts w with w % numParts == part
+    int            groupSize;       // the layout's G (padding entries use slot G + windowed union)
+    int            ncluster;        // the far-away dummy cluster of the padding entries
+    float          rlist2;
+    const float4*  xq;
+    const float4*  chunkShift;      // per chunk: its i-entry's shift vector
+    const int2*    waveStart;       // [numWaves + 1]
+    const int4*    groups;          // [ngroups]: chunk range, union offset, windowed union size
+    const int4*    chunksOuter;     // the layout as built
+    const uint2*   workOuter;
+    int*           chunkWave;       // set-up: each chunk's wavefront
+    int*           chunkGroup;      // set-up: each chunk's group
+    int*           keptCount;       // per chunk: kept entries
+    int*           keptMasked;      // per chunk: kept entries with exclusion or diagonal bits
+    unsigned short* tileKeep;       // per outer tile: one bit per kept entry
+    int4*          chunks;          // the pruned layout
+    uint2*         work;
+};
+void launchLayoutPruneSetup(hipStream_t s, const LayoutPruneArgs& a);
+void launchLayoutPrune(hipStream_t s, const LayoutPruneArgs& a);
+
 // -- grid.hip
 void launchGridColumns(hipStream_t s, const GridDims& g, const float* x, int* atomCol, int* atomRank, int* cxy_na);
 void launchGridScan(hipStream_t s, int ncol, const int* cxy_na, int* cxy_ind, int* ncluster_out);

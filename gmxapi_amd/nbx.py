@@ -153,6 +153,7 @@ def library() -> C.CDLL:
     lib.nbx_put_on_grid.argtypes = [vp, vp, vp, i]
     lib.nbx_construct_pairlist.argtypes = [vp]
     lib.nbx_prune_pairlist.argtypes = [vp, d]
+    lib.nbx_prune_rolling.argtypes = [vp, d, i, i]
     lib.nbx_convert_coordinates.argtypes = [vp, vp, i]
     lib.nbx_dispatch_kernel.argtypes = [vp, u]
     lib.nbx_add_nbat_f_to_f.argtypes = [vp, vp, i]
@@ -293,6 +294,14 @@ class Context:
         """Dynamic pruning (dispatchPruneKernelCpu, nbnxm.h:312-313): the list
         as searched pruned to r_inner with the current coordinates."""
         self._check(self._lib.nbx_prune_pairlist(self._h, float(r_inner)), "nbx_prune_pairlist")
+
+    def prune_rolling(self, r_inner: float, part: int = 0, num_parts: int = 1):
+        """Rolling dynamic pruning inside the kernel's work layout
+        (nbnxm.h:306-316, pairlistparams.h:136-138): the kernel wavefronts
+        of `part` out of `num_parts` get the layout as built pruned to
+        r_inner with the current coordinates; asynchronous."""
+        self._check(self._lib.nbx_prune_rolling(self._h, float(r_inner), int(part), int(num_parts)),
+                    "nbx_prune_rolling")
 
     def compute(self, x, energy=False, shift_forces=False, virial=False) -> np.ndarray:
         """Forces in atom order (host arrays, synchronous)."""

@@ -184,6 +184,23 @@ int nbx_construct_pairlist(nbx_context* context);
  * rcoulomb <= rlist_inner <= rlist. */
 int nbx_prune_pairlist(nbx_context* context, double rlist_inner);
 
+/* Dynamic pruning inside the step loop, rolling over parts
+ * (nonbonded_verlet_t::dispatchPruneKernelCpu/Gpu, nbnxm.h:306-316, with
+ * the list split into numRollingPruningParts parts, pairlistparams.h:136-138,
+ * and the outer list kept as pairlist.cpp:4264-4283 keeps it; the prune
+ * criterion of kernel_ref_prune.cpp:39-123): the pair kernel's work layout,
+ * as built from the list as searched, is pruned to the j-clusters with at
+ * least one atom pair closer than rlist_inner with the current coordinates
+ * (those of the last put_on_grid, convert_coordinates or compute), for the
+ * kernel wavefronts w with w % num_parts == part; the other parts keep their
+ * last prune.  Every prune starts from the layout as built.  The layout's
+ * groups, window slots, schedule and staging stay, so nothing is rebuilt,
+ * nothing is allocated after the first call and the host does not wait:
+ * the call only queues three kernels.  nbx_get_pairlist still returns the
+ * list as searched; nbx_get_work_groups shows the pruned layout.
+ * rcoulomb <= rlist_inner <= rlist, 0 <= part < num_parts. */
+int nbx_prune_rolling(nbx_context* context, double rlist_inner, int part, int num_parts);
+
 /* nonbonded_verlet_t::convertCoordinates (nbnxm.cpp:129-141). */
 int nbx_convert_coordinates(nbx_context* context, const float* x, int x_on_device);
 

@@ -102,3 +102,113 @@ def test_prune_radius_errors(case):
     for r in (0.85, 1.2):
         with pytest.raises(nbx.NbxInputError):
             ctx.prune_pairlist(r)
+
+
+# -- pruning inside the kernel's work layout (nbx_prune_rolling) ----------
+
+
+def _layout_entries(wg, nc):
+    """Per chunk: (ci, shift, [(cj, mask) of its non-padding work entries in
+    order]), with the tile and masked-tile counts."""
+    out = []
+    for ci, flags, ws, tiles in wg["chunks"]:
+        ents = [(int(wg["work"][k, 0]), int(wg["work"][k, 1]) & 0xFFFF) for k in range(ws, ws + 16 * tiles)]
+        real = [e for e in ents if e[0] != nc]
+        # padding only at the end
+        assert all(e[0] == nc for e in ents[len(real):])
+        out.append((int(ci), int(flags) & 0xFF, real, int(tiles), int(flags) >> 9))
+    return out
+
+
+def test_rolling_prune_matches_the_oracle(case):
+    print("[rolling prune] 1200-atom water, rlist 1.15 -> 1.0, one part", flush=True)
+    s, x1 = case
+    nc = nbx.context_for(s).info()["num_clusters"]
+    outer_ctx = nbx.context_for(s)
+    outer = _layout_entries(outer_ctx.work_groups(), nc)
+    ctx = nbx.context_for(s)
+    ctx.convert_coordinates(x1)
+    ctx.prune_rolling(R_INNER)
+    ctx.synchronize()
+    pruned = _layout_entries(ctx.work_groups(), nc)
+    o = oracle_for(s, "f32")
+    o.prune(R_INNER, x=x1)
+    ci, cj, ex = o.pairlist()
+    want = set(_entries(ci, cj, ex))
+    assert len(pruned) == len(outer)
+    n_out = n_in = 0
+    for (c, sh, ents, tiles, special), (c0, sh0, ents0, _t0, _s0) in zip(pruned, outer):
+        assert (c, sh) == (c0, sh0)
+        # the kept entries: the oracle's pruned entries among the layout's,
+        # in the layout's order, masked ones first
+        keep0 = [e for e in ents0 if (c, sh, e[0], e[1]) in want]
+        assert ents == keep0
+        assert tiles == max(1, (len(ents) + 15) // 16)
+        nmask = sum(1 for e in ents if e[1] != 0xFFFF)
+        assert special == (nmask + 15) // 16
+        assert all(e[1] != 0xFFFF for e in ents[:nmask])
+        n_out += len(ents0)
+        n_in += len(ents)
+    print(f"  {n_out} -> {n_in} entries", flush=True)
+    assert 0 < n_in < n_out
+
+
+def test_rolling_prune_forces(case):
+    s, x1 = case
+    ctx = nbx.context_for(s)
+    ctx.convert_coordinates(x1)
+    ctx.prune_rolling(R_INNER)
+    f = ctx.compute(x1, energy=True, virial=True)
+    e = ctx.energies()
+    full = nbx.context_for(s)
+    f_full = full.compute(x1, energy=True, virial=True)
+    e_full = full.energies()
+    r64 = oracle_for(s, "f64").compute(x1)
+    err = rms(f - r64["f"]) / rms(r64["f"])
+    print(f"  force rms err vs f64 {err:.2e}; vs unpruned {rms(f - f_full) / rms(f_full):.2e}", flush=True)
+    assert err <= 5e-4
+    assert rms(f - f_full) <= 1e-6 * rms(f_full)
+    for a, b in zip(e, e_full):
+        assert abs(a - b) <= 1e-6 * abs(b)
+    np.testing.assert_allclose(ctx.virial(), full.virial(), rtol=0, atol=1e-6 * np.abs(full.virial()).max())
+
+
+def test_rolling_parts_compose(case):
+    # pruning the parts one after another gives the one-part prune; a part
+    # left alone keeps the layout as built (or its last prune)
+    s, x1 = case
+    one = nbx.context_for(s)
+    one.convert_coordinates(x1)
+    one.prune_rolling(R_INNER)
+    parts = nbx.context_for(s)
+    parts.convert_coordinates(x1)
+    parts.prune_rolling(R_INNER, 0, 3)
+    parts.synchronize()
+    half = parts.work_groups()
+    outer = nbx.context_for(s).work_groups()
+    wg1 = one.work_groups()
+    changed = np.any(half["chunks"] != outer["chunks"], axis=1)
+    assert changed.any() and not changed.all()
+    parts.prune_rolling(R_INNER, 1, 3)
+    parts.prune_rolling(R_INNER, 2, 3)
+    parts.synchronize()
+    ws = parts.work_groups()
+    np.testing.assert_array_equal(ws["chunks"], wg1["chunks"])
+    nc = parts.info()["num_clusters"]
+    assert _layout_entries(ws, nc) == _layout_entries(wg1, nc)
+    # pruning again from the layout as built, at a larger radius, keeps more
+    parts.prune_rolling(1.1)
+    parts.synchronize()
+    assert parts.work_groups()["chunks"][:, 3].sum() >= wg1["chunks"][:, 3].sum()
+    # a new search rebuilds the layout and the outer copy
+    parts.put_on_grid(s.box, s.x)
+    parts.construct_pairlist()
+    np.testing.assert_array_equal(parts.work_groups()["chunks"], outer["chunks"])
+
+
+def test_rolling_prune_errors(case):
+    s, _ = case
+    ctx = nbx.context_for(s)
+    for args in ((0.85, 0, 1), (1.2, 0, 1), (1.0, 1, 1), (1.0, -1, 2), (1.0, 0, 0)):
+        with pytest.raises(nbx.NbxInputError):
+            ctx.prune_rolling(*args)
