@@ -42,6 +42,9 @@
  */
 #include <float.h>
 #include <math.h>
+#ifdef ORC_OMP
+#    include <omp.h>
+#endif
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -1608,15 +1611,11 @@ int64_t orc_brute_pairs(int natoms, const real* x, const real* box3, double r2d,
 /* Plain-C reference kernel (kernel_ref_outer.h / kernel_ref_inner.h)  */
 /* ------------------------------------------------------------------ */
 
-/* One nbnxn_kernel_Elec{RF,QSTab}_VdwLJ_{F,VF}_ref call over the whole list
- * (a single list, i.e. one OpenMP thread), clearing nbat f first as
- * enbvClearFYes does.  The inner loop is the "LJ + Coulomb with exclusion
- * check" variant (kernel_ref_inner.h with CHECK_EXCLS and CALC_COULOMB): the
- * reference runs every entry through it because its all-ones mask is
- * 0xffffffff while the loop tests for 0xffff (kernel_ref_outer.h:329). */
-void orc_kernel(orc_sys* s, int energy)
+/* The i-entries [e0, e1) of the list: forces into f (cluster layout),
+ * shift forces into fshift, energies added to *pVvdw and *pVc. */
+static void kernel_entries(orc_sys* s, int energy, const real* ljc, int e0, int e1, real* f, real* fshift,
+                           real* pVvdw, real* pVc)
 {
-    const int  nslot    = s->ncells * ORC_NC;
     const real rcut2    = s->rcoulomb * s->rcoulomb;
     const int  twin     = s->rvdw < s->rcoulomb;
     const real rvdw2    = s->rvdw * s->rvdw;
@@ -1626,39 +1625,9 @@ void orc_kernel(orc_sys* s, int energy)
     const real halfsp   = (real)(0.5 / s->tab_scale);
     const int  tab      = (s->coulomb == 1 && !s->ewald_exact);
     const int  exact    = (s->coulomb == 1 && s->ewald_exact);
-    /* LJ-PME grid parameters per type from the nbfp diagonal (nbfp_comb,
-     * atomdata.cpp:288-330) */
-    real* ljc = (real*)calloc((size_t)(2 * s->numTypes + 2), sizeof(real));
-    for (int t = 0; s->lj_pme && t < s->numTypes; t++)
-    {
-        real c6  = s->nbfp[(t * s->numTypes + t) * 2];
-        real c12 = s->nbfp[(t * s->numTypes + t) * 2 + 1];
-        if (s->ljpme_comb == 0)
-        {
-            ljc[t * 2]     = (real)sqrt((double)c6);
-            ljc[t * 2 + 1] = (real)sqrt((double)c12);
-        }
-        else if (c6 > 0 && c12 > 0)
-        {
-            ljc[t * 2]     = (real)(0.5 * pow((double)(c12 / c6), 1.0 / 6.0));
-            ljc[t * 2 + 1] = (real)sqrt((double)(c6 * c6 / c12));
-        }
-    }
-    for (int k = 0; k < nslot * 3; k++)
-    {
-        s->fnb[k] = 0;
-    }
-    for (int k = 0; k < ORC_SHIFTS * 3; k++)
-    {
-        s->fshift[k] = 0;
-    }
-    s->Vvdw = 0;
-    s->Vc   = 0;
-    real* f = s->fnb;
     const real* x = s->xnb;
     const real* q = s->nbq;
-
-    for (int e = 0; e < s->nci; e++)
+    for (int e = e0; e < e1; e++)
     {
         int  ci    = s->ci[4 * e];
         int  ish   = s->ci[4 * e + 1];
@@ -1699,12 +1668,12 @@ void orc_kernel(orc_sys* s, int energy)
             {
                 for (int i = 0; i < ORC_NC; i++)
                 {
-                    s->Vc -= qi[i] * q[ci * ORC_NC + i] * Vc_sub_self;
+                    (*pVc) -= qi[i] * q[ci * ORC_NC + i] * Vc_sub_self;
                     if (s->lj_pme)
                     {
                         /* LJ Ewald self interaction (kernel_ref_outer.h:315-322) */
                         int t = s->nbtype[ci * ORC_NC + i];
-                        s->Vvdw += (real)0.5 * s->nbfp[(t * s->numTypes + t) * 2] / 6 * s->lje_coeff6_6;
+                        (*pVvdw) += (real)0.5 * s->nbfp[(t * s->numTypes + t) * 2] / 6 * s->lje_coeff6_6;
                     }
                 }
             }
@@ -1890,17 +1859,132 @@ void orc_kernel(orc_sys* s, int energy)
         {
             for (int d = 0; d < 3; d++)
             {
-                s->fshift[ish * 3 + d] += fi[i * 3 + d];
+                fshift[ish * 3 + d] += fi[i * 3 + d];
             }
         }
         if (energy)
         {
-            s->Vvdw += Vvdw_ci;
-            s->Vc += Vc_ci;
+            (*pVvdw) += Vvdw_ci;
+            (*pVc) += Vc_ci;
         }
     }
+}
+
+/* LJ-PME grid parameters per type from the nbfp diagonal (nbfp_comb,
+ * atomdata.cpp:288-330); the caller frees the array. */
+static real* lj_pme_grid_params(const orc_sys* s)
+{
+    real* ljc = (real*)calloc((size_t)(2 * s->numTypes + 2), sizeof(real));
+    for (int t = 0; s->lj_pme && t < s->numTypes; t++)
+    {
+        real c6  = s->nbfp[(t * s->numTypes + t) * 2];
+        real c12 = s->nbfp[(t * s->numTypes + t) * 2 + 1];
+        if (s->ljpme_comb == 0)
+        {
+            ljc[t * 2]     = (real)sqrt((double)c6);
+            ljc[t * 2 + 1] = (real)sqrt((double)c12);
+        }
+        else if (c6 > 0 && c12 > 0)
+        {
+            ljc[t * 2]     = (real)(0.5 * pow((double)(c12 / c6), 1.0 / 6.0));
+            ljc[t * 2 + 1] = (real)sqrt((double)(c6 * c6 / c12));
+        }
+    }
+    return ljc;
+}
+
+/* One nbnxn_kernel_Elec{RF,QSTab}_VdwLJ_{F,VF}_ref call over the whole list
+ * (a single list, i.e. one OpenMP thread), clearing nbat f first as
+ * enbvClearFYes does.  The inner loop is the "LJ + Coulomb with exclusion
+ * check" variant (kernel_ref_inner.h with CHECK_EXCLS and CALC_COULOMB): the
+ * reference runs every entry through it because its all-ones mask is
+ * 0xffffffff while the loop tests for 0xffff (kernel_ref_outer.h:329). */
+void orc_kernel(orc_sys* s, int energy)
+{
+    const int nslot = s->ncells * ORC_NC;
+    real*     ljc   = lj_pme_grid_params(s);
+    for (int k = 0; k < nslot * 3; k++)
+    {
+        s->fnb[k] = 0;
+    }
+    for (int k = 0; k < ORC_SHIFTS * 3; k++)
+    {
+        s->fshift[k] = 0;
+    }
+    s->Vvdw = 0;
+    s->Vc   = 0;
+    kernel_entries(s, energy, ljc, 0, s->nci, s->fnb, s->fshift, &s->Vvdw, &s->Vc);
     free(ljc);
 }
+
+#ifdef ORC_OMP
+void orc_add_nbat_f_to_f(const orc_sys* s, real* fout);
+
+/* The CPU baseline only (built apart with OpenMP, never the checker): the
+ * kernel over the list with OpenMP threads, each with its own force, shift
+ * force and energy outputs (nbnxn_atomdata_output_t per thread,
+ * kerneldispatch.cpp:243-356), reduced over the threads afterwards as
+ * reduceForces does (atomdata.cpp:1425-1469). */
+void orc_kernel_threads(orc_sys* s, int energy, int nthreads)
+{
+    const int nslot = s->ncells * ORC_NC;
+    real*     ljc   = lj_pme_grid_params(s);
+    real**    fth   = (real**)calloc((size_t)nthreads, sizeof(real*));
+    real*     fsh   = (real*)calloc((size_t)nthreads * ORC_SHIFTS * 3, sizeof(real));
+    real*     vv    = (real*)calloc((size_t)nthreads * 2, sizeof(real));
+#    pragma omp parallel num_threads(nthreads)
+    {
+        const int th = omp_get_thread_num();
+        fth[th]      = (real*)calloc((size_t)nslot * 3, sizeof(real));
+#    pragma omp for schedule(dynamic, 32)
+        for (int e = 0; e < s->nci; e++)
+        {
+            kernel_entries(s, energy, ljc, e, e + 1, fth[th], fsh + th * ORC_SHIFTS * 3, vv + 2 * th,
+                           vv + 2 * th + 1);
+        }
+#    pragma omp for schedule(static)
+        for (int k = 0; k < nslot * 3; k++)
+        {
+            real sum = 0;
+            for (int t = 0; t < nthreads; t++)
+            {
+                sum += fth[t][k];
+            }
+            s->fnb[k] = sum;
+        }
+    }
+    for (int k = 0; k < ORC_SHIFTS * 3; k++)
+    {
+        s->fshift[k] = 0;
+        for (int t = 0; t < nthreads; t++)
+        {
+            s->fshift[k] += fsh[t * ORC_SHIFTS * 3 + k];
+        }
+    }
+    s->Vvdw = 0;
+    s->Vc   = 0;
+    for (int t = 0; t < nthreads; t++)
+    {
+        s->Vvdw += vv[2 * t];
+        s->Vc += vv[2 * t + 1];
+        free(fth[t]);
+    }
+    free(fth);
+    free(fsh);
+    free(vv);
+    free(ljc);
+}
+
+/* convertCoordinates, the threaded kernel and add_nbat_f_to_f. */
+void orc_compute_threads(orc_sys* s, const real* x, real* f, int energy, int nthreads, double* out_e)
+{
+    orc_convert_coordinates(s, x);
+    orc_kernel_threads(s, energy, nthreads);
+    orc_add_nbat_f_to_f(s, f);
+    out_e[0] = s->Vvdw;
+    out_e[1] = s->Vc;
+}
+#endif
 
 /* add_nbat_f_to_f for one output buffer (atomdata.cpp:1181-1228), with the
  * zeroing nblib does first (gmxcalculator.cpp:76-82): f[a] = fnb[cells[a]]. */

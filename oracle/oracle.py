@@ -29,15 +29,19 @@ def build() -> None:
 
 
 def _lib(precision: str) -> C.CDLL:
-    if precision not in ("f32", "f64"):
-        raise ValueError(f"precision must be f32 or f64, not {precision!r}")
+    """f32 / f64: the checker; bench_f32: the CPU-baseline build of the float
+    oracle (-O3, AVX2/FMA, OpenMP), which only bench.py's cpu_baseline uses."""
+    if precision not in ("f32", "f64", "bench_f32"):
+        raise ValueError(f"precision must be f32, f64 or bench_f32, not {precision!r}")
     if precision in _LIBS:
         return _LIBS[precision]
     path = os.path.join(_BUILD, f"liboracle_{precision}.so")
     if not os.path.exists(path):
         build()
     lib = C.CDLL(path)
-    real = C.c_float if precision == "f32" else C.c_double
+    real = C.c_double if precision == "f64" else C.c_float
+    if precision == "bench_f32":
+        lib.orc_compute_threads.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]
     vp = C.c_void_p
     lib.orc_create.restype = vp
     lib.orc_create.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
@@ -206,7 +210,7 @@ class Oracle:
                  vdw_modifier="potential-shift", rvdw_switch=0.0, lj_pme=None, ewald_rtol_lj=1e-3, rvdw=None):
         self.precision = precision
         self.lib = _lib(precision)
-        self.dt = np.float32 if precision == "f32" else np.float64
+        self.dt = np.float64 if precision == "f64" else np.float32
         self.coulomb = coulomb
         rc = rlist if rcoulomb is None else rcoulomb
         self.h = self.lib.orc_create(1 if coulomb == "ewald" else 0, 1 if ewald_exact else 0, rlist, rc,
@@ -316,6 +320,18 @@ class Oracle:
         out = np.zeros(max(n, 1), np.int64)
         self.lib.orc_list_pairs(self.h, r2, _p(out), n)
         return np.sort(out[:n])
+
+    def compute_threads(self, x, nthreads: int, energy=True):
+        """The CPU baseline (precision "bench_f32" only): the kernel over
+        the list with nthreads OpenMP threads and per-thread outputs reduced
+        afterwards.  Returns dict(f, Vvdw, Vc)."""
+        if self.precision != "bench_f32":
+            raise ValueError("compute_threads needs the bench_f32 build")
+        x = np.ascontiguousarray(x, self.dt).reshape(-1)
+        f = np.zeros_like(x)
+        e = np.zeros(2)
+        self.lib.orc_compute_threads(self.h, _p(x), _p(f), 1 if energy else 0, int(nthreads), _p(e))
+        return dict(f=f.reshape(-1, 3), Vvdw=float(e[0]), Vc=float(e[1]))
 
     def compute(self, x, energy=True):
         """GmxForceCalculator::compute plus energies, shift forces and virial.
