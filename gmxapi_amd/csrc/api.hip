@@ -636,14 +636,16 @@ int nbx_create(const nbx_options* options, nbx_context** context)
         ic.cpot_rep12 = static_cast<float>(static_cast<double>(ic.cpot_rep) / 12.0);
         const float epsr = static_cast<float>(o.epsilon_r);
         ic.epsfac        = epsr != 0 ? static_cast<float>(one4PiEps0() / epsr) : 0.0f;
-        if (o.coulomb_type == NBX_COULOMB_REACTION_FIELD)
+        if (o.coulomb_type != NBX_COULOMB_CUTOFF)
         {
-            // calc_rffac (mdlib/rf_util.cpp:54-66)
+            // calc_rffac (mdlib/rf_util.cpp:54-66), which nblib calls for
+            // every Coulomb type (gmxsetup.cpp:266-270); only the
+            // reaction-field kernels read k_rf and c_rf
             const float epsrf = static_cast<float>(o.epsilon_rf), Rc = c->rcoulomb;
             ic.k_rf = (epsrf == 0) ? 1 / (2 * Rc * Rc * Rc) : (epsrf - epsr) / (2 * epsrf + epsr) / (Rc * Rc * Rc);
             ic.c_rf = 1 / Rc + ic.k_rf * Rc * Rc;
         }
-        else if (o.coulomb_type == NBX_COULOMB_CUTOFF)
+        else
         {
             // plain cut-off with the potential shift (forcerec.cpp:902-915)
             ic.k_rf = 0;
@@ -690,7 +692,8 @@ int nbx_create(const nbx_options* options, nbx_context** context)
         else
         {
             c->elec      = kElecRF;
-            ic.tab_scale = 1.0f;
+            // no correction table: nblib leaves its scale unset (zero)
+            ic.tab_scale = 0.0f;
             ic.tab_n     = 0;
             ic.tab_rs_max = 0.0f;
             ic.self_v0   = ic.c_rf;

@@ -583,7 +583,7 @@ __global__ __launch_bounds__(64) void countPairsKernel(const int4* __restrict__ 
                     }
                     const float4 xj = xq[cj * kClusterSize + j];
                     const float  dx = __fsub_rn(xs, xj.x), dy = __fsub_rn(ys, xj.y), dz = __fsub_rn(zs, xj.z);
-                    const float  d2 = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+                    const float  d2 = canonicalD2(dx, dy, dz);
                     n += (d2 < r2) ? 1u : 0u;
                 }
             }
@@ -633,7 +633,7 @@ __global__ void expandPairsKernel(const int4* __restrict__ ientry, int nci, cons
                 }
                 const float4 xj = xq[cj * kClusterSize + j];
                 const float  dx = __fsub_rn(xs, xj.x), dy = __fsub_rn(ys, xj.y), dz = __fsub_rn(zs, xj.z);
-                const float  d2 = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+                const float  d2 = canonicalD2(dx, dy, dz);
                 if (d2 < r2)
                 {
                     const unsigned long long n = atomicAdd(counter, 1ull);

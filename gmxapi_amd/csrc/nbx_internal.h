@@ -38,6 +38,21 @@ constexpr int kMaxColumnAtoms = 65536;
 // wavefront's LDS j-force window; a group's j-entries are sorted in LDS by
 // the group pass (at most kGroupMaxJ)
 constexpr int kGroupSizeMax = 16;
+
+// The canonical squared distance fl(fl(fl(dx*dx) + fl(dy*dy)) + fl(dz*dz)),
+// the rounding sequence that decides which pairs are within rlist.  hipcc
+// contracts a multiply feeding an add into one fused multiply-add by default
+// (and __fmul_rn/__fadd_rn are plain operators in its headers), which rounds
+// dx*dx + dy*dy once instead of three times and moves pairs at d^2 == rlist^2
+// across the cut; contraction is switched off for this expression.
+__device__ __forceinline__ float canonicalD2(float dx, float dy, float dz)
+{
+#pragma clang fp contract(off)
+    const float a = dx * dx;
+    const float b = dy * dy;
+    const float c = dz * dz;
+    return (a + b) + c;
+}
 // candidate group sizes: kGroupSizeMax, kGroupSizeMax / 2, ..., 1
 constexpr int kGroupSizeCount = 5;
 constexpr int kGroupMaxJ    = 4096;

@@ -25,11 +25,6 @@ namespace
 
 constexpr int kPruneWaves = 4;
 
-__device__ __forceinline__ float pruneD2(float dx, float dy, float dz)
-{
-    return __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
-}
-
 template<bool kFill>
 __global__ __launch_bounds__(64 * kPruneWaves) void pruneKernel(const PruneArgs a)
 {
@@ -72,7 +67,7 @@ __global__ __launch_bounds__(64 * kPruneWaves) void pruneKernel(const PruneArgs 
 #pragma unroll
                     for (int i = 0; i < kClusterSize; i++)
                     {
-                        keep = keep || pruneD2(__fsub_rn(xs[i], p.x), __fsub_rn(ys[i], p.y), __fsub_rn(zs[i], p.z)) < a.rlist2;
+                        keep = keep || canonicalD2(__fsub_rn(xs[i], p.x), __fsub_rn(ys[i], p.y), __fsub_rn(zs[i], p.z)) < a.rlist2;
                     }
                 }
             }
@@ -165,7 +160,7 @@ __global__ __launch_bounds__(64 * kPruneWaves) void layoutPruneEvalKernel(const 
                     for (int i = 0; i < kClusterSize; i++)
                     {
                         keep = keep
-                               || pruneD2(__fsub_rn(xs[i], p.x), __fsub_rn(ys[i], p.y), __fsub_rn(zs[i], p.z))
+                               || canonicalD2(__fsub_rn(xs[i], p.x), __fsub_rn(ys[i], p.y), __fsub_rn(zs[i], p.z))
                                           < a.rlist2;
                     }
                 }

@@ -40,12 +40,6 @@ __device__ __forceinline__ int xyz2is(int x, int y, int z)
     return 5 * (3 * (z + 1) + y + 1) + x + 2;
 }
 
-// Canonical squared distance: fl(fl(fl(dx*dx) + fl(dy*dy)) + fl(dz*dz)).
-__device__ __forceinline__ float canonD2(float dx, float dy, float dz)
-{
-    return __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
-}
-
 // clusterBoundingBoxDistance2 (pairlist.cpp:351-372) with the same rounding order.
 __device__ __forceinline__ float bbDist2(float4 ilo, float4 ihi, float4 jlo, float4 jhi)
 {
@@ -59,7 +53,7 @@ __device__ __forceinline__ float bbDist2(float4 ilo, float4 ihi, float4 jlo, flo
         const float dh = __fsub_rn(jl[k], ih[k]);
         d[k]           = fmaxf(fmaxf(dl, dh), 0.0f);
     }
-    return canonD2(d[0], d[1], d[2]);
+    return canonicalD2(d[0], d[1], d[2]);
 }
 
 __device__ __forceinline__ int wavePrefixExclusive(int v, int lane, int* total)
@@ -256,7 +250,7 @@ __global__ __launch_bounds__(kWave* kSearchWaves) void searchKernel(SearchArgs a
 #pragma unroll
                                         for (int i = 0; i < kClusterSize; i++)
                                         {
-                                            const float d2 = canonD2(__fsub_rn(xs[i], xj.x), __fsub_rn(ys[i], xj.y),
+                                            const float d2 = canonicalD2(__fsub_rn(xs[i], xj.x), __fsub_rn(ys[i], xj.y),
                                                                      __fsub_rn(zs[i], xj.z));
                                             hit = hit || (iIndexReal[i] >= 0 && d2 < rlist2);
                                         }

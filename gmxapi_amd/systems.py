@@ -265,6 +265,37 @@ def small_water_box(n_molecules: int = 216, seed: int = WATER_SEED + 7, box_leng
     return water_box(n_molecules, L, seed=seed)
 
 
+# A displacement (dx, dy, 0) whose float32 squared length rounds to exactly
+# 1.25^2 when each product and sum is rounded (fl(fl(dx^2) + fl(dy^2)) =
+# 1.5625) but to just below it when either square is fused into the sum
+# (fma(dx, dx, fl(dy^2)) and fma(dy, dy, fl(dx^2)) both give 1.5624999).
+BOUNDARY_DX = 0.6729803085327148
+BOUNDARY_DY = 1.0533742904663086
+
+
+def boundary_pair_system(seed: int = 17) -> System:
+    """A pair at exactly rlist = 1.25 nm in float32 among 124 lattice atoms
+    in a 5 nm box (rc 1.2): the pair is outside the list by the rounding of
+    the reference's distance test, and inside it if the multiply-adds are
+    fused.  Atoms 0 and 1 are the pair; both coordinates and their
+    difference are exact in float32."""
+    rng = np.random.default_rng(seed)
+    L = 5.0
+    g = np.arange(5) + 0.5
+    lat = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    lat = lat[np.linalg.norm(lat - 2.5, axis=1) > 0.6]  # keep the middle free
+    lat = lat + rng.uniform(-0.05, 0.05, lat.shape)
+    pair = np.array([[2.0, 2.0, 2.0], [2.0 + BOUNDARY_DX, 2.0 + BOUNDARY_DY, 2.0]])
+    x = np.concatenate([pair, lat])
+    n = len(x)
+    types = np.zeros(n, np.int32)
+    charges = np.zeros(n)
+    c6 = np.array([4 * EPSILON_OW * SIGMA_OW ** 6])
+    c12 = np.array([4 * EPSILON_OW * SIGMA_OW ** 12])
+    off, idx = _csr([[a] for a in range(n)], n)
+    return System("boundary_pair", np.full(3, L), x, types, charges, c6, c12, off, idx, 1.2, 1.25)
+
+
 def ensemble(n_replicas: int = 64, seed: int = WATER_SEED + 3, min_molecules: int = 1000,
              max_molecules: int = 16000, density: float = 33.4) -> list[System]:
     """The batched-ensemble workload: water boxes built like the 3,000-water

@@ -58,6 +58,23 @@ def test_pair_set_is_bit_exact(name, s):
     np.testing.assert_array_equal(gpu, O.brute_pairs(s.x, s.box, s.rlist, "f32"))
 
 
+def test_pair_at_exactly_rlist():
+    # a pair whose float32 squared distance rounds to exactly rlist^2 (outside)
+    # and to just below it if a product is fused into the sum: the search, the
+    # pair expansion and the prune must all round as the reference's test does
+    s = S.boundary_pair_system()
+    print(f"[pairs] boundary pair, {s.natoms} atoms", flush=True)
+    ctx = nbx.context_for(s)
+    ref = oracle_for(s, "f32").list_pairs(s.rlist)
+    gpu = ctx.list_pairs(s.rlist)
+    assert (0 << 32 | 1) not in set(gpu.tolist())
+    np.testing.assert_array_equal(gpu, ref)
+    r_in = float(np.nextafter(np.float32(1.25), np.float32(2)))
+    assert (0 << 32 | 1) in set(ctx.list_pairs(r_in).tolist())
+    ctx.prune_pairlist(1.25)
+    np.testing.assert_array_equal(ctx.list_pairs(s.rlist), ref)
+
+
 def test_pair_set_config0(water3000):
     print("[pairs] config0 9000 atoms", flush=True)
     s = water3000

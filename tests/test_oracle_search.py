@@ -119,3 +119,33 @@ def test_ensemble_generator():
     for s in reps[:4]:
         assert abs(s.natoms / 3 / s.box[0] ** 3 - 33.4) < 1e-6 * 33.4
         assert s.box[0] > 2 * s.rlist
+
+
+def test_pair_at_exactly_rlist_is_outside():
+    # the float32 test d^2 < rlist^2 with every product and sum rounded puts
+    # this pair exactly on rlist^2, outside the list; fusing either square
+    # into the sum (a multiply-add) would round it to just below and list it
+    from fractions import Fraction as Fr
+
+    s = S.boundary_pair_system()
+    dx, dy = np.float32(S.BOUNDARY_DX), np.float32(S.BOUNDARY_DY)
+    assert float(np.float32(s.x[1, 0]) - np.float32(s.x[0, 0])) == float(dx)
+    assert float(np.float32(s.x[1, 1]) - np.float32(s.x[0, 1])) == float(dy)
+    r2 = np.float32(np.float32(s.rlist) * np.float32(s.rlist))
+    assert np.float32(np.float32(dx * dx) + np.float32(dy * dy)) == r2
+
+    def fused(u, v):  # round(u*u + fl(v*v)) to float32, exactly
+        e = Fr(float(u)) ** 2 + Fr(float(np.float32(v * v)))
+        f = np.float32(float(e))
+        return min((f, np.nextafter(f, np.float32(2)), np.nextafter(f, np.float32(0))),
+                   key=lambda c: abs(Fr(float(c)) - e))
+
+    assert fused(dx, dy) < r2 and fused(dy, dx) < r2
+    key = (0 << 32) | 1
+    for prec in ("f32",):
+        lp = oracle_for(s, prec).list_pairs(s.rlist)
+        bp = O.brute_pairs(s.x, s.box, s.rlist, prec)
+        np.testing.assert_array_equal(lp, bp)
+        assert key not in set(lp.tolist())
+        # just inside: listed
+        assert key in set(oracle_for(s, prec).list_pairs(np.nextafter(np.float32(1.25), np.float32(2))).tolist())
