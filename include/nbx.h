@@ -281,6 +281,18 @@ int nbx_get_pairlist(nbx_context* context, int* i_entries, uint32_t* j_entries);
  * negative error); writes at most cap. */
 int64_t nbx_list_pairs(nbx_context* context, double r, int64_t* out, int64_t cap);
 
+/* No counterpart in the reference: nbx_list_pairs for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops that
+ * forbid hipMalloc).  The same pair set, written in list order (i-entry,
+ * j-entry, i-atom, j-atom) to `out`, a DEVICE array of cap keys; returns the
+ * number of pairs (writes at most cap) or a negative error.  Nothing is
+ * allocated: per-i-entry counts and offsets live in a static device array of
+ * 2 MiB that each GPU it runs on keeps until the process ends, so lists of
+ * at most 262,144 i-entries are taken (larger ones are an input error).
+ * Calls on one GPU share that array: calls that overlap on one GPU race.
+ * The host waits for the result (one 4-byte read). */
+int64_t nbx_list_pairs_noalloc(nbx_context* context, double r, int64_t* out, int64_t cap);
+
 /* The kernel's work groups (no counterpart in the reference, whose CPU
  * kernels need none): info[5] = {group size G, number of groups, number of
  * chunks, LDS window slots, work-list length}; groups[4 * num_groups] =
