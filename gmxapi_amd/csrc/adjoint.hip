@@ -83,9 +83,9 @@ __device__ __forceinline__ void adjointPairI(const IAtoms& I, const IGrad& Gi, c
         skip = !(rsq < ic.rcoulomb2);
     }
     rsq           = fmaxf(rsq, kMinRsq);
+    // the hardware estimate (within one ulp, nearly unbiased; see the pair
+    // kernel)
     float rinv    = __builtin_amdgcn_rsqf(rsq);
-    const float h = 0.5f * rsq;
-    rinv          = rinv * fmaf(-h * rinv, rinv, 1.5f); // one Newton-Raphson step
     rinv          = skip ? 0.0f : rinv;
     const float smask  = skip ? 0.0f : 1.0f;
     const float rinvsq = rinv * rinv;

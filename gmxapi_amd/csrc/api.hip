@@ -670,7 +670,6 @@ int nbx_create(const nbx_options* options, nbx_context** context)
             ic.sh_ewald  = o.ewald_potential_shift ? std::erfc(ic.beta * c->rcoulomb) / c->rcoulomb : 0.0f;
             ic.tab_scale = ewaldTableScale(ic.beta, c->rcoulomb);
             ic.tab_n     = static_cast<int>(c->rcoulomb * ic.tab_scale) + 2;
-            ic.tab_rs_max = std::nextafter(static_cast<float>(ic.tab_n - 1), 0.0f);
             std::vector<float>  fdv0  = ewaldTableFDV0<float>(ic.tab_n, ic.tab_scale, ic.beta);
             std::vector<double> fdv0d = ewaldTableFDV0<double>(ic.tab_n, ic.tab_scale, ic.beta);
             ic.self_v0   = fdv0[2];
@@ -695,7 +694,6 @@ int nbx_create(const nbx_options* options, nbx_context** context)
             // no correction table: nblib leaves its scale unset (zero)
             ic.tab_scale = 0.0f;
             ic.tab_n     = 0;
-            ic.tab_rs_max = 0.0f;
             ic.self_v0   = ic.c_rf;
             c->selfCoef  = 0.5 * static_cast<double>(ic.epsfac) * static_cast<double>(ic.c_rf);
         }
@@ -1376,6 +1374,7 @@ static void enqueueStep(nbx_context* c, unsigned flags, const float* xd, float* 
     a.vdwModifier   = c->opt.vdw_modifier;
     a.ljPme         = c->opt.vdw_type == NBX_VDW_PME ? 1 : 0;
     a.twinCut       = c->twinCut ? 1 : 0;
+    a.windowSpill   = c->lay.spill ? 1 : 0;
     a.c6grid        = c->c6grid.p;
     a.ntype         = c->ntype;
     a.tabFDV0       = c->tabFDV0.p;

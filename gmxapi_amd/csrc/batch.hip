@@ -35,6 +35,7 @@ struct nbx_batch
 
     bool built = false;
     int  totAtoms = 0, totClusters = 0, ngroups = 0, nchunk = 0, nwork = 0, windowSlots = 0, numBlocks = 0;
+    bool spill = false; // some member's work entries take global atomics
     int  groupSize = 0, blocksPerCU = 0, chunkWeight = kChunkWeight;
     long long ncj = 0;
 
@@ -156,6 +157,7 @@ void buildBatch(nbx_batch* b, int G, int blocksPerCU, int chunkWeight)
     std::vector<int4>   hChunks, hGroups, hInfo;
     std::vector<float4> hShift, hChunkShift;
     int                 clusterOff = 0, atomOffset = 0, maxSlots = 0;
+    bool                anySpill = false;
     long long           ncj        = 0;
     int                 totClusters = 0;
     for (nbx_context* c : b->sys)
@@ -235,6 +237,7 @@ void buildBatch(nbx_batch* b, int G, int blocksPerCU, int chunkWeight)
             hExIdx.push_back(v + atomOffset);
         }
         maxSlots = std::max(maxSlots, c->lay.windowSlots);
+        anySpill = anySpill || c->lay.spill;
         ncj += c->ncj;
         clusterOff += nc;
         atomOffset += na;
@@ -269,6 +272,7 @@ void buildBatch(nbx_batch* b, int G, int blocksPerCU, int chunkWeight)
     b->nchunk      = static_cast<int>(hChunks.size());
     b->nwork       = static_cast<int>(hWork.size());
     b->windowSlots = maxSlots;
+    b->spill       = anySpill;
     b->ncj         = ncj;
     b->groupSize   = G > 0 ? G : b->sys[0]->lay.groupSize;
     buildOccurrences(b->stream, b->gather, b->ngroups, b->groups.p, b->groupInfo.p, b->unionList.p, totClusters);
@@ -302,6 +306,7 @@ void enqueueCompute(nbx_batch* b, const float* x, float* f, unsigned flags)
     a.vdwModifier  = c0->opt.vdw_modifier;
     a.ljPme        = c0->opt.vdw_type == NBX_VDW_PME ? 1 : 0;
     a.twinCut      = c0->twinCut ? 1 : 0;
+    a.windowSpill  = b->spill ? 1 : 0;
     a.c6grid       = c0->c6grid.p;
     a.ntype        = c0->ntype;
     a.tabFDV0      = c0->tabFDV0.p;

@@ -130,7 +130,6 @@ struct InteractionConst
     float cpot_rep12; // cpot_rep / 12, rounded once
     float tab_scale;
     int   tab_n;
-    float tab_rs_max; // largest float below tab_n - 1: clamps the scaled distance to the last interval
     float self_v0; // V(0) of the Ewald correction (or c_rf for RF)
     float beta2;   // beta^2 and beta^3 for the analytical correction
     float beta3;
@@ -329,6 +328,7 @@ struct KernelArgs
     int           vdwModifier; // host: NBX_VDW_* modifier
     int           ljPme;       // host: LJ-PME (c6grid is read)
     int           twinCut;     // host: rvdw < rcoulomb (launch the twin-cut-off variant)
+    int           windowSpill; // some work entries take global atomics (no window slot)
     const float*  c6grid;      // LJ-PME: the grid's 6 C6 per type pair
     int           ntype;  // including the padding type
     const float4* tabFDV0;
