@@ -328,7 +328,15 @@ void buildWorkGroups(nbx_context* c, WorkLayout& L, bool adjoint)
     // global atomics (long cut-offs, many types), the last pass takes the
     // smallest (or the chosen) size with whatever spill its window leaves.
     const int forced = adjoint ? 0 : c->opt.kernel_group_size;
-    for (int k = 0; k <= kGroupSizeCount; k++)
+    // a layout rebuilt for a new list starts from the size its last build
+    // chose: the sizes above it did not fit then and, for a system that
+    // moved by a few steps, do not fit now (saves a count pass per search)
+    int first = 0;
+    while (forced <= 0 && L.groupSize > 0 && first < kGroupSizeCount - 1 && (kGroupSizeMax >> first) > L.groupSize)
+    {
+        first++;
+    }
+    for (int k = first; k <= kGroupSizeCount; k++)
     {
         const bool fallback = (k == kGroupSizeCount);
         const int  G        = fallback ? (forced > 0 ? forced : 1) : kGroupSizeMax >> k;

@@ -46,6 +46,79 @@ __global__ void gridColumnsKernel(GridDims g, const float* __restrict__ x, int* 
     atomRank[a]   = atomicAdd(&cxy_na[col], 1);
 }
 
+// The same with the column counts aggregated per workgroup in LDS: each
+// workgroup bins kColumnsAtomsPerBlock consecutive atoms with LDS atomics
+// (an atom's rank within the workgroup), then takes one global atomic per
+// column it touched for its base.  Ranks only place an atom in its column
+// before the per-column sort by (z, index), so their order does not affect
+// the grid.  For up to kColumnsLdsMax columns.
+constexpr int kColumnsThreads       = 256;
+constexpr int kColumnsAtomsPerBlock = 2048;
+constexpr int kColumnsLdsMax        = 12288;
+
+__global__ __launch_bounds__(kColumnsThreads) void gridColumnsLdsKernel(GridDims g, const float* __restrict__ x,
+                                                                        int* __restrict__ atomCol,
+                                                                        int* __restrict__ atomRank,
+                                                                        int* __restrict__ cxy_na)
+{
+    __shared__ int hist[kColumnsLdsMax];
+    for (int c = threadIdx.x; c < g.ncol; c += kColumnsThreads)
+    {
+        hist[c] = 0;
+    }
+    __syncthreads();
+    constexpr int kPer = kColumnsAtomsPerBlock / kColumnsThreads;
+    const int     a0   = blockIdx.x * kColumnsAtomsPerBlock + threadIdx.x;
+    int           col[kPer], rank[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; k++)
+    {
+        const int a = a0 + k * kColumnsThreads;
+        col[k]      = -2; // no atom
+        rank[k]     = 0;
+        if (a < g.natoms)
+        {
+            int cx = static_cast<int>(__fmul_rn(x[3 * a + 0], g.invCellSize[0]));
+            int cy = static_cast<int>(__fmul_rn(x[3 * a + 1], g.invCellSize[1]));
+            cx     = min(cx, g.ncx - 1);
+            cy     = min(cy, g.ncy - 1);
+            if (cx < 0 || cy < 0)
+            {
+                // below the grid: flagged in the extra counter slot, the host raises
+                atomicAdd(&cxy_na[g.ncol], 1);
+                col[k] = -1;
+            }
+            else
+            {
+                col[k]  = cx * g.ncy + cy;
+                rank[k] = atomicAdd(&hist[col[k]], 1);
+            }
+        }
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < g.ncol; c += kColumnsThreads)
+    {
+        const int n = hist[c];
+        hist[c]     = n > 0 ? atomicAdd(&cxy_na[c], n) : 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kPer; k++)
+    {
+        const int a = a0 + k * kColumnsThreads;
+        if (col[k] >= 0)
+        {
+            atomCol[a]  = col[k];
+            atomRank[a] = hist[col[k]] + rank[k];
+        }
+        else if (col[k] == -1)
+        {
+            atomCol[a]  = -1;
+            atomRank[a] = 0;
+        }
+    }
+}
+
 // Exclusive scan of ceil(na/4) over the columns in one workgroup of 1024.
 __global__ __launch_bounds__(1024) void gridScanKernel(int ncol, const int* __restrict__ cxy_na,
                                                        int* __restrict__ cxy_ind, int* __restrict__ total)
@@ -194,7 +267,15 @@ void launchGridColumns(hipStream_t s, const GridDims& g, const float* x, int* at
     {
         return;
     }
-    gridColumnsKernel<<<(g.natoms + 255) / 256, 256, 0, s>>>(g, x, atomCol, atomRank, cxy_na);
+    if (g.ncol <= kColumnsLdsMax)
+    {
+        gridColumnsLdsKernel<<<(g.natoms + kColumnsAtomsPerBlock - 1) / kColumnsAtomsPerBlock, kColumnsThreads, 0, s>>>(
+                g, x, atomCol, atomRank, cxy_na);
+    }
+    else
+    {
+        gridColumnsKernel<<<(g.natoms + 255) / 256, 256, 0, s>>>(g, x, atomCol, atomRank, cxy_na);
+    }
 }
 
 void launchGridScan(hipStream_t s, int ncol, const int* cxy_na, int* cxy_ind, int* ncluster_out)

@@ -62,6 +62,56 @@ __device__ void bitonicSort(int* keys, int n)
     }
 }
 
+// Exclusive scan of v[0, n) in LDS by the whole workgroup (kGroupThreads
+// threads, every one calling it): each thread scans a contiguous run, the
+// run totals are scanned across the lanes with shuffles and across the
+// wavefronts through waveTot[kGroupThreads / 64].  Returns the total.
+__device__ int blockExclusiveScan(int* v, int n, int* waveTot)
+{
+    const int per  = (n + kGroupThreads - 1) / kGroupThreads;
+    const int t0   = min(static_cast<int>(threadIdx.x) * per, n);
+    const int t1   = min(t0 + per, n);
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    int       sum  = 0;
+    for (int t = t0; t < t1; t++)
+    {
+        sum += v[t];
+    }
+    int incl = sum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1)
+    {
+        const int x = __shfl_up(incl, off, 64);
+        if (lane >= off)
+        {
+            incl += x;
+        }
+    }
+    if (lane == 63)
+    {
+        waveTot[wave] = incl;
+    }
+    __syncthreads();
+    int base = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kGroupThreads / 64; w++)
+    {
+        const int x = waveTot[w];
+        base += (w < wave) ? x : 0;
+        total += x;
+    }
+    int run = base + incl - sum;
+    for (int t = t0; t < t1; t++)
+    {
+        const int x = v[t];
+        v[t]        = run;
+        run += x;
+    }
+    __syncthreads(); // v is read and waveTot rewritten after the call
+    return total;
+}
+
 template<bool kFill>
 __global__ __launch_bounds__(kGroupThreads) void buildGroupsKernel(GroupArgs a)
 {
@@ -72,6 +122,7 @@ __global__ __launch_bounds__(kGroupThreads) void buildGroupsKernel(GroupArgs a)
     __shared__ int sWork;
     __shared__ int sChunkBase[kGroupThreads];
     __shared__ int sWorkBase[kGroupThreads];
+    __shared__ int sWaveTot[kGroupThreads / 64];
 
     const int g  = blockIdx.x;
     const int c0 = g * a.groupSize;
@@ -104,33 +155,30 @@ __global__ __launch_bounds__(kGroupThreads) void buildGroupsKernel(GroupArgs a)
         sFlag[t]    = (k != INT_MAX && (t == 0 || sKeys[t - 1] != k)) ? 1 : 0;
     }
     __syncthreads();
-    // exclusive scan of the flags (ranks of the unique keys), and the chunk
-    // and padded work offsets of the group's i-entries, all serial: a group
-    // holds at most kGroupMaxJ j-entries and a few dozen i-entries
-    if (threadIdx.x == 0)
+    // the group's i-entries (at most kGroupThreads; more is flagged as
+    // overflow below): their chunk counts and padded work lengths
     {
-        int run = 0;
-        for (int t = 0; t < p; t++)
-        {
-            const int f = sFlag[t];
-            sFlag[t]    = run;
-            run += f;
-        }
-        sUnion = run;
-    }
-    else if (threadIdx.x == 64)
-    {
-        int run = 0, work = 0;
-        for (int e = 0; e < ni && e < kGroupThreads; e++)
+        const int e = threadIdx.x;
+        int       nc = 0, nw = 0;
+        if (e < ni)
         {
             const int4 ie = a.ientry[ib + e];
-            sChunkBase[e] = run;
-            sWorkBase[e]  = work;
-            run += (ie.w - ie.z + a.chunkEntries - 1) / a.chunkEntries;
-            work += (ie.w - ie.z + kTileJ - 1) / kTileJ * kTileJ;
+            nc            = (ie.w - ie.z + a.chunkEntries - 1) / a.chunkEntries;
+            nw            = (ie.w - ie.z + kTileJ - 1) / kTileJ * kTileJ;
         }
-        sChunks = run;
-        sWork   = work;
+        sChunkBase[e] = nc;
+        sWorkBase[e]  = nw;
+    }
+    // exclusive scans: ranks of the unique keys, and the chunk and padded
+    // work offsets of the i-entries
+    const int unionTotal = blockExclusiveScan(sFlag, p, sWaveTot);
+    const int chunkTotal = blockExclusiveScan(sChunkBase, kGroupThreads, sWaveTot);
+    const int workTotal  = blockExclusiveScan(sWorkBase, kGroupThreads, sWaveTot);
+    if (threadIdx.x == 0)
+    {
+        sUnion  = unionTotal;
+        sChunks = chunkTotal;
+        sWork   = workTotal;
     }
     __syncthreads();
     const int U      = sUnion;
@@ -264,19 +312,32 @@ __global__ __launch_bounds__(kGroupThreads) void buildGroupsKernel(GroupArgs a)
 // max over groups of (offJ[c0 + G] - offJ[c0]) for the candidate sizes
 __global__ void groupSpanKernel(int ncluster, const int* __restrict__ offJ, int* __restrict__ maxSpan)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncluster)
+    int m[kGroupSizeCount] = {};
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncluster; c += gridDim.x * blockDim.x)
     {
-        return;
+#pragma unroll
+        for (int k = 0; k < kGroupSizeCount; k++)
+        {
+            const int G = kGroupSizeMax >> k;
+            if (c % G == 0)
+            {
+                m[k] = max(m[k], offJ[min(c + G, ncluster)] - offJ[c]);
+            }
+        }
     }
+    // one atomic per wavefront and size
 #pragma unroll
     for (int k = 0; k < kGroupSizeCount; k++)
     {
-        const int G = kGroupSizeMax >> k;
-        if (c % G == 0)
+        int v = m[k];
+#pragma unroll
+        for (int o = 1; o < 64; o *= 2)
         {
-            const int e = min(c + G, ncluster);
-            atomicMax(&maxSpan[k], offJ[e] - offJ[c]);
+            v = max(v, __shfl_xor(v, o, 64));
+        }
+        if ((threadIdx.x & 63) == 0 && v > 0)
+        {
+            atomicMax(&maxSpan[k], v);
         }
     }
 }
@@ -720,7 +781,7 @@ void launchGroupSpans(hipStream_t s, int ncluster, const int* offJ, int* maxSpan
 {
     if (ncluster > 0)
     {
-        groupSpanKernel<<<(ncluster + 255) / 256, 256, 0, s>>>(ncluster, offJ, maxSpan);
+        groupSpanKernel<<<std::min(64, (ncluster + 255) / 256), 256, 0, s>>>(ncluster, offJ, maxSpan);
     }
 }
 

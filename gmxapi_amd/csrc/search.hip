@@ -24,6 +24,8 @@
 // cluster each, hits are compacted in candidate order with a ballot, so the
 // output is deterministic.  The same kernel runs twice: a count pass (per
 // i-cluster j and i counts, scanned on the device) and a fill pass.
+#include <cstdint>
+
 #include "nbx_internal.h"
 
 namespace nbx
@@ -97,10 +99,41 @@ __global__ __launch_bounds__(kWave* kSearchWaves) void searchKernel(SearchArgs a
     const float rlist2 = a.rlist2;
 
     int jpos = 0, ipos = 0;
+    // fill pass: the i-cluster's topology exclusions (other than an atom
+    // with itself), one per lane: the excluded atom's slot and the i-atom's
+    // index in the cluster; numExcl is wave-uniform.  Every hit's mask is
+    // then built from these with shuffles instead of walking the exclusion
+    // lists per candidate.  More than a wavefront's worth (not seen for
+    // bonded exclusions) falls back to the walk.
+    int numExcl = 0, exSlot = -1, exI = 0;
     if (kFill)
     {
         jpos = a.offJ[ci];
         ipos = a.offI[ci];
+        int cnt[kClusterSize], beg[kClusterSize];
+#pragma unroll
+        for (int i = 0; i < kClusterSize; i++)
+        {
+            const int iAtom = iIndexReal[i];
+            beg[i]          = iAtom >= 0 ? a.exclOff[iAtom] : 0;
+            cnt[i]          = iAtom >= 0 ? a.exclOff[iAtom + 1] - beg[i] : 0;
+            numExcl += cnt[i];
+        }
+        if (numExcl <= kWave)
+        {
+            int base = 0;
+#pragma unroll
+            for (int i = 0; i < kClusterSize; i++)
+            {
+                if (lane >= base && lane < base + cnt[i])
+                {
+                    const int jAtom = a.exclIdx[beg[i] + lane - base];
+                    exSlot          = jAtom == iIndexReal[i] ? -1 : a.cells[jAtom];
+                    exI             = i;
+                }
+                base += cnt[i];
+            }
+        }
     }
     int countJ = 0, countI = 0;
 
@@ -260,7 +293,30 @@ __global__ __launch_bounds__(kWave* kSearchWaves) void searchKernel(SearchArgs a
                         }
                         const unsigned long long ballot = __ballot(hit);
                         const int before = __popcll(ballot & ((1ull << lane) - 1ull));
-                        if (kFill && hit)
+                        // exclusion bits 4 i + j of the exclusions in this
+                        // j-cluster (setExclusionsForIEntry,
+                        // pairlist.cpp:1367-1435), every lane taking part in
+                        // the shuffles; centrally a pair is excluded only in
+                        // the entry of its lower slot
+                        unsigned exclMask = 0u;
+                        if (kFill && ballot != 0 && numExcl <= kWave)
+                        {
+                            for (int k = 0; k < numExcl; k++)
+                            {
+                                const int js = __shfl(exSlot, k, kWave);
+                                const int ii = __shfl(exI, k, kWave);
+                                if (js >= 0 && (js >> 2) == cj && !(central && js <= ci * kClusterSize + ii))
+                                {
+                                    exclMask |= 1u << (ii * kClusterSize + (js & 3));
+                                }
+                            }
+                        }
+                        if (kFill && hit && numExcl <= kWave)
+                        {
+                            const unsigned mask = ((central && cj == ci) ? kMaskDiag : kMaskAll) & ~exclMask;
+                            a.jentry[entryPos + entryJ + before] = make_uint2(static_cast<unsigned>(cj), mask);
+                        }
+                        else if (kFill && hit)
                         {
                             unsigned mask = (central && cj == ci) ? kMaskDiag : kMaskAll;
                             for (int i = 0; i < kClusterSize; i++)
@@ -313,32 +369,66 @@ __global__ __launch_bounds__(kWave* kSearchWaves) void searchKernel(SearchArgs a
     }
 }
 
-// Exclusive scan over n ints in one workgroup; writes the total.
 // Exclusive scan of n ints by one workgroup (the lists scanned here hold
-// one entry per cluster or group, at most a few 100k): rounds of 8192
-// elements, each thread summing 8 consecutive ones, a wavefront shuffle scan
-// over the threads and a scan over the 16 wavefront totals; out[n] and
-// *total receive the sum.
+// one entry per cluster or group, at most a few 100k): rounds of 32768
+// elements, each thread summing 32 consecutive ones (int4 loads), a
+// wavefront shuffle scan over the threads and a scan over the 16 wavefront
+// totals; out[n] and *total receive the sum.
 constexpr int kScanThreads = 1024;
-constexpr int kScanPer     = 8;
+constexpr int kScanVec     = 8;              // int4 loads per thread per round
+constexpr int kScanPer     = 4 * kScanVec;   // elements per thread per round
+constexpr int kScanRound   = kScanThreads * kScanPer;
+
+// One round's elements of this thread: [first, first + kScanPer) of in,
+// zero past n; whole int4 loads when the run lies inside and in is 16-byte
+// aligned.
+__device__ __forceinline__ void scanLoad(const int* __restrict__ in, int n, int first, bool aligned, int4* v)
+{
+    if (aligned && first + kScanPer <= n)
+    {
+        const int4* p = reinterpret_cast<const int4*>(in + first);
+#pragma unroll
+        for (int k = 0; k < kScanVec; k++)
+        {
+            v[k] = p[k];
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int k = 0; k < kScanVec; k++)
+        {
+            const int e = first + 4 * k;
+            v[k]        = make_int4(e < n ? in[e] : 0, e + 1 < n ? in[e + 1] : 0, e + 2 < n ? in[e + 2] : 0,
+                                    e + 3 < n ? in[e + 3] : 0);
+        }
+    }
+}
 
 __global__ __launch_bounds__(kScanThreads) void exclusiveScanKernel(int n, const int* __restrict__ in,
                                                                    int* __restrict__ out, int* __restrict__ total)
 {
     __shared__ int sWave[kScanThreads / 64];
-    const int      lane  = threadIdx.x & 63;
-    const int      wave  = threadIdx.x >> 6;
-    int            carry = 0;
-    for (int base = 0; base < n; base += kScanThreads * kScanPer)
+    const int      lane     = threadIdx.x & 63;
+    const int      wave     = threadIdx.x >> 6;
+    const bool     alignIn  = (reinterpret_cast<uintptr_t>(in) & 15) == 0;
+    const bool     alignOut = (reinterpret_cast<uintptr_t>(out) & 15) == 0;
+    int            carry    = 0;
+    // rounds of kScanRound elements, each thread a contiguous run of
+    // kScanPer; the next round's elements are loaded before this round's
+    // scan, so the load latency overlaps the shuffles and barriers
+    int4 cur[kScanVec];
+    scanLoad(in, n, threadIdx.x * kScanPer, alignIn, cur);
+    for (int base = 0; base < n; base += kScanRound)
     {
         const int first = base + threadIdx.x * kScanPer;
-        int       v[kScanPer];
-        int       sum = 0;
+        int4      nxt[kScanVec];
+        scanLoad(in, n, first + kScanRound, alignIn, nxt);
+        int sum = 0;
 #pragma unroll
-        for (int k = 0; k < kScanPer; k++)
+        for (int k = 0; k < kScanVec; k++)
         {
-            v[k] = (first + k < n) ? in[first + k] : 0;
-            sum += v[k];
+            sum += (cur[k].x + cur[k].y) + (cur[k].z + cur[k].w);
         }
         // inclusive scan of the thread sums within the wavefront
         int incl = sum;
@@ -362,16 +452,58 @@ __global__ __launch_bounds__(kScanThreads) void exclusiveScanKernel(int n, const
             roundTotal += t;
         }
         int run = carry + waveBase + incl - sum;
+        int4 res[kScanVec];
 #pragma unroll
-        for (int k = 0; k < kScanPer; k++)
+        for (int k = 0; k < kScanVec; k++)
         {
-            if (first + k < n)
+            res[k].x = run;
+            run += cur[k].x;
+            res[k].y = run;
+            run += cur[k].y;
+            res[k].z = run;
+            run += cur[k].z;
+            res[k].w = run;
+            run += cur[k].w;
+        }
+        if (alignOut && first + kScanPer <= n)
+        {
+            int4* p = reinterpret_cast<int4*>(out + first);
+#pragma unroll
+            for (int k = 0; k < kScanVec; k++)
             {
-                out[first + k] = run;
+                p[k] = res[k];
             }
-            run += v[k];
+        }
+        else
+        {
+#pragma unroll
+            for (int k = 0; k < kScanVec; k++)
+            {
+                const int e = first + 4 * k;
+                if (e < n)
+                {
+                    out[e] = res[k].x;
+                }
+                if (e + 1 < n)
+                {
+                    out[e + 1] = res[k].y;
+                }
+                if (e + 2 < n)
+                {
+                    out[e + 2] = res[k].z;
+                }
+                if (e + 3 < n)
+                {
+                    out[e + 3] = res[k].w;
+                }
+            }
         }
         carry += roundTotal;
+#pragma unroll
+        for (int k = 0; k < kScanVec; k++)
+        {
+            cur[k] = nxt[k];
+        }
         __syncthreads(); // sWave is rewritten next round
     }
     if (threadIdx.x == 0)
