@@ -183,12 +183,15 @@ __global__ __launch_bounds__(64 * kPruneWaves) void layoutPruneEvalKernel(const 
     }
 }
 
-// One thread per wavefront of the selected part: the pruned chunks of its
-// run laid out back to back from the start of its outer region, each with
-// at least one tile, and their descriptors (tiles, masked tiles).
-__global__ void layoutPruneOffsetsKernel(const LayoutPruneArgs a)
+// One wavefront per pair-kernel wavefront of the selected part, its lanes
+// over the chunks of that wavefront's run: the pruned chunks laid out back
+// to back from the start of its outer region (a wave prefix sum of their
+// padded lengths), each with at least one tile, and their descriptors
+// (tiles, masked tiles).
+__global__ __launch_bounds__(64 * kPruneWaves) void layoutPruneOffsetsKernel(const LayoutPruneArgs a)
 {
-    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int w    = blockIdx.x * kPruneWaves + (threadIdx.x >> 6);
     if (w >= a.numWaves || w % a.numParts != a.part)
     {
         return;
@@ -198,14 +201,35 @@ __global__ void layoutPruneOffsetsKernel(const LayoutPruneArgs a)
     {
         return;
     }
-    int start = a.chunksOuter[k0].z;
-    for (int k = k0; k < k1; k++)
+    const int start0 = a.chunksOuter[k0].z;
+    int       base   = 0;
+    for (int kb = k0; kb < k1; kb += 64)
     {
-        const int4 cd      = a.chunksOuter[k];
-        const int  tiles   = max(1, (a.keptCount[k] + kTileJ - 1) / kTileJ);
-        const int  special = (a.keptMasked[k] + kTileJ - 1) / kTileJ;
-        a.chunks[k]        = make_int4(cd.x, (cd.y & 0x1ff) | (special << 9), start, tiles);
-        start += tiles * kTileJ;
+        const int k     = kb + lane;
+        int4      cd    = make_int4(0, 0, 0, 0);
+        int       tiles = 0, special = 0;
+        if (k < k1)
+        {
+            cd      = a.chunksOuter[k];
+            tiles   = max(1, (a.keptCount[k] + kTileJ - 1) / kTileJ);
+            special = (a.keptMasked[k] + kTileJ - 1) / kTileJ;
+        }
+        const int len  = tiles * kTileJ;
+        int       incl = len;
+#pragma unroll
+        for (int off = 1; off < 64; off *= 2)
+        {
+            const int t = __shfl_up(incl, off, 64);
+            if (lane >= off)
+            {
+                incl += t;
+            }
+        }
+        if (k < k1)
+        {
+            a.chunks[k] = make_int4(cd.x, (cd.y & 0x1ff) | (special << 9), start0 + base + incl - len, tiles);
+        }
+        base += __shfl(incl, 63, 64);
     }
 }
 
@@ -274,7 +298,7 @@ void launchLayoutPrune(hipStream_t s, const LayoutPruneArgs& a)
     }
     const int blocks = (a.nchunk + kPruneWaves - 1) / kPruneWaves;
     layoutPruneEvalKernel<<<blocks, 64 * kPruneWaves, 0, s>>>(a);
-    layoutPruneOffsetsKernel<<<(a.numWaves + 255) / 256, 256, 0, s>>>(a);
+    layoutPruneOffsetsKernel<<<(a.numWaves + kPruneWaves - 1) / kPruneWaves, 64 * kPruneWaves, 0, s>>>(a);
     layoutPruneCompactKernel<<<blocks, 64 * kPruneWaves, 0, s>>>(a);
 }
 
