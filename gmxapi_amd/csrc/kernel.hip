@@ -297,7 +297,7 @@ struct LdsLayout
         grid  = nbfp + (ntype * ntype + 1) / 2;
         tab   = grid + (ljPme ? (ntype * ntype + 3) / 4 : 0);
         fsh   = tab + tabN;
-        ix    = fsh + (kNumShifts * 3 + 3) / 4;
+        ix    = fsh + (kWavesBlock * kNumShifts * 3 + 3) / 4;
         it    = ix + kWavesBlock * kGroupSizeMax * kClusterSize;
         win   = it + kWavesBlock * kGroupSizeMax;
         end_  = win + kWavesBlock * windowSlots * 3;
@@ -334,7 +334,9 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
     {
         sTab[t] = t > 0 ? a.tabFDV0[t - 1] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
-    for (int t = threadIdx.x; t < kNumShifts * 3; t += kThreads)
+    // one shift-force array per wavefront: each is summed in the wavefront's
+    // own (fixed) order, and the four are added in wavefront order at the end
+    for (int t = threadIdx.x; t < kWavesBlock * kNumShifts * 3; t += kThreads)
     {
         sFsh[t] = 0.0f;
     }
@@ -485,7 +487,7 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
                 const float s3 = __shfl(r, 4 * (9 + d), kWave);
                 if (lane < 3)
                 {
-                    atomicAdd(sFsh + sh * 3 + lane, (s0 + s1) + (s2 + s3));
+                    sFsh[(wave * kNumShifts + sh) * 3 + lane] += (s0 + s1) + (s2 + s3);
                 }
             }
         };
@@ -598,7 +600,13 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
         __syncthreads();
         for (int t = threadIdx.x; t < kNumShifts * 3; t += kThreads)
         {
-            a.fshiftPartial[blockIdx.x * kNumShifts * 3 + t] = sFsh[t];
+            float sum = 0.0f;
+#pragma unroll
+            for (int w = 0; w < kWavesBlock; w++)
+            {
+                sum += sFsh[w * kNumShifts * 3 + t];
+            }
+            a.fshiftPartial[blockIdx.x * kNumShifts * 3 + t] = sum;
         }
     }
 }

@@ -383,6 +383,38 @@ def test_repeated_compute_is_reproducible():
     np.testing.assert_array_equal(va, vb)
 
 
+def test_shift_forces_and_virial_are_reproducible(water3000):
+    # each wavefront sums its own shift forces and the workgroup adds the
+    # wavefronts' arrays in order: shift forces and the virial's shift term
+    # are bit-identical from call to call (the 4.48 nm box puts most
+    # clusters within rlist of a face, so many wavefronts carry shifts)
+    # The j-forces of clusters that do not fit a group's LDS window (slot
+    # 0xffff in the work list; long cut-offs relative to the box) are added
+    # with float atomics: then forces and the virial's x (x) f term agree to
+    # rounding only, while the shift forces (from the i-forces) and the
+    # energies stay bit-identical.
+    s = water3000
+    ctx = nbx.context_for(s)
+    spill = bool(np.any((ctx.work_groups()["work"][:, 1] >> 16) == 0xFFFF))
+    print(f"[reproducible] config0, window spill: {spill}", flush=True)
+    ref = None
+    for _ in range(5):
+        f = ctx.compute(s.x, energy=True, shift_forces=True, virial=True)
+        got = (f, ctx.shift_forces(), ctx.virial(), tuple(ctx.energies()))
+        if ref is None:
+            ref = got
+            assert np.abs(got[1]).max() > 0
+            continue
+        np.testing.assert_array_equal(got[1], ref[1])
+        assert got[3] == ref[3]
+        if spill:
+            np.testing.assert_allclose(got[0], ref[0], rtol=0, atol=1e-6 * np.abs(ref[0]).max())
+            np.testing.assert_allclose(got[2], ref[2], rtol=0, atol=1e-6 * np.abs(ref[2]).max())
+        else:
+            np.testing.assert_array_equal(got[0], ref[0])
+            np.testing.assert_array_equal(got[2], ref[2])
+
+
 def test_update_changes_forces():
     # nbkernelsystem.cpp:169-184: forces change after the coordinates move
     s = S.small_water_box(216, seed=11)
