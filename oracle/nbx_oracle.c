@@ -1984,6 +1984,9 @@ void orc_compute_threads(orc_sys* s, const real* x, real* f, int energy, int nth
     out_e[0] = s->Vvdw;
     out_e[1] = s->Vc;
 }
+
+/* the AVX-512 baseline kernel (the reference's SIMD scheme) */
+#    include "baseline_avx512.c"
 #endif
 
 /* add_nbat_f_to_f for one output buffer (atomdata.cpp:1181-1228), with the

@@ -42,6 +42,10 @@ def _lib(precision: str) -> C.CDLL:
     real = C.c_double if precision == "f64" else C.c_float
     if precision == "bench_f32":
         lib.orc_compute_threads.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+        lib.orc_compute_simd.restype = C.c_int
+        lib.orc_compute_simd.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+        lib.orc_baseline_simd_available.restype = C.c_int
+        lib.orc_baseline_simd_available.argtypes = [C.c_void_p]
     vp = C.c_void_p
     lib.orc_create.restype = vp
     lib.orc_create.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
@@ -331,6 +335,24 @@ class Oracle:
         f = np.zeros_like(x)
         e = np.zeros(2)
         self.lib.orc_compute_threads(self.h, _p(x), _p(f), 1 if energy else 0, int(nthreads), _p(e))
+        return dict(f=f.reshape(-1, 3), Vvdw=float(e[0]), Vc=float(e[1]))
+
+    def simd_available(self) -> bool:
+        """Whether compute_simd runs here (precision "bench_f32", an AVX-512
+        host, Ewald with potential-shifted LJ and geometric parameters)."""
+        return self.precision == "bench_f32" and bool(self.lib.orc_baseline_simd_available(self.h))
+
+    def compute_simd(self, x, nthreads: int, energy=True):
+        """The AVX-512 CPU baseline (the reference's SIMD kernel scheme with
+        the analytical Ewald correction), nthreads OpenMP threads.  Returns
+        dict(f, Vvdw, Vc), or None where it does not apply."""
+        if self.precision != "bench_f32":
+            raise ValueError("compute_simd needs the bench_f32 build")
+        x = np.ascontiguousarray(x, self.dt).reshape(-1)
+        f = np.zeros_like(x)
+        e = np.zeros(2)
+        if not self.lib.orc_compute_simd(self.h, _p(x), _p(f), 1 if energy else 0, int(nthreads), _p(e)):
+            return None
         return dict(f=f.reshape(-1, 3), Vvdw=float(e[0]), Vc=float(e[1]))
 
     def compute(self, x, energy=True):
