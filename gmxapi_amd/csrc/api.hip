@@ -367,6 +367,13 @@ void buildWorkGroups(nbx_context* c, WorkLayout& L, bool adjoint)
         L.unionOff.ensure(ng + 1);
         L.chunkOff.ensure(ng + 1);
         L.workOff.ensure(ng + 1);
+        L.groups.ensure(ng);
+        L.groupInfo.ensure(ng);
+        // one pass writes the layout at upper-bound bases (no count pass);
+        // the scans of its per-group counts give the compact offsets
+        L.unionTmp.ensure(std::max(c->ncj, 1));
+        L.chunksTmp.ensure(std::max(c->nci, 1));
+        L.workTmp.ensure(std::max(c->ncj + (kTileJ - 1) * c->nci, 1));
         GroupArgs ga{};
         ga.ncluster     = nc;
         ga.groupSize    = G;
@@ -381,7 +388,17 @@ void buildWorkGroups(nbx_context* c, WorkLayout& L, bool adjoint)
         ga.chunkCount   = L.chunkCount.p;
         ga.workCount    = L.workCount.p;
         ga.overflow     = L.overflow.p;
-        launchBuildGroups(c->stream, ga, false);
+        ga.unionOff     = L.unionOff.p;
+        ga.chunkOff     = L.chunkOff.p;
+        ga.workOff      = L.workOff.p;
+        ga.unionList    = L.unionTmp.p;
+        ga.work         = L.workTmp.p;
+        ga.chunks       = L.chunksTmp.p;
+        ga.groups       = L.groups.p;
+        ga.groupInfo    = L.groupInfo.p;
+        ga.windowCap    = 1 << 30; // uncapped slots; the compaction applies the window capacity
+        ga.upperBound   = 1;
+        launchBuildGroups(c->stream, ga, true);
         check(hipMemsetAsync(c->groupStats.p + kGroupSizeCount, 0, sizeof(int) * 2, c->stream), "clear stats");
         launchMaxReduce(c->stream, ng, L.unionCount.p, c->groupStats.p + kGroupSizeCount);
         launchMaxReduce(c->stream, ng, L.overflow.p, c->groupStats.p + kGroupSizeCount + 1);
@@ -426,21 +443,15 @@ void buildWorkGroups(nbx_context* c, WorkLayout& L, bool adjoint)
             }
             L.spill = spilled > 0;
         }
+        // the compaction gives union clusters beyond the window the overflow slot
+        ga.windowCap = slots;
         L.unionList.ensure(std::max(unionTotal, 1));
         L.chunks.ensure(std::max(chunkTotal, 1));
-        L.groups.ensure(ng);
-        L.groupInfo.ensure(ng);
         L.work.ensure(std::max(workTotal, 1));
-        ga.unionOff  = L.unionOff.p;
-        ga.chunkOff  = L.chunkOff.p;
-        ga.workOff   = L.workOff.p;
         ga.unionList = L.unionList.p;
         ga.work      = L.work.p;
         ga.chunks    = L.chunks.p;
-        ga.groups    = L.groups.p;
-        ga.groupInfo = L.groupInfo.p;
-        ga.windowCap = slots;
-        launchBuildGroups(c->stream, ga, true);
+        launchGroupCompact(c->stream, ga, L.unionTmp.p, L.workTmp.p, L.chunksTmp.p);
         L.chunkShift.ensure(std::max(chunkTotal, 1));
         launchChunkShifts(c->stream, chunkTotal, L.chunks.p, c->shiftVec.p, L.chunkShift.p);
         check(hipGetLastError(), "group kernels");

@@ -166,6 +166,10 @@ struct WorkLayout
     bool           spill     = false; // some union clusters take global atomics (no window slot)
     DevBuf<int>    unionCount, chunkCount, workCount, overflow, unionOff, chunkOff, workOff, unionList;
     DevBuf<int4>   chunks, groups, groupInfo;
+    // the group pass's output at upper-bound bases, before compaction
+    DevBuf<int>    unionTmp;
+    DevBuf<int4>   chunksTmp;
+    DevBuf<uint2>  workTmp;
     DevBuf<float4> chunkShift;
     DevBuf<uint2>  work;
     DevBuf<int2>   waveStart; // static schedule: each wavefront's first chunk and its group

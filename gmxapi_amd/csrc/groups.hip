@@ -185,7 +185,7 @@ __global__ __launch_bounds__(kGroupThreads) void buildGroupsKernel(GroupArgs a)
     // union clusters that get a window slot (the rest overflow)
     const int Uw     = kFill ? min(U, a.windowCap - 1 - a.groupSize) : U;
     const int nchunk = sChunks;
-    if (!kFill)
+    if (!kFill || a.upperBound)
     {
         if (threadIdx.x == 0)
         {
@@ -194,10 +194,17 @@ __global__ __launch_bounds__(kGroupThreads) void buildGroupsKernel(GroupArgs a)
             a.workCount[g]  = sWork;
             a.overflow[g]   = (ni > kGroupThreads) ? 1 : 0;
         }
-        return;
+        if (!kFill || ni > kGroupThreads)
+        {
+            return;
+        }
     }
-    // compact the unique keys in place at the front of sKeys (rank order)
-    const int uoff = a.unionOff[g];
+    // where the group's union, chunks and work go: the scanned offsets, or
+    // (upperBound) bounds that need no scan of other groups: the union has
+    // at most n entries, the chunks (one per i-entry) at most ni, the padded
+    // work at most n + (kTileJ - 1) ni, so bases from the cluster offsets
+    // never overlap; groupCompactKernel moves them together afterwards
+    const int uoff = a.upperBound ? jb : a.unionOff[g];
     for (int t = threadIdx.x; t < p; t += kGroupThreads)
     {
         const int k = sKeys[t];
@@ -212,7 +219,7 @@ __global__ __launch_bounds__(kGroupThreads) void buildGroupsKernel(GroupArgs a)
         sKeys[t] = a.unionList[uoff + t];
     }
     __syncthreads();
-    const int coff = a.chunkOff[g];
+    const int coff = a.upperBound ? ib : a.chunkOff[g];
     if (threadIdx.x == 0)
     {
         a.groups[g]    = make_int4(coff, coff + nchunk, uoff, Uw);
@@ -222,7 +229,7 @@ __global__ __launch_bounds__(kGroupThreads) void buildGroupsKernel(GroupArgs a)
     // wave per entry)
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int woff = a.workOff[g];
+    const int woff = a.upperBound ? jb + (kTileJ - 1) * ib : a.workOff[g];
     for (int e = wave; e < ni; e += kGroupThreads / 64)
     {
         const int4 ie      = a.ientry[ib + e];
@@ -306,6 +313,54 @@ __global__ __launch_bounds__(kGroupThreads) void buildGroupsKernel(GroupArgs a)
                 a.chunks[base + k] = make_int4(ci, shift | (self << 8) | (special << 9), wb + s, tiles);
             }
         }
+    }
+}
+
+// The layout written at the upper-bound bases moved to its scanned
+// offsets (one workgroup per group): union, work entries and chunks, the
+// chunks' work starts and the group record rebased.  With a window of
+// a.windowCap slots, union clusters beyond it take kSlotOverflow and the
+// padding entries the spare slot after the last windowed one.
+__global__ __launch_bounds__(kGroupThreads) void groupCompactKernel(GroupArgs a, const int* __restrict__ unionTmp,
+                                                                    const uint2* __restrict__ workTmp,
+                                                                    const int4* __restrict__ chunksTmp)
+{
+    const int g  = blockIdx.x;
+    const int c0 = g * a.groupSize;
+    const int jb = a.offJ[c0];
+    const int ib = a.offI[c0];
+    const int wb = jb + (kTileJ - 1) * ib;
+    const int U = a.unionCount[g], nch = a.chunkCount[g], nw = a.workCount[g];
+    const int uo = a.unionOff[g], co = a.chunkOff[g], wo = a.workOff[g];
+    const int Uw = min(U, a.windowCap - 1 - a.groupSize);
+    const unsigned firstOver = static_cast<unsigned>(a.groupSize + Uw);
+    for (int t = threadIdx.x; t < U; t += kGroupThreads)
+    {
+        a.unionList[uo + t] = unionTmp[jb + t];
+    }
+    for (int t = threadIdx.x; t < nw; t += kGroupThreads)
+    {
+        uint2          e    = workTmp[wb + t];
+        const unsigned slot = e.y >> 16;
+        if (static_cast<int>(e.x) == a.ncluster)
+        {
+            e.y = (firstOver << 16) | (e.y & 0xffffu);
+        }
+        else if (slot >= firstOver && slot != kSlotOverflow)
+        {
+            e.y = (kSlotOverflow << 16) | (e.y & 0xffffu);
+        }
+        a.work[wo + t] = e;
+    }
+    for (int t = threadIdx.x; t < nch; t += kGroupThreads)
+    {
+        int4 c         = chunksTmp[ib + t];
+        c.z            = wo + (c.z - wb);
+        a.chunks[co + t] = c;
+    }
+    if (threadIdx.x == 0)
+    {
+        a.groups[g] = make_int4(co, co + nch, uo, Uw);
     }
 }
 
@@ -775,6 +830,15 @@ void launchSystemSlots(hipStream_t s, int numSystems, const int* groupOff, int n
 {
     systemSlotKernel<<<(numSystems + 256) / 256, 256, 0, s>>>(numSystems, groupOff, ngroups, groups, numWaves,
                                                              waveStart, slotOff);
+}
+
+void launchGroupCompact(hipStream_t s, const GroupArgs& a, const int* unionTmp, const uint2* workTmp,
+                        const int4* chunksTmp)
+{
+    if (a.ngroups > 0)
+    {
+        groupCompactKernel<<<a.ngroups, kGroupThreads, 0, s>>>(a, unionTmp, workTmp, chunksTmp);
+    }
 }
 
 void launchGroupSpans(hipStream_t s, int ncluster, const int* offJ, int* maxSpan)

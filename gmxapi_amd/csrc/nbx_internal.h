@@ -267,12 +267,20 @@ struct GroupArgs
     int4*        groups;     // {chunk_start, chunk_end, union_offset, union_size}
     int4*        groupInfo;  // {first own cluster, own count | G << 16, shift-table offset, group index}
     int          windowCap;  // fill pass: window slots (own, union, spare); union beyond -> kSlotOverflow
+    // fill pass without a count pass: writes the per-group counts and the
+    // layout at upper-bound bases (union at offJ[c0], chunks at offI[c0],
+    // work at offJ[c0] + (kTileJ - 1) offI[c0]) for launchGroupCompact
+    int          upperBound;
 };
 // window slot of a j-entry whose cluster did not fit the group's window:
 // the kernel adds its forces with global atomics
 constexpr unsigned kSlotOverflow = 0xffffu;
 void launchGroupSpans(hipStream_t s, int ncluster, const int* offJ, int* maxSpan);
 void launchBuildGroups(hipStream_t s, const GroupArgs& a, bool fill);
+// moves an upper-bound layout (unionTmp, workTmp, chunksTmp) to a.unionOff,
+// a.workOff, a.chunkOff in a.unionList, a.work, a.chunks; rebases a.groups
+void launchGroupCompact(hipStream_t s, const GroupArgs& a, const int* unionTmp, const uint2* workTmp,
+                        const int4* chunksTmp);
 void launchMaxReduce(hipStream_t s, int n, const int* v, int* out);
 // Occurrences of every cluster in the groups' j-force windows (own slot and
 // windowed-union slots) as a CSR list of {group, slot}, sorted per cluster.
