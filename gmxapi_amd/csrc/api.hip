@@ -1172,7 +1172,7 @@ int nbx_construct_pairlist(nbx_context* c)
         c->countI.ensure(nc + 1);
         c->offJ.ensure(nc + 1);
         c->offI.ensure(nc + 1);
-        c->totals.ensure(2);
+        c->totals.ensure(3);
         SearchArgs a{};
         a.g           = c->g;
         a.rlist       = c->rlist;
@@ -1187,21 +1187,63 @@ int nbx_construct_pairlist(nbx_context* c)
         a.exclIdx     = c->exclIdx.p;
         a.countJ      = c->countJ.p;
         a.countI      = c->countI.p;
-        launchSearch(c->stream, a, false);
-        launchExclusiveScan(c->stream, nc, c->countJ.p, c->offJ.p, c->totals.p);
-        launchExclusiveScan(c->stream, nc, c->countI.p, c->offI.p, c->totals.p + 1);
-        int tot[2] = { 0, 0 };
-        check(hipMemcpyAsync(tot, c->totals.p, sizeof(int) * 2, hipMemcpyDeviceToHost, c->stream), "read totals");
-        check(hipStreamSynchronize(c->stream), "search sync");
-        c->ncj = tot[0];
-        c->nci = tot[1];
-        c->ientry.ensure(c->nci);
-        c->jentry.ensure(c->ncj);
-        a.offJ   = c->offJ.p;
-        a.offI   = c->offI.p;
-        a.ientry = c->ientry.p;
-        a.jentry = c->jentry.p;
-        launchSearch(c->stream, a, true);
+        a.offJ        = c->offJ.p;
+        a.offI        = c->offI.p;
+        // After a first search, the fill runs without a count pass: each
+        // cluster writes into a region sized from the largest count seen
+        // (with a margin) and a compaction follows the scans; a cluster that
+        // outgrows it sends the search through the counted path again.
+        bool done = false;
+        if (c->searchCapJ > 0)
+        {
+            const int cap = c->searchCapJ;
+            c->searchJTmp.ensure(static_cast<size_t>(nc) * cap);
+            c->searchITmp.ensure(static_cast<size_t>(nc) * kMaxIEntriesPerCluster);
+            a.capJ   = cap;
+            a.jentry = c->searchJTmp.p;
+            a.ientry = c->searchITmp.p;
+            launchSearch(c->stream, a, true);
+            launchExclusiveScan(c->stream, nc, c->countJ.p, c->offJ.p, c->totals.p);
+            launchExclusiveScan(c->stream, nc, c->countI.p, c->offI.p, c->totals.p + 1);
+            check(hipMemsetAsync(c->totals.p + 2, 0, sizeof(int), c->stream), "clear max");
+            launchMaxReduce(c->stream, nc, c->countJ.p, c->totals.p + 2);
+            int tot[3] = { 0, 0, 0 };
+            check(hipMemcpyAsync(tot, c->totals.p, sizeof(tot), hipMemcpyDeviceToHost, c->stream), "read totals");
+            check(hipStreamSynchronize(c->stream), "search sync");
+            if (tot[2] <= cap)
+            {
+                c->ncj = tot[0];
+                c->nci = tot[1];
+                c->ientry.ensure(c->nci);
+                c->jentry.ensure(c->ncj);
+                launchSearchCompact(c->stream, nc, cap, c->countJ.p, c->countI.p, c->offJ.p, c->offI.p,
+                                    c->searchJTmp.p, c->searchITmp.p, c->jentry.p, c->ientry.p);
+                c->searchCapJ = tot[2] + tot[2] / 4 + 32;
+                done          = true;
+            }
+        }
+        if (!done)
+        {
+            a.capJ   = 0;
+            a.ientry = nullptr;
+            a.jentry = nullptr;
+            launchSearch(c->stream, a, false);
+            launchExclusiveScan(c->stream, nc, c->countJ.p, c->offJ.p, c->totals.p);
+            launchExclusiveScan(c->stream, nc, c->countI.p, c->offI.p, c->totals.p + 1);
+            check(hipMemsetAsync(c->totals.p + 2, 0, sizeof(int), c->stream), "clear max");
+            launchMaxReduce(c->stream, nc, c->countJ.p, c->totals.p + 2);
+            int tot[3] = { 0, 0, 0 };
+            check(hipMemcpyAsync(tot, c->totals.p, sizeof(tot), hipMemcpyDeviceToHost, c->stream), "read totals");
+            check(hipStreamSynchronize(c->stream), "search sync");
+            c->ncj = tot[0];
+            c->nci = tot[1];
+            c->ientry.ensure(c->nci);
+            c->jentry.ensure(c->ncj);
+            a.ientry = c->ientry.p;
+            a.jentry = c->jentry.p;
+            launchSearch(c->stream, a, true);
+            c->searchCapJ = tot[2] + tot[2] / 4 + 32;
+        }
         check(hipGetLastError(), "search kernels");
         launchExclusionSlots(c->stream, c->numExclPairs, c->exclPairs.p, c->cells.p, c->exclPairSlots.p);
         c->haveOuter = false;

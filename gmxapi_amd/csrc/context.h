@@ -257,6 +257,11 @@ struct nbx_context
     // list
     bool           listValid = false;
     int            nci = 0, ncj = 0;
+    // the fill-only search: j-entries per cluster region (0 before the
+    // first search) and the regions
+    int                 searchCapJ = 0;
+    nbx::DevBuf<uint2>  searchJTmp;
+    nbx::DevBuf<int4>   searchITmp;
     nbx::DevBuf<int>    countJ, countI, offJ, offI, totals;
     nbx::DevBuf<int4>   ientry;
     nbx::DevBuf<uint2>  jentry, jentryTmp;

@@ -238,8 +238,17 @@ struct SearchArgs
     const int* offI; // per ci: first i entry
     int4*  ientry;
     uint2* jentry;
+    // fill pass without a count pass (capJ > 0): cluster ci's entries at
+    // jentry[ci * capJ] and ientry[ci * kMaxIEntriesPerCluster], counts
+    // written, entries past capJ dropped (the host sees the count)
+    int    capJ;
 };
+// shift indices at or below the central one: at most 14 i-entries per cluster
+constexpr int kMaxIEntriesPerCluster = 14;
 void launchSearch(hipStream_t s, const SearchArgs& a, bool fill);
+void launchSearchCompact(hipStream_t s, int ncluster, int capJ, const int* countJ, const int* countI,
+                         const int* offJ, const int* offI, const uint2* jentryTmp, const int4* ientryTmp,
+                         uint2* jentry, int4* ientry);
 void launchExclusiveScan(hipStream_t s, int n, const int* in, int* out, int* total);
 
 // -- groups.hip

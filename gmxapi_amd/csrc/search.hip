@@ -108,8 +108,11 @@ __global__ __launch_bounds__(kWave* kSearchWaves) void searchKernel(SearchArgs a
     int numExcl = 0, exSlot = -1, exI = 0;
     if (kFill)
     {
-        jpos = a.offJ[ci];
-        ipos = a.offI[ci];
+        // capJ > 0: a fill without a count pass, into this cluster's own
+        // region of capJ j-entries and kMaxIEntriesPerCluster i-entries
+        // (searchCompactKernel moves them together after the scans)
+        jpos = a.capJ > 0 ? ci * a.capJ : a.offJ[ci];
+        ipos = a.capJ > 0 ? ci * kMaxIEntriesPerCluster : a.offI[ci];
         int cnt[kClusterSize], beg[kClusterSize];
 #pragma unroll
         for (int i = 0; i < kClusterSize; i++)
@@ -311,12 +314,15 @@ __global__ __launch_bounds__(kWave* kSearchWaves) void searchKernel(SearchArgs a
                                 }
                             }
                         }
-                        if (kFill && hit && numExcl <= kWave)
+                        // a capped region past its end: not written, the
+                        // count (above capJ) sends the host to the counted path
+                        const bool room = a.capJ == 0 || entryPos + entryJ + before < jpos + a.capJ;
+                        if (kFill && hit && room && numExcl <= kWave)
                         {
                             const unsigned mask = ((central && cj == ci) ? kMaskDiag : kMaskAll) & ~exclMask;
                             a.jentry[entryPos + entryJ + before] = make_uint2(static_cast<unsigned>(cj), mask);
                         }
-                        else if (kFill && hit)
+                        else if (kFill && hit && room)
                         {
                             unsigned mask = (central && cj == ci) ? kMaskDiag : kMaskAll;
                             for (int i = 0; i < kClusterSize; i++)
@@ -362,10 +368,40 @@ __global__ __launch_bounds__(kWave* kSearchWaves) void searchKernel(SearchArgs a
             }
         }
     }
-    if (!kFill && lane == 0)
+    if ((!kFill || a.capJ > 0) && lane == 0)
     {
         a.countJ[ci] = countJ;
         a.countI[ci] = countI;
+    }
+}
+
+// A capped fill's lists moved to the scanned offsets (one wavefront per
+// cluster), the i-entries' j ranges rebased.
+__global__ __launch_bounds__(256) void searchCompactKernel(int ncluster, int capJ, const int* __restrict__ countJ,
+                                                           const int* __restrict__ countI,
+                                                           const int* __restrict__ offJ, const int* __restrict__ offI,
+                                                           const uint2* __restrict__ jentryTmp,
+                                                           const int4* __restrict__ ientryTmp,
+                                                           uint2* __restrict__ jentry, int4* __restrict__ ientry)
+{
+    const int lane = threadIdx.x & 63;
+    const int ci   = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (ci >= ncluster)
+    {
+        return;
+    }
+    const int nJ = countJ[ci], nI = countI[ci];
+    const int src = ci * capJ, dst = offJ[ci];
+    for (int t = lane; t < nJ; t += 64)
+    {
+        jentry[dst + t] = jentryTmp[src + t];
+    }
+    if (lane < nI)
+    {
+        int4 ie = ientryTmp[ci * kMaxIEntriesPerCluster + lane];
+        ie.z    = dst + (ie.z - src);
+        ie.w    = dst + (ie.w - src);
+        ientry[offI[ci] + lane] = ie;
     }
 }
 
@@ -529,6 +565,17 @@ void launchSearch(hipStream_t s, const SearchArgs& a, bool fill)
     else
     {
         searchKernel<false><<<blocks, kWave * kSearchWaves, 0, s>>>(a);
+    }
+}
+
+void launchSearchCompact(hipStream_t s, int ncluster, int capJ, const int* countJ, const int* countI,
+                         const int* offJ, const int* offI, const uint2* jentryTmp, const int4* ientryTmp,
+                         uint2* jentry, int4* ientry)
+{
+    if (ncluster > 0)
+    {
+        searchCompactKernel<<<(ncluster + 3) / 4, 256, 0, s>>>(ncluster, capJ, countJ, countI, offJ, offI, jentryTmp,
+                                                              ientryTmp, jentry, ientry);
     }
 }
 

@@ -75,6 +75,28 @@ def test_pair_at_exactly_rlist():
     np.testing.assert_array_equal(ctx.list_pairs(s.rlist), ref)
 
 
+def test_rebuilt_lists_match_the_oracle():
+    # a list rebuilt on a context fills each cluster's region sized from the
+    # previous search (no count pass) and falls back to counting when a
+    # cluster outgrows it: dilute box first (small regions), then the dense
+    # box (regions too small: counted), then moved atoms (regions fit)
+    import dataclasses
+
+    _, s = _systems()[1]
+    ctx = nbx.context_for(s)
+    dilute = dataclasses.replace(s, box=s.box * 1.6, x=s.x * 1.6)
+    rng = np.random.default_rng(5)
+    moved = dataclasses.replace(s, x=np.mod(s.x + 0.02 * rng.standard_normal(s.x.shape), s.box))
+    for name, t in (("dilute", dilute), ("dense", s), ("moved", moved)):
+        print(f"[rebuild] {name}", flush=True)
+        ctx.put_on_grid(t.box, t.x)
+        ctx.construct_pairlist()
+        np.testing.assert_array_equal(ctx.list_pairs(t.rlist), oracle_for(t, "f32").list_pairs(t.rlist))
+        f = ctx.compute(t.x)
+        r64 = oracle_for(t, "f64").compute(t.x)
+        assert rms(f - r64["f"]) / rms(r64["f"]) <= FORCE_TOL_RMS
+
+
 def test_pair_set_config0(water3000):
     print("[pairs] config0 9000 atoms", flush=True)
     s = water3000
