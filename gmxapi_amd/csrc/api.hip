@@ -304,7 +304,7 @@ const float* deviceCoordinates(nbx_context* c, const float* x, int onDevice)
 // (groups.hip): the largest group size whose j-entries fit the LDS sort and
 // whose window fits the kernel's LDS budget (the pair kernel's, or with
 // adjoint the adjoint kernel's, whose window holds 4 floats per atom).
-void buildWorkGroups(nbx_context* c, WorkLayout& L, bool adjoint)
+void buildWorkGroups(nbx_context* c, WorkLayout& L, bool adjoint, const int* spansKnown = nullptr)
 {
     auto ldsBytes = [&](int slots) {
         return adjoint ? adjointLdsBytes(c->ntype, c->ic.tab_n, c->elec, slots)
@@ -315,11 +315,18 @@ void buildWorkGroups(nbx_context* c, WorkLayout& L, bool adjoint)
     // max union, overflow flag and the three scan totals of one pass
     constexpr int kStats = kGroupSizeCount + 5;
     c->groupStats.ensure(kStats);
-    check(hipMemsetAsync(c->groupStats.p, 0, sizeof(int) * kStats, c->stream), "clear stats");
-    launchGroupSpans(c->stream, nc, c->offJ.p, c->groupStats.p);
     int spans[kGroupSizeCount] = {};
-    check(hipMemcpyAsync(spans, c->groupStats.p, sizeof(spans), hipMemcpyDeviceToHost, c->stream), "read spans");
-    check(hipStreamSynchronize(c->stream), "group sync");
+    if (spansKnown)
+    {
+        std::copy(spansKnown, spansKnown + kGroupSizeCount, spans);
+    }
+    else
+    {
+        check(hipMemsetAsync(c->groupStats.p, 0, sizeof(int) * kStats, c->stream), "clear stats");
+        launchGroupSpans(c->stream, nc, c->offJ.p, c->groupStats.p);
+        check(hipMemcpyAsync(spans, c->groupStats.p, sizeof(spans), hipMemcpyDeviceToHost, c->stream), "read spans");
+        check(hipStreamSynchronize(c->stream), "group sync");
+    }
     c->jentryTmp.ensure(c->ncj);
     // the largest group size (or the chosen one) whose j-force windows, one
     // per wavefront, fit the workgroup's LDS budget: larger groups share more
@@ -456,7 +463,7 @@ void buildWorkGroups(nbx_context* c, WorkLayout& L, bool adjoint)
         launchChunkShifts(c->stream, chunkTotal, L.chunks.p, c->shiftVec.p, L.chunkShift.p);
         check(hipGetLastError(), "group kernels");
         c->jentry.swap(c->jentryTmp);
-        buildOccurrences(c->stream, L.gather, ng, L.groups.p, L.groupInfo.p, L.unionList.p, nc);
+        buildOccurrences(c->stream, L.gather, ng, L.groups.p, L.groupInfo.p, L.unionList.p, nc, nc + unionTotal);
         L.groupSize   = G;
         L.ngroups     = ng;
         L.nchunk      = chunkTotal;
@@ -1140,9 +1147,9 @@ int nbx_put_on_grid(nbx_context* c, const float* box, const float* x, int xOnDev
 
 // After a search or a prune: the pair kernel's work layout and schedule and
 // the per-step output buffers sized for it.
-static void finishList(nbx_context* c)
+static void finishList(nbx_context* c, const int* spans = nullptr)
 {
-    buildWorkGroups(c, c->lay, false);
+    buildWorkGroups(c, c->lay, false, spans);
     scheduleLayout(c, c->lay, false);
     const int numWaves = c->lay.numBlocks * kKernelWaves;
     c->energyPartial.ensure(2 * static_cast<size_t>(c->lay.ngroups + numWaves));
@@ -1189,6 +1196,16 @@ int nbx_construct_pairlist(nbx_context* c)
         a.countI      = c->countI.p;
         a.offJ        = c->offJ.p;
         a.offI        = c->offI.p;
+        // the group pass's largest j-entry span per group size, read with
+        // the list's totals (one wait for both)
+        int  spans[kGroupSizeCount] = {};
+        auto readSpans              = [&] {
+            c->groupStats.ensure(kGroupSizeCount + 5);
+            check(hipMemsetAsync(c->groupStats.p, 0, sizeof(int) * kGroupSizeCount, c->stream), "clear spans");
+            launchGroupSpans(c->stream, nc, c->offJ.p, c->groupStats.p);
+            check(hipMemcpyAsync(spans, c->groupStats.p, sizeof(spans), hipMemcpyDeviceToHost, c->stream),
+                  "read spans");
+        };
         // After a first search, the fill runs without a count pass: each
         // cluster writes into a region sized from the largest count seen
         // (with a margin) and a compaction follows the scans; a cluster that
@@ -1207,6 +1224,7 @@ int nbx_construct_pairlist(nbx_context* c)
             launchExclusiveScan(c->stream, nc, c->countI.p, c->offI.p, c->totals.p + 1);
             check(hipMemsetAsync(c->totals.p + 2, 0, sizeof(int), c->stream), "clear max");
             launchMaxReduce(c->stream, nc, c->countJ.p, c->totals.p + 2);
+            readSpans();
             int tot[3] = { 0, 0, 0 };
             check(hipMemcpyAsync(tot, c->totals.p, sizeof(tot), hipMemcpyDeviceToHost, c->stream), "read totals");
             check(hipStreamSynchronize(c->stream), "search sync");
@@ -1232,6 +1250,7 @@ int nbx_construct_pairlist(nbx_context* c)
             launchExclusiveScan(c->stream, nc, c->countI.p, c->offI.p, c->totals.p + 1);
             check(hipMemsetAsync(c->totals.p + 2, 0, sizeof(int), c->stream), "clear max");
             launchMaxReduce(c->stream, nc, c->countJ.p, c->totals.p + 2);
+            readSpans();
             int tot[3] = { 0, 0, 0 };
             check(hipMemcpyAsync(tot, c->totals.p, sizeof(tot), hipMemcpyDeviceToHost, c->stream), "read totals");
             check(hipStreamSynchronize(c->stream), "search sync");
@@ -1247,7 +1266,7 @@ int nbx_construct_pairlist(nbx_context* c)
         check(hipGetLastError(), "search kernels");
         launchExclusionSlots(c->stream, c->numExclPairs, c->exclPairs.p, c->cells.p, c->exclPairSlots.p);
         c->haveOuter = false;
-        finishList(c);
+        finishList(c, spans);
     });
 }
 

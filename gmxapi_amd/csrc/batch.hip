@@ -275,7 +275,8 @@ void buildBatch(nbx_batch* b, int G, int blocksPerCU, int chunkWeight)
     b->spill       = anySpill;
     b->ncj         = ncj;
     b->groupSize   = G > 0 ? G : b->sys[0]->lay.groupSize;
-    buildOccurrences(b->stream, b->gather, b->ngroups, b->groups.p, b->groupInfo.p, b->unionList.p, totClusters);
+    buildOccurrences(b->stream, b->gather, b->ngroups, b->groups.p, b->groupInfo.p, b->unionList.p, totClusters,
+                     totClusters + static_cast<int>(hUnion.size()));
     scheduleBatch(b, blocksPerCU, chunkWeight);
     b->built      = true;
     b->haveEnergy = false;

@@ -92,8 +92,11 @@ struct WindowGather
 };
 
 // Occurrence lists of the clusters in the groups' windows (once per list).
+// A group lists its own clusters and its windowed union, so the lists hold
+// at most ncluster + (union entries of all groups) = maxTotal occurrences:
+// with maxTotal given the host does not wait for the exact count.
 inline void buildOccurrences(hipStream_t s, WindowGather& w, int ngroups, const int4* groups, const int4* groupInfo,
-                             const int* unionList, int ncluster)
+                             const int* unionList, int ncluster, int maxTotal = -1)
 {
     w.occCount.ensure(ncluster + 1);
     w.occOff.ensure(ncluster + 1);
@@ -106,9 +109,13 @@ inline void buildOccurrences(hipStream_t s, WindowGather& w, int ngroups, const 
     check(hipMemsetAsync(w.occCount.p, 0, sizeof(int) * ncluster, s), "clear occurrences");
     launchOccurrences(s, ngroups, groups, groupInfo, unionList, ncluster, w.occCount.p, nullptr, nullptr, false);
     launchExclusiveScan(s, ncluster, w.occCount.p, w.occOff.p, w.occOff.p + ncluster);
-    int total = 0;
-    check(hipMemcpyAsync(&total, w.occOff.p + ncluster, sizeof(int), hipMemcpyDeviceToHost, s), "read occurrences");
-    check(hipStreamSynchronize(s), "occurrence sync");
+    int total = maxTotal;
+    if (total < 0)
+    {
+        check(hipMemcpyAsync(&total, w.occOff.p + ncluster, sizeof(int), hipMemcpyDeviceToHost, s),
+              "read occurrences");
+        check(hipStreamSynchronize(s), "occurrence sync");
+    }
     w.occ.ensure(std::max(total, 1));
     check(hipMemsetAsync(w.occCount.p, 0, sizeof(int) * ncluster, s), "clear occurrences");
     launchOccurrences(s, ngroups, groups, groupInfo, unionList, ncluster, w.occCount.p, w.occOff.p, w.occ.p, true);
@@ -138,14 +145,15 @@ inline void buildSegments(hipStream_t s, WindowGather& w, int ngroups, const int
     // the staging positions each cluster's gather reads
     w.gposCount.ensure(ncluster + 1);
     w.gposOff.ensure(ncluster + 1);
-    int npos = 0;
+    // every staged slot is read by at most one cluster's gather (a group's
+    // segment holds G + union slots per wavefront, its occurrences own +
+    // union entries), so the positions number at most total / slotF4
+    const int npos = total / slotF4;
     if (ncluster > 0)
     {
         launchGatherList(s, ncluster, o.occOff.p, o.occ.p, w.segInfo.p, slotF4, w.gposCount.p, nullptr, nullptr,
                          false);
         launchExclusiveScan(s, ncluster, w.gposCount.p, w.gposOff.p, w.gposOff.p + ncluster);
-        check(hipMemcpyAsync(&npos, w.gposOff.p + ncluster, sizeof(int), hipMemcpyDeviceToHost, s), "read gather");
-        check(hipStreamSynchronize(s), "gather sync");
     }
     else
     {
