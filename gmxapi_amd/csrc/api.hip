@@ -1,6 +1,7 @@
 // C ABI of libnbx: context management and the host-side sequencing of the
 // grid, search and kernel launches.  See include/nbx.h for the contract.
 #include <algorithm>
+#include <climits>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -500,8 +501,25 @@ void scheduleLayout(nbx_context* c, WorkLayout& L, bool adjoint)
     }
     L.numBlocks = numWaves / kKernelWaves;
     L.waveStart.ensure(numWaves + 1);
-    launchWaveSchedule(c->stream, numWaves, L.nchunk, L.chunks.p, L.nwork, kChunkWeight, L.ngroups, L.groups.p,
-                       L.waveStart.p);
+    // the forward kernel's runs by the measured cost model (unless the cost
+    // total could overflow an int), with the first of two resident
+    // generations of workgroups taking the larger share
+    const long long costBound = static_cast<long long>(L.nwork / kTileJ + 1) * (kCostTile + 6)
+                                + static_cast<long long>(kCostChunk) * L.nchunk;
+    if (!adjoint && costBound < INT_MAX)
+    {
+        L.chunkCost.ensure(L.nchunk + 2);
+        L.costOff.ensure(L.nchunk + 1);
+        launchChunkCosts(c->stream, L.nchunk, L.chunks.p, c->nbtype.p, c->typeLJ.p, L.chunkCost.p);
+        launchExclusiveScan(c->stream, L.nchunk, L.chunkCost.p, L.costOff.p, L.chunkCost.p + L.nchunk + 1);
+        launchWaveSchedule(c->stream, numWaves, L.nchunk, L.chunks.p, L.nwork, kChunkWeight, L.ngroups, L.groups.p,
+                           L.waveStart.p, L.costOff.p, L.chunkCost.p + L.nchunk + 1, c->numCU * kKernelWaves);
+    }
+    else
+    {
+        launchWaveSchedule(c->stream, numWaves, L.nchunk, L.chunks.p, L.nwork, kChunkWeight, L.ngroups, L.groups.p,
+                           L.waveStart.p);
+    }
     check(hipGetLastError(), "schedule kernel");
     buildSegments(c->stream, L.gather, L.ngroups, L.groups.p, L.groupInfo.p, numWaves, L.waveStart.p, c->ncluster,
                   adjoint ? 4 : 3);

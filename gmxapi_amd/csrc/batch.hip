@@ -126,7 +126,7 @@ void scheduleBatch(nbx_batch* b, int blocksPerCU, int chunkWeight)
     b->slotOff.ensure(R + 1);
     b->energyPartial.ensure(2 * static_cast<size_t>(b->ngroups + numWaves));
     launchWaveSchedule(b->stream, numWaves, b->nchunk, b->chunks.p, b->nwork, chunkWeight, b->ngroups, b->groups.p,
-                       b->waveStart.p);
+                       b->waveStart.p, nullptr, nullptr, b->numCU * kKernelWaves);
     launchSystemSlots(b->stream, R, b->groupOff.p, b->ngroups, b->groups.p, numWaves, b->waveStart.p, b->slotOff.p);
     check(hipGetLastError(), "schedule kernels");
     buildSegments(b->stream, b->gather, b->ngroups, b->groups.p, b->groupInfo.p, numWaves, b->waveStart.p,

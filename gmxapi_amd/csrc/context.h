@@ -181,6 +181,7 @@ struct WorkLayout
     DevBuf<float4> chunkShift;
     DevBuf<uint2>  work;
     DevBuf<int2>   waveStart; // static schedule: each wavefront's first chunk and its group
+    DevBuf<int>    chunkCost, costOff; // the schedule's per-chunk costs (and their total) and their scan
     WindowGather   gather;
 };
 

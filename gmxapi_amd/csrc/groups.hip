@@ -663,19 +663,65 @@ __device__ int groupOfChunk(int ngroups, const int4* __restrict__ groups, int k)
     return lo;
 }
 
+// Measured cost of a chunk in the pair kernel, in 1/32 tile: 32 per tile,
+// plus 1.5 per tile for each i-atom whose type has LJ (the kernel skips the
+// LJ terms of the others), plus 27 per chunk (its set-up and i-force
+// reduction).  Fitted to the run times of individual wavefronts on the
+// 288,000-atom water box, where it leaves 2.3 us of 205 us unexplained.
+__global__ void chunkCostKernel(int nchunk, const int4* __restrict__ chunks, const int* __restrict__ type,
+                                const int* __restrict__ typeLJ, int* __restrict__ cost)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < nchunk)
+    {
+        const int4 cd  = chunks[m];
+        int        nlj = 0;
+#pragma unroll
+        for (int i = 0; i < kClusterSize; i++)
+        {
+            nlj += typeLJ[type[cd.x * kClusterSize + i]];
+        }
+        cost[m] = cd.w * (kCostTile + (3 * nlj) / 2) + kCostChunk;
+    }
+}
+
+// Share of the work given to each resident generation of workgroups.  The
+// dispatcher places workgroup b on CU b mod numCU, so with two workgroups
+// per CU the first numCU workgroups share every SIMD with the second ones;
+// the SIMD's arbiter issues for the older wavefront first, and at equal
+// work the second generation measured 9% slower, so the first generation
+// takes (1 + skew) and the second (1 - skew) of the average.  Other numbers
+// of generations are not weighted (not measured).  Returns the cumulative
+// weight of wavefronts [0, w) in 1/1000.
+__device__ long long generationWeight(int w, int numWaves, int genWaves)
+{
+    if (genWaves <= 0 || numWaves != 2 * genWaves)
+    {
+        return 1000LL * w;
+    }
+    const long long first = 1000 + kGenerationSkewMilli, second = 1000 - kGenerationSkewMilli;
+    return w <= genWaves ? first * w : first * genWaves + second * (w - genWaves);
+}
+
 // One workgroup.  Wavefront w's target start is the first chunk whose cost
-// offset (work offset plus chunkWeight per earlier chunk) reaches w/numWaves
-// of the total; equal targets are merged, so the runs are non-empty and in
-// order, and the wavefronts left over (more wavefronts than distinct
-// starts) get empty runs at the end.  A group's chunks are then run by
-// consecutive wavefronts, each of which writes its staging segment.
+// offset reaches the wavefront's cumulative share of the total; equal
+// targets are merged, so the runs are non-empty and in order, and the
+// wavefronts left over (more wavefronts than distinct starts) get empty
+// runs at the end.  A group's chunks are then run by consecutive
+// wavefronts, each of which writes its staging segment.  The cost offset of
+// chunk m is costOff[m] (total costTotal) when given, else its work offset
+// plus chunkWeight per earlier chunk.
 __global__ __launch_bounds__(1024) void waveScheduleKernel(int numWaves, int nchunk, const int4* __restrict__ chunks,
                                                           int nwork, int chunkWeight, int ngroups,
-                                                          const int4* __restrict__ groups, int2* __restrict__ out)
+                                                          const int4* __restrict__ groups, const int* __restrict__ costOff,
+                                                          const int* __restrict__ costTotal, int genWaves,
+                                                          int2* __restrict__ out)
 {
     __shared__ int sLo[1024];
     __shared__ int sFlag[1024];
-    const long long total = static_cast<long long>(nwork) + static_cast<long long>(chunkWeight) * nchunk;
+    const long long total  = costOff ? static_cast<long long>(*costTotal)
+                                     : static_cast<long long>(nwork) + static_cast<long long>(chunkWeight) * nchunk;
+    const long long wTotal = generationWeight(numWaves, numWaves, genWaves);
     int             carry = 0, prevLo = -1;
     for (int base = 0; base < numWaves; base += 1024)
     {
@@ -683,13 +729,15 @@ __global__ __launch_bounds__(1024) void waveScheduleKernel(int numWaves, int nch
         int       lo = nchunk;
         if (w < numWaves)
         {
-            const long long target = total * w / numWaves;
+            const long long target = total * generationWeight(w, numWaves, genWaves) / wTotal;
             int             hi     = nchunk;
             lo                     = 0;
             while (lo < hi)
             {
-                const int m = (lo + hi) / 2;
-                if (chunks[m].z + static_cast<long long>(chunkWeight) * m < target)
+                const int       m = (lo + hi) / 2;
+                const long long c = costOff ? static_cast<long long>(costOff[m])
+                                            : chunks[m].z + static_cast<long long>(chunkWeight) * m;
+                if (c < target)
                 {
                     lo = m + 1;
                 }
@@ -820,9 +868,19 @@ void launchChunkShifts(hipStream_t s, int nchunk, const int4* chunks, const floa
 }
 
 void launchWaveSchedule(hipStream_t s, int numWaves, int nchunk, const int4* chunks, int nwork, int chunkWeight,
-                        int ngroups, const int4* groups, int2* out)
+                        int ngroups, const int4* groups, int2* out, const int* costOff, const int* costTotal,
+                        int genWaves)
 {
-    waveScheduleKernel<<<1, 1024, 0, s>>>(numWaves, nchunk, chunks, nwork, chunkWeight, ngroups, groups, out);
+    waveScheduleKernel<<<1, 1024, 0, s>>>(numWaves, nchunk, chunks, nwork, chunkWeight, ngroups, groups, costOff,
+                                          costTotal, genWaves, out);
+}
+
+void launchChunkCosts(hipStream_t s, int nchunk, const int4* chunks, const int* type, const int* typeLJ, int* cost)
+{
+    if (nchunk > 0)
+    {
+        chunkCostKernel<<<(nchunk + 255) / 256, 256, 0, s>>>(nchunk, chunks, type, typeLJ, cost);
+    }
 }
 
 void launchSystemSlots(hipStream_t s, int numSystems, const int* groupOff, int ngroups, const int4* groups,

@@ -63,6 +63,11 @@ constexpr int kChunkEntries = 1 << 20;
 // cost of one chunk (i-entry) in work entries for the static schedule: its
 // i-data set-up and i-force reduction
 constexpr int kChunkWeight = 8;
+// chunk cost model of the static schedule (1/32 tile) and the extra share of
+// the first resident generation of workgroups (1/1000)
+constexpr int kCostTile            = 32;
+constexpr int kCostChunk           = 27;
+constexpr int kGenerationSkewMilli = 45;
 // wavefronts per nonbonded workgroup, and the LDS a workgroup may use (two
 // resident workgroups per CU of 160 KiB, eight wavefronts); the adjoint may
 // use a CU's whole LDS
@@ -313,13 +318,20 @@ void launchGatherList(hipStream_t s, int ncluster, const int* occOff, const int2
 // so that the pair kernel loads it together with the chunk descriptor.
 void launchChunkShifts(hipStream_t s, int nchunk, const int4* chunks, const float4* shiftVec, float4* out);
 // Static schedule of the pair kernel: the chunk list (in group order) cut into
-// numWaves contiguous runs of about equal cost (work entries plus
-// chunkWeight per chunk).  out[w] = {first chunk, group of that chunk} for
-// w in [0, numWaves], out[numWaves] = {nchunk, ngroups}; runs are non-empty
-// except for the trailing wavefronts when there are more wavefronts than
-// distinct starts.
+// numWaves contiguous runs of about equal cost (the exclusive scan costOff of
+// launchChunkCosts' per-chunk costs, total *costTotal, when costOff is given;
+// else work entries plus chunkWeight per chunk).  With genWaves > 0 and
+// numWaves = 2 * genWaves (two workgroups per CU, genWaves wavefronts per
+// resident generation), the first generation takes a larger share.
+// out[w] = {first chunk, group of that chunk} for w in [0, numWaves],
+// out[numWaves] = {nchunk, ngroups}; runs are non-empty except for the
+// trailing wavefronts when there are more wavefronts than distinct starts.
 void launchWaveSchedule(hipStream_t s, int numWaves, int nchunk, const int4* chunks, int nwork, int chunkWeight,
-                        int ngroups, const int4* groups, int2* out);
+                        int ngroups, const int4* groups, int2* out, const int* costOff = nullptr,
+                        const int* costTotal = nullptr, int genWaves = 0);
+// Per-chunk pair-kernel cost (1/32 tile units) from its tiles and the LJ
+// i-atoms of its cluster: type per cluster slot, typeLJ per type.
+void launchChunkCosts(hipStream_t s, int nchunk, const int4* chunks, const int* type, const int* typeLJ, int* cost);
 // Energy-slot range of each system of a batch: slotOff[k] for k in [0, R]
 // (the kernel's slot g + w of the first chunk of the system's first group).
 void launchSystemSlots(hipStream_t s, int numSystems, const int* groupOff, int ngroups, const int4* groups,
