@@ -358,8 +358,8 @@ def bench_flavours(s, stream, reps=10):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="water288k", choices=["water288k", "water9k", "chains150k"])
     ap.add_argument("--ewald", default="table", choices=["table", "analytical"],
                     help="Ewald correction: the reference's spline table or the analytical form")
