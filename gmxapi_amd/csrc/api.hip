@@ -1485,6 +1485,28 @@ static void enqueueStep(nbx_context* c, unsigned flags, const float* xd, float* 
     a.energyPartial = c->energyPartial.p;
     a.waveEnergy    = 1;
     a.fshiftPartial = c->fshiftPartial.p;
+    if (energy)
+    {
+        ExclusionArgs& e = a.excl;
+        e.natoms         = c->natoms;
+        e.nslot          = c->ncluster * kClusterSize;
+        e.npairs         = c->numExclPairs;
+        e.pairSlots      = c->exclPairSlots.p;
+        e.xq             = c->xq.p;
+        e.box[0]         = c->g.box[0];
+        e.box[1]         = c->g.box[1];
+        e.box[2]         = c->g.box[2];
+        e.elec           = c->elec;
+        e.ic             = c->ic;
+        e.rc2            = static_cast<double>(c->ic.rcoulomb2);
+        e.epsfacD        = static_cast<double>(c->ic.epsfac);
+        e.selfCoef       = c->selfCoef;
+        e.tabD           = c->tabFDV0d.p;
+        e.partial        = c->exclPartial.p;
+        // one workgroup per CU: each takes a few items per thread
+        const int items = std::max(e.nslot, e.npairs);
+        a.exclBlocks    = std::max(1, std::min({ kExclMaxBlocks, c->numCU, (items + 255) / 256 }));
+    }
     const bool timed = 2 * static_cast<size_t>(c->timingRecorded) < c->timingEvents.size();
     check(launchNonbonded(c->stream, a, c->elec, energy, shift, c->lay.numBlocks,
                           timed ? c->timingEvents[2 * c->timingRecorded] : nullptr,
@@ -1494,31 +1516,12 @@ static void enqueueStep(nbx_context* c, unsigned flags, const float* xd, float* 
     {
         c->timingRecorded++;
     }
-    ExclusionArgs e{};
-    if (energy)
-    {
-        e.natoms    = c->natoms;
-        e.nslot     = c->ncluster * kClusterSize;
-        e.npairs    = c->numExclPairs;
-        e.pairSlots = c->exclPairSlots.p;
-        e.xq        = c->xq.p;
-        e.box[0]    = c->g.box[0];
-        e.box[1]    = c->g.box[1];
-        e.box[2]    = c->g.box[2];
-        e.elec      = c->elec;
-        e.ic        = c->ic;
-        e.rc2       = static_cast<double>(c->ic.rcoulomb2);
-        e.epsfacD   = static_cast<double>(c->ic.epsfac);
-        e.selfCoef  = c->selfCoef;
-        e.tabD      = c->tabFDV0d.p;
-        e.partial   = c->exclPartial.p;
-    }
     // the gather's workgroups also sum the virial's x (x) f term
     const int numVir =
             launchGatherForces(c->stream, c->ncluster, c->lay.gather.gposOff.p, c->lay.gather.gpos.p,
                                c->lay.gather.staging.p, useFnb ? c->fnb.p : nullptr, c->atomIndices.p, fOut,
                                (flags & NBX_COMPUTE_ADD_FORCES) != 0, c->xq.p, virial ? c->virialPartial.p : nullptr);
-    const int numExcl = energy ? launchExclusionPairEnergy(c->stream, e) : 0;
+    const int numExcl = energy ? a.exclBlocks : 0;
     if (energy || shift || virial)
     {
         launchFinalize(c->stream, energy ? numSlots : 0, c->energyPartial.p, energy ? numExcl : 0,

@@ -13,14 +13,12 @@
 //   virial      -1/2 sum x (x) f - 1/2 sum shift (x) fshift in double
 //               (calcvir.cpp:103-148, sim_util.cpp:157-177)
 //   exclusion   the Coulomb energy terms the pair kernel leaves out, in
-//   energies    double: the self energy -1/2 epsfac q_i^2 V(0) of every atom
-//               (kernel_ref_outer.h:286-326) and, for every excluded pair
-//               inside the cut-off, -epsfac q_i q_j V_corr(r) (Ewald) or
-//               epsfac q_i q_j (k_rf r^2 - c_rf) (reaction field)
-//               (kernel_ref_inner.h: excluded pairs keep the correction)
+//   energies    double (exclusion.h), per atom for batches; a single
+//               system's pass runs in the pair kernel's launch
 // All are HBM-bandwidth bound: one coalesced pass over the arrays.
 #include <algorithm>
 
+#include "exclusion.h"
 #include "kernel_common.h"
 
 namespace nbx
@@ -249,22 +247,6 @@ __global__ __launch_bounds__(kFinalizeThreads) void finalizeKernel(int numEnergy
 
 constexpr int kExclThreads = 256;
 
-// V_corr(r) = erf(beta r)/r in double: the double build's table (the pair
-// kernel's interpolation formula) or the closed form
-__device__ double ewaldCorrectionD(double r, const ExclusionArgs& a)
-{
-    if (a.elec == kElecEwaldTable)
-    {
-        const double rs    = r * static_cast<double>(a.ic.tab_scale);
-        const int    ri    = min(static_cast<int>(rs), a.ic.tab_n - 2);
-        const double frac  = rs - static_cast<double>(ri);
-        const double f0    = a.tabD[4 * ri];
-        const double fexcl = f0 + frac * a.tabD[4 * ri + 1];
-        return a.tabD[4 * ri + 2] - (0.5 / static_cast<double>(a.ic.tab_scale)) * frac * (f0 + fexcl);
-    }
-    return erf(static_cast<double>(a.ic.beta) * r) / r;
-}
-
 // Atom i's self energy and the Coulomb energies of its excluded partners
 // j > i inside the cut-off, in double.
 __device__ double atomExclusionEnergy(const ExclusionArgs& a, int i)
@@ -385,48 +367,6 @@ __global__ __launch_bounds__(kFinalizeThreads) void batchFinalizeKernel(const in
     }
 }
 
-// Item t of the single-system exclusion pass, in double: the self term
-// -selfCoef q^2 of cluster slot t (fillers carry q = 0) and the energy of
-// excluded pair t inside the cut-off (both atoms as cluster slots, each pair
-// once).  Every load depends only on t or on one list entry.
-__device__ double exclusionItemEnergy(const ExclusionArgs& a, int t)
-{
-    double e = 0.0;
-    if (t < a.nslot)
-    {
-        const double q = a.xq[t].w;
-        e -= a.selfCoef * q * q;
-    }
-    if (t < a.npairs)
-    {
-        const int2   ps   = a.pairSlots[t];
-        const float4 pi   = a.xq[ps.x];
-        const float4 pj   = a.xq[ps.y];
-        double       d[3] = { static_cast<double>(pi.x) - pj.x, static_cast<double>(pi.y) - pj.y,
-                        static_cast<double>(pi.z) - pj.z };
-        double       r2   = 0.0;
-#pragma unroll
-        for (int c = 0; c < 3; c++)
-        {
-            const double L = a.box[c];
-            d[c] -= L * rint(d[c] / L);
-            r2 += d[c] * d[c];
-        }
-        if (r2 < a.rc2)
-        {
-            const double qq = a.epsfacD * static_cast<double>(pi.w) * static_cast<double>(pj.w);
-            if (a.elec == kElecRF)
-            {
-                e += qq * (static_cast<double>(a.ic.k_rf) * r2 - static_cast<double>(a.ic.c_rf));
-            }
-            else
-            {
-                e -= qq * ewaldCorrectionD(sqrt(r2), a);
-            }
-        }
-    }
-    return e;
-}
 
 // Forces of each cluster from the staged windows that hold it: a wavefront
 // per cluster, lane l adding copies l, l + 64, ... (12 floats each), then a
@@ -648,33 +588,6 @@ __global__ void expandPairsKernel(const int4* __restrict__ ientry, int nci, cons
     }
 }
 
-// Self energies and the energies of excluded pairs inside the cut-off for one
-// system (exclusionItemEnergy over the threads), one partial per workgroup.
-__global__ __launch_bounds__(kExclThreads) void exclusionPairEnergyKernel(const ExclusionArgs a)
-{
-    __shared__ double s[kExclThreads / 64];
-    double            e = 0.0;
-    const int         n = max(a.nslot, a.npairs);
-    for (int t = blockIdx.x * kExclThreads + threadIdx.x; t < n; t += gridDim.x * kExclThreads)
-    {
-        e += exclusionItemEnergy(a, t);
-    }
-    e = waveSumD(e);
-    if ((threadIdx.x & 63) == 0)
-    {
-        s[threadIdx.x / 64] = e;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0)
-    {
-        double t = 0.0;
-        for (int w = 0; w < kExclThreads / 64; w++)
-        {
-            t += s[w];
-        }
-        a.partial[blockIdx.x] = t;
-    }
-}
 
 } // namespace
 
@@ -726,13 +639,6 @@ int launchGatherForces(hipStream_t s, int ncluster, const int* gposOff, const in
     return blocks;
 }
 
-int launchExclusionPairEnergy(hipStream_t s, const ExclusionArgs& a)
-{
-    const int n      = std::max(a.nslot, a.npairs);
-    const int blocks = std::max(1, std::min(kExclMaxBlocks, (n + kExclThreads - 1) / kExclThreads));
-    exclusionPairEnergyKernel<<<blocks, kExclThreads, 0, s>>>(a);
-    return blocks;
-}
 
 void launchSetCharges(hipStream_t s, int nslot, const int* atomIndices, const float* q, float4* xq)
 {

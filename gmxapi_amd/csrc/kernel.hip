@@ -64,6 +64,7 @@
 #include <utility>
 
 #include "../../include/nbx.h"
+#include "exclusion.h"
 #include "kernel_common.h"
 
 namespace nbx
@@ -309,6 +310,15 @@ template<int kElec, bool kEnergy, bool kShift, int kLJ>
 __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
 {
     extern __shared__ float4 smem4[];
+    if (kEnergy && a.exclBlocks > 0 && static_cast<int>(blockIdx.x) >= static_cast<int>(gridDim.x) - a.exclBlocks)
+    {
+        // the single-system exclusion pass: these workgroups are dispatched
+        // after the pair workgroups and run where those have finished
+        const int first = static_cast<int>(gridDim.x) - a.exclBlocks;
+        exclusionPairBlock(a.excl, static_cast<int>(blockIdx.x) - first, a.exclBlocks, kThreads,
+                           reinterpret_cast<double*>(smem4));
+        return;
+    }
     const int       ntype  = a.ntype;
     const int       ntype2 = ntype * ntype;
     // a zero entry and the table
@@ -624,6 +634,8 @@ hipError_t launchOne(hipStream_t s, const KernelArgs& a, int numBlocks, int lds,
     {
         return e;
     }
+    // the exclusion pass's workgroups follow the pair workgroups
+    numBlocks += kEnergy ? a.exclBlocks : 0;
     if (start || stop)
     {
         void* args[] = { const_cast<KernelArgs*>(&a) };

@@ -338,6 +338,34 @@ void launchSystemSlots(hipStream_t s, int numSystems, const int* groupOff, int n
                        int numWaves, const int2* waveStart, int* slotOff);
 
 // -- kernel.hip
+// the self energy and excluded-pair Coulomb energies in double
+struct ExclusionArgs
+{
+    int              natoms;
+    const int*       cells;
+    const float4*    xq;
+    const int*       exclOff;
+    const int*       exclIdx;
+    float            box[3];
+    int              elec;
+    InteractionConst ic;
+    double           rc2;      // the float rcoulomb^2, as double
+    double           epsfacD;  // the float epsfac, as double
+    double           selfCoef; // epsfac V(0) / 2
+    const double*    tabD;     // FDV0 table in double (table flavour)
+    double*          partial;  // one per block
+    // batches: the atom's system (boxes[3 * system]) and per-atom energies
+    // instead of block partials; nullptr for a single system
+    const int*       atomSystem;
+    const float*     boxes;
+    double*          atomEnergy;
+    // single system, pair form: nslot cluster slots (self terms) and npairs
+    // excluded pairs as cluster slots
+    int              nslot;
+    int              npairs;
+    const int2*      pairSlots;
+};
+
 struct KernelArgs
 {
     const int4*   chunks;
@@ -373,7 +401,12 @@ struct KernelArgs
     // g + w (per-system sums of a batch)
     double*  energyPartial;
     int      waveEnergy;
-    float*   fshiftPartial; // [gridDim.x][45*3]
+    float*   fshiftPartial; // [gridDim.x - exclBlocks][45*3]
+    // with energies: the last exclBlocks workgroups of the launch run the
+    // single-system exclusion pass instead (partials excl.partial[0,
+    // exclBlocks)), in the tail the pair workgroups leave; 0 for none
+    int           exclBlocks;
+    ExclusionArgs excl;
 };
 // Lets kernel fn use lds bytes of dynamic LDS (above the 64 KiB default) on
 // the current device.  hipFuncSetAttribute is a host call that can hold up
@@ -455,38 +488,9 @@ void launchFinalize(hipStream_t s, int numEnergyPartials, const double* energyPa
                     const double* exclPartial, int numShiftPartials, const float* fshiftPartial, bool shiftForces,
                     double* energyOut, double* fshiftOut, int numVirPartials, const double* virPartial,
                     const float4* shiftVec, double* virialOut);
-// the self energy and excluded-pair Coulomb energies in double
-struct ExclusionArgs
-{
-    int              natoms;
-    const int*       cells;
-    const float4*    xq;
-    const int*       exclOff;
-    const int*       exclIdx;
-    float            box[3];
-    int              elec;
-    InteractionConst ic;
-    double           rc2;      // the float rcoulomb^2, as double
-    double           epsfacD;  // the float epsfac, as double
-    double           selfCoef; // epsfac V(0) / 2
-    const double*    tabD;     // FDV0 table in double (table flavour)
-    double*          partial;  // one per block
-    // batches: the atom's system (boxes[3 * system]) and per-atom energies
-    // instead of block partials; nullptr for a single system
-    const int*       atomSystem;
-    const float*     boxes;
-    double*          atomEnergy;
-    // single system, pair form: nslot cluster slots (self terms) and npairs
-    // excluded pairs as cluster slots
-    int              nslot;
-    int              npairs;
-    const int2*      pairSlots;
-};
 constexpr int kExclMaxBlocks = 1024;
 // per-atom form (batches: per-atom energies or block partials)
 int launchExclusionEnergy(hipStream_t s, const ExclusionArgs& a);
-// pair form (one system): block partials; returns their count
-int launchExclusionPairEnergy(hipStream_t s, const ExclusionArgs& a);
 // the excluded pairs (atom indices, each pair once) as cluster slots
 void launchExclusionSlots(hipStream_t s, int npairs, const int2* pairs, const int* cells, int2* pairSlots);
 void launchBatchFinalize(hipStream_t s, int numSystems, const int* slotOff, const double* energyPartial,
