@@ -325,8 +325,10 @@ void buildWorkGroups(nbx_context* c, WorkLayout& L, bool adjoint, const int* spa
     {
         check(hipMemsetAsync(c->groupStats.p, 0, sizeof(int) * kStats, c->stream), "clear stats");
         launchGroupSpans(c->stream, nc, c->offJ.p, c->groupStats.p);
-        check(hipMemcpyAsync(spans, c->groupStats.p, sizeof(spans), hipMemcpyDeviceToHost, c->stream), "read spans");
+        int* const hp = pinnedInts();
+        check(hipMemcpyAsync(hp, c->groupStats.p, sizeof(spans), hipMemcpyDeviceToHost, c->stream), "read spans");
         check(hipStreamSynchronize(c->stream), "group sync");
+        std::copy(hp, hp + kGroupSizeCount, spans);
     }
     c->jentryTmp.ensure(c->ncj);
     // the largest group size (or the chosen one) whose j-force windows, one
@@ -413,9 +415,11 @@ void buildWorkGroups(nbx_context* c, WorkLayout& L, bool adjoint, const int* spa
         launchExclusiveScan(c->stream, ng, L.unionCount.p, L.unionOff.p, c->groupStats.p + kGroupSizeCount + 2);
         launchExclusiveScan(c->stream, ng, L.chunkCount.p, L.chunkOff.p, c->groupStats.p + kGroupSizeCount + 3);
         launchExclusiveScan(c->stream, ng, L.workCount.p, L.workOff.p, c->groupStats.p + kGroupSizeCount + 4);
-        int st[5];
-        check(hipMemcpyAsync(st, c->groupStats.p + kGroupSizeCount, sizeof(st), hipMemcpyDeviceToHost, c->stream), "read stats");
+        int* const hp = pinnedInts();
+        check(hipMemcpyAsync(hp, c->groupStats.p + kGroupSizeCount, 5 * sizeof(int), hipMemcpyDeviceToHost, c->stream),
+              "read stats");
         check(hipStreamSynchronize(c->stream), "group sync");
+        const int st[5] = { hp[0], hp[1], hp[2], hp[3], hp[4] };
         const int maxUnion = st[0], overflow = st[1], unionTotal = st[2], chunkTotal = st[3], workTotal = st[4];
         if (overflow)
         {
@@ -1117,12 +1121,12 @@ int nbx_put_on_grid(nbx_context* c, const float* box, const float* x, int xOnDev
         check(hipMemsetAsync(c->cxy_na.p, 0, sizeof(int) * (g.ncol + 1), c->stream), "clear counts");
         launchGridColumns(c->stream, g, c->x.p, c->atomCol.p, c->atomRank.p, c->cxy_na.p);
         launchGridScan(c->stream, g.ncol, c->cxy_na.p, c->cxy_ind.p, c->counter.p);
-        int hostCounts[3] = { 0, 0, 0 };
-        check(hipMemcpyAsync(&hostCounts[0], c->counter.p, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream),
-              "read count");
-        check(hipMemcpyAsync(&hostCounts[2], c->cxy_na.p + g.ncol, sizeof(int), hipMemcpyDeviceToHost, c->stream),
+        int* const hp = pinnedInts();
+        check(hipMemcpyAsync(hp, c->counter.p, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream), "read count");
+        check(hipMemcpyAsync(hp + 2, c->cxy_na.p + g.ncol, sizeof(int), hipMemcpyDeviceToHost, c->stream),
               "read flag");
         check(hipStreamSynchronize(c->stream), "grid sync");
+        const int hostCounts[3] = { hp[0], hp[1], hp[2] };
         if (hostCounts[2] != 0)
         {
             c->gridValid = false;
@@ -1144,19 +1148,9 @@ int nbx_put_on_grid(nbx_context* c, const float* box, const float* x, int xOnDev
         c->nbtype.ensure(nslot + kClusterSize);
         c->bb.ensure(2 * static_cast<size_t>(c->ncluster));
         c->fnb.ensure(nslot + kClusterSize);
+        // the fill also writes the far-away filler cluster after the real ones
         launchGridFill(c->stream, c->ncluster, c->x.p, c->atomIndices.p, c->type.p, c->charge.p, c->ntype - 1, c->xq.p,
                        c->nbtype.p, c->bb.p);
-        {
-            static const float4 far[kClusterSize] = { { kFarAway, kFarAway, kFarAway, 0.0f },
-                                                      { kFarAway, kFarAway, kFarAway, 0.0f },
-                                                      { kFarAway, kFarAway, kFarAway, 0.0f },
-                                                      { kFarAway, kFarAway, kFarAway, 0.0f } };
-            const int pad[kClusterSize] = { c->ntype - 1, c->ntype - 1, c->ntype - 1, c->ntype - 1 };
-            check(hipMemcpyAsync(c->xq.p + nslot, far, sizeof(far), hipMemcpyHostToDevice, c->stream), "dummy x");
-            check(hipMemcpyAsync(c->nbtype.p + nslot, pad, sizeof(pad), hipMemcpyHostToDevice, c->stream),
-                  "dummy type");
-            check(hipStreamSynchronize(c->stream), "dummy sync");
-        }
         check(hipGetLastError(), "grid kernels");
         c->gridValid = true;
         c->listValid = false;
@@ -1221,8 +1215,15 @@ int nbx_construct_pairlist(nbx_context* c)
             c->groupStats.ensure(kGroupSizeCount + 5);
             check(hipMemsetAsync(c->groupStats.p, 0, sizeof(int) * kGroupSizeCount, c->stream), "clear spans");
             launchGroupSpans(c->stream, nc, c->offJ.p, c->groupStats.p);
-            check(hipMemcpyAsync(spans, c->groupStats.p, sizeof(spans), hipMemcpyDeviceToHost, c->stream),
+            check(hipMemcpyAsync(pinnedInts() + 8, c->groupStats.p, sizeof(spans), hipMemcpyDeviceToHost,
+                                 c->stream),
                   "read spans");
+        };
+        // after the wait: the spans and the list totals from the pinned buffer
+        auto takeReads = [&](int* tot) {
+            const int* hp = pinnedInts();
+            std::copy(hp + 8, hp + 8 + kGroupSizeCount, spans);
+            std::copy(hp, hp + 3, tot);
         };
         // After a first search, the fill runs without a count pass: each
         // cluster writes into a region sized from the largest count seen
@@ -1244,8 +1245,10 @@ int nbx_construct_pairlist(nbx_context* c)
             launchMaxReduce(c->stream, nc, c->countJ.p, c->totals.p + 2);
             readSpans();
             int tot[3] = { 0, 0, 0 };
-            check(hipMemcpyAsync(tot, c->totals.p, sizeof(tot), hipMemcpyDeviceToHost, c->stream), "read totals");
+            check(hipMemcpyAsync(pinnedInts(), c->totals.p, sizeof(tot), hipMemcpyDeviceToHost, c->stream),
+                  "read totals");
             check(hipStreamSynchronize(c->stream), "search sync");
+            takeReads(tot);
             if (tot[2] <= cap)
             {
                 c->ncj = tot[0];
@@ -1270,8 +1273,10 @@ int nbx_construct_pairlist(nbx_context* c)
             launchMaxReduce(c->stream, nc, c->countJ.p, c->totals.p + 2);
             readSpans();
             int tot[3] = { 0, 0, 0 };
-            check(hipMemcpyAsync(tot, c->totals.p, sizeof(tot), hipMemcpyDeviceToHost, c->stream), "read totals");
+            check(hipMemcpyAsync(pinnedInts(), c->totals.p, sizeof(tot), hipMemcpyDeviceToHost, c->stream),
+                  "read totals");
             check(hipStreamSynchronize(c->stream), "search sync");
+            takeReads(tot);
             c->ncj = tot[0];
             c->nci = tot[1];
             c->ientry.ensure(c->nci);

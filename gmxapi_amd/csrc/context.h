@@ -41,6 +41,24 @@ inline void check(hipError_t e, const char* what)
     }
 }
 
+// The calling thread's pinned host buffer (kPinnedInts ints, allocated on
+// first use and kept for the thread's life) for the list build's small
+// device-to-host reads: a copy into pageable memory waits for its own staging
+// copy, one into pinned memory is queued, so several reads share one
+// stream synchronisation.  Callers use disjoint ranges within one wait.
+constexpr int kPinnedInts = 64;
+inline int* pinnedInts()
+{
+    thread_local int* p = nullptr;
+    if (!p)
+    {
+        void* q = nullptr;
+        check(hipHostMalloc(&q, sizeof(int) * kPinnedInts, hipHostMallocDefault), "hipHostMalloc");
+        p = static_cast<int*>(q);
+    }
+    return p;
+}
+
 template<typename T>
 struct DevBuf
 {
@@ -112,9 +130,10 @@ inline void buildOccurrences(hipStream_t s, WindowGather& w, int ngroups, const 
     int total = maxTotal;
     if (total < 0)
     {
-        check(hipMemcpyAsync(&total, w.occOff.p + ncluster, sizeof(int), hipMemcpyDeviceToHost, s),
-              "read occurrences");
+        int* const h = pinnedInts();
+        check(hipMemcpyAsync(h, w.occOff.p + ncluster, sizeof(int), hipMemcpyDeviceToHost, s), "read occurrences");
         check(hipStreamSynchronize(s), "occurrence sync");
+        total = h[0];
     }
     w.occ.ensure(std::max(total, 1));
     check(hipMemsetAsync(w.occCount.p, 0, sizeof(int) * ncluster, s), "clear occurrences");
@@ -138,8 +157,10 @@ inline void buildSegments(hipStream_t s, WindowGather& w, int ngroups, const int
     {
         launchSegments(s, ngroups, groups, groupInfo, numWaves, waveStart, slotF4, w.segSize.p, w.segBase.p,
                        w.segBase.p + ngroups, w.segInfo.p);
-        check(hipMemcpyAsync(&total, w.segBase.p + ngroups, sizeof(int), hipMemcpyDeviceToHost, s), "read segments");
+        int* const h = pinnedInts();
+        check(hipMemcpyAsync(h, w.segBase.p + ngroups, sizeof(int), hipMemcpyDeviceToHost, s), "read segments");
         check(hipStreamSynchronize(s), "segment sync");
+        total = h[0];
     }
     w.staging.ensure(std::max(total, 1));
     // the staging positions each cluster's gather reads

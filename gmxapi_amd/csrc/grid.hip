@@ -231,6 +231,16 @@ __global__ void gridFillKernel(int ncluster, const float* __restrict__ x, const 
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncluster)
     {
+        // one extra cluster of far-away fillers after the real ones (the
+        // pair kernel points lanes past the end of a j-list at it)
+        if (c == ncluster)
+        {
+            for (int k = 0; k < kClusterSize; k++)
+            {
+                xq[c * kClusterSize + k]     = make_float4(kFarAway, kFarAway, kFarAway, 0.0f);
+                nbtype[c * kClusterSize + k] = paddingType;
+            }
+        }
         return;
     }
     float lo[3] = { 0, 0, 0 }, hi[3] = { 0, 0, 0 };
@@ -297,12 +307,8 @@ void launchGridSortColumns(hipStream_t s, const GridDims& g, const float* x, con
 void launchGridFill(hipStream_t s, int ncluster, const float* x, const int* atomIndices, const int* type, const float* q,
                     int paddingType, float4* xq, int* nbtype, float4* bb)
 {
-    if (ncluster == 0)
-    {
-        return;
-    }
-    gridFillKernel<<<(ncluster + 127) / 128, 128, 0, s>>>(ncluster, x, atomIndices, type, q, paddingType, xq, nbtype,
-                                                          bb);
+    gridFillKernel<<<(ncluster + 1 + 127) / 128, 128, 0, s>>>(ncluster, x, atomIndices, type, q, paddingType, xq,
+                                                              nbtype, bb);
 }
 
 } // namespace nbx
