@@ -5,6 +5,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -338,6 +339,10 @@ void buildWorkGroups(nbx_context* c, WorkLayout& L, bool adjoint, const int* spa
     // global atomics (long cut-offs, many types), the last pass takes the
     // smallest (or the chosen) size with whatever spill its window leaves.
     const int forced = adjoint ? 0 : c->opt.kernel_group_size;
+    // NBX_UNION_SORT=1 takes the group pass's sort path for every group
+    // (otherwise only for groups wider than its bitmap): for tests
+    const char* sortEnv     = std::getenv("NBX_UNION_SORT");
+    const bool  unionBySort = sortEnv != nullptr && sortEnv[0] == '1';
     // a layout rebuilt for a new list starts from the size its last build
     // chose: the sizes above it did not fit then and, for a system that
     // moved by a few steps, do not fit now (saves a count pass per search)
@@ -408,6 +413,7 @@ void buildWorkGroups(nbx_context* c, WorkLayout& L, bool adjoint, const int* spa
         ga.groupInfo    = L.groupInfo.p;
         ga.windowCap    = 1 << 30; // uncapped slots; the compaction applies the window capacity
         ga.upperBound   = 1;
+        ga.unionBySort  = unionBySort ? 1 : 0;
         launchBuildGroups(c->stream, ga, true);
         check(hipMemsetAsync(c->groupStats.p + kGroupSizeCount, 0, sizeof(int) * 2, c->stream), "clear stats");
         launchMaxReduce(c->stream, ng, L.unionCount.p, c->groupStats.p + kGroupSizeCount);

@@ -22,6 +22,11 @@
 //     self marks the piece that carries the cluster's Ewald/RF self energy
 //     (first piece of the central entry, kernel_ref_outer.h:286-326),
 //     special counts its leading tiles that hold masked entries.
+// The union comes from a bitmap over the range of the group's j-cluster
+// indices (one bit per cluster, set with LDS atomics; the per-word bit
+// counts scanned give each cluster's rank, so a j-entry's slot is one
+// popcount away), or from a bitonic sort of the keys when the range is wider
+// than the bitmap.
 // Like the search it runs as a count pass, a device scan, and a fill pass.
 #include <algorithm>
 #include <climits>
@@ -131,30 +136,92 @@ __global__ __launch_bounds__(kGroupThreads) void buildGroupsKernel(GroupArgs a)
     const int n  = a.offJ[c1] - jb;
     const int ib = a.offI[c0];
     const int ni = a.offI[c1] - ib;
-    int       p  = 1;
-    while (p < n)
+    // range of the group's other j-clusters: a bitmap over it (in sKeys)
+    // gives the sorted union and each cluster's rank directly; a range
+    // wider than the bitmap takes a bitonic sort of the keys instead
+    __shared__ int sMin[kGroupThreads / 64], sMax[kGroupThreads / 64];
+    int lo = INT_MAX, hi = -1;
+    for (int t = threadIdx.x; t < n; t += kGroupThreads)
     {
-        p <<= 1;
+        const int cj = static_cast<int>(a.jentryIn[jb + t].x);
+        if (cj < c0 || cj >= c1)
+        {
+            lo = min(lo, cj);
+            hi = max(hi, cj);
+        }
     }
-    for (int t = threadIdx.x; t < p; t += kGroupThreads)
+#pragma unroll
+    for (int m = 1; m < 64; m *= 2)
     {
-        int key = INT_MAX;
-        if (t < n)
+        lo = min(lo, __shfl_xor(lo, m, 64));
+        hi = max(hi, __shfl_xor(hi, m, 64));
+    }
+    if ((threadIdx.x & 63) == 0)
+    {
+        sMin[threadIdx.x >> 6] = lo;
+        sMax[threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < kGroupThreads / 64; w++)
+    {
+        lo = min(lo, sMin[w]);
+        hi = max(hi, sMax[w]);
+    }
+    const bool bitmap = !a.unionBySort && (hi < lo || hi - lo < kGroupMaxJ * 32);
+    unsigned*  bm     = reinterpret_cast<unsigned*>(sKeys);
+    const int  nwords = hi < lo ? 0 : ((hi - lo) >> 5) + 1;
+    int        p      = 1;
+    if (bitmap)
+    {
+        for (int w = threadIdx.x; w < nwords; w += kGroupThreads)
+        {
+            bm[w] = 0u;
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < n; t += kGroupThreads)
         {
             const int cj = static_cast<int>(a.jentryIn[jb + t].x);
-            key          = (cj >= c0 && cj < c1) ? INT_MAX : cj;
+            if (cj < c0 || cj >= c1)
+            {
+                atomicOr(bm + ((cj - lo) >> 5), 1u << ((cj - lo) & 31));
+            }
         }
-        sKeys[t] = key;
+        __syncthreads();
+        // sFlag: per word its set-bit count, scanned below into the rank of
+        // the word's first cluster
+        for (int w = threadIdx.x; w < nwords; w += kGroupThreads)
+        {
+            sFlag[w] = __popc(bm[w]);
+        }
+        __syncthreads();
     }
-    __syncthreads();
-    bitonicSort(sKeys, p);
-    // unique: rank of each first occurrence
-    for (int t = threadIdx.x; t < p; t += kGroupThreads)
+    else
     {
-        const int k = sKeys[t];
-        sFlag[t]    = (k != INT_MAX && (t == 0 || sKeys[t - 1] != k)) ? 1 : 0;
+        while (p < n)
+        {
+            p <<= 1;
+        }
+        for (int t = threadIdx.x; t < p; t += kGroupThreads)
+        {
+            int key = INT_MAX;
+            if (t < n)
+            {
+                const int cj = static_cast<int>(a.jentryIn[jb + t].x);
+                key          = (cj >= c0 && cj < c1) ? INT_MAX : cj;
+            }
+            sKeys[t] = key;
+        }
+        __syncthreads();
+        bitonicSort(sKeys, p);
+        // unique: rank of each first occurrence
+        for (int t = threadIdx.x; t < p; t += kGroupThreads)
+        {
+            const int k = sKeys[t];
+            sFlag[t]    = (k != INT_MAX && (t == 0 || sKeys[t - 1] != k)) ? 1 : 0;
+        }
+        __syncthreads();
     }
-    __syncthreads();
     // the group's i-entries (at most kGroupThreads; more is flagged as
     // overflow below): their chunk counts and padded work lengths
     {
@@ -171,7 +238,7 @@ __global__ __launch_bounds__(kGroupThreads) void buildGroupsKernel(GroupArgs a)
     }
     // exclusive scans: ranks of the unique keys, and the chunk and padded
     // work offsets of the i-entries
-    const int unionTotal = blockExclusiveScan(sFlag, p, sWaveTot);
+    const int unionTotal = blockExclusiveScan(sFlag, bitmap ? nwords : p, sWaveTot);
     const int chunkTotal = blockExclusiveScan(sChunkBase, kGroupThreads, sWaveTot);
     const int workTotal  = blockExclusiveScan(sWorkBase, kGroupThreads, sWaveTot);
     if (threadIdx.x == 0)
@@ -205,18 +272,32 @@ __global__ __launch_bounds__(kGroupThreads) void buildGroupsKernel(GroupArgs a)
     // work at most n + (kTileJ - 1) ni, so bases from the cluster offsets
     // never overlap; groupCompactKernel moves them together afterwards
     const int uoff = a.upperBound ? jb : a.unionOff[g];
-    for (int t = threadIdx.x; t < p; t += kGroupThreads)
+    if (bitmap)
     {
-        const int k = sKeys[t];
-        if (k != INT_MAX && (t == 0 || sKeys[t - 1] != k))
+        for (int w = threadIdx.x; w < nwords; w += kGroupThreads)
         {
-            a.unionList[uoff + sFlag[t]] = k;
+            int r = sFlag[w];
+            for (unsigned bits = bm[w]; bits; bits &= bits - 1u)
+            {
+                a.unionList[uoff + r++] = lo + 32 * w + (__ffs(bits) - 1);
+            }
         }
     }
-    __syncthreads();
-    for (int t = threadIdx.x; t < U; t += kGroupThreads)
+    else
     {
-        sKeys[t] = a.unionList[uoff + t];
+        for (int t = threadIdx.x; t < p; t += kGroupThreads)
+        {
+            const int k = sKeys[t];
+            if (k != INT_MAX && (t == 0 || sKeys[t - 1] != k))
+            {
+                a.unionList[uoff + sFlag[t]] = k;
+            }
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < U; t += kGroupThreads)
+        {
+            sKeys[t] = a.unionList[uoff + t];
+        }
     }
     __syncthreads();
     const int coff = a.upperBound ? ib : a.chunkOff[g];
@@ -265,17 +346,28 @@ __global__ __launch_bounds__(kGroupThreads) void buildGroupsKernel(GroupArgs a)
                     }
                     else
                     {
-                        int l = 0, r = U;
-                        while (l < r)
+                        // the cluster's rank in the sorted union
+                        int l;
+                        if (bitmap)
                         {
-                            const int m = (l + r) >> 1;
-                            if (sKeys[m] < cj)
+                            const int d = cj - lo;
+                            l           = sFlag[d >> 5] + __popc(bm[d >> 5] & ((1u << (d & 31)) - 1u));
+                        }
+                        else
+                        {
+                            l     = 0;
+                            int r = U;
+                            while (l < r)
                             {
-                                l = m + 1;
-                            }
-                            else
-                            {
-                                r = m;
+                                const int m = (l + r) >> 1;
+                                if (sKeys[m] < cj)
+                                {
+                                    l = m + 1;
+                                }
+                                else
+                                {
+                                    r = m;
+                                }
                             }
                         }
                         slot = l < Uw ? a.groupSize + l : static_cast<int>(kSlotOverflow);

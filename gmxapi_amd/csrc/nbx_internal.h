@@ -285,6 +285,9 @@ struct GroupArgs
     // layout at upper-bound bases (union at offJ[c0], chunks at offI[c0],
     // work at offJ[c0] + (kTileJ - 1) offI[c0]) for launchGroupCompact
     int          upperBound;
+    // 1: every group's union by a bitonic sort of its keys (the path taken
+    // by groups whose j-clusters span more than the bitmap covers), for tests
+    int          unionBySort;
 };
 // window slot of a j-entry whose cluster did not fit the group's window:
 // the kernel adds its forces with global atomics

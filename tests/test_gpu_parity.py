@@ -97,6 +97,25 @@ def test_rebuilt_lists_match_the_oracle():
         assert rms(f - r64["f"]) / rms(r64["f"]) <= FORCE_TOL_RMS
 
 
+def test_group_union_paths_agree(water3000, monkeypatch):
+    # the group pass builds each group's union from a bitmap over its
+    # j-cluster range, or by sorting the keys when the range is too wide;
+    # both must give the same work layout (union order, slots, chunks)
+    print("[groups] bitmap and sort unions, 9000 atoms", flush=True)
+    s = water3000
+    layouts = []
+    for sort in ("0", "1"):
+        monkeypatch.setenv("NBX_UNION_SORT", sort)
+        ctx = nbx.context_for(s)
+        layouts.append((ctx.work_groups(), ctx.compute(s.x)))
+    (a, fa), (b, fb) = layouts
+    for key in ("group_size", "num_groups", "num_chunks", "window_slots"):
+        assert a[key] == b[key], key
+    for key in ("groups", "chunks", "union", "work"):
+        np.testing.assert_array_equal(a[key], b[key], err_msg=key)
+    np.testing.assert_allclose(fa, fb, rtol=1e-5, atol=1e-3)
+
+
 def test_pair_set_config0(water3000):
     print("[pairs] config0 9000 atoms", flush=True)
     s = water3000
