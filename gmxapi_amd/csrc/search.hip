@@ -74,6 +74,42 @@ __device__ __forceinline__ int wavePrefixExclusive(int v, int lane, int* total)
     return x - v;
 }
 
+// The first m in [s, e) with pred(m) true (e if none) for a predicate that
+// is false then true along the range: up to three probes stepping from the
+// guess (an exact guess settles in two), then bisection of what is left.
+template<typename P>
+__device__ __forceinline__ int firstTrue(int s, int e, int guess, P pred)
+{
+    int lo = s, hi = e; // the answer lies in [lo, hi]
+    int m  = min(max(guess, s), e - 1);
+    for (int it = 0; it < 3 && lo < hi && m >= lo && m < hi; it++)
+    {
+        if (pred(m))
+        {
+            hi = m;
+            m--;
+        }
+        else
+        {
+            lo = m + 1;
+            m++;
+        }
+    }
+    while (lo < hi)
+    {
+        const int mid = (lo + hi) >> 1;
+        if (pred(mid))
+        {
+            hi = mid;
+        }
+        else
+        {
+            lo = mid + 1;
+        }
+    }
+    return lo;
+}
+
 template<bool kFill>
 __global__ __launch_bounds__(kWave* kSearchWaves) void searchKernel(SearchArgs a)
 {
@@ -198,37 +234,15 @@ __global__ __launch_bounds__(kWave* kSearchWaves) void searchKernel(SearchArgs a
                         const int col = cx * g.ncy + cy;
                         int       s   = a.cxy_ind[col];
                         int       e   = a.cxy_ind[col + 1];
-                        // first cluster with upper z >= zlo
-                        int l = s, r = e;
-                        while (l < r)
-                        {
-                            const int m = (l + r) >> 1;
-                            if (a.bb[2 * m + 1].z < zlo)
-                            {
-                                l = m + 1;
-                            }
-                            else
-                            {
-                                r = m;
-                            }
-                        }
-                        const int first = l;
-                        // first cluster with lower z > zhi
-                        l = first;
-                        r = e;
-                        while (l < r)
-                        {
-                            const int m = (l + r) >> 1;
-                            if (a.bb[2 * m].z <= zhi)
-                            {
-                                l = m + 1;
-                            }
-                            else
-                            {
-                                r = m;
-                            }
-                        }
-                        int last = l; // exclusive
+                        // the column's clusters are sorted by z (both bounds
+                        // non-decreasing): the first with upper z >= zlo and
+                        // the first with lower z > zhi, each searched from
+                        // where an even spread over the box height puts it
+                        const float perZ  = static_cast<float>(e - s) * g.invBoxZ;
+                        const int   first = firstTrue(s, e, s + static_cast<int>(zlo * perZ),
+                                                      [&](int m) { return a.bb[2 * m + 1].z >= zlo; });
+                        int last = firstTrue(first, e, s + static_cast<int>(zhi * perZ) + 1,
+                                             [&](int m) { return a.bb[2 * m].z > zhi; }); // exclusive
                         k0       = first;
                         if (central)
                         {
