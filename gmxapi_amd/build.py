@@ -15,6 +15,8 @@ import glob
 import os
 import shutil
 import subprocess
+import tempfile
+import concurrent.futures
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -71,24 +73,34 @@ def build_nblib(verbose: bool = False) -> str:
 
 
 def build(verbose: bool = False) -> str:
-    """Compile the library; returns its path."""
+    """Compile the library; returns its path.  Each source file is compiled
+    on its own (in parallel, one process per file up to the CPU count), then
+    the objects are linked."""
     os.makedirs(LIBDIR, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
     out = library_path()
     tmp = out + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
-           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-munsafe-fp-atomics", "-Wall", "-Wno-unused-function",
-           "-I", os.path.join(os.path.dirname(HERE), "include"), "-o", tmp] + srcs
-    if verbose:
-        print(" ".join(cmd))
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    errs = "\n".join(l for l in (res.stderr or "").splitlines()
-                     if "packed-fp32-ops" not in l)
-    if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed ({res.returncode}):\n{errs}")
-    if verbose and errs.strip():
-        print(errs)
+    flags = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-munsafe-fp-atomics", "-Wall", "-Wno-unused-function",
+             "-I", os.path.join(os.path.dirname(HERE), "include")]
+
+    def run(cmd):
+        if verbose:
+            print(" ".join(cmd))
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        errs = "\n".join(l for l in (res.stderr or "").splitlines() if "packed-fp32-ops" not in l)
+        if res.returncode != 0:
+            raise RuntimeError(f"hipcc failed ({res.returncode}):\n{errs}")
+        if verbose and errs.strip():
+            print(errs)
+
+    with tempfile.TemporaryDirectory(prefix="nbx_build_") as objdir:
+        objs = [os.path.join(objdir, os.path.basename(src) + ".o") for src in srcs]
+        workers = max(1, min(len(srcs), os.cpu_count() or 1, 16))
+        with concurrent.futures.ThreadPoolExecutor(max_workers=workers) as pool:
+            list(pool.map(lambda so: run([hipcc()] + flags + ["-c", so[0], "-o", so[1]]), zip(srcs, objs)))
+        run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, out)
     build_nblib(verbose)
     return out

@@ -1154,6 +1154,7 @@ int nbx_put_on_grid(nbx_context* c, const float* box, const float* x, int xOnDev
         c->nbtype.ensure(nslot + kClusterSize);
         c->bb.ensure(2 * static_cast<size_t>(c->ncluster));
         c->fnb.ensure(nslot + kClusterSize);
+        c->fixF.ensure(3 * static_cast<size_t>(nslot + kClusterSize));
         // the fill also writes the far-away filler cluster after the real ones
         launchGridFill(c->stream, c->ncluster, c->x.p, c->atomIndices.p, c->type.p, c->charge.p, c->ntype - 1, c->xq.p,
                        c->nbtype.p, c->bb.p);
@@ -1459,12 +1460,15 @@ static void enqueueStep(nbx_context* c, unsigned flags, const float* xd, float* 
     const bool virial = (flags & NBX_COMPUTE_VIRIAL) != 0 && fOut && xd;
     // one energy slot per wavefront
     const int numSlots = c->lay.numBlocks * kKernelWaves;
-    // the cluster force buffer holds the spills of windows that did not fit
-    // (and is the output when no atom-order forces are asked for); without
-    // either it stays zero and is neither cleared nor read
-    const bool useFnb = c->lay.spill || !fOut;
+    // the cluster force buffer is the output when no atom-order forces are
+    // asked for, and holds the float atomics of union clusters beyond their
+    // window; with NBX_COMPUTE_REPRODUCIBLE those go to the fixed-point
+    // buffer instead.  Buffers not needed are neither cleared nor read.
+    const bool reproducible = (flags & NBX_COMPUTE_REPRODUCIBLE) != 0;
+    const bool useFnb       = !fOut || (c->lay.spill && !reproducible);
+    unsigned long long* const fix = c->lay.spill && reproducible ? c->fixF.p : nullptr;
     launchPrepareStep(c->stream, c->ncluster * kClusterSize, c->atomIndices.p, xd, c->xq.p,
-                      useFnb ? c->fnb.p : nullptr, energy ? c->energyPartial.p : nullptr, 2 * numSlots);
+                      useFnb ? c->fnb.p : nullptr, energy ? c->energyPartial.p : nullptr, 2 * numSlots, fix);
     KernelArgs a{};
     a.chunks        = c->lay.chunks.p;
     a.groups        = c->lay.groups.p;
@@ -1493,6 +1497,7 @@ static void enqueueStep(nbx_context* c, unsigned flags, const float* xd, float* 
     a.segInfo       = c->lay.gather.segInfo.p;
     a.staging       = c->lay.gather.staging.p;
     a.fnb           = c->fnb.p;
+    a.fixF          = fix;
     a.energyPartial = c->energyPartial.p;
     a.waveEnergy    = 1;
     a.fshiftPartial = c->fshiftPartial.p;
@@ -1531,7 +1536,8 @@ static void enqueueStep(nbx_context* c, unsigned flags, const float* xd, float* 
     const int numVir =
             launchGatherForces(c->stream, c->ncluster, c->lay.gather.gposOff.p, c->lay.gather.gpos.p,
                                c->lay.gather.staging.p, useFnb ? c->fnb.p : nullptr, c->atomIndices.p, fOut,
-                               (flags & NBX_COMPUTE_ADD_FORCES) != 0, c->xq.p, virial ? c->virialPartial.p : nullptr);
+                               (flags & NBX_COMPUTE_ADD_FORCES) != 0, c->xq.p, virial ? c->virialPartial.p : nullptr,
+                               fix);
     const int numExcl = energy ? a.exclBlocks : 0;
     if (energy || shift || virial)
     {

@@ -35,11 +35,12 @@ struct nbx_batch
 
     bool built = false;
     int  totAtoms = 0, totClusters = 0, ngroups = 0, nchunk = 0, nwork = 0, windowSlots = 0, numBlocks = 0;
-    bool spill = false; // some member's work entries take global atomics
+    bool spill = false; // some member's union clusters lie beyond their group's window (fixed-point atomics)
     int  groupSize = 0, blocksPerCU = 0, chunkWeight = kChunkWeight;
     long long ncj = 0;
 
     DevBuf<float4> xq, fnb, shiftVec, chunkShift;
+    DevBuf<unsigned long long> fixF; // window-overflow forces in fixed point, 3 per atom slot
     DevBuf<int>    nbtype, atomIndices, cells, exclOff, exclIdx, unionList, atomSystem, groupOff, atomOff, slotOff;
     DevBuf<int2>   waveStart;
     DevBuf<float>  boxes;
@@ -263,6 +264,7 @@ void buildBatch(nbx_batch* b, int G, int blocksPerCU, int chunkWeight)
     upload(b->atomOff, hAtomOff);
     upload(b->boxes, b->boxHost);
     b->fnb.ensure(4 * static_cast<size_t>(totClusters) + 4);
+    b->fixF.ensure(3 * (4 * static_cast<size_t>(totClusters) + 4));
     b->atomEnergy.ensure(std::max(atomOffset, 1));
     b->energy.ensure(2 * static_cast<size_t>(R));
 
@@ -287,8 +289,14 @@ void enqueueCompute(nbx_batch* b, const float* x, float* f, unsigned flags)
     nbx_context* c0     = b->sys[0];
     const bool   energy = (flags & NBX_COMPUTE_ENERGY) != 0;
     const int numSlots = b->ngroups + b->numBlocks * kKernelWaves;
-    launchPrepareStep(b->stream, 4 * b->totClusters, b->atomIndices.p, x, b->xq.p, b->fnb.p,
-                      energy ? b->energyPartial.p : nullptr, 2 * numSlots);
+    // the cluster force buffer when the forces stay in the cluster layout
+    // or windows overflow with float atomics; the fixed-point buffer when
+    // they overflow with NBX_COMPUTE_REPRODUCIBLE
+    const bool                reproducible = (flags & NBX_COMPUTE_REPRODUCIBLE) != 0;
+    float4* const             fnb = (!f || (b->spill && !reproducible)) ? b->fnb.p : nullptr;
+    unsigned long long* const fix = b->spill && reproducible ? b->fixF.p : nullptr;
+    launchPrepareStep(b->stream, 4 * b->totClusters, b->atomIndices.p, x, b->xq.p, fnb,
+                      energy ? b->energyPartial.p : nullptr, 2 * numSlots, fix);
     KernelArgs a{};
     a.chunks       = b->chunks.p;
     a.groups       = b->groups.p;
@@ -317,11 +325,12 @@ void enqueueCompute(nbx_batch* b, const float* x, float* f, unsigned flags)
     a.segInfo      = b->gather.segInfo.p;
     a.staging      = b->gather.staging.p;
     a.fnb          = b->fnb.p;
+    a.fixF         = fix;
     a.energyPartial = b->energyPartial.p;
     a.fshiftPartial = nullptr;
     check(launchNonbonded(b->stream, a, c0->elec, energy, false, b->numBlocks), "nonbonded kernel");
     launchGatherForces(b->stream, b->totClusters, b->gather.gposOff.p, b->gather.gpos.p, b->gather.staging.p,
-                       b->fnb.p, b->atomIndices.p, f, (flags & NBX_COMPUTE_ADD_FORCES) != 0, nullptr, nullptr);
+                       fnb, b->atomIndices.p, f, (flags & NBX_COMPUTE_ADD_FORCES) != 0, nullptr, nullptr, fix);
     if (energy)
     {
         ExclusionArgs e{};

@@ -45,7 +45,8 @@ __global__ void clearF4Kernel(float4* __restrict__ p, int n, double* __restrict_
 // A step's set-up in one launch: coordinates into the cluster layout (when
 // x is given), the cluster force buffer and the energy slots cleared.
 __global__ void prepareStepKernel(int nslot, const int* __restrict__ atomIndices, const float* __restrict__ x,
-                                  float4* __restrict__ xq, float4* __restrict__ fnb, double* __restrict__ d, int nd)
+                                  float4* __restrict__ xq, float4* __restrict__ fnb, double* __restrict__ d, int nd,
+                                  unsigned long long* __restrict__ fixF)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < nslot)
@@ -53,6 +54,12 @@ __global__ void prepareStepKernel(int nslot, const int* __restrict__ atomIndices
         if (fnb)
         {
             fnb[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        if (fixF)
+        {
+            fixF[3 * i]     = 0ull;
+            fixF[3 * i + 1] = 0ull;
+            fixF[3 * i + 2] = 0ull;
         }
         if (x)
         {
@@ -387,7 +394,8 @@ __global__ __launch_bounds__(kGatherThreads) void gatherForcesKernel(int ncluste
                                                                      const int* __restrict__ atomIndices,
                                                                      float* __restrict__ fOut, int addToF,
                                                                      const float4* __restrict__ xq,
-                                                                     double* __restrict__ virPart)
+                                                                     double* __restrict__ virPart,
+                                                                     const unsigned long long* __restrict__ fixF)
 {
     constexpr int kWaves = kGatherThreads / 64;
     __shared__ double sVir[kWaves][12][3];
@@ -428,13 +436,15 @@ __global__ __launch_bounds__(kGatherThreads) void gatherForcesKernel(int ncluste
         {
             const int slot = c * kClusterSize + idx / 3;
             float*    dst  = fnb ? reinterpret_cast<float*>(fnb + slot) + idx % 3 : nullptr;
+            // the fixed-point sums of the window overflow, if any
+            const float over = fixF ? fromFixed(fixF[3 * slot + idx % 3]) : 0.0f;
             if (kToAtoms)
             {
                 // straight to the caller's atom-order forces (nbat-f -> f)
                 const int a = atomIndices[slot];
                 if (a >= 0)
                 {
-                    const float fv = (fnb ? *dst : 0.0f) + r;
+                    const float fv = (fnb ? *dst : 0.0f) + (r + over);
                     float*      fo = fOut + 3 * a + idx % 3;
                     *fo            = addToF ? *fo + fv : fv;
                     if (virPart)
@@ -448,7 +458,7 @@ __global__ __launch_bounds__(kGatherThreads) void gatherForcesKernel(int ncluste
             }
             else
             {
-                *dst += r;
+                *dst += r + over;
             }
         }
     }
@@ -601,12 +611,12 @@ void launchClearF4(hipStream_t s, float4* p, int n, double* d, int nd)
 }
 
 void launchPrepareStep(hipStream_t s, int nslot, const int* atomIndices, const float* x, float4* xq, float4* fnb,
-                       double* d, int nd)
+                       double* d, int nd, unsigned long long* fixF)
 {
     const int m = std::max(nslot, d ? nd : 0);
     if (m > 0)
     {
-        prepareStepKernel<<<(m + 255) / 256, 256, 0, s>>>(nslot, atomIndices, x, xq, fnb, d, d ? nd : 0);
+        prepareStepKernel<<<(m + 255) / 256, 256, 0, s>>>(nslot, atomIndices, x, xq, fnb, d, d ? nd : 0, fixF);
     }
 }
 
@@ -619,7 +629,8 @@ void launchConvertX(hipStream_t s, int nslot, const int* atomIndices, const floa
 }
 
 int launchGatherForces(hipStream_t s, int ncluster, const int* gposOff, const int* gpos, const float4* staging,
-                       float4* fnb, const int* atomIndices, float* f, bool addToF, const float4* xq, double* virPart)
+                       float4* fnb, const int* atomIndices, float* f, bool addToF, const float4* xq, double* virPart,
+                       const unsigned long long* fixF)
 {
     if (ncluster == 0)
     {
@@ -629,12 +640,12 @@ int launchGatherForces(hipStream_t s, int ncluster, const int* gposOff, const in
     if (f)
     {
         gatherForcesKernel<true><<<blocks, kGatherThreads, 0, s>>>(ncluster, gposOff, gpos, staging, fnb, atomIndices,
-                                                                    f, addToF ? 1 : 0, xq, virPart);
+                                                                    f, addToF ? 1 : 0, xq, virPart, fixF);
     }
     else
     {
         gatherForcesKernel<false><<<blocks, kGatherThreads, 0, s>>>(ncluster, gposOff, gpos, staging, fnb, nullptr,
-                                                                     nullptr, 0, nullptr, nullptr);
+                                                                     nullptr, 0, nullptr, nullptr, fixF);
     }
     return blocks;
 }

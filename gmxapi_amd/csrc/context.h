@@ -192,7 +192,7 @@ struct WorkLayout
 {
     int            groupSize = 0, ngroups = 0, nchunk = 0, windowSlots = 0, nwork = 0;
     int            numBlocks = 0; // workgroups of kKernelWaves wavefronts (the schedule's wavefronts / 4)
-    bool           spill     = false; // some union clusters take global atomics (no window slot)
+    bool           spill     = false; // some union clusters lie beyond their window (global atomics)
     DevBuf<int>    unionCount, chunkCount, workCount, overflow, unionOff, chunkOff, workOff, unionList;
     DevBuf<int4>   chunks, groups, groupInfo;
     // the group pass's output at upper-bound bases, before compaction
@@ -283,6 +283,7 @@ struct nbx_context
     nbx::DevBuf<float> x;          // device copy of coordinates in atom order
     nbx::DevBuf<int>   cxy_na, cxy_ind, atomCol, atomRank, bucket, atomIndices, cells, nbtype, counter;
     nbx::DevBuf<float4> xq, bb, fnb;
+    nbx::DevBuf<unsigned long long> fixF; // window-overflow forces in fixed point, 3 per atom slot
 
     // list
     bool           listValid = false;

@@ -306,7 +306,7 @@ struct LdsLayout
     int end_;
 };
 
-template<int kElec, bool kEnergy, bool kShift, int kLJ>
+template<int kElec, bool kEnergy, bool kShift, int kLJ, bool kFixed>
 __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
 {
     extern __shared__ float4 smem4[];
@@ -518,8 +518,10 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
             }
             // the 16 entries of a tile have distinct slots (padding entries
             // share the spare slot, adding zeros): plain read-modify-write;
-            // clusters that did not fit the window take global atomics (a
-            // layout without such clusters skips the test)
+            // clusters that did not fit the window take global atomics, in
+            // float or (kFixed) on fixed-point sums, whose integer additions
+            // do not depend on their order (a layout without such clusters
+            // skips the test)
             const unsigned slot = e.y >> 16;
             if (!a.windowSpill || slot != kSlotOverflow)
             {
@@ -527,6 +529,13 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
                 w[0] += fjx;
                 w[1] += fjy;
                 w[2] += fjz;
+            }
+            else if constexpr (kFixed)
+            {
+                unsigned long long* dst = a.fixF + 3 * (e.x * kClusterSize + jj);
+                atomicAdd(dst, toFixed(fjx));
+                atomicAdd(dst + 1, toFixed(fjy));
+                atomicAdd(dst + 2, toFixed(fjz));
             }
             else
             {
@@ -639,10 +648,10 @@ __global__ __launch_bounds__(kThreads) void nonbondedKernel(const KernelArgs a)
 // With events, the launch itself records the kernel's start and end (the
 // dispatch packet's own timestamps): no marker packets in the stream, so
 // timing the kernel leaves the step's timeline unchanged.
-template<int kElec, bool kEnergy, bool kShift, int kLJ>
+template<int kElec, bool kEnergy, bool kShift, int kLJ, bool kFixed>
 hipError_t launchOne(hipStream_t s, const KernelArgs& a, int numBlocks, int lds, hipEvent_t start, hipEvent_t stop)
 {
-    const void* fn = reinterpret_cast<const void*>(&nonbondedKernel<kElec, kEnergy, kShift, kLJ>);
+    const void* fn = reinterpret_cast<const void*>(&nonbondedKernel<kElec, kEnergy, kShift, kLJ, kFixed>);
     const hipError_t e = allowDynamicLds(fn, lds);
     if (e != hipSuccess)
     {
@@ -655,7 +664,7 @@ hipError_t launchOne(hipStream_t s, const KernelArgs& a, int numBlocks, int lds,
         void* args[] = { const_cast<KernelArgs*>(&a) };
         return hipExtLaunchKernel(fn, dim3(numBlocks), dim3(kThreads), args, lds, s, start, stop, 0);
     }
-    nonbondedKernel<kElec, kEnergy, kShift, kLJ><<<numBlocks, kThreads, lds, s>>>(a);
+    nonbondedKernel<kElec, kEnergy, kShift, kLJ, kFixed><<<numBlocks, kThreads, lds, s>>>(a);
     return hipGetLastError();
 }
 
@@ -663,19 +672,26 @@ template<int kElec, int kLJ>
 hipError_t launchFlags(hipStream_t s, const KernelArgs& a, bool energy, bool shiftForces, int numBlocks, int lds,
                        hipEvent_t start, hipEvent_t stop)
 {
+    // fixed-point overflow sums (reproducible mode with overflowing windows)
+    // in kernels of their own, so the default kernels carry no code for them
+    const bool fixed = a.windowSpill && a.fixF;
     if (energy && shiftForces)
     {
-        return launchOne<kElec, true, true, kLJ>(s, a, numBlocks, lds, start, stop);
+        return fixed ? launchOne<kElec, true, true, kLJ, true>(s, a, numBlocks, lds, start, stop)
+                     : launchOne<kElec, true, true, kLJ, false>(s, a, numBlocks, lds, start, stop);
     }
     if (energy)
     {
-        return launchOne<kElec, true, false, kLJ>(s, a, numBlocks, lds, start, stop);
+        return fixed ? launchOne<kElec, true, false, kLJ, true>(s, a, numBlocks, lds, start, stop)
+                     : launchOne<kElec, true, false, kLJ, false>(s, a, numBlocks, lds, start, stop);
     }
     if (shiftForces)
     {
-        return launchOne<kElec, false, true, kLJ>(s, a, numBlocks, lds, start, stop);
+        return fixed ? launchOne<kElec, false, true, kLJ, true>(s, a, numBlocks, lds, start, stop)
+                     : launchOne<kElec, false, true, kLJ, false>(s, a, numBlocks, lds, start, stop);
     }
-    return launchOne<kElec, false, false, kLJ>(s, a, numBlocks, lds, start, stop);
+    return fixed ? launchOne<kElec, false, false, kLJ, true>(s, a, numBlocks, lds, start, stop)
+                 : launchOne<kElec, false, false, kLJ, false>(s, a, numBlocks, lds, start, stop);
 }
 
 // the LJ flavour: the switches, or the potential shift with the variant

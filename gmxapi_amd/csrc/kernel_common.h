@@ -161,6 +161,18 @@ __device__ __forceinline__ float waveReduceScatter16(const float* f, int lane)
     return quadTail(swapStep16(a0, a4), swapStep16(a1, a5), swapStep16(a2, a6), swapStep16(a3, a7), lane);
 }
 
+// v in fixed point with kFixBits fraction bits, as the two's-complement bits
+// of a 64-bit integer (for integer atomics), and back
+__device__ __forceinline__ unsigned long long toFixed(float v)
+{
+    return static_cast<unsigned long long>(static_cast<long long>(rintf(v * static_cast<float>(1 << kFixBits))));
+}
+
+__device__ __forceinline__ float fromFixed(unsigned long long u)
+{
+    return static_cast<float>(static_cast<double>(static_cast<long long>(u)) * (1.0 / static_cast<double>(1 << kFixBits)));
+}
+
 __device__ __forceinline__ double waveSumD(double v)
 {
 #pragma unroll

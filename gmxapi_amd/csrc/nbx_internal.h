@@ -292,6 +292,11 @@ struct GroupArgs
 // window slot of a j-entry whose cluster did not fit the group's window:
 // the kernel adds its forces with global atomics
 constexpr unsigned kSlotOverflow = 0xffffu;
+// Forces of union clusters beyond a window are summed in fixed point with
+// kFixBits fraction bits in 64-bit integers: integer addition does not depend
+// on the order of the atomics, so the sums are reproducible.  Resolution
+// 2^-20 (about 1e-6 kJ mol^-1 nm^-1), range +-2^43 (about 8.8e12).
+constexpr int kFixBits = 20;
 void launchGroupSpans(hipStream_t s, int ncluster, const int* offJ, int* maxSpan);
 void launchBuildGroups(hipStream_t s, const GroupArgs& a, bool fill);
 // moves an upper-bound layout (unionTmp, workTmp, chunksTmp) to a.unionOff,
@@ -388,7 +393,7 @@ struct KernelArgs
     int           vdwModifier; // host: NBX_VDW_* modifier
     int           ljPme;       // host: LJ-PME (c6grid is read)
     int           twinCut;     // host: rvdw < rcoulomb (launch the twin-cut-off variant)
-    int           windowSpill; // some work entries take global atomics (no window slot)
+    int           windowSpill; // some union clusters lie beyond their window (fixed-point atomics)
     const float*  c6grid;      // LJ-PME: the grid's 6 C6 per type pair
     int           ntype;  // including the padding type
     const float4* tabFDV0;
@@ -397,7 +402,11 @@ struct KernelArgs
     const int2* waveStart; // [4 * gridDim.x + 1]: each wavefront's first chunk and its group
     const int4* segInfo;   // per group: window staging layout (launchSegments)
     float4*     staging;   // the wavefronts' j-force windows, gathered after the kernel
-    float4*  fnb;
+    // union clusters beyond their group's window: added to fnb with float
+    // atomics, or (fixF given) to 3 fixed-point sums (kFixBits fraction
+    // bits) per atom slot with integer atomics, independent of their order
+    float4*             fnb;
+    unsigned long long* fixF;
     // zeroed before the launch: with waveEnergy, [4 * gridDim.x][2], the
     // energies of wavefront w (its groups in order) in slot w; otherwise
     // [ngroups + 4 * gridDim.x][2], those of wavefront w in group g in slot
@@ -477,11 +486,11 @@ void launchConvertX(hipStream_t s, int nslot, const int* atomIndices, const floa
 constexpr int kGatherMaxBlocks = 4096;
 int launchGatherForces(hipStream_t s, int ncluster, const int* gposOff, const int* gpos, const float4* staging,
                        float4* fnb, const int* atomIndices, float* f, bool addToF, const float4* xq,
-                       double* virPart);
+                       double* virPart, const unsigned long long* fixF = nullptr);
 // coordinates into the cluster layout (x may be NULL), cluster forces and
 // nd doubles at d cleared, in one launch
 void launchPrepareStep(hipStream_t s, int nslot, const int* atomIndices, const float* x, float4* xq, float4* fnb,
-                       double* d, int nd);
+                       double* d, int nd, unsigned long long* fixF = nullptr);
 // charges (atom order) into the w component of the cluster layout
 void launchSetCharges(hipStream_t s, int nslot, const int* atomIndices, const float* q, float4* xq);
 void launchReduceF(hipStream_t s, int natoms, const int* cells, const float4* fnb, float* f);

@@ -38,6 +38,7 @@ COMPUTE_ENERGY = 2
 COMPUTE_SHIFT_FORCES = 4
 COMPUTE_VIRIAL = 8
 COMPUTE_ADD_FORCES = 16  # add to f instead of overwriting it
+COMPUTE_REPRODUCIBLE = 32  # window overflow summed in fixed point: bitwise reproducible in every case
 
 
 class NbxError(RuntimeError):
@@ -303,12 +304,12 @@ class Context:
         self._check(self._lib.nbx_prune_rolling(self._h, float(r_inner), int(part), int(num_parts)),
                     "nbx_prune_rolling")
 
-    def compute(self, x, energy=False, shift_forces=False, virial=False) -> np.ndarray:
+    def compute(self, x, energy=False, shift_forces=False, virial=False, reproducible=False) -> np.ndarray:
         """Forces in atom order (host arrays, synchronous)."""
         xa = np.ascontiguousarray(x, np.float32).reshape(-1)
         f = np.empty_like(xa)
         flags = COMPUTE_FORCES | (COMPUTE_ENERGY if energy else 0) | (COMPUTE_SHIFT_FORCES if shift_forces else 0)
-        flags |= COMPUTE_VIRIAL if virial else 0
+        flags |= (COMPUTE_VIRIAL if virial else 0) | (COMPUTE_REPRODUCIBLE if reproducible else 0)
         self._check(self._lib.nbx_compute(self._h, _ptr(xa), _ptr(f), flags, 0), "nbx_compute")
         return f.reshape(-1, 3)
 

@@ -84,6 +84,15 @@ extern "C" {
  * atomdata_add_nbat_f_to_f_gpu (nbnxm.h:345-359), for a caller that sums
  * several force sources in one buffer */
 #define NBX_COMPUTE_ADD_FORCES 16u
+/* bitwise reproducible forces, virial and shift forces in every case: the
+ * forces of j-clusters that do not fit a group's LDS window (a small share
+ * of the cluster pairs at long cut-offs or with large groups) are summed in
+ * 64-bit fixed point (2^-20 kJ mol^-1 nm^-1, range +-8.8e12) instead of
+ * float atomics, whose order varies from run to run.  No counterpart in the
+ * reference's compute flags (its CPU kernels are reproducible and its GPU
+ * kernels are not); costs a few per cent of the pair kernel when windows
+ * overflow, nothing otherwise. */
+#define NBX_COMPUTE_REPRODUCIBLE 32u
 
 typedef struct nbx_context nbx_context;
 

@@ -434,10 +434,20 @@ def test_shift_forces_and_virial_are_reproducible(water3000):
     # with float atomics: then forces and the virial's x (x) f term agree to
     # rounding only, while the shift forces (from the i-forces) and the
     # energies stay bit-identical.
+    # NBX_COMPUTE_REPRODUCIBLE sums those in fixed point instead: then
+    # everything is bit-identical, and equal to the default to rounding.
     s = water3000
     ctx = nbx.context_for(s)
     spill = bool(np.any((ctx.work_groups()["work"][:, 1] >> 16) == 0xFFFF))
     print(f"[reproducible] config0, window spill: {spill}", flush=True)
+    fixed = []
+    for _ in range(3):
+        f = ctx.compute(s.x, energy=True, shift_forces=True, virial=True, reproducible=True)
+        fixed.append((f, ctx.shift_forces(), ctx.virial(), tuple(ctx.energies())))
+    for got in fixed[1:]:
+        for k in range(3):
+            np.testing.assert_array_equal(got[k], fixed[0][k])
+        assert got[3] == fixed[0][3]
     ref = None
     for _ in range(5):
         f = ctx.compute(s.x, energy=True, shift_forces=True, virial=True)
@@ -454,6 +464,29 @@ def test_shift_forces_and_virial_are_reproducible(water3000):
         else:
             np.testing.assert_array_equal(got[0], ref[0])
             np.testing.assert_array_equal(got[2], ref[2])
+    np.testing.assert_allclose(fixed[0][0], ref[0], rtol=0, atol=1e-5 * np.abs(ref[0]).max())
+    np.testing.assert_allclose(fixed[0][2], ref[2], rtol=0, atol=1e-6 * np.abs(ref[2]).max())
+    assert fixed[0][3] == ref[3]
+
+
+def test_water288k_reproducible_mode():
+    # the 288,000-atom water box overflows some windows (about 0.2 % of the
+    # j-entries): with NBX_COMPUTE_REPRODUCIBLE its forces, virial and shift
+    # forces are bit-identical from call to call and agree with the FP64
+    # oracle as the default path does
+    s = S.config(1)
+    ctx = nbx.context_for(s)
+    spill = int(np.sum((ctx.work_groups()["work"][:, 1] >> 16) == 0xFFFF))
+    print(f"[reproducible] water 288k, overflow entries: {spill}", flush=True)
+    runs = []
+    for _ in range(3):
+        f = ctx.compute(s.x, energy=True, shift_forces=True, virial=True, reproducible=True)
+        runs.append((f, ctx.virial(), ctx.shift_forces()))
+    for got in runs[1:]:
+        for k in range(3):
+            np.testing.assert_array_equal(got[k], runs[0][k])
+    r64 = oracle_for(s, "f64").compute(s.x)
+    assert rms(runs[0][0] - r64["f"]) / rms(r64["f"]) <= FORCE_TOL_RMS
 
 
 def test_update_changes_forces():
