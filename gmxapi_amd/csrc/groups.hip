@@ -536,17 +536,13 @@ __global__ void occurrenceKernel(int ngroups, const int4* __restrict__ groups, c
 // longer lists (not expected at supported cut-offs) by one thread.
 constexpr int kSortMax = 1024;
 
-__global__ __launch_bounds__(256) void occurrenceSortKernel(const int* __restrict__ occOff, int2* __restrict__ occ)
+__global__ __launch_bounds__(64) void occurrenceSortKernel(const int* __restrict__ occOff, int2* __restrict__ occ)
 {
     __shared__ unsigned long long key[kSortMax];
     const int c = blockIdx.x;
     const int b = occOff[c], n = occOff[c + 1] - b;
     if (n <= 64)
     {
-        if (threadIdx.x >= 64)
-        {
-            return;
-        }
         const int lane = threadIdx.x;
         // keys (group, slot) as one 64-bit value; unused lanes sort last
         unsigned hi = 0xffffffffu, lo = 0xffffffffu;
@@ -643,30 +639,53 @@ __global__ __launch_bounds__(256) void occurrenceSortKernel(const int* __restric
 // all its occurrences, one per wavefront segment of the occurrence's group,
 // in occurrence order.
 template<bool kFill>
-__global__ void gatherListKernel(int ncluster, const int* __restrict__ occOff, const int2* __restrict__ occ,
-                                 const int4* __restrict__ segInfo, int slotF4, int* __restrict__ count,
-                                 const int* __restrict__ gposOff, int* __restrict__ gpos)
+__global__ __launch_bounds__(256) void gatherListKernel(int ncluster, const int* __restrict__ occOff,
+                                                        const int2* __restrict__ occ, const int4* __restrict__ segInfo,
+                                                        int slotF4, int* __restrict__ count,
+                                                        const int* __restrict__ gposOff, int* __restrict__ gpos)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    // one wavefront per cluster: lane l takes occurrences l, l + 64, ...;
+    // each occurrence stands for one staged copy per wavefront that ran its
+    // group, placed in occurrence order after a wave prefix sum
+    const int lane = threadIdx.x & 63;
+    const int c    = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (c >= ncluster)
     {
         return;
     }
-    int n = kFill ? gposOff[c] : 0;
-    for (int o = occOff[c]; o < occOff[c + 1]; o++)
+    const int o1 = occOff[c + 1];
+    int       n  = kFill ? gposOff[c] : 0;
+    for (int o0 = occOff[c]; o0 < o1; o0 += 64)
     {
-        const int2 e  = occ[o];
-        const int4 si = segInfo[e.x];
-        for (int w = 0; w < si.z; w++)
+        const int o  = o0 + lane;
+        int2      e  = make_int2(0, 0);
+        int4      si = make_int4(0, 0, 0, 0);
+        if (o < o1)
         {
-            if (kFill)
-            {
-                gpos[n] = si.x + w * si.w + e.y * slotF4;
-            }
-            n++;
+            e  = occ[o];
+            si = segInfo[e.x];
         }
+        int incl = si.z;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1)
+        {
+            const int t = __shfl_up(incl, off, 64);
+            if (lane >= off)
+            {
+                incl += t;
+            }
+        }
+        if (kFill)
+        {
+            int pos = n + incl - si.z;
+            for (int w = 0; w < si.z; w++, pos++)
+            {
+                gpos[pos] = si.x + w * si.w + e.y * slotF4;
+            }
+        }
+        n += __shfl(incl, 63, 64);
     }
-    if (!kFill)
+    if (!kFill && lane == 0)
     {
         count[c] = n;
     }
@@ -916,7 +935,7 @@ void launchOccurrences(hipStream_t s, int ngroups, const int4* groups, const int
         return;
     }
     occurrenceKernel<true><<<ngroups, 256, 0, s>>>(ngroups, groups, groupInfo, unionList, count, occOff, occ);
-    occurrenceSortKernel<<<ncluster, 256, 0, s>>>(occOff, occ);
+    occurrenceSortKernel<<<ncluster, 64, 0, s>>>(occOff, occ);
 }
 
 void launchSegments(hipStream_t s, int ngroups, const int4* groups, const int4* groupInfo, int numWaves,
@@ -941,13 +960,13 @@ void launchGatherList(hipStream_t s, int ncluster, const int* occOff, const int2
     }
     if (fill)
     {
-        gatherListKernel<true><<<(ncluster + 255) / 256, 256, 0, s>>>(ncluster, occOff, occ, segInfo, slotF4, count,
-                                                                      gposOff, gpos);
+        gatherListKernel<true><<<(ncluster + 3) / 4, 256, 0, s>>>(ncluster, occOff, occ, segInfo, slotF4, count,
+                                                                  gposOff, gpos);
     }
     else
     {
-        gatherListKernel<false><<<(ncluster + 255) / 256, 256, 0, s>>>(ncluster, occOff, occ, segInfo, slotF4, count,
-                                                                       gposOff, gpos);
+        gatherListKernel<false><<<(ncluster + 3) / 4, 256, 0, s>>>(ncluster, occOff, occ, segInfo, slotF4, count,
+                                                                   gposOff, gpos);
     }
 }
 
