@@ -418,9 +418,13 @@ void buildWorkGroups(nbx_context* c, WorkLayout& L, bool adjoint, const int* spa
         check(hipMemsetAsync(c->groupStats.p + kGroupSizeCount, 0, sizeof(int) * 2, c->stream), "clear stats");
         launchMaxReduce(c->stream, ng, L.unionCount.p, c->groupStats.p + kGroupSizeCount);
         launchMaxReduce(c->stream, ng, L.overflow.p, c->groupStats.p + kGroupSizeCount + 1);
-        launchExclusiveScan(c->stream, ng, L.unionCount.p, L.unionOff.p, c->groupStats.p + kGroupSizeCount + 2);
-        launchExclusiveScan(c->stream, ng, L.chunkCount.p, L.chunkOff.p, c->groupStats.p + kGroupSizeCount + 3);
-        launchExclusiveScan(c->stream, ng, L.workCount.p, L.workOff.p, c->groupStats.p + kGroupSizeCount + 4);
+        {
+            const int* in[3]  = { L.unionCount.p, L.chunkCount.p, L.workCount.p };
+            int*       out[3] = { L.unionOff.p, L.chunkOff.p, L.workOff.p };
+            int*       tot[3] = { c->groupStats.p + kGroupSizeCount + 2, c->groupStats.p + kGroupSizeCount + 3,
+                                  c->groupStats.p + kGroupSizeCount + 4 };
+            launchExclusiveScans(c->stream, ng, 3, in, out, tot);
+        }
         int* const hp = pinnedInts();
         check(hipMemcpyAsync(hp, c->groupStats.p + kGroupSizeCount, 5 * sizeof(int), hipMemcpyDeviceToHost, c->stream),
               "read stats");
@@ -1215,6 +1219,13 @@ int nbx_construct_pairlist(nbx_context* c)
         a.countI      = c->countI.p;
         a.offJ        = c->offJ.p;
         a.offI        = c->offI.p;
+        // the j- and i-entry counts scanned together (two workgroups, one launch)
+        auto scanCounts = [&] {
+            const int* in[2]  = { c->countJ.p, c->countI.p };
+            int*       out[2] = { c->offJ.p, c->offI.p };
+            int*       tot[2] = { c->totals.p, c->totals.p + 1 };
+            launchExclusiveScans(c->stream, nc, 2, in, out, tot);
+        };
         // the group pass's largest j-entry span per group size, read with
         // the list's totals (one wait for both)
         int  spans[kGroupSizeCount] = {};
@@ -1246,8 +1257,7 @@ int nbx_construct_pairlist(nbx_context* c)
             a.jentry = c->searchJTmp.p;
             a.ientry = c->searchITmp.p;
             launchSearch(c->stream, a, true);
-            launchExclusiveScan(c->stream, nc, c->countJ.p, c->offJ.p, c->totals.p);
-            launchExclusiveScan(c->stream, nc, c->countI.p, c->offI.p, c->totals.p + 1);
+            scanCounts();
             check(hipMemsetAsync(c->totals.p + 2, 0, sizeof(int), c->stream), "clear max");
             launchMaxReduce(c->stream, nc, c->countJ.p, c->totals.p + 2);
             readSpans();
@@ -1274,8 +1284,7 @@ int nbx_construct_pairlist(nbx_context* c)
             a.ientry = nullptr;
             a.jentry = nullptr;
             launchSearch(c->stream, a, false);
-            launchExclusiveScan(c->stream, nc, c->countJ.p, c->offJ.p, c->totals.p);
-            launchExclusiveScan(c->stream, nc, c->countI.p, c->offI.p, c->totals.p + 1);
+            scanCounts();
             check(hipMemsetAsync(c->totals.p + 2, 0, sizeof(int), c->stream), "clear max");
             launchMaxReduce(c->stream, nc, c->countJ.p, c->totals.p + 2);
             readSpans();
@@ -1345,8 +1354,12 @@ int nbx_prune_pairlist(nbx_context* c, double rlistInner)
         a.countI      = c->countI.p;
         a.countJ      = c->countJ.p;
         launchPrune(c->stream, a, false);
-        launchExclusiveScan(c->stream, nc, c->countJ.p, c->offJ.p, c->totals.p);
-        launchExclusiveScan(c->stream, nc, c->countI.p, c->offI.p, c->totals.p + 1);
+        {
+            const int* in[2]  = { c->countJ.p, c->countI.p };
+            int*       out[2] = { c->offJ.p, c->offI.p };
+            int*       tot[2] = { c->totals.p, c->totals.p + 1 };
+            launchExclusiveScans(c->stream, nc, 2, in, out, tot);
+        }
         int tot[2] = { 0, 0 };
         check(hipMemcpyAsync(tot, c->totals.p, sizeof(int) * 2, hipMemcpyDeviceToHost, c->stream), "read totals");
         check(hipStreamSynchronize(c->stream), "prune sync");

@@ -255,6 +255,9 @@ void launchSearchCompact(hipStream_t s, int ncluster, int capJ, const int* count
                          const int* offJ, const int* offI, const uint2* jentryTmp, const int4* ientryTmp,
                          uint2* jentry, int4* ientry);
 void launchExclusiveScan(hipStream_t s, int n, const int* in, int* out, int* total);
+// count (at most 3) independent scans of n elements each in one launch, one
+// workgroup each
+void launchExclusiveScans(hipStream_t s, int n, int count, const int* const* in, int* const* out, int* const* total);
 
 // -- groups.hip
 struct GroupArgs

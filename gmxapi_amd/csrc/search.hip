@@ -455,9 +455,19 @@ __device__ __forceinline__ void scanLoad(const int* __restrict__ in, int n, int 
     }
 }
 
-__global__ __launch_bounds__(kScanThreads) void exclusiveScanKernel(int n, const int* __restrict__ in,
-                                                                   int* __restrict__ out, int* __restrict__ total)
+// Up to three independent scans of n elements each, one per workgroup.
+struct ScanSet
 {
+    const int* in[3];
+    int*       out[3];
+    int*       total[3];
+};
+
+__global__ __launch_bounds__(kScanThreads) void exclusiveScanKernel(int n, ScanSet set)
+{
+    const int* __restrict__ in    = set.in[blockIdx.x];
+    int* __restrict__       out   = set.out[blockIdx.x];
+    int* __restrict__       total = set.total[blockIdx.x];
     __shared__ int sWave[kScanThreads / 64];
     const int      lane     = threadIdx.x & 63;
     const int      wave     = threadIdx.x >> 6;
@@ -595,7 +605,20 @@ void launchSearchCompact(hipStream_t s, int ncluster, int capJ, const int* count
 
 void launchExclusiveScan(hipStream_t s, int n, const int* in, int* out, int* total)
 {
-    exclusiveScanKernel<<<1, kScanThreads, 0, s>>>(n, in, out, total);
+    ScanSet set{ { in }, { out }, { total } };
+    exclusiveScanKernel<<<1, kScanThreads, 0, s>>>(n, set);
+}
+
+void launchExclusiveScans(hipStream_t s, int n, int count, const int* const* in, int* const* out, int* const* total)
+{
+    ScanSet set{};
+    for (int k = 0; k < count; k++)
+    {
+        set.in[k]    = in[k];
+        set.out[k]   = out[k];
+        set.total[k] = total[k];
+    }
+    exclusiveScanKernel<<<count, kScanThreads, 0, s>>>(n, set);
 }
 
 } // namespace nbx
