@@ -378,7 +378,25 @@ __global__ __launch_bounds__(kAdjThreads) void adjointKernel(const AdjointArgs a
         float4 gA = ldg(a.gq, eA.x * kClusterSize + jj);
         beginGroup(g);
         beginChunk(cFirst, a.chunkShift[chAll0], 0);
-        for (int t = 0; t < T; t += 2)
+        // an odd tile count: the first tile alone, so that the loop below
+        // always runs whole pairs of tiles (no exit between its two tiles:
+        // the accumulators stay in their registers around the loop)
+        int t0 = 0;
+        if (T & 1)
+        {
+            if (chEnd == 0)
+            {
+                advance(0);
+            }
+            doTile(eA, xA, tA, gA, 0 - chStart < nspec);
+            xA = ldg(a.xq, eB.x * kClusterSize + jj);
+            tA = ldg(a.type, eB.x * kClusterSize + jj);
+            gA = ldg(a.gq, eB.x * kClusterSize + jj);
+            eA = eB;
+            eB = entryAt(2);
+            t0 = 1;
+        }
+        for (int t = t0; t < T; t += 2)
         {
             const float4 xB  = ldg(a.xq, eB.x * kClusterSize + jj);
             const int    tB  = ldg(a.type, eB.x * kClusterSize + jj);
@@ -389,10 +407,6 @@ __global__ __launch_bounds__(kAdjThreads) void adjointKernel(const AdjointArgs a
                 advance(t);
             }
             doTile(eA, xA, tA, gA, t - chStart < nspec);
-            if (t + 1 >= T)
-            {
-                break;
-            }
             const uint2 eB2 = entryAt(t + 3);
             xA              = ldg(a.xq, eA2.x * kClusterSize + jj);
             tA              = ldg(a.type, eA2.x * kClusterSize + jj);
