@@ -135,3 +135,37 @@ def test_batch_input_errors(replicas):
     with pytest.raises(nbx.NbxStateError):
         b.energies()
     torch.cuda.synchronize()
+
+
+def test_reproducible_flag_gives_identical_forces(replicas):
+    # with NBX_COMPUTE_REPRODUCIBLE, j-forces beyond a group's window are
+    # summed in fixed point: eager calls and graph replays are bit-identical
+    # (the fixed-point sums are cleared by the step's set-up kernel, not a
+    # memset node, so the graph replays them too)
+    import torch
+
+    flags = FLAGS | nbx.COMPUTE_REPRODUCIBLE
+    b = nbx.batch_for(replicas)
+    stream = torch.cuda.Stream()
+    b.set_stream(stream.cuda_stream)
+    xd, fd = _device_arrays(replicas)
+    with torch.cuda.stream(stream):
+        runs = []
+        for _ in range(3):
+            b.compute_device(xd.data_ptr(), fd.data_ptr(), flags)
+            stream.synchronize()
+            runs.append((fd.cpu().numpy().copy(), b.energies().copy()))
+        for f, e in runs[1:]:
+            np.testing.assert_array_equal(f, runs[0][0])
+            np.testing.assert_array_equal(e, runs[0][1])
+        b.capture_graph(xd.data_ptr(), fd.data_ptr(), flags)
+        stream.synchronize()
+        for _ in range(2):
+            fd.zero_()
+            stream.synchronize()
+            b.launch_graph()
+            stream.synchronize()
+            np.testing.assert_array_equal(fd.cpu().numpy(), runs[0][0])
+    fs = _split(runs[0][0], replicas)
+    for k, (f_ref, _) in enumerate(_single_results(replicas)):
+        assert rms(fs[k] - f_ref) <= 1e-6 * rms(f_ref)
