@@ -674,3 +674,42 @@ def test_interaction_constants_match_oracle(opts, oracle_opts):
 
 def ref_sh(o):
     return o.constants()["sh_ewald"]
+
+
+def _sparse(name, box, x, q):
+    n = len(x)
+    return S.System(name, np.array(box, float), np.array(x, float), np.zeros(n, np.int32), np.array(q, float),
+                    np.array([0.0026]), np.array([2.6e-6]), np.arange(n + 1, dtype=np.int32),
+                    np.arange(n, dtype=np.int32), 1.0, 1.0)
+
+
+def _corner_lattice():
+    g = np.arange(3) * 0.35 + 0.2
+    x = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    return _sparse("corner27", [6.0, 6.0, 6.0], x, np.tile([0.4, -0.4], 14)[:27])
+
+
+@pytest.mark.parametrize("s", [
+    _sparse("two-far", [5.0, 5.0, 5.0], [[0.5, 0.5, 0.5], [3.0, 3.0, 3.0]], [0.5, -0.5]),
+    _sparse("two-near", [5.0, 5.0, 5.0], [[0.5, 0.5, 0.5], [0.9, 0.5, 0.5]], [0.5, -0.5]),
+    _sparse("two-across-the-boundary", [5.0, 5.0, 5.0], [[0.1, 2.5, 2.5], [4.8, 2.5, 2.5]], [0.5, -0.5]),
+    _corner_lattice(),
+], ids=lambda s: s.name)
+def test_sparse_systems(s):
+    # nearly empty boxes: single-atom clusters, columns without atoms, a
+    # pair interacting only through the periodic boundary, a lattice filling
+    # one corner; forces and energies against the FP64 oracle.  The energy
+    # bar adds what float coordinates allow: a distance formed from
+    # coordinates up to L is off by up to about L 2^-23, which changes a
+    # pair's energy by |F| times that (a single pair across a 5 nm boundary,
+    # d = 0.1 + 5 - 4.8 in float, is 1e-6 off: 3e-5 of its LJ energy)
+    print(f"[sparse] {s.name}: {s.natoms} atoms", flush=True)
+    ctx = nbx.context_for(s)
+    f = ctx.compute(s.x, energy=True, virial=True)
+    vlj, vc = ctx.energies()
+    r64 = oracle_for(s, "f64").compute(s.x)
+    np.testing.assert_allclose(f, r64["f"], rtol=0, atol=1e-4 * max(1.0, np.abs(r64["f"]).max()))
+    coord = 0.5 * np.abs(r64["f"]).sum() * np.max(s.box) * 2.0 ** -23
+    assert abs(vlj - r64["Vvdw"]) <= 1e-6 * abs(r64["Vvdw"]) + coord
+    assert abs(vc - r64["Vc"]) <= 1e-6 * abs(r64["Vc"]) + coord
+    np.testing.assert_allclose(ctx.list_pairs(s.rlist), oracle_for(s, "f32").list_pairs(s.rlist))
