@@ -713,3 +713,30 @@ def test_sparse_systems(s):
     assert abs(vlj - r64["Vvdw"]) <= 1e-6 * abs(r64["Vvdw"]) + coord
     assert abs(vc - r64["Vc"]) <= 1e-6 * abs(r64["Vc"]) + coord
     np.testing.assert_allclose(ctx.list_pairs(s.rlist), oracle_for(s, "f32").list_pairs(s.rlist))
+
+
+def test_water288k_invariances():
+    # size-independent properties at full size: the forces sum to zero
+    # (every pair force appears twice, opposite), and translating every atom
+    # (wrapped into the box, so grid, clusters and list all change) leaves
+    # each atom's force and the energies unchanged up to float rounding: the
+    # translated coordinates round differently in float (measured 8e-6 of the
+    # RMS force), while one O-O pair missing from the list would move the RMS
+    # difference by about 1.4e-4 of it
+    s = S.config(1)
+    ctx = nbx.context_for(s)
+    f = ctx.compute(s.x, energy=True).astype(np.float64)
+    e = ctx.energies()
+    scale = rms(f)
+    print(f"[invariance] 288k: |sum f| {np.abs(f.sum(0)).max():.3e}, rms f {scale:.3e}", flush=True)
+    assert np.abs(f.sum(0)).max() <= 1e-6 * scale * np.sqrt(s.natoms)
+    t = np.array([0.37, 1.1, 2.3])
+    moved = np.mod(s.x + t, s.box)
+    ctx2 = nbx.context_for(s)
+    ctx2.put_on_grid(s.box, moved)
+    ctx2.construct_pairlist()
+    f2 = ctx2.compute(moved, energy=True).astype(np.float64)
+    e2 = ctx2.energies()
+    assert rms(f2 - f) <= 3e-5 * scale
+    assert abs(e2[0] - e[0]) <= 1e-6 * abs(e[0])
+    assert abs(e2[1] - e[1]) <= 1e-6 * abs(e[1])
