@@ -379,9 +379,9 @@ void buildWorkGroups(nbx_context* c, WorkLayout& L, bool adjoint, const int* spa
         L.chunkCount.ensure(ng + 1);
         L.workCount.ensure(ng + 1);
         L.overflow.ensure(ng + 1);
-        L.unionOff.ensure(ng + 1);
-        L.chunkOff.ensure(ng + 1);
-        L.workOff.ensure(ng + 1);
+        L.unionOff.ensure(scanOutSize(ng));
+        L.chunkOff.ensure(scanOutSize(ng));
+        L.workOff.ensure(scanOutSize(ng));
         L.groups.ensure(ng);
         L.groupInfo.ensure(ng);
         // one pass writes the layout at upper-bound bases (no count pass);
@@ -423,7 +423,7 @@ void buildWorkGroups(nbx_context* c, WorkLayout& L, bool adjoint, const int* spa
             int*       out[3] = { L.unionOff.p, L.chunkOff.p, L.workOff.p };
             int*       tot[3] = { c->groupStats.p + kGroupSizeCount + 2, c->groupStats.p + kGroupSizeCount + 3,
                                   c->groupStats.p + kGroupSizeCount + 4 };
-            launchExclusiveScans(c->stream, ng, 3, in, out, tot);
+            launchExclusiveScansWide(c->stream, ng, 3, in, out, tot);
         }
         int* const hp = pinnedInts();
         check(hipMemcpyAsync(hp, c->groupStats.p + kGroupSizeCount, 5 * sizeof(int), hipMemcpyDeviceToHost, c->stream),
@@ -523,9 +523,9 @@ void scheduleLayout(nbx_context* c, WorkLayout& L, bool adjoint)
     if (!adjoint && costBound < INT_MAX)
     {
         L.chunkCost.ensure(L.nchunk + 2);
-        L.costOff.ensure(L.nchunk + 1);
+        L.costOff.ensure(scanOutSize(L.nchunk));
         launchChunkCosts(c->stream, L.nchunk, L.chunks.p, c->nbtype.p, c->typeLJ.p, L.chunkCost.p);
-        launchExclusiveScan(c->stream, L.nchunk, L.chunkCost.p, L.costOff.p, L.chunkCost.p + L.nchunk + 1);
+        launchExclusiveScanWide(c->stream, L.nchunk, L.chunkCost.p, L.costOff.p, L.chunkCost.p + L.nchunk + 1);
         launchWaveSchedule(c->stream, numWaves, L.nchunk, L.chunks.p, L.nwork, kChunkWeight, L.ngroups, L.groups.p,
                            L.waveStart.p, L.costOff.p, L.chunkCost.p + L.nchunk + 1, c->numCU * kKernelWaves);
     }
@@ -1200,8 +1200,8 @@ int nbx_construct_pairlist(nbx_context* c)
         const int nc = c->ncluster;
         c->countJ.ensure(nc + 1);
         c->countI.ensure(nc + 1);
-        c->offJ.ensure(nc + 1);
-        c->offI.ensure(nc + 1);
+        c->offJ.ensure(scanOutSize(nc));
+        c->offI.ensure(scanOutSize(nc));
         c->totals.ensure(3);
         SearchArgs a{};
         a.g           = c->g;
@@ -1224,7 +1224,7 @@ int nbx_construct_pairlist(nbx_context* c)
             const int* in[2]  = { c->countJ.p, c->countI.p };
             int*       out[2] = { c->offJ.p, c->offI.p };
             int*       tot[2] = { c->totals.p, c->totals.p + 1 };
-            launchExclusiveScans(c->stream, nc, 2, in, out, tot);
+            launchExclusiveScansWide(c->stream, nc, 2, in, out, tot);
         };
         // the group pass's largest j-entry span per group size, read with
         // the list's totals (one wait for both)
@@ -1358,7 +1358,7 @@ int nbx_prune_pairlist(nbx_context* c, double rlistInner)
             const int* in[2]  = { c->countJ.p, c->countI.p };
             int*       out[2] = { c->offJ.p, c->offI.p };
             int*       tot[2] = { c->totals.p, c->totals.p + 1 };
-            launchExclusiveScans(c->stream, nc, 2, in, out, tot);
+            launchExclusiveScansWide(c->stream, nc, 2, in, out, tot);
         }
         int tot[2] = { 0, 0 };
         check(hipMemcpyAsync(tot, c->totals.p, sizeof(int) * 2, hipMemcpyDeviceToHost, c->stream), "read totals");

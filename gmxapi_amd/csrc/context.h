@@ -117,7 +117,7 @@ inline void buildOccurrences(hipStream_t s, WindowGather& w, int ngroups, const 
                              const int* unionList, int ncluster, int maxTotal = -1)
 {
     w.occCount.ensure(ncluster + 1);
-    w.occOff.ensure(ncluster + 1);
+    w.occOff.ensure(scanOutSize(ncluster));
     if (ncluster == 0)
     {
         check(hipMemsetAsync(w.occOff.p, 0, sizeof(int), s), "clear occurrences");
@@ -126,7 +126,7 @@ inline void buildOccurrences(hipStream_t s, WindowGather& w, int ngroups, const 
     }
     check(hipMemsetAsync(w.occCount.p, 0, sizeof(int) * ncluster, s), "clear occurrences");
     launchOccurrences(s, ngroups, groups, groupInfo, unionList, ncluster, w.occCount.p, nullptr, nullptr, false);
-    launchExclusiveScan(s, ncluster, w.occCount.p, w.occOff.p, w.occOff.p + ncluster);
+    launchExclusiveScanWide(s, ncluster, w.occCount.p, w.occOff.p, w.occOff.p + ncluster);
     int total = maxTotal;
     if (total < 0)
     {
@@ -165,7 +165,7 @@ inline void buildSegments(hipStream_t s, WindowGather& w, int ngroups, const int
     w.staging.ensure(std::max(total, 1));
     // the staging positions each cluster's gather reads
     w.gposCount.ensure(ncluster + 1);
-    w.gposOff.ensure(ncluster + 1);
+    w.gposOff.ensure(scanOutSize(ncluster));
     // every staged slot is read by at most one cluster's gather (a group's
     // segment holds G + union slots per wavefront, its occurrences own +
     // union entries), so the positions number at most total / slotF4
@@ -174,7 +174,7 @@ inline void buildSegments(hipStream_t s, WindowGather& w, int ngroups, const int
     {
         launchGatherList(s, ncluster, o.occOff.p, o.occ.p, w.segInfo.p, slotF4, w.gposCount.p, nullptr, nullptr,
                          false);
-        launchExclusiveScan(s, ncluster, w.gposCount.p, w.gposOff.p, w.gposOff.p + ncluster);
+        launchExclusiveScanWide(s, ncluster, w.gposCount.p, w.gposOff.p, w.gposOff.p + ncluster);
     }
     else
     {

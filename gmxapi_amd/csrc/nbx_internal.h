@@ -258,6 +258,22 @@ void launchExclusiveScan(hipStream_t s, int n, const int* in, int* out, int* tot
 // count (at most 3) independent scans of n elements each in one launch, one
 // workgroup each
 void launchExclusiveScans(hipStream_t s, int n, int count, const int* const* in, int* const* out, int* const* total);
+// The same scans over many workgroups for long lists (n >= kScanWideMin):
+// one launch sums tiles of kScanTile elements, a second scans each tile
+// after the sum of the tiles before it.  The tile sums are kept in the
+// output's tail, after out[n]: every out holds scanOutSize(n) ints.
+constexpr int kScanTile    = 4096;
+constexpr int kScanWideMin = 16384;
+inline size_t scanOutSize(size_t n)
+{
+    return n + 1 + (n + kScanTile - 1) / kScanTile;
+}
+void launchExclusiveScansWide(hipStream_t s, int n, int count, const int* const* in, int* const* out,
+                              int* const* total);
+inline void launchExclusiveScanWide(hipStream_t s, int n, const int* in, int* out, int* total)
+{
+    launchExclusiveScansWide(s, n, 1, &in, &out, &total);
+}
 
 // -- groups.hip
 struct GroupArgs

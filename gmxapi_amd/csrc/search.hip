@@ -573,6 +573,184 @@ __global__ __launch_bounds__(kScanThreads) void exclusiveScanKernel(int n, ScanS
     }
 }
 
+// Tiled scans: a workgroup of kTileThreads per tile of kScanTile elements,
+// each thread four int4 loads at (k * kTileThreads + thread) * 4 within the
+// tile, so every load instruction of a wavefront reads 1 KiB contiguously.
+constexpr int kTileThreads = 256;
+constexpr int kTileVec     = kScanTile / (4 * kTileThreads);
+static_assert(kTileVec * 4 * kTileThreads == kScanTile, "tile of whole int4 loads");
+
+__device__ __forceinline__ int4 tileLoad(const int* __restrict__ in, int n, int first, bool aligned)
+{
+    if (aligned && first + 4 <= n)
+    {
+        return *reinterpret_cast<const int4*>(in + first);
+    }
+    return make_int4(first < n ? in[first] : 0, first + 1 < n ? in[first + 1] : 0, first + 2 < n ? in[first + 2] : 0,
+                     first + 3 < n ? in[first + 3] : 0);
+}
+
+__device__ __forceinline__ int waveSum(int v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off /= 2)
+    {
+        v += __shfl_xor(v, off, 64);
+    }
+    return v;
+}
+
+// sums[tile] = the sum of the tile's elements (sums = out + n + 1)
+__global__ __launch_bounds__(kTileThreads) void scanTileSumKernel(int n, ScanSet set)
+{
+    const int* __restrict__ in = set.in[blockIdx.y];
+    int* __restrict__ sums     = set.out[blockIdx.y] + n + 1;
+    __shared__ int  sWave[kTileThreads / 64];
+    const bool      aligned = (reinterpret_cast<uintptr_t>(in) & 15) == 0;
+    const int       base    = blockIdx.x * kScanTile;
+    int             sum     = 0;
+#pragma unroll
+    for (int k = 0; k < kTileVec; k++)
+    {
+        const int4 v = tileLoad(in, n, base + (k * kTileThreads + threadIdx.x) * 4, aligned);
+        sum += (v.x + v.y) + (v.z + v.w);
+    }
+    sum = waveSum(sum);
+    if ((threadIdx.x & 63) == 0)
+    {
+        sWave[threadIdx.x >> 6] = sum;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        int t = 0;
+#pragma unroll
+        for (int w = 0; w < kTileThreads / 64; w++)
+        {
+            t += sWave[w];
+        }
+        sums[blockIdx.x] = t;
+    }
+}
+
+// The tile's exclusive scan after the sum of the tiles before it; the last
+// tile writes out[n] and *total.  Reads the tile sums, writes out[0, n].
+__global__ __launch_bounds__(kTileThreads) void scanTileKernel(int n, ScanSet set)
+{
+    const int* __restrict__ in   = set.in[blockIdx.y];
+    int* __restrict__       out  = set.out[blockIdx.y];
+    const int* __restrict__ sums = out + n + 1;
+    __shared__ int  sPrefix[kTileThreads / 64];
+    __shared__ int  sWave[kTileVec][kTileThreads / 64];
+    const int       lane     = threadIdx.x & 63;
+    const int       wave     = threadIdx.x >> 6;
+    const bool      alignIn  = (reinterpret_cast<uintptr_t>(in) & 15) == 0;
+    const bool      alignOut = (reinterpret_cast<uintptr_t>(out) & 15) == 0;
+    const int       base     = blockIdx.x * kScanTile;
+    int4            v[kTileVec];
+#pragma unroll
+    for (int k = 0; k < kTileVec; k++)
+    {
+        v[k] = tileLoad(in, n, base + (k * kTileThreads + threadIdx.x) * 4, alignIn);
+    }
+    int prefix = 0;
+    for (int t = threadIdx.x; t < static_cast<int>(blockIdx.x); t += kTileThreads)
+    {
+        prefix += sums[t];
+    }
+    prefix = waveSum(prefix);
+    // per load k: the thread's sum and its inclusive scan over the wavefront
+    int sum[kTileVec], incl[kTileVec];
+#pragma unroll
+    for (int k = 0; k < kTileVec; k++)
+    {
+        sum[k]  = (v[k].x + v[k].y) + (v[k].z + v[k].w);
+        incl[k] = sum[k];
+    }
+#pragma unroll
+    for (int off = 1; off < 64; off *= 2)
+    {
+#pragma unroll
+        for (int k = 0; k < kTileVec; k++)
+        {
+            const int t = __shfl_up(incl[k], off, 64);
+            incl[k] += (lane >= off) ? t : 0;
+        }
+    }
+    if (lane == 0)
+    {
+        sPrefix[wave] = prefix;
+    }
+    if (lane == 63)
+    {
+#pragma unroll
+        for (int k = 0; k < kTileVec; k++)
+        {
+            sWave[k][wave] = incl[k];
+        }
+    }
+    __syncthreads();
+    int run = 0;
+#pragma unroll
+    for (int w = 0; w < kTileThreads / 64; w++)
+    {
+        run += sPrefix[w];
+    }
+    // load k covers elements [k * 4 * kTileThreads, (k + 1) * 4 * kTileThreads)
+    // of the tile: its base is the running sum of the loads before it
+#pragma unroll
+    for (int k = 0; k < kTileVec; k++)
+    {
+        int before = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < kTileThreads / 64; w++)
+        {
+            const int t = sWave[k][w];
+            before += (w < wave) ? t : 0;
+            total += t;
+        }
+        int        r = run + before + incl[k] - sum[k];
+        int4       res;
+        res.x = r;
+        r += v[k].x;
+        res.y = r;
+        r += v[k].y;
+        res.z = r;
+        r += v[k].z;
+        res.w        = r;
+        const int e0 = base + (k * kTileThreads + threadIdx.x) * 4;
+        if (alignOut && e0 + 4 <= n)
+        {
+            *reinterpret_cast<int4*>(out + e0) = res;
+        }
+        else
+        {
+            if (e0 < n)
+            {
+                out[e0] = res.x;
+            }
+            if (e0 + 1 < n)
+            {
+                out[e0 + 1] = res.y;
+            }
+            if (e0 + 2 < n)
+            {
+                out[e0 + 2] = res.z;
+            }
+            if (e0 + 3 < n)
+            {
+                out[e0 + 3] = res.w;
+            }
+        }
+        run += total;
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
+    {
+        out[n]                 = run;
+        *set.total[blockIdx.y] = run;
+    }
+}
+
 } // namespace
 
 void launchSearch(hipStream_t s, const SearchArgs& a, bool fill)
@@ -619,6 +797,26 @@ void launchExclusiveScans(hipStream_t s, int n, int count, const int* const* in,
         set.total[k] = total[k];
     }
     exclusiveScanKernel<<<count, kScanThreads, 0, s>>>(n, set);
+}
+
+void launchExclusiveScansWide(hipStream_t s, int n, int count, const int* const* in, int* const* out,
+                              int* const* total)
+{
+    if (n < kScanWideMin)
+    {
+        launchExclusiveScans(s, n, count, in, out, total);
+        return;
+    }
+    ScanSet set{};
+    for (int k = 0; k < count; k++)
+    {
+        set.in[k]    = in[k];
+        set.out[k]   = out[k];
+        set.total[k] = total[k];
+    }
+    const dim3 grid((n + kScanTile - 1) / kScanTile, count);
+    scanTileSumKernel<<<grid, kTileThreads, 0, s>>>(n, set);
+    scanTileKernel<<<grid, kTileThreads, 0, s>>>(n, set);
 }
 
 } // namespace nbx
