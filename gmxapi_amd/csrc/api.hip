@@ -313,10 +313,8 @@ void buildWorkGroups(nbx_context* c, WorkLayout& L, bool adjoint, const int* spa
                        : nonbondedLdsBytes(c->ntype, c->ic.tab_n, c->elec, slots, c->opt.vdw_type == NBX_VDW_PME);
     };
     const int nc = c->ncluster;
-    // [0, kGroupSizeCount): largest j-entry span per candidate size, then
-    // max union, overflow flag and the three scan totals of one pass
-    constexpr int kStats = kGroupSizeCount + 5;
-    c->groupStats.ensure(kStats);
+    // the three scan totals of one pass (union entries, chunks, work items)
+    c->groupStats.ensure(3);
     int spans[kGroupSizeCount] = {};
     if (spansKnown)
     {
@@ -324,12 +322,15 @@ void buildWorkGroups(nbx_context* c, WorkLayout& L, bool adjoint, const int* spa
     }
     else
     {
-        check(hipMemsetAsync(c->groupStats.p, 0, sizeof(int) * kStats, c->stream), "clear stats");
-        launchGroupSpans(c->stream, nc, c->offJ.p, c->groupStats.p);
-        int* const hp = pinnedInts();
-        check(hipMemcpyAsync(hp, c->groupStats.p, sizeof(spans), hipMemcpyDeviceToHost, c->stream), "read spans");
+        c->statsPartial.ensure(kPartialMaxBlocks * partialMaxStride(2, true));
+        const PartialMaxArgs pm{ { nullptr, nullptr }, 0, c->offJ.p, nc, c->statsPartial.p };
+        const int            nb = launchPartialMax(c->stream, pm);
+        int* const           hp = pinnedInts();
+        check(hipMemcpyAsync(hp, c->statsPartial.p, sizeof(int) * nb * kGroupSizeCount, hipMemcpyDeviceToHost,
+                             c->stream),
+              "read spans");
         check(hipStreamSynchronize(c->stream), "group sync");
-        std::copy(hp, hp + kGroupSizeCount, spans);
+        reducePartialMax(hp, nb, kGroupSizeCount, spans);
     }
     c->jentryTmp.ensure(c->ncj);
     // the largest group size (or the chosen one) whose j-force windows, one
@@ -415,22 +416,25 @@ void buildWorkGroups(nbx_context* c, WorkLayout& L, bool adjoint, const int* spa
         ga.upperBound   = 1;
         ga.unionBySort  = unionBySort ? 1 : 0;
         launchBuildGroups(c->stream, ga, true);
-        check(hipMemsetAsync(c->groupStats.p + kGroupSizeCount, 0, sizeof(int) * 2, c->stream), "clear stats");
-        launchMaxReduce(c->stream, ng, L.unionCount.p, c->groupStats.p + kGroupSizeCount);
-        launchMaxReduce(c->stream, ng, L.overflow.p, c->groupStats.p + kGroupSizeCount + 1);
+        // the largest union and the overflow flag, as per-workgroup maxima
+        c->statsPartial.ensure(kPartialMaxBlocks * partialMaxStride(2, true));
+        const PartialMaxArgs pm{ { L.unionCount.p, L.overflow.p }, 2, nullptr, ng, c->statsPartial.p };
+        const int            statBlocks = launchPartialMax(c->stream, pm);
         {
             const int* in[3]  = { L.unionCount.p, L.chunkCount.p, L.workCount.p };
             int*       out[3] = { L.unionOff.p, L.chunkOff.p, L.workOff.p };
-            int*       tot[3] = { c->groupStats.p + kGroupSizeCount + 2, c->groupStats.p + kGroupSizeCount + 3,
-                                  c->groupStats.p + kGroupSizeCount + 4 };
+            int*       tot[3] = { c->groupStats.p, c->groupStats.p + 1, c->groupStats.p + 2 };
             launchExclusiveScansWide(c->stream, ng, 3, in, out, tot);
         }
         int* const hp = pinnedInts();
-        check(hipMemcpyAsync(hp, c->groupStats.p + kGroupSizeCount, 5 * sizeof(int), hipMemcpyDeviceToHost, c->stream),
+        check(hipMemcpyAsync(hp, c->groupStats.p, 3 * sizeof(int), hipMemcpyDeviceToHost, c->stream), "read totals");
+        check(hipMemcpyAsync(hp + 16, c->statsPartial.p, sizeof(int) * statBlocks * 2, hipMemcpyDeviceToHost,
+                             c->stream),
               "read stats");
         check(hipStreamSynchronize(c->stream), "group sync");
-        const int st[5] = { hp[0], hp[1], hp[2], hp[3], hp[4] };
-        const int maxUnion = st[0], overflow = st[1], unionTotal = st[2], chunkTotal = st[3], workTotal = st[4];
+        int mx[2];
+        reducePartialMax(hp + 16, statBlocks, 2, mx);
+        const int maxUnion = mx[0], overflow = mx[1], unionTotal = hp[0], chunkTotal = hp[1], workTotal = hp[2];
         if (overflow)
         {
             continue;
@@ -1226,22 +1230,32 @@ int nbx_construct_pairlist(nbx_context* c)
             int*       tot[2] = { c->totals.p, c->totals.p + 1 };
             launchExclusiveScansWide(c->stream, nc, 2, in, out, tot);
         };
-        // the group pass's largest j-entry span per group size, read with
-        // the list's totals (one wait for both)
+        // the list's totals, its largest j-entry count and the group pass's
+        // largest j-entry span per group size (per-workgroup maxima), read
+        // with one wait
         int  spans[kGroupSizeCount] = {};
-        auto readSpans              = [&] {
-            c->groupStats.ensure(kGroupSizeCount + 5);
-            check(hipMemsetAsync(c->groupStats.p, 0, sizeof(int) * kGroupSizeCount, c->stream), "clear spans");
-            launchGroupSpans(c->stream, nc, c->offJ.p, c->groupStats.p);
-            check(hipMemcpyAsync(pinnedInts() + 8, c->groupStats.p, sizeof(spans), hipMemcpyDeviceToHost,
-                                 c->stream),
-                  "read spans");
+        int  statBlocks             = 0;
+        auto readStats              = [&] {
+            constexpr int kStride = partialMaxStride(1, true);
+            c->statsPartial.ensure(kPartialMaxBlocks * partialMaxStride(2, true));
+            const PartialMaxArgs pm{ { c->countJ.p, nullptr }, 1, c->offJ.p, nc, c->statsPartial.p };
+            statBlocks = launchPartialMax(c->stream, pm);
+            check(hipMemcpyAsync(pinnedInts() + 16, c->statsPartial.p, sizeof(int) * statBlocks * kStride,
+                                 hipMemcpyDeviceToHost, c->stream),
+                  "read stats");
+            check(hipMemcpyAsync(pinnedInts(), c->totals.p, sizeof(int) * 2, hipMemcpyDeviceToHost, c->stream),
+                  "read totals");
         };
-        // after the wait: the spans and the list totals from the pinned buffer
+        // after the wait: j- and i-entry totals and the largest j-entry count
+        // in tot, the spans in spans
         auto takeReads = [&](int* tot) {
             const int* hp = pinnedInts();
-            std::copy(hp + 8, hp + 8 + kGroupSizeCount, spans);
-            std::copy(hp, hp + 3, tot);
+            int        st[partialMaxStride(1, true)];
+            reducePartialMax(hp + 16, statBlocks, partialMaxStride(1, true), st);
+            tot[0] = hp[0];
+            tot[1] = hp[1];
+            tot[2] = st[0];
+            std::copy(st + 1, st + 1 + kGroupSizeCount, spans);
         };
         // After a first search, the fill runs without a count pass: each
         // cluster writes into a region sized from the largest count seen
@@ -1258,13 +1272,9 @@ int nbx_construct_pairlist(nbx_context* c)
             a.ientry = c->searchITmp.p;
             launchSearch(c->stream, a, true);
             scanCounts();
-            check(hipMemsetAsync(c->totals.p + 2, 0, sizeof(int), c->stream), "clear max");
-            launchMaxReduce(c->stream, nc, c->countJ.p, c->totals.p + 2);
-            readSpans();
-            int tot[3] = { 0, 0, 0 };
-            check(hipMemcpyAsync(pinnedInts(), c->totals.p, sizeof(tot), hipMemcpyDeviceToHost, c->stream),
-                  "read totals");
+            readStats();
             check(hipStreamSynchronize(c->stream), "search sync");
+            int tot[3] = { 0, 0, 0 };
             takeReads(tot);
             if (tot[2] <= cap)
             {
@@ -1285,13 +1295,9 @@ int nbx_construct_pairlist(nbx_context* c)
             a.jentry = nullptr;
             launchSearch(c->stream, a, false);
             scanCounts();
-            check(hipMemsetAsync(c->totals.p + 2, 0, sizeof(int), c->stream), "clear max");
-            launchMaxReduce(c->stream, nc, c->countJ.p, c->totals.p + 2);
-            readSpans();
-            int tot[3] = { 0, 0, 0 };
-            check(hipMemcpyAsync(pinnedInts(), c->totals.p, sizeof(tot), hipMemcpyDeviceToHost, c->stream),
-                  "read totals");
+            readStats();
             check(hipStreamSynchronize(c->stream), "search sync");
+            int tot[3] = { 0, 0, 0 };
             takeReads(tot);
             c->ncj = tot[0];
             c->nci = tot[1];

@@ -46,7 +46,7 @@ inline void check(hipError_t e, const char* what)
 // device-to-host reads: a copy into pageable memory waits for its own staging
 // copy, one into pinned memory is queued, so several reads share one
 // stream synchronisation.  Callers use disjoint ranges within one wait.
-constexpr int kPinnedInts = 64;
+constexpr int kPinnedInts = 256;
 inline int* pinnedInts()
 {
     thread_local int* p = nullptr;
@@ -307,6 +307,7 @@ struct nbx_context
     // group size, sized for its larger window per cluster slot; built on
     // the first adjoint call after a search)
     nbx::DevBuf<int>    groupStats;
+    nbx::DevBuf<int>    statsPartial; // per-workgroup maxima (launchPartialMax)
     nbx::DevBuf<int>    typeLJ; // per type (padding type included): 1 if it has LJ with any type
     bool                anyTypeWithoutLJ = false; // among the user's types
     // LJ-PME: the grid's 6 C6 per type pair; the host's 6 C6 diagonal and

@@ -456,25 +456,42 @@ __global__ __launch_bounds__(kGroupThreads) void groupCompactKernel(GroupArgs a,
     }
 }
 
-// max over groups of (offJ[c0 + G] - offJ[c0]) for the candidate sizes
-__global__ void groupSpanKernel(int ncluster, const int* __restrict__ offJ, int* __restrict__ maxSpan)
+// Per-workgroup maxima for the host's buffer sizing, reduced by the host
+// (no zeroed accumulator, no atomics): workgroup b writes the maxima of the
+// a.nv arrays, then with a.offJ given those of the j-entry spans
+// offJ[min(c + G, n)] - offJ[c] of the groups of every candidate size G, to
+// partial[b * stride, (b + 1) * stride) with stride = partialMaxStride.
+__global__ __launch_bounds__(kPartialMaxThreads) void partialMaxKernel(PartialMaxArgs a)
 {
-    int m[kGroupSizeCount] = {};
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncluster; c += gridDim.x * blockDim.x)
+    constexpr int kVals = 2 + kGroupSizeCount;
+    int           m[kVals] = {};
+    for (int c = blockIdx.x * kPartialMaxThreads + threadIdx.x; c < a.n; c += gridDim.x * kPartialMaxThreads)
     {
 #pragma unroll
-        for (int k = 0; k < kGroupSizeCount; k++)
+        for (int k = 0; k < 2; k++)
         {
-            const int G = kGroupSizeMax >> k;
-            if (c % G == 0)
+            if (k < a.nv)
             {
-                m[k] = max(m[k], offJ[min(c + G, ncluster)] - offJ[c]);
+                m[k] = max(m[k], a.v[k][c]);
+            }
+        }
+        if (a.offJ)
+        {
+            const int o = a.offJ[c];
+#pragma unroll
+            for (int k = 0; k < kGroupSizeCount; k++)
+            {
+                const int G = kGroupSizeMax >> k;
+                if (c % G == 0)
+                {
+                    m[2 + k] = max(m[2 + k], a.offJ[min(c + G, a.n)] - o);
+                }
             }
         }
     }
-    // one atomic per wavefront and size
+    __shared__ int sWave[kPartialMaxThreads / 64][kVals];
 #pragma unroll
-    for (int k = 0; k < kGroupSizeCount; k++)
+    for (int k = 0; k < kVals; k++)
     {
         int v = m[k];
 #pragma unroll
@@ -482,27 +499,23 @@ __global__ void groupSpanKernel(int ncluster, const int* __restrict__ offJ, int*
         {
             v = max(v, __shfl_xor(v, o, 64));
         }
-        if ((threadIdx.x & 63) == 0 && v > 0)
+        if ((threadIdx.x & 63) == 0)
         {
-            atomicMax(&maxSpan[k], v);
+            sWave[threadIdx.x >> 6][k] = v;
         }
     }
-}
-
-__global__ void maxReduceKernel(int n, const int* __restrict__ v, int* __restrict__ out)
-{
-    int m = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    __syncthreads();
+    const int k = threadIdx.x;
+    if (k < kVals && (k < a.nv || (k >= 2 && a.offJ)))
     {
-        m = max(m, v[i]);
-    }
-    for (int o = 1; o < 64; o *= 2)
-    {
-        m = max(m, __shfl_xor(m, o, 64));
-    }
-    if ((threadIdx.x & 63) == 0)
-    {
-        atomicMax(out, m);
+        int v = 0;
+#pragma unroll
+        for (int w = 0; w < kPartialMaxThreads / 64; w++)
+        {
+            v = max(v, sWave[w][k]);
+        }
+        const int slot = k < 2 ? k : a.nv + (k - 2);
+        a.partial[blockIdx.x * partialMaxStride(a.nv, a.offJ != nullptr) + slot] = v;
     }
 }
 
@@ -1010,14 +1023,6 @@ void launchGroupCompact(hipStream_t s, const GroupArgs& a, const int* unionTmp, 
     }
 }
 
-void launchGroupSpans(hipStream_t s, int ncluster, const int* offJ, int* maxSpan)
-{
-    if (ncluster > 0)
-    {
-        groupSpanKernel<<<std::min(64, (ncluster + 255) / 256), 256, 0, s>>>(ncluster, offJ, maxSpan);
-    }
-}
-
 void launchBuildGroups(hipStream_t s, const GroupArgs& a, bool fill)
 {
     if (a.ngroups == 0)
@@ -1034,11 +1039,26 @@ void launchBuildGroups(hipStream_t s, const GroupArgs& a, bool fill)
     }
 }
 
-void launchMaxReduce(hipStream_t s, int n, const int* v, int* out)
+int launchPartialMax(hipStream_t s, const PartialMaxArgs& a)
 {
-    if (n > 0)
+    if (a.n <= 0)
     {
-        maxReduceKernel<<<std::min(1024, (n + 255) / 256), 256, 0, s>>>(n, v, out);
+        return 0;
+    }
+    const int blocks = std::min(kPartialMaxBlocks, (a.n + kPartialMaxThreads - 1) / kPartialMaxThreads);
+    partialMaxKernel<<<blocks, kPartialMaxThreads, 0, s>>>(a);
+    return blocks;
+}
+
+void reducePartialMax(const int* partial, int blocks, int stride, int* out)
+{
+    std::fill(out, out + stride, 0);
+    for (int b = 0; b < blocks; b++)
+    {
+        for (int j = 0; j < stride; j++)
+        {
+            out[j] = std::max(out[j], partial[b * stride + j]);
+        }
     }
 }
 

@@ -316,13 +316,33 @@ constexpr unsigned kSlotOverflow = 0xffffu;
 // on the order of the atomics, so the sums are reproducible.  Resolution
 // 2^-20 (about 1e-6 kJ mol^-1 nm^-1), range +-2^43 (about 8.8e12).
 constexpr int kFixBits = 20;
-void launchGroupSpans(hipStream_t s, int ncluster, const int* offJ, int* maxSpan);
 void launchBuildGroups(hipStream_t s, const GroupArgs& a, bool fill);
 // moves an upper-bound layout (unionTmp, workTmp, chunksTmp) to a.unionOff,
 // a.workOff, a.chunkOff in a.unionList, a.work, a.chunks; rebases a.groups
 void launchGroupCompact(hipStream_t s, const GroupArgs& a, const int* unionTmp, const uint2* workTmp,
                         const int4* chunksTmp);
-void launchMaxReduce(hipStream_t s, int n, const int* v, int* out);
+// Maxima for buffer sizing, as per-workgroup partials that the host reads
+// and reduces (reducePartialMax): of nv (at most 2) arrays of n non-negative
+// ints, and with offJ (the scanned j-entry counts of n clusters) of the
+// j-entry span of every group of each candidate size (kGroupSizeMax, ...,
+// 1, in that order, after the arrays').  partial holds kPartialMaxBlocks *
+// stride ints; launchPartialMax returns the number of workgroups.
+constexpr int kPartialMaxThreads = 256;
+constexpr int kPartialMaxBlocks  = 32;
+struct PartialMaxArgs
+{
+    const int* v[2];
+    int        nv;
+    const int* offJ;
+    int        n;
+    int*       partial;
+};
+__host__ __device__ constexpr int partialMaxStride(int nv, bool spans)
+{
+    return nv + (spans ? kGroupSizeCount : 0);
+}
+int  launchPartialMax(hipStream_t s, const PartialMaxArgs& a);
+void reducePartialMax(const int* partial, int blocks, int stride, int* out);
 // Occurrences of every cluster in the groups' j-force windows (own slot and
 // windowed-union slots) as a CSR list of {group, slot}, sorted per cluster.
 // Call once with fill = false (count must be zeroed; it returns the counts),
