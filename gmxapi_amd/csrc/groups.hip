@@ -768,25 +768,6 @@ __global__ void chunkShiftKernel(int nchunk, const int4* __restrict__ chunks, co
     }
 }
 
-// the group holding chunk k: the first whose chunk range ends after it
-__device__ int groupOfChunk(int ngroups, const int4* __restrict__ groups, int k)
-{
-    int lo = 0, hi = ngroups;
-    while (lo < hi)
-    {
-        const int m = (lo + hi) / 2;
-        if (groups[m].y <= k)
-        {
-            lo = m + 1;
-        }
-        else
-        {
-            hi = m;
-        }
-    }
-    return lo;
-}
-
 // Measured cost of a chunk in the pair kernel, in 1/32 tile: 32 per tile,
 // plus 1.5 per tile for each i-atom whose type has LJ (the kernel skips the
 // LJ terms of the others), plus 27 per chunk (its set-up and i-force
@@ -827,6 +808,49 @@ __device__ long long generationWeight(int w, int numWaves, int genWaves)
     return w <= genWaves ? first * w : first * genWaves + second * (w - genWaves);
 }
 
+// Smallest m in [0, n) with key(m) >= target (n if none) for a
+// non-decreasing key, from an LDS sample s[i] = key(i * stride), i < ns =
+// ceil(n / stride): a search of the sample, then of the at most stride
+// entries it leaves, so the dependent global loads number log2(stride).
+template<typename T, typename Key>
+__device__ int sampledLowerBound(const T* s, int ns, int stride, int n, T target, Key key)
+{
+    int a = 0, b = ns;
+    while (a < b)
+    {
+        const int m = (a + b) / 2;
+        if (s[m] < target)
+        {
+            a = m + 1;
+        }
+        else
+        {
+            b = m;
+        }
+    }
+    // key((a - 1) * stride) < target <= key(a * stride) (or a == ns)
+    int lo = a > 0 ? (a - 1) * stride + 1 : 0;
+    int hi = min(a * stride, n);
+    while (lo < hi)
+    {
+        const int m = (lo + hi) / 2;
+        if (key(m) < target)
+        {
+            lo = m + 1;
+        }
+        else
+        {
+            hi = m;
+        }
+    }
+    return lo;
+}
+
+constexpr int kSchedThreads    = 1024;
+constexpr int kSchedSample     = 4096; // LDS sample entries of each key
+constexpr int kSchedCostStride = 64;   // least sample stride over the chunks
+constexpr int kSchedGroupStride = 16;  // and over the groups
+
 // One workgroup.  Wavefront w's target start is the first chunk whose cost
 // offset reaches the wavefront's cumulative share of the total; equal
 // targets are merged, so the runs are non-empty and in order, and the
@@ -834,63 +858,79 @@ __device__ long long generationWeight(int w, int numWaves, int genWaves)
 // runs at the end.  A group's chunks are then run by consecutive
 // wavefronts, each of which writes its staging segment.  The cost offset of
 // chunk m is costOff[m] (total costTotal) when given, else its work offset
-// plus chunkWeight per earlier chunk.
-__global__ __launch_bounds__(1024) void waveScheduleKernel(int numWaves, int nchunk, const int4* __restrict__ chunks,
-                                                          int nwork, int chunkWeight, int ngroups,
-                                                          const int4* __restrict__ groups, const int* __restrict__ costOff,
-                                                          const int* __restrict__ costTotal, int genWaves,
-                                                          int2* __restrict__ out)
+// plus chunkWeight per earlier chunk.  The searches (a start's chunk, the
+// group holding it) go through LDS samples of the cost offsets and of the
+// groups' end chunks.
+__global__ __launch_bounds__(kSchedThreads) void waveScheduleKernel(int numWaves, int nchunk,
+                                                                   const int4* __restrict__ chunks, int nwork,
+                                                                   int chunkWeight, int ngroups,
+                                                                   const int4* __restrict__ groups,
+                                                                   const int* __restrict__ costOff,
+                                                                   const int* __restrict__ costTotal, int genWaves,
+                                                                   int2* __restrict__ out)
 {
-    __shared__ int sLo[1024];
-    __shared__ int sFlag[1024];
+    __shared__ long long sCost[kSchedSample];
+    __shared__ int       sGroupEnd[kSchedSample];
+    __shared__ int       sLo[kSchedThreads];
+    __shared__ int       sWave[kSchedThreads / 64];
+    auto costKey = [&](int m) -> long long {
+        return costOff ? static_cast<long long>(costOff[m]) : chunks[m].z + static_cast<long long>(chunkWeight) * m;
+    };
+    auto endKey = [&](int g) -> int { return groups[g].y; };
+    const int cStride = max(kSchedCostStride, (nchunk + kSchedSample - 1) / kSchedSample);
+    const int cn      = (nchunk + cStride - 1) / cStride;
+    const int gStride = max(kSchedGroupStride, (ngroups + kSchedSample - 1) / kSchedSample);
+    const int gn      = (ngroups + gStride - 1) / gStride;
+    for (int i = threadIdx.x; i < cn; i += kSchedThreads)
+    {
+        sCost[i] = costKey(i * cStride);
+    }
+    for (int i = threadIdx.x; i < gn; i += kSchedThreads)
+    {
+        sGroupEnd[i] = endKey(i * gStride);
+    }
+    __syncthreads();
     const long long total  = costOff ? static_cast<long long>(*costTotal)
                                      : static_cast<long long>(nwork) + static_cast<long long>(chunkWeight) * nchunk;
     const long long wTotal = generationWeight(numWaves, numWaves, genWaves);
+    const int       lane   = threadIdx.x & 63;
+    const int       wave   = threadIdx.x >> 6;
     int             carry = 0, prevLo = -1;
-    for (int base = 0; base < numWaves; base += 1024)
+    for (int base = 0; base < numWaves; base += kSchedThreads)
     {
         const int w  = base + threadIdx.x;
         int       lo = nchunk;
         if (w < numWaves)
         {
             const long long target = total * generationWeight(w, numWaves, genWaves) / wTotal;
-            int             hi     = nchunk;
-            lo                     = 0;
-            while (lo < hi)
-            {
-                const int       m = (lo + hi) / 2;
-                const long long c = costOff ? static_cast<long long>(costOff[m])
-                                            : chunks[m].z + static_cast<long long>(chunkWeight) * m;
-                if (c < target)
-                {
-                    lo = m + 1;
-                }
-                else
-                {
-                    hi = m;
-                }
-            }
+            lo                     = sampledLowerBound(sCost, cn, cStride, nchunk, target, costKey);
         }
         sLo[threadIdx.x] = lo;
         __syncthreads();
-        const int prev = threadIdx.x > 0 ? sLo[threadIdx.x - 1] : prevLo;
-        const int flag = (w < numWaves && lo < nchunk && lo != prev) ? 1 : 0;
-        sFlag[threadIdx.x] = flag;
-        __syncthreads();
-        for (int off = 1; off < 1024; off *= 2)
+        const int                prev = threadIdx.x > 0 ? sLo[threadIdx.x - 1] : prevLo;
+        const bool               flag = w < numWaves && lo < nchunk && lo != prev;
+        const unsigned long long bal  = __ballot(flag);
+        if (lane == 0)
         {
-            const int t = threadIdx.x >= off ? sFlag[threadIdx.x - off] : 0;
-            __syncthreads();
-            sFlag[threadIdx.x] += t;
-            __syncthreads();
+            sWave[wave] = __popcll(bal);
+        }
+        __syncthreads();
+        int before = 0, roundTotal = 0;
+#pragma unroll
+        for (int v = 0; v < kSchedThreads / 64; v++)
+        {
+            const int t = sWave[v];
+            before += (v < wave) ? t : 0;
+            roundTotal += t;
         }
         if (flag)
         {
-            const int r = carry + sFlag[threadIdx.x] - 1;
-            out[r]      = make_int2(lo, groupOfChunk(ngroups, groups, lo));
+            const int r = carry + before + __popcll(bal & ((1ull << lane) - 1ull));
+            // the group holding chunk lo: the first whose end exceeds it
+            out[r] = make_int2(lo, sampledLowerBound(sGroupEnd, gn, gStride, ngroups, lo + 1, endKey));
         }
-        carry += sFlag[1023];
-        prevLo = sLo[1023];
+        carry += roundTotal;
+        prevLo = sLo[kSchedThreads - 1];
         __syncthreads();
     }
     for (int r = carry + threadIdx.x; r <= numWaves; r += blockDim.x)
@@ -995,8 +1035,8 @@ void launchWaveSchedule(hipStream_t s, int numWaves, int nchunk, const int4* chu
                         int ngroups, const int4* groups, int2* out, const int* costOff, const int* costTotal,
                         int genWaves)
 {
-    waveScheduleKernel<<<1, 1024, 0, s>>>(numWaves, nchunk, chunks, nwork, chunkWeight, ngroups, groups, costOff,
-                                          costTotal, genWaves, out);
+    waveScheduleKernel<<<1, kSchedThreads, 0, s>>>(numWaves, nchunk, chunks, nwork, chunkWeight, ngroups, groups,
+                                                   costOff, costTotal, genWaves, out);
 }
 
 void launchChunkCosts(hipStream_t s, int nchunk, const int4* chunks, const int* type, const int* typeLJ, int* cost)
