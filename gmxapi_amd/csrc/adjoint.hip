@@ -483,7 +483,11 @@ __global__ __launch_bounds__(256) void adjointGatherStagedKernel(int ncluster, c
                                                                  float* __restrict__ outX, double* __restrict__ outQ)
 {
     const int lane = threadIdx.x & 63;
-    const int c    = blockIdx.x * 4 + (threadIdx.x >> 6);
+    // the workgroups of one XCD (dealt round robin, the grid a multiple of
+    // kGatherXcds) take consecutive clusters, so a cache line that
+    // neighbouring clusters' staged slots share is fetched into one L2
+    const int lb = (blockIdx.x % kGatherXcds) * (gridDim.x / kGatherXcds) + blockIdx.x / kGatherXcds;
+    const int c  = lb * 4 + (threadIdx.x >> 6);
     if (c >= ncluster)
     {
         return;
@@ -620,7 +624,8 @@ void launchAdjoint(hipStream_t s, const AdjointArgs& a, int elec, int forwardBlo
     }
     if (a.ncluster > 0 && (gradXOut || gradQOut))
     {
-        adjointGatherStagedKernel<<<(a.ncluster + 3) / 4, 256, 0, s>>>(a.ncluster, a.gposOff, a.gpos, a.staging,
+        const int blocks = ((a.ncluster + 3) / 4 + kGatherXcds - 1) / kGatherXcds * kGatherXcds;
+        adjointGatherStagedKernel<<<blocks, 256, 0, s>>>(a.ncluster, a.gposOff, a.gpos, a.staging,
                                                                         a.gradX, a.gradQ, atomIndices, gradXOut,
                                                                         gradQOut);
     }

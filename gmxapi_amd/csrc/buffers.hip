@@ -403,7 +403,14 @@ __global__ __launch_bounds__(kGatherThreads) void gatherForcesKernel(int ncluste
     const int         wave = threadIdx.x >> 6;
     const int         idx  = lane >> 2; // component 3 * atom + d of this lane's sum
     double            vir[3] = { 0.0, 0.0, 0.0 };
-    for (int c = blockIdx.x * kWaves + wave; c < ncluster; c += gridDim.x * kWaves)
+    // The dispatcher deals workgroups round robin over the XCDs, each with
+    // its own L2: with the grid a multiple of kGatherXcds, the workgroups of
+    // one XCD take consecutive clusters, so a cache line that neighbouring
+    // clusters' staged slots share is fetched into one L2 only.
+    const int lb = gridDim.x % kGatherXcds == 0
+                       ? (blockIdx.x % kGatherXcds) * (gridDim.x / kGatherXcds) + blockIdx.x / kGatherXcds
+                       : blockIdx.x;
+    for (int c = lb * kWaves + wave; c < ncluster; c += gridDim.x * kWaves)
     {
         float f[12];
 #pragma unroll
@@ -636,7 +643,8 @@ int launchGatherForces(hipStream_t s, int ncluster, const int* gposOff, const in
     {
         return 0;
     }
-    const int blocks = std::min(kGatherMaxBlocks, (ncluster + 3) / 4);
+    // a multiple of kGatherXcds (kGatherMaxBlocks is one)
+    const int blocks = std::min(kGatherMaxBlocks, ((ncluster + 3) / 4 + kGatherXcds - 1) / kGatherXcds * kGatherXcds);
     if (f)
     {
         gatherForcesKernel<true><<<blocks, kGatherThreads, 0, s>>>(ncluster, gposOff, gpos, staging, fnb, atomIndices,

@@ -523,6 +523,10 @@ void launchConvertX(hipStream_t s, int nslot, const int* atomIndices, const floa
 // coordinates xq, 9 doubles per workgroup.  Returns the workgroup count
 // (at most kGatherMaxBlocks).
 constexpr int kGatherMaxBlocks = 4096;
+// MI355X: 8 XCDs of 32 CUs, each with its own L2; the gathers map
+// consecutive clusters to one XCD's workgroups
+constexpr int kGatherXcds = 8;
+static_assert(kGatherMaxBlocks % kGatherXcds == 0, "gather grid a multiple of the XCDs");
 int launchGatherForces(hipStream_t s, int ncluster, const int* gposOff, const int* gpos, const float4* staging,
                        float4* fnb, const int* atomIndices, float* f, bool addToF, const float4* xq,
                        double* virPart, const unsigned long long* fixF = nullptr);
