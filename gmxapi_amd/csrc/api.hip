@@ -302,6 +302,15 @@ const float* deviceCoordinates(nbx_context* c, const float* x, int onDevice)
     return c->x.p;
 }
 
+// The list build's sizing reads: kStatsHead ints of scan totals, then the
+// per-workgroup maxima of launchPartialMax, read back in one copy.
+constexpr int kStatsHead = 4;
+static_assert(kStatsHead + kPartialMaxBlocks * partialMaxStride(2, true) <= kPinnedInts, "stats fit the pinned buffer");
+static void ensureStats(nbx_context* c)
+{
+    c->statsPartial.ensure(kStatsHead + kPartialMaxBlocks * partialMaxStride(2, true));
+}
+
 // Groups of consecutive i-clusters for the kernel's LDS j-force window
 // (groups.hip): the largest group size whose j-entries fit the LDS sort and
 // whose window fits the kernel's LDS budget (the pair kernel's, or with
@@ -313,8 +322,6 @@ void buildWorkGroups(nbx_context* c, WorkLayout& L, bool adjoint, const int* spa
                        : nonbondedLdsBytes(c->ntype, c->ic.tab_n, c->elec, slots, c->opt.vdw_type == NBX_VDW_PME);
     };
     const int nc = c->ncluster;
-    // the three scan totals of one pass (union entries, chunks, work items)
-    c->groupStats.ensure(3);
     int spans[kGroupSizeCount] = {};
     if (spansKnown)
     {
@@ -322,12 +329,12 @@ void buildWorkGroups(nbx_context* c, WorkLayout& L, bool adjoint, const int* spa
     }
     else
     {
-        c->statsPartial.ensure(kPartialMaxBlocks * partialMaxStride(2, true));
-        const PartialMaxArgs pm{ { nullptr, nullptr }, 0, c->offJ.p, nc, c->statsPartial.p };
+        ensureStats(c);
+        const PartialMaxArgs pm{ { nullptr, nullptr }, 0, c->offJ.p, nc, c->statsPartial.p + kStatsHead };
         const int            nb = launchPartialMax(c->stream, pm);
         int* const           hp = pinnedInts();
-        check(hipMemcpyAsync(hp, c->statsPartial.p, sizeof(int) * nb * kGroupSizeCount, hipMemcpyDeviceToHost,
-                             c->stream),
+        check(hipMemcpyAsync(hp, c->statsPartial.p + kStatsHead, sizeof(int) * nb * kGroupSizeCount,
+                             hipMemcpyDeviceToHost, c->stream),
               "read spans");
         check(hipStreamSynchronize(c->stream), "group sync");
         reducePartialMax(hp, nb, kGroupSizeCount, spans);
@@ -417,23 +424,23 @@ void buildWorkGroups(nbx_context* c, WorkLayout& L, bool adjoint, const int* spa
         ga.unionBySort  = unionBySort ? 1 : 0;
         launchBuildGroups(c->stream, ga, true);
         // the largest union and the overflow flag, as per-workgroup maxima
-        c->statsPartial.ensure(kPartialMaxBlocks * partialMaxStride(2, true));
-        const PartialMaxArgs pm{ { L.unionCount.p, L.overflow.p }, 2, nullptr, ng, c->statsPartial.p };
+        // after the three scan totals (one read)
+        ensureStats(c);
+        const PartialMaxArgs pm{ { L.unionCount.p, L.overflow.p }, 2, nullptr, ng, c->statsPartial.p + kStatsHead };
         const int            statBlocks = launchPartialMax(c->stream, pm);
         {
             const int* in[3]  = { L.unionCount.p, L.chunkCount.p, L.workCount.p };
             int*       out[3] = { L.unionOff.p, L.chunkOff.p, L.workOff.p };
-            int*       tot[3] = { c->groupStats.p, c->groupStats.p + 1, c->groupStats.p + 2 };
+            int*       tot[3] = { c->statsPartial.p, c->statsPartial.p + 1, c->statsPartial.p + 2 };
             launchExclusiveScansWide(c->stream, ng, 3, in, out, tot);
         }
         int* const hp = pinnedInts();
-        check(hipMemcpyAsync(hp, c->groupStats.p, 3 * sizeof(int), hipMemcpyDeviceToHost, c->stream), "read totals");
-        check(hipMemcpyAsync(hp + 16, c->statsPartial.p, sizeof(int) * statBlocks * 2, hipMemcpyDeviceToHost,
+        check(hipMemcpyAsync(hp, c->statsPartial.p, sizeof(int) * (kStatsHead + statBlocks * 2), hipMemcpyDeviceToHost,
                              c->stream),
               "read stats");
         check(hipStreamSynchronize(c->stream), "group sync");
         int mx[2];
-        reducePartialMax(hp + 16, statBlocks, 2, mx);
+        reducePartialMax(hp + kStatsHead, statBlocks, 2, mx);
         const int maxUnion = mx[0], overflow = mx[1], unionTotal = hp[0], chunkTotal = hp[1], workTotal = hp[2];
         if (overflow)
         {
@@ -1207,6 +1214,7 @@ int nbx_construct_pairlist(nbx_context* c)
         c->offJ.ensure(scanOutSize(nc));
         c->offI.ensure(scanOutSize(nc));
         c->totals.ensure(3);
+        ensureStats(c);
         SearchArgs a{};
         a.g           = c->g;
         a.rlist       = c->rlist;
@@ -1227,7 +1235,7 @@ int nbx_construct_pairlist(nbx_context* c)
         auto scanCounts = [&] {
             const int* in[2]  = { c->countJ.p, c->countI.p };
             int*       out[2] = { c->offJ.p, c->offI.p };
-            int*       tot[2] = { c->totals.p, c->totals.p + 1 };
+            int*       tot[2] = { c->statsPartial.p, c->statsPartial.p + 1 };
             launchExclusiveScansWide(c->stream, nc, 2, in, out, tot);
         };
         // the list's totals, its largest j-entry count and the group pass's
@@ -1237,21 +1245,18 @@ int nbx_construct_pairlist(nbx_context* c)
         int  statBlocks             = 0;
         auto readStats              = [&] {
             constexpr int kStride = partialMaxStride(1, true);
-            c->statsPartial.ensure(kPartialMaxBlocks * partialMaxStride(2, true));
-            const PartialMaxArgs pm{ { c->countJ.p, nullptr }, 1, c->offJ.p, nc, c->statsPartial.p };
+            const PartialMaxArgs pm{ { c->countJ.p, nullptr }, 1, c->offJ.p, nc, c->statsPartial.p + kStatsHead };
             statBlocks = launchPartialMax(c->stream, pm);
-            check(hipMemcpyAsync(pinnedInts() + 16, c->statsPartial.p, sizeof(int) * statBlocks * kStride,
+            check(hipMemcpyAsync(pinnedInts(), c->statsPartial.p, sizeof(int) * (kStatsHead + statBlocks * kStride),
                                  hipMemcpyDeviceToHost, c->stream),
                   "read stats");
-            check(hipMemcpyAsync(pinnedInts(), c->totals.p, sizeof(int) * 2, hipMemcpyDeviceToHost, c->stream),
-                  "read totals");
         };
         // after the wait: j- and i-entry totals and the largest j-entry count
         // in tot, the spans in spans
         auto takeReads = [&](int* tot) {
             const int* hp = pinnedInts();
             int        st[partialMaxStride(1, true)];
-            reducePartialMax(hp + 16, statBlocks, partialMaxStride(1, true), st);
+            reducePartialMax(hp + kStatsHead, statBlocks, partialMaxStride(1, true), st);
             tot[0] = hp[0];
             tot[1] = hp[1];
             tot[2] = st[0];

@@ -46,7 +46,7 @@ inline void check(hipError_t e, const char* what)
 // device-to-host reads: a copy into pageable memory waits for its own staging
 // copy, one into pinned memory is queued, so several reads share one
 // stream synchronisation.  Callers use disjoint ranges within one wait.
-constexpr int kPinnedInts = 256;
+constexpr int kPinnedInts = 1024;
 inline int* pinnedInts()
 {
     thread_local int* p = nullptr;
@@ -306,7 +306,6 @@ struct nbx_context
     // kernel work layouts: the pair kernel's, and the adjoint's (its own
     // group size, sized for its larger window per cluster slot; built on
     // the first adjoint call after a search)
-    nbx::DevBuf<int>    groupStats;
     nbx::DevBuf<int>    statsPartial; // per-workgroup maxima (launchPartialMax)
     nbx::DevBuf<int>    typeLJ; // per type (padding type included): 1 if it has LJ with any type
     bool                anyTypeWithoutLJ = false; // among the user's types

@@ -328,7 +328,7 @@ void launchGroupCompact(hipStream_t s, const GroupArgs& a, const int* unionTmp, 
 // 1, in that order, after the arrays').  partial holds kPartialMaxBlocks *
 // stride ints; launchPartialMax returns the number of workgroups.
 constexpr int kPartialMaxThreads = 256;
-constexpr int kPartialMaxBlocks  = 32;
+constexpr int kPartialMaxBlocks  = 128;
 struct PartialMaxArgs
 {
     const int* v[2];
