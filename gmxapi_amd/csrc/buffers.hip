@@ -184,7 +184,7 @@ __global__ __launch_bounds__(kFinalizeThreads) void finalizeKernel(int numEnergy
     if (virOut && comp >= vir0)
     {
         const int k = comp - vir0, d1 = k / 3, d2 = k % 3;
-        acc         = strideSum(virPart, numVirPartials, 9, k);
+        acc         = strideSum(virPart + static_cast<size_t>(k) * numVirPartials, numVirPartials, 1, 0);
         // the shift-force partials as one array of elements i = 45 b + s
         // (component d2 at 3 i + d2): consecutive threads read consecutive
         // elements, eight independent loads per thread and round
@@ -490,7 +490,8 @@ __global__ __launch_bounds__(kGatherThreads) void gatherForcesKernel(int ncluste
                     t += sVir[w][3 * at + d2][d1];
                 }
             }
-            virPart[blockIdx.x * 9 + threadIdx.x] = -0.5 * t;
+            // component-major, [9][workgroup], for the final sums
+            virPart[static_cast<size_t>(threadIdx.x) * gridDim.x + blockIdx.x] = -0.5 * t;
         }
     }
 }
