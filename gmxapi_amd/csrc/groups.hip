@@ -524,9 +524,13 @@ __global__ __launch_bounds__(kPartialMaxThreads) void partialMaxKernel(PartialMa
 template<bool kFill>
 __global__ void occurrenceKernel(int ngroups, const int4* __restrict__ groups, const int4* __restrict__ groupInfo,
                                  const int* __restrict__ unionList, int* __restrict__ count,
-                                 const int* __restrict__ occOff, int2* __restrict__ occ)
+                                 const int* __restrict__ occOff, int2* __restrict__ occ, int ncluster)
 {
     const int  g    = blockIdx.x;
+    if (kFill && g == 0 && threadIdx.x == 0)
+    {
+        count[ncluster] = 0; // the sort's long-list counter
+    }
     const int4 grp  = groups[g];
     const int4 gi   = groupInfo[g];
     const int  nOwn = gi.y & 0xffff;
@@ -542,109 +546,213 @@ __global__ void occurrenceKernel(int ngroups, const int4* __restrict__ groups, c
     }
 }
 
-// each cluster's occurrences in (group, slot) order, so that the force
-// gather adds them in a fixed order.  One workgroup per cluster: up to 64
-// entries are sorted by its first wavefront with a bitonic network across
-// the lanes, up to kSortMax by the workgroup with a bitonic network in LDS,
-// longer lists (not expected at supported cut-offs) by one thread.
-constexpr int kSortMax = 1024;
-
-__global__ __launch_bounds__(64) void occurrenceSortKernel(const int* __restrict__ occOff, int2* __restrict__ occ)
+// Each cluster's occurrences in (group, slot) order, so that the force
+// gather adds them in a fixed order; a (group, slot) pair is one 64-bit key,
+// and a cluster occurs at most once per group, so the keys are distinct.
+// A wavefront per cluster sorts up to 256 entries in registers (R = 1, 2 or
+// 4 keys per lane, key e = 64 r + lane in register r) with a bitonic
+// network: partners 64 or more apart are in the lane's own registers, nearer
+// ones in another lane (shuffles).  Longer lists (not expected at supported
+// cut-offs) are listed and sorted afterwards by a workgroup each, in LDS.
+__device__ __forceinline__ unsigned long long shflXorU64(unsigned long long v, int j)
 {
-    __shared__ unsigned long long key[kSortMax];
-    const int c = blockIdx.x;
-    const int b = occOff[c], n = occOff[c + 1] - b;
-    if (n <= 64)
+    const unsigned lo = __shfl_xor(static_cast<unsigned>(v), j, 64);
+    const unsigned hi = __shfl_xor(static_cast<unsigned>(v >> 32), j, 64);
+    return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+
+template<int R>
+__device__ void waveBitonicSort(unsigned long long (&k)[R], int lane)
+{
+#pragma unroll
+    for (int s = 2; s <= 64 * R; s <<= 1)
     {
-        const int lane = threadIdx.x;
-        // keys (group, slot) as one 64-bit value; unused lanes sort last
-        unsigned hi = 0xffffffffu, lo = 0xffffffffu;
-        if (lane < n)
+#pragma unroll
+        for (int j = s >> 1; j > 0; j >>= 1)
         {
-            const int2 v = occ[b + lane];
-            hi           = static_cast<unsigned>(v.x);
-            lo           = static_cast<unsigned>(v.y);
-        }
-        for (int k = 2; k <= 64; k <<= 1)
-        {
-            for (int j = k >> 1; j > 0; j >>= 1)
+            if (j >= 64)
             {
-                const unsigned ohi     = __shfl_xor(hi, j, 64);
-                const unsigned olo     = __shfl_xor(lo, j, 64);
-                const bool     less    = ohi < hi || (ohi == hi && olo < lo); // other < mine
-                const bool     keepMin = ((lane & j) == 0) == ((lane & k) == 0);
-                if (keepMin == less)
+                const int jr = j / 64;
+#pragma unroll
+                for (int r = 0; r < R; r++)
                 {
-                    hi = ohi;
-                    lo = olo;
-                }
-            }
-        }
-        if (lane < n)
-        {
-            occ[b + lane] = make_int2(static_cast<int>(hi), static_cast<int>(lo));
-        }
-        return;
-    }
-    if (n > kSortMax)
-    {
-        if (threadIdx.x == 0)
-        {
-            for (int i = b + 1; i < b + n; i++)
-            {
-                const int2 v = occ[i];
-                int        j = i - 1;
-                while (j >= b && (occ[j].x > v.x || (occ[j].x == v.x && occ[j].y > v.y)))
-                {
-                    occ[j + 1] = occ[j];
-                    j--;
-                }
-                occ[j + 1] = v;
-            }
-        }
-        return;
-    }
-    int m = 128;
-    while (m < n)
-    {
-        m <<= 1;
-    }
-    for (int i = threadIdx.x; i < m; i += blockDim.x)
-    {
-        if (i < n)
-        {
-            const int2 v = occ[b + i];
-            key[i] = (static_cast<unsigned long long>(static_cast<unsigned>(v.x)) << 32) | static_cast<unsigned>(v.y);
-        }
-        else
-        {
-            key[i] = ~0ull;
-        }
-    }
-    __syncthreads();
-    for (int k = 2; k <= m; k <<= 1)
-    {
-        for (int j = k >> 1; j > 0; j >>= 1)
-        {
-            for (int i = threadIdx.x; i < m; i += blockDim.x)
-            {
-                const int l = i ^ j;
-                if (l > i)
-                {
-                    const unsigned long long x = key[i], y = key[l];
-                    if ((x > y) == ((i & k) == 0))
+                    const int p = r ^ jr;
+                    if (p > r)
                     {
-                        key[i] = y;
-                        key[l] = x;
+                        // r holds the lower position: the smaller key when ascending
+                        const bool               asc = ((r * 64 + lane) & s) == 0;
+                        const unsigned long long x = k[r], y = k[p];
+                        if ((x > y) == asc)
+                        {
+                            k[r] = y;
+                            k[p] = x;
+                        }
                     }
                 }
             }
-            __syncthreads();
+            else
+            {
+#pragma unroll
+                for (int r = 0; r < R; r++)
+                {
+                    const unsigned long long o     = shflXorU64(k[r], j);
+                    const bool               asc   = ((r * 64 + lane) & s) == 0;
+                    const bool               lower = (lane & j) == 0;
+                    // the lower position keeps the smaller key when ascending,
+                    // the larger when descending; the upper one the other
+                    const bool take = (lower == asc) ? (o < k[r]) : (o > k[r]);
+                    if (take)
+                    {
+                        k[r] = o;
+                    }
+                }
+            }
         }
     }
-    for (int i = threadIdx.x; i < n; i += blockDim.x)
+}
+
+template<int R>
+__device__ void sortOccurrencesWave(int2* __restrict__ occ, int n, int lane)
+{
+    unsigned long long k[R];
+#pragma unroll
+    for (int r = 0; r < R; r++)
     {
-        occ[b + i] = make_int2(static_cast<int>(key[i] >> 32), static_cast<int>(key[i] & 0xffffffffu));
+        const int e = r * 64 + lane;
+        k[r]        = ~0ull; // past the end: sorts last
+        if (e < n)
+        {
+            const int2 v = occ[e];
+            k[r]         = (static_cast<unsigned long long>(static_cast<unsigned>(v.x)) << 32)
+                   | static_cast<unsigned>(v.y);
+        }
+    }
+    waveBitonicSort<R>(k, lane);
+#pragma unroll
+    for (int r = 0; r < R; r++)
+    {
+        const int e = r * 64 + lane;
+        if (e < n)
+        {
+            occ[e] = make_int2(static_cast<int>(k[r] >> 32), static_cast<int>(k[r] & 0xffffffffu));
+        }
+    }
+}
+
+constexpr int kSortWaveMax = 256;
+constexpr int kSortMax     = 1024;
+constexpr int kSortLongBlocks = 64;
+
+// A wavefront per cluster; lists longer than kSortWaveMax go to
+// longList[*longCount] (longCount zeroed by the occurrence fill).
+__global__ __launch_bounds__(256) void occurrenceSortKernel(int ncluster, const int* __restrict__ occOff,
+                                                            int2* __restrict__ occ, int* __restrict__ longList,
+                                                            int* __restrict__ longCount)
+{
+    const int lane = threadIdx.x & 63;
+    const int c    = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (c >= ncluster)
+    {
+        return;
+    }
+    const int b = occOff[c], n = occOff[c + 1] - b;
+    if (n <= 1)
+    {
+        return;
+    }
+    if (n <= 64)
+    {
+        sortOccurrencesWave<1>(occ + b, n, lane);
+    }
+    else if (n <= 128)
+    {
+        sortOccurrencesWave<2>(occ + b, n, lane);
+    }
+    else if (n <= kSortWaveMax)
+    {
+        sortOccurrencesWave<4>(occ + b, n, lane);
+    }
+    else if (lane == 0)
+    {
+        longList[atomicAdd(longCount, 1)] = c;
+    }
+}
+
+// The listed long lists, a workgroup each (every thread of a workgroup runs
+// the same iterations): up to kSortMax entries with a bitonic network in
+// LDS, longer ones by one thread.
+__global__ __launch_bounds__(64) void occurrenceSortLongKernel(const int* __restrict__ occOff, int2* __restrict__ occ,
+                                                               const int* __restrict__ longList,
+                                                               const int* __restrict__ longCount)
+{
+    __shared__ unsigned long long key[kSortMax];
+    const int numLong = *longCount;
+    for (int t = blockIdx.x; t < numLong; t += gridDim.x)
+    {
+        const int c = longList[t];
+        const int b = occOff[c], n = occOff[c + 1] - b;
+        if (n > kSortMax)
+        {
+            if (threadIdx.x == 0)
+            {
+                for (int i = b + 1; i < b + n; i++)
+                {
+                    const int2 v = occ[i];
+                    int        j = i - 1;
+                    while (j >= b && (occ[j].x > v.x || (occ[j].x == v.x && occ[j].y > v.y)))
+                    {
+                        occ[j + 1] = occ[j];
+                        j--;
+                    }
+                    occ[j + 1] = v;
+                }
+            }
+            continue;
+        }
+        int m = 2 * kSortWaveMax;
+        while (m < n)
+        {
+            m <<= 1;
+        }
+        for (int i = threadIdx.x; i < m; i += blockDim.x)
+        {
+            if (i < n)
+            {
+                const int2 v = occ[b + i];
+                key[i] = (static_cast<unsigned long long>(static_cast<unsigned>(v.x)) << 32)
+                         | static_cast<unsigned>(v.y);
+            }
+            else
+            {
+                key[i] = ~0ull;
+            }
+        }
+        __syncthreads();
+        for (int k = 2; k <= m; k <<= 1)
+        {
+            for (int j = k >> 1; j > 0; j >>= 1)
+            {
+                for (int i = threadIdx.x; i < m; i += blockDim.x)
+                {
+                    const int l = i ^ j;
+                    if (l > i)
+                    {
+                        const unsigned long long x = key[i], y = key[l];
+                        if ((x > y) == ((i & k) == 0))
+                        {
+                            key[i] = y;
+                            key[l] = x;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (int i = threadIdx.x; i < n; i += blockDim.x)
+        {
+            occ[b + i] = make_int2(static_cast<int>(key[i] >> 32), static_cast<int>(key[i] & 0xffffffffu));
+        }
+        __syncthreads(); // key is rewritten by the next list
     }
 }
 
@@ -984,11 +1092,19 @@ void launchOccurrences(hipStream_t s, int ngroups, const int4* groups, const int
     }
     if (!fill)
     {
-        occurrenceKernel<false><<<ngroups, 256, 0, s>>>(ngroups, groups, groupInfo, unionList, count, nullptr, nullptr);
+        occurrenceKernel<false><<<ngroups, 256, 0, s>>>(ngroups, groups, groupInfo, unionList, count, nullptr, nullptr,
+                                                        ncluster);
         return;
     }
-    occurrenceKernel<true><<<ngroups, 256, 0, s>>>(ngroups, groups, groupInfo, unionList, count, occOff, occ);
-    occurrenceSortKernel<<<ncluster, 64, 0, s>>>(occOff, occ);
+    occurrenceKernel<true><<<ngroups, 256, 0, s>>>(ngroups, groups, groupInfo, unionList, count, occOff, occ,
+                                                   ncluster);
+    // the per-cluster cursors in count are spent: count[0, ncluster) takes
+    // the long lists, count[ncluster] their number
+    if (ncluster > 0)
+    {
+        occurrenceSortKernel<<<(ncluster + 3) / 4, 256, 0, s>>>(ncluster, occOff, occ, count, count + ncluster);
+        occurrenceSortLongKernel<<<kSortLongBlocks, 64, 0, s>>>(occOff, occ, count, count + ncluster);
+    }
 }
 
 void launchSegments(hipStream_t s, int ngroups, const int4* groups, const int4* groupInfo, int numWaves,

@@ -346,7 +346,8 @@ void reducePartialMax(const int* partial, int blocks, int stride, int* out);
 // Occurrences of every cluster in the groups' j-force windows (own slot and
 // windowed-union slots) as a CSR list of {group, slot}, sorted per cluster.
 // Call once with fill = false (count must be zeroed; it returns the counts),
-// scan the counts into occOff[ncluster + 1], zero count, call with fill = true.
+// scan the counts into occOff[ncluster + 1], zero count, call with fill = true
+// (count holds ncluster + 1 ints; the fill's sort overwrites it).
 void launchOccurrences(hipStream_t s, int ngroups, const int4* groups, const int4* groupInfo, const int* unionList,
                        int ncluster, int* count, int* occOff, int2* occ, bool fill);
 // The pair kernel's window staging layout for a schedule: per group
