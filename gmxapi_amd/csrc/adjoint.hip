@@ -443,7 +443,9 @@ __global__ __launch_bounds__(kAdjThreads) void adjointKernel(const AdjointArgs a
 // wavefront per (type pair, C6 or C12) component, lane l taking workgroups
 // l, l + 64, ..., then a butterfly over the lanes.  The padding type's row
 // and column are dropped: out6/out12 are numTypes x numTypes.
-__global__ __launch_bounds__(kWave) void adjointParamKernel(int numBlocks, int ntype, int numTypes,
+constexpr int kParamThreads = 256;
+
+__global__ __launch_bounds__(kParamThreads) void adjointParamKernel(int numBlocks, int ntype, int numTypes,
                                                             const double* __restrict__ partial,
                                                             double* __restrict__ out6, double* __restrict__ out12)
 {
@@ -455,17 +457,48 @@ __global__ __launch_bounds__(kWave) void adjointParamKernel(int numBlocks, int n
     {
         return;
     }
-    const int n = 2 * ntype * ntype;
-    double    s = 0.0;
-    for (int b = threadIdx.x; b < numBlocks; b += kWave)
+    // four independent sums per thread (loads in flight together), then the
+    // wavefronts' sums in order
+    const int    n  = 2 * ntype * ntype;
+    constexpr int T = kParamThreads;
+    double       s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int          b  = threadIdx.x;
+    for (; b + 3 * T < numBlocks; b += 4 * T)
     {
-        s += partial[static_cast<size_t>(b) * n + k];
+        s0 += partial[static_cast<size_t>(b) * n + k];
+        s1 += partial[static_cast<size_t>(b + T) * n + k];
+        s2 += partial[static_cast<size_t>(b + 2 * T) * n + k];
+        s3 += partial[static_cast<size_t>(b + 3 * T) * n + k];
     }
-    s = waveSumD(s);
+    if (b < numBlocks)
+    {
+        s0 += partial[static_cast<size_t>(b) * n + k];
+    }
+    if (b + T < numBlocks)
+    {
+        s1 += partial[static_cast<size_t>(b + T) * n + k];
+    }
+    if (b + 2 * T < numBlocks)
+    {
+        s2 += partial[static_cast<size_t>(b + 2 * T) * n + k];
+    }
+    double s = waveSumD((s0 + s1) + (s2 + s3));
+    __shared__ double sWave[T / 64];
+    if ((threadIdx.x & 63) == 0)
+    {
+        sWave[threadIdx.x >> 6] = s;
+    }
+    __syncthreads();
     double* out = (k & 1) ? out12 : out6;
     if (threadIdx.x == 0 && out)
     {
-        out[ti * numTypes + tj] = s;
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < T / 64; w++)
+        {
+            t += sWave[w];
+        }
+        out[ti * numTypes + tj] = t;
     }
 }
 
@@ -615,7 +648,7 @@ void launchAdjoint(hipStream_t s, const AdjointArgs& a, int elec, int forwardBlo
     if (gradC6Out || gradC12Out)
     {
         launchElec<true>(s, a, elec, numBlocks, lds);
-        adjointParamKernel<<<2 * a.ntype * a.ntype, kWave, 0, s>>>(numBlocks, a.ntype, numTypes, a.paramPartial,
+        adjointParamKernel<<<2 * a.ntype * a.ntype, kParamThreads, 0, s>>>(numBlocks, a.ntype, numTypes, a.paramPartial,
                                                                    gradC6Out, gradC12Out);
     }
     else
