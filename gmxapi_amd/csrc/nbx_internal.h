@@ -523,7 +523,9 @@ void launchConvertX(hipStream_t s, int nslot, const int* atomIndices, const floa
 // and virPart, also the -1/2 x (x) f virial term from the cluster-layout
 // coordinates xq, 9 doubles per workgroup.  Returns the workgroup count
 // (at most kGatherMaxBlocks).
-constexpr int kGatherMaxBlocks = 4096;
+// 2048 workgroups of four wavefronts fill the 256 CUs once at full
+// occupancy (measured: 1024, 4096 and 8192 slower on the 288k water step)
+constexpr int kGatherMaxBlocks = 2048;
 // MI355X: 8 XCDs of 32 CUs, each with its own L2; the gathers map
 // consecutive clusters to one XCD's workgroups
 constexpr int kGatherXcds = 8;
