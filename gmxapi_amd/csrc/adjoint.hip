@@ -443,13 +443,11 @@ __global__ __launch_bounds__(kAdjThreads) void adjointKernel(const AdjointArgs a
 // wavefront per (type pair, C6 or C12) component, lane l taking workgroups
 // l, l + 64, ..., then a butterfly over the lanes.  The padding type's row
 // and column are dropped: out6/out12 are numTypes x numTypes.
-constexpr int kParamThreads = 256;
+constexpr int kParamThreads = 256; // the finishing launch's workgroup size
 
-__global__ __launch_bounds__(kParamThreads) void adjointParamKernel(int numBlocks, int ntype, int numTypes,
-                                                            const double* __restrict__ partial,
-                                                            double* __restrict__ out6, double* __restrict__ out12)
+__device__ void adjointParamBlock(int k, int numBlocks, int ntype, int numTypes, const double* __restrict__ partial,
+                                  double* __restrict__ out6, double* __restrict__ out12)
 {
-    const int k  = blockIdx.x;
     const int p  = k >> 1;
     const int ti = p / ntype;
     const int tj = p - ti * ntype;
@@ -507,19 +505,18 @@ __global__ __launch_bounds__(kParamThreads) void adjointParamKernel(int numBlock
 // double; plus the cluster-layout buffers, which hold the contributions of
 // clusters that spilled out of a window, written to the atom-order outputs
 // (either may be NULL).
-__global__ __launch_bounds__(256) void adjointGatherStagedKernel(int ncluster, const int* __restrict__ gposOff,
-                                                                 const int* __restrict__ gpos,
-                                                                 const float4* __restrict__ staging,
-                                                                 const float4* __restrict__ gradX,
-                                                                 const double* __restrict__ gradQ,
-                                                                 const int* __restrict__ atomIndices,
-                                                                 float* __restrict__ outX, double* __restrict__ outQ)
+__device__ void adjointGatherStagedBlock(int blk, int nblk, int ncluster, const int* __restrict__ gposOff,
+                                         const int* __restrict__ gpos, const float4* __restrict__ staging,
+                                         const float4* __restrict__ gradX, const double* __restrict__ gradQ,
+                                         const int* __restrict__ atomIndices, float* __restrict__ outX,
+                                         double* __restrict__ outQ)
 {
     const int lane = threadIdx.x & 63;
-    // the workgroups of one XCD (dealt round robin, the grid a multiple of
-    // kGatherXcds) take consecutive clusters, so a cache line that
-    // neighbouring clusters' staged slots share is fetched into one L2
-    const int lb = (blockIdx.x % kGatherXcds) * (gridDim.x / kGatherXcds) + blockIdx.x / kGatherXcds;
+    // the workgroups of one XCD (dealt round robin; blk counts from a
+    // multiple of kGatherXcds and nblk is one) take consecutive clusters, so
+    // a cache line that neighbouring clusters' staged slots share is
+    // fetched into one L2
+    const int lb = (blk % kGatherXcds) * (nblk / kGatherXcds) + blk / kGatherXcds;
     const int c  = lb * 4 + (threadIdx.x >> 6);
     if (c >= ncluster)
     {
@@ -576,6 +573,37 @@ __global__ __launch_bounds__(256) void adjointGatherStagedKernel(int ncluster, c
 // A backward step's set-up in one launch: coordinates (when x is given) and
 // the cotangent g into the cluster layout (fillers and the padding cluster
 // after the last one get g = 0), the spill accumulators cleared.
+// The backward's last launch: the C6/C12 gradient sums (one workgroup per
+// type pair and parameter, a multiple of kGatherXcds with idle ones at the
+// end) first, then the staged dL/dx, dL/dq gather; they do not depend on
+// each other, so the short sums run beside the gather.
+__global__ __launch_bounds__(256) void adjointFinishKernel(int paramBlocks, int gatherBlocks, int ncluster,
+                                                           const int* __restrict__ gposOff,
+                                                           const int* __restrict__ gpos,
+                                                           const float4* __restrict__ staging,
+                                                           const float4* __restrict__ gradX,
+                                                           const double* __restrict__ gradQ,
+                                                           const int* __restrict__ atomIndices,
+                                                           float* __restrict__ outX, double* __restrict__ outQ,
+                                                           int numBlocks, int ntype, int numTypes,
+                                                           const double* __restrict__ partial,
+                                                           double* __restrict__ out6, double* __restrict__ out12)
+{
+    const int b = blockIdx.x;
+    if (b < paramBlocks)
+    {
+        if (b < 2 * ntype * ntype)
+        {
+            adjointParamBlock(b, numBlocks, ntype, numTypes, partial, out6, out12);
+        }
+    }
+    else
+    {
+        adjointGatherStagedBlock(b - paramBlocks, gatherBlocks, ncluster, gposOff, gpos, staging, gradX, gradQ,
+                                 atomIndices, outX, outQ);
+    }
+}
+
 __global__ void adjointPrepareKernel(int nslot, const int* __restrict__ atomIndices, const float* __restrict__ x,
                                      const float* __restrict__ g, float4* __restrict__ xq, float4* __restrict__ gq,
                                      float4* __restrict__ gradX, double* __restrict__ gradQ)
@@ -645,22 +673,25 @@ void launchAdjoint(hipStream_t s, const AdjointArgs& a, int elec, int forwardBlo
 {
     const int numBlocks = adjointBlocks(forwardBlocks);
     const int lds = adjointLdsBytes(a.ntype, a.ic.tab_n, elec, a.windowSlots);
-    if (gradC6Out || gradC12Out)
+    const bool params = gradC6Out || gradC12Out;
+    if (params)
     {
         launchElec<true>(s, a, elec, numBlocks, lds);
-        adjointParamKernel<<<2 * a.ntype * a.ntype, kParamThreads, 0, s>>>(numBlocks, a.ntype, numTypes, a.paramPartial,
-                                                                   gradC6Out, gradC12Out);
     }
     else
     {
         launchElec<false>(s, a, elec, numBlocks, lds);
     }
-    if (a.ncluster > 0 && (gradXOut || gradQOut))
+    // both parts a multiple of kGatherXcds workgroups, so that the gather's
+    // workgroups keep their XCD order
+    auto roundXcd          = [](int n) { return (n + kGatherXcds - 1) / kGatherXcds * kGatherXcds; };
+    const int paramBlocks  = params ? roundXcd(2 * a.ntype * a.ntype) : 0;
+    const int gatherBlocks = a.ncluster > 0 && (gradXOut || gradQOut) ? roundXcd((a.ncluster + 3) / 4) : 0;
+    if (paramBlocks + gatherBlocks > 0)
     {
-        const int blocks = ((a.ncluster + 3) / 4 + kGatherXcds - 1) / kGatherXcds * kGatherXcds;
-        adjointGatherStagedKernel<<<blocks, 256, 0, s>>>(a.ncluster, a.gposOff, a.gpos, a.staging,
-                                                                        a.gradX, a.gradQ, atomIndices, gradXOut,
-                                                                        gradQOut);
+        adjointFinishKernel<<<paramBlocks + gatherBlocks, 256, 0, s>>>(
+                paramBlocks, gatherBlocks, a.ncluster, a.gposOff, a.gpos, a.staging, a.gradX, a.gradQ, atomIndices,
+                gradXOut, gradQOut, numBlocks, a.ntype, numTypes, a.paramPartial, gradC6Out, gradC12Out);
     }
 }
 
