@@ -373,8 +373,8 @@ def main():
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the batched ensemble and "
-              "spatial decomposition would shard replicas or bricks across GPUs; see DESIGN.md). "
+        print("bench.py measures one GPU: the library has no multi-GPU path (a batched ensemble would "
+              "shard its replicas across GPUs, a spatial decomposition the bricks of one system). "
               "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
